@@ -1,0 +1,207 @@
+/* ironav_h264a.h — C ABI of the MI355X (gfx950) H.264 per-macroblock analysis stage.
+ *
+ * The stage replaces the per-slice analysis loop of nervosys/IronAV's software H.264 encoder:
+ * the `FrameCoder::encode` loop that calls `code_mb` for every macroblock of a slice
+ * (crates/ironav-codec/src/h264/enc/frame.rs:452-487, code_mb at :715) together with the half-sample
+ * planes of each reference picture (`HalfPel::new`, crates/ironav-codec/src/h264/enc/subpel.rs:42-115).
+ * The caller keeps what stays on the CPU in the reference encoder: picture types and QP
+ * (enc/mod.rs:404-416), CABAC/CAVLC writing of the returned macroblock codes (enc/frame.rs:1922-2279),
+ * deblocking (h264/deblock.rs:410) and the reference list (enc/mod.rs:559,663-667).
+ *
+ * Conventions
+ *  - Every function returns IA_OK (0) or a negative IA_E* code; nothing unwinds across the ABI.
+ *    ia_last_error() describes the last failure of a context.
+ *  - The caller owns every host buffer. The context owns its device buffers and `nrefs` reference slots.
+ *  - Pictures are 8-bit 4:2:0, padded to whole macroblocks by repeating the right column and bottom
+ *    row (the reference's `H264Encoder::source`, enc/mod.rs:481-529): luma is mb_w*16 x mb_h*16 with
+ *    stride mb_w*16, each chroma plane mb_w*8 x mb_h*8 with stride mb_w*8.
+ *  - Within each slice the results are those of running `code_mb` in raster order; slices are the
+ *    bands i*mb_h/n .. (i+1)*mb_h/n of enc/mod.rs:603 with n = ia_config.slices.
+ *  - A context is not thread safe; use one per encoder thread.
+ */
+#ifndef IRONAV_H264A_H
+#define IRONAV_H264A_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  IA_OK = 0,
+  IA_EINVAL = -1,        /* an argument is out of range */
+  IA_EDEVICE = -2,       /* a HIP call failed, or no gfx950 device is present */
+  IA_ENOMEM = -3,        /* a device allocation failed */
+  IA_ESTATE = -4,        /* the call does not fit the context's state (e.g. an empty reference slot) */
+  IA_EUNSUPPORTED = -5   /* a configuration the GPU path does not implement */
+};
+
+/* Macroblock kinds, numbered as h264/mb.rs:21-30 `Kind`. */
+enum { IA_KIND_NONE = 0, IA_KIND_I4X4 = 1, IA_KIND_I8X8 = 2, IA_KIND_I16X16 = 3, IA_KIND_IPCM = 4, IA_KIND_INTER = 5 };
+
+/* Integer motion search of `motion_search` (enc/frame.rs:949-1019):
+ *  IA_SEARCH_DIAMOND  the reference's predictor candidates and diamond, exactly;
+ *  IA_SEARCH_FULL     the same with one more candidate per partition and reference: the exhaustive
+ *                     integer-sample SAD winner over +-search_range (see ia_motion_search);
+ *  IA_SEARCH_LEARNED  reserved for the learned proposal. */
+enum { IA_SEARCH_DIAMOND = 0, IA_SEARCH_FULL = 1, IA_SEARCH_LEARNED = 2 };
+
+/* Picture kinds of `code_picture` (enc/mod.rs:539): 0 IDR (I slices), 1 P. */
+enum { IA_PIC_IDR = 0, IA_PIC_P = 1 };
+
+typedef struct ia_config {
+  int32_t width, height;   /* luma size in samples (even); mb_w = ceil(width/16), mb_h = ceil(height/16) */
+  int32_t bit_depth;       /* 8 */
+  int32_t cabac;           /* 1: High profile, decisions by rate-distortion with the CABAC bit model
+                              (enc/frame.rs:757-758); 0: decisions by SATD (CAVLC) */
+  int32_t transform8;      /* transform_8x8_mode_flag; used with cabac only (enc/frame.rs:409) */
+  int32_t sub8x8;          /* P_8x8 sub-partitions, active at qp <= 20 (enc/frame.rs:200,436) */
+  int32_t fast;            /* `realtime()` decisions; must be 0 */
+  int32_t nrefs;           /* reference slots, 1..16 */
+  int32_t slices;          /* slice count, 1..mb_h (fixed: it changes the decisions) */
+  int32_t search_mode;     /* IA_SEARCH_* */
+  int32_t search_range;    /* integer-sample range of the full search, 1..64 */
+  float fallback_ratio;    /* learned proposal: fall back to full search when cost > ratio x predicted */
+} ia_config;
+
+/* What later pictures and the loop filter read about a coded macroblock: the P/I subset of
+ * h264/mb.rs:34-68 `MbInfo` that `finish_mb` (enc/frame.rs:2281-2339) fills. */
+typedef struct ia_mbinfo {
+  uint16_t slice;          /* 1-based slice id */
+  uint8_t kind;            /* IA_KIND_* */
+  int8_t qp;
+  uint8_t cbp;
+  uint8_t skip;
+  uint8_t t8x8;
+  uint8_t reserved;
+  uint8_t nnz[16];         /* luma coefficient counts, raster 4x4 blocks */
+  int16_t mv[16][2];       /* list 0 vectors per raster 4x4 block, quarter samples */
+  int8_t ref_idx[4];       /* list 0 reference index per 8x8 block, -1 when unused */
+  int32_t ref_id[4];       /* 1 + ref_idx, -1 when unused */
+} ia_mbinfo;
+
+/* A macroblock's decisions and levels, ready to write: the P/I subset of `MbCode`
+ * (enc/frame.rs:275-310). Levels fit int16 at 8 bits. */
+typedef struct ia_mbcode {
+  uint8_t kind;            /* IA_KIND_* */
+  uint8_t skip;            /* P_Skip */
+  uint8_t i16_mode;
+  uint8_t chroma_mode;
+  uint8_t part;            /* 0 16x16, 1 16x8, 2 8x16, 3 8x8 */
+  uint8_t t8x8;
+  uint8_t cbp;
+  uint8_t reserved;
+  uint8_t modes[16];       /* Intra4x4/8x8 modes, raster 4x4 blocks */
+  uint8_t refs[4];         /* reference index per partition (per 8x8 block for P_8x8) */
+  uint8_t sub[4];          /* sub_mb_type per 8x8 block of P_8x8 */
+  uint8_t nnz[16];
+  uint8_t cnnz[2][4];
+  int16_t mvs[16][2];      /* per partition in coding order (sub-partitions of P_8x8 in coding order) */
+  int16_t mvds[16][2];     /* vector minus its predictor, per partition */
+  int16_t luma[16][16];    /* per raster 4x4 block, zig-zag scan order (DC slot unused for Intra16x16) */
+  int16_t luma8[4][64];    /* per 8x8 block, 8x8 zig-zag scan order, when t8x8 */
+  int16_t dc[16];          /* Intra16x16 luma DC, scan order */
+  int16_t cdc[2][4];       /* chroma DC per component */
+  int16_t cac[2][4][16];   /* chroma AC per component and 4x4 block, scan order (slot 0 unused) */
+} ia_mbcode;
+
+/* The costs the decisions were taken on, for checking and for rate studies. */
+typedef struct ia_mbstats {
+  int32_t skip_sad;        /* SAD of the P_Skip prediction (enc/frame.rs:729); -1 in I slices */
+  int32_t i16_cost;        /* best Intra16x16 SATD + lambda cost (enc/frame.rs:812) */
+  int32_t inter_cost;      /* best_inter's cost (enc/frame.rs:839); -1 when not searched */
+  int32_t inter_part;      /* best_inter's partitioning; -1 when not searched */
+  double rd_cost;          /* J of the kept candidate (enc/frame.rs:1453-1457); 0 when decided before RD */
+} ia_mbstats;
+
+/* One exhaustive-search winner: integer-sample vector in quarter samples, and its SAD. */
+typedef struct ia_fsresult {
+  int16_t mv[2];
+  uint32_t sad;
+} ia_fsresult;
+
+/* Partitions of a macroblock searched by ia_motion_search, 41 per macroblock and reference:
+ *   0        16x16
+ *   1..2     16x8  (top, bottom)
+ *   3..4     8x16  (left, right)
+ *   5..8     8x8   (raster)
+ *   9..16    8x4   (9 + 2*b8 + half)
+ *   17..24   4x8   (17 + 2*b8 + half)
+ *   25..40   4x4   (25 + 4*b8 + raster index inside the 8x8 block) */
+#define IA_NPART 41
+
+typedef struct ia_ctx ia_ctx;
+
+/* Creates a context on the current HIP device: checks the configuration, allocates the device
+ * buffers of one picture, its `nrefs` reference slots and the scratch of the analysis. Replaces
+ * the per-picture set-up of `FrameCoder::new` (enc/frame.rs:381-413). */
+int ia_create(ia_ctx **out, const ia_config *cfg);
+
+/* Frees every device buffer of the context. */
+void ia_destroy(ia_ctx *ctx);
+
+/* The last failure of `ctx` (or of ia_create when ctx is NULL), as text. */
+const char *ia_last_error(const ia_ctx *ctx);
+
+/* Enqueues the context's work on `hip_stream` (a hipStream_t; NULL for the context's own stream). */
+int ia_set_stream(ia_ctx *ctx, void *hip_stream);
+
+/* Loads a reference picture into `slot`: its deblocked luma and chroma (host, padded layout) and
+ * its macroblock records (mb_w*mb_h, for the co-located vector of enc/frame.rs:944-947; may be NULL
+ * for none inter-coded), then builds its half-sample planes on the GPU (`HalfPel::new`,
+ * enc/subpel.rs:57-115). Synchronous. */
+int ia_set_reference(ia_ctx *ctx, int slot, const uint8_t *y, const uint8_t *cb, const uint8_t *cr,
+                     const ia_mbinfo *mbs);
+
+/* Copies the 4 half-sample planes of `slot` (G, B, H, J; each (mb_w*16+96) x (mb_h*16+96), 48 samples
+ * of border) to `out`. For checking. */
+int ia_read_halfpel(ia_ctx *ctx, int slot, uint8_t *out);
+
+/* Uploads the picture to analyse (host, padded layout) to the device, on the context's stream. */
+int ia_upload_source(ia_ctx *ctx, const uint8_t *y, const uint8_t *cb, const uint8_t *cr);
+
+/* Exhaustive integer-sample SAD search of the uploaded picture against the references in
+ * `ref_slots[0..nref)`, for all 41 partitions of every macroblock. The window of macroblock
+ * (mbx, mby) is dx in [max(-R, -(16*mbx+24)), min(R, 16*mb_w-16*mbx+8)], and likewise dy, R the
+ * search range: the vectors `motion_search` accepts for the whole macroblock (enc/frame.rs:952-954).
+ * The winner is the least SAD, ties to the first in raster order of (dy, dx). Results stay on the
+ * device for ia_analyse_uploaded and ia_read_motion. Asynchronous on the context's stream; this is
+ * the throughput kernel of the stage. */
+int ia_motion_search(ia_ctx *ctx, const int *ref_slots, int nref);
+
+/* Copies the last ia_motion_search results, laid out [ref][mb][IA_NPART]. Synchronous. */
+int ia_read_motion(ia_ctx *ctx, ia_fsresult *out);
+
+/* Analyses the uploaded picture: `code_mb` for every macroblock of every slice
+ * (enc/frame.rs:715-790), with the CABAC context states carried from one macroblock to the next as
+ * the writer carries them (enc/frame.rs:465,2020-2024). kind is IA_PIC_IDR or IA_PIC_P; qp the
+ * slice QP (IDR pictures are coded at qp-3 by the reference's driver, enc/mod.rs:36,404).
+ * With IA_SEARCH_FULL the last ia_motion_search results over the same ref_slots are used.
+ * Asynchronous on the context's stream. */
+int ia_analyse_uploaded(ia_ctx *ctx, int qp, int kind, const int *ref_slots, int nref);
+
+/* Copies the results of the last analysis. Any output pointer may be NULL. Synchronous. */
+int ia_read_analysis(ia_ctx *ctx, ia_mbcode *codes, ia_mbstats *stats, uint8_t *recon_y, uint8_t *recon_cb,
+                     uint8_t *recon_cr, ia_mbinfo *mbs);
+
+/* Upload, (full search,) analysis and read-back in one call: the drop-in for one picture's
+ * `FrameCoder::new(..).with_refs(..).encode(..)` across all slices, minus the entropy coding. */
+int ia_analyse_picture(ia_ctx *ctx, const uint8_t *y, const uint8_t *cb, const uint8_t *cr, int qp, int kind,
+                       const int *ref_slots, int nref, ia_mbcode *codes, ia_mbstats *stats, uint8_t *recon_y,
+                       uint8_t *recon_cb, uint8_t *recon_cr, ia_mbinfo *mbs);
+
+/* Waits for the context's stream. */
+int ia_sync(ia_ctx *ctx);
+
+/* Tile shapes of the full-search kernel, for the autotuner (no counterpart in the reference, whose
+ * search is a scalar loop): the number of compiled shapes, each one's name, and the shape a context
+ * uses (-1: the default). */
+int ia_fullsearch_variants(void);
+const char *ia_fullsearch_variant_name(int variant);
+int ia_set_fullsearch_variant(ia_ctx *ctx, int variant);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,121 @@
+"""Seeded synthetic 8-bit 4:2:0 clips for the analysis stage.
+
+Luma is a sum of random-phase 2-D sinusoids (periods 8..128 samples) plus textured rectangles that
+each move by a per-object integer vector every frame (plus an optional global pan), plus Gaussian
+noise, clipped to 0..255. Chroma is the 2x2 average of luma with an offset. After the sine table
+(rounded to integers), everything is integer arithmetic on numpy's PCG64 stream, so a (size, seed,
+frame) triple gives the same bytes on every host with this numpy.
+
+`make_clip` returns unpadded planes; `pad_planes` repeats the right column and bottom row up to
+whole macroblocks as the encoder's `source()` does (crates/ironav-codec/src/h264/enc/mod.rs:481-529).
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_LUT_BITS = 10
+_LUT = np.round(np.sin(2.0 * np.pi * np.arange(1 << _LUT_BITS) / (1 << _LUT_BITS)) * 1024.0).astype(np.int32)
+
+
+@dataclass
+class ClipSpec:
+    width: int
+    height: int
+    frames: int
+    seed: int
+    n_waves: int = 6
+    n_rects: int = 4
+    max_motion: int = 12          # per-object integer motion, uniform in +-max_motion per frame
+    pan: int = 0                  # global pan, uniform in +-pan per frame (fixed per clip)
+    noise_sigma: float = 2.0
+    cuts: tuple = field(default_factory=tuple)  # frames at which the scene is redrawn
+
+
+def _scene(rng, spec):
+    waves = []
+    for _ in range(spec.n_waves):
+        px, py = rng.integers(8, 129, size=2)
+        sx, sy = rng.choice([-1, 1], size=2)
+        # frequency in 1/65536 cycles per sample; one axis may be nearly flat
+        fx = int(sx * round(65536 / px)) if rng.random() < 0.8 else 0
+        fy = int(sy * round(65536 / py)) if (fx == 0 or rng.random() < 0.8) else 0
+        waves.append((fx, fy, int(rng.integers(0, 65536)), int(rng.integers(6, 19))))
+    rects = []
+    for _ in range(spec.n_rects):
+        rw = int(rng.integers(max(16, spec.width // 16), max(17, spec.width // 4)))
+        rh = int(rng.integers(max(16, spec.height // 16), max(17, spec.height // 4)))
+        x0 = int(rng.integers(0, spec.width))
+        y0 = int(rng.integers(0, spec.height))
+        vx, vy = (int(v) for v in rng.integers(-spec.max_motion, spec.max_motion + 1, size=2))
+        tfx, tfy = (int(v) for v in rng.integers(1200, 9000, size=2) * rng.choice([-1, 1], size=2))
+        base = int(rng.integers(40, 216))
+        amp = int(rng.integers(20, 60))
+        rects.append((rw, rh, x0, y0, vx, vy, tfx, tfy, base, amp))
+    pan = tuple(int(v) for v in rng.integers(-spec.pan, spec.pan + 1, size=2)) if spec.pan else (0, 0)
+    return waves, rects, pan
+
+
+def _luma(spec, scene, t, rng):
+    waves, rects, (panx, pany) = scene
+    W, H = spec.width, spec.height
+    ys, xs = np.mgrid[0:H, 0:W].astype(np.int64)
+    bx, by = xs + panx * t, ys + pany * t
+    acc = np.full((H, W), 128 << _LUT_BITS, dtype=np.int64)
+    for fx, fy, ph, amp in waves:
+        idx = ((fx * bx + fy * by + ph) & 0xFFFF) >> (16 - _LUT_BITS)
+        acc += amp * _LUT[idx]
+    img = acc >> _LUT_BITS
+    for rw, rh, x0, y0, vx, vy, tfx, tfy, base, amp in rects:
+        u = (xs - (x0 + vx * t)) % W
+        v = (ys - (y0 + vy * t)) % H
+        inside = (u < rw) & (v < rh)
+        phase = ((tfx * u + tfy * v) & 0xFFFF) >> (16 - _LUT_BITS)
+        checker = (((u >> 3) ^ (v >> 3)) & 1) * 24 - 12
+        tex = base + ((amp * _LUT[phase]) >> _LUT_BITS) + checker
+        img = np.where(inside, tex, img)
+    noise = np.rint(rng.normal(0.0, spec.noise_sigma, size=(H, W))).astype(np.int64)
+    return np.clip(img + noise, 0, 255).astype(np.uint8)
+
+
+def _chroma(y):
+    h, w = y.shape
+    q = y.astype(np.int32)
+    avg = (q[0::2, 0::2] + q[1::2, 0::2] + q[0::2, 1::2] + q[1::2, 1::2] + 2) >> 2
+    cb = np.clip((avg >> 1) + 72, 0, 255).astype(np.uint8)
+    cr = np.clip(184 - (avg >> 1), 0, 255).astype(np.uint8)
+    return cb, cr
+
+
+def make_clip(spec: ClipSpec):
+    """Yields (y, cb, cr) uint8 planes of size height x width and height/2 x width/2 per frame."""
+    if spec.width % 2 or spec.height % 2:
+        raise ValueError("width and height must be even")
+    rng = np.random.Generator(np.random.PCG64(spec.seed))
+    scene = _scene(rng, spec)
+    t0 = 0
+    for t in range(spec.frames):
+        if t in spec.cuts:
+            scene = _scene(rng, spec)
+            t0 = t
+        y = _luma(spec, scene, t - t0, rng)
+        cb, cr = _chroma(y)
+        yield y, cb, cr
+
+
+def pad_planes(y, cb, cr):
+    """Pads planes to whole macroblocks by repeating the last column and row."""
+    h, w = y.shape
+    pw, ph = (w + 15) // 16 * 16, (h + 15) // 16 * 16
+    py = np.pad(y, ((0, ph - h), (0, pw - w)), mode="edge")
+    pcb = np.pad(cb, ((0, ph // 2 - h // 2), (0, pw // 2 - w // 2)), mode="edge")
+    pcr = np.pad(cr, ((0, ph // 2 - h // 2), (0, pw // 2 - w // 2)), mode="edge")
+    return np.ascontiguousarray(py), np.ascontiguousarray(pcb), np.ascontiguousarray(pcr)
+
+
+# The clips of the workloads (sizes, lengths, motion and noise as the workloads state them).
+def workload_720p(seed=720):
+    return ClipSpec(1280, 720, 30, seed, max_motion=12, noise_sigma=2.0)
+
+
+def workload_1080p(seed=1080):
+    return ClipSpec(1920, 1080, 60, seed, max_motion=12, noise_sigma=2.0)

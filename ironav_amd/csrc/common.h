@@ -1,0 +1,50 @@
+// common.h — shared definitions of the gfx950 kernels and the host side of the H.264 analysis library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ironav_h264a.h"
+
+namespace ia {
+
+// Border of the half-sample planes: a block 24 samples outside the picture plus the 6-tap reach
+// (crates/ironav-codec/src/h264/enc/subpel.rs:7).
+constexpr int PAD = 48;
+
+// Geometry of one picture size.
+struct Geom {
+  int mb_w, mb_h;  // macroblocks
+  int pw, ph;      // padded luma size
+  int sw, sh;      // half-sample plane size (pw + 2*PAD, ph + 2*PAD)
+  int cw, ch;      // chroma plane size
+};
+
+inline Geom make_geom(int width, int height) {
+  Geom g;
+  g.mb_w = (width + 15) / 16;
+  g.mb_h = (height + 15) / 16;
+  g.pw = g.mb_w * 16;
+  g.ph = g.mb_h * 16;
+  g.sw = g.pw + 2 * PAD;
+  g.sh = g.ph + 2 * PAD;
+  g.cw = g.pw / 2;
+  g.ch = g.ph / 2;
+  return g;
+}
+
+// Launchers, one per kernel file. Each returns the hipError_t of its launch.
+hipError_t launch_halfpel(const uint8_t *luma, int pw, int ph, uint8_t *planes, hipStream_t stream);
+
+struct FsArgs {
+  const uint8_t *src_y;       // padded luma of the picture, stride pw
+  const uint8_t *ref_g[16];   // integer plane G of each reference (stride sw, PAD border)
+  ia_fsresult *out;           // [nref][mb][IA_NPART]
+  int nref, mb_w, mb_h, range;
+};
+// `variant` selects a tile shape of the search (see fullsearch.hip); -1 picks the default.
+hipError_t launch_fullsearch(const FsArgs &a, int variant, hipStream_t stream);
+int fullsearch_variants();
+const char *fullsearch_variant_name(int v);
+
+}  // namespace ia

@@ -1,0 +1,249 @@
+// fullsearch.hip — exhaustive integer-sample SAD search of all 41 partitions on gfx950.
+//
+// One workgroup searches one macroblock against one reference over the window of ia_motion_search
+// (include/ironav_h264a.h): every integer displacement (dx, dy) the whole macroblock may take
+// (crates/ironav-codec/src/h264/enc/frame.rs:952-954) within +-range. It returns, per partition, the
+// least SAD and its displacement, ties to the first in raster order of (dy, dx).
+//
+// Design (integer VALU bound; SAD is not a dense contraction, so no MFMA):
+//  - The reference window is staged once in LDS as aligned dwords (coalesced loads, clipped at the
+//    plane's edges). A candidate's 16 samples of a row are 5 aligned dwords realigned in registers
+//    by v_alignbyte; 4 neighbouring lanes read the same dwords, so a wave touches 17 banks at most.
+//  - A lane owns one column of displacements and walks a strip of K rows of it: each reference row
+//    loaded from LDS feeds the K candidates that use it (one row is read once per K candidates).
+//  - The 64 source dwords of the macroblock are uniform: they sit in scalar registers.
+//  - v_sad_hi_u8 accumulates each 4x4 block's SAD into the high half of a 32-bit word whose low half
+//    holds the candidate's row index in the window. Partition SADs are sums of 4x4 words, so every
+//    partition's word is already a sort key (SAD, then row) at one add per partition, and a running
+//    minimum per partition costs one v_min (v_min3 for two candidates) per candidate.
+//  - A lane's strips are taken in increasing row order (the lane stride is at least the window
+//    width), so a smaller row index means an earlier candidate in raster order; at the end the keys
+//    are rewritten as (SAD, raster index) and reduced over the workgroup.
+#include "common.h"
+
+namespace ia {
+namespace {
+
+constexpr int RMAX = 64;
+
+__device__ __forceinline__ uint32_t ld_u32(const uint8_t *p) { return *reinterpret_cast<const uint32_t *>(p); }
+
+// The 41 partition words of one candidate from its 16 4x4 words (raster order), numbered as
+// IA_NPART documents. A word is (SAD << 16) + (4x4 blocks summed) * row, so one add per partition.
+__device__ __forceinline__ void partition_keys(const uint32_t *b, uint32_t *key) {
+#pragma unroll
+  for (int b8 = 0; b8 < 4; b8++) {
+    const int o = (b8 >> 1) * 8 + (b8 & 1) * 2;  // raster index of the 8x8 block's first 4x4
+    const uint32_t t = b[o] + b[o + 1], bt = b[o + 4] + b[o + 5];
+    key[9 + 2 * b8] = t;
+    key[10 + 2 * b8] = bt;
+    key[17 + 2 * b8] = b[o] + b[o + 4];
+    key[18 + 2 * b8] = b[o + 1] + b[o + 5];
+    key[5 + b8] = t + bt;
+    key[25 + 4 * b8] = b[o];
+    key[26 + 4 * b8] = b[o + 1];
+    key[27 + 4 * b8] = b[o + 4];
+    key[28 + 4 * b8] = b[o + 5];
+  }
+  key[1] = key[5] + key[6];
+  key[2] = key[7] + key[8];
+  key[3] = key[5] + key[7];
+  key[4] = key[6] + key[8];
+  key[0] = key[1] + key[2];
+}
+
+template <int CTRL, int ROW_MASK, int BANK_MASK>
+__device__ __forceinline__ uint32_t dpp_min(uint32_t v) {
+  // lanes whose DPP source is out of range keep their value (bound_ctrl off, old = identity)
+  return min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)0xffffffff, (int)v, CTRL, ROW_MASK, BANK_MASK, false));
+}
+
+// Minimum over the 64 lanes of a wave: an inclusive scan by DPP (row shifts, then row broadcasts),
+// whose last lane holds the whole wave's minimum.
+__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
+  v = dpp_min<0x111, 0xf, 0xf>(v);
+  v = dpp_min<0x112, 0xf, 0xf>(v);
+  v = dpp_min<0x113, 0xf, 0xf>(v);
+  v = dpp_min<0x114, 0xf, 0xe>(v);
+  v = dpp_min<0x118, 0xf, 0xc>(v);
+  v = dpp_min<0x142, 0xa, 0xf>(v);
+  v = dpp_min<0x143, 0xc, 0xf>(v);
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// K: candidates per strip (rows of one column); NT: threads; RAWS: dwords per staged window row;
+// WPE: waves per SIMD the register allocation must allow.
+template <int K, int NT, int RAWS, int WPE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void fullsearch_kernel(FsArgs a,
+                                                                                                 int rows_lds) {
+  extern __shared__ uint32_t raw[];
+  const int mb = blockIdx.x, r = blockIdx.y;
+  const int mbx = mb % a.mb_w, mby = mb / a.mb_w;
+  const int pw = a.mb_w * 16, ph = a.mb_h * 16, sw = pw + 2 * PAD, sh = ph + 2 * PAD;
+  const int x0 = mbx * 16, y0 = mby * 16, R = a.range;
+  const int dxlo = max(-R, -(x0 + 24)), dxhi = min(R, pw - x0 + 8);
+  const int dylo = max(-R, -(y0 + 24)), dyhi = min(R, ph - y0 + 8);
+  const int C = dxhi - dxlo + 1, NR = dyhi - dylo + 1;
+  const int S = (NR + K - 1) / K;
+  const uint8_t *g = a.ref_g[r];
+  const int tid = threadIdx.x;
+
+  // stage the window's rows as aligned dwords: row rr holds plane bytes from `al` on, where the
+  // window's first column sits `wxo` bytes in
+  const int wx = x0 + dxlo + PAD, wy = y0 + dylo + PAD;
+  const int al = wx & ~3, wxo = wx - al;
+  for (int e = tid; e < rows_lds * RAWS; e += NT) {
+    const int rr = e / RAWS, k = e - rr * RAWS;
+    const int gy = min(wy + rr, sh - 1);
+    raw[e] = ld_u32(g + (size_t)gy * sw + min(al + 4 * k, sw - 4));
+  }
+  // the macroblock's samples, uniform across the workgroup
+  uint32_t src[64];
+  const uint8_t *sp = a.src_y + (size_t)y0 * pw + x0;
+#pragma unroll
+  for (int i = 0; i < 64; i++) src[i] = __builtin_amdgcn_readfirstlane(ld_u32(sp + (size_t)(i >> 2) * pw + (i & 3) * 4));
+  __syncthreads();
+
+  uint32_t run[IA_NPART];
+#pragma unroll
+  for (int p = 0; p < IA_NPART; p++) run[p] = 0xffffffffu;
+
+  for (int q = tid; q < S * C; q += NT) {
+    const int s = q / C, c = q - s * C;
+    const int b0 = wxo + c;                 // byte of the candidate's first sample in a staged row
+    const uint32_t sh8 = (uint32_t)(b0 & 3);
+    const uint32_t *col = raw + (b0 >> 2) + s * K * RAWS;
+    const int i0 = s * K;
+    uint32_t acc[K][16];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+      for (int b = 0; b < 16; b++) acc[k][b] = (uint32_t)(i0 + k);
+#pragma unroll
+    for (int j = 0; j < K + 15; j++) {
+      const uint32_t *rp = col + j * RAWS;
+      const uint32_t w0 = rp[0], w1 = rp[1], w2 = rp[2], w3 = rp[3], w4 = rp[4];
+      const uint32_t r0 = __builtin_amdgcn_alignbyte(w1, w0, sh8), r1 = __builtin_amdgcn_alignbyte(w2, w1, sh8);
+      const uint32_t r2 = __builtin_amdgcn_alignbyte(w3, w2, sh8), r3 = __builtin_amdgcn_alignbyte(w4, w3, sh8);
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int rr = j - k;
+        if (rr >= 0 && rr < 16) {
+          const int by = rr >> 2;
+          acc[k][by * 4 + 0] = __builtin_amdgcn_sad_hi_u8(src[rr * 4 + 0], r0, acc[k][by * 4 + 0]);
+          acc[k][by * 4 + 1] = __builtin_amdgcn_sad_hi_u8(src[rr * 4 + 1], r1, acc[k][by * 4 + 1]);
+          acc[k][by * 4 + 2] = __builtin_amdgcn_sad_hi_u8(src[rr * 4 + 2], r2, acc[k][by * 4 + 2]);
+          acc[k][by * 4 + 3] = __builtin_amdgcn_sad_hi_u8(src[rr * 4 + 3], r3, acc[k][by * 4 + 3]);
+        }
+      }
+    }
+    if (i0 + K <= NR) {
+      // every candidate of the strip lies in the window: two candidates per v_min3
+#pragma unroll
+      for (int k = 0; k < K; k += 2) {
+        uint32_t ka[IA_NPART], kb[IA_NPART];
+        partition_keys(acc[k], ka);
+        if (k + 1 < K) {
+          partition_keys(acc[k + 1], kb);
+#pragma unroll
+          for (int p = 0; p < IA_NPART; p++) run[p] = min(run[p], min(ka[p], kb[p]));
+        } else {
+#pragma unroll
+          for (int p = 0; p < IA_NPART; p++) run[p] = min(run[p], ka[p]);
+        }
+      }
+    } else {
+      const int nvalid = NR - i0;
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        if (k < nvalid) {
+          uint32_t ka[IA_NPART];
+          partition_keys(acc[k], ka);
+#pragma unroll
+          for (int p = 0; p < IA_NPART; p++) run[p] = min(run[p], ka[p]);
+        }
+      }
+    }
+  }
+
+  // rewrite each running minimum as (SAD << 15) | raster index, then reduce over the workgroup
+  __shared__ uint32_t red[NT / 64][IA_NPART];
+  const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+  for (int p = 0; p < IA_NPART; p++) {
+    const int lg = p == 0 ? 4 : p < 5 ? 3 : p < 9 ? 2 : p < 25 ? 1 : 0;  // log2 of the 4x4 blocks summed
+    const uint32_t v = run[p];
+    uint32_t gk = 0xffffffffu;
+    if (v != 0xffffffffu) {
+      const uint32_t sad = v >> 16;
+      const int i = (int)((v & 0xffffu) >> lg);
+      const int s = i / K;
+      const int t = s * C > tid ? (s * C - tid + NT - 1) / NT : 0;
+      const int c = tid + t * NT - s * C;
+      gk = (sad << 15) | (uint32_t)(i * C + c);
+    }
+    gk = wave_min(gk);
+    if (lane == 0) red[wave][p] = gk;
+  }
+  __syncthreads();
+  if (tid < IA_NPART) {
+    uint32_t gk = red[0][tid];
+#pragma unroll
+    for (int w = 1; w < NT / 64; w++) gk = min(gk, red[w][tid]);
+    ia_fsresult res;
+    if (gk == 0xffffffffu) {
+      res.mv[0] = res.mv[1] = 0;
+      res.sad = 0xffffffffu;
+    } else {
+      const int idx = (int)(gk & 0x7fffu), i = idx / C, c = idx - i * C;
+      res.mv[0] = (int16_t)((dxlo + c) * 4);
+      res.mv[1] = (int16_t)((dylo + i) * 4);
+      res.sad = gk >> 15;
+    }
+    a.out[((size_t)r * a.mb_w * a.mb_h + mb) * IA_NPART + tid] = res;
+  }
+}
+
+// Dwords per staged row: the window's first byte sits up to 3 bytes into the first dword, and the
+// last candidate column reads 5 dwords from its own.
+constexpr int raws_for(int rmax) { return ((3 + 2 * rmax + 16) >> 2) + 2; }
+
+struct Variant {
+  const char *name;
+  int k, nt;
+  void (*fn32)(FsArgs, int);  // ranges up to 32
+  void (*fn64)(FsArgs, int);  // ranges up to 64
+};
+
+#define IA_FS_VARIANT(K, NT, W)                                                                         \
+  {"k" #K "_t" #NT "_w" #W, K, NT, fullsearch_kernel<K, NT, raws_for(32), W>, fullsearch_kernel<K, NT, raws_for(64), W>}
+const Variant VARIANTS[] = {
+    IA_FS_VARIANT(4, 256, 3), IA_FS_VARIANT(6, 256, 3), IA_FS_VARIANT(8, 256, 2), IA_FS_VARIANT(4, 128, 3),
+    IA_FS_VARIANT(6, 128, 3), IA_FS_VARIANT(6, 256, 2),
+};
+#undef IA_FS_VARIANT
+constexpr int NVARIANTS = sizeof(VARIANTS) / sizeof(VARIANTS[0]);
+constexpr int DEFAULT_VARIANT = 4;  // k6_t128_w3: fastest on MI355X at 1080p +-32 (measured)
+
+}  // namespace
+
+int fullsearch_variants() { return NVARIANTS; }
+const char *fullsearch_variant_name(int v) { return v >= 0 && v < NVARIANTS ? VARIANTS[v].name : nullptr; }
+
+hipError_t launch_fullsearch(const FsArgs &a, int variant, hipStream_t stream) {
+  if (variant < 0 || variant >= NVARIANTS) variant = DEFAULT_VARIANT;
+  if (a.range < 1 || a.range > RMAX || a.nref < 1 || a.nref > 16) return hipErrorInvalidValue;
+  // a lane's strips must come in increasing row order: at least one lane per window column
+  if (2 * a.range + 1 > VARIANTS[variant].nt) variant = DEFAULT_VARIANT;
+  const Variant &v = VARIANTS[variant];
+  const bool small = a.range <= 32;
+  const int raws = small ? raws_for(32) : raws_for(64);
+  // rows a strip may read: the last strip starts below row NR-1 and reads K+15 rows
+  const int rows_lds = 2 * a.range + 1 + v.k + 15;
+  const size_t shmem = (size_t)rows_lds * raws * sizeof(uint32_t);
+  dim3 grid(a.mb_w * a.mb_h, a.nref);
+  hipLaunchKernelGGL(small ? v.fn32 : v.fn64, grid, dim3(v.nt), shmem, stream, a, rows_lds);
+  return hipGetLastError();
+}
+
+}  // namespace ia

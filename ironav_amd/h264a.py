@@ -1,0 +1,186 @@
+"""Host side of the MI355X H.264 analysis stage: the reference's slice-coder interface over the C ABI.
+
+`Analyser` stands where nervosys/IronAV's `code_picture` builds one `FrameCoder` per slice and runs
+`code_mb` over every macroblock (crates/ironav-codec/src/h264/enc/mod.rs:603-628,
+crates/ironav-codec/src/h264/enc/frame.rs:381-487): it takes the same inputs (the padded source, the
+list-0 references with their half-sample planes, the slice QP, the slice bands, the CABAC /
+8x8-transform / sub-partition options) and returns what the slice loop hands to the entropy coder
+(the `MbCode` of every macroblock, enc/frame.rs:275-310), the reconstruction before deblocking and
+the `MbInfo` records (h264/mb.rs:34-68), plus the per-slice counts [intra, inter, skip]
+(enc/frame.rs:374,2322).
+
+Errors: invalid arguments raise ValueError (the reference's `H264Encoder::new`/`source` return
+`Error::invalid_input`, enc/mod.rs:137-153,483-495); device failures raise RuntimeError. The GPU
+library must be built (`python -m ironav_amd.build`); nothing here falls back to a CPU path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_LIB = None
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libironav_h264a.so")
+
+
+def library():
+    """The loaded gfx950 library; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m ironav_amd.build`")
+        _LIB = abi.bind(C.CDLL(LIB_PATH))
+    return _LIB
+
+
+class AnalysisError(RuntimeError):
+    pass
+
+
+def _check(rc, ctx=None):
+    if rc == abi.IA_OK:
+        return
+    msg = library().ia_last_error(ctx).decode(errors="replace")
+    if rc in (abi.IA_EINVAL,):
+        raise ValueError(msg)
+    raise AnalysisError(f"error {rc}: {msg}")
+
+
+class Picture:
+    """A padded 8-bit 4:2:0 picture: luma mb_w*16 x mb_h*16, chroma half that."""
+
+    def __init__(self, y, cb, cr):
+        self.y = np.ascontiguousarray(y, dtype=np.uint8)
+        self.cb = np.ascontiguousarray(cb, dtype=np.uint8)
+        self.cr = np.ascontiguousarray(cr, dtype=np.uint8)
+        if self.y.shape[0] % 16 or self.y.shape[1] % 16:
+            raise ValueError("luma must be padded to whole macroblocks")
+        if self.cb.shape != (self.y.shape[0] // 2, self.y.shape[1] // 2) or self.cr.shape != self.cb.shape:
+            raise ValueError("chroma planes must be half the luma size")
+
+
+class Analysis:
+    """The results of analysing one picture."""
+
+    def __init__(self, codes, stats, recon, mbs, mb_w, mb_h, rows):
+        self.codes = codes      # ia_mbcode per macroblock, raster order
+        self.stats = stats      # ia_mbstats per macroblock
+        self.recon = recon      # Picture: reconstruction before deblocking
+        self.mbs = mbs          # ia_mbinfo per macroblock
+        self.mb_w, self.mb_h, self.rows = mb_w, mb_h, rows
+
+    def counts(self):
+        """[intra, inter, skip] per slice, as each `FrameCoder` counts them."""
+        out = []
+        for r0, r1 in self.rows:
+            c = self.codes[r0 * self.mb_w:r1 * self.mb_w]
+            skip = int(np.count_nonzero(c["skip"]))
+            inter = int(np.count_nonzero(c["kind"] == abi.KIND_INTER)) - skip
+            out.append([len(c) - inter - skip, inter, skip])
+        return out
+
+
+def slice_rows(mb_h, slices):
+    """The slice bands of enc/mod.rs:603."""
+    n = max(1, min(slices, mb_h))
+    return [(i * mb_h // n, (i + 1) * mb_h // n) for i in range(n)]
+
+
+class Analyser:
+    """One encoder's analysis context on the current HIP device."""
+
+    def __init__(self, width, height, *, nrefs=3, slices=1, cabac=True, transform8=True, sub8x8=False,
+                 search_mode=abi.SEARCH_DIAMOND, search_range=32, fallback_ratio=1.1):
+        self.cfg = abi.Config(width, height, 8, int(cabac), int(transform8), int(sub8x8), 0, nrefs, slices,
+                              search_mode, search_range, fallback_ratio)
+        lib = library()
+        h = C.c_void_p()
+        _check(lib.ia_create(C.byref(h), C.byref(self.cfg)))
+        self._h = h
+        self.mb_w, self.mb_h = (width + 15) // 16, (height + 15) // 16
+        self.pw, self.ph = self.mb_w * 16, self.mb_h * 16
+        self.rows = slice_rows(self.mb_h, slices)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().ia_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _ok(self, rc):
+        _check(rc, self._h)
+
+    def set_stream(self, stream_handle):
+        """Enqueue on a caller's HIP stream (an integer handle; 0 for the context's own)."""
+        self._ok(library().ia_set_stream(self._h, stream_handle or None))
+
+    def _pic(self, pic):
+        if pic.y.shape != (self.ph, self.pw):
+            raise ValueError(f"picture is {pic.y.shape}, expected {(self.ph, self.pw)}")
+        return pic
+
+    def set_reference(self, slot, pic, mbs=None):
+        """Loads a deblocked reference picture (and its macroblock records) into `slot`."""
+        pic = self._pic(pic)
+        if mbs is not None:
+            mbs = np.ascontiguousarray(mbs, dtype=abi.MBINFO_DTYPE)
+            if mbs.shape != (self.mb_w * self.mb_h,):
+                raise ValueError("mbs must hold one record per macroblock")
+        self._ok(library().ia_set_reference(self._h, slot, abi.ptr(pic.y), abi.ptr(pic.cb), abi.ptr(pic.cr),
+                                            abi.ptr(mbs)))
+
+    def read_halfpel(self, slot):
+        sw, sh = self.pw + 96, self.ph + 96
+        out = np.empty((4, sh, sw), dtype=np.uint8)
+        self._ok(library().ia_read_halfpel(self._h, slot, abi.ptr(out)))
+        return out
+
+    def upload(self, pic):
+        pic = self._pic(pic)
+        self._pic_keep = pic  # the copy is asynchronous: keep the host planes alive until the next sync
+        self._ok(library().ia_upload_source(self._h, abi.ptr(pic.y), abi.ptr(pic.cb), abi.ptr(pic.cr)))
+
+    def motion_search(self, ref_slots):
+        refs = abi.int_array(list(ref_slots))
+        self._ok(library().ia_motion_search(self._h, refs, len(ref_slots)))
+
+    def read_motion(self, nref):
+        out = np.empty((nref, self.mb_w * self.mb_h, abi.NPART), dtype=abi.FSRESULT_DTYPE)
+        self._ok(library().ia_read_motion(self._h, abi.ptr(out)))
+        return out
+
+    def sync(self):
+        self._ok(library().ia_sync(self._h))
+
+    def analyse(self, pic, qp, kind, ref_slots=()):
+        """Analyses every macroblock of `pic` (`code_mb` across all slices) and reads the results back."""
+        pic = self._pic(pic)
+        n = self.mb_w * self.mb_h
+        codes = np.zeros(n, dtype=abi.MBCODE_DTYPE)
+        stats = np.zeros(n, dtype=abi.MBSTATS_DTYPE)
+        mbs = np.zeros(n, dtype=abi.MBINFO_DTYPE)
+        ry = np.empty_like(pic.y)
+        rcb = np.empty_like(pic.cb)
+        rcr = np.empty_like(pic.cr)
+        refs = abi.int_array(list(ref_slots))
+        self._ok(library().ia_analyse_picture(self._h, abi.ptr(pic.y), abi.ptr(pic.cb), abi.ptr(pic.cr), qp, kind,
+                                              refs, len(ref_slots), abi.ptr(codes), abi.ptr(stats), abi.ptr(ry),
+                                              abi.ptr(rcb), abi.ptr(rcr), abi.ptr(mbs)))
+        return Analysis(codes, stats, Picture(ry, rcb, rcr), mbs, self.mb_w, self.mb_h, self.rows)
+
+    def set_fullsearch_variant(self, v):
+        self._ok(library().ia_set_fullsearch_variant(self._h, v))
+
+
+def fullsearch_variants():
+    lib = library()
+    return [lib.ia_fullsearch_variant_name(i).decode() for i in range(lib.ia_fullsearch_variants())]

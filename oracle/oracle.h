@@ -1,0 +1,93 @@
+/* oracle.h — CPU restatement of nervosys/IronAV's H.264 per-macroblock analysis, for checking.
+ *
+ * TEST INFRASTRUCTURE ONLY. Nothing in the product (ironav_amd/) includes, links or calls this code;
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it, as the checker.
+ *
+ * Each function names the reference file:line it follows (paths relative to the IronAV checkout;
+ * enc/ = crates/ironav-codec/src/h264/enc/, h264/ = crates/ironav-codec/src/h264/).
+ * Samples are held as uint16_t as the reference holds them (enc/frame.rs:319-323).
+ *
+ * Parity pins (what each part is checked against, in tests/):
+ *  - halfpel build/predict: the decoder's interpolation (h264/inter.rs:466-547) restated separately from
+ *    H.264 8.4.2.2.1 here (or_luma_mc_ref), over the vector grid of enc/subpel.rs:160-193 at 8 and 10 bits.
+ *  - fdct4/quant/idct4, quant8/idct8: the reference's round-trip tests (enc/frame.rs:2443-2479), and
+ *    closed forms (the transforms are the H.264 matrices; tests check them against numpy matrix products).
+ *  - decimate_score: the known answers of enc/frame.rs:2482-2492.
+ *  - entropy table: its closed form (enc/cabac.rs:162-173) recomputed in numpy.
+ *  - se/ue bits: Exp-Golomb lengths (enc/bitw.rs:81-89) against the codes' definition.
+ *  - full search: brute force in numpy on small pictures.
+ *  - intra predictors: closed forms of H.264 8.3 on small edge sets (numpy).
+ *  - motion_search / partition decision / code_mb / decide_rd / mb_bins: parity unpinned at the level of
+ *    exact decisions (the reference commits no encoder outputs and cannot be built here); checked by
+ *    invariants (reconstruction = prediction + dequantised residual, costs recomputed from outputs,
+ *    mvd = mv - predictor).
+ */
+#ifndef IRONAV_ORACLE_H
+#define IRONAV_ORACLE_H
+
+#include <stdint.h>
+#include "../include/ironav_h264a.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_PAD 48
+
+/* Half-sample planes (enc/subpel.rs:9-19): G, B, H, J, each (pw+96) x (ph+96). */
+void or_halfpel_build(const uint16_t *luma, int pw, int ph, int bd, uint16_t *planes);
+void or_halfpel_predict(const uint16_t *planes, int pw, int ph, int x, int y, int mvx, int mvy, int w, int h,
+                        uint16_t *out);
+/* The decoder's luma interpolation (h264/inter.rs:466-547), restated from H.264 8.4.2.2.1 per sample. */
+void or_luma_mc_ref(const uint16_t *refp, int stride, int pw, int ph, int x, int y, int mx, int my, int w, int h,
+                    uint16_t *out, int maxv);
+/* Chroma eighth-sample bilinear prediction (h264/inter.rs:404-422). */
+void or_chroma_mc(const uint16_t *refp, int pw, int ph, int x, int y, int mx, int my, int w, int h, uint16_t *out);
+
+/* Primitives (enc/frame.rs, h264/idct.rs), exposed for the known-answer tests. */
+void or_fdct4(const int32_t *d, int32_t *out);
+void or_fdct8(const int32_t *d, int32_t *out);
+void or_idct4_add(const int32_t *c, uint16_t *dst, int stride, int maxv);
+void or_idct8_add(const int32_t *c, uint16_t *dst, int stride, int maxv);
+void or_hadamard4(const int32_t *c, int32_t *out);
+int32_t or_quant(int32_t c, int32_t mf, int32_t f, int qbits);
+void or_quant8(const int32_t *src, const int32_t *pred, int qp, int intra, int32_t *lv, int32_t *co);
+int32_t or_satd4(const int32_t *d);
+int32_t or_decimate_score(const int32_t *lv);
+int32_t or_decimate_score8(const int32_t *lv);
+uint32_t or_ue_bits(uint32_t v);
+uint32_t or_se_bits(int32_t v);
+int32_t or_lambda(int qp);
+void or_entropy_table(uint32_t *out /* 64*2 */);
+void or_predict_nxn(int n, int mode, const int32_t *top16, const int32_t *left8, int32_t topleft, int have_top,
+                    int have_left, int have_topleft, uint16_t *out, int bd);
+void or_filter_8x8_edges(int32_t *top16, int32_t *left8, int32_t *topleft, int have_top, int have_left,
+                         int have_topleft);
+void or_predict_16x16(int mode, const int32_t *top, const int32_t *left, int32_t topleft, int have_top,
+                      int have_left, uint16_t *out, int bd);
+void or_predict_chroma(int mode, const int32_t *top8, const int32_t *left16, int32_t topleft, int have_top,
+                       int have_left, uint16_t *out, int bd, int h);
+
+/* Exhaustive integer SAD search, brute force, with the window and order of ia_motion_search.
+ * src_y: padded luma (stride mb_w*16); ref_y[r]: deblocked reference luma (same layout).
+ * out: [nref][mb][IA_NPART]. */
+void or_fullsearch(const uint8_t *src_y, const uint8_t *const *ref_y, int nref, int mb_w, int mb_h, int range,
+                   ia_fsresult *out);
+
+/* sizeof of ia_config, ia_mbinfo, ia_mbcode, ia_mbstats and ia_fsresult, for the binding tests. */
+void or_abi_sizes(int32_t *out);
+
+/* One picture's analysis across all slices: the oracle of ia_analyse_picture. Inputs are 8-bit
+ * padded planes; references are given by their deblocked planes and macroblock records (mbs may be
+ * NULL). fs: full-search winners [nref][mb][IA_NPART] for IA_SEARCH_FULL, else NULL. Returns 0, or
+ * -1 on an invalid argument. */
+int or_analyse_picture(const ia_config *cfg, const uint8_t *y, const uint8_t *cb, const uint8_t *cr, int qp,
+                       int kind, const uint8_t *const *ref_y, const uint8_t *const *ref_cb,
+                       const uint8_t *const *ref_cr, const ia_mbinfo *const *ref_mbs, int nref,
+                       const ia_fsresult *fs, ia_mbcode *codes, ia_mbstats *stats, uint8_t *recon_y,
+                       uint8_t *recon_cb, uint8_t *recon_cr, ia_mbinfo *mbs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,63 @@
+"""The C-ABI library loads on a host without a GPU and exports every entry point include/*.h declares."""
+import ctypes as C
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+from ironav_amd import abi, clipgen, h264a
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b(ia_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_entry_point():
+    lib = h264a.library()
+    names = declared_functions()
+    assert "ia_create" in names and "ia_analyse_picture" in names
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(abi.PROTOTYPES), "abi.py must bind exactly the declared entry points"
+
+
+def test_variants_listed_without_a_device():
+    names = h264a.fullsearch_variants()
+    assert len(names) >= 1 and all(names)
+
+
+def test_create_rejects_bad_configs_before_touching_a_device():
+    lib = h264a.library()
+    h = C.c_void_p()
+    bad = [
+        abi.Config(15, 16, 8, 1, 1, 0, 0, 1, 1, 0, 16, 1.1),    # odd width
+        abi.Config(64, 64, 10, 1, 1, 0, 0, 1, 1, 0, 16, 1.1),   # 10-bit
+        abi.Config(64, 64, 8, 1, 1, 0, 1, 1, 1, 0, 16, 1.1),    # fast
+        abi.Config(64, 64, 8, 1, 1, 0, 0, 0, 1, 0, 16, 1.1),    # nrefs 0
+        abi.Config(64, 64, 8, 1, 1, 0, 0, 1, 5, 0, 16, 1.1),    # more slices than MB rows
+        abi.Config(64, 64, 8, 1, 1, 0, 0, 1, 1, 0, 65, 1.1),    # range
+    ]
+    for cfg in bad:
+        rc = lib.ia_create(C.byref(h), C.byref(cfg))
+        assert rc in (abi.IA_EINVAL, abi.IA_EUNSUPPORTED), rc
+        assert lib.ia_last_error(None)
+
+
+def test_clip_generator_is_deterministic_and_pinned():
+    spec = clipgen.ClipSpec(64, 48, 3, 1234)
+    a = [np.concatenate([p.ravel() for p in f]) for f in clipgen.make_clip(spec)]
+    b = [np.concatenate([p.ravel() for p in f]) for f in clipgen.make_clip(spec)]
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    # motion: consecutive frames differ, and the picture is not flat
+    assert not np.array_equal(a[0], a[1]) and a[0][: 64 * 48].std() > 10
+    y, cb, cr = clipgen.pad_planes(*next(clipgen.make_clip(clipgen.ClipSpec(40, 26, 1, 5))))
+    assert y.shape == (32, 48) and cb.shape == (16, 24)
+    assert np.array_equal(y[26:, :40], np.repeat(y[25:26, :40], 6, axis=0))
