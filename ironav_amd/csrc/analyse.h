@@ -1,0 +1,42 @@
+// analyse.h — parameters of the per-macroblock analysis kernel (analyse.hip) and its state records.
+#pragma once
+
+#include "common.h"
+#include "h264_tables.h"
+
+namespace ia {
+
+// What later macroblocks of a slice need to know about a coded one (enc/frame.rs:239-272), P/I subset.
+struct MbState {
+  uint8_t i4, inter, t8x8, skip, kind, cbp, chroma_mode, cbf_dc;
+  uint8_t modes[16], nnz[16], cnnz[2][4];
+  int16_t mvs[16][2];
+  int8_t refs[4];
+  uint8_t mvd[16][2];
+};
+
+struct AnParams {
+  const uint8_t *src[3];        // padded source planes
+  const uint8_t *hp[16];        // half-sample planes of each list-0 reference
+  const uint8_t *rcb[16], *rcr[16];
+  const ia_mbinfo *rmbs0;       // macroblock records of reference 0 (co-located vectors), may be null
+  const ia_fsresult *fs;        // exhaustive-search winners [nref][mb][IA_NPART], or null
+  uint8_t *rec[3];              // reconstruction before deblocking
+  ia_mbcode *codes;
+  ia_mbstats *stats;
+  ia_mbinfo *mbs;
+  MbState *state;               // per macroblock, for the neighbours of later ones
+  const uint32_t *entropy;      // [64][2] CABAC bit costs in 1/256 bits
+  int mb_w, mb_h, nslices, nref;
+  int qp_slice;                 // slice QP
+  int qp;                       // QP'Y of the transform (qp_slice at 8 bits)
+  int qpc;                      // QP'C
+  int lambda;
+  int p_slice, sub8, use_cabac, transform8;
+  double lam2;                  // 1.6 * 2^((QP'Y - 12) / 3)
+  double skip_thresh;           // 256 * 0.625 * 2^(QP'Y / 6)
+};
+
+hipError_t launch_analyse(const AnParams &p, hipStream_t stream);
+
+}  // namespace ia

@@ -1,0 +1,2034 @@
+// analyse.hip — the per-macroblock analysis of H.264 P and I slices on gfx950 (code_mb and what it calls).
+//
+// Reproduces nervosys/IronAV's slice coder decisions (crates/ironav-codec/src/h264/enc/frame.rs):
+// P_Skip test, Intra16x16 mode choice, motion search and partition decision over every list-0
+// reference, the rate-distortion choice among P_Skip, inter (4x4 and 8x8 transform), Intra16x16,
+// Intra4x4 and Intra8x8 with the CABAC bit model, the transforms, quantisation and reconstruction,
+// and the neighbour state and CABAC context evolution from one macroblock to the next.
+//
+// A slice's macroblocks depend on their left, top and top-right neighbours' final decisions and
+// reconstruction, and on the CABAC contexts the previous macroblock left, so each slice is one
+// workgroup walking its macroblocks in raster order; the parallelism inside a macroblock is across
+// the lanes of a wave (pixels, 4x4 blocks, prediction modes). All lanes run the decision logic in
+// lock step on the same values; loops over samples and blocks are split across lanes and summed
+// with DPP reductions.
+//
+// The macroblock being coded lives in an LDS canvas together with the samples its intra
+// predictions read (the row above including the above-right macroblock, the column to the left):
+// candidates are coded into the canvas as the reference codes them into its picture, and the kept
+// one is written to the picture at the end. Decision arithmetic in f64 (the skip threshold, lam2,
+// J = SSD + lam2 * bits / 256) is computed without contraction (the file is built with
+// -ffp-contract=off), with the powers of two computed on the host by the C library as the
+// reference computes them.
+#include "analyse.h"
+
+namespace ia {
+namespace {
+
+enum { K_NONE = 0, K_I4 = 1, K_I8 = 2, K_I16 = 3, K_INTER = 5 };
+
+// ---- tables --------------------------------------------------------------------------------------
+
+// quarter-sample fraction -> (plane a, dx, dy, plane b, dx, dy), indexed [fy][fx]; planes 0 G, 1 B,
+// 2 H, 3 J: the two integer/half samples H.264 8.4.2.2.1 averages (enc/subpel.rs:26-38)
+__constant__ uint8_t QMIX[4][4][6] = {
+    {{0, 0, 0, 0, 0, 0}, {0, 0, 0, 1, 0, 0}, {1, 0, 0, 1, 0, 0}, {0, 1, 0, 1, 0, 0}},
+    {{0, 0, 0, 2, 0, 0}, {1, 0, 0, 2, 0, 0}, {1, 0, 0, 3, 0, 0}, {1, 0, 0, 2, 1, 0}},
+    {{2, 0, 0, 2, 0, 0}, {2, 0, 0, 3, 0, 0}, {3, 0, 0, 3, 0, 0}, {3, 0, 0, 2, 1, 0}},
+    {{0, 0, 1, 2, 0, 0}, {2, 0, 0, 1, 0, 1}, {3, 0, 0, 1, 0, 1}, {2, 1, 0, 1, 0, 1}},
+};
+
+__constant__ uint8_t PARTS[4][4][4] = {
+    {{0, 0, 4, 4}}, {{0, 0, 4, 2}, {0, 2, 4, 2}}, {{0, 0, 2, 4}, {2, 0, 2, 4}},
+    {{0, 0, 2, 2}, {2, 0, 2, 2}, {0, 2, 2, 2}, {2, 2, 2, 2}}};
+__constant__ int NPARTS[4] = {1, 2, 2, 4};
+__constant__ uint8_t SUBS[4][4][4] = {
+    {{0, 0, 2, 2}}, {{0, 0, 2, 1}, {0, 1, 2, 1}}, {{0, 0, 1, 2}, {1, 0, 1, 2}},
+    {{0, 0, 1, 1}, {1, 0, 1, 1}, {0, 1, 1, 1}, {1, 1, 1, 1}}};
+__constant__ int NSUBS[4] = {1, 2, 2, 4};
+
+__constant__ uint8_t ZIG4[16] = {0, 1, 4, 8, 5, 2, 3, 6, 9, 12, 13, 10, 7, 11, 14, 15};
+__constant__ uint8_t ZIG8[64] = {0,  1,  8,  16, 9,  2,  3,  10, 17, 24, 32, 25, 18, 11, 4,  5,  12, 19, 26, 33, 40, 48,
+                                 41, 34, 27, 20, 13, 6,  7,  14, 21, 28, 35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15, 23,
+                                 30, 37, 44, 51, 58, 59, 52, 45, 38, 31, 39, 46, 53, 60, 61, 54, 47, 55, 62, 63};
+
+struct Rect {
+  int x, y, w, h;
+};
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+__device__ __forceinline__ int iabs(int v) { return v < 0 ? -v : v; }
+
+// ---- wave reductions -------------------------------------------------------------------------------
+
+template <int CTRL, int ROW_MASK, int BANK_MASK>
+__device__ __forceinline__ int dpp_add(int v) {
+  return v + __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, BANK_MASK, false);
+}
+
+// Sum over the 64 lanes (inclusive DPP scan; the last lane holds the total), the same in every lane.
+__device__ __forceinline__ int wave_sum(int v) {
+  v = dpp_add<0x111, 0xf, 0xf>(v);
+  v = dpp_add<0x112, 0xf, 0xf>(v);
+  v = dpp_add<0x113, 0xf, 0xf>(v);
+  v = dpp_add<0x114, 0xf, 0xe>(v);
+  v = dpp_add<0x118, 0xf, 0xc>(v);
+  v = dpp_add<0x142, 0xa, 0xf>(v);
+  v = dpp_add<0x143, 0xc, 0xf>(v);
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+__device__ __forceinline__ int lane() { return threadIdx.x; }
+
+__device__ __forceinline__ void sync() { __syncthreads(); }
+
+// ---- exp-Golomb lengths and the MV cost (enc/bitw.rs:81-89, enc/frame.rs:164-171) -----------------
+
+__device__ __forceinline__ int ue_bits(uint32_t v) { return 2 * (32 - __clz((int)(v + 1))) - 1; }
+__device__ __forceinline__ int se_bits(int v) { return ue_bits(v > 0 ? (uint32_t)(2 * v - 1) : (uint32_t)(-2 * v)); }
+__device__ __forceinline__ int mvd_bits(int mx, int my, int px, int py) { return se_bits(mx - px) + se_bits(my - py); }
+
+__device__ __forceinline__ int16_t median3(int16_t a, int16_t b, int16_t c) {
+  int16_t mx = a > b ? a : b, mn = a < b ? a : b;
+  int16_t t = mn > c ? mn : c;
+  return mx < t ? mx : t;
+}
+
+// ---- per-slice working set in LDS --------------------------------------------------------------
+
+struct Lds {
+  uint8_t src_y[256];
+  uint8_t src_c[2][64];
+  // the macroblock and the samples its intra predictions read: luma rows -1..15, columns -1..23
+  // (columns 16..23 of row -1 are the above-right macroblock), chroma rows -1..7, columns -1..7
+  uint8_t cy[17][28];
+  uint8_t cc[2][9][12];
+  uint8_t cabac[1024];  // live context states of the slice
+  uint8_t tmp[1024];    // a copy counted on for a rate estimate
+  MbState nb[4];        // left, top, top-right, top-left neighbours (valid when nb_ok)
+  int nb_ok[4];
+  ia_mbcode code;       // the candidate being coded
+  ia_mbcode keep;       // the candidate kept by the rate-distortion decision
+  ia_mbcode i4;         // Intra4x4 coded ahead of the SATD decision (CAVLC)
+  uint8_t keep_rec[384];
+  uint8_t pred[256];
+  uint8_t cpred[2][64];
+  int32_t co[16][16];   // dequantised coefficients per 4x4 block (raster)
+  int32_t co8[4][64];
+  int32_t dcs[16];
+  int32_t score[16];
+  int32_t ival[64];
+  int16_t st_mvs[16][2];  // sub-partition search state (enc/frame.rs:866)
+  int8_t st_refs[16];
+  int16_t mv8[16][2];
+};
+
+// ---- the slice coder ---------------------------------------------------------------------------
+
+struct Coder {
+  const AnParams &P;
+  Lds &L;
+  int mbx, mby, row0, row1, slice_id;
+
+  __device__ Coder(const AnParams &p, Lds &l) : P(p), L(l) {}
+
+  __device__ int ls() const { return P.mb_w * 16; }
+  __device__ int cs() const { return P.mb_w * 8; }
+  __device__ int up() const { return mby > row0; }
+
+  // canvas accessors (x, y relative to the macroblock; -1 is the neighbour row/column)
+  __device__ uint8_t &cyv(int x, int y) { return L.cy[y + 1][x + 1]; }
+  __device__ uint8_t &ccv(int c, int x, int y) { return L.cc[c][y + 1][x + 1]; }
+
+  // ---- neighbours (enc/frame.rs:509-603) ----
+
+  // neighbour index of macroblock (mbx+dx, mby+dy): 0 left, 1 top, 2 top-right, 3 top-left
+  __device__ const MbState *nb(int dx, int dy) const {
+    int k = dy == 0 ? 0 : dx == 0 ? 1 : dx > 0 ? 2 : 3;
+    return L.nb_ok[k] ? &L.nb[k] : nullptr;
+  }
+
+  __device__ int nb_motion(const int16_t (*cur)[2], const int8_t *cref, uint32_t done, int x, int y, int *r,
+                           int16_t *mv) const {
+    int dx, dy, bx, by;
+    if (x < 0 && y < 0) { dx = -1; dy = -1; bx = 3; by = 3; }
+    else if (x < 0) { dx = -1; dy = 0; bx = 3; by = y; }
+    else if (x >= 4 && y < 0) { dx = 1; dy = -1; bx = 0; by = 3; }
+    else if (x >= 4) return 0;
+    else if (y < 0) { dx = 0; dy = -1; bx = x; by = 3; }
+    else {
+      int b = y * 4 + x;
+      if (!((done >> b) & 1)) return 0;
+      if ((done >> (16 + b)) & 1) { *r = -1; mv[0] = mv[1] = 0; }
+      else { *r = cref[b]; mv[0] = cur[b][0]; mv[1] = cur[b][1]; }
+      return 1;
+    }
+    const MbState *s = nb(dx, dy);
+    if (!s) return 0;
+    int rr = s->refs[(by / 2) * 2 + bx / 2];
+    if (s->inter && rr >= 0) { *r = rr; mv[0] = s->mvs[by * 4 + bx][0]; mv[1] = s->mvs[by * 4 + bx][1]; }
+    else { *r = -1; mv[0] = mv[1] = 0; }
+    return 1;
+  }
+
+  __device__ void pred_mv(const int16_t (*cur)[2], const int8_t *cref, uint32_t done, int bx, int by, int w, int dir,
+                          int rf, int16_t *out) const {
+    int ra = -1, rb = -1, rc = -1;
+    int16_t ma[2] = {0, 0}, mb[2] = {0, 0}, mc[2] = {0, 0};
+    int ha = nb_motion(cur, cref, done, bx - 1, by, &ra, ma);
+    int hb = nb_motion(cur, cref, done, bx, by - 1, &rb, mb);
+    int hc = nb_motion(cur, cref, done, bx + w, by - 1, &rc, mc);
+    if (!hc) hc = nb_motion(cur, cref, done, bx - 1, by - 1, &rc, mc);
+    if (!ha) { ra = -1; ma[0] = ma[1] = 0; }
+    if (!hb) { rb = -1; mb[0] = mb[1] = 0; }
+    if (!hc) { rc = -1; mc[0] = mc[1] = 0; }
+    if (dir == 1) {
+      if (by == 0 && rb == rf) { out[0] = mb[0]; out[1] = mb[1]; return; }
+      if (by != 0 && ra == rf) { out[0] = ma[0]; out[1] = ma[1]; return; }
+    } else if (dir == 2) {
+      if (bx == 0 && ra == rf) { out[0] = ma[0]; out[1] = ma[1]; return; }
+      if (bx != 0 && rc == rf) { out[0] = mc[0]; out[1] = mc[1]; return; }
+    }
+    if (!hb && !hc && ha) { out[0] = ma[0]; out[1] = ma[1]; return; }
+    int ea = ra == rf, eb = rb == rf, ec = rc == rf;
+    if (ea + eb + ec == 1) {
+      const int16_t *m = ea ? ma : eb ? mb : mc;
+      out[0] = m[0];
+      out[1] = m[1];
+      return;
+    }
+    out[0] = median3(ma[0], mb[0], mc[0]);
+    out[1] = median3(ma[1], mb[1], mc[1]);
+  }
+
+  __device__ void mv_pred(int rf, int16_t *out) const {
+    const int16_t z[16][2] = {};
+    const int8_t nr[16] = {};
+    pred_mv(z, nr, 0, 0, 0, 4, 0, rf, out);
+  }
+
+  __device__ static int rects(int part, const uint8_t *sub, Rect *out) {
+    int n = 0;
+    if (part != 3) {
+      for (int k = 0; k < NPARTS[part]; k++) out[n++] = Rect{PARTS[part][k][0], PARTS[part][k][1], PARTS[part][k][2], PARTS[part][k][3]};
+      return n;
+    }
+    for (int k = 0; k < 4; k++)
+      for (int s = 0; s < NSUBS[sub[k]]; s++) {
+        const uint8_t *r = SUBS[sub[k]][s];
+        out[n++] = Rect{(k & 1) * 2 + r[0], (k >> 1) * 2 + r[1], r[2], r[3]};
+      }
+    return n;
+  }
+
+  __device__ static int ref_of(int part, int n, int x, int y) { return part == 3 ? (y / 2) * 2 + x / 2 : n; }
+
+  __device__ void part_mvps(int part, const uint8_t *sub, const int16_t (*mvs)[2], const uint8_t *refs,
+                            int16_t (*out)[2]) const {
+    int16_t cur[16][2] = {};
+    int8_t cref[16] = {};
+    uint32_t done = 0;
+    Rect rs[16];
+    int n = rects(part, sub, rs);
+    const int dir = part == 1 ? 1 : part == 2 ? 2 : 0;
+    for (int k = 0; k < n; k++) {
+      int rf = refs[ref_of(part, k, rs[k].x, rs[k].y)];
+      pred_mv(cur, cref, done, rs[k].x, rs[k].y, rs[k].w, dir, rf, out[k]);
+      for (int by = rs[k].y; by < rs[k].y + rs[k].h; by++)
+        for (int bx = rs[k].x; bx < rs[k].x + rs[k].w; bx++) {
+          cur[by * 4 + bx][0] = mvs[k][0];
+          cur[by * 4 + bx][1] = mvs[k][1];
+          cref[by * 4 + bx] = (int8_t)rf;
+          done |= 1u << (by * 4 + bx);
+        }
+    }
+  }
+
+  __device__ void skip_mv(int16_t *out) const {
+    const int16_t z[16][2] = {};
+    const int8_t nr[16] = {};
+    int ra, rb;
+    int16_t ma[2], mb[2];
+    int ha = nb_motion(z, nr, 0, -1, 0, &ra, ma);
+    int hb = nb_motion(z, nr, 0, 0, -1, &rb, mb);
+    if (!ha || !hb || (ra == 0 && ma[0] == 0 && ma[1] == 0) || (rb == 0 && mb[0] == 0 && mb[1] == 0)) {
+      out[0] = out[1] = 0;
+      return;
+    }
+    mv_pred(0, out);
+  }
+
+  __device__ int pred_i4_mode(const uint8_t *cur, int bx, int by) const {
+    int a = -1, b = -1;
+    if (bx > 0) a = cur[by * 4 + bx - 1];
+    else if (const MbState *s = nb(-1, 0)) a = s->i4 ? s->modes[by * 4 + 3] : 2;
+    if (by > 0) b = cur[(by - 1) * 4 + bx];
+    else if (const MbState *s = nb(0, -1)) b = s->i4 ? s->modes[12 + bx] : 2;
+    if (a >= 0 && b >= 0) return a < b ? a : b;
+    return 2;
+  }
+
+  // ---- samples ----
+
+  // SAD of the w x h source block at (bx, by) of the macroblock against reference r's integer plane
+  // at the integer part of (mx, my) (enc/frame.rs:662-671)
+  __device__ int sad_int(int r, int bx, int by, int w, int h, int mx, int my) const {
+    const int sw = P.mb_w * 16 + 2 * PAD;
+    const uint8_t *g = P.hp[r] + (size_t)(mby * 16 + by + (my >> 2) + PAD) * sw + (mbx * 16 + bx + (mx >> 2) + PAD);
+    int s = 0;
+    const int lw = __builtin_ctz(w);
+    for (int i = lane(); i < w * h; i += 64) {
+      int rr = i >> lw, c = i & (w - 1);
+      s += iabs((int)L.src_y[(by + rr) * 16 + bx + c] - (int)g[(size_t)rr * sw + c]);
+    }
+    return wave_sum(s);
+  }
+
+  // quarter-sample prediction of the w x h block at picture (x, y) from reference r into out
+  // (row stride ostride); enc/subpel.rs:122-145
+  __device__ void luma_pred(int r, int x, int y, int w, int h, int mx, int my, uint8_t *out, int ostride) const {
+    const int pw = P.mb_w * 16, ph = P.mb_h * 16, sw = pw + 2 * PAD, sh = ph + 2 * PAD;
+    const size_t n = (size_t)sw * sh;
+    const int fx = mx & 3, fy = my & 3;
+    const int ix = clampi(x + (mx >> 2), -w - 3, pw + 1), iy = clampi(y + (my >> 2), -h - 3, ph + 1);
+    const uint8_t *m = QMIX[fy][fx];
+    const uint8_t *a = P.hp[r] + m[0] * n + (size_t)(iy + m[2] + PAD) * sw + (ix + m[1] + PAD);
+    const uint8_t *b = P.hp[r] + m[3] * n + (size_t)(iy + m[5] + PAD) * sw + (ix + m[4] + PAD);
+    const int lw = __builtin_ctz(w);
+    for (int i = lane(); i < w * h; i += 64) {
+      int rr = i >> lw, c = i & (w - 1);
+      int va = a[(size_t)rr * sw + c], vb = b[(size_t)rr * sw + c];
+      out[rr * ostride + c] = (uint8_t)((va + vb + 1) >> 1);
+    }
+  }
+
+  // SATD of the w x h source block at (bx, by) of plane `pl` (0 luma, 1/2 chroma) against pred
+  // (row stride pstride), enc/frame.rs:641-659
+  __device__ int satd(int pl, int bx, int by, int w, int h, const uint8_t *pred, int pstride) const {
+    const int nbx = w >> 2, nb = nbx * (h >> 2);
+    int s = 0;
+    if (lane() < nb) {
+      const int ox = (lane() % nbx) * 4, oy = (lane() / nbx) * 4;
+      const uint8_t *src = pl == 0 ? L.src_y : L.src_c[pl - 1];
+      const int ss = pl == 0 ? 16 : 8;
+      int d[16];
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) d[rr * 4 + c] = (int)src[(by + oy + rr) * ss + bx + ox + c] - (int)pred[(oy + rr) * pstride + ox + c];
+      s = satd4(d);
+    }
+    return wave_sum(s);
+  }
+
+  __device__ static int satd4(const int *d) {
+    int t[16];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      int a = d[r * 4], b = d[r * 4 + 1], c = d[r * 4 + 2], e = d[r * 4 + 3];
+      int s0 = a + b, s1 = c + e, d0 = a - b, d1 = c - e;
+      t[r * 4] = s0 + s1;
+      t[r * 4 + 1] = s0 - s1;
+      t[r * 4 + 2] = d0 - d1;
+      t[r * 4 + 3] = d0 + d1;
+    }
+    int sum = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      int a = t[c], b = t[4 + c], cc = t[8 + c], e = t[12 + c];
+      int s0 = a + b, s1 = cc + e, d0 = a - b, d1 = cc - e;
+      sum += iabs(s0 + s1) + iabs(s0 - s1) + iabs(d0 - d1) + iabs(d0 + d1);
+    }
+    return sum >> 1;
+  }
+
+  __device__ int ref_bits(int r) const {
+    if (P.nref <= 1) return 0;
+    if (P.nref == 2) return 1;
+    int v = r + 1, lg = 0;
+    while (v >>= 1) lg++;
+    return 2 * lg + 1;
+  }
+
+  __device__ int col_mv(int16_t *mv) const {
+    if (!P.rmbs0) return 0;
+    const ia_mbinfo *col = &P.rmbs0[mby * P.mb_w + mbx];
+    if (col->kind != K_INTER) return 0;
+    mv[0] = col->mv[5][0];
+    mv[1] = col->mv[5][1];
+    return 1;
+  }
+
+  __device__ static int fs_index(int x, int y, int w, int h) {
+    if (w == 4 && h == 4) return 0;
+    if (w == 4 && h == 2) return 1 + y / 2;
+    if (w == 2 && h == 4) return 3 + x / 2;
+    int b8 = (y / 2) * 2 + x / 2;
+    if (w == 2 && h == 2) return 5 + b8;
+    if (w == 2 && h == 1) return 9 + 2 * b8 + (y & 1);
+    if (w == 1 && h == 2) return 17 + 2 * b8 + (x & 1);
+    return 25 + 4 * b8 + (y & 1) * 2 + (x & 1);
+  }
+
+  // ---- motion search (enc/frame.rs:949-1019) ----
+
+  __device__ int motion_search(int r, int bx, int by, int bw, int bh, const int16_t *mvp, const int16_t (*extra)[2],
+                               int nextra, int16_t *out) {
+    const int lam = P.lambda;
+    const int x0 = mbx * 16 + bx * 4, y0 = mby * 16 + by * 4, w = bw * 4, h = bh * 4;
+    const int lo_x = max(-(x0 + 24) * 4, -2048), hi_x = min((P.mb_w * 16 - x0 + 8) * 4, 2047);
+    const int lo_y = max(-(y0 + 24) * 4, -512), hi_y = min((P.mb_h * 16 - y0 + 8) * 4, 511);
+    int16_t cands[16][2];
+    int nc = 0;
+    auto push = [&](int vx, int vy) {
+      cands[nc][0] = (int16_t)((clampi(vx, lo_x, hi_x) + 2) & ~3);
+      cands[nc][1] = (int16_t)((clampi(vy, lo_y, hi_y) + 2) & ~3);
+      nc++;
+    };
+    push(mvp[0], mvp[1]);
+    cands[nc][0] = 0;
+    cands[nc][1] = 0;
+    nc++;
+    {
+      const int16_t z[16][2] = {};
+      const int8_t nr[16] = {};
+      const int NB[3][2] = {{-1, 0}, {0, -1}, {4, -1}};
+      for (int k = 0; k < 3; k++) {
+        int rr;
+        int16_t mv[2];
+        if (nb_motion(z, nr, 0, NB[k][0], NB[k][1], &rr, mv) && rr == r) push(mv[0], mv[1]);
+      }
+    }
+    for (int k = 0; k < nextra; k++) push(extra[k][0], extra[k][1]);
+    if (P.fs) {
+      const ia_fsresult *f = &P.fs[((size_t)r * P.mb_w * P.mb_h + (size_t)mby * P.mb_w + mbx) * IA_NPART + fs_index(bx, by, bw, bh)];
+      push(f->mv[0], f->mv[1]);
+    }
+    const int bxp = bx * 4, byp = by * 4;
+    int bestx = 0, besty = 0, best_cost = 0x7fffffff;
+    for (int k = 0; k < nc; k++) {
+      int cost = sad_int(r, bxp, byp, w, h, cands[k][0], cands[k][1]) + lam * mvd_bits(cands[k][0], cands[k][1], mvp[0], mvp[1]);
+      if (cost < best_cost) { bestx = cands[k][0]; besty = cands[k][1]; best_cost = cost; }
+    }
+    for (int si = 0; si < 3; si++) {
+      const int step = 16 >> si, iters = step == 4 ? 32 : 4;
+      for (int it = 0; it < iters; it++) {
+        int moved = 0;
+        for (int d = 0; d < 4; d++) {
+          int dx = d == 0 ? -step : d == 1 ? step : 0, dy = d == 2 ? -step : d == 3 ? step : 0;
+          int cx = clampi(bestx + dx, lo_x, hi_x), cy = clampi(besty + dy, lo_y, hi_y);
+          int cost = sad_int(r, bxp, byp, w, h, cx, cy) + lam * mvd_bits(cx, cy, mvp[0], mvp[1]);
+          if (cost < best_cost) { bestx = cx; besty = cy; best_cost = cost; moved = 1; }
+        }
+        if (!moved) break;
+      }
+    }
+    // sub-sample refinement by SATD: 8 half samples around, then 4 quarter samples
+    auto cost_sub = [&](int vx, int vy) {
+      luma_pred(r, x0, y0, w, h, vx, vy, L.pred, 16);
+      sync();
+      int c = satd(0, bxp, byp, w, h, L.pred, 16) + lam * mvd_bits(vx, vy, mvp[0], mvp[1]);
+      sync();
+      return c;
+    };
+    best_cost = cost_sub(bestx, besty);
+    const int SQ[8][2] = {{-1, -1}, {0, -1}, {1, -1}, {-1, 0}, {1, 0}, {-1, 1}, {0, 1}, {1, 1}};
+    const int DM[4][2] = {{0, -1}, {-1, 0}, {1, 0}, {0, 1}};
+    for (int pass = 0; pass < 2; pass++) {
+      const int step = pass == 0 ? 2 : 1, np = pass == 0 ? 8 : 4;
+      const int cx0 = bestx, cy0 = besty;
+      for (int k = 0; k < np; k++) {
+        const int ddx = pass == 0 ? SQ[k][0] : DM[k][0], ddy = pass == 0 ? SQ[k][1] : DM[k][1];
+        int cx = clampi(cx0 + ddx * step, lo_x, hi_x), cy = clampi(cy0 + ddy * step, lo_y, hi_y);
+        int cost = cost_sub(cx, cy);
+        if (cost < best_cost) { bestx = cx; besty = cy; best_cost = cost; }
+      }
+    }
+    out[0] = (int16_t)bestx;
+    out[1] = (int16_t)besty;
+    return best_cost - lam * mvd_bits(bestx, besty, mvp[0], mvp[1]);
+  }
+
+  // enc/frame.rs:911-940; the search state lives in L.st_*
+  __device__ int sub_search(int r, int k, const int16_t *extra, uint32_t *done, int *cost_out) {
+    const int lam = P.lambda, ox = (k & 1) * 2, oy = (k >> 1) * 2;
+    int best_s = -1, best_cost = 0;
+    int16_t bmv[4][2] = {};
+    uint32_t bdone = 0;
+    for (int s = 0; s < (P.sub8 ? 4 : 1); s++) {
+      int16_t tm[16][2];
+      int8_t tr[16];
+      for (int i = 0; i < 16; i++) { tm[i][0] = L.st_mvs[i][0]; tm[i][1] = L.st_mvs[i][1]; tr[i] = L.st_refs[i]; }
+      uint32_t td = *done;
+      int cost = lam * (s == 0 ? 0 : 2);
+      int16_t sm[4][2];
+      for (int q = 0; q < NSUBS[s]; q++) {
+        const int x = ox + SUBS[s][q][0], y = oy + SUBS[s][q][1], w = SUBS[s][q][2], h = SUBS[s][q][3];
+        int16_t mvp[2], m[2];
+        pred_mv(tm, tr, td, x, y, w, 0, r, mvp);
+        int cc = motion_search(r, x, y, w, h, mvp, (const int16_t(*)[2])extra, 1, m);
+        cost += cc + lam * mvd_bits(m[0], m[1], mvp[0], mvp[1]);
+        sm[q][0] = m[0];
+        sm[q][1] = m[1];
+        for (int yy = y; yy < y + h; yy++)
+          for (int xx = x; xx < x + w; xx++) {
+            tm[yy * 4 + xx][0] = m[0];
+            tm[yy * 4 + xx][1] = m[1];
+            tr[yy * 4 + xx] = (int8_t)r;
+            td |= 1u << (yy * 4 + xx);
+          }
+      }
+      if (best_s < 0 || cost < best_cost) {
+        best_s = s;
+        best_cost = cost;
+        bdone = td;
+        for (int q = 0; q < NSUBS[s]; q++) { bmv[q][0] = sm[q][0]; bmv[q][1] = sm[q][1]; }
+      }
+    }
+    // paint the kept sub-partitioning's vectors into the state
+    for (int q = 0; q < NSUBS[best_s]; q++) {
+      const int x = ox + SUBS[best_s][q][0], y = oy + SUBS[best_s][q][1], w = SUBS[best_s][q][2], h = SUBS[best_s][q][3];
+      for (int yy = y; yy < y + h; yy++)
+        for (int xx = x; xx < x + w; xx++) {
+          if (lane() == 0) {
+            L.st_mvs[yy * 4 + xx][0] = bmv[q][0];
+            L.st_mvs[yy * 4 + xx][1] = bmv[q][1];
+            L.st_refs[yy * 4 + xx] = (int8_t)r;
+          }
+        }
+    }
+    sync();
+    *done = bdone;
+    *cost_out = best_cost;
+    return best_s;
+  }
+
+  struct Inter {
+    int part;
+    int16_t mvs[16][2];
+    uint8_t refs[4], sub[4];
+    int cost;
+  };
+
+  // enc/frame.rs:852-906
+  __device__ void best_inter_ref(int r, Inter &best) {
+    const int lam = P.lambda, rb = lam * ref_bits(r);
+    uint8_t refs[4] = {(uint8_t)r, (uint8_t)r, (uint8_t)r, (uint8_t)r};
+    int16_t mvp[2], mv[2], col[1][2];
+    mv_pred(r, mvp);
+    int ncol = col_mv(col[0]);
+    int cost = motion_search(r, 0, 0, 4, 4, mvp, (const int16_t(*)[2])col, ncol, mv);
+    best.part = 0;
+    for (int i = 0; i < 16; i++) { best.mvs[i][0] = mv[0]; best.mvs[i][1] = mv[1]; }
+    for (int k = 0; k < 4; k++) { best.refs[k] = (uint8_t)r; best.sub[k] = 0; }
+    best.cost = cost + lam * (mvd_bits(mv[0], mv[1], mvp[0], mvp[1]) + 1) + rb;
+    if (lane() == 0)
+      for (int i = 0; i < 16; i++) { L.st_mvs[i][0] = 0; L.st_mvs[i][1] = 0; L.st_refs[i] = 0; }
+    sync();
+    uint32_t done = 0;
+    uint8_t sub[4] = {0, 0, 0, 0};
+    int cost8 = lam * 9;
+    for (int k = 0; k < 4; k++) {
+      int cc;
+      sub[k] = (uint8_t)sub_search(r, k, mv, &done, &cc);
+      cost8 += cc + rb;
+      if (cost8 >= best.cost) break;
+    }
+    if (cost8 < best.cost) {
+      Rect rs[16];
+      int n = rects(3, sub, rs);
+      best.part = 3;
+      for (int i = 0; i < 16; i++) { best.mvs[i][0] = 0; best.mvs[i][1] = 0; }
+      for (int k = 0; k < n; k++) { best.mvs[k][0] = L.st_mvs[rs[k].y * 4 + rs[k].x][0]; best.mvs[k][1] = L.st_mvs[rs[k].y * 4 + rs[k].x][1]; }
+      for (int k = 0; k < 4; k++) best.sub[k] = sub[k];
+      best.cost = cost8;
+      int16_t at8[4][2];
+      for (int b = 0; b < 4; b++) {
+        at8[b][0] = L.st_mvs[(b / 2) * 8 + (b % 2) * 2][0];
+        at8[b][1] = L.st_mvs[(b / 2) * 8 + (b % 2) * 2][1];
+      }
+      for (int part = 1; part < 3; part++) {
+        int16_t mvs[16][2] = {};
+        int pc = lam * 3;
+        const uint8_t z4[4] = {0, 0, 0, 0};
+        for (int k = 0; k < 2; k++) {
+          int16_t ps[16][2] = {}, m[2], cands[2][2];
+          part_mvps(part, z4, mvs, refs, ps);
+          int b0 = part == 1 ? 2 * k : k, b1 = part == 1 ? 2 * k + 1 : k + 2;
+          cands[0][0] = at8[b0][0]; cands[0][1] = at8[b0][1];
+          cands[1][0] = at8[b1][0]; cands[1][1] = at8[b1][1];
+          const uint8_t *pr = PARTS[part][k];
+          int cc = motion_search(r, pr[0], pr[1], pr[2], pr[3], ps[k], (const int16_t(*)[2])cands, 2, m);
+          mvs[k][0] = m[0];
+          mvs[k][1] = m[1];
+          pc += cc + lam * mvd_bits(m[0], m[1], ps[k][0], ps[k][1]) + rb;
+          if (pc >= best.cost) break;
+        }
+        if (pc < best.cost) {
+          best.part = part;
+          for (int i = 0; i < 16; i++) { best.mvs[i][0] = mvs[i][0]; best.mvs[i][1] = mvs[i][1]; }
+          for (int k = 0; k < 4; k++) best.sub[k] = 0;
+          best.cost = pc;
+        }
+      }
+    }
+    for (int k = 0; k < 4; k++) best.refs[k] = (uint8_t)r;
+  }
+
+  __device__ void best_inter(Inter &out) {
+    for (int r = 0; r < P.nref; r++) {
+      Inter t;
+      best_inter_ref(r, t);
+      if (r == 0 || t.cost < out.cost) out = t;
+    }
+  }
+
+  // ---- intra 16x16 choice (enc/frame.rs:792-834) ----
+
+  __device__ void i16_pred(int mode, uint8_t *out) {
+    const int ta = up(), la = mbx > 0;
+    // lanes over samples; edges from the canvas
+    for (int i = lane(); i < 256; i += 64) {
+      const int x = i & 15, y = i >> 4;
+      int v;
+      if (mode == 0) v = cyv(x, -1);
+      else if (mode == 1) v = cyv(-1, y);
+      else if (mode == 2) {
+        int st = 0, sl = 0;
+        if (ta) for (int k = 0; k < 16; k++) st += cyv(k, -1);
+        if (la) for (int k = 0; k < 16; k++) sl += cyv(-1, k);
+        if (ta && la) v = (st + sl + 16) >> 5;
+        else if (la) v = (sl + 8) >> 4;
+        else if (ta) v = (st + 8) >> 4;
+        else v = 128;
+      } else {
+        int h = 0, vv = 0;
+        for (int k = 0; k < 8; k++) {
+          h += (k + 1) * ((int)cyv(8 + k, -1) - (int)cyv(6 - k, -1));
+          vv += (k + 1) * ((int)cyv(-1, 8 + k) - (int)cyv(-1, 6 - k));
+        }
+        const int a = 16 * ((int)cyv(-1, 15) + (int)cyv(15, -1)), b = (5 * h + 32) >> 6, c = (5 * vv + 32) >> 6;
+        v = clampi((a + b * (x - 7) + c * (y - 7) + 16) >> 5, 0, 255);
+      }
+      out[i] = (uint8_t)v;
+    }
+  }
+
+  __device__ int best_i16(int *cost_out) {
+    const int ta = up(), la = mbx > 0;
+    int best_m = 2, best_c = 0x7fffffff;
+    for (int mode = 0; mode < 4; mode++) {
+      int ok = mode == 0 ? ta : mode == 1 ? la : mode == 2 ? 1 : (ta && la);
+      if (!ok) continue;
+      i16_pred(mode, L.pred);
+      sync();
+      int cost = satd(0, 0, 0, 16, 16, L.pred, 16) + P.lambda * (mode == 2 ? 0 : 1);
+      sync();
+      if (cost < best_c) { best_m = mode; best_c = cost; }
+    }
+    *cost_out = best_c;
+    return best_m;
+  }
+
+  // ---- transforms (enc/frame.rs:113-141, h264/idct.rs) ----
+
+  __device__ static void fdct4(const int *d, int *o) {
+    int t[16];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int *a = d + r * 4;
+      int s03 = a[0] + a[3], d03 = a[0] - a[3], s12 = a[1] + a[2], d12 = a[1] - a[2];
+      t[r * 4] = s03 + s12;
+      t[r * 4 + 1] = 2 * d03 + d12;
+      t[r * 4 + 2] = s03 - s12;
+      t[r * 4 + 3] = d03 - 2 * d12;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      int s03 = t[c] + t[12 + c], d03 = t[c] - t[12 + c], s12 = t[4 + c] + t[8 + c], d12 = t[4 + c] - t[8 + c];
+      o[c] = s03 + s12;
+      o[4 + c] = 2 * d03 + d12;
+      o[8 + c] = s03 - s12;
+      o[12 + c] = d03 - 2 * d12;
+    }
+  }
+
+  __device__ static int quant(int c, int mf, int f, int qbits) {
+    long long a = ((long long)(c < 0 ? -(long long)c : c) * mf + f) >> qbits;
+    return c < 0 ? -(int)a : (int)a;
+  }
+
+  __device__ static int class4(int i) {
+    int a = i & 1, b = (i >> 2) & 1;
+    return (a == 0 && b == 0) ? 0 : (a == 1 && b == 1) ? 1 : 2;
+  }
+
+  __device__ static int class8(int p) {
+    int i = p / 8, j = p % 8;
+    if (i % 4 == 0 && j % 4 == 0) return 0;
+    if (i % 2 == 1 && j % 2 == 1) return 1;
+    if (i % 4 == 2 && j % 4 == 2) return 2;
+    if ((i % 4 == 0 && j % 2 == 1) || (i % 2 == 1 && j % 4 == 0)) return 3;
+    if ((i % 4 == 0 && j % 4 == 2) || (i % 4 == 2 && j % 4 == 0)) return 4;
+    return 5;
+  }
+
+  // enc/frame.rs:1025-1046: levels in scan order (lv), dequantised raster coefficients (co), DC
+  __device__ static int quant_block(const int *d, int qp, int intra, int skip_dc, int16_t *lv, int32_t *co) {
+    int c[16];
+    fdct4(d, c);
+    const int qbits = 15 + qp / 6, f = (1 << qbits) / (intra ? 3 : 6), m = qp % 6;
+    for (int k = 0; k < 16; k++) {
+      lv[k] = 0;
+      co[k] = 0;
+    }
+    for (int k = 0; k < 16; k++) {
+      const int p = ZIG4[k];
+      if (skip_dc && p == 0) continue;
+      const int l = quant(c[p], IA_MF[m][class4(p)], f, qbits);
+      lv[k] = (int16_t)l;
+      co[p] = (l * IA_V[m][class4(p)]) << (qp / 6);
+    }
+    return c[0];
+  }
+
+  __device__ static void fdct8_1d(const int *d, int *o) {
+    int s07 = d[0] + d[7], s16 = d[1] + d[6], s25 = d[2] + d[5], s34 = d[3] + d[4];
+    int a0 = s07 + s34, a1 = s16 + s25, a2 = s07 - s34, a3 = s16 - s25;
+    int d07 = d[0] - d[7], d16 = d[1] - d[6], d25 = d[2] - d[5], d34 = d[3] - d[4];
+    int a4 = d16 + d25 + (d07 + (d07 >> 1));
+    int a5 = d07 - d34 - (d25 + (d25 >> 1));
+    int a6 = d07 + d34 - (d16 + (d16 >> 1));
+    int a7 = d16 - d25 + (d34 + (d34 >> 1));
+    o[0] = a0 + a1;
+    o[1] = a4 + (a7 >> 2);
+    o[2] = a2 + (a3 >> 1);
+    o[3] = a5 + (a6 >> 2);
+    o[4] = a0 - a1;
+    o[5] = a6 - (a5 >> 2);
+    o[6] = (a2 >> 1) - a3;
+    o[7] = (a4 >> 2) - a7;
+  }
+
+  // enc/frame.rs:88-105 on one 8x8 residual (raster, in place -> coefficients)
+  __device__ static void quant8(int *d, int qp, int intra, int16_t *lv, int32_t *co) {
+    int t[8], o[8];
+    for (int r = 0; r < 8; r++) {
+      for (int k = 0; k < 8; k++) t[k] = d[r * 8 + k];
+      fdct8_1d(t, o);
+      for (int k = 0; k < 8; k++) d[r * 8 + k] = o[k];
+    }
+    for (int c = 0; c < 8; c++) {
+      for (int r = 0; r < 8; r++) t[r] = d[r * 8 + c];
+      fdct8_1d(t, o);
+      for (int r = 0; r < 8; r++) d[r * 8 + c] = o[r];
+    }
+    const int m = qp % 6, qbits = 16 + qp / 6, f = (1 << qbits) / (intra ? 3 : 6);
+    for (int k = 0; k < 64; k++) co[k] = 0;
+    for (int k = 0; k < 64; k++) {
+      const int p = ZIG8[k];
+      const int l = quant(d[p], IA_MF8[m][class8(p)], f, qbits);
+      lv[k] = (int16_t)l;
+      if (l != 0) {
+        const int v = l * 16 * IA_DEQUANT8_COEFF_INIT[m][class8(p)];
+        co[p] = qp >= 36 ? v << (qp / 6 - 6) : (v + (1 << (5 - qp / 6))) >> (6 - qp / 6);
+      }
+    }
+  }
+
+  __device__ static int wadd(int a, int b) { return (int)((uint32_t)a + (uint32_t)b); }
+  __device__ static int wsub(int a, int b) { return (int)((uint32_t)a - (uint32_t)b); }
+
+  // h264/idct.rs:8-32 into 4x4 samples at dst (stride)
+  __device__ static void idct4_add(const int32_t *c, uint8_t *dst, int stride) {
+    int t[16];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int32_t *d = c + i * 4;
+      int e0 = wadd(d[0], d[2]), e1 = wsub(d[0], d[2]), e2 = wsub(d[1] >> 1, d[3]), e3 = wadd(d[1], d[3] >> 1);
+      t[i * 4] = wadd(e0, e3);
+      t[i * 4 + 1] = wadd(e1, e2);
+      t[i * 4 + 2] = wsub(e1, e2);
+      t[i * 4 + 3] = wsub(e0, e3);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      int g0 = wadd(t[j], t[8 + j]), g1 = wsub(t[j], t[8 + j]), g2 = wsub(t[4 + j] >> 1, t[12 + j]), g3 = wadd(t[4 + j], t[12 + j] >> 1);
+      int hh[4] = {wadd(g0, g3), wadd(g1, g2), wsub(g1, g2), wsub(g0, g3)};
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        uint8_t *p = dst + i * stride + j;
+        *p = (uint8_t)clampi(*p + (wadd(hh[i], 32) >> 6), 0, 255);
+      }
+    }
+  }
+
+  __device__ static void row8(const int *d, int *o) {
+    int a0 = wadd(d[0], d[4]), a4 = wsub(d[0], d[4]), a2 = wsub(d[2] >> 1, d[6]), a6 = wadd(d[2], d[6] >> 1);
+    int b0 = wadd(a0, a6), b2 = wadd(a4, a2), b4 = wsub(a4, a2), b6 = wsub(a0, a6);
+    int a1 = wsub(wsub(wsub(d[5], d[3]), d[7]), d[7] >> 1);
+    int a3 = wsub(wsub(wadd(d[1], d[7]), d[3]), d[3] >> 1);
+    int a5 = wadd(wadd(wsub(d[7], d[1]), d[5]), d[5] >> 1);
+    int a7 = wadd(wadd(wadd(d[3], d[5]), d[1]), d[1] >> 1);
+    int b1 = wadd(a1, a7 >> 2), b7 = wsub(a7, a1 >> 2), b3 = wadd(a3, a5 >> 2), b5 = wsub(a3 >> 2, a5);
+    o[0] = wadd(b0, b7);
+    o[1] = wadd(b2, b5);
+    o[2] = wadd(b4, b3);
+    o[3] = wadd(b6, b1);
+    o[4] = wsub(b6, b1);
+    o[5] = wsub(b4, b3);
+    o[6] = wsub(b2, b5);
+    o[7] = wsub(b0, b7);
+  }
+
+  __device__ static void idct8_add(const int32_t *c, uint8_t *dst, int stride) {
+    int t[64], d[8], hh[8];
+    for (int i = 0; i < 8; i++) {
+      int rr[8];
+      for (int k = 0; k < 8; k++) rr[k] = c[i * 8 + k];
+      row8(rr, t + i * 8);
+    }
+    for (int j = 0; j < 8; j++) {
+      for (int i = 0; i < 8; i++) d[i] = t[i * 8 + j];
+      row8(d, hh);
+      for (int i = 0; i < 8; i++) {
+        uint8_t *p = dst + i * stride + j;
+        *p = (uint8_t)clampi(*p + (wadd(hh[i], 32) >> 6), 0, 255);
+      }
+    }
+  }
+
+  __device__ static void hadamard4(const int32_t *c, int32_t *o) {
+    int t[16];
+    for (int i = 0; i < 4; i++) {
+      const int32_t *d = c + i * 4;
+      int s0 = wadd(d[0], d[1]), s1 = wsub(d[0], d[1]), s2 = wadd(d[2], d[3]), s3 = wsub(d[2], d[3]);
+      t[i * 4] = wadd(s0, s2);
+      t[i * 4 + 1] = wsub(s0, s2);
+      t[i * 4 + 2] = wsub(s1, s3);
+      t[i * 4 + 3] = wadd(s1, s3);
+    }
+    for (int j = 0; j < 4; j++) {
+      int s0 = wadd(t[j], t[4 + j]), s1 = wsub(t[j], t[4 + j]), s2 = wadd(t[8 + j], t[12 + j]), s3 = wsub(t[8 + j], t[12 + j]);
+      o[j] = wadd(s0, s2);
+      o[4 + j] = wsub(s0, s2);
+      o[8 + j] = wsub(s1, s3);
+      o[12 + j] = wadd(s1, s3);
+    }
+  }
+
+  __device__ static int decimate_score(const int16_t *lv) {
+    const int RUN[16] = {3, 2, 2, 1, 1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int score = 0, i = 15;
+    while (i >= 0 && lv[i] == 0) i--;
+    while (i >= 0) {
+      if (iabs(lv[i]) > 1) return 9;
+      i--;
+      int run = 0;
+      while (i >= 0 && lv[i] == 0) { i--; run++; }
+      score += RUN[min(run, 15)];
+    }
+    return score;
+  }
+
+  __device__ static int decimate_score8(const int16_t *lv) {
+    const int RUN[25] = {3, 3, 3, 3, 2, 2, 2, 2, 2, 2, 2, 2, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 0};
+    int score = 0, i = 63;
+    while (i >= 0 && lv[i] == 0) i--;
+    while (i >= 0) {
+      if (iabs(lv[i]) > 1) return 9;
+      i--;
+      int run = 0;
+      while (i >= 0 && lv[i] == 0) { i--; run++; }
+      score += RUN[min(run, 24)];
+    }
+    return score;
+  }
+
+  __device__ static int count_nz(const int16_t *v, int n) {
+    int k = 0;
+    for (int i = 0; i < n; i++) k += v[i] != 0;
+    return k;
+  }
+
+  // ---- coding of the candidates (enc/frame.rs:1050-1918) ----
+
+  __device__ void code_new(int kind) {
+    // every lane clears its share of the code
+    uint32_t *w = reinterpret_cast<uint32_t *>(&L.code);
+    for (int i = lane(); i < (int)(sizeof(ia_mbcode) / 4); i += 64) w[i] = 0;
+    sync();
+    if (lane() == 0) {
+      L.code.kind = (uint8_t)kind;
+      for (int i = 0; i < 16; i++) L.code.modes[i] = 2;
+    }
+    sync();
+  }
+
+  __device__ static void block_pos(int n, int *bx, int *by) {
+    *bx = ((n >> 2) & 1) * 2 + (n & 1);
+    *by = ((n >> 3) & 1) * 2 + ((n >> 1) & 1);
+  }
+
+  __device__ static int zorder(int bx, int by) { return ((by >> 1) << 3) | ((bx >> 1) << 2) | ((by & 1) << 1) | (bx & 1); }
+
+  // directional 4x4/8x8 prediction of sample (x, y) (h264/intra.rs:19-127); t(i)/l(j) read the edges
+  __device__ static int nxn_sample(int n, int mode, int x, int y, const int *top, const int *left, int tl, int have_top,
+                                   int have_left) {
+#define T(i) ((i) < 0 ? tl : top[(i)])
+#define LF(j) ((j) < 0 ? tl : left[(j)])
+    const int shift = n == 4 ? 2 : 3;
+    switch (mode) {
+      case 0: return T(x);
+      case 1: return LF(y);
+      case 2: {
+        int st = 0, sl = 0;
+        for (int i = 0; i < n; i++) { st += T(i); sl += LF(i); }
+        if (have_top && have_left) return (st + sl + n) >> (shift + 1);
+        if (have_left) return (sl + n / 2) >> shift;
+        if (have_top) return (st + n / 2) >> shift;
+        return 128;
+      }
+      case 3:
+        if (x == n - 1 && y == n - 1) return (T(2 * n - 2) + 3 * T(2 * n - 1) + 2) >> 2;
+        return (T(x + y) + 2 * T(x + y + 1) + T(x + y + 2) + 2) >> 2;
+      case 4:
+        if (x > y) return (T(x - y - 2) + 2 * T(x - y - 1) + T(x - y) + 2) >> 2;
+        if (x < y) return (LF(y - x - 2) + 2 * LF(y - x - 1) + LF(y - x) + 2) >> 2;
+        return (T(0) + 2 * tl + LF(0) + 2) >> 2;
+      case 5: {
+        int z = 2 * x - y;
+        if (z >= 0 && z % 2 == 0) return (T(x - (y >> 1) - 1) + T(x - (y >> 1)) + 1) >> 1;
+        if (z >= 0) return (T(x - (y >> 1) - 2) + 2 * T(x - (y >> 1) - 1) + T(x - (y >> 1)) + 2) >> 2;
+        if (z == -1) return (LF(0) + 2 * tl + T(0) + 2) >> 2;
+        return (LF(y - 2 * x - 1) + 2 * LF(y - 2 * x - 2) + LF(y - 2 * x - 3) + 2) >> 2;
+      }
+      case 6: {
+        int z = 2 * y - x;
+        if (z >= 0 && z % 2 == 0) return (LF(y - (x >> 1) - 1) + LF(y - (x >> 1)) + 1) >> 1;
+        if (z >= 0) return (LF(y - (x >> 1) - 2) + 2 * LF(y - (x >> 1) - 1) + LF(y - (x >> 1)) + 2) >> 2;
+        if (z == -1) return (LF(0) + 2 * tl + T(0) + 2) >> 2;
+        return (T(x - 2 * y - 1) + 2 * T(x - 2 * y - 2) + T(x - 2 * y - 3) + 2) >> 2;
+      }
+      case 7:
+        if (y % 2 == 0) return (T(x + (y >> 1)) + T(x + (y >> 1) + 1) + 1) >> 1;
+        return (T(x + (y >> 1)) + 2 * T(x + (y >> 1) + 1) + T(x + (y >> 1) + 2) + 2) >> 2;
+      default: {
+        int z = x + 2 * y, edge = 2 * n - 3;
+        if (z > edge) return LF(n - 1);
+        if (z == edge) return (LF(n - 2) + 3 * LF(n - 1) + 2) >> 2;
+        if (z % 2 == 0) return (LF(y + (x >> 1)) + LF(y + (x >> 1) + 1) + 1) >> 1;
+        return (LF(y + (x >> 1)) + 2 * LF(y + (x >> 1) + 1) + LF(y + (x >> 1) + 2) + 2) >> 2;
+      }
+    }
+#undef T
+#undef LF
+  }
+
+  __device__ static int mode_ok(int mode, int have_top, int have_left, int have_tl) {
+    if (mode == 0 || mode == 3 || mode == 7) return have_top;
+    if (mode == 1 || mode == 8) return have_left;
+    if (mode == 2) return 1;
+    return have_top && have_left && have_tl;
+  }
+
+  // enc/frame.rs:1050-1138: Intra4x4 into L.code and the canvas; returns the SATD cost
+  __device__ int code_i4() {
+    code_new(K_I4);
+    const int ta = up(), la = mbx > 0, tla = mbx > 0 && ta, tra = ta && mbx + 1 < P.mb_w;
+    const int qp = P.qp;
+    int cost = 0;
+    for (int n = 0; n < 16; n++) {
+      int bx, by;
+      block_pos(n, &bx, &by);
+      const int x = bx * 4, y = by * 4;
+      const int have_left = bx > 0 || la, have_top = by > 0 || ta;
+      int have_tl, have_tr;
+      if (bx > 0 && by > 0) have_tl = 1;
+      else if (by > 0) have_tl = la;
+      else if (bx > 0) have_tl = ta;
+      else have_tl = tla;
+      if (by == 0) have_tr = bx < 3 ? ta : tra;
+      else if (bx == 3) have_tr = 0;
+      else have_tr = zorder(bx + 1, by - 1) < zorder(bx, by);
+      int top[16] = {}, left[8] = {}, tl = 0;
+      if (have_top) {
+        for (int i = 0; i < 4; i++) top[i] = cyv(x + i, y - 1);
+        for (int i = 4; i < 8; i++) top[i] = have_tr ? cyv(x + i, y - 1) : top[3];
+      }
+      if (have_left)
+        for (int j = 0; j < 4; j++) left[j] = cyv(x - 1, y + j);
+      if (have_tl) tl = cyv(x - 1, y - 1);
+      const int predicted = pred_i4_mode(L.code.modes, bx, by);
+      // lane m < 9: mode m's prediction and SATD
+      int c = 0x7fffffff;
+      const int m = lane();
+      if (m < 9 && mode_ok(m, have_top, have_left, have_tl)) {
+        int d[16];
+        for (int i = 0; i < 16; i++)
+          d[i] = (int)L.src_y[(y + (i >> 2)) * 16 + x + (i & 3)] - nxn_sample(4, m, i & 3, i >> 2, top, left, tl, have_top, have_left);
+        c = satd4(d) + P.lambda * (m == predicted ? 1 : 4);
+      }
+      // the first mode of least cost (strict <, in mode order)
+      int best_m = 2, best_c = 0x7fffffff;
+      for (int k = 0; k < 9; k++) {
+        int ck = __builtin_amdgcn_readlane(c, k);
+        if (ck < best_c) { best_c = ck; best_m = k; }
+      }
+      cost += best_c;
+      // lane 0 codes the block: prediction, transform, quantisation, reconstruction
+      if (lane() == 0) {
+        int d[16];
+        for (int i = 0; i < 16; i++) {
+          const int pv = nxn_sample(4, best_m, i & 3, i >> 2, top, left, tl, have_top, have_left);
+          d[i] = (int)L.src_y[(y + (i >> 2)) * 16 + x + (i & 3)] - pv;
+          cyv(x + (i & 3), y + (i >> 2)) = (uint8_t)pv;
+        }
+        int16_t *lv = L.code.luma[by * 4 + bx];
+        int32_t co[16];
+        quant_block(d, qp, 1, 0, lv, co);
+        int nz = count_nz(lv, 16);
+        if (nz) idct4_add(co, &cyv(x, y), 28);
+        L.code.nnz[by * 4 + bx] = (uint8_t)nz;
+        L.code.modes[by * 4 + bx] = (uint8_t)best_m;
+      }
+      sync();
+    }
+    if (lane() == 0) {
+      int cbp = 0;
+      for (int i8 = 0; i8 < 4; i8++) {
+        int bx = (i8 & 1) * 2, by = (i8 >> 1) * 2, any = 0;
+        for (int dy = 0; dy < 2; dy++)
+          for (int dx = 0; dx < 2; dx++) any |= L.code.nnz[(by + dy) * 4 + bx + dx] > 0;
+        if (any) cbp |= 1 << i8;
+      }
+      L.code.cbp = (uint8_t)cbp;
+    }
+    sync();
+    return cost;
+  }
+
+  // enc/frame.rs:1140-1197
+  __device__ void code_i16(int mode) {
+    code_new(K_I16);
+    if (lane() == 0) L.code.i16_mode = (uint8_t)mode;
+    i16_pred(mode, L.pred);
+    sync();
+    const int qp = P.qp;
+    // lane b < 16: block b's AC levels and DC
+    if (lane() < 16) {
+      const int b = lane(), bx = b % 4, by = b / 4;
+      int d[16];
+      for (int i = 0; i < 16; i++) {
+        const int o = (by * 4 + (i >> 2)) * 16 + bx * 4 + (i & 3);
+        d[i] = (int)L.src_y[o] - (int)L.pred[o];
+      }
+      L.dcs[b] = quant_block(d, qp, 1, 1, L.code.luma[b], L.co[b]);
+    }
+    sync();
+    if (lane() == 0) {
+      int32_t h[16], dl[16] = {};
+      hadamard4(L.dcs, h);
+      const int qbits = 16 + qp / 6, f = (1 << qbits) / 3, m = qp % 6;
+      for (int k = 0; k < 16; k++) {
+        const int p = ZIG4[k];
+        const int l = quant(h[p] >> 1, IA_MF[m][0], f, qbits);
+        L.code.dc[k] = (int16_t)l;
+        dl[p] = l;
+      }
+      int ac = 0;
+      for (int b = 0; b < 16; b++) ac |= count_nz(L.code.luma[b], 16) > 0;
+      if (!ac)
+        for (int b = 0; b < 16; b++)
+          for (int k = 0; k < 16; k++) { L.code.luma[b][k] = 0; L.co[b][k] = 0; }
+      L.code.cbp = ac ? 15 : 0;
+      int32_t ff[16];
+      hadamard4(dl, ff);
+      const int lsd = 16 * IA_V[m][0];
+      for (int b = 0; b < 16; b++)
+        L.co[b][0] = qp >= 36 ? (ff[b] * lsd) << (qp / 6 - 6) : (ff[b] * lsd + (1 << (5 - qp / 6))) >> (6 - qp / 6);
+    }
+    sync();
+    for (int i = lane(); i < 256; i += 64) cyv(i & 15, i >> 4) = L.pred[i];
+    sync();
+    if (lane() < 16) {
+      const int b = lane(), bx = b % 4, by = b / 4;
+      int any = 0;
+      for (int k = 0; k < 16; k++) any |= L.co[b][k] != 0;
+      if (any) idct4_add(L.co[b], &cyv(bx * 4, by * 4), 28);
+      L.code.nnz[b] = (uint8_t)count_nz(L.code.luma[b], 16);
+    }
+    sync();
+  }
+
+  // enc/frame.rs:1856-1918: chroma of the candidate against L.cpred
+  __device__ void code_chroma(int intra) {
+    const int qp = P.qpc, m = qp % 6;
+    // lane (c, blk) < 8: the 4x4 block's AC levels and DC
+    if (lane() < 8) {
+      const int ci = lane() >> 2, blk = lane() & 3, bx = blk & 1, by = blk >> 1;
+      int d[16];
+      for (int i = 0; i < 16; i++) {
+        const int o = (by * 4 + (i >> 2)) * 8 + bx * 4 + (i & 3);
+        d[i] = (int)L.src_c[ci][o] - (int)L.cpred[ci][o];
+      }
+      L.dcs[lane()] = quant_block(d, qp, intra, 1, L.code.cac[ci][blk], L.co[lane()]);
+      L.score[lane()] = decimate_score(L.code.cac[ci][blk]);
+    }
+    sync();
+    if (lane() == 0) {
+      int any_dc = 0, any_ac = 0;
+      for (int ci = 0; ci < 2; ci++) {
+        const int *dcs = L.dcs + ci * 4;
+        const int f[4] = {dcs[0] + dcs[1] + dcs[2] + dcs[3], dcs[0] - dcs[1] + dcs[2] - dcs[3], dcs[0] + dcs[1] - dcs[2] - dcs[3],
+                          dcs[0] - dcs[1] - dcs[2] + dcs[3]};
+        const int qbits = 16 + qp / 6, fr = (1 << qbits) / (intra ? 3 : 6);
+        for (int k = 0; k < 4; k++) L.code.cdc[ci][k] = (int16_t)quant(f[k], IA_MF[m][0], fr, qbits);
+        any_dc |= count_nz(L.code.cdc[ci], 4) > 0;
+        for (int blk = 0; blk < 4; blk++) any_ac |= count_nz(L.code.cac[ci][blk], 16) > 0;
+      }
+      int ac_score = 0;
+      for (int k = 0; k < 8; k++) ac_score += L.score[k];
+      if (!intra && ac_score < 7) any_ac = 0;
+      if (!any_ac)
+        for (int k = 0; k < 8; k++)
+          for (int q = 0; q < 16; q++) { L.code.cac[k >> 2][k & 3][q] = 0; L.co[k][q] = 0; }
+      const int cbpc = any_ac ? 2 : (any_dc ? 1 : 0);
+      L.code.cbp |= (uint8_t)(cbpc << 4);
+      const int lsd = 16 * IA_V[m][0];
+      for (int ci = 0; ci < 2; ci++) {
+        const int16_t *l = L.code.cdc[ci];
+        const int f[4] = {l[0] + l[1] + l[2] + l[3], l[0] - l[1] + l[2] - l[3], l[0] + l[1] - l[2] - l[3], l[0] - l[1] - l[2] + l[3]};
+        for (int blk = 0; blk < 4; blk++) L.co[ci * 4 + blk][0] = cbpc > 0 ? ((f[blk] * lsd) << (qp / 6)) >> 5 : 0;
+      }
+    }
+    sync();
+    for (int i = lane(); i < 128; i += 64) {
+      const int ci = i >> 6, o = i & 63;
+      ccv(ci, o & 7, o >> 3) = L.cpred[ci][o];
+    }
+    sync();
+    if (lane() < 8) {
+      const int ci = lane() >> 2, blk = lane() & 3, bx = blk & 1, by = blk >> 1;
+      L.code.cnnz[ci][blk] = (uint8_t)count_nz(L.code.cac[ci][blk], 16);
+      int any = 0;
+      for (int k = 0; k < 16; k++) any |= L.co[lane()][k] != 0;
+      if (any) idct4_add(L.co[lane()], &ccv(ci, bx * 4, by * 4), 12);
+    }
+    sync();
+  }
+
+  // enc/frame.rs:1254-1304 (the luma against L.pred with the 8x8 transform)
+  __device__ void code_luma8() {
+    const int qp = P.qp;
+    if (lane() < 4) {
+      const int i8 = lane(), bx = (i8 & 1) * 8, by = (i8 >> 1) * 8;
+      int d[64];
+      for (int k = 0; k < 64; k++) {
+        const int o = (by + k / 8) * 16 + bx + k % 8;
+        d[k] = (int)L.src_y[o] - (int)L.pred[o];
+      }
+      quant8(d, qp, 0, L.code.luma8[i8], L.co8[i8]);
+      L.score[i8] = decimate_score8(L.code.luma8[i8]);
+    }
+    sync();
+    for (int i = lane(); i < 256; i += 64) cyv(i & 15, i >> 4) = L.pred[i];
+    sync();
+    const int total = L.score[0] + L.score[1] + L.score[2] + L.score[3];
+    if (lane() < 4) {
+      const int i8 = lane();
+      int n = 0;
+      if (L.score[i8] < 4 || total < 6) {
+        for (int k = 0; k < 64; k++) L.code.luma8[i8][k] = 0;
+      } else {
+        n = count_nz(L.code.luma8[i8], 64);
+        if (n > 0) idct8_add(L.co8[i8], &cyv((i8 & 1) * 8, (i8 >> 1) * 8), 28);
+        for (int j = 0; j < 4; j++) L.code.nnz[((i8 >> 1) * 2 + (j >> 1)) * 4 + (i8 & 1) * 2 + (j & 1)] = (uint8_t)n;
+      }
+      L.ival[i8] = n;
+    }
+    sync();
+    if (lane() == 0) {
+      int cbp = L.code.cbp;
+      for (int i8 = 0; i8 < 4; i8++)
+        if (!(L.score[i8] < 4 || total < 6) && L.ival[i8] > 0) cbp |= 1 << i8;
+      L.code.cbp = (uint8_t)cbp;
+      L.code.t8x8 = (cbp & 15) != 0;
+    }
+    sync();
+  }
+
+  // enc/frame.rs:1783-1805: luma and chroma prediction of an inter macroblock into L.pred/L.cpred
+  __device__ void inter_pred(int part, const uint8_t *sub, const int16_t (*mvs)[2], const uint8_t *refs) {
+    Rect rs[16];
+    const int n = rects(part, sub, rs);
+    const int cw = P.mb_w * 8, ch = P.mb_h * 8;
+    for (int k = 0; k < n; k++) {
+      const int bx = rs[k].x, by = rs[k].y, w = rs[k].w, h = rs[k].h;
+      const int r = refs[ref_of(part, k, bx, by)];
+      const int mx = mvs[k][0], my = mvs[k][1];
+      luma_pred(r, mbx * 16 + bx * 4, mby * 16 + by * 4, w * 4, h * 4, mx, my, L.pred + by * 4 * 16 + bx * 4, 16);
+      // chroma: eighth-sample bilinear from a window clamped to the plane (h264/inter.rs:377-422)
+      const int fx = mx & 7, fy = my & 7;
+      const int x0 = mbx * 8 + bx * 2 + (mx >> 3), y0 = mby * 8 + by * 2 + (my >> 3);
+      const int wa = (8 - fx) * (8 - fy), wb = fx * (8 - fy), wc = (8 - fx) * fy, wd = fx * fy;
+      const int cwn = w * 2, chn = h * 2;
+      for (int i = lane(); i < 2 * cwn * chn; i += 64) {
+        const int ci = i / (cwn * chn), j = i % (cwn * chn), rr = j / cwn, c = j % cwn;
+        const uint8_t *pl = ci == 0 ? P.rcb[r] : P.rcr[r];
+        const int X = x0 + c, Y = y0 + rr;
+#define S(xx, yy) ((int)pl[(size_t)clampi((yy), 0, ch - 1) * cw + clampi((xx), 0, cw - 1)])
+        const int v = (wa * S(X, Y) + wb * S(X + 1, Y) + wc * S(X, Y + 1) + wd * S(X + 1, Y + 1) + 32) >> 6;
+#undef S
+        L.cpred[ci][(by * 2 + rr) * 8 + bx * 2 + c] = (uint8_t)v;
+      }
+    }
+    sync();
+  }
+
+  // enc/frame.rs:1217-1252 (residual of an inter macroblock against L.pred/L.cpred)
+  __device__ void code_inter_residual(int t8) {
+    if (t8) {
+      code_luma8();
+    } else {
+      const int qp = P.qp;
+      if (lane() < 16) {
+        const int b = lane(), bx = b % 4, by = b / 4;
+        int d[16];
+        for (int i = 0; i < 16; i++) {
+          const int o = (by * 4 + (i >> 2)) * 16 + bx * 4 + (i & 3);
+          d[i] = (int)L.src_y[o] - (int)L.pred[o];
+        }
+        quant_block(d, qp, 0, 0, L.code.luma[b], L.co[b]);
+        L.score[b] = decimate_score(L.code.luma[b]);
+      }
+      sync();
+      for (int i = lane(); i < 256; i += 64) cyv(i & 15, i >> 4) = L.pred[i];
+      sync();
+      if (lane() < 16) {
+        const int b = lane(), bx = b % 4, by = b / 4, i8 = (by / 2) * 2 + bx / 2;
+        int s8[4] = {0, 0, 0, 0};
+        for (int q = 0; q < 16; q++) s8[((q / 4) / 2) * 2 + (q % 4) / 2] += L.score[q];
+        const int total = s8[0] + s8[1] + s8[2] + s8[3];
+        if (s8[i8] < 4 || total < 6)
+          for (int k = 0; k < 16; k++) { L.code.luma[b][k] = 0; L.co[b][k] = 0; }
+        const int nz = count_nz(L.code.luma[b], 16);
+        L.code.nnz[b] = (uint8_t)nz;
+        if (nz > 0) idct4_add(L.co[b], &cyv(bx * 4, by * 4), 28);
+        L.ival[b] = nz;
+      }
+      sync();
+      if (lane() == 0) {
+        int cbp = 0;
+        for (int b = 0; b < 16; b++)
+          if (L.ival[b] > 0) cbp |= 1 << (((b / 4) / 2) * 2 + (b % 4) / 2);
+        L.code.cbp = (uint8_t)cbp;
+      }
+      sync();
+    }
+    code_chroma(0);
+  }
+
+  // enc/frame.rs:1201-1214
+  __device__ void code_inter(int part, const int16_t (*mvs)[2], const uint8_t *refs, const uint8_t *sub, int t8) {
+    code_new(K_INTER);
+    int16_t mvps[16][2] = {};
+    part_mvps(part, sub, mvs, refs, mvps);
+    if (lane() == 0) {
+      L.code.part = (uint8_t)part;
+      for (int i = 0; i < 16; i++) {
+        L.code.mvs[i][0] = mvs[i][0];
+        L.code.mvs[i][1] = mvs[i][1];
+        L.code.mvds[i][0] = (int16_t)(mvs[i][0] - mvps[i][0]);
+        L.code.mvds[i][1] = (int16_t)(mvs[i][1] - mvps[i][1]);
+      }
+      for (int k = 0; k < 4; k++) { L.code.refs[k] = refs[k]; L.code.sub[k] = sub[k]; }
+    }
+    sync();
+    inter_pred(part, sub, mvs, refs);
+    code_inter_residual(t8);
+  }
+
+  // enc/frame.rs:1307-1388
+  __device__ void code_i8() {
+    code_new(K_I8);
+    if (lane() == 0) L.code.t8x8 = 1;
+    sync();
+    const int ta = up(), la = mbx > 0, tla = mbx > 0 && ta, tra = ta && mbx + 1 < P.mb_w;
+    const int qp = P.qp;
+    for (int i8 = 0; i8 < 4; i8++) {
+      const int bx = i8 & 1, by = i8 >> 1, x = bx * 8, y = by * 8;
+      const int have_left = bx > 0 || la, have_top = by > 0 || ta;
+      int have_tl;
+      if (bx > 0 && by > 0) have_tl = 1;
+      else if (by > 0) have_tl = la;
+      else if (bx > 0) have_tl = ta;
+      else have_tl = tla;
+      const int have_tr = i8 == 0 ? ta : i8 == 1 ? tra : i8 == 2 ? 1 : 0;
+      int pt[16] = {}, pl[8] = {}, ptl = 0;
+      if (have_top) {
+        for (int i = 0; i < 8; i++) pt[i] = cyv(x + i, y - 1);
+        for (int i = 8; i < 16; i++) pt[i] = have_tr ? cyv(x + i, y - 1) : pt[7];
+      }
+      if (have_left)
+        for (int j = 0; j < 8; j++) pl[j] = cyv(x - 1, y + j);
+      if (have_tl) ptl = cyv(x - 1, y - 1);
+      // reference-sample smoothing (h264/intra.rs:130-159)
+      int top[16], left[8], tl = ptl;
+      for (int i = 0; i < 16; i++) top[i] = pt[i];
+      for (int j = 0; j < 8; j++) left[j] = pl[j];
+      if (have_top) {
+        top[0] = have_tl ? (ptl + 2 * pt[0] + pt[1] + 2) >> 2 : (3 * pt[0] + pt[1] + 2) >> 2;
+        for (int i = 1; i < 15; i++) top[i] = (pt[i - 1] + 2 * pt[i] + pt[i + 1] + 2) >> 2;
+        top[15] = (pt[14] + 3 * pt[15] + 2) >> 2;
+      }
+      if (have_tl) {
+        if (have_top && have_left) tl = (pt[0] + 2 * ptl + pl[0] + 2) >> 2;
+        else if (have_top) tl = (3 * ptl + pt[0] + 2) >> 2;
+        else if (have_left) tl = (3 * ptl + pl[0] + 2) >> 2;
+      }
+      if (have_left) {
+        left[0] = have_tl ? (ptl + 2 * pl[0] + pl[1] + 2) >> 2 : (3 * pl[0] + pl[1] + 2) >> 2;
+        for (int j = 1; j < 7; j++) left[j] = (pl[j - 1] + 2 * pl[j] + pl[j + 1] + 2) >> 2;
+        left[7] = (pl[6] + 3 * pl[7] + 2) >> 2;
+      }
+      const int predicted = pred_i4_mode(L.code.modes, bx * 2, by * 2);
+      int c = 0x7fffffff;
+      const int m = lane();
+      if (m < 9 && mode_ok(m, have_top, have_left, have_tl)) {
+        int s = 0;
+        for (int q = 0; q < 4; q++) {
+          const int ox = (q & 1) * 4, oy = (q >> 1) * 4;
+          int d[16];
+          for (int i = 0; i < 16; i++) {
+            const int xx = ox + (i & 3), yy = oy + (i >> 2);
+            d[i] = (int)L.src_y[(y + yy) * 16 + x + xx] - nxn_sample(8, m, xx, yy, top, left, tl, have_top, have_left);
+          }
+          s += satd4(d);
+        }
+        c = s + P.lambda * (m == predicted ? 1 : 4);
+      }
+      int best_m = 2, best_c = 0x7fffffff;
+      for (int k = 0; k < 9; k++) {
+        int ck = __builtin_amdgcn_readlane(c, k);
+        if (ck < best_c) { best_c = ck; best_m = k; }
+      }
+      if (lane() == 0) {
+        int d[64];
+        for (int i = 0; i < 64; i++) {
+          const int pv = nxn_sample(8, best_m, i & 7, i >> 3, top, left, tl, have_top, have_left);
+          d[i] = (int)L.src_y[(y + (i >> 3)) * 16 + x + (i & 7)] - pv;
+          cyv(x + (i & 7), y + (i >> 3)) = (uint8_t)pv;
+        }
+        quant8(d, qp, 1, L.code.luma8[i8], L.co8[i8]);
+        const int n = count_nz(L.code.luma8[i8], 64);
+        if (n > 0) {
+          idct8_add(L.co8[i8], &cyv(x, y), 28);
+          L.code.cbp |= (uint8_t)(1 << i8);
+        }
+        for (int j = 0; j < 4; j++) {
+          const int b = (by * 2 + (j >> 1)) * 4 + bx * 2 + (j & 1);
+          L.code.modes[b] = (uint8_t)best_m;
+          L.code.nnz[b] = (uint8_t)n;
+        }
+      }
+      sync();
+    }
+  }
+
+  // enc/frame.rs:1807-1852
+  __device__ void code_intra_chroma() {
+    const int la = mbx > 0, ta = up();
+    // lane (mode, c) < 8: the chroma prediction's SATD of component c under mode m
+    int cst = 0;
+    const int m = lane() >> 1, ci = lane() & 1;
+    int okm = 0;
+    if (lane() < 8) {
+      okm = m == 0 ? 1 : m == 1 ? la : m == 2 ? ta : (la && ta);
+      if (okm) {
+        for (int q = 0; q < 4; q++) {
+          const int ox = (q & 1) * 4, oy = (q >> 1) * 4;
+          int d[16];
+          for (int i = 0; i < 16; i++) {
+            const int xx = ox + (i & 3), yy = oy + (i >> 2);
+            d[i] = (int)L.src_c[ci][yy * 8 + xx] - chroma_sample(ci, m, xx, yy, ta, la);
+          }
+          cst += satd4(d);
+        }
+      }
+    }
+    int best_m = 0, best_c = 0x7fffffff;
+    for (int mm = 0; mm < 4; mm++) {
+      const int ok = mm == 0 ? 1 : mm == 1 ? la : mm == 2 ? ta : (la && ta);
+      if (!ok) continue;
+      const int cost = P.lambda * (mm == 0 ? 1 : 3) + __builtin_amdgcn_readlane(cst, 2 * mm) + __builtin_amdgcn_readlane(cst, 2 * mm + 1);
+      if (cost < best_c) { best_c = cost; best_m = mm; }
+    }
+    for (int i = lane(); i < 128; i += 64) {
+      const int c2 = i >> 6, o = i & 63;
+      L.cpred[c2][o] = (uint8_t)chroma_sample(c2, best_m, o & 7, o >> 3, ta, la);
+    }
+    if (lane() == 0) L.code.chroma_mode = (uint8_t)best_m;
+    sync();
+    code_chroma(1);
+  }
+
+  // chroma intra prediction of sample (x, y) from the canvas (h264/intra.rs:212-287, 8x8)
+  __device__ int chroma_sample(int ci, int mode, int x, int y, int ta, int la) {
+    if (mode == 1) return la ? ccv(ci, -1, y) : 0;
+    if (mode == 2) return ta ? ccv(ci, x, -1) : 0;
+    if (mode == 0) {
+      const int bx = x >> 2, by = y >> 2;
+      int st = 0, sl = 0;
+      for (int i = 0; i < 4; i++) {
+        st += ta ? ccv(ci, bx * 4 + i, -1) : 0;
+        sl += la ? ccv(ci, -1, by * 4 + i) : 0;
+      }
+      if ((bx == 0 && by == 0) || (bx > 0 && by > 0)) {
+        if (ta && la) return (st + sl + 4) >> 3;
+        if (la) return (sl + 2) >> 2;
+        if (ta) return (st + 2) >> 2;
+        return 128;
+      }
+      if (bx > 0) {
+        if (ta) return (st + 2) >> 2;
+        if (la) return (sl + 2) >> 2;
+        return 128;
+      }
+      if (la) return (sl + 2) >> 2;
+      if (ta) return (st + 2) >> 2;
+      return 128;
+    }
+    // plane
+    const int tl = (la && ta) ? ccv(ci, -1, -1) : 0;
+    auto tt = [&](int i) { return i < 0 ? tl : (ta ? (int)ccv(ci, i, -1) : 0); };
+    auto lf = [&](int j) { return j < 0 ? tl : (la ? (int)ccv(ci, -1, j) : 0); };
+    int hh = 0, vv = 0;
+    for (int k = 0; k < 4; k++) {
+      hh += (k + 1) * (tt(4 + k) - tt(2 - k));
+      vv += (k + 1) * (lf(4 + k) - lf(2 - k));
+    }
+    const int a = 16 * (lf(7) + tt(7)), b = (34 * hh + 32) >> 6, c = (34 * vv + 32) >> 6;
+    return clampi((a + b * (x - 3) + c * (y - 3) + 16) >> 5, 0, 255);
+  }
+
+  // ---- rate-distortion decision (enc/frame.rs:1391-1513) ----
+
+  __device__ void save_rec(uint8_t *rec) {
+    for (int i = lane(); i < 384; i += 64) {
+      if (i < 256) rec[i] = cyv(i & 15, i >> 4);
+      else {
+        const int ci = (i - 256) >> 6, o = (i - 256) & 63;
+        rec[i] = ccv(ci, o & 7, o >> 3);
+      }
+    }
+  }
+
+  __device__ void restore_rec(const uint8_t *rec) {
+    for (int i = lane(); i < 384; i += 64) {
+      if (i < 256) cyv(i & 15, i >> 4) = rec[i];
+      else {
+        const int ci = (i - 256) >> 6, o = (i - 256) & 63;
+        ccv(ci, o & 7, o >> 3) = rec[i];
+      }
+    }
+  }
+
+  __device__ long long ssd_mb() {
+    int s = 0;
+    for (int i = lane(); i < 384; i += 64) {
+      int a, b;
+      if (i < 256) { a = L.src_y[i]; b = cyv(i & 15, i >> 4); }
+      else {
+        const int ci = (i - 256) >> 6, o = (i - 256) & 63;
+        a = L.src_c[ci][o];
+        b = ccv(ci, o & 7, o >> 3);
+      }
+      s += (a - b) * (a - b);
+    }
+    return (long long)wave_sum(s);  // at most 384 * 255^2 < 2^31
+  }
+
+  // ---- CABAC bins (enc/frame.rs:2028-2279, enc/cabac.rs:150-246), counted or applied ----
+
+  struct Bins {
+    uint8_t *state;
+    uint32_t bits;
+    const uint32_t *ent;
+    __device__ void decision(int ctx, int bin) {
+      const uint8_t s = state[ctx];
+      bits += ent[(s >> 1) * 2 + (bin != (s & 1))];
+      const int p = s >> 1, mps = s & 1;
+      uint8_t ns;
+      if (bin != mps) ns = (uint8_t)((IA_TRANS_LPS[p] << 1) | (p == 0 ? !mps : mps));
+      else ns = (uint8_t)(((p < 62 ? p + 1 : p) << 1) | mps);
+      state[ctx] = ns;
+    }
+    __device__ void bypass(int) { bits += 256; }
+    __device__ void terminate(int bin) { bits += bin ? 7 * 256 : 1; }
+    __device__ void exp_golomb(uint32_t v, uint32_t k) {
+      while (v >= (1u << k)) {
+        bypass(1);
+        v -= 1u << k;
+        k++;
+      }
+      bypass(0);
+      while (k > 0) {
+        k--;
+        bypass((v >> k) & 1);
+      }
+    }
+  };
+
+  __device__ static void write_block(Bins &c, int cat, int cbf_ctx, const int16_t *levels, int max) {
+    int total = 0, last = -1;
+    for (int i = 0; i < max; i++)
+      if (levels[i] != 0) { total++; last = i; }
+    if (cbf_ctx >= 0) c.decision(cbf_ctx, total > 0);
+    if (total == 0) return;
+    for (int i = 0; i < max - 1; i++) {
+      int sig, lst;
+      if (cat == 3) { sig = min(i, 2); lst = min(i, 2); }
+      else if (cat == 5) { sig = IA_SIG_OFF_8X8[0][i]; lst = IA_LAST_OFF_8X8[i]; }
+      else { sig = i; lst = i; }
+      const int l = levels[i];
+      c.decision(IA_SIG_OFF[0][cat] + sig, l != 0);
+      if (l != 0) {
+        c.decision(IA_LAST_OFF[0][cat] + lst, i == last);
+        if (i == last) break;
+      }
+    }
+    const int base = IA_ABS_OFF[cat];
+    int eq1 = 0, gt1 = 0;
+    for (int i = last; i >= 0; i--) {
+      const int l = levels[i];
+      if (l == 0) continue;
+      const uint32_t a = (uint32_t)iabs(l);
+      const int inc0 = gt1 != 0 ? 0 : min(1 + eq1, 4);
+      if (a == 1) {
+        c.decision(base + inc0, 0);
+        eq1++;
+      } else {
+        c.decision(base + inc0, 1);
+        const int incn = base + 5 + min(gt1, 4 - (cat == 3));
+        for (int k = 0; k < min((int)a - 2, 13); k++) c.decision(incn, 1);
+        if (a < 15) c.decision(incn, 0);
+        else c.exp_golomb(a - 15, 0);
+        gt1++;
+      }
+      c.bypass(l < 0);
+    }
+  }
+
+  __device__ static void mvd_bins(Bins &c, int comp, int d, uint32_t sum) {
+    const int base = comp == 0 ? 40 : 47;
+    const int inc = sum < 3 ? 0 : (sum > 32 ? 2 : 1);
+    const uint32_t a = (uint32_t)iabs(d);
+    c.decision(base + inc, a > 0);
+    if (a > 0) {
+      int ctx = base + 3;
+      for (uint32_t k = 1; k < (a < 9 ? a : 9); k++) {
+        c.decision(ctx, 1);
+        if (ctx < base + 6) ctx++;
+      }
+      if (a < 9) c.decision(ctx, 0);
+      else c.exp_golomb(a - 9, 3);
+      c.bypass(d < 0);
+    }
+  }
+
+  __device__ void mb_bins(Bins &b, const ia_mbcode &code) const {
+    const MbState *left = nb(-1, 0), *top = nb(0, -1);
+    const int kind = code.kind;
+    if (P.p_slice) {
+      b.decision(11 + (left && !left->skip) + (top && !top->skip), code.skip);
+      if (code.skip) return;
+    }
+    const int cbpl = code.cbp & 15, cbpc = code.cbp >> 4, prefix_ok = P.p_slice, base = P.p_slice ? 17 : 3;
+    if (kind == K_INTER) {
+      const int b1 = code.part == 1 || code.part == 2, b2 = code.part == 1 || code.part == 3;
+      b.decision(14, 0);
+      b.decision(15, b1);
+      b.decision(b1 ? 17 : 16, b2);
+    } else {
+      int first;
+      if (prefix_ok) {
+        b.decision(14, 1);
+        first = 17;
+      } else {
+        first = 3 + (left && !(left->kind == K_I4 || left->kind == K_I8)) + (top && !(top->kind == K_I4 || top->kind == K_I8));
+      }
+      if (kind == K_I4 || kind == K_I8) {
+        b.decision(first, 0);
+        if (P.transform8) b.decision(399 + (left && left->t8x8) + (top && top->t8x8), kind == K_I8);
+      } else {
+        b.decision(first, 1);
+        b.terminate(0);
+        const int b1 = prefix_ok ? base + 1 : base + 3, b2 = prefix_ok ? base + 2 : base + 4, b3 = prefix_ok ? base + 3 : base + 6;
+        b.decision(b1, cbpl != 0);
+        b.decision(b2, cbpc != 0);
+        if (cbpc != 0) b.decision(prefix_ok ? base + 2 : base + 5, cbpc == 2);
+        b.decision(b3, (code.i16_mode >> 1) == 1);
+        b.decision(prefix_ok ? base + 3 : base + 7, (code.i16_mode & 1) == 1);
+      }
+    }
+    if (kind == K_I4 || kind == K_I8) {
+      const int t8 = kind == K_I8;
+      uint8_t so_far[16];
+      for (int i = 0; i < 16; i++) so_far[i] = 2;
+      for (int n = 0; n < (t8 ? 4 : 16); n++) {
+        int bx, by;
+        if (t8) { bx = (n & 1) * 2; by = (n >> 1) * 2; }
+        else block_pos(n, &bx, &by);
+        const int mode = code.modes[by * 4 + bx];
+        const int pred = pred_i4_mode(so_far, bx, by);
+        so_far[by * 4 + bx] = (uint8_t)mode;
+        if (t8) {
+          so_far[by * 4 + bx + 1] = (uint8_t)mode;
+          so_far[(by + 1) * 4 + bx] = (uint8_t)mode;
+          so_far[(by + 1) * 4 + bx + 1] = (uint8_t)mode;
+        }
+        b.decision(68, mode == pred);
+        if (mode != pred) {
+          const int rem = mode < pred ? mode : mode - 1;
+          for (int k = 0; k < 3; k++) b.decision(69, (rem >> k) & 1);
+        }
+      }
+    }
+    if (kind != K_INTER) {
+      const int m = code.chroma_mode;
+      const int ca = left && !left->inter && left->chroma_mode != 0, cb = top && !top->inter && top->chroma_mode != 0;
+      b.decision(64 + ca + cb, m > 0);
+      if (m > 0) {
+        b.decision(67, m > 1);
+        if (m > 1) b.decision(67, m > 2);
+      }
+    } else {
+      if (code.part == 3) {
+        for (int k = 0; k < 4; k++) {
+          const int s = code.sub[k];
+          if (s == 0) b.decision(21, 1);
+          else if (s == 1) { b.decision(21, 0); b.decision(22, 0); }
+          else if (s == 2) { b.decision(21, 0); b.decision(22, 1); b.decision(23, 1); }
+          else { b.decision(21, 0); b.decision(22, 1); b.decision(23, 0); }
+        }
+      }
+      if (P.nref > 1) {
+        uint8_t grid[4] = {0, 0, 0, 0};
+        Rect parts[4];
+        int np;
+        if (code.part == 3) {
+          np = 4;
+          for (int k = 0; k < 4; k++) parts[k] = Rect{(k & 1) * 2, (k >> 1) * 2, 2, 2};
+        } else {
+          np = NPARTS[code.part];
+          for (int k = 0; k < np; k++) parts[k] = Rect{PARTS[code.part][k][0], PARTS[code.part][k][1], PARTS[code.part][k][2], PARTS[code.part][k][3]};
+        }
+        for (int k = 0; k < np; k++) {
+          const int bx = parts[k].x, by = parts[k].y, w = parts[k].w, h = parts[k].h;
+          const int a = bx > 0 ? grid[(by / 2) * 2 + (bx - 1) / 2] > 0 : (left && left->refs[(by / 2) * 2 + 1] > 0);
+          const int bb = by > 0 ? grid[((by - 1) / 2) * 2 + bx / 2] > 0 : (top && top->refs[2 + bx / 2] > 0);
+          const int r = code.refs[k], cbase = 54 + a + 2 * bb;
+          for (int i = 0; i <= r; i++) b.decision(i == 0 ? cbase : (i == 1 ? 58 : 59), i < r);
+          for (int y8 = by / 2; y8 <= (by + h - 1) / 2; y8++)
+            for (int x8 = bx / 2; x8 <= (bx + w - 1) / 2; x8++) grid[y8 * 2 + x8] = code.refs[k];
+        }
+      }
+      Rect rs[16];
+      const int n = rects(code.part, code.sub, rs);
+      uint8_t grid[16][2];
+      mvd_grid(rs, n, code.mvds, grid);
+      for (int k = 0; k < n; k++) {
+        const int bx = rs[k].x, by = rs[k].y;
+        for (int comp = 0; comp < 2; comp++) {
+          const int a = bx > 0 ? grid[by * 4 + bx - 1][comp] : (left ? left->mvd[by * 4 + 3][comp] : 0);
+          const int bb = by > 0 ? grid[(by - 1) * 4 + bx][comp] : (top ? top->mvd[12 + bx][comp] : 0);
+          mvd_bins(b, comp, code.mvds[k][comp], (uint32_t)(a + bb));
+        }
+      }
+    }
+    if (kind != K_I16) {
+      int luma = 0;
+      for (int b8 = 0; b8 < 4; b8++) {
+        const int cond_a = (b8 & 1) ? ((luma >> (b8 - 1)) & 1) == 0 : (left && ((left->cbp >> (b8 + 1)) & 1) == 0);
+        const int cond_b = b8 >= 2 ? ((luma >> (b8 - 2)) & 1) == 0 : (top && ((top->cbp >> (b8 + 2)) & 1) == 0);
+        const int bit = (cbpl >> b8) & 1;
+        b.decision(73 + cond_a + 2 * cond_b, bit);
+        luma |= bit << b8;
+      }
+      int ca = left && (left->cbp >> 4) != 0, cb = top && (top->cbp >> 4) != 0;
+      b.decision(77 + ca + 2 * cb, cbpc != 0);
+      if (cbpc != 0) {
+        ca = left && (left->cbp >> 4) == 2;
+        cb = top && (top->cbp >> 4) == 2;
+        b.decision(81 + ca + 2 * cb, cbpc == 2);
+      }
+    }
+    if (kind == K_INTER && P.transform8 && cbpl != 0 && (code.part != 3 || (code.sub[0] | code.sub[1] | code.sub[2] | code.sub[3]) == 0))
+      b.decision(399 + (left && left->t8x8) + (top && top->t8x8), code.t8x8);
+    if (code.cbp != 0 || kind == K_I16) {
+      b.decision(60, 0);
+      const int un = kind != K_INTER;
+      if (kind == K_I16) {
+        const int da = left ? (left->cbf_dc & 1) != 0 : un, db = top ? (top->cbf_dc & 1) != 0 : un;
+        write_block(b, 0, 85 + da + 2 * db, code.dc, 16);
+      }
+      const int cat = kind == K_I16 ? 1 : 2, cbase = kind == K_I16 ? 85 + 4 : 85 + 8;
+      for (int i8 = 0; i8 < 4; i8++)
+        if (code.t8x8 && (cbpl & (1 << i8))) write_block(b, 5, -1, code.luma8[i8], 64);
+      for (int n = 0; n < 16; n++) {
+        int bx, by;
+        block_pos(n, &bx, &by);
+        if (code.t8x8 || !(cbpl & (1 << (n / 4)))) continue;
+        const int a = bx > 0 ? code.nnz[by * 4 + bx - 1] > 0 : (left ? left->nnz[by * 4 + 3] > 0 : un);
+        const int bb = by > 0 ? code.nnz[(by - 1) * 4 + bx] > 0 : (top ? top->nnz[12 + bx] > 0 : un);
+        const int16_t *blk = code.luma[by * 4 + bx];
+        if (cat == 1) write_block(b, cat, cbase + a + 2 * bb, blk + 1, 15);
+        else write_block(b, cat, cbase + a + 2 * bb, blk, 16);
+      }
+      if (cbpc > 0)
+        for (int comp = 0; comp < 2; comp++) {
+          const int da = left ? (left->cbf_dc & (2 << comp)) != 0 : un, db = top ? (top->cbf_dc & (2 << comp)) != 0 : un;
+          write_block(b, 3, 85 + 12 + da + 2 * db, code.cdc[comp], 4);
+        }
+      if (cbpc == 2)
+        for (int comp = 0; comp < 2; comp++)
+          for (int blk = 0; blk < 4; blk++) {
+            const int bx = blk & 1, by = blk >> 1;
+            const uint8_t *cur = code.cnnz[comp];
+            const int a = bx > 0 ? cur[by * 2] > 0 : (left ? left->cnnz[comp][by * 2 + 1] > 0 : un);
+            const int bb = by > 0 ? cur[bx] > 0 : (top ? top->cnnz[comp][2 + bx] > 0 : un);
+            write_block(b, 4, 85 + 16 + a + 2 * bb, code.cac[comp][blk] + 1, 15);
+          }
+    }
+  }
+
+  __device__ static void mvd_grid(const Rect *rs, int n, const int16_t (*mvds)[2], uint8_t (*out)[2]) {
+    for (int i = 0; i < 16; i++) out[i][0] = out[i][1] = 0;
+    for (int k = 0; k < n; k++)
+      for (int by = rs[k].y; by < rs[k].y + rs[k].h; by++)
+        for (int bx = rs[k].x; bx < rs[k].x + rs[k].w; bx++)
+          for (int q = 0; q < 2; q++) out[by * 4 + bx][q] = (uint8_t)min(iabs(mvds[k][q]), 64);
+  }
+
+  // J of the candidate in L.code whose reconstruction is in the canvas (enc/frame.rs:1453-1457)
+  __device__ double rd_cost(double lam2) {
+    for (int i = lane(); i < 1024; i += 64) L.tmp[i] = L.cabac[i];
+    sync();
+    uint32_t bits = 0;
+    if (lane() == 0) {
+      Bins b{L.tmp, 0, P.entropy};
+      mb_bins(b, L.code);
+      L.ival[0] = (int)b.bits;
+    }
+    sync();
+    bits = (uint32_t)L.ival[0];
+    const long long ssd = ssd_mb();
+    sync();
+    return (double)ssd + lam2 * ((double)bits / 256.0);
+  }
+
+  __device__ void copy_code(ia_mbcode *dst, const ia_mbcode *src) {
+    const uint32_t *s = reinterpret_cast<const uint32_t *>(src);
+    uint32_t *d = reinterpret_cast<uint32_t *>(dst);
+    for (int i = lane(); i < (int)(sizeof(ia_mbcode) / 4); i += 64) d[i] = s[i];
+  }
+
+  __device__ void keep(double j, int *have, double *best_j) {
+    if (!*have || j < *best_j) {
+      *have = 1;
+      *best_j = j;
+      copy_code(&L.keep, &L.code);
+      save_rec(L.keep_rec);
+    }
+    sync();
+  }
+
+  __device__ void decide_rd(int i16, const Inter *inter, ia_mbstats *st) {
+    const double lam2 = P.lam2;
+    int have = 0;
+    double best_j = 0.0;
+    if (inter) {
+      int16_t sm[2];
+      skip_mv(sm);
+      int16_t mvs[16][2];
+      for (int i = 0; i < 16; i++) { mvs[i][0] = sm[0]; mvs[i][1] = sm[1]; }
+      const uint8_t z4[4] = {0, 0, 0, 0};
+      code_new(K_INTER);
+      if (lane() == 0) {
+        L.code.skip = 1;
+        for (int i = 0; i < 16; i++) { L.code.mvs[i][0] = sm[0]; L.code.mvs[i][1] = sm[1]; }
+      }
+      sync();
+      inter_pred(0, z4, mvs, z4);
+      for (int i = lane(); i < 256; i += 64) cyv(i & 15, i >> 4) = L.pred[i];
+      for (int i = lane(); i < 128; i += 64) ccv(i >> 6, (i & 63) & 7, (i & 63) >> 3) = L.cpred[i >> 6][i & 63];
+      sync();
+      keep(rd_cost(lam2), &have, &best_j);
+      code_inter(inter->part, inter->mvs, inter->refs, inter->sub, 0);
+      if (!(L.code.cbp == 0 && inter->part == 0 && inter->mvs[0][0] == sm[0] && inter->mvs[0][1] == sm[1] && inter->refs[0] == 0))
+        keep(rd_cost(lam2), &have, &best_j);
+      if (P.transform8 && (inter->sub[0] | inter->sub[1] | inter->sub[2] | inter->sub[3]) == 0) {
+        code_inter(inter->part, inter->mvs, inter->refs, inter->sub, 1);
+        if (L.code.t8x8) keep(rd_cost(lam2), &have, &best_j);
+      }
+    }
+    code_i16(i16);
+    code_intra_chroma();
+    keep(rd_cost(lam2), &have, &best_j);
+    code_i4();
+    code_intra_chroma();
+    keep(rd_cost(lam2), &have, &best_j);
+    if (P.transform8) {
+      code_i8();
+      code_intra_chroma();
+      keep(rd_cost(lam2), &have, &best_j);
+    }
+    restore_rec(L.keep_rec);
+    copy_code(&L.code, &L.keep);
+    sync();
+    st->rd_cost = best_j;
+  }
+
+  // enc/frame.rs:715-790; the decision lands in L.code and the canvas
+  __device__ void code_mb(ia_mbstats *st) {
+    const int lam = P.lambda;
+    st->skip_sad = -1;
+    st->i16_cost = -1;
+    st->inter_cost = -1;
+    st->inter_part = -1;
+    st->rd_cost = 0.0;
+    if (P.p_slice) {
+      int16_t smv[2];
+      skip_mv(smv);
+      luma_pred(0, mbx * 16, mby * 16, 16, 16, smv[0], smv[1], L.pred, 16);
+      sync();
+      int s = 0;
+      for (int i = lane(); i < 256; i += 64) s += iabs((int)L.src_y[i] - (int)L.pred[i]);
+      const int sad = wave_sum(s);
+      sync();
+      st->skip_sad = sad;
+      if ((double)sad < P.skip_thresh) {
+        int16_t mvs[16][2];
+        for (int i = 0; i < 16; i++) { mvs[i][0] = smv[0]; mvs[i][1] = smv[1]; }
+        const uint8_t z4[4] = {0, 0, 0, 0};
+        code_inter(0, mvs, z4, z4, 0);
+        if (L.code.cbp == 0) {
+          if (lane() == 0) L.code.skip = 1;
+          sync();
+          return;
+        }
+      }
+    }
+    int i16c;
+    const int i16 = best_i16(&i16c);
+    st->i16_cost = i16c;
+    int best_intra = i16c + lam * 4;
+    Inter inter;
+    if (P.p_slice) {
+      best_inter(inter);
+      st->inter_cost = inter.cost;
+      st->inter_part = inter.part;
+    }
+    if (P.use_cabac) {
+      decide_rd(i16, P.p_slice ? &inter : nullptr, st);
+      return;
+    }
+    // SATD decisions (CAVLC), enc/frame.rs:760-789
+    const int try_i4 = !P.p_slice || best_intra < inter.cost;
+    int have_i4 = 0;
+    if (try_i4) {
+      const int cost = code_i4() + lam * 6;
+      if (cost < best_intra) {
+        best_intra = cost;
+        have_i4 = 1;
+        copy_code(&L.i4, &L.code);
+        save_rec(L.keep_rec);
+        sync();
+      }
+    }
+    if (P.p_slice && inter.cost <= best_intra) {
+      code_inter(inter.part, inter.mvs, inter.refs, inter.sub, 0);
+      int16_t sm[2];
+      skip_mv(sm);
+      if (L.code.cbp == 0 && inter.part == 0 && inter.mvs[0][0] == sm[0] && inter.mvs[0][1] == sm[1] && inter.refs[0] == 0) {
+        if (lane() == 0) L.code.skip = 1;
+        sync();
+      }
+      return;
+    }
+    if (have_i4) {
+      copy_code(&L.code, &L.i4);
+      restore_rec(L.keep_rec);
+      sync();
+    } else {
+      code_i16(i16);
+    }
+    code_intra_chroma();
+  }
+
+  // enc/frame.rs:2281-2339: the macroblock's state for later macroblocks and its MbInfo
+  __device__ void finish_mb(const ia_mbcode &code, MbState &s, ia_mbinfo &info) const {
+    const int inter = code.kind == K_INTER;
+    Rect rs[16];
+    const int n = rects(code.part, code.sub, rs);
+    s = MbState{};
+    if (inter && !code.skip) mvd_grid(rs, n, code.mvds, s.mvd);
+    if (inter)
+      for (int k = 0; k < n; k++)
+        for (int by = rs[k].y; by < rs[k].y + rs[k].h; by++)
+          for (int bx = rs[k].x; bx < rs[k].x + rs[k].w; bx++) {
+            s.mvs[by * 4 + bx][0] = code.mvs[k][0];
+            s.mvs[by * 4 + bx][1] = code.mvs[k][1];
+          }
+    const int cbpc = code.cbp >> 4;
+    s.cbf_dc = (uint8_t)((code.kind == K_I16 && count_nz(code.dc, 16) > 0) | ((cbpc > 0 && count_nz(code.cdc[0], 4) > 0) << 1) |
+                         ((cbpc > 0 && count_nz(code.cdc[1], 4) > 0) << 2));
+    int8_t r16[16];
+    for (int k = 0; k < NPARTS[code.part]; k++) {
+      const uint8_t *pr = PARTS[code.part][k];
+      for (int by = pr[1]; by < pr[1] + pr[3]; by++)
+        for (int bx = pr[0]; bx < pr[0] + pr[2]; bx++) r16[by * 4 + bx] = (int8_t)code.refs[k];
+    }
+    int8_t r8[4];
+    for (int b8 = 0; b8 < 4; b8++) r8[b8] = r16[(b8 / 2) * 8 + (b8 % 2) * 2];
+    s.i4 = code.kind == K_I4 || code.kind == K_I8;
+    s.t8x8 = code.t8x8;
+    s.inter = (uint8_t)inter;
+    for (int i = 0; i < 16; i++) {
+      s.modes[i] = code.modes[i];
+      s.nnz[i] = code.nnz[i];
+    }
+    for (int c = 0; c < 2; c++)
+      for (int k = 0; k < 4; k++) s.cnnz[c][k] = code.cnnz[c][k];
+    for (int b8 = 0; b8 < 4; b8++) s.refs[b8] = inter ? r8[b8] : -1;
+    s.skip = code.skip;
+    s.kind = code.kind;
+    s.cbp = code.cbp;
+    s.chroma_mode = inter ? 0 : code.chroma_mode;
+    info = ia_mbinfo{};
+    info.slice = (uint16_t)slice_id;
+    info.kind = code.kind;
+    info.qp = (int8_t)P.qp_slice;
+    for (int i = 0; i < 16; i++) info.nnz[i] = code.nnz[i];
+    info.cbp = code.cbp;
+    info.skip = code.skip;
+    info.t8x8 = code.t8x8;
+    for (int i = 0; i < 16; i++) {
+      info.mv[i][0] = s.mvs[i][0];
+      info.mv[i][1] = s.mvs[i][1];
+    }
+    for (int b8 = 0; b8 < 4; b8++) {
+      info.ref_idx[b8] = inter ? r8[b8] : -1;
+      info.ref_id[b8] = inter ? 1 + r8[b8] : -1;
+    }
+  }
+
+  // ---- the slice loop (enc/frame.rs:452-487) ----
+
+  __device__ void load_mb() {
+    const int ls_ = ls(), cs_ = cs();
+    const int x0 = mbx * 16, y0 = mby * 16;
+    for (int i = lane(); i < 256; i += 64) L.src_y[i] = P.src[0][(size_t)(y0 + (i >> 4)) * ls_ + x0 + (i & 15)];
+    for (int i = lane(); i < 128; i += 64) {
+      const int ci = i >> 6, o = i & 63;
+      L.src_c[ci][o] = P.src[1 + ci][(size_t)(mby * 8 + (o >> 3)) * cs_ + mbx * 8 + (o & 7)];
+    }
+    // canvas edges from the reconstruction of earlier macroblocks of the slice
+    const int ta = up(), la = mbx > 0;
+    for (int i = lane(); i < 28; i += 64) {
+      const int x = i - 1;  // -1..26
+      uint8_t v = 0;
+      if (ta && x0 + x >= 0 && x0 + x < ls_ && x <= 23 && (x >= 0 || la)) v = P.rec[0][(size_t)(y0 - 1) * ls_ + x0 + x];
+      L.cy[0][i] = v;
+    }
+    for (int i = lane(); i < 16; i += 64) L.cy[i + 1][0] = la ? P.rec[0][(size_t)(y0 + i) * ls_ + x0 - 1] : 0;
+    for (int i = lane(); i < 2 * 9; i += 64) {
+      const int ci = i / 9, x = i % 9 - 1;
+      uint8_t v = 0;
+      if (ta && (x >= 0 || la)) v = P.rec[1 + ci][(size_t)(mby * 8 - 1) * cs_ + mbx * 8 + x];
+      L.cc[ci][0][x + 1] = v;
+    }
+    for (int i = lane(); i < 16; i += 64) {
+      const int ci = i >> 3, y = i & 7;
+      L.cc[ci][y + 1][0] = la ? P.rec[1 + ci][(size_t)(mby * 8 + y) * cs_ + mbx * 8 - 1] : 0;
+    }
+    // neighbour states: left, top, top-right, top-left
+    const int nx[4] = {mbx - 1, mbx, mbx + 1, mbx - 1}, ny[4] = {mby, mby - 1, mby - 1, mby - 1};
+    for (int k = 0; k < 4; k++) {
+      const int ok = nx[k] >= 0 && nx[k] < P.mb_w && ny[k] >= row0 && ny[k] < P.mb_h;
+      if (lane() == 0) {
+        L.nb_ok[k] = ok;
+        if (ok) L.nb[k] = P.state[ny[k] * P.mb_w + nx[k]];
+      }
+    }
+    sync();
+  }
+
+  __device__ void store_mb(int idx, const ia_mbstats &st) {
+    const int ls_ = ls(), cs_ = cs();
+    const int x0 = mbx * 16, y0 = mby * 16;
+    for (int i = lane(); i < 256; i += 64) P.rec[0][(size_t)(y0 + (i >> 4)) * ls_ + x0 + (i & 15)] = cyv(i & 15, i >> 4);
+    for (int i = lane(); i < 128; i += 64) {
+      const int ci = i >> 6, o = i & 63;
+      P.rec[1 + ci][(size_t)(mby * 8 + (o >> 3)) * cs_ + mbx * 8 + (o & 7)] = ccv(ci, o & 7, o >> 3);
+    }
+    // the code as it will be written
+    const uint32_t *s = reinterpret_cast<const uint32_t *>(&L.code);
+    uint32_t *d = reinterpret_cast<uint32_t *>(&P.codes[idx]);
+    for (int i = lane(); i < (int)(sizeof(ia_mbcode) / 4); i += 64) d[i] = s[i];
+    if (lane() == 0) {
+      MbState ms;
+      ia_mbinfo info;
+      finish_mb(L.code, ms, info);
+      P.state[idx] = ms;
+      P.mbs[idx] = info;
+      P.stats[idx] = st;
+      // the chosen macroblock's bins adapt the live contexts as the writer would
+      if (P.use_cabac) {
+        Bins b{L.cabac, 0, P.entropy};
+        mb_bins(b, L.code);
+      }
+    }
+    __threadfence_block();
+    sync();
+  }
+};
+
+__global__ __launch_bounds__(64) void analyse_kernel(AnParams p) {
+  __shared__ Lds lds;
+  Coder c(p, lds);
+  const int s = blockIdx.x;
+  c.row0 = s * p.mb_h / p.nslices;
+  c.row1 = (s + 1) * p.mb_h / p.nslices;
+  c.slice_id = s + 1;
+  // context states of the slice (enc/cabac.rs:54-64)
+  if (p.use_cabac) {
+    const int qp = clampi(p.qp_slice, 0, 51);
+    for (int i = threadIdx.x; i < 1024; i += 64) {
+      const int m = p.p_slice ? IA_CABAC_INIT_P[i][0] : IA_CABAC_INIT_I[i][0];
+      const int n = p.p_slice ? IA_CABAC_INIT_P[i][1] : IA_CABAC_INIT_I[i][1];
+      const int pre = clampi(((m * qp) >> 4) + n, 1, 126);
+      lds.cabac[i] = pre <= 63 ? (uint8_t)((63 - pre) << 1) : (uint8_t)(((pre - 64) << 1) | 1);
+    }
+  }
+  __syncthreads();
+  for (int mby = c.row0; mby < c.row1; mby++)
+    for (int mbx = 0; mbx < p.mb_w; mbx++) {
+      c.mbx = mbx;
+      c.mby = mby;
+      c.load_mb();
+      ia_mbstats st;
+      c.code_mb(&st);
+      c.store_mb(mby * p.mb_w + mbx, st);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_analyse(const AnParams &p, hipStream_t stream) {
+  hipLaunchKernelGGL(analyse_kernel, dim3(p.nslices), dim3(64), 0, stream, p);
+  return hipGetLastError();
+}
+
+}  // namespace ia

@@ -1,10 +1,11 @@
 // api.hip — the C ABI of include/ironav_h264a.h: context, device buffers and launches.
+#include <math.h>
 #include <stdio.h>
 #include <string.h>
 
 #include <string>
 
-#include "common.h"
+#include "analyse.h"
 
 using ia::Geom;
 
@@ -27,6 +28,14 @@ struct ia_ctx {
   int fs_nref = 0, fs_slots[16] = {0};
   bool fs_valid = false;
   int fs_variant = -1;
+  // analysis outputs and working state
+  uint8_t *rec[3] = {nullptr, nullptr, nullptr};
+  ia_mbcode *codes = nullptr;
+  ia_mbstats *stats = nullptr;
+  ia_mbinfo *mbs = nullptr;
+  ia::MbState *state = nullptr;
+  uint32_t *entropy = nullptr;
+  bool an_valid = false;
   std::string err;
 };
 
@@ -114,6 +123,23 @@ int ia_create(ia_ctx **out, const ia_config *cfg) {
   ok = ok && alloc((void **)&c->src[0], ylen) == hipSuccess && alloc((void **)&c->src[1], clen) == hipSuccess &&
        alloc((void **)&c->src[2], clen) == hipSuccess && alloc((void **)&c->tmp_luma, ylen) == hipSuccess &&
        alloc((void **)&c->fs, (size_t)cfg->nrefs * nmb * IA_NPART * sizeof(ia_fsresult)) == hipSuccess;
+  ok = ok && alloc((void **)&c->rec[0], ylen) == hipSuccess && alloc((void **)&c->rec[1], clen) == hipSuccess &&
+       alloc((void **)&c->rec[2], clen) == hipSuccess && alloc((void **)&c->codes, nmb * sizeof(ia_mbcode)) == hipSuccess &&
+       alloc((void **)&c->stats, nmb * sizeof(ia_mbstats)) == hipSuccess &&
+       alloc((void **)&c->mbs, nmb * sizeof(ia_mbinfo)) == hipSuccess &&
+       alloc((void **)&c->state, nmb * sizeof(ia::MbState)) == hipSuccess &&
+       alloc((void **)&c->entropy, 128 * sizeof(uint32_t)) == hipSuccess;
+  if (ok) {
+    // -log2 of the MPS/LPS probability per CABAC state in 1/256 bits (enc/cabac.rs:162-173), computed
+    // with the C library's pow/log2 as the reference computes it with Rust's (which call the same)
+    uint32_t ent[128];
+    for (int p = 0; p < 64; p++) {
+      const double lps = 0.5 * pow(0.01875 / 0.5, (double)p / 63.0);
+      ent[2 * p] = (uint32_t)round(-log2(1.0 - lps) * 256.0);
+      ent[2 * p + 1] = (uint32_t)round(-log2(lps) * 256.0);
+    }
+    ok = hipMemcpy(c->entropy, ent, sizeof ent, hipMemcpyHostToDevice) == hipSuccess;
+  }
   for (int s = 0; ok && s < cfg->nrefs; s++) {
     IaSlot &sl = c->slots[s];
     ok = alloc((void **)&sl.hp, hplen) == hipSuccess && alloc((void **)&sl.cb, clen) == hipSuccess &&
@@ -132,6 +158,12 @@ void ia_destroy(ia_ctx *c) {
   if (c->own) (void)hipStreamSynchronize(c->own);
   if (c->stream && c->stream != c->own) (void)hipStreamSynchronize(c->stream);
   for (auto *p : c->src) (void)hipFree(p);
+  for (auto *p : c->rec) (void)hipFree(p);
+  (void)hipFree(c->codes);
+  (void)hipFree(c->stats);
+  (void)hipFree(c->mbs);
+  (void)hipFree(c->state);
+  (void)hipFree(c->entropy);
   (void)hipFree(c->tmp_luma);
   (void)hipFree(c->fs);
   for (auto &s : c->slots) {
@@ -230,15 +262,83 @@ int ia_read_motion(ia_ctx *c, ia_fsresult *out) {
   return IA_OK;
 }
 
+static int chroma_qp(int qp) {
+  static const uint8_t T[22] = {29, 30, 31, 32, 32, 33, 34, 34, 35, 35, 36, 36, 37, 37, 37, 38, 38, 38, 39, 39, 39, 39};
+  const int q = qp < 0 ? 0 : (qp > 51 ? 51 : qp);
+  return q < 30 ? q : T[q - 30];
+}
+
 int ia_analyse_uploaded(ia_ctx *c, int qp, int kind, const int *ref_slots, int nref) {
   if (!c) return IA_EINVAL;
-  return fail(c, IA_EUNSUPPORTED, "the analysis kernel is not built yet");
+  if (qp < 0 || qp > 51) return fail(c, IA_EINVAL, "qp must be within 0..51");
+  if (kind != IA_PIC_IDR && kind != IA_PIC_P) return fail(c, IA_EINVAL, "kind must be IA_PIC_IDR or IA_PIC_P");
+  ia::AnParams p;
+  memset(&p, 0, sizeof p);
+  const bool pslice = kind == IA_PIC_P;
+  if (pslice) {
+    int rc = check_refs(c, ref_slots, nref);
+    if (rc) return rc;
+  } else {
+    nref = 0;
+  }
+  if (pslice && c->cfg.search_mode == IA_SEARCH_FULL) {
+    if (!c->fs_valid || c->fs_nref != nref) return fail(c, IA_ESTATE, "no exhaustive-search results for these references");
+    for (int r = 0; r < nref; r++)
+      if (c->fs_slots[r] != ref_slots[r]) return fail(c, IA_ESTATE, "exhaustive-search results are for other references");
+    p.fs = c->fs;
+  }
+  if (pslice && c->cfg.search_mode == IA_SEARCH_LEARNED) return fail(c, IA_EUNSUPPORTED, "the learned proposal is not built");
+  for (int k = 0; k < 3; k++) {
+    p.src[k] = c->src[k];
+    p.rec[k] = c->rec[k];
+  }
+  for (int r = 0; r < nref; r++) {
+    const IaSlot &sl = c->slots[ref_slots[r]];
+    p.hp[r] = sl.hp;
+    p.rcb[r] = sl.cb;
+    p.rcr[r] = sl.cr;
+  }
+  p.rmbs0 = nref > 0 ? c->slots[ref_slots[0]].mbs : nullptr;
+  p.codes = c->codes;
+  p.stats = c->stats;
+  p.mbs = c->mbs;
+  p.state = c->state;
+  p.entropy = c->entropy;
+  p.mb_w = c->g.mb_w;
+  p.mb_h = c->g.mb_h;
+  p.nslices = c->cfg.slices < c->g.mb_h ? c->cfg.slices : c->g.mb_h;
+  p.nref = nref;
+  p.qp_slice = qp;
+  p.qp = qp;  // + 6 * (bit_depth - 8) at 8 bits
+  p.qpc = chroma_qp(qp);
+  // lambda = max(1, round(2^((qp - 12) / 6))), enc/frame.rs:164-166
+  p.lambda = (int)round(pow(2.0, ((double)p.qp - 12.0) / 6.0));
+  if (p.lambda < 1) p.lambda = 1;
+  p.p_slice = pslice;
+  p.sub8 = c->cfg.sub8x8 && qp <= 20;
+  p.use_cabac = c->cfg.cabac != 0;
+  p.transform8 = c->cfg.cabac && c->cfg.transform8;
+  p.lam2 = 1.6 * pow(2.0, ((double)p.qp - 12.0) / 3.0);
+  p.skip_thresh = 256.0 * 0.625 * pow(2.0, (double)p.qp / 6.0);
+  IA_HIP(c, ia::launch_analyse(p, c->stream));
+  c->an_valid = true;
+  return IA_OK;
 }
 
 int ia_read_analysis(ia_ctx *c, ia_mbcode *codes, ia_mbstats *stats, uint8_t *ry, uint8_t *rcb, uint8_t *rcr,
                      ia_mbinfo *mbs) {
   if (!c) return IA_EINVAL;
-  return fail(c, IA_EUNSUPPORTED, "the analysis kernel is not built yet");
+  if (!c->an_valid) return fail(c, IA_ESTATE, "no analysis results");
+  const Geom &g = c->g;
+  const size_t ylen = (size_t)g.pw * g.ph, clen = (size_t)g.cw * g.ch, nmb = (size_t)g.mb_w * g.mb_h;
+  if (codes) IA_HIP(c, hipMemcpyAsync(codes, c->codes, nmb * sizeof(ia_mbcode), hipMemcpyDeviceToHost, c->stream));
+  if (stats) IA_HIP(c, hipMemcpyAsync(stats, c->stats, nmb * sizeof(ia_mbstats), hipMemcpyDeviceToHost, c->stream));
+  if (mbs) IA_HIP(c, hipMemcpyAsync(mbs, c->mbs, nmb * sizeof(ia_mbinfo), hipMemcpyDeviceToHost, c->stream));
+  if (ry) IA_HIP(c, hipMemcpyAsync(ry, c->rec[0], ylen, hipMemcpyDeviceToHost, c->stream));
+  if (rcb) IA_HIP(c, hipMemcpyAsync(rcb, c->rec[1], clen, hipMemcpyDeviceToHost, c->stream));
+  if (rcr) IA_HIP(c, hipMemcpyAsync(rcr, c->rec[2], clen, hipMemcpyDeviceToHost, c->stream));
+  IA_HIP(c, hipStreamSynchronize(c->stream));
+  return IA_OK;
 }
 
 int ia_analyse_picture(ia_ctx *c, const uint8_t *y, const uint8_t *cb, const uint8_t *cr, int qp, int kind,
