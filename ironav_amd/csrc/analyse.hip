@@ -66,13 +66,15 @@ __device__ __forceinline__ int dpp_add(int v) {
   return v + __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, BANK_MASK, false);
 }
 
-// Sum over the 64 lanes (inclusive DPP scan; the last lane holds the total), the same in every lane.
+// Sum over the 64 lanes, the same in every lane: a Hillis-Steele scan within each row of 16 lanes
+// (shifts 1, 2, 4, 8), then row 0's total into row 1 and row 2's into row 3 (row_bcast:15), then
+// rows 0-1's into rows 2-3 (row_bcast:31); the last lane holds the total. (Each shift adds the
+// partial sums once, unlike the 1, 2, 3 shift sequence that suits only idempotent operations.)
 __device__ __forceinline__ int wave_sum(int v) {
   v = dpp_add<0x111, 0xf, 0xf>(v);
   v = dpp_add<0x112, 0xf, 0xf>(v);
-  v = dpp_add<0x113, 0xf, 0xf>(v);
-  v = dpp_add<0x114, 0xf, 0xe>(v);
-  v = dpp_add<0x118, 0xf, 0xc>(v);
+  v = dpp_add<0x114, 0xf, 0xf>(v);
+  v = dpp_add<0x118, 0xf, 0xf>(v);
   v = dpp_add<0x142, 0xa, 0xf>(v);
   v = dpp_add<0x143, 0xc, 0xf>(v);
   return __builtin_amdgcn_readlane(v, 63);
