@@ -58,6 +58,7 @@ def main():
     from ironav_amd.h264a import Analyser, Picture, fullsearch_variants
 
     width, height, nref = 1920, 1080, args.refs
+    mbs_ = ((width + 15) // 16) * ((height + 15) // 16)
     spec = clipgen.workload_1080p()
     spec.frames = nref + 1
     pics = [Picture(*clipgen.pad_planes(*f)) for f in clipgen.make_clip(spec)]
@@ -91,6 +92,29 @@ def main():
     achieved = absdiff / (ms_kernel / 1000.0) / 1e12
     got = an.read_motion(nref)
     an.close()
+
+    # the exact analysis (code_mb over every slice) of the same P picture, timed the same way
+    analysis = {}
+    for slices in (8, 68):
+        with Analyser(width, height, nrefs=nref, slices=slices, search_mode=abi.SEARCH_FULL,
+                      search_range=args.range) as az:
+            az.set_stream(stream.cuda_stream)
+            refres = az.analyse(pics[0], 25, abi.PIC_IDR)
+            for r in range(nref):
+                az.set_reference(r, pics[nref - 1 - r] if r else refres.recon, None if r else refres.mbs)
+            az.upload(pics[nref])
+            az.motion_search(refs)
+            az.sync()
+            reps = 3
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                az.motion_search(refs)
+                az.analyse_uploaded(28, abi.PIC_P, refs)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            analysis[f"slices{slices}"] = {"ms_per_picture": round(ms, 3), "MB_per_s": round(mbs_ / (ms / 1000.0), 1)}
 
     cpu = None
     if not args.no_cpu:
@@ -127,6 +151,8 @@ def main():
                      "kernel": "fullsearch_kernel", "ms_per_launch": round(ms_kernel, 4),
                      "work_per_launch": absdiff},
         "cpu_baseline": cpu,
+        "analysis": {"what": "exhaustive search + exact code_mb analysis of the same P picture (QP 28, "
+                             "3 references, CABAC, 8x8 transform), per slice count", **analysis},
         "check": {"sad16_mean": float(np.mean(got["sad"][:, :, 0]))},
     }
     print(json.dumps(out))

@@ -158,6 +158,11 @@ class Analyser:
         self._ok(library().ia_read_motion(self._h, abi.ptr(out)))
         return out
 
+    def analyse_uploaded(self, qp, kind, ref_slots=()):
+        """Enqueues the analysis of the uploaded picture (results stay on the device)."""
+        refs = abi.int_array(list(ref_slots))
+        self._ok(library().ia_analyse_uploaded(self._h, qp, kind, refs, len(ref_slots)))
+
     def sync(self):
         self._ok(library().ia_sync(self._h))
 

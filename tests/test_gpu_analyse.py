@@ -114,3 +114,11 @@ def test_extreme_qps(oracle):
 
 def test_large_motion_and_noise(oracle):
     run_chain(oracle, 160, 96, 3, 28, nrefs=2, slices=2, seed=21, max_motion=24, noise_sigma=4.0)
+
+
+@pytest.mark.parametrize("qp", [22, 28, 34])
+def test_1080p_workload_chain(oracle, qp):
+    """The plain GPU path at the workload's size: 1920x1080, IDR + 2 P pictures, 3 references, 8
+    slices, exhaustive search +-32 as the extra candidate, 8x8 transform, CABAC; bit-exact."""
+    run_chain(oracle, 1920, 1080, 3, qp, nrefs=3, slices=8, search_mode=abi.SEARCH_FULL, search_range=32,
+              seed=1080 + qp)
