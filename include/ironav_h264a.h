@@ -44,7 +44,8 @@ enum { IA_KIND_NONE = 0, IA_KIND_I4X4 = 1, IA_KIND_I8X8 = 2, IA_KIND_I16X16 = 3,
  *  IA_SEARCH_DIAMOND  the reference's predictor candidates and diamond, exactly;
  *  IA_SEARCH_FULL     the same with one more candidate per partition and reference: the exhaustive
  *                     integer-sample SAD winner over +-search_range (see ia_motion_search);
- *  IA_SEARCH_LEARNED  reserved for the learned proposal. */
+ *  IA_SEARCH_LEARNED  the same with the learned proposal's winner as that candidate (see
+ *                     ia_load_proposal_weights). */
 enum { IA_SEARCH_DIAMOND = 0, IA_SEARCH_FULL = 1, IA_SEARCH_LEARNED = 2 };
 
 /* Picture kinds of `code_picture` (enc/mod.rs:539): 0 IDR (I slices), 1 P. */
@@ -193,6 +194,32 @@ int ia_analyse_picture(ia_ctx *ctx, const uint8_t *y, const uint8_t *cb, const u
 
 /* Waits for the context's stream. */
 int ia_sync(ia_ctx *ctx);
+
+/* The learned motion proposal (IA_SEARCH_LEARNED; no counterpart in the reference). With it,
+ * ia_motion_search proposes 8 candidate cells per macroblock and reference with a 4-layer 3x3 BF16
+ * CNN on a 9x9 grid of 4-sample displacement cells around the reference's temporal predictor,
+ * refines them by exact SAD over +-2 samples for all 41 partitions, and runs the exhaustive search
+ * instead where the refined 16x16 SAD exceeds fallback_ratio x the CNN's predicted SAD. References
+ * are limited to 3. Results have the layout of the exhaustive search. */
+#define IA_CNN_PARAMS 21988  /* f32 parameters: per layer, weights [cout][cin][3][3] then biases [cout];
+                                (cin, cout) = (8, 32), (32, 32), (32, 32), (32, 4) */
+#define IA_CNN_CELLS 81
+#define IA_CNN_FEATURES 8
+#define IA_CNN_OUTPUTS 4
+
+/* Loads the CNN's parameters (IA_CNN_PARAMS floats, converted to BF16 on the device). */
+int ia_load_proposal_weights(ia_ctx *ctx, const float *params, int n);
+
+/* The CNN's input features [mb][IA_CNN_CELLS][IA_CNN_FEATURES] and grid centres [mb][3][2]
+ * (samples) of the uploaded picture against ref_slots (1..3), for training. Synchronous. */
+int ia_proposal_features(ia_ctx *ctx, const int *ref_slots, int nref, float *features, int16_t *centres);
+
+/* The CNN's outputs [n][IA_CNN_CELLS][IA_CNN_OUTPUTS] on given features [n][IA_CNN_CELLS][IA_CNN_FEATURES]
+ * (n <= macroblocks of the context), for checking. Synchronous. */
+int ia_proposal_forward(ia_ctx *ctx, const float *features, int n, float *outputs);
+
+/* The number of (macroblock, reference) pairs the last learned search sent to the exhaustive search. */
+int ia_read_fallback_count(ia_ctx *ctx, int *count);
 
 /* Tile shapes of the full-search kernel, for the autotuner (no counterpart in the reference, whose
  * search is a scalar loop): the number of compiled shapes, each one's name, and the shape a context

@@ -8,6 +8,7 @@ KIND_NONE, KIND_I4X4, KIND_I8X8, KIND_I16X16, KIND_IPCM, KIND_INTER = range(6)
 SEARCH_DIAMOND, SEARCH_FULL, SEARCH_LEARNED = 0, 1, 2
 PIC_IDR, PIC_P = 0, 1
 NPART = 41
+CNN_PARAMS, CNN_CELLS, CNN_FEATURES, CNN_OUTPUTS = 21988, 81, 8, 4
 
 # (x, y, w, h) in 4x4 blocks of the 41 searched partitions, numbered as IA_NPART documents.
 PARTITIONS = (
@@ -82,6 +83,10 @@ PROTOTYPES = {
     "ia_read_analysis": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
     "ia_analyse_picture": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, ip, C.c_int, vp, vp, vp, vp, vp, vp]),
     "ia_sync": (C.c_int, [vp]),
+    "ia_load_proposal_weights": (C.c_int, [vp, vp, C.c_int]),
+    "ia_proposal_features": (C.c_int, [vp, ip, C.c_int, vp, vp]),
+    "ia_proposal_forward": (C.c_int, [vp, vp, C.c_int, vp]),
+    "ia_read_fallback_count": (C.c_int, [vp, C.POINTER(C.c_int)]),
     "ia_fullsearch_variants": (C.c_int, []),
     "ia_fullsearch_variant_name": (C.c_char_p, [C.c_int]),
     "ia_set_fullsearch_variant": (C.c_int, [vp, C.c_int]),

@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libironav_h264a.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["api.hip", "halfpel.hip", "fullsearch.hip", "analyse.hip"]
+SOURCES = ["api.hip", "halfpel.hip", "fullsearch.hip", "analyse.hip", "proposal.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950",
@@ -56,5 +56,23 @@ def build(verbose=False):
     return OUT
 
 
+TRAIN_OUT = os.path.join(HERE, "libironav_train.so")
+
+
+def build_train():
+    """The FP32 training kernels of the proposal CNN (offline tooling), ironav_amd/libironav_train.so."""
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    obj = os.path.join(objdir, "train.o")
+    _compile("train.hip", obj)
+    tmp = TRAIN_OUT + ".tmp"
+    p = subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, obj], capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError(f"link failed:\n{p.stderr}")
+    os.replace(tmp, TRAIN_OUT)
+    return TRAIN_OUT
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_train())

@@ -6,6 +6,7 @@
 #include <string>
 
 #include "analyse.h"
+#include "proposal.h"
 
 using ia::Geom;
 
@@ -36,6 +37,12 @@ struct ia_ctx {
   ia::MbState *state = nullptr;
   uint32_t *entropy = nullptr;
   bool an_valid = false;
+  // learned proposal
+  ia::CnnWeights *wts = nullptr;
+  bool wts_loaded = false;
+  int *fb = nullptr;           // [0] count, [1..] entries
+  float *feat = nullptr, *cnn_out = nullptr;
+  int16_t *centres = nullptr;
   std::string err;
 };
 
@@ -128,7 +135,9 @@ int ia_create(ia_ctx **out, const ia_config *cfg) {
        alloc((void **)&c->stats, nmb * sizeof(ia_mbstats)) == hipSuccess &&
        alloc((void **)&c->mbs, nmb * sizeof(ia_mbinfo)) == hipSuccess &&
        alloc((void **)&c->state, nmb * sizeof(ia::MbState)) == hipSuccess &&
-       alloc((void **)&c->entropy, 128 * sizeof(uint32_t)) == hipSuccess;
+       alloc((void **)&c->entropy, 128 * sizeof(uint32_t)) == hipSuccess &&
+       alloc((void **)&c->wts, sizeof(ia::CnnWeights)) == hipSuccess &&
+       alloc((void **)&c->fb, (1 + 4 * nmb) * sizeof(int)) == hipSuccess;
   if (ok) {
     // -log2 of the MPS/LPS probability per CABAC state in 1/256 bits (enc/cabac.rs:162-173), computed
     // with the C library's pow/log2 as the reference computes it with Rust's (which call the same)
@@ -164,6 +173,11 @@ void ia_destroy(ia_ctx *c) {
   (void)hipFree(c->mbs);
   (void)hipFree(c->state);
   (void)hipFree(c->entropy);
+  (void)hipFree(c->wts);
+  (void)hipFree(c->fb);
+  (void)hipFree(c->feat);
+  (void)hipFree(c->cnn_out);
+  (void)hipFree(c->centres);
   (void)hipFree(c->tmp_luma);
   (void)hipFree(c->fs);
   for (auto &s : c->slots) {
@@ -232,10 +246,51 @@ static int check_refs(ia_ctx *c, const int *ref_slots, int nref) {
   return IA_OK;
 }
 
+namespace {
+__global__ void zero_ints(int *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+}  // namespace
+
 int ia_motion_search(ia_ctx *c, const int *ref_slots, int nref) {
   if (!c) return IA_EINVAL;
   int rc = check_refs(c, ref_slots, nref);
   if (rc) return rc;
+  if (c->cfg.search_mode == IA_SEARCH_LEARNED) {
+    if (!c->wts_loaded) return fail(c, IA_ESTATE, "the proposal's weights are not loaded");
+    if (nref > 3) return fail(c, IA_EUNSUPPORTED, "the learned proposal takes at most 3 references");
+    ia::PropArgs p;
+    memset(&p, 0, sizeof p);
+    p.src_y = c->src[0];
+    for (int r = 0; r < nref; r++) p.ref_g[r] = c->slots[ref_slots[r]].hp;
+    p.rmbs0 = c->slots[ref_slots[0]].mbs;
+    p.wts = c->wts;
+    p.out = c->fs;
+    p.fb_count = c->fb;
+    p.fb_list = c->fb + 1;
+    p.mb_w = c->g.mb_w;
+    p.mb_h = c->g.mb_h;
+    p.nref = nref;
+    p.fallback_ratio = c->cfg.fallback_ratio;
+    hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, c->stream, c->fb, 1);
+    IA_HIP(c, hipGetLastError());
+    IA_HIP(c, ia::launch_proposal(p, c->stream));
+    ia::FsArgs a;
+    memset(&a, 0, sizeof a);
+    a.src_y = c->src[0];
+    for (int r = 0; r < nref; r++) a.ref_g[r] = c->slots[ref_slots[r]].hp;
+    a.out = c->fs;
+    a.nref = nref;
+    a.mb_w = c->g.mb_w;
+    a.mb_h = c->g.mb_h;
+    a.range = c->cfg.search_range;
+    IA_HIP(c, ia::launch_fullsearch_list(a, c->fb + 1, c->fb, c->stream));
+    c->fs_nref = nref;
+    for (int r = 0; r < nref; r++) c->fs_slots[r] = ref_slots[r];
+    c->fs_valid = true;
+    return IA_OK;
+  }
   ia::FsArgs a;
   memset(&a, 0, sizeof a);
   a.src_y = c->src[0];
@@ -281,13 +336,12 @@ int ia_analyse_uploaded(ia_ctx *c, int qp, int kind, const int *ref_slots, int n
   } else {
     nref = 0;
   }
-  if (pslice && c->cfg.search_mode == IA_SEARCH_FULL) {
+  if (pslice && c->cfg.search_mode != IA_SEARCH_DIAMOND) {
     if (!c->fs_valid || c->fs_nref != nref) return fail(c, IA_ESTATE, "no exhaustive-search results for these references");
     for (int r = 0; r < nref; r++)
       if (c->fs_slots[r] != ref_slots[r]) return fail(c, IA_ESTATE, "exhaustive-search results are for other references");
     p.fs = c->fs;
   }
-  if (pslice && c->cfg.search_mode == IA_SEARCH_LEARNED) return fail(c, IA_EUNSUPPORTED, "the learned proposal is not built");
   for (int k = 0; k < 3; k++) {
     p.src[k] = c->src[k];
     p.rec[k] = c->rec[k];
@@ -347,7 +401,7 @@ int ia_analyse_picture(ia_ctx *c, const uint8_t *y, const uint8_t *cb, const uin
   if (!c) return IA_EINVAL;
   int rc = ia_upload_source(c, y, cb, cr);
   if (rc) return rc;
-  if (kind == IA_PIC_P && c->cfg.search_mode == IA_SEARCH_FULL) {
+  if (kind == IA_PIC_P && c->cfg.search_mode != IA_SEARCH_DIAMOND) {
     rc = ia_motion_search(c, ref_slots, nref);
     if (rc) return rc;
   }
@@ -358,6 +412,101 @@ int ia_analyse_picture(ia_ctx *c, const uint8_t *y, const uint8_t *cb, const uin
 
 int ia_sync(ia_ctx *c) {
   if (!c) return IA_EINVAL;
+  IA_HIP(c, hipStreamSynchronize(c->stream));
+  return IA_OK;
+}
+
+static uint16_t f2bf_host(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+int ia_load_proposal_weights(ia_ctx *c, const float *params, int n) {
+  if (!c) return IA_EINVAL;
+  if (!params || n != IA_CNN_PARAMS) return fail(c, IA_EINVAL, "expected IA_CNN_PARAMS parameters");
+  static const int CIN[4] = {8, 32, 32, 32}, COUT[4] = {32, 32, 32, 4};
+  ia::CnnWeights *w = new ia::CnnWeights();
+  memset(w, 0, sizeof *w);
+  const float *q = params;
+  for (int l = 0; l < 4; l++) {
+    for (int co = 0; co < COUT[l]; co++)
+      for (int ci = 0; ci < CIN[l]; ci++)
+        for (int t = 0; t < 9; t++) w->w[l][t][co][ci] = f2bf_host(q[(co * CIN[l] + ci) * 9 + t]);
+    q += COUT[l] * CIN[l] * 9;
+    for (int co = 0; co < COUT[l]; co++) w->b[l][co] = q[co];
+    q += COUT[l];
+  }
+  hipError_t e = hipMemcpy(c->wts, w, sizeof *w, hipMemcpyHostToDevice);
+  delete w;
+  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(weights)");
+  c->wts_loaded = true;
+  return IA_OK;
+}
+
+static int ensure_tool_buffers(ia_ctx *c) {
+  const size_t nmb = (size_t)c->g.mb_w * c->g.mb_h;
+  if (!c->feat) IA_HIP(c, hipMalloc((void **)&c->feat, nmb * IA_CNN_CELLS * IA_CNN_FEATURES * sizeof(float)));
+  if (!c->cnn_out) IA_HIP(c, hipMalloc((void **)&c->cnn_out, nmb * IA_CNN_CELLS * IA_CNN_OUTPUTS * sizeof(float)));
+  if (!c->centres) IA_HIP(c, hipMalloc((void **)&c->centres, nmb * 6 * sizeof(int16_t)));
+  return IA_OK;
+}
+
+int ia_proposal_features(ia_ctx *c, const int *ref_slots, int nref, float *features, int16_t *centres) {
+  if (!c) return IA_EINVAL;
+  int rc = check_refs(c, ref_slots, nref);
+  if (rc) return rc;
+  if (nref > 3) return fail(c, IA_EUNSUPPORTED, "the learned proposal takes at most 3 references");
+  rc = ensure_tool_buffers(c);
+  if (rc) return rc;
+  ia::PropArgs p;
+  memset(&p, 0, sizeof p);
+  p.src_y = c->src[0];
+  for (int r = 0; r < nref; r++) p.ref_g[r] = c->slots[ref_slots[r]].hp;
+  p.rmbs0 = c->slots[ref_slots[0]].mbs;
+  p.wts = c->wts;
+  p.feat = c->feat;
+  p.centre = c->centres;
+  p.mb_w = c->g.mb_w;
+  p.mb_h = c->g.mb_h;
+  p.nref = nref;
+  p.features_only = 1;
+  IA_HIP(c, ia::launch_proposal(p, c->stream));
+  const size_t nmb = (size_t)c->g.mb_w * c->g.mb_h;
+  if (features)
+    IA_HIP(c, hipMemcpyAsync(features, c->feat, nmb * IA_CNN_CELLS * IA_CNN_FEATURES * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  if (centres) IA_HIP(c, hipMemcpyAsync(centres, c->centres, nmb * 6 * sizeof(int16_t), hipMemcpyDeviceToHost, c->stream));
+  IA_HIP(c, hipStreamSynchronize(c->stream));
+  return IA_OK;
+}
+
+int ia_proposal_forward(ia_ctx *c, const float *features, int n, float *outputs) {
+  if (!c) return IA_EINVAL;
+  const int nmb = c->g.mb_w * c->g.mb_h;
+  if (!features || !outputs || n < 1 || n > nmb) return fail(c, IA_EINVAL, "bad features or count");
+  if (!c->wts_loaded) return fail(c, IA_ESTATE, "the proposal's weights are not loaded");
+  int rc = ensure_tool_buffers(c);
+  if (rc) return rc;
+  IA_HIP(c, hipMemcpyAsync(c->feat, features, (size_t)n * IA_CNN_CELLS * IA_CNN_FEATURES * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  ia::PropArgs p;
+  memset(&p, 0, sizeof p);
+  p.src_y = c->src[0];
+  p.wts = c->wts;
+  p.feat_in = c->feat;
+  p.cnn_out = c->cnn_out;
+  p.mb_w = n;  // one row of n macroblocks: only the count matters with given features
+  p.mb_h = 1;
+  p.nref = 1;
+  IA_HIP(c, ia::launch_proposal(p, c->stream));
+  IA_HIP(c, hipMemcpyAsync(outputs, c->cnn_out, (size_t)n * IA_CNN_CELLS * IA_CNN_OUTPUTS * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  IA_HIP(c, hipStreamSynchronize(c->stream));
+  return IA_OK;
+}
+
+int ia_read_fallback_count(ia_ctx *c, int *count) {
+  if (!c || !count) return IA_EINVAL;
+  IA_HIP(c, hipMemcpyAsync(count, c->fb, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   IA_HIP(c, hipStreamSynchronize(c->stream));
   return IA_OK;
 }

@@ -73,11 +73,18 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t v) {
 
 // K: candidates per strip (rows of one column); NT: threads; RAWS: dwords per staged window row;
 // WPE: waves per SIMD the register allocation must allow.
+// With a list, block i searches entry list[i] = mb * 4 + ref (blocks past *count do nothing).
 template <int K, int NT, int RAWS, int WPE>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void fullsearch_kernel(FsArgs a,
-                                                                                                 int rows_lds) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void fullsearch_kernel(
+    FsArgs a, int rows_lds, const int *list, const int *count) {
   extern __shared__ uint32_t raw[];
-  const int mb = blockIdx.x, r = blockIdx.y;
+  int mb = blockIdx.x, r = blockIdx.y;
+  if (list) {
+    if ((int)blockIdx.x >= *count) return;
+    const int e = list[blockIdx.x];
+    mb = e >> 2;
+    r = e & 3;
+  }
   const int mbx = mb % a.mb_w, mby = mb / a.mb_w;
   const int pw = a.mb_w * 16, ph = a.mb_h * 16, sw = pw + 2 * PAD, sh = ph + 2 * PAD;
   const int x0 = mbx * 16, y0 = mby * 16, R = a.range;
@@ -211,8 +218,8 @@ constexpr int raws_for(int rmax) { return ((3 + 2 * rmax + 16) >> 2) + 2; }
 struct Variant {
   const char *name;
   int k, nt;
-  void (*fn32)(FsArgs, int);  // ranges up to 32
-  void (*fn64)(FsArgs, int);  // ranges up to 64
+  void (*fn32)(FsArgs, int, const int *, const int *);  // ranges up to 32
+  void (*fn64)(FsArgs, int, const int *, const int *);  // ranges up to 64
 };
 
 #define IA_FS_VARIANT(K, NT, W)                                                                         \
@@ -242,7 +249,21 @@ hipError_t launch_fullsearch(const FsArgs &a, int variant, hipStream_t stream) {
   const int rows_lds = 2 * a.range + 1 + v.k + 15;
   const size_t shmem = (size_t)rows_lds * raws * sizeof(uint32_t);
   dim3 grid(a.mb_w * a.mb_h, a.nref);
-  hipLaunchKernelGGL(small ? v.fn32 : v.fn64, grid, dim3(v.nt), shmem, stream, a, rows_lds);
+  hipLaunchKernelGGL(small ? v.fn32 : v.fn64, grid, dim3(v.nt), shmem, stream, a, rows_lds, (const int *)nullptr,
+                     (const int *)nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_fullsearch_list(const FsArgs &a, const int *list, const int *count, hipStream_t stream) {
+  const Variant &v = VARIANTS[DEFAULT_VARIANT];
+  if (a.range < 1 || a.range > RMAX || a.nref < 1 || a.nref > 4 || 2 * a.range + 1 > v.nt) return hipErrorInvalidValue;
+  const bool small = a.range <= 32;
+  const int raws = small ? raws_for(32) : raws_for(64);
+  const int rows_lds = 2 * a.range + 1 + v.k + 15;
+  const size_t shmem = (size_t)rows_lds * raws * sizeof(uint32_t);
+  // one block per possible entry; those past the device-side count return at once
+  dim3 grid(a.mb_w * a.mb_h * a.nref);
+  hipLaunchKernelGGL(small ? v.fn32 : v.fn64, grid, dim3(v.nt), shmem, stream, a, rows_lds, list, count);
   return hipGetLastError();
 }
 

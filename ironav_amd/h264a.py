@@ -30,6 +30,12 @@ def library():
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m ironav_amd.build`")
+        try:
+            # one HIP runtime per process: torch ships its own copy under a different file name, so
+            # when torch is present it is loaded first and the library binds to that runtime
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _LIB = abi.bind(C.CDLL(LIB_PATH))
     return _LIB
 
@@ -181,6 +187,32 @@ class Analyser:
                                               refs, len(ref_slots), abi.ptr(codes), abi.ptr(stats), abi.ptr(ry),
                                               abi.ptr(rcb), abi.ptr(rcr), abi.ptr(mbs)))
         return Analysis(codes, stats, Picture(ry, rcb, rcr), mbs, self.mb_w, self.mb_h, self.rows)
+
+    def load_proposal_weights(self, params):
+        """Loads the learned proposal's CNN parameters (abi.CNN_PARAMS float32, see proposal.py)."""
+        params = np.ascontiguousarray(params, dtype=np.float32).ravel()
+        self._ok(library().ia_load_proposal_weights(self._h, abi.ptr(params), params.size))
+
+    def proposal_features(self, ref_slots):
+        """CNN input features [mb][81][8] and grid centres [mb][3][2] of the uploaded picture."""
+        n = self.mb_w * self.mb_h
+        feat = np.empty((n, abi.CNN_CELLS, abi.CNN_FEATURES), dtype=np.float32)
+        ctr = np.empty((n, 3, 2), dtype=np.int16)
+        refs = abi.int_array(list(ref_slots))
+        self._ok(library().ia_proposal_features(self._h, refs, len(ref_slots), abi.ptr(feat), abi.ptr(ctr)))
+        return feat, ctr
+
+    def proposal_forward(self, feat):
+        """The CNN's outputs [n][81][4] on given features (BF16 MFMA path)."""
+        feat = np.ascontiguousarray(feat, dtype=np.float32)
+        out = np.empty((feat.shape[0], abi.CNN_CELLS, abi.CNN_OUTPUTS), dtype=np.float32)
+        self._ok(library().ia_proposal_forward(self._h, abi.ptr(feat), feat.shape[0], abi.ptr(out)))
+        return out
+
+    def fallback_count(self):
+        c = C.c_int()
+        self._ok(library().ia_read_fallback_count(self._h, C.byref(c)))
+        return c.value
 
     def set_fullsearch_variant(self, v):
         self._ok(library().ia_set_fullsearch_variant(self._h, v))

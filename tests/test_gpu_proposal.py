@@ -1,0 +1,161 @@
+"""The learned motion proposal on the GPU: the BF16 MFMA CNN against an FP32 reference, the FP32
+training kernels against torch autograd on the CPU, and the learned search's invariants."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from ironav_amd import abi, clipgen, proposal
+from ironav_amd.h264a import Analyser, Picture
+
+pytestmark = pytest.mark.gpu
+
+
+def frames(w, h, n, seed, **kw):
+    return [Picture(*clipgen.pad_planes(*f)) for f in clipgen.make_clip(clipgen.ClipSpec(w, h, n, seed, **kw))]
+
+
+def test_cnn_forward_bf16_mfma_matches_fp32_reference():
+    """Tolerance: BF16 operands (8-bit significands) at every layer with FP32 accumulation; outputs
+    within 3% of the output range of the FP32 network."""
+    rng = np.random.default_rng(0)
+    params = proposal.init_params(3)
+    feat = rng.uniform(0.0, 1.0, size=(24, 81, 8)).astype(np.float32)
+    feat[:, :, 3:6] = (rng.random((24, 81, 3)) < 0.05).astype(np.float32)
+    feat[:, :, 6] = 1.0
+    with Analyser(96, 64, nrefs=1) as an:
+        an.load_proposal_weights(params)
+        got = an.proposal_forward(feat)
+    want = proposal.forward_reference(params, feat)
+    scale = np.abs(want).max()
+    assert scale > 0.1
+    err = np.abs(got - want).max()
+    assert err <= 0.03 * scale, (err, scale)
+    # the argmax cells mostly agree (ties aside)
+    agree = np.mean(np.argmax(got[:, :, :3], axis=1) == np.argmax(want[:, :, :3], axis=1))
+    assert agree > 0.9
+
+
+def test_training_kernels_match_torch_autograd():
+    """Gradients of the HIP FP32 training kernels against torch autograd of the same network and loss
+    on the CPU (torch fp32 reference); tolerance 1e-3 relative to the largest gradient."""
+    import torch
+
+    rng = np.random.default_rng(1)
+    B = 8
+    params = proposal.init_params(5)
+    feat = rng.uniform(0, 1, size=(B, 81, 8)).astype(np.float32)
+    cell = rng.integers(-1, 81, size=(B, 3)).astype(np.int32)
+    cost = rng.uniform(4, 12, size=(B, 3)).astype(np.float32)
+    cost_w = 0.05
+    # torch reference (CPU, fp32)
+    P = torch.tensor(params, requires_grad=True)
+    x = torch.tensor(feat).reshape(B, 9, 9, 8).permute(0, 3, 1, 2)
+    for li, ((w, b), (cin, cout)) in enumerate(zip(proposal.param_slices(), proposal.LAYERS)):
+        x = torch.nn.functional.conv2d(x, P[w[0]:w[1]].reshape(cout, cin, 3, 3), P[b[0]:b[1]], padding=1)
+        if li < 3:
+            x = torch.relu(x)
+    out = x.permute(0, 2, 3, 1).reshape(B, 81, 4)
+    loss = torch.zeros(())
+    for n in range(B):
+        for r in range(3):
+            tc = int(cell[n, r])
+            if tc < 0:
+                continue
+            loss = loss + (torch.logsumexp(out[n, :, r], 0) - out[n, tc, r] + cost_w * (out[n, tc, 3] - float(cost[n, r])) ** 2) / B
+    loss.backward()
+    want = P.grad.numpy()
+    # HIP kernels: one batch, no update (lr 0)
+    L = proposal._TrainLib()
+    dev = torch.device("cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    Pd = torch.tensor(params, device=dev)
+    G = torch.zeros_like(Pd)
+    X = torch.tensor(feat, device=dev)
+    TC = torch.tensor(cell, device=dev)
+    TK = torch.tensor(cost, device=dev)
+    acts = [torch.empty((B, 81, co), device=dev) for _, co in proposal.LAYERS]
+    grads = [torch.empty((B, 81, co), device=dev) for _, co in proposal.LAYERS]
+    lossd = torch.zeros(1, device=dev)
+    ptr = lambda t: C.c_void_p(t.data_ptr())
+    sl = proposal.param_slices()
+    inp = X
+    for li, ((w, b), (cin, cout)) in enumerate(zip(sl, proposal.LAYERS)):
+        L.it_conv_fwd(ptr(inp), ptr(Pd[w[0]:]), ptr(Pd[b[0]:]), ptr(acts[li]), B, cin, cout, int(li < 3), stream)
+        inp = acts[li]
+    grads[3].zero_()
+    L.it_loss_grad(ptr(acts[3]), ptr(TC), ptr(TK), ptr(grads[3]), ptr(lossd), B, cost_w, stream)
+    for li in range(3, -1, -1):
+        (w, b), (cin, cout) = sl[li], proposal.LAYERS[li]
+        src = X if li == 0 else acts[li - 1]
+        L.it_conv_bwd_w(ptr(src), ptr(grads[li]), ptr(G[w[0]:]), ptr(G[b[0]:]), B, cin, cout, stream)
+        if li > 0:
+            L.it_conv_bwd_data(ptr(grads[li]), ptr(Pd[w[0]:]), ptr(grads[li - 1]), B, cin, cout, stream)
+            L.it_relu_bwd(ptr(grads[li - 1]), ptr(acts[li - 1]), B * 81 * cin, stream)
+    torch.cuda.synchronize()
+    got = G.cpu().numpy()
+    assert abs(float(lossd.item()) - float(loss.item())) <= 1e-4 * max(1.0, abs(float(loss.item())))
+    assert np.allclose(acts[3].cpu().numpy(), out.detach().numpy(), atol=1e-4, rtol=1e-4)
+    assert np.abs(got - want).max() <= 1e-3 * np.abs(want).max(), np.abs(got - want).max()
+
+
+def _learned(w, h, nref, ratio, params, seed=7, **clip):
+    pics = frames(w, h, nref + 2, seed, **clip)
+    with Analyser(w, h, nrefs=3, slices=2, search_mode=abi.SEARCH_LEARNED, search_range=32,
+                  fallback_ratio=ratio) as an:
+        an.load_proposal_weights(params)
+        # reference 0 carries the encoder's vectors: analyse the pictures before
+        res = an.analyse(pics[0], 25, abi.PIC_IDR)
+        refs = [(res.recon, res.mbs)]
+        for t in range(1, nref + 1):
+            for s in range(len(refs)):
+                an.set_reference(s, refs[s][0], refs[s][1])
+            res = an.analyse(pics[t], 28, abi.PIC_P, list(range(len(refs))))
+            refs.insert(0, (res.recon, res.mbs))
+        for s in range(nref):
+            an.set_reference(s, refs[s][0], refs[s][1])
+        an.upload(pics[nref + 1])
+        an.motion_search(list(range(nref)))
+        got = an.read_motion(nref)
+        fb = an.fallback_count()
+    return pics[nref + 1], [r[0] for r in refs[:nref]], got, fb
+
+
+def test_learned_search_sads_are_exact():
+    params = proposal.init_params(11)
+    src, refs, got, fb = _learned(128, 96, 3, 1e9, params, max_motion=8)
+    assert fb == 0
+    mb_w = 8
+    for r in range(3):
+        pad = np.pad(refs[r].y.astype(np.int32), 48, mode="edge")
+        for mb in range(0, 48, 5):
+            mbx, mby = mb % mb_w, mb // mb_w
+            for p, (x, y, w, h) in enumerate(abi.PARTITIONS):
+                dx, dy = (int(v) // 4 for v in got[r, mb]["mv"][p])
+                x0, y0 = mbx * 16 + x * 4, mby * 16 + y * 4
+                blk = pad[y0 + dy + 48:y0 + dy + 48 + h * 4, x0 + dx + 48:x0 + dx + 48 + w * 4]
+                sad = int(np.abs(src.y[y0:y0 + h * 4, x0:x0 + w * 4].astype(np.int32) - blk).sum())
+                assert sad == int(got[r, mb]["sad"][p]), (r, mb, p)
+                assert -(mbx * 16 + 24) <= dx <= 128 - mbx * 16 + 8 and -(mby * 16 + 24) <= dy <= 96 - mby * 16 + 8
+
+
+def test_learned_search_falls_back_to_full_search(oracle):
+    """With a fallback ratio of 0 every (macroblock, reference) goes to the exhaustive search through
+    the fallback list: the results are the oracle's exhaustive search exactly."""
+    params = proposal.init_params(12)
+    src, refs, got, fb = _learned(96, 64, 2, 0.0, params)
+    assert fb == 24 * 2
+    want = oracle.fullsearch(src.y, [r.y for r in refs], 6, 4, 32)
+    assert np.array_equal(got["sad"], want["sad"]) and np.array_equal(got["mv"], want["mv"])
+
+
+def test_learned_analysis_runs_end_to_end():
+    """IA_SEARCH_LEARNED through ia_analyse_picture: the learned winners are the extra candidates."""
+    params = proposal.init_params(13)
+    pics = frames(96, 64, 3, 8)
+    with Analyser(96, 64, nrefs=2, slices=2, search_mode=abi.SEARCH_LEARNED, search_range=32) as an:
+        an.load_proposal_weights(params)
+        res = an.analyse(pics[0], 25, abi.PIC_IDR)
+        an.set_reference(0, res.recon, res.mbs)
+        p = an.analyse(pics[1], 28, abi.PIC_P, [0])
+    assert np.all(p.codes["kind"] > 0)
