@@ -1,14 +1,21 @@
-"""Throughput of the H.264 motion-analysis stage on one MI355X.
+"""Throughput of the H.264 motion-analysis stage on one MI355X: the learned proposal against the
+exhaustive search, at 1080p.
 
-Workload (default "fullsearch-1080p"): one 1920x1080 P picture of the seeded synthetic clip searched
-exhaustively (+-32 integer samples, all 41 partitions from 16x16 to 4x4) against 3 reference
-pictures - the plain GPU full search the learned proposal is measured against. A step is one
-picture (8,160 macroblocks x 3 references), inputs already resident in HBM. Metric: macroblocks per
-second.
+Workload: the seeded synthetic 1920x1080 clip; pictures 0-3 are coded IPPP through the analysis
+(so reference 0 carries the encoder's own vectors and reconstructions are the references), and
+picture 4 is searched against 3 references [3, 2, 1]:
+  - full search: every integer vector within +-32 for all 41 partitions from 16x16 to 4x4;
+  - learned proposal: the BF16 CNN on MFMA proposes 8 cells per reference, an exact +-2 sample SAD
+    refinement gives all 41 partitions' winners, and (macroblock, reference) pairs whose refined
+    16x16 SAD exceeds 1.1x the CNN's predicted SAD go to the full search.
+A step is one picture (8,160 macroblocks against 3 references), inputs resident in HBM. value is the
+learned path's macroblocks per second; the full search's is reported beside it, with the cost gap
+(SAD of the chosen vectors, and the encoder's own costs after the exact analysis with either result
+as its extra candidate) and the fallback rate.
 
-Prints one JSON line. The roofline object is that of the search kernel, timed with HIP events on the
-stream it is launched on; its work is the algorithmic absolute-difference count of the clipped
-windows. cpu_baseline is the oracle's scalar exhaustive search of the same picture against one
+The roofline object is that of the exhaustive-search kernel (integer VALU), and learned_roofline that
+of the proposal kernel (BF16 MFMA for the CNN); both timed with HIP events on the stream they are
+launched on. cpu_baseline is the oracle's scalar exhaustive search of the same picture against one
 reference on one host core, scaled to three references.
 """
 import argparse
@@ -23,6 +30,8 @@ sys.path.insert(0, ROOT)
 MI355X_CU, MI355X_CLK_GHZ = 256, 2.4
 # v_sad_u8 / v_sad_hi_u8: 4 absolute differences per lane per instruction, 64 lanes per CU per clock
 VALU_ABSDIFF_PEAK_T = MI355X_CU * 64 * 4 * MI355X_CLK_GHZ / 1000.0   # 157.3 T abs-diff/s
+MFMA_BF16_PEAK_T = 2500.0                                            # dense bf16 TFLOP/s
+CNN_FLOP_PER_MB = 2 * 81 * 9 * (8 * 32 + 32 * 32 + 32 * 32 + 32 * 4)  # multiply-adds x 2 of the 4 layers
 
 
 def window_positions(mb_w, mb_h, rng):
@@ -36,13 +45,38 @@ def window_positions(mb_w, mb_h, rng):
     return tot
 
 
+def timed(stream, fn, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1000.0 / steps
+    return e0.elapsed_time(e1) / steps, wall
+
+
+def read_stats(an, n):
+    from ironav_amd import abi
+    from ironav_amd.h264a import library
+
+    stats = __import__("numpy").zeros(n, dtype=abi.MBSTATS_DTYPE)
+    an._ok(library().ia_read_analysis(an._h, None, abi.ptr(stats), None, None, None, None))
+    return stats
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--range", type=int, default=32)
-    ap.add_argument("--refs", type=int, default=3)
     ap.add_argument("--variant", type=int, default=-1, help="full-search tile shape (-1: default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = ap.parse_args()
@@ -54,67 +88,71 @@ def main():
     import numpy as np
     import torch
 
-    from ironav_amd import abi, clipgen
+    from ironav_amd import abi, clipgen, proposal
     from ironav_amd.h264a import Analyser, Picture, fullsearch_variants
 
-    width, height, nref = 1920, 1080, args.refs
-    mbs_ = ((width + 15) // 16) * ((height + 15) // 16)
+    width, height, nref = 1920, 1080, 3
     spec = clipgen.workload_1080p()
-    spec.frames = nref + 1
+    spec.frames = 5
     pics = [Picture(*clipgen.pad_planes(*f)) for f in clipgen.make_clip(spec)]
     torch.cuda.init()
     stream = torch.cuda.Stream()  # default priority; the library enqueues on it, the events time it
-    an = Analyser(width, height, nrefs=nref, search_mode=abi.SEARCH_FULL, search_range=args.range)
-    an.set_stream(stream.cuda_stream)
-    an.set_fullsearch_variant(args.variant)
-    for r in range(nref):
-        an.set_reference(r, pics[nref - 1 - r])   # list 0: most recent first
-    an.upload(pics[nref])
-    an.sync()
-    refs = list(range(nref))
-    for _ in range(args.warmup):
-        an.motion_search(refs)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record(stream)
-    for _ in range(args.steps):
-        an.motion_search(refs)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_kernel = ev0.elapsed_time(ev1) / args.steps
-    ms_step = max(ms_kernel, wall * 1000.0 / args.steps)
-    mb_w, mb_h = an.mb_w, an.mb_h
+    params = proposal.load_weights()
+    common = dict(nrefs=nref, slices=68, search_range=args.range)
+    full = Analyser(width, height, search_mode=abi.SEARCH_FULL, **common)
+    learned = Analyser(width, height, search_mode=abi.SEARCH_LEARNED, fallback_ratio=1.1, **common)
+    nofb = Analyser(width, height, search_mode=abi.SEARCH_LEARNED, fallback_ratio=1e30, **common)
+    for an in (full, learned, nofb):
+        an.set_stream(stream.cuda_stream)
+    full.set_fullsearch_variant(args.variant)
+    for an in (learned, nofb):
+        an.load_proposal_weights(params)
+    # code pictures 0-3 (IDR at 25, P at 28) with the learned search: references with the encoder's vectors
+    refs = []
+    for t in range(4):
+        n = min(len(refs), nref)
+        for s in range(n):
+            learned.set_reference(s, refs[s][0], refs[s][1])
+        res = learned.analyse(pics[t], 25 if t == 0 else 28, abi.PIC_IDR if t == 0 else abi.PIC_P, list(range(n)))
+        refs.insert(0, (res.recon, res.mbs))
+    slots = list(range(nref))
+    for an in (full, learned, nofb):
+        for s in slots:
+            an.set_reference(s, refs[s][0], refs[s][1])
+        an.upload(pics[4])
+        an.sync()
+    mb_w, mb_h = full.mb_w, full.mb_h
     mbs = mb_w * mb_h
-    value = mbs / (ms_step / 1000.0)
-    absdiff = window_positions(mb_w, mb_h, args.range) * 256 * nref
-    achieved = absdiff / (ms_kernel / 1000.0) / 1e12
-    got = an.read_motion(nref)
-    an.close()
 
-    # the exact analysis (code_mb over every slice) of the same P picture, timed the same way
-    analysis = {}
-    for slices in (8, 68):
-        with Analyser(width, height, nrefs=nref, slices=slices, search_mode=abi.SEARCH_FULL,
-                      search_range=args.range) as az:
-            az.set_stream(stream.cuda_stream)
-            refres = az.analyse(pics[0], 25, abi.PIC_IDR)
-            for r in range(nref):
-                az.set_reference(r, pics[nref - 1 - r] if r else refres.recon, None if r else refres.mbs)
-            az.upload(pics[nref])
-            az.motion_search(refs)
-            az.sync()
-            reps = 3
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            for _ in range(reps):
-                az.motion_search(refs)
-                az.analyse_uploaded(28, abi.PIC_P, refs)
-            e1.record(stream)
-            torch.cuda.synchronize()
-            ms = e0.elapsed_time(e1) / reps
-            analysis[f"slices{slices}"] = {"ms_per_picture": round(ms, 3), "MB_per_s": round(mbs_ / (ms / 1000.0), 1)}
+    ms_full, wall_full = timed(stream, lambda: full.motion_search(slots), args.steps, args.warmup)
+    ms_learn, wall_learn = timed(stream, lambda: learned.motion_search(slots), args.steps, args.warmup)
+    ms_prop, _ = timed(stream, lambda: nofb.motion_search(slots), args.steps, args.warmup)
+    fs = full.read_motion(nref)
+    lr = learned.read_motion(nref)
+    nfb = learned.fallback_count()
+
+    # cost gap: SAD of the winners, per reference (16x16), best reference (16x16), partitions to 8x8
+    s_full = fs["sad"][:, :, 0].astype(np.float64)
+    s_lrn = lr["sad"][:, :, 0].astype(np.float64)
+    gap_ref = [float(s_lrn[r].sum() / s_full[r].sum() - 1.0) for r in range(nref)]
+    gap_best = float(s_lrn.min(axis=0).sum() / s_full.min(axis=0).sum() - 1.0)
+    gap_parts = float(lr["sad"][:, :, :9].astype(np.float64).sum() / fs["sad"][:, :, :9].astype(np.float64).sum() - 1.0)
+    # the encoder's costs after the exact analysis with either result as its extra candidate
+    full.analyse_uploaded(28, abi.PIC_P, slots)
+    cf = read_stats(full, mbs)
+    learned.analyse_uploaded(28, abi.PIC_P, slots)
+    cl = read_stats(learned, mbs)
+    inter = (cf["inter_cost"] >= 0) & (cl["inter_cost"] >= 0)
+    gap_inter = float(cl["inter_cost"][inter].astype(np.float64).sum() / max(1, cf["inter_cost"][inter].sum()) - 1.0)
+    gap_rd = float(cl["rd_cost"].sum() / cf["rd_cost"].sum() - 1.0)
+    for an in (full, learned, nofb):
+        an.close()
+
+    value = mbs / (max(ms_learn, wall_learn) / 1000.0)
+    value_full = mbs / (max(ms_full, wall_full) / 1000.0)
+    absdiff = window_positions(mb_w, mb_h, args.range) * 256 * nref
+    achieved = absdiff / (ms_full / 1000.0) / 1e12
+    cnn_t = CNN_FLOP_PER_MB * mbs / (ms_prop / 1000.0) / 1e12
 
     cpu = None
     if not args.no_cpu:
@@ -122,10 +160,10 @@ def main():
 
         pyoracle.build()
         t = time.perf_counter()
-        pyoracle.fullsearch(pics[nref].y, [pics[nref - 1].y], mb_w, mb_h, args.range)
+        pyoracle.fullsearch(pics[4].y, [refs[0][0].y], mb_w, mb_h, args.range)
         dt = time.perf_counter() - t
         cpu = {"value": round(mbs / (dt * nref), 1), "unit": "MB/s", "cores": 1, "kind": "port",
-               "sample": f"oracle scalar exhaustive search of one 1080p picture against 1 reference "
+               "sample": f"oracle scalar exhaustive search of the 1080p picture against 1 reference "
                          f"(+-{args.range}, 41 partitions), {dt:.1f} s, scaled to {nref} references"}
 
     out = {
@@ -135,25 +173,33 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": round(max(ms_learn, wall_learn), 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u8",
-        "data": "synthetic (seeded generator clip, 1920x1080 4:2:0, inputs resident in HBM)",
-        "config": {"workload": "plain GPU exhaustive integer search, 1920x1080 P picture, "
-                               f"+-{args.range}, 41 partitions, {nref} references (the full-search baseline; "
-                               "the learned proposal is not built yet)",
-                   "macroblocks_per_step": mbs, "references": nref, "search_range": args.range,
-                   "kernel_variant": fullsearch_variants()[args.variant] if args.variant >= 0 else "default"},
+        "data": "synthetic (seeded generator clip 1920x1080 4:2:0; references coded IPPP by the analysis; "
+                "inputs resident in HBM)",
+        "config": {"workload": "learned motion proposal at 1920x1080: P picture against 3 references, CNN "
+                               "4x(3x3 conv, 32 ch) BF16 on MFMA, 8 candidates/reference refined by exact +-2 SAD "
+                               "for 41 partitions, fallback to +-32 full search at 1.1x predicted cost",
+                   "macroblocks_per_step": mbs, "references": nref, "search_range": args.range},
+        "full_search": {"MB_per_s": round(value_full, 1), "ms_per_picture": round(max(ms_full, wall_full), 4),
+                        "kernel_variant": fullsearch_variants()[args.variant] if args.variant >= 0 else "default"},
+        "speedup_vs_full_search": round(value / value_full, 3),
+        "cost_gap": {"sad16_per_reference": [round(g, 5) for g in gap_ref], "sad16_best_reference": round(gap_best, 5),
+                     "sad_partitions_16x16_to_8x8": round(gap_parts, 5), "encoder_inter_cost": round(gap_inter, 5),
+                     "encoder_rd_cost": round(gap_rd, 5)},
+        "fallback_rate": round(nfb / (mbs * nref), 4),
         "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": round(VALU_ABSDIFF_PEAK_T, 1),
                      "unit": "Tabsdiff/s", "frac": round(achieved / VALU_ABSDIFF_PEAK_T, 4), "traffic": None,
-                     "kernel": "fullsearch_kernel", "ms_per_launch": round(ms_kernel, 4),
-                     "work_per_launch": absdiff},
+                     "kernel": "fullsearch_kernel", "ms_per_launch": round(ms_full, 4), "work_per_launch": absdiff},
+        "learned_roofline": {"bound": "mfma", "achieved": round(cnn_t, 2), "peak": MFMA_BF16_PEAK_T, "unit": "TFLOP/s",
+                             "frac": round(cnn_t / MFMA_BF16_PEAK_T, 5), "kernel": "proposal_kernel",
+                             "ms_per_launch": round(ms_prop, 4), "cnn_flop_per_mb": CNN_FLOP_PER_MB,
+                             "note": "CNN FLOPs over the whole proposal step (fallback off), which also computes "
+                                     "the cost volume and the refinement SADs"},
         "cpu_baseline": cpu,
-        "analysis": {"what": "exhaustive search + exact code_mb analysis of the same P picture (QP 28, "
-                             "3 references, CABAC, 8x8 transform), per slice count", **analysis},
-        "check": {"sad16_mean": float(np.mean(got["sad"][:, :, 0]))},
     }
     print(json.dumps(out))
 

@@ -16,8 +16,9 @@
 //    holds the candidate's row index in the window. Partition SADs are sums of 4x4 words, so every
 //    partition's word is already a sort key (SAD, then row) at one add per partition, and a running
 //    minimum per partition costs one v_min (v_min3 for two candidates) per candidate.
-//  - A lane's strips are taken in increasing row order (the lane stride is at least the window
-//    width), so a smaller row index means an earlier candidate in raster order; at the end the keys
+//  - A lane's candidates are taken in increasing row order, and within one row (possible when the
+//    window is wider than the workgroup) in increasing column order, so the low half orders them as
+//    row * TQ + column step, TQ being the most columns of one row a lane visits. At the end the keys
 //    are rewritten as (SAD, raster index) and reduced over the workgroup.
 #include "common.h"
 
@@ -71,20 +72,16 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
+// Dwords per staged row: the window's first byte sits up to 3 bytes into the first dword, and the
+// last candidate column reads 5 dwords from its own.
+constexpr int raws_for(int rmax) { return ((3 + 2 * rmax + 16) >> 2) + 2; }
+// The kernels come in two window sizes: ranges up to 32 and up to 64.
+constexpr int rmax_of(int raws) { return raws == raws_for(32) ? 32 : 64; }
+
 // K: candidates per strip (rows of one column); NT: threads; RAWS: dwords per staged window row;
 // WPE: waves per SIMD the register allocation must allow.
-// With a list, block i searches entry list[i] = mb * 4 + ref (blocks past *count do nothing).
-template <int K, int NT, int RAWS, int WPE>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void fullsearch_kernel(
-    FsArgs a, int rows_lds, const int *list, const int *count) {
-  extern __shared__ uint32_t raw[];
-  int mb = blockIdx.x, r = blockIdx.y;
-  if (list) {
-    if ((int)blockIdx.x >= *count) return;
-    const int e = list[blockIdx.x];
-    mb = e >> 2;
-    r = e & 3;
-  }
+template <int K, int NT, int RAWS>
+__device__ __forceinline__ void search_one(const FsArgs &a, int rows_lds, int mb, int r, uint32_t *raw) {
   const int mbx = mb % a.mb_w, mby = mb / a.mb_w;
   const int pw = a.mb_w * 16, ph = a.mb_h * 16, sw = pw + 2 * PAD, sh = ph + 2 * PAD;
   const int x0 = mbx * 16, y0 = mby * 16, R = a.range;
@@ -92,6 +89,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   const int dylo = max(-R, -(y0 + 24)), dyhi = min(R, ph - y0 + 8);
   const int C = dxhi - dxlo + 1, NR = dyhi - dylo + 1;
   const int S = (NR + K - 1) / K;
+  // a lane may visit several columns of one strip (when C > NT): its keys order them by row, then
+  // by the lane's column step c / NT, so that ties resolve in raster order and decode exactly
+  constexpr bool MULTI = 2 * rmax_of(RAWS) + 1 > NT;  // can a lane visit two columns of a row?
+  const int TQ = MULTI ? (C + NT - 1) / NT : 1;
   const uint8_t *g = a.ref_g[r];
   const int tid = threadIdx.x;
 
@@ -125,7 +126,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 #pragma unroll
     for (int k = 0; k < K; k++)
 #pragma unroll
-      for (int b = 0; b < 16; b++) acc[k][b] = (uint32_t)(i0 + k);
+      for (int b = 0; b < 16; b++) acc[k][b] = (uint32_t)((i0 + k) * TQ + (MULTI ? c / NT : 0));
 #pragma unroll
     for (int j = 0; j < K + 15; j++) {
       const uint32_t *rp = col + j * RAWS;
@@ -183,10 +184,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     uint32_t gk = 0xffffffffu;
     if (v != 0xffffffffu) {
       const uint32_t sad = v >> 16;
-      const int i = (int)((v & 0xffffu) >> lg);
+      const int o = (int)((v & 0xffffu) >> lg), i = o / TQ;
       const int s = i / K;
       const int t = s * C > tid ? (s * C - tid + NT - 1) / NT : 0;
-      const int c = tid + t * NT - s * C;
+      const int c = tid + t * NT - s * C + (o - i * TQ) * NT;  // the lane's first column of the strip, stepped
       gk = (sad << 15) | (uint32_t)(i * C + c);
     }
     gk = wave_min(gk);
@@ -211,15 +212,34 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   }
 }
 
-// Dwords per staged row: the window's first byte sits up to 3 bytes into the first dword, and the
-// last candidate column reads 5 dwords from its own.
-constexpr int raws_for(int rmax) { return ((3 + 2 * rmax + 16) >> 2) + 2; }
+// Block (x, y) searches macroblock x against reference y.
+template <int K, int NT, int RAWS, int WPE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void fullsearch_kernel(FsArgs a,
+                                                                                                 int rows_lds) {
+  extern __shared__ uint32_t raw[];
+  search_one<K, NT, RAWS>(a, rows_lds, blockIdx.x, blockIdx.y, raw);
+}
+
+// The blocks walk the entries list[i] = mb * 4 + ref, i < *count, grid-stride (a separate kernel, so
+// that the loop's registers do not weigh on the whole-picture kernel).
+template <int K, int NT, int RAWS, int WPE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void fullsearch_list_kernel(
+    FsArgs a, int rows_lds, const int *list, const int *count) {
+  extern __shared__ uint32_t raw[];
+  const int n = *count;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int e = list[i];
+    search_one<K, NT, RAWS>(a, rows_lds, e >> 2, e & 3, raw);
+    __syncthreads();  // the next entry restages the window and the reduction scratch
+  }
+}
+
 
 struct Variant {
   const char *name;
   int k, nt;
-  void (*fn32)(FsArgs, int, const int *, const int *);  // ranges up to 32
-  void (*fn64)(FsArgs, int, const int *, const int *);  // ranges up to 64
+  void (*fn32)(FsArgs, int);  // ranges up to 32
+  void (*fn64)(FsArgs, int);  // ranges up to 64
 };
 
 #define IA_FS_VARIANT(K, NT, W)                                                                         \
@@ -231,6 +251,8 @@ const Variant VARIANTS[] = {
 #undef IA_FS_VARIANT
 constexpr int NVARIANTS = sizeof(VARIANTS) / sizeof(VARIANTS[0]);
 constexpr int DEFAULT_VARIANT = 4;  // k6_t128_w3: fastest on MI355X at 1080p +-32 (measured)
+// the list kernel's tile: its grid-stride loop needs more registers than 3 waves per SIMD allow
+constexpr int LIST_K = 6, LIST_NT = 128, LIST_W = 2;
 
 }  // namespace
 
@@ -240,8 +262,6 @@ const char *fullsearch_variant_name(int v) { return v >= 0 && v < NVARIANTS ? VA
 hipError_t launch_fullsearch(const FsArgs &a, int variant, hipStream_t stream) {
   if (variant < 0 || variant >= NVARIANTS) variant = DEFAULT_VARIANT;
   if (a.range < 1 || a.range > RMAX || a.nref < 1 || a.nref > 16) return hipErrorInvalidValue;
-  // a lane's strips must come in increasing row order: at least one lane per window column
-  if (2 * a.range + 1 > VARIANTS[variant].nt) variant = DEFAULT_VARIANT;
   const Variant &v = VARIANTS[variant];
   const bool small = a.range <= 32;
   const int raws = small ? raws_for(32) : raws_for(64);
@@ -249,21 +269,21 @@ hipError_t launch_fullsearch(const FsArgs &a, int variant, hipStream_t stream) {
   const int rows_lds = 2 * a.range + 1 + v.k + 15;
   const size_t shmem = (size_t)rows_lds * raws * sizeof(uint32_t);
   dim3 grid(a.mb_w * a.mb_h, a.nref);
-  hipLaunchKernelGGL(small ? v.fn32 : v.fn64, grid, dim3(v.nt), shmem, stream, a, rows_lds, (const int *)nullptr,
-                     (const int *)nullptr);
+  hipLaunchKernelGGL(small ? v.fn32 : v.fn64, grid, dim3(v.nt), shmem, stream, a, rows_lds);
   return hipGetLastError();
 }
 
 hipError_t launch_fullsearch_list(const FsArgs &a, const int *list, const int *count, hipStream_t stream) {
-  const Variant &v = VARIANTS[DEFAULT_VARIANT];
-  if (a.range < 1 || a.range > RMAX || a.nref < 1 || a.nref > 4 || 2 * a.range + 1 > v.nt) return hipErrorInvalidValue;
+  if (a.range < 1 || a.range > RMAX || a.nref < 1 || a.nref > 4) return hipErrorInvalidValue;
   const bool small = a.range <= 32;
   const int raws = small ? raws_for(32) : raws_for(64);
-  const int rows_lds = 2 * a.range + 1 + v.k + 15;
+  const int rows_lds = 2 * a.range + 1 + LIST_K + 15;
   const size_t shmem = (size_t)rows_lds * raws * sizeof(uint32_t);
-  // one block per possible entry; those past the device-side count return at once
-  dim3 grid(a.mb_w * a.mb_h * a.nref);
-  hipLaunchKernelGGL(small ? v.fn32 : v.fn64, grid, dim3(v.nt), shmem, stream, a, rows_lds, list, count);
+  // a grid of at most 2048 blocks walks the device-side list
+  dim3 grid(min(a.mb_w * a.mb_h * a.nref, 2048));
+  hipLaunchKernelGGL(small ? (fullsearch_list_kernel<LIST_K, LIST_NT, raws_for(32), LIST_W>)
+                           : (fullsearch_list_kernel<LIST_K, LIST_NT, raws_for(64), LIST_W>),
+                     grid, dim3(LIST_NT), shmem, stream, a, rows_lds, list, count);
   return hipGetLastError();
 }
 

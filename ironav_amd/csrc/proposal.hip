@@ -29,6 +29,10 @@ constexpr int WAVES = 4;
 constexpr int PG = CNN_GRID + 2;   // padded grid side (one cell of zeros around)
 constexpr int PCELLS = PG * PG;
 constexpr int AST = 40;            // bf16 per cell in an activation buffer (32 channels + 8 of padding)
+constexpr int WKS = 40;            // bf16 per output channel's weight row of one tap (32 inputs + 8 of padding)
+constexpr int WIN = 52;            // staged reference window side: +-18 displacements around the centre + 16
+constexpr int WDW = 16;            // dwords staged per window row
+constexpr int WST = 19;            // dwords per staged window row in LDS (padded against bank conflicts)
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
   uint32_t u = __float_as_uint(f);
@@ -37,18 +41,6 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 
 __device__ __forceinline__ uint32_t ld_u32(const uint8_t *p) { return *reinterpret_cast<const uint32_t *>(p); }
-
-// one row of 16 reference samples at byte address p (any alignment) as 4 dwords
-__device__ __forceinline__ void ld_row16(const uint8_t *p, uint32_t &r0, uint32_t &r1, uint32_t &r2, uint32_t &r3) {
-  const uintptr_t a = (uintptr_t)p;
-  const uint32_t *q = reinterpret_cast<const uint32_t *>(a & ~(uintptr_t)3);
-  const uint32_t sh = (uint32_t)(a & 3);
-  const uint32_t w0 = q[0], w1 = q[1], w2 = q[2], w3 = q[3], w4 = q[4];
-  r0 = __builtin_amdgcn_alignbyte(w1, w0, sh);
-  r1 = __builtin_amdgcn_alignbyte(w2, w1, sh);
-  r2 = __builtin_amdgcn_alignbyte(w3, w2, sh);
-  r3 = __builtin_amdgcn_alignbyte(w4, w3, sh);
-}
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -115,17 +107,18 @@ __device__ __forceinline__ uint32_t ford(float f) {
 
 struct alignas(16) WaveLds {
   uint16_t x0[PCELLS][8];          // input features, bf16, zero border
-  uint16_t act[2][PCELLS][AST];    // activations, bf16, zero border
+  union {
+    uint16_t act[2][PCELLS][AST];  // activations, bf16, zero border
+    uint32_t win[3][WIN][WST];     // before and after the CNN: each reference's window, aligned dwords
+  };
   float out[CNN_CELLS][CNN_OUT];   // last layer
-  int cand[3][CNN_TOPK];           // selected cells per reference
-  int16_t ctr[3][2];               // grid centre per reference (samples)
 };
 
 struct PropLds {
   uint16_t w0[12][CNN_CH][8];      // layer 0: K = 12 taps (9 real) x 8 channels
-  uint16_t w1[9][CNN_CH][CNN_CH];
-  uint16_t w2[9][CNN_CH][CNN_CH];
-  uint16_t w3[9][16][CNN_CH];      // layer 3: 16 outputs (CNN_OUT real)
+  uint16_t w1[9][CNN_CH][WKS];
+  uint16_t w2[9][CNN_CH][WKS];
+  uint16_t w3[9][16][WKS];         // layer 3: 16 outputs (CNN_OUT real)
   float b[CNN_LAYERS][CNN_CH];
   WaveLds wv[WAVES];
 };
@@ -140,37 +133,51 @@ __device__ void conv_layer(const uint16_t *in, int ist, const uint16_t *w, const
                            float *outf) {
   const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
   constexpr int KSTEPS = IN0 ? 3 : 9;
+  // the lane's B fragments of every K step and output tile, held in registers for the layer
+  bf16x8 bw[KSTEPS][NT];
+#pragma unroll
+  for (int ks = 0; ks < KSTEPS; ks++)
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) {
+      if (IN0) {
+        const int t = ks * 4 + g;  // this lane's tap; taps 9..11 are zero
+        bw[ks][nt] = t < 9 ? *reinterpret_cast<const bf16x8 *>(w + ((size_t)t * CNN_CH + nt * 16 + m) * 8) : bf16x8{};
+      } else {
+        bw[ks][nt] = *reinterpret_cast<const bf16x8 *>(w + ((size_t)ks * (NT * 16) + nt * 16 + m) * WKS + g * 8);
+      }
+    }
+  float bv[NT];
+#pragma unroll
+  for (int nt = 0; nt < NT; nt++) bv[nt] = bias[nt * 16 + m];
+#pragma unroll 2
   for (int mt = 0; mt < 6; mt++) {
     const int p = min(mt * 16 + m, CNN_CELLS - 1);  // this lane's A row (grid cell)
     const int pp = padpos(p);
+    f32x4 acc[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < KSTEPS; ks++) {
+      bf16x8 a;
+      if (IN0) {
+        const int t = ks * 4 + g;
+        a = t < 9 ? *reinterpret_cast<const bf16x8 *>(in + (size_t)(pp + tapoff(t)) * ist) : bf16x8{};
+      } else {
+        a = *reinterpret_cast<const bf16x8 *>(in + (size_t)(pp + tapoff(ks)) * ist + g * 8);
+      }
+      // one A fragment feeds every output tile
+#pragma unroll
+      for (int nt = 0; nt < NT; nt++) acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[ks][nt], acc[nt], 0, 0, 0);
+    }
+    // D: this lane holds output channel nt*16 + m for cells mt*16 + 4g + i
 #pragma unroll
     for (int nt = 0; nt < NT; nt++) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int ks = 0; ks < KSTEPS; ks++) {
-        bf16x8 a, b;
-        if (IN0) {
-          const int t = ks * 4 + g;  // this lane's tap; taps 9..11 are zero
-          if (t < 9) {
-            a = *reinterpret_cast<const bf16x8 *>(in + (size_t)(pp + tapoff(t)) * ist);
-            b = *reinterpret_cast<const bf16x8 *>(w + ((size_t)t * CNN_CH + nt * 16 + m) * 8);
-          } else {
-            a = bf16x8{};
-            b = bf16x8{};
-          }
-        } else {
-          a = *reinterpret_cast<const bf16x8 *>(in + (size_t)(pp + tapoff(ks)) * ist + g * 8);
-          b = *reinterpret_cast<const bf16x8 *>(w + ((size_t)ks * (NT * 16) + nt * 16 + m) * CNN_CH + g * 8);
-        }
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
-      }
-      // D: this lane holds output channel nt*16 + m for cells mt*16 + 4g + i
       const int co = nt * 16 + m;
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int q = mt * 16 + 4 * g + i;
         if (q < CNN_CELLS) {
-          float v = acc[i] + bias[co];
+          float v = acc[nt][i] + bv[nt];
           if (RELU) v = fmaxf(v, 0.f);
           if (outb) outb[(size_t)padpos(q) * AST + co] = f2bf(v);
           if (outf && co < CNN_OUT) outf[q * CNN_OUT + co] = v;
@@ -180,97 +187,127 @@ __device__ void conv_layer(const uint16_t *in, int ist, const uint16_t *w, const
   }
 }
 
+// stage reference r's window (WIN x WIN samples from plane row wy, column wx) as aligned dwords;
+// returns the byte offset of the window's first sample in its first dword
+__device__ __forceinline__ int stage_window(const uint8_t *g, int sw, int wx, int wy, uint32_t (*win)[WST]) {
+  const int lane = threadIdx.x & 63, al = wx & ~3;
+  const uint8_t *base = g + (size_t)wy * sw + al;
+#pragma unroll
+  for (int k = 0; k < WIN * WDW / 64; k++) {
+    const int i = lane + k * 64, row = i / WDW, d = i % WDW;
+    win[row][d] = ld_u32(base + (size_t)row * sw + 4 * d);
+  }
+  return wx - al;
+}
+
+// the 4 dwords of 16 window samples at window column x of row y
+__device__ __forceinline__ void win_row16(const uint32_t (*win)[WST], int off, int x, int y, uint32_t &r0, uint32_t &r1,
+                                          uint32_t &r2, uint32_t &r3) {
+  const int b = off + x, q = b >> 2;
+  const uint32_t sh = (uint32_t)(b & 3);
+  const uint32_t *p = win[y] + q;
+  const uint32_t w0 = p[0], w1 = p[1], w2 = p[2], w3 = p[3], w4 = p[4];
+  r0 = __builtin_amdgcn_alignbyte(w1, w0, sh);
+  r1 = __builtin_amdgcn_alignbyte(w2, w1, sh);
+  r2 = __builtin_amdgcn_alignbyte(w3, w2, sh);
+  r3 = __builtin_amdgcn_alignbyte(w4, w3, sh);
+}
+
+// Persistent: each workgroup loads the weights once, then its waves take macroblocks in turn.
 __global__ __launch_bounds__(64 * WAVES) void proposal_kernel(PropArgs a) {
   extern __shared__ uint8_t lds_raw[];
   PropLds &L = *reinterpret_cast<PropLds *>(lds_raw);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  // weights into LDS
   {
     const CnnWeights &W = *a.wts;
-    for (int i = tid; i < 12 * CNN_CH * 8; i += 64 * WAVES) {
-      const int t = i / (CNN_CH * 8), r = i % (CNN_CH * 8), co = r / 8, ci = r % 8;
-      (&L.w0[0][0][0])[i] = t < 9 ? W.w[0][t][co][ci] : 0;
+    // layer 0: taps 9..11 zero; layers 1, 2 copied as 16-byte vectors; layer 3: the first 16 outputs
+    for (int i = tid; i < 12 * CNN_CH; i += 64 * WAVES) {
+      const int t = i / CNN_CH, co = i % CNN_CH;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (t < 9) v = *reinterpret_cast<const uint4 *>(&W.w[0][t][co][0]);
+      *reinterpret_cast<uint4 *>(&L.w0[t][co][0]) = v;
     }
-    for (int i = tid; i < 9 * CNN_CH * CNN_CH; i += 64 * WAVES) {
-      (&L.w1[0][0][0])[i] = (&W.w[1][0][0][0])[i];
-      (&L.w2[0][0][0])[i] = (&W.w[2][0][0][0])[i];
+    // rows of 32 bf16 (one output channel of one tap) into rows of WKS
+    for (int i = tid; i < 9 * CNN_CH * 4; i += 64 * WAVES) {
+      const int row = i >> 2, part = i & 3;
+      *reinterpret_cast<uint4 *>(&L.w1[0][0][0] + row * WKS + part * 8) =
+          reinterpret_cast<const uint4 *>(&W.w[1][0][0][0])[i];
+      *reinterpret_cast<uint4 *>(&L.w2[0][0][0] + row * WKS + part * 8) =
+          reinterpret_cast<const uint4 *>(&W.w[2][0][0][0])[i];
     }
-    for (int i = tid; i < 9 * 16 * CNN_CH; i += 64 * WAVES) {
-      const int t = i / (16 * CNN_CH), r = i % (16 * CNN_CH), co = r / CNN_CH, ci = r % CNN_CH;
-      (&L.w3[0][0][0])[i] = W.w[3][t][co][ci];
+    for (int i = tid; i < 9 * 16 * 4; i += 64 * WAVES) {
+      const int t = i / 64, row = (i >> 2) & 15, part = i & 3;
+      *reinterpret_cast<uint4 *>(&L.w3[t][row][part * 8]) =
+          reinterpret_cast<const uint4 *>(&W.w[3][t][row][0])[part];
     }
     for (int i = tid; i < CNN_LAYERS * CNN_CH; i += 64 * WAVES) (&L.b[0][0])[i] = (&W.b[0][0])[i];
   }
   __syncthreads();
-  const int mb = blockIdx.x * WAVES + wave;
-  const int nmb = a.mb_w * a.mb_h;
-  if (mb >= nmb) return;
   WaveLds &V = L.wv[wave];
-  const int mbx = mb % a.mb_w, mby = mb / a.mb_w;
+  const int nmb = a.mb_w * a.mb_h;
   const int pw = a.mb_w * 16, ph = a.mb_h * 16, sw = pw + 2 * PAD;
-  const int x0 = mbx * 16, y0 = mby * 16;
-  const int lox = -(x0 + 24), hix = pw - x0 + 8, loy = -(y0 + 24), hiy = ph - y0 + 8;
+  for (int mb = blockIdx.x * WAVES + wave; mb < nmb; mb += gridDim.x * WAVES) {
+    const int mbx = mb % a.mb_w, mby = mb / a.mb_w;
+    const int x0 = mbx * 16, y0 = mby * 16;
+    const int lox = -(x0 + 24), hix = pw - x0 + 8, loy = -(y0 + 24), hiy = ph - y0 + 8;
 
-  // clear the padded buffers
-  for (int i = lane; i < PCELLS * 8; i += 64) (&V.x0[0][0])[i] = 0;
-  for (int i = lane; i < 2 * PCELLS * AST; i += 64) (&V.act[0][0][0])[i] = 0;
-
-  // the macroblock's samples, uniform
-  uint32_t src[64];
-  {
-    const uint8_t *sp = a.src_y + (size_t)y0 * pw + x0;
+    // the macroblock's samples, uniform
+    uint32_t src[64];
+    {
+      const uint8_t *sp = a.src_y + (size_t)y0 * pw + x0;
 #pragma unroll
-    for (int i = 0; i < 64; i++) src[i] = __builtin_amdgcn_readfirstlane(ld_u32(sp + (size_t)(i >> 2) * pw + (i & 3) * 4));
-  }
-  // temporal predictors: reference 0's vectors (quarter samples) at this macroblock, left, above
-  int tmv[3][2] = {{0, 0}, {0, 0}, {0, 0}}, tok[3] = {0, 0, 0};
-  if (a.rmbs0) {
-    const int nx[3] = {mbx, mbx - 1, mbx}, ny[3] = {mby, mby, mby - 1};
-    for (int k = 0; k < 3; k++)
-      if (nx[k] >= 0 && ny[k] >= 0) {
-        const ia_mbinfo &m = a.rmbs0[ny[k] * a.mb_w + nx[k]];
-        if (m.kind == 5) {
-          tmv[k][0] = m.mv[5][0];
-          tmv[k][1] = m.mv[5][1];
-          tok[k] = 1;
+      for (int i = 0; i < 64; i++) src[i] = __builtin_amdgcn_readfirstlane(ld_u32(sp + (size_t)(i >> 2) * pw + (i & 3) * 4));
+    }
+    // temporal predictors: reference 0's vectors (quarter samples) at this macroblock, left, above
+    int tmv[3][2] = {{0, 0}, {0, 0}, {0, 0}}, tok[3] = {0, 0, 0};
+    if (a.rmbs0) {
+      const int nx[3] = {mbx, mbx - 1, mbx}, ny[3] = {mby, mby, mby - 1};
+      for (int k = 0; k < 3; k++)
+        if (nx[k] >= 0 && ny[k] >= 0) {
+          const ia_mbinfo &m = a.rmbs0[ny[k] * a.mb_w + nx[k]];
+          if (m.kind == 5) {
+            tmv[k][0] = m.mv[5][0];
+            tmv[k][1] = m.mv[5][1];
+            tok[k] = 1;
+          }
         }
-      }
-  }
-  // grid centres: the co-located vector scaled by the reference's distance, to a multiple of 4
-  // samples, clamped so the grid lies within the macroblock's vectors
-  int cx[3], cy[3];
-  for (int r = 0; r < 3; r++) {
-    int qx = (r + 1) * tmv[0][0], qy = (r + 1) * tmv[0][1];
-    int px = (int)floorf((qx + 8) / 16.0f) * 4, py = (int)floorf((qy + 8) / 16.0f) * 4;
-    cx[r] = min(max(px, lox + 16), hix - 16);
-    cy[r] = min(max(py, loy + 16), hiy - 16);
-  }
-  if (lane == 0)
+    }
+    // grid centres: the co-located vector scaled by the reference's distance, to a multiple of 4
+    // samples, clamped so the grid lies within the macroblock's vectors
+    int cx[3], cy[3];
     for (int r = 0; r < 3; r++) {
-      V.ctr[r][0] = (int16_t)cx[r];
-      V.ctr[r][1] = (int16_t)cy[r];
-      if (a.centre) {
+      const int qx = (r + 1) * tmv[0][0], qy = (r + 1) * tmv[0][1];
+      const int px = (int)floorf((qx + 8) / 16.0f) * 4, py = (int)floorf((qy + 8) / 16.0f) * 4;
+      cx[r] = min(max(px, lox + 16), hix - 16);
+      cy[r] = min(max(py, loy + 16), hiy - 16);
+    }
+    if (lane == 0 && a.centre)
+      for (int r = 0; r < 3; r++) {
         a.centre[(mb * 3 + r) * 2] = (int16_t)cx[r];
         a.centre[(mb * 3 + r) * 2 + 1] = (int16_t)cy[r];
       }
-    }
-  wave_sync();
+    const int nref = a.feat_in ? 0 : a.nref;
+    int woff[3] = {0, 0, 0};
+    for (int r = 0; r < nref; r++)
+      woff[r] = stage_window(a.ref_g[r], sw, x0 + cx[r] - 18 + PAD, y0 + cy[r] - 18 + PAD, V.win[r]);
+    for (int i = lane; i < PCELLS * 8 / 8; i += 64) reinterpret_cast<uint4 *>(&V.x0[0][0])[i] = make_uint4(0, 0, 0, 0);
+    wave_sync();
 
-  // features
-  if (a.feat_in) {
-    for (int i = lane; i < CNN_CELLS * CNN_IN; i += 64)
-      V.x0[padpos(i / CNN_IN)][i % CNN_IN] = f2bf(a.feat_in[(size_t)mb * CNN_CELLS * CNN_IN + i]);
-  } else {
-    // cost volume: lane handles cells lane and lane + 64 of each reference
-    for (int r = 0; r < a.nref; r++) {
-      const uint8_t *g = a.ref_g[r];
-      for (int c = lane; c < CNN_CELLS; c += 64) {
-        const int dx = cx[r] + 4 * (c % CNN_GRID - 4), dy = cy[r] + 4 * (c / CNN_GRID - 4);
-        const uint8_t *rp = g + (size_t)(y0 + dy + PAD) * sw + (x0 + dx + PAD);
+    // features
+    if (a.feat_in) {
+      for (int i = lane; i < CNN_CELLS * CNN_IN; i += 64)
+        V.x0[padpos(i / CNN_IN)][i % CNN_IN] = f2bf(a.feat_in[(size_t)mb * CNN_CELLS * CNN_IN + i]);
+    } else {
+      // cost volume from the staged windows: the nref * 81 (reference, cell) pairs spread over the lanes
+      for (int t = lane; t < nref * CNN_CELLS; t += 64) {
+        const int r = t >= 2 * CNN_CELLS ? 2 : t >= CNN_CELLS ? 1 : 0, c = t - r * CNN_CELLS;
+        const int off = r == 2 ? woff[2] : r == 1 ? woff[1] : woff[0];
+        const int wxp = 4 * (c % CNN_GRID) + 2, wyp = 4 * (c / CNN_GRID) + 2;  // window offset of the cell
         uint32_t s = 0;
+#pragma unroll
         for (int row = 0; row < 16; row++) {
           uint32_t r0, r1, r2, r3;
-          ld_row16(rp + (size_t)row * sw, r0, r1, r2, r3);
+          win_row16(V.win[r], off, wxp, wyp + row, r0, r1, r2, r3);
           s = __builtin_amdgcn_sad_u8(src[row * 4 + 0], r0, s);
           s = __builtin_amdgcn_sad_u8(src[row * 4 + 1], r1, s);
           s = __builtin_amdgcn_sad_u8(src[row * 4 + 2], r2, s);
@@ -280,111 +317,141 @@ __global__ __launch_bounds__(64 * WAVES) void proposal_kernel(PropArgs a) {
         V.x0[padpos(c)][r] = f2bf(f);
         if (a.feat) a.feat[((size_t)mb * CNN_CELLS + c) * CNN_IN + r] = f;
       }
-    }
-    for (int c = lane; c < CNN_CELLS; c += 64) {
-      const int gx = c % CNN_GRID, gy = c / CNN_GRID;
-      float fv[CNN_IN] = {0, 0, 0, 0, 0, 0, 1.0f, 0};
-      for (int k = 0; k < 3; k++) {
-        if (!tok[k]) continue;
-        const int kx = (int)floorf((tmv[k][0] - 4 * cx[0] + 8) / 16.0f) + 4;
-        const int ky = (int)floorf((tmv[k][1] - 4 * cy[0] + 8) / 16.0f) + 4;
-        if (kx == gx && ky == gy) fv[3 + k] = 1.0f;
+      for (int c = lane; c < CNN_CELLS; c += 64) {
+        const int gx = c % CNN_GRID, gy = c / CNN_GRID;
+        float fv[CNN_IN] = {0, 0, 0, 0, 0, 0, 1.0f, 0};
+        for (int k = 0; k < 3; k++) {
+          if (!tok[k]) continue;
+          const int kx = (int)floorf((tmv[k][0] - 4 * cx[0] + 8) / 16.0f) + 4;
+          const int ky = (int)floorf((tmv[k][1] - 4 * cy[0] + 8) / 16.0f) + 4;
+          if (kx == gx && ky == gy) fv[3 + k] = 1.0f;
+        }
+        for (int ch = 3; ch < CNN_IN; ch++) {
+          V.x0[padpos(c)][ch] = f2bf(fv[ch]);
+          if (a.feat) a.feat[((size_t)mb * CNN_CELLS + c) * CNN_IN + ch] = fv[ch];
+        }
+        if (a.feat)
+          for (int r = a.nref; r < 3; r++) a.feat[((size_t)mb * CNN_CELLS + c) * CNN_IN + r] = 0.f;
       }
-      for (int ch = 3; ch < CNN_IN; ch++) {
-        V.x0[padpos(c)][ch] = f2bf(fv[ch]);
-        if (a.feat) a.feat[((size_t)mb * CNN_CELLS + c) * CNN_IN + ch] = fv[ch];
+    }
+    wave_sync();
+    if (a.features_only) continue;
+
+    // the CNN (the windows' space now holds the activations, borders zero)
+    for (int i = lane; i < 2 * PCELLS * AST / 8; i += 64) reinterpret_cast<uint4 *>(&V.act[0][0][0])[i] = make_uint4(0, 0, 0, 0);
+    wave_sync();
+    conv_layer<true, 2, true>(&V.x0[0][0], 8, &L.w0[0][0][0], L.b[0], &V.act[0][0][0], nullptr);
+    wave_sync();
+    conv_layer<false, 2, true>(&V.act[0][0][0], AST, &L.w1[0][0][0], L.b[1], &V.act[1][0][0], nullptr);
+    wave_sync();
+    conv_layer<false, 2, true>(&V.act[1][0][0], AST, &L.w2[0][0][0], L.b[2], &V.act[0][0][0], nullptr);
+    wave_sync();
+    conv_layer<false, 1, false>(&V.act[0][0][0], AST, &L.w3[0][0][0], L.b[3], nullptr, &V.out[0][0]);
+    wave_sync();
+    if (a.cnn_out)
+      for (int i = lane; i < CNN_CELLS * CNN_OUT; i += 64) a.cnn_out[(size_t)mb * CNN_CELLS * CNN_OUT + i] = (&V.out[0][0])[i];
+    if (a.feat_in) continue;
+
+    // the windows again, for the refinement (the activations are done with)
+    for (int r = 0; r < nref; r++) stage_window(a.ref_g[r], sw, x0 + cx[r] - 18 + PAD, y0 + cy[r] - 18 + PAD, V.win[r]);
+    wave_sync();
+
+    for (int r = 0; r < nref; r++) {
+      // the 8 best cells of the reference's score map (ties to the lower cell index), uniform
+      int cand[CNN_TOPK];
+      {
+        uint32_t k0 = (ford(V.out[lane][r]) & ~0x7fu) | (uint32_t)(127 - lane);
+        uint32_t k1 = lane + 64 < CNN_CELLS ? (ford(V.out[lane + 64][r]) & ~0x7fu) | (uint32_t)(127 - lane - 64) : 0u;
+#pragma unroll
+        for (int k = 0; k < CNN_TOPK; k++) {
+          const uint32_t best = wave_max(max(k0, k1));
+          cand[k] = 127 - (int)(best & 0x7f);
+          if (cand[k] == lane) k0 = 0;
+          if (cand[k] == lane + 64) k1 = 0;
+        }
       }
-      if (a.feat)
-        for (int r = a.nref; r < 3; r++) a.feat[((size_t)mb * CNN_CELLS + c) * CNN_IN + r] = 0.f;
-    }
-  }
-  wave_sync();
-  if (a.features_only) return;
-
-  // the CNN
-  conv_layer<true, 2, true>(&V.x0[0][0], 8, &L.w0[0][0][0], L.b[0], &V.act[0][0][0], nullptr);
-  wave_sync();
-  conv_layer<false, 2, true>(&V.act[0][0][0], AST, &L.w1[0][0][0], L.b[1], &V.act[1][0][0], nullptr);
-  wave_sync();
-  conv_layer<false, 2, true>(&V.act[1][0][0], AST, &L.w2[0][0][0], L.b[2], &V.act[0][0][0], nullptr);
-  wave_sync();
-  conv_layer<false, 1, false>(&V.act[0][0][0], AST, &L.w3[0][0][0], L.b[3], nullptr, &V.out[0][0]);
-  wave_sync();
-  if (a.cnn_out)
-    for (int i = lane; i < CNN_CELLS * CNN_OUT; i += 64) a.cnn_out[(size_t)mb * CNN_CELLS * CNN_OUT + i] = (&V.out[0][0])[i];
-  if (a.feat_in) return;
-
-  // the 8 best cells per reference
-  for (int r = 0; r < a.nref; r++) {
-    uint32_t k0 = (ford(V.out[lane][r]) & ~0x7fu) | (uint32_t)(127 - lane);
-    uint32_t k1 = lane + 64 < CNN_CELLS ? (ford(V.out[lane + 64][r]) & ~0x7fu) | (uint32_t)(127 - lane - 64) : 0u;
-    for (int k = 0; k < CNN_TOPK; k++) {
-      const uint32_t best = wave_max(max(k0, k1));
-      const int cell = 127 - (int)(best & 0x7f);
-      if (lane == 0) V.cand[r][k] = cell;
-      if (cell == lane) k0 = 0;
-      if (cell == lane + 64) k1 = 0;
-    }
-  }
-  wave_sync();
-
-  // exact refinement of all 41 partitions around the selected cells
-  for (int r = 0; r < a.nref; r++) {
-    const uint8_t *g = a.ref_g[r];
-    uint32_t run[IA_NPART];
+      // exact refinement of all 41 partitions around the selected cells, +-2 samples: lane 5k + x of
+      // lanes 0..39 takes cell rank k, column x of its 5x5 block, and walks the column's 5
+      // displacements as a strip (each staged row read once feeds every candidate that uses it)
+      uint32_t run[IA_NPART];
 #pragma unroll
-    for (int p = 0; p < IA_NPART; p++) run[p] = 0xffffffffu;
-    for (int idx = lane; idx < CNN_TOPK * 25; idx += 64) {
-      const int k = idx / 25, j = idx % 25;
-      const int cell = V.cand[r][k];
-      const int dx = cx[r] + 4 * (cell % CNN_GRID - 4) + (j % 5 - 2);
-      const int dy = cy[r] + 4 * (cell / CNN_GRID - 4) + (j / 5 - 2);
-      if (dx < lox || dx > hix || dy < loy || dy > hiy) continue;
-      const uint8_t *rp = g + (size_t)(y0 + dy + PAD) * sw + (x0 + dx + PAD);
-      uint32_t acc[16];
+      for (int p = 0; p < IA_NPART; p++) run[p] = 0xffffffffu;
+      if (lane < 5 * CNN_TOPK) {
+        const int k = lane / 5, x = lane - 5 * k;
+        int cell = cand[0];
 #pragma unroll
-      for (int b = 0; b < 16; b++) acc[b] = (uint32_t)idx;
+        for (int q = 1; q < CNN_TOPK; q++)
+          if (q == k) cell = cand[q];
+        const int ox = 4 * (cell % CNN_GRID) + x, oy = 4 * (cell / CNN_GRID);  // window offset of the strip top
+        const int dx = cx[r] - 18 + ox, dy0 = cy[r] - 18 + oy;
+        const int off = r == 2 ? woff[2] : r == 1 ? woff[1] : woff[0];
+        uint32_t acc[5][16];
 #pragma unroll
-      for (int row = 0; row < 16; row++) {
-        uint32_t r0, r1, r2, r3;
-        ld_row16(rp + (size_t)row * sw, r0, r1, r2, r3);
-        const int by = row >> 2;
-        acc[by * 4 + 0] = __builtin_amdgcn_sad_hi_u8(src[row * 4 + 0], r0, acc[by * 4 + 0]);
-        acc[by * 4 + 1] = __builtin_amdgcn_sad_hi_u8(src[row * 4 + 1], r1, acc[by * 4 + 1]);
-        acc[by * 4 + 2] = __builtin_amdgcn_sad_hi_u8(src[row * 4 + 2], r2, acc[by * 4 + 2]);
-        acc[by * 4 + 3] = __builtin_amdgcn_sad_hi_u8(src[row * 4 + 3], r3, acc[by * 4 + 3]);
+        for (int i = 0; i < 5; i++)
+#pragma unroll
+          for (int b = 0; b < 16; b++) acc[i][b] = (uint32_t)(k * 25 + i * 5 + x);  // order: rank, then raster
+#pragma unroll
+        for (int y = 0; y < 20; y++) {
+          uint32_t r0, r1, r2, r3;
+          win_row16(V.win[r], off, ox, oy + y, r0, r1, r2, r3);
+#pragma unroll
+          for (int i = 0; i < 5; i++) {
+            const int row = y - i;
+            if (row >= 0 && row < 16) {
+              const int by = row >> 2;
+              acc[i][by * 4 + 0] = __builtin_amdgcn_sad_hi_u8(src[row * 4 + 0], r0, acc[i][by * 4 + 0]);
+              acc[i][by * 4 + 1] = __builtin_amdgcn_sad_hi_u8(src[row * 4 + 1], r1, acc[i][by * 4 + 1]);
+              acc[i][by * 4 + 2] = __builtin_amdgcn_sad_hi_u8(src[row * 4 + 2], r2, acc[i][by * 4 + 2]);
+              acc[i][by * 4 + 3] = __builtin_amdgcn_sad_hi_u8(src[row * 4 + 3], r3, acc[i][by * 4 + 3]);
+            }
+          }
+        }
+        const bool xin = dx >= lox && dx <= hix;
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+          if (xin && dy0 + i >= loy && dy0 + i <= hiy) {
+            uint32_t key[IA_NPART];
+            partition_keys(acc[i], key);
+#pragma unroll
+            for (int p = 0; p < IA_NPART; p++) run[p] = min(run[p], key[p]);
+          }
+        }
       }
-      uint32_t key[IA_NPART];
-      partition_keys(acc, key);
+      // lane p takes partition p's minimum, then lanes 0..40 decode and store their winners at once
+      uint32_t mine = 0xffffffffu;
 #pragma unroll
-      for (int p = 0; p < IA_NPART; p++) run[p] = min(run[p], key[p]);
-    }
-    uint32_t sad16 = 0xffffffffu;
-#pragma unroll
-    for (int p = 0; p < IA_NPART; p++) {
-      const int lg = p == 0 ? 4 : p < 5 ? 3 : p < 9 ? 2 : p < 25 ? 1 : 0;
-      const uint32_t v = wave_min(run[p]);
-      if (lane == p) {
+      for (int p = 0; p < IA_NPART; p++) {
+        const uint32_t v = wave_min(run[p]);
+        mine = lane == p ? v : mine;
+      }
+      const uint32_t best16 = (uint32_t)__builtin_amdgcn_readlane((int)mine, 0);
+      if (lane < IA_NPART) {
+        const int lg = lane == 0 ? 4 : lane < 5 ? 3 : lane < 9 ? 2 : lane < 25 ? 1 : 0;
         ia_fsresult res;
-        if (v == 0xffffffffu) {
+        if (mine == 0xffffffffu) {
           res.mv[0] = res.mv[1] = 0;
           res.sad = 0xffffffffu;
         } else {
-          const int idx = (int)((v & 0xffffu) >> lg), k = idx / 25, j = idx % 25, cell = V.cand[r][k];
+          const int idx = (int)((mine & 0xffffu) >> lg), k = idx / 25, j = idx % 25;
+          int cell = cand[0];
+#pragma unroll
+          for (int q = 1; q < CNN_TOPK; q++)
+            if (q == k) cell = cand[q];
           res.mv[0] = (int16_t)((cx[r] + 4 * (cell % CNN_GRID - 4) + (j % 5 - 2)) * 4);
           res.mv[1] = (int16_t)((cy[r] + 4 * (cell / CNN_GRID - 4) + (j / 5 - 2)) * 4);
-          res.sad = v >> 16;
+          res.sad = mine >> 16;
         }
-        a.out[((size_t)r * nmb + mb) * IA_NPART + p] = res;
+        a.out[((size_t)r * nmb + mb) * IA_NPART + lane] = res;
       }
-      if (p == 0) sad16 = v == 0xffffffffu ? v : v >> 16;
+      // fallback when the refined cost exceeds the ratio times the predicted one
+      const uint32_t sad16 = best16 == 0xffffffffu ? best16 : best16 >> 16;
+      const float pred = exp2f(V.out[cand[0]][3]);
+      if (lane == 0 && a.fb_list && (sad16 == 0xffffffffu || (float)sad16 > a.fallback_ratio * pred)) {
+        const int slot = atomicAdd(a.fb_count, 1);
+        a.fb_list[slot] = mb * 4 + r;
+      }
     }
-    // fallback when the refined cost exceeds the ratio times the predicted one
-    const float pred = exp2f(V.out[V.cand[r][0]][3]);
-    if (lane == 0 && a.fb_list && (sad16 == 0xffffffffu || (float)sad16 > a.fallback_ratio * pred)) {
-      const int slot = atomicAdd(a.fb_count, 1);
-      a.fb_list[slot] = mb * 4 + r;
-    }
+    wave_sync();  // the next macroblock restages the windows and features
   }
 }
 
@@ -393,8 +460,10 @@ __global__ __launch_bounds__(64 * WAVES) void proposal_kernel(PropArgs a) {
 hipError_t launch_proposal(const PropArgs &a, hipStream_t stream) {
   const int nmb = a.mb_w * a.mb_h;
   if (a.nref < 1 || a.nref > 3) return hipErrorInvalidValue;
-  dim3 grid((nmb + WAVES - 1) / WAVES);
+  // persistent: one workgroup per CU (its LDS holds the weights), waves walk the macroblocks
+  dim3 grid(min((nmb + WAVES - 1) / WAVES, 256));
   const size_t shmem = sizeof(PropLds);
+  static_assert(sizeof(PropLds) <= 160 * 1024, "the weights and four waves' buffers fit one CU's LDS");
   hipLaunchKernelGGL(proposal_kernel, grid, dim3(64 * WAVES), shmem, stream, a);
   return hipGetLastError();
 }

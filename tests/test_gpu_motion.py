@@ -46,10 +46,13 @@ def test_fullsearch_matches_oracle(oracle, w, h, rng, nref):
     assert np.array_equal(got["mv"], want["mv"])
 
 
-def test_fullsearch_variants_agree(oracle):
+@pytest.mark.parametrize("rng", [32, 64])
+def test_fullsearch_variants_agree(oracle, rng):
+    """Every tile shape against the oracle; at +-64 the window (129 columns) is wider than the
+    128-thread tiles, so a lane visits two columns of one row."""
     pics = frames(176, 144, 2, 77, max_motion=20)
     results = []
-    with Analyser(176, 144, nrefs=1, search_mode=abi.SEARCH_FULL, search_range=32) as an:
+    with Analyser(176, 144, nrefs=1, search_mode=abi.SEARCH_FULL, search_range=rng) as an:
         an.set_reference(0, pics[0])
         an.upload(pics[1])
         from ironav_amd.h264a import fullsearch_variants
@@ -57,10 +60,31 @@ def test_fullsearch_variants_agree(oracle):
             an.set_fullsearch_variant(v)
             an.motion_search([0])
             results.append(an.read_motion(1))
-    want = oracle.fullsearch(pics[1].y, [pics[0].y], an.mb_w, an.mb_h, 32)
+    want = oracle.fullsearch(pics[1].y, [pics[0].y], an.mb_w, an.mb_h, rng)
     for v, got in enumerate(results):
         assert np.array_equal(got["sad"], want["sad"]), v
         assert np.array_equal(got["mv"], want["mv"]), v
+
+
+def test_fullsearch_ties_resolve_in_raster_order(oracle):
+    """A flat reference with one textured square: most displacements tie, and each partition's winner
+    must be the first in raster order of (dy, dx), for every tile shape, at +-64."""
+    w, h = 192, 160
+    y = np.full((h, w), 90, dtype=np.uint8)
+    y[64:80, 96:112] = np.arange(256, dtype=np.uint8).reshape(16, 16)
+    c = np.full((h // 2, w // 2), 128, dtype=np.uint8)
+    ref, src = Picture(y, c, c), Picture(np.roll(y, (3, -5), axis=(0, 1)), c, c)
+    want = oracle.fullsearch(src.y, [ref.y], w // 16, h // 16, 64)
+    with Analyser(w, h, nrefs=1, search_mode=abi.SEARCH_FULL, search_range=64) as an:
+        an.set_reference(0, ref)
+        an.upload(src)
+        from ironav_amd.h264a import fullsearch_variants
+        for v in range(len(fullsearch_variants())):
+            an.set_fullsearch_variant(v)
+            an.motion_search([0])
+            got = an.read_motion(1)
+            assert np.array_equal(got["sad"], want["sad"]), v
+            assert np.array_equal(got["mv"], want["mv"]), v
 
 
 def _brute_one(src, refpad, mbx, mby, pw, ph, rng):
