@@ -1,14 +1,15 @@
 // proposal.hip — the learned motion proposal on gfx950: cost volume -> BF16 CNN on MFMA -> top-8
 // cells per reference -> exact SAD refinement of all 41 partitions -> fallback list.
 //
-// One wave per macroblock, four macroblocks per workgroup sharing the CNN weights in LDS.
+// Persistent: one workgroup of 8 waves per CU holds the CNN weights in LDS; each wave takes one
+// macroblock at a time.
 //  1. Cost volume: the exact 16x16 SAD at a 9x9 grid of displacements 4 samples apart around each
 //     reference's temporal predictor (reference 0's co-located vector, scaled by the reference's
 //     distance), clamped so the grid stays within the vectors the macroblock may take.
 //  2. Features on the grid (8 channels): log2(1 + SAD) / 16 per reference; one-hot cells of the three
 //     temporal neighbours' vectors (co-located, left and above in reference 0); a grid marker.
 //  3. Four 3x3 convolutions, 32 channels, ReLU between, BF16 operands and FP32 accumulation with
-//     v_mfma_f32_16x16x32_bf16 (implicit GEMM: M = 16 grid cells, N = 16 output channels, K = 32 =
+//     v_mfma_f32_16x16x32_bf16 (implicit GEMM: M = 16 output channels, N = 16 grid cells, K = 32 =
 //     one tap's 32 input channels; the first layer packs 4 taps of 8 channels per K step). Outputs:
 //     one score map per reference and a predicted log2 SAD per cell.
 //  4. The 8 best-scoring cells per reference (ties to the lower cell index), by wave-wide DPP maxima.
@@ -24,14 +25,14 @@ namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 
-constexpr int WAVES = 4;
+constexpr int WAVES = 8;
 constexpr int PG = CNN_GRID + 2;   // padded grid side (one cell of zeros around)
 constexpr int PCELLS = PG * PG;
 constexpr int AST = 40;            // bf16 per cell in an activation buffer (32 channels + 8 of padding)
-constexpr int WKS = 40;            // bf16 per output channel's weight row of one tap (32 inputs + 8 of padding)
 constexpr int WIN = 52;            // staged reference window side: +-18 displacements around the centre + 16
-constexpr int WDW = 16;            // dwords staged per window row
 constexpr int WST = 19;            // dwords per staged window row in LDS (padded against bank conflicts)
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
@@ -106,19 +107,21 @@ __device__ __forceinline__ uint32_t ford(float f) {
 }
 
 struct alignas(16) WaveLds {
-  uint16_t x0[PCELLS][8];          // input features, bf16, zero border
   union {
-    uint16_t act[2][PCELLS][AST];  // activations, bf16, zero border
+    uint16_t x0[PCELLS][8];        // input features, bf16, zero border
+    float out[CNN_CELLS][CNN_OUT]; // last layer (once the first has read the features)
+  };
+  union {
+    uint16_t act[PCELLS][AST];     // activations, bf16, zero border; each layer overwrites its input
     uint32_t win[3][WIN][WST];     // before and after the CNN: each reference's window, aligned dwords
   };
-  float out[CNN_CELLS][CNN_OUT];   // last layer
 };
 
 struct PropLds {
   uint16_t w0[12][CNN_CH][8];      // layer 0: K = 12 taps (9 real) x 8 channels
-  uint16_t w1[9][CNN_CH][WKS];
-  uint16_t w2[9][CNN_CH][WKS];
-  uint16_t w3[9][16][WKS];         // layer 3: 16 outputs (CNN_OUT real)
+  uint16_t w1[9][CNN_CH][CNN_CH];
+  uint16_t w2[9][CNN_CH][CNN_CH];
+  uint16_t w3[9][16][CNN_CH];      // layer 3: 16 outputs (CNN_OUT real)
   float b[CNN_LAYERS][CNN_CH];
   WaveLds wv[WAVES];
 };
@@ -129,9 +132,9 @@ __device__ __forceinline__ int tapoff(int t) { return (t / 3 - 1) * PG + (t % 3 
 // One 3x3 convolution layer on a wave's 9x9 grid. IN0: the first layer (8-channel input); NT: 16-wide
 // output tiles; RELU: activation. Reads `in` (stride `ist`), writes bf16 to `outb` or f32 to `outf`.
 template <bool IN0, int NT, bool RELU>
-__device__ void conv_layer(const uint16_t *in, int ist, const uint16_t *w, const float *bias, uint16_t *outb,
+__device__ void conv_layer(int lane, const uint16_t *in, int ist, const uint16_t *w, const float *bias, uint16_t *outb,
                            float *outf) {
-  const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
+  const int m = lane & 15, g = lane >> 4;
   constexpr int KSTEPS = IN0 ? 3 : 9;
   // the lane's B fragments of every K step and output tile, held in registers for the layer
   bf16x8 bw[KSTEPS][NT];
@@ -143,45 +146,58 @@ __device__ void conv_layer(const uint16_t *in, int ist, const uint16_t *w, const
         const int t = ks * 4 + g;  // this lane's tap; taps 9..11 are zero
         bw[ks][nt] = t < 9 ? *reinterpret_cast<const bf16x8 *>(w + ((size_t)t * CNN_CH + nt * 16 + m) * 8) : bf16x8{};
       } else {
-        bw[ks][nt] = *reinterpret_cast<const bf16x8 *>(w + ((size_t)ks * (NT * 16) + nt * 16 + m) * WKS + g * 8);
+        bw[ks][nt] = *reinterpret_cast<const bf16x8 *>(w + ((size_t)ks * (NT * 16) + nt * 16 + m) * CNN_CH + g * 8);
       }
     }
-  float bv[NT];
+  // the whole layer's output stays in registers until every lane has read the input, so the output
+  // may overwrite it. Operands: A = the weights (rows = output channels), B = the activations
+  // (columns = cells), so D gives each lane 4 consecutive output channels of one cell.
+  f32x4 acc[6][NT];
 #pragma unroll
-  for (int nt = 0; nt < NT; nt++) bv[nt] = bias[nt * 16 + m];
-#pragma unroll 2
   for (int mt = 0; mt < 6; mt++) {
-    const int p = min(mt * 16 + m, CNN_CELLS - 1);  // this lane's A row (grid cell)
+    const int p = min(mt * 16 + m, CNN_CELLS - 1);  // this lane's B column (grid cell)
     const int pp = padpos(p);
-    f32x4 acc[NT];
 #pragma unroll
-    for (int nt = 0; nt < NT; nt++) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int nt = 0; nt < NT; nt++) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int ks = 0; ks < KSTEPS; ks++) {
-      bf16x8 a;
+      bf16x8 x;
       if (IN0) {
         const int t = ks * 4 + g;
-        a = t < 9 ? *reinterpret_cast<const bf16x8 *>(in + (size_t)(pp + tapoff(t)) * ist) : bf16x8{};
+        x = t < 9 ? *reinterpret_cast<const bf16x8 *>(in + (size_t)(pp + tapoff(t)) * ist) : bf16x8{};
       } else {
-        a = *reinterpret_cast<const bf16x8 *>(in + (size_t)(pp + tapoff(ks)) * ist + g * 8);
+        x = *reinterpret_cast<const bf16x8 *>(in + (size_t)(pp + tapoff(ks)) * ist + g * 8);
       }
-      // one A fragment feeds every output tile
+      // one activation fragment feeds every output tile
 #pragma unroll
-      for (int nt = 0; nt < NT; nt++) acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[ks][nt], acc[nt], 0, 0, 0);
+      for (int nt = 0; nt < NT; nt++)
+        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[ks][nt], x, acc[mt][nt], 0, 0, 0);
     }
-    // D: this lane holds output channel nt*16 + m for cells mt*16 + 4g + i
+  }
+  wave_sync();
+  // D: this lane holds output channels nt*16 + 4g + i, i = 0..3, of cell mt*16 + m
 #pragma unroll
-    for (int nt = 0; nt < NT; nt++) {
-      const int co = nt * 16 + m;
+  for (int nt = 0; nt < NT; nt++) {
+    const int c0 = nt * 16 + 4 * g;
+    const f32x4 bv = *reinterpret_cast<const f32x4 *>(bias + c0);
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int q = mt * 16 + 4 * g + i;
-        if (q < CNN_CELLS) {
-          float v = acc[nt][i] + bv[nt];
-          if (RELU) v = fmaxf(v, 0.f);
-          if (outb) outb[(size_t)padpos(q) * AST + co] = f2bf(v);
-          if (outf && co < CNN_OUT) outf[q * CNN_OUT + co] = v;
+    for (int mt = 0; mt < 6; mt++) {
+      const int q = mt * 16 + m;
+      if (q < CNN_CELLS) {
+        f32x4 v = acc[mt][nt] + bv;
+        if (RELU) {
+#pragma unroll
+          for (int i = 0; i < 4; i++) v[i] = fmaxf(v[i], 0.f);
         }
+        if (outb) {
+          const bf16x2 lo = __builtin_convertvector(f32x2{v[0], v[1]}, bf16x2);
+          const bf16x2 hi = __builtin_convertvector(f32x2{v[2], v[3]}, bf16x2);
+          uint2 w2;
+          w2.x = __builtin_bit_cast(uint32_t, lo);
+          w2.y = __builtin_bit_cast(uint32_t, hi);
+          *reinterpret_cast<uint2 *>(outb + (size_t)padpos(q) * AST + c0) = w2;
+        }
+        if (outf && c0 < CNN_OUT) *reinterpret_cast<f32x4 *>(outf + q * CNN_OUT + c0) = v;
       }
     }
   }
@@ -189,13 +205,16 @@ __device__ void conv_layer(const uint16_t *in, int ist, const uint16_t *w, const
 
 // stage reference r's window (WIN x WIN samples from plane row wy, column wx) as aligned dwords;
 // returns the byte offset of the window's first sample in its first dword
-__device__ __forceinline__ int stage_window(const uint8_t *g, int sw, int wx, int wy, uint32_t (*win)[WST]) {
-  const int lane = threadIdx.x & 63, al = wx & ~3;
-  const uint8_t *base = g + (size_t)wy * sw + al;
+__device__ __forceinline__ int stage_window(int lane, const uint8_t *g, int sw, int wx, int wy, uint32_t (*win)[WST]) {
+  const int al = wx & ~3;
+  // 16 dwords per row (64 samples from the aligned start cover the 52 + 3 needed); lane takes dword
+  // lane % 16 of rows lane / 16 + 4k
+  const uint8_t *p = g + (size_t)(wy + (lane >> 4)) * sw + al + 4 * (lane & 15);
+  uint32_t *q = &win[lane >> 4][lane & 15];
 #pragma unroll
-  for (int k = 0; k < WIN * WDW / 64; k++) {
-    const int i = lane + k * 64, row = i / WDW, d = i % WDW;
-    win[row][d] = ld_u32(base + (size_t)row * sw + 4 * d);
+  for (int k = 0; k < WIN / 4; k++) {
+    q[k * 4 * WST] = ld_u32(p);
+    p += 4 * sw;
   }
   return wx - al;
 }
@@ -217,7 +236,7 @@ __device__ __forceinline__ void win_row16(const uint32_t (*win)[WST], int off, i
 __global__ __launch_bounds__(64 * WAVES) void proposal_kernel(PropArgs a) {
   extern __shared__ uint8_t lds_raw[];
   PropLds &L = *reinterpret_cast<PropLds *>(lds_raw);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, wave = tid >> 6;
   {
     const CnnWeights &W = *a.wts;
     // layer 0: taps 9..11 zero; layers 1, 2 copied as 16-byte vectors; layer 3: the first 16 outputs
@@ -227,18 +246,13 @@ __global__ __launch_bounds__(64 * WAVES) void proposal_kernel(PropArgs a) {
       if (t < 9) v = *reinterpret_cast<const uint4 *>(&W.w[0][t][co][0]);
       *reinterpret_cast<uint4 *>(&L.w0[t][co][0]) = v;
     }
-    // rows of 32 bf16 (one output channel of one tap) into rows of WKS
-    for (int i = tid; i < 9 * CNN_CH * 4; i += 64 * WAVES) {
-      const int row = i >> 2, part = i & 3;
-      *reinterpret_cast<uint4 *>(&L.w1[0][0][0] + row * WKS + part * 8) =
-          reinterpret_cast<const uint4 *>(&W.w[1][0][0][0])[i];
-      *reinterpret_cast<uint4 *>(&L.w2[0][0][0] + row * WKS + part * 8) =
-          reinterpret_cast<const uint4 *>(&W.w[2][0][0][0])[i];
+    for (int i = tid; i < 9 * CNN_CH * CNN_CH / 8; i += 64 * WAVES) {
+      reinterpret_cast<uint4 *>(&L.w1[0][0][0])[i] = reinterpret_cast<const uint4 *>(&W.w[1][0][0][0])[i];
+      reinterpret_cast<uint4 *>(&L.w2[0][0][0])[i] = reinterpret_cast<const uint4 *>(&W.w[2][0][0][0])[i];
     }
-    for (int i = tid; i < 9 * 16 * 4; i += 64 * WAVES) {
-      const int t = i / 64, row = (i >> 2) & 15, part = i & 3;
-      *reinterpret_cast<uint4 *>(&L.w3[t][row][part * 8]) =
-          reinterpret_cast<const uint4 *>(&W.w[3][t][row][0])[part];
+    for (int i = tid; i < 9 * 16 * CNN_CH / 8; i += 64 * WAVES) {
+      const int t = i / (16 * CNN_CH / 8), r = i % (16 * CNN_CH / 8);
+      reinterpret_cast<uint4 *>(&L.w3[t][0][0])[r] = reinterpret_cast<const uint4 *>(&W.w[3][t][0][0])[r];
     }
     for (int i = tid; i < CNN_LAYERS * CNN_CH; i += 64 * WAVES) (&L.b[0][0])[i] = (&W.b[0][0])[i];
   }
@@ -247,6 +261,11 @@ __global__ __launch_bounds__(64 * WAVES) void proposal_kernel(PropArgs a) {
   const int nmb = a.mb_w * a.mb_h;
   const int pw = a.mb_w * 16, ph = a.mb_h * 16, sw = pw + 2 * PAD;
   for (int mb = blockIdx.x * WAVES + wave; mb < nmb; mb += gridDim.x * WAVES) {
+    // the lane index afresh in each iteration (an empty asm the compiler cannot see through), so
+    // that lane-derived addresses are recomputed here rather than hoisted out of the loop and held
+    // in registers for its whole length
+    int lane = tid & 63;
+    asm volatile("" : "+v"(lane));
     const int mbx = mb % a.mb_w, mby = mb / a.mb_w;
     const int x0 = mbx * 16, y0 = mby * 16;
     const int lox = -(x0 + 24), hix = pw - x0 + 8, loy = -(y0 + 24), hiy = ph - y0 + 8;
@@ -289,7 +308,7 @@ __global__ __launch_bounds__(64 * WAVES) void proposal_kernel(PropArgs a) {
     const int nref = a.feat_in ? 0 : a.nref;
     int woff[3] = {0, 0, 0};
     for (int r = 0; r < nref; r++)
-      woff[r] = stage_window(a.ref_g[r], sw, x0 + cx[r] - 18 + PAD, y0 + cy[r] - 18 + PAD, V.win[r]);
+      woff[r] = stage_window(lane, a.ref_g[r], sw, x0 + cx[r] - 18 + PAD, y0 + cy[r] - 18 + PAD, V.win[r]);
     for (int i = lane; i < PCELLS * 8 / 8; i += 64) reinterpret_cast<uint4 *>(&V.x0[0][0])[i] = make_uint4(0, 0, 0, 0);
     wave_sync();
 
@@ -338,22 +357,22 @@ __global__ __launch_bounds__(64 * WAVES) void proposal_kernel(PropArgs a) {
     if (a.features_only) continue;
 
     // the CNN (the windows' space now holds the activations, borders zero)
-    for (int i = lane; i < 2 * PCELLS * AST / 8; i += 64) reinterpret_cast<uint4 *>(&V.act[0][0][0])[i] = make_uint4(0, 0, 0, 0);
+    for (int i = lane; i < PCELLS * AST / 8; i += 64) reinterpret_cast<uint4 *>(&V.act[0][0])[i] = make_uint4(0, 0, 0, 0);
     wave_sync();
-    conv_layer<true, 2, true>(&V.x0[0][0], 8, &L.w0[0][0][0], L.b[0], &V.act[0][0][0], nullptr);
+    conv_layer<true, 2, true>(lane, &V.x0[0][0], 8, &L.w0[0][0][0], L.b[0], &V.act[0][0], nullptr);
     wave_sync();
-    conv_layer<false, 2, true>(&V.act[0][0][0], AST, &L.w1[0][0][0], L.b[1], &V.act[1][0][0], nullptr);
+    conv_layer<false, 2, true>(lane, &V.act[0][0], AST, &L.w1[0][0][0], L.b[1], &V.act[0][0], nullptr);
     wave_sync();
-    conv_layer<false, 2, true>(&V.act[1][0][0], AST, &L.w2[0][0][0], L.b[2], &V.act[0][0][0], nullptr);
+    conv_layer<false, 2, true>(lane, &V.act[0][0], AST, &L.w2[0][0][0], L.b[2], &V.act[0][0], nullptr);
     wave_sync();
-    conv_layer<false, 1, false>(&V.act[0][0][0], AST, &L.w3[0][0][0], L.b[3], nullptr, &V.out[0][0]);
+    conv_layer<false, 1, false>(lane, &V.act[0][0], AST, &L.w3[0][0][0], L.b[3], nullptr, &V.out[0][0]);
     wave_sync();
     if (a.cnn_out)
       for (int i = lane; i < CNN_CELLS * CNN_OUT; i += 64) a.cnn_out[(size_t)mb * CNN_CELLS * CNN_OUT + i] = (&V.out[0][0])[i];
     if (a.feat_in) continue;
 
     // the windows again, for the refinement (the activations are done with)
-    for (int r = 0; r < nref; r++) stage_window(a.ref_g[r], sw, x0 + cx[r] - 18 + PAD, y0 + cy[r] - 18 + PAD, V.win[r]);
+    for (int r = 0; r < nref; r++) stage_window(lane, a.ref_g[r], sw, x0 + cx[r] - 18 + PAD, y0 + cy[r] - 18 + PAD, V.win[r]);
     wave_sync();
 
     for (int r = 0; r < nref; r++) {
@@ -460,7 +479,7 @@ __global__ __launch_bounds__(64 * WAVES) void proposal_kernel(PropArgs a) {
 hipError_t launch_proposal(const PropArgs &a, hipStream_t stream) {
   const int nmb = a.mb_w * a.mb_h;
   if (a.nref < 1 || a.nref > 3) return hipErrorInvalidValue;
-  // persistent: one workgroup per CU (its LDS holds the weights), waves walk the macroblocks
+  // persistent: one workgroup per CU (its LDS holds the weights), two waves per SIMD walk the macroblocks
   dim3 grid(min((nmb + WAVES - 1) / WAVES, 256));
   const size_t shmem = sizeof(PropLds);
   static_assert(sizeof(PropLds) <= 160 * 1024, "the weights and four waves' buffers fit one CU's LDS");
