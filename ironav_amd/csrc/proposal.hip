@@ -15,7 +15,8 @@
 //  4. The 8 best-scoring cells per reference (ties to the lower cell index), by wave-wide DPP maxima.
 //  5. Exact refinement: every integer displacement within +-2 samples of each selected cell
 //     (200 per reference), all 41 partitions' SADs with the keyed minima of the exhaustive search
-//     (key = SAD << 16 + blocks * order, order = cell rank * 25 + raster offset).
+//     (key = SAD << 16 + blocks * order, order = cell rank * 25 + raster offset); the per-lane minima
+//     are reduced through a transpose in LDS.
 //  6. If the refined 16x16 SAD exceeds fallback_ratio x 2^(predicted log2 SAD at the best cell), the
 //     (macroblock, reference) is queued for the exhaustive search, which overwrites its result.
 #include "proposal.h"
@@ -50,23 +51,10 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 template <int CTRL, int ROW_MASK, int BANK_MASK>
-__device__ __forceinline__ uint32_t dpp_min(uint32_t v) {
-  return min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)0xffffffff, (int)v, CTRL, ROW_MASK, BANK_MASK, false));
-}
-template <int CTRL, int ROW_MASK, int BANK_MASK>
 __device__ __forceinline__ uint32_t dpp_max(uint32_t v) {
   return max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, BANK_MASK, false));
 }
-// wave-wide minimum / maximum (the row-shift sequence suits idempotent operations)
-__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
-  v = dpp_min<0x111, 0xf, 0xf>(v);
-  v = dpp_min<0x112, 0xf, 0xf>(v);
-  v = dpp_min<0x114, 0xf, 0xf>(v);
-  v = dpp_min<0x118, 0xf, 0xf>(v);
-  v = dpp_min<0x142, 0xa, 0xf>(v);
-  v = dpp_min<0x143, 0xc, 0xf>(v);
-  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
-}
+// wave-wide maximum (the row-shift sequence suits idempotent operations)
 __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
   v = dpp_max<0x111, 0xf, 0xf>(v);
   v = dpp_max<0x112, 0xf, 0xf>(v);
@@ -108,8 +96,11 @@ __device__ __forceinline__ uint32_t ford(float f) {
 
 struct alignas(16) WaveLds {
   union {
-    uint16_t x0[PCELLS][8];        // input features, bf16, zero border
-    float out[CNN_CELLS][CNN_OUT]; // last layer (once the first has read the features)
+    uint16_t x0[PCELLS][8];          // input features, bf16, zero border
+    struct {                         // once the first layer has read the features:
+      float out[CNN_CELLS][CNN_OUT]; // the last layer
+      int cand[3][CNN_TOPK];         // the selected cells per reference
+    };
   };
   union {
     uint16_t act[PCELLS][AST];     // activations, bf16, zero border; each layer overwrites its input
@@ -205,18 +196,25 @@ __device__ void conv_layer(int lane, const uint16_t *in, int ist, const uint16_t
 
 // stage reference r's window (WIN x WIN samples from plane row wy, column wx) as aligned dwords;
 // returns the byte offset of the window's first sample in its first dword
-__device__ __forceinline__ int stage_window(int lane, const uint8_t *g, int sw, int wx, int wy, uint32_t (*win)[WST]) {
+// Load reference r's window (WIN x WIN samples from plane row wy, column wx) as aligned dwords into
+// 13 registers per lane: 16 dwords per row (64 samples from the aligned start cover the 52 + 3
+// needed), lane taking dword lane % 16 of rows lane / 16 + 4k. Returns the byte offset of the
+// window's first sample in its first dword.
+__device__ __forceinline__ int load_window(int lane, const uint8_t *g, int sw, int wx, int wy, uint32_t *w) {
   const int al = wx & ~3;
-  // 16 dwords per row (64 samples from the aligned start cover the 52 + 3 needed); lane takes dword
-  // lane % 16 of rows lane / 16 + 4k
   const uint8_t *p = g + (size_t)(wy + (lane >> 4)) * sw + al + 4 * (lane & 15);
-  uint32_t *q = &win[lane >> 4][lane & 15];
 #pragma unroll
   for (int k = 0; k < WIN / 4; k++) {
-    q[k * 4 * WST] = ld_u32(p);
+    w[k] = ld_u32(p);
     p += 4 * sw;
   }
   return wx - al;
+}
+
+__device__ __forceinline__ void put_window(int lane, const uint32_t *w, uint32_t (*win)[WST]) {
+  uint32_t *q = &win[lane >> 4][lane & 15];
+#pragma unroll
+  for (int k = 0; k < WIN / 4; k++) q[k * 4 * WST] = w[k];
 }
 
 // the 4 dwords of 16 window samples at window column x of row y
@@ -270,26 +268,26 @@ __global__ __launch_bounds__(64 * WAVES) void proposal_kernel(PropArgs a) {
     const int x0 = mbx * 16, y0 = mby * 16;
     const int lox = -(x0 + 24), hix = pw - x0 + 8, loy = -(y0 + 24), hiy = ph - y0 + 8;
 
-    // the macroblock's samples, uniform
-    uint32_t src[64];
-    {
-      const uint8_t *sp = a.src_y + (size_t)y0 * pw + x0;
-#pragma unroll
-      for (int i = 0; i < 64; i++) src[i] = __builtin_amdgcn_readfirstlane(ld_u32(sp + (size_t)(i >> 2) * pw + (i & 3) * 4));
+    // the macroblock's samples (lane i loads dword i: row i / 4, columns 4 (i % 4)..+3) and reference
+    // 0's records of this macroblock, the left and the above one (lane k < 3), all in flight at once
+    const uint32_t sv = ld_u32(a.src_y + (size_t)(y0 + (lane >> 2)) * pw + x0 + (lane & 3) * 4);
+    int tkind = 0, tmvw = 0;
+    if (a.rmbs0 && lane < 3) {
+      const int nx = lane == 1 ? mbx - 1 : mbx, ny = lane == 2 ? mby - 1 : mby;
+      if (nx >= 0 && ny >= 0) {
+        const ia_mbinfo *m = a.rmbs0 + ny * a.mb_w + nx;
+        tkind = m->kind;
+        tmvw = *reinterpret_cast<const int *>(&m->mv[5][0]);
+      }
     }
-    // temporal predictors: reference 0's vectors (quarter samples) at this macroblock, left, above
-    int tmv[3][2] = {{0, 0}, {0, 0}, {0, 0}}, tok[3] = {0, 0, 0};
-    if (a.rmbs0) {
-      const int nx[3] = {mbx, mbx - 1, mbx}, ny[3] = {mby, mby, mby - 1};
-      for (int k = 0; k < 3; k++)
-        if (nx[k] >= 0 && ny[k] >= 0) {
-          const ia_mbinfo &m = a.rmbs0[ny[k] * a.mb_w + nx[k]];
-          if (m.kind == 5) {
-            tmv[k][0] = m.mv[5][0];
-            tmv[k][1] = m.mv[5][1];
-            tok[k] = 1;
-          }
-        }
+    // temporal predictors: those records' vectors (quarter samples) when inter coded
+    int tmv[3][2], tok[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      tok[k] = __builtin_amdgcn_readlane(tkind, k) == 5;
+      const int w = tok[k] ? __builtin_amdgcn_readlane(tmvw, k) : 0;
+      tmv[k][0] = (int16_t)(w & 0xffff);
+      tmv[k][1] = (int16_t)((uint32_t)w >> 16);
     }
     // grid centres: the co-located vector scaled by the reference's distance, to a multiple of 4
     // samples, clamped so the grid lies within the macroblock's vectors
@@ -306,9 +304,19 @@ __global__ __launch_bounds__(64 * WAVES) void proposal_kernel(PropArgs a) {
         a.centre[(mb * 3 + r) * 2 + 1] = (int16_t)cy[r];
       }
     const int nref = a.feat_in ? 0 : a.nref;
+    // the reference windows, held in registers until the refinement (the CNN reuses their LDS)
+    uint32_t wreg[3][WIN / 4];
     int woff[3] = {0, 0, 0};
-    for (int r = 0; r < nref; r++)
-      woff[r] = stage_window(lane, a.ref_g[r], sw, x0 + cx[r] - 18 + PAD, y0 + cy[r] - 18 + PAD, V.win[r]);
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+      if (r < nref) woff[r] = load_window(lane, a.ref_g[r], sw, x0 + cx[r] - 18 + PAD, y0 + cy[r] - 18 + PAD, wreg[r]);
+    // the samples as scalars
+    uint32_t src[64];
+#pragma unroll
+    for (int i = 0; i < 64; i++) src[i] = __builtin_amdgcn_readlane(sv, i);
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+      if (r < nref) put_window(lane, wreg[r], V.win[r]);
     for (int i = lane; i < PCELLS * 8 / 8; i += 64) reinterpret_cast<uint4 *>(&V.x0[0][0])[i] = make_uint4(0, 0, 0, 0);
     wave_sync();
 
@@ -372,38 +380,56 @@ __global__ __launch_bounds__(64 * WAVES) void proposal_kernel(PropArgs a) {
     if (a.feat_in) continue;
 
     // the windows again, for the refinement (the activations are done with)
-    for (int r = 0; r < nref; r++) stage_window(lane, a.ref_g[r], sw, x0 + cx[r] - 18 + PAD, y0 + cy[r] - 18 + PAD, V.win[r]);
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+      if (r < nref) put_window(lane, wreg[r], V.win[r]);
     wave_sync();
 
-    for (int r = 0; r < nref; r++) {
-      // the 8 best cells of the reference's score map (ties to the lower cell index), uniform
-      int cand[CNN_TOPK];
-      {
-        uint32_t k0 = (ford(V.out[lane][r]) & ~0x7fu) | (uint32_t)(127 - lane);
-        uint32_t k1 = lane + 64 < CNN_CELLS ? (ford(V.out[lane + 64][r]) & ~0x7fu) | (uint32_t)(127 - lane - 64) : 0u;
+    // the 8 best cells of each reference's score map (ties to the lower cell index), the three
+    // references' selections interleaved; lane k < 8 stores the k-th
+    {
+      uint32_t k0[3], k1[3];
+      int mine[3];
 #pragma unroll
-        for (int k = 0; k < CNN_TOPK; k++) {
-          const uint32_t best = wave_max(max(k0, k1));
-          cand[k] = 127 - (int)(best & 0x7f);
-          if (cand[k] == lane) k0 = 0;
-          if (cand[k] == lane + 64) k1 = 0;
-        }
+      for (int r = 0; r < 3; r++) {
+        k0[r] = (ford(V.out[lane][r]) & ~0x7fu) | (uint32_t)(127 - lane);
+        k1[r] = lane + 64 < CNN_CELLS ? (ford(V.out[lane + 64][r]) & ~0x7fu) | (uint32_t)(127 - lane - 64) : 0u;
+        mine[r] = 0;
       }
-      // exact refinement of all 41 partitions around the selected cells, +-2 samples: lane 5k + x of
-      // lanes 0..39 takes cell rank k, column x of its 5x5 block, and walks the column's 5
-      // displacements as a strip (each staged row read once feeds every candidate that uses it)
-      uint32_t run[IA_NPART];
 #pragma unroll
-      for (int p = 0; p < IA_NPART; p++) run[p] = 0xffffffffu;
-      if (lane < 5 * CNN_TOPK) {
-        const int k = lane / 5, x = lane - 5 * k;
-        int cell = cand[0];
+      for (int k = 0; k < CNN_TOPK; k++)
 #pragma unroll
-        for (int q = 1; q < CNN_TOPK; q++)
-          if (q == k) cell = cand[q];
+        for (int r = 0; r < 3; r++)
+          if (r < nref) {
+            const int cell = 127 - (int)(wave_max(max(k0[r], k1[r])) & 0x7f);
+            if (cell == lane) k0[r] = 0;
+            if (cell == lane + 64) k1[r] = 0;
+            if (lane == k) mine[r] = cell;
+          }
+      wave_sync();
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+        if (r < nref && lane < CNN_TOPK) V.cand[r][lane] = mine[r];
+      wave_sync();
+    }
+
+    // exact refinement of all 41 partitions around the selected cells, +-2 samples. Lanes 21r..21r+20
+    // take reference r; its 40 strips (cell rank k, column x of the cell's 5x5 block) go two per
+    // lane, and a strip walks its column's 5 displacements (each staged row read once feeds every
+    // candidate that uses it). Lane 63 idles.
+    uint32_t run[IA_NPART];
+#pragma unroll
+    for (int p = 0; p < IA_NPART; p++) run[p] = 0xffffffffu;
+    const int rg = lane / 21;
+    if (rg < nref) {
+      const int off = rg == 2 ? woff[2] : rg == 1 ? woff[1] : woff[0];
+      const int crx = rg == 2 ? cx[2] : rg == 1 ? cx[1] : cx[0];
+      const int cry = rg == 2 ? cy[2] : rg == 1 ? cy[1] : cy[0];
+      for (int strip = lane - 21 * rg; strip < 5 * CNN_TOPK; strip += 21) {
+        const int k = strip / 5, x = strip - 5 * k;
+        const int cell = V.cand[rg][k];
         const int ox = 4 * (cell % CNN_GRID) + x, oy = 4 * (cell / CNN_GRID);  // window offset of the strip top
-        const int dx = cx[r] - 18 + ox, dy0 = cy[r] - 18 + oy;
-        const int off = r == 2 ? woff[2] : r == 1 ? woff[1] : woff[0];
+        const int dx = crx - 18 + ox, dy0 = cry - 18 + oy;
         uint32_t acc[5][16];
 #pragma unroll
         for (int i = 0; i < 5; i++)
@@ -412,7 +438,7 @@ __global__ __launch_bounds__(64 * WAVES) void proposal_kernel(PropArgs a) {
 #pragma unroll
         for (int y = 0; y < 20; y++) {
           uint32_t r0, r1, r2, r3;
-          win_row16(V.win[r], off, ox, oy + y, r0, r1, r2, r3);
+          win_row16(V.win[rg], off, ox, oy + y, r0, r1, r2, r3);
 #pragma unroll
           for (int i = 0; i < 5; i++) {
             const int row = y - i;
@@ -436,38 +462,41 @@ __global__ __launch_bounds__(64 * WAVES) void proposal_kernel(PropArgs a) {
           }
         }
       }
-      // lane p takes partition p's minimum, then lanes 0..40 decode and store their winners at once
-      uint32_t mine = 0xffffffffu;
+    }
+    // the minima over each reference's lanes by a transpose through LDS (over the windows, now read):
+    // T[lane][p] = run[p]; output o = 41 r + p takes the minimum of column p over rows 21r..21r+20
+    wave_sync();
+    uint32_t *T = &V.win[0][0][0];
 #pragma unroll
-      for (int p = 0; p < IA_NPART; p++) {
-        const uint32_t v = wave_min(run[p]);
-        mine = lane == p ? v : mine;
+    for (int p = 0; p < IA_NPART; p++) T[lane * IA_NPART + p] = run[p];
+    wave_sync();
+    for (int o = lane; o < nref * IA_NPART; o += 64) {
+      const int r = o >= 2 * IA_NPART ? 2 : o >= IA_NPART ? 1 : 0, p = o - IA_NPART * r;
+      uint32_t v = 0xffffffffu;
+#pragma unroll
+      for (int l = 0; l < 21; l++) v = min(v, T[(21 * r + l) * IA_NPART + p]);
+      const int lg = p == 0 ? 4 : p < 5 ? 3 : p < 9 ? 2 : p < 25 ? 1 : 0;  // log2 of the 4x4 blocks summed
+      ia_fsresult res;
+      if (v == 0xffffffffu) {
+        res.mv[0] = res.mv[1] = 0;
+        res.sad = 0xffffffffu;
+      } else {
+        const int idx = (int)((v & 0xffffu) >> lg), k = idx / 25, j = idx % 25;
+        const int cell = V.cand[r][k];
+        const int crx = r == 2 ? cx[2] : r == 1 ? cx[1] : cx[0];
+        const int cry = r == 2 ? cy[2] : r == 1 ? cy[1] : cy[0];
+        res.mv[0] = (int16_t)((crx + 4 * (cell % CNN_GRID - 4) + (j % 5 - 2)) * 4);
+        res.mv[1] = (int16_t)((cry + 4 * (cell / CNN_GRID - 4) + (j / 5 - 2)) * 4);
+        res.sad = v >> 16;
       }
-      const uint32_t best16 = (uint32_t)__builtin_amdgcn_readlane((int)mine, 0);
-      if (lane < IA_NPART) {
-        const int lg = lane == 0 ? 4 : lane < 5 ? 3 : lane < 9 ? 2 : lane < 25 ? 1 : 0;
-        ia_fsresult res;
-        if (mine == 0xffffffffu) {
-          res.mv[0] = res.mv[1] = 0;
-          res.sad = 0xffffffffu;
-        } else {
-          const int idx = (int)((mine & 0xffffu) >> lg), k = idx / 25, j = idx % 25;
-          int cell = cand[0];
-#pragma unroll
-          for (int q = 1; q < CNN_TOPK; q++)
-            if (q == k) cell = cand[q];
-          res.mv[0] = (int16_t)((cx[r] + 4 * (cell % CNN_GRID - 4) + (j % 5 - 2)) * 4);
-          res.mv[1] = (int16_t)((cy[r] + 4 * (cell / CNN_GRID - 4) + (j / 5 - 2)) * 4);
-          res.sad = mine >> 16;
+      a.out[((size_t)r * nmb + mb) * IA_NPART + p] = res;
+      // fallback when the refined 16x16 cost exceeds the ratio times the predicted one
+      if (p == 0 && a.fb_list) {
+        const float pred = exp2f(V.out[V.cand[r][0]][3]);
+        if (res.sad == 0xffffffffu || (float)res.sad > a.fallback_ratio * pred) {
+          const int slot = atomicAdd(a.fb_count, 1);
+          a.fb_list[slot] = mb * 4 + r;
         }
-        a.out[((size_t)r * nmb + mb) * IA_NPART + lane] = res;
-      }
-      // fallback when the refined cost exceeds the ratio times the predicted one
-      const uint32_t sad16 = best16 == 0xffffffffu ? best16 : best16 >> 16;
-      const float pred = exp2f(V.out[cand[0]][3]);
-      if (lane == 0 && a.fb_list && (sad16 == 0xffffffffu || (float)sad16 > a.fallback_ratio * pred)) {
-        const int slot = atomicAdd(a.fb_count, 1);
-        a.fb_list[slot] = mb * 4 + r;
       }
     }
     wave_sync();  // the next macroblock restages the windows and features
@@ -482,6 +511,7 @@ hipError_t launch_proposal(const PropArgs &a, hipStream_t stream) {
   // persistent: one workgroup per CU (its LDS holds the weights), two waves per SIMD walk the macroblocks
   dim3 grid(min((nmb + WAVES - 1) / WAVES, 256));
   const size_t shmem = sizeof(PropLds);
+  static_assert(sizeof(WaveLds::win) >= 64 * IA_NPART * sizeof(uint32_t), "the transpose fits over the windows");
   static_assert(sizeof(PropLds) <= 160 * 1024, "the weights and four waves' buffers fit one CU's LDS");
   hipLaunchKernelGGL(proposal_kernel, grid, dim3(64 * WAVES), shmem, stream, a);
   return hipGetLastError();
