@@ -13,10 +13,15 @@ learned path's macroblocks per second; the full search's is reported beside it, 
 (SAD of the chosen vectors, and the encoder's own costs after the exact analysis with either result
 as its extra candidate) and the fallback rate.
 
-The roofline object is that of the exhaustive-search kernel (integer VALU), and learned_roofline that
-of the proposal kernel (BF16 MFMA for the CNN); both timed with HIP events on the stream they are
-launched on. cpu_baseline is the oracle's scalar exhaustive search of the same picture against one
-reference on one host core, scaled to three references.
+The roofline object is that of the learned step's dominant kernel, proposal_kernel, timed with HIP
+events on the stream it is launched on (fallback off, so the launch holds the proposal alone): its
+exact SADs (the 9x9 cost volume and the +-2 refinement of 8 cells, per reference) are integer VALU
+work against the packed-u8 SAD issue peak, and its CNN is BF16 MFMA work, reported beside it, with
+the fraction of the launch that both would take at their peaks. full_search.roofline is the
+exhaustive-search kernel's. traffic is the HBM bytes per launch that rocprofv3's FETCH_SIZE and
+WRITE_SIZE counters read (scaled by what they read for a copy of known size, --copy-calibration).
+cpu_baseline is the oracle's scalar exhaustive search of the same picture against one reference on
+one host core, scaled to three references.
 """
 import argparse
 import json
@@ -32,6 +37,14 @@ MI355X_CU, MI355X_CLK_GHZ = 256, 2.4
 VALU_ABSDIFF_PEAK_T = MI355X_CU * 64 * 4 * MI355X_CLK_GHZ / 1000.0   # 157.3 T abs-diff/s
 MFMA_BF16_PEAK_T = 2500.0                                            # dense bf16 TFLOP/s
 CNN_FLOP_PER_MB = 2 * 81 * 9 * (8 * 32 + 32 * 32 + 32 * 32 + 32 * 4)  # multiply-adds x 2 of the 4 layers
+# exact SADs of the proposal per macroblock and reference: the 81-cell cost volume and 8 cells x 25
+# displacements of refinement, 256 absolute differences each
+PROP_ABSDIFF_PER_MB_REF = (81 + 8 * 25) * 256
+# HBM bytes per launch at this workload from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per
+# dispatch), each scaled by the counters' reading for a 256 MiB device copy (profiles/
+# traffic_1080p_3ref_pmc.md); None until measured
+TRAFFIC_PROPOSAL = None
+TRAFFIC_FULLSEARCH = None
 
 
 def window_positions(mb_w, mb_h, rng):
@@ -79,6 +92,8 @@ def main():
     ap.add_argument("--range", type=int, default=32)
     ap.add_argument("--variant", type=int, default=-1, help="full-search tile shape (-1: default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 256 MiB device copy (the counters' scale for the traffic figures)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (streams or slice bands would "
@@ -101,7 +116,7 @@ def main():
     common = dict(nrefs=nref, slices=68, search_range=args.range)
     full = Analyser(width, height, search_mode=abi.SEARCH_FULL, **common)
     learned = Analyser(width, height, search_mode=abi.SEARCH_LEARNED, fallback_ratio=1.1, **common)
-    nofb = Analyser(width, height, search_mode=abi.SEARCH_LEARNED, fallback_ratio=1e30, **common)
+    nofb = Analyser(width, height, search_mode=abi.SEARCH_LEARNED, fallback_ratio=float("inf"), **common)
     for an in (full, learned, nofb):
         an.set_stream(stream.cuda_stream)
     full.set_fullsearch_variant(args.variant)
@@ -124,6 +139,13 @@ def main():
     mb_w, mb_h = full.mb_w, full.mb_h
     mbs = mb_w * mb_h
 
+    if args.copy_calibration:
+        with torch.cuda.stream(stream):
+            a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
+            b = torch.empty_like(a)
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     ms_full, wall_full = timed(stream, lambda: full.motion_search(slots), args.steps, args.warmup)
     ms_learn, wall_learn = timed(stream, lambda: learned.motion_search(slots), args.steps, args.warmup)
     ms_prop, _ = timed(stream, lambda: nofb.motion_search(slots), args.steps, args.warmup)
@@ -152,7 +174,12 @@ def main():
     value_full = mbs / (max(ms_full, wall_full) / 1000.0)
     absdiff = window_positions(mb_w, mb_h, args.range) * 256 * nref
     achieved = absdiff / (ms_full / 1000.0) / 1e12
-    cnn_t = CNN_FLOP_PER_MB * mbs / (ms_prop / 1000.0) / 1e12
+    prop_absdiff = PROP_ABSDIFF_PER_MB_REF * mbs * nref
+    prop_t = prop_absdiff / (ms_prop / 1000.0) / 1e12
+    cnn_flop = CNN_FLOP_PER_MB * mbs
+    cnn_t = cnn_flop / (ms_prop / 1000.0) / 1e12
+    # the time both kinds of work would take at their peaks, over the launch's time
+    at_peak_ms = (prop_absdiff / (VALU_ABSDIFF_PEAK_T * 1e12) + cnn_flop / (MFMA_BF16_PEAK_T * 1e12)) * 1000.0
 
     cpu = None
     if not args.no_cpu:
@@ -185,20 +212,25 @@ def main():
                                "for 41 partitions, fallback to +-32 full search at 1.1x predicted cost",
                    "macroblocks_per_step": mbs, "references": nref, "search_range": args.range},
         "full_search": {"MB_per_s": round(value_full, 1), "ms_per_picture": round(max(ms_full, wall_full), 4),
-                        "kernel_variant": fullsearch_variants()[args.variant] if args.variant >= 0 else "default"},
+                        "kernel_variant": fullsearch_variants()[args.variant] if args.variant >= 0 else "default",
+                        "roofline": {"bound": "valu", "achieved": round(achieved, 2),
+                                     "peak": round(VALU_ABSDIFF_PEAK_T, 1), "unit": "Tabsdiff/s",
+                                     "frac": round(achieved / VALU_ABSDIFF_PEAK_T, 4), "traffic": TRAFFIC_FULLSEARCH,
+                                     "kernel": "fullsearch_kernel", "ms_per_launch": round(ms_full, 4),
+                                     "work_per_launch": absdiff}},
         "speedup_vs_full_search": round(value / value_full, 3),
         "cost_gap": {"sad16_per_reference": [round(g, 5) for g in gap_ref], "sad16_best_reference": round(gap_best, 5),
                      "sad_partitions_16x16_to_8x8": round(gap_parts, 5), "encoder_inter_cost": round(gap_inter, 5),
                      "encoder_rd_cost": round(gap_rd, 5)},
         "fallback_rate": round(nfb / (mbs * nref), 4),
-        "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": round(VALU_ABSDIFF_PEAK_T, 1),
-                     "unit": "Tabsdiff/s", "frac": round(achieved / VALU_ABSDIFF_PEAK_T, 4), "traffic": None,
-                     "kernel": "fullsearch_kernel", "ms_per_launch": round(ms_full, 4), "work_per_launch": absdiff},
-        "learned_roofline": {"bound": "mfma", "achieved": round(cnn_t, 2), "peak": MFMA_BF16_PEAK_T, "unit": "TFLOP/s",
-                             "frac": round(cnn_t / MFMA_BF16_PEAK_T, 5), "kernel": "proposal_kernel",
-                             "ms_per_launch": round(ms_prop, 4), "cnn_flop_per_mb": CNN_FLOP_PER_MB,
-                             "note": "CNN FLOPs over the whole proposal step (fallback off), which also computes "
-                                     "the cost volume and the refinement SADs"},
+        "roofline": {"bound": "valu", "achieved": round(prop_t, 2), "peak": round(VALU_ABSDIFF_PEAK_T, 1),
+                     "unit": "Tabsdiff/s", "frac": round(prop_t / VALU_ABSDIFF_PEAK_T, 4), "traffic": TRAFFIC_PROPOSAL,
+                     "kernel": "proposal_kernel", "ms_per_launch": round(ms_prop, 4), "work_per_launch": prop_absdiff,
+                     "mfma": {"achieved": round(cnn_t, 2), "peak": MFMA_BF16_PEAK_T, "unit": "TFLOP/s",
+                              "frac": round(cnn_t / MFMA_BF16_PEAK_T, 4), "flop_per_launch": cnn_flop},
+                     "frac_both_at_peak": round(at_peak_ms / ms_prop, 4),
+                     "note": "launch with fallback off: cost volume, CNN, top-8 and refinement of every "
+                             "(macroblock, reference)"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

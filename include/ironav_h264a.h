@@ -63,7 +63,8 @@ typedef struct ia_config {
   int32_t slices;          /* slice count, 1..mb_h (fixed: it changes the decisions) */
   int32_t search_mode;     /* IA_SEARCH_* */
   int32_t search_range;    /* integer-sample range of the full search, 1..64 */
-  float fallback_ratio;    /* learned proposal: fall back to full search when cost > ratio x predicted */
+  float fallback_ratio;    /* learned proposal: fall back to full search when cost > ratio x predicted;
+                              a non-finite ratio (infinity) turns the fallback off */
 } ia_config;
 
 /* What later pictures and the loop filter read about a coded macroblock: the P/I subset of

@@ -3,6 +3,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <cmath>
 #include <string>
 
 #include "analyse.h"
@@ -40,7 +41,8 @@ struct ia_ctx {
   // learned proposal
   ia::CnnWeights *wts = nullptr;
   bool wts_loaded = false;
-  int *fb = nullptr;           // [0] count, [1..] entries
+  int *fb = nullptr;           // [0], [1] counts (alternating calls), [2..] entries
+  int fb_gen = 0;              // the count the next learned search uses
   float *feat = nullptr, *cnn_out = nullptr;
   int16_t *centres = nullptr;
   std::string err;
@@ -137,7 +139,8 @@ int ia_create(ia_ctx **out, const ia_config *cfg) {
        alloc((void **)&c->state, nmb * sizeof(ia::MbState)) == hipSuccess &&
        alloc((void **)&c->entropy, 128 * sizeof(uint32_t)) == hipSuccess &&
        alloc((void **)&c->wts, sizeof(ia::CnnWeights)) == hipSuccess &&
-       alloc((void **)&c->fb, (1 + 4 * nmb) * sizeof(int)) == hipSuccess;
+       alloc((void **)&c->fb, (2 + 4 * nmb) * sizeof(int)) == hipSuccess;
+  ok = ok && hipMemset(c->fb, 0, 2 * sizeof(int)) == hipSuccess;
   if (ok) {
     // -log2 of the MPS/LPS probability per CABAC state in 1/256 bits (enc/cabac.rs:162-173), computed
     // with the C library's pow/log2 as the reference computes it with Rust's (which call the same)
@@ -246,13 +249,6 @@ static int check_refs(ia_ctx *c, const int *ref_slots, int nref) {
   return IA_OK;
 }
 
-namespace {
-__global__ void zero_ints(int *p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
-}
-}  // namespace
-
 int ia_motion_search(ia_ctx *c, const int *ref_slots, int nref) {
   if (!c) return IA_EINVAL;
   int rc = check_refs(c, ref_slots, nref);
@@ -267,14 +263,16 @@ int ia_motion_search(ia_ctx *c, const int *ref_slots, int nref) {
     p.rmbs0 = c->slots[ref_slots[0]].mbs;
     p.wts = c->wts;
     p.out = c->fs;
-    p.fb_count = c->fb;
-    p.fb_list = c->fb + 1;
+    // the fallback count alternates between two words: each call's list search clears the other
+    // one, which the next call counts in (a non-finite ratio turns the fallback off)
+    const int gen = c->fb_gen;
+    const bool fallback = std::isfinite(c->cfg.fallback_ratio);
+    p.fb_count = c->fb + gen;
+    p.fb_list = fallback ? c->fb + 2 : nullptr;
     p.mb_w = c->g.mb_w;
     p.mb_h = c->g.mb_h;
     p.nref = nref;
     p.fallback_ratio = c->cfg.fallback_ratio;
-    hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, c->stream, c->fb, 1);
-    IA_HIP(c, hipGetLastError());
     IA_HIP(c, ia::launch_proposal(p, c->stream));
     ia::FsArgs a;
     memset(&a, 0, sizeof a);
@@ -285,7 +283,10 @@ int ia_motion_search(ia_ctx *c, const int *ref_slots, int nref) {
     a.mb_w = c->g.mb_w;
     a.mb_h = c->g.mb_h;
     a.range = c->cfg.search_range;
-    IA_HIP(c, ia::launch_fullsearch_list(a, c->fb + 1, c->fb, c->stream));
+    if (fallback) {
+      IA_HIP(c, ia::launch_fullsearch_list(a, c->fb + 2, c->fb + gen, c->fb + (gen ^ 1), c->stream));
+      c->fb_gen = gen ^ 1;
+    }
     c->fs_nref = nref;
     for (int r = 0; r < nref; r++) c->fs_slots[r] = ref_slots[r];
     c->fs_valid = true;
@@ -506,7 +507,8 @@ int ia_proposal_forward(ia_ctx *c, const float *features, int n, float *outputs)
 
 int ia_read_fallback_count(ia_ctx *c, int *count) {
   if (!c || !count) return IA_EINVAL;
-  IA_HIP(c, hipMemcpyAsync(count, c->fb, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  // the count of the last learned search (the other word once a search has flipped them)
+  IA_HIP(c, hipMemcpyAsync(count, c->fb + (c->fb_gen ^ 1), sizeof(int), hipMemcpyDeviceToHost, c->stream));
   IA_HIP(c, hipStreamSynchronize(c->stream));
   return IA_OK;
 }

@@ -221,11 +221,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 }
 
 // The blocks walk the entries list[i] = mb * 4 + ref, i < *count, grid-stride (a separate kernel, so
-// that the loop's registers do not weigh on the whole-picture kernel).
+// that the loop's registers do not weigh on the whole-picture kernel). Block 0 clears *reset.
 template <int K, int NT, int RAWS, int WPE>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void fullsearch_list_kernel(
-    FsArgs a, int rows_lds, const int *list, const int *count) {
+    FsArgs a, int rows_lds, const int *list, const int *count, int *reset) {
   extern __shared__ uint32_t raw[];
+  if (blockIdx.x == 0 && threadIdx.x == 0) *reset = 0;
   const int n = *count;
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     const int e = list[i];
@@ -251,8 +252,9 @@ const Variant VARIANTS[] = {
 #undef IA_FS_VARIANT
 constexpr int NVARIANTS = sizeof(VARIANTS) / sizeof(VARIANTS[0]);
 constexpr int DEFAULT_VARIANT = 4;  // k6_t128_w3: fastest on MI355X at 1080p +-32 (measured)
-// the list kernel's tile: its grid-stride loop needs more registers than 3 waves per SIMD allow
-constexpr int LIST_K = 6, LIST_NT = 128, LIST_W = 2;
+// the list kernel's tile: a fallback list is short (a few percent of the (mb, ref) pairs), so
+// wide workgroups shorten the one round it takes (fastest measured at 1080p +-32)
+constexpr int LIST_K = 6, LIST_NT = 256, LIST_W = 3;
 
 }  // namespace
 
@@ -273,7 +275,8 @@ hipError_t launch_fullsearch(const FsArgs &a, int variant, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_fullsearch_list(const FsArgs &a, const int *list, const int *count, hipStream_t stream) {
+hipError_t launch_fullsearch_list(const FsArgs &a, const int *list, const int *count, int *reset,
+                                  hipStream_t stream) {
   if (a.range < 1 || a.range > RMAX || a.nref < 1 || a.nref > 4) return hipErrorInvalidValue;
   const bool small = a.range <= 32;
   const int raws = small ? raws_for(32) : raws_for(64);
@@ -283,7 +286,7 @@ hipError_t launch_fullsearch_list(const FsArgs &a, const int *list, const int *c
   dim3 grid(min(a.mb_w * a.mb_h * a.nref, 2048));
   hipLaunchKernelGGL(small ? (fullsearch_list_kernel<LIST_K, LIST_NT, raws_for(32), LIST_W>)
                            : (fullsearch_list_kernel<LIST_K, LIST_NT, raws_for(64), LIST_W>),
-                     grid, dim3(LIST_NT), shmem, stream, a, rows_lds, list, count);
+                     grid, dim3(LIST_NT), shmem, stream, a, rows_lds, list, count, reset);
   return hipGetLastError();
 }
 

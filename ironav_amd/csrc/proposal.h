@@ -24,7 +24,7 @@ struct PropArgs {
   const ia_mbinfo *rmbs0;       // reference 0's macroblock records (temporal predictors), may be null
   const CnnWeights *wts;
   ia_fsresult *out;             // [nref][mb][IA_NPART]
-  int *fb_list;                 // fallback (mb * 4 + ref) entries
+  int *fb_list;                 // fallback (mb * 4 + ref) entries; null: no fallback
   int *fb_count;
   float *feat;                  // optional: CNN input features [mb][81][CNN_IN] (for training)
   float *cnn_out;               // optional: CNN outputs [mb][81][CNN_OUT]
@@ -36,7 +36,8 @@ struct PropArgs {
 };
 
 hipError_t launch_proposal(const PropArgs &a, hipStream_t stream);
-// the exhaustive search over the (mb, ref) entries of a fallback list
-hipError_t launch_fullsearch_list(const FsArgs &a, const int *list, const int *count, hipStream_t stream);
+// the exhaustive search over the (mb, ref) entries of a fallback list; the search also sets *reset
+// to 0 (the count the next list will be gathered in)
+hipError_t launch_fullsearch_list(const FsArgs &a, const int *list, const int *count, int *reset, hipStream_t stream);
 
 }  // namespace ia

@@ -159,3 +159,33 @@ def test_learned_analysis_runs_end_to_end():
         an.set_reference(0, res.recon, res.mbs)
         p = an.analyse(pics[1], 28, abi.PIC_P, [0])
     assert np.all(p.codes["kind"] > 0)
+
+
+def test_fallback_count_per_call_and_fallback_off(oracle):
+    """The fallback count is per call (each call's list search clears the count the next call uses),
+    and an infinite ratio turns the fallback off (no entries, the proposal's own results)."""
+    params = proposal.init_params(14)
+    pics = frames(96, 64, 3, 9)
+    with Analyser(96, 64, nrefs=2, search_mode=abi.SEARCH_LEARNED, search_range=16, fallback_ratio=0.0) as an:
+        an.load_proposal_weights(params)
+        for s in range(2):
+            an.set_reference(s, pics[s])
+        an.upload(pics[2])
+        want = oracle.fullsearch(pics[2].y, [pics[0].y, pics[1].y], 6, 4, 16)
+        for nref in (2, 1, 2):
+            an.motion_search(list(range(nref)))
+            got = an.read_motion(nref)
+            assert an.fallback_count() == 24 * nref
+            assert np.array_equal(got["sad"], want["sad"][:nref]) and np.array_equal(got["mv"], want["mv"][:nref])
+    with Analyser(96, 64, nrefs=2, search_mode=abi.SEARCH_LEARNED, search_range=16,
+                  fallback_ratio=float("inf")) as an:
+        an.load_proposal_weights(params)
+        for s in range(2):
+            an.set_reference(s, pics[s])
+        an.upload(pics[2])
+        for _ in range(2):
+            an.motion_search([0, 1])
+            assert an.fallback_count() == 0
+        got = an.read_motion(2)
+        # the proposal's own winners (its grid is not bounded by the exhaustive search's range)
+        assert np.all(got["sad"][:, :, 0] < 0xFFFFFFFF)
