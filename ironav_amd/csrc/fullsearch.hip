@@ -26,6 +26,7 @@ namespace ia {
 namespace {
 
 constexpr int RMAX = 64;
+constexpr int NXCD = 8;  // MI355X: accelerator complex dies, each with its own L2
 
 __device__ __forceinline__ uint32_t ld_u32(const uint8_t *p) { return *reinterpret_cast<const uint32_t *>(p); }
 
@@ -212,12 +213,16 @@ __device__ __forceinline__ void search_one(const FsArgs &a, int rows_lds, int mb
   }
 }
 
-// Block (x, y) searches macroblock x against reference y.
+// Block (x, y) searches a macroblock against reference y. Workgroups are dispatched to the 8 XCDs in
+// turn and each XCD has its own L2, so block x takes macroblock (x % 8) * (n / 8) + x / 8: each XCD
+// walks one band of consecutive macroblocks, whose overlapping windows then meet in its L2.
 template <int K, int NT, int RAWS, int WPE>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void fullsearch_kernel(FsArgs a,
                                                                                                  int rows_lds) {
   extern __shared__ uint32_t raw[];
-  search_one<K, NT, RAWS>(a, rows_lds, blockIdx.x, blockIdx.y, raw);
+  const int x = blockIdx.x, band = gridDim.x / NXCD;
+  const int mb = x < band * NXCD ? (x % NXCD) * band + x / NXCD : x;
+  search_one<K, NT, RAWS>(a, rows_lds, mb, blockIdx.y, raw);
 }
 
 // The blocks walk the entries list[i] = mb * 4 + ref, i < *count, grid-stride (a separate kernel, so

@@ -258,7 +258,11 @@ __global__ __launch_bounds__(64 * WAVES) void proposal_kernel(PropArgs a) {
   WaveLds &V = L.wv[wave];
   const int nmb = a.mb_w * a.mb_h;
   const int pw = a.mb_w * 16, ph = a.mb_h * 16, sw = pw + 2 * PAD;
-  for (int mb = blockIdx.x * WAVES + wave; mb < nmb; mb += gridDim.x * WAVES) {
+  // workgroups go to the 8 XCDs in turn (each with its own L2): renumber them so that those on one
+  // XCD take consecutive chunks of macroblocks, whose windows overlap
+  const int nxcd = 8, per = gridDim.x / nxcd;
+  const int vb = (int)blockIdx.x < per * nxcd ? (int)(blockIdx.x % nxcd) * per + (int)blockIdx.x / nxcd : (int)blockIdx.x;
+  for (int mb = vb * WAVES + wave; mb < nmb; mb += gridDim.x * WAVES) {
     // the lane index afresh in each iteration (an empty asm the compiler cannot see through), so
     // that lane-derived addresses are recomputed here rather than hoisted out of the loop and held
     // in registers for its whole length
