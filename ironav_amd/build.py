@@ -73,6 +73,22 @@ def build_train():
     return TRAIN_OUT
 
 
+PROBE_SRC = os.path.join(HERE, "probes", "issue_rates.hip")
+PROBE_OUT = os.path.join(HERE, "probes", "issue_rates")
+
+
+def build_probes():
+    """The issue-rate probe (ironav_amd/probes/issue_rates): the VALU rates behind the SAD rooflines."""
+    tmp = PROBE_OUT + ".tmp"
+    p = subprocess.run([HIPCC, *[f for f in FLAGS if f not in ("-fPIC",)], PROBE_SRC, "-o", tmp],
+                       capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError(f"hipcc failed on {PROBE_SRC}:\n{p.stderr}")
+    os.replace(tmp, PROBE_OUT)
+    return PROBE_OUT
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
     print(build_train())
+    print(build_probes())

@@ -225,19 +225,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   search_one<K, NT, RAWS>(a, rows_lds, mb, blockIdx.y, raw);
 }
 
-// The blocks walk the entries list[i] = mb * 4 + ref, i < *count, grid-stride (a separate kernel, so
-// that the loop's registers do not weigh on the whole-picture kernel). Block 0 clears *reset.
+// Block i searches entry list[i] = mb * 4 + ref when i < *count and returns at once otherwise (the
+// grid covers every possible entry: the count is known on the device only). Block 0 clears *reset.
 template <int K, int NT, int RAWS, int WPE>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void fullsearch_list_kernel(
     FsArgs a, int rows_lds, const int *list, const int *count, int *reset) {
   extern __shared__ uint32_t raw[];
   if (blockIdx.x == 0 && threadIdx.x == 0) *reset = 0;
-  const int n = *count;
-  for (int i = blockIdx.x; i < n; i += gridDim.x) {
-    const int e = list[i];
-    search_one<K, NT, RAWS>(a, rows_lds, e >> 2, e & 3, raw);
-    __syncthreads();  // the next entry restages the window and the reduction scratch
-  }
+  if ((int)blockIdx.x >= *count) return;
+  const int e = list[blockIdx.x];
+  search_one<K, NT, RAWS>(a, rows_lds, e >> 2, e & 3, raw);
 }
 
 
@@ -257,8 +254,8 @@ const Variant VARIANTS[] = {
 #undef IA_FS_VARIANT
 constexpr int NVARIANTS = sizeof(VARIANTS) / sizeof(VARIANTS[0]);
 constexpr int DEFAULT_VARIANT = 4;  // k6_t128_w3: fastest on MI355X at 1080p +-32 (measured)
-// the list kernel's tile: a fallback list is short (a few percent of the (mb, ref) pairs), so
-// wide workgroups shorten the one round it takes (fastest measured at 1080p +-32)
+// the list kernel's tile: a fallback list is short (a few percent of the (mb, ref) pairs), so wide
+// workgroups shorten the one round it takes (fastest measured at 1080p +-32, 3 references)
 constexpr int LIST_K = 6, LIST_NT = 256, LIST_W = 3;
 
 }  // namespace
@@ -287,8 +284,7 @@ hipError_t launch_fullsearch_list(const FsArgs &a, const int *list, const int *c
   const int raws = small ? raws_for(32) : raws_for(64);
   const int rows_lds = 2 * a.range + 1 + LIST_K + 15;
   const size_t shmem = (size_t)rows_lds * raws * sizeof(uint32_t);
-  // a grid of at most 2048 blocks walks the device-side list
-  dim3 grid(min(a.mb_w * a.mb_h * a.nref, 2048));
+  dim3 grid(a.mb_w * a.mb_h * a.nref);
   hipLaunchKernelGGL(small ? (fullsearch_list_kernel<LIST_K, LIST_NT, raws_for(32), LIST_W>)
                            : (fullsearch_list_kernel<LIST_K, LIST_NT, raws_for(64), LIST_W>),
                      grid, dim3(LIST_NT), shmem, stream, a, rows_lds, list, count, reset);
