@@ -200,8 +200,9 @@ int ia_sync(ia_ctx *ctx);
  * ia_motion_search proposes 8 candidate cells per macroblock and reference with a 4-layer 3x3 BF16
  * CNN on a 9x9 grid of 4-sample displacement cells around the reference's temporal predictor,
  * refines them by exact SAD over +-2 samples for all 41 partitions, and runs the exhaustive search
- * instead where the refined 16x16 SAD exceeds fallback_ratio x the CNN's predicted SAD. References
- * are limited to 3. Results have the layout of the exhaustive search. */
+ * instead where the refined 16x16 SAD exceeds fallback_ratio x the CNN's predicted SAD (a
+ * non-finite fallback_ratio turns that off). References are limited to 3. Results have the layout
+ * of the exhaustive search. */
 #define IA_CNN_PARAMS 21988  /* f32 parameters: per layer, weights [cout][cin][3][3] then biases [cout];
                                 (cin, cout) = (8, 32), (32, 32), (32, 32), (32, 4) */
 #define IA_CNN_CELLS 81
