@@ -223,6 +223,37 @@ int ia_proposal_forward(ia_ctx *ctx, const float *features, int n, float *output
 /* The number of (macroblock, reference) pairs the last learned search sent to the exhaustive search. */
 int ia_read_fallback_count(ia_ctx *ctx, int *count);
 
+/* ---- Batching (no counterpart in the reference, whose encoder codes one stream at a time) ----
+ * A batch groups contexts (streams of any sizes) so that a frame step of all of them runs as one set
+ * of launches: the exhaustive search of every (stream, macroblock, reference), one persistent
+ * analysis kernel whose workgroups take (stream, slice) tasks from a queue (longest first), and the
+ * promotion of each stream's reconstruction to a reference (half-sample planes, chroma, records).
+ * With use_graph the step is captured once as a hipGraph and replayed: the per-frame parameters
+ * travel in a table copied from pinned host memory by the graph's first node. Each context's
+ * results are read with ia_read_analysis / ia_read_motion after ia_batch_sync. While a batch
+ * exists its contexts take no other calls but uploads, reads and reference loads between steps.
+ * Contexts with IA_SEARCH_LEARNED are not batched (IA_EUNSUPPORTED). */
+typedef struct ia_batch ia_batch;
+
+typedef struct ia_frame {
+  int qp;                 /* slice QP, 0..51 */
+  int kind;               /* IA_PIC_IDR or IA_PIC_P */
+  int nref;               /* P pictures: references used, 1..nrefs */
+  int ref_slots[16];      /* P pictures: slots, newest first (as plist, enc/mod.rs:559) */
+  int promote_slot;       /* -1, or the slot that receives this picture's reconstruction and records
+                             as a reference when the step is done (not deblocked) */
+  const uint8_t *y, *cb, *cr;  /* the picture's padded planes in device memory, or null: the context's
+                                  uploaded source (ia_upload_source) */
+} ia_frame;
+
+int ia_batch_create(ia_batch **out, ia_ctx *const *ctxs, int n, int use_graph);
+void ia_batch_destroy(ia_batch *b);
+const char *ia_batch_last_error(const ia_batch *b);
+/* Enqueues one frame step: frames[i] for context i. Asynchronous. */
+int ia_batch_step(ia_batch *b, const ia_frame *frames);
+/* Waits for the batch's stream. */
+int ia_batch_sync(ia_batch *b);
+
 /* Tile shapes of the full-search kernel, for the autotuner (no counterpart in the reference, whose
  * search is a scalar loop): the number of compiled shapes, each one's name, and the shape a context
  * uses (-1: the default). */

@@ -59,6 +59,14 @@ class FsResult(C.Structure):
     _fields_ = [("mv", C.c_int16 * 2), ("sad", C.c_uint32)]
 
 
+class Frame(C.Structure):
+    """ia_frame: one stream's picture in a batch step."""
+    _fields_ = [
+        ("qp", C.c_int), ("kind", C.c_int), ("nref", C.c_int), ("ref_slots", C.c_int * 16),
+        ("promote_slot", C.c_int), ("y", C.c_void_p), ("cb", C.c_void_p), ("cr", C.c_void_p),
+    ]
+
+
 MBINFO_DTYPE = np.dtype(MbInfo)
 MBCODE_DTYPE = np.dtype(MbCode)
 MBSTATS_DTYPE = np.dtype(MbStats)
@@ -90,6 +98,11 @@ PROTOTYPES = {
     "ia_fullsearch_variants": (C.c_int, []),
     "ia_fullsearch_variant_name": (C.c_char_p, [C.c_int]),
     "ia_set_fullsearch_variant": (C.c_int, [vp, C.c_int]),
+    "ia_batch_create": (C.c_int, [C.POINTER(vp), C.POINTER(vp), C.c_int, C.c_int]),
+    "ia_batch_destroy": (None, [vp]),
+    "ia_batch_last_error": (C.c_char_p, [vp]),
+    "ia_batch_step": (C.c_int, [vp, C.POINTER(Frame)]),
+    "ia_batch_sync": (C.c_int, [vp]),
 }
 
 

@@ -38,5 +38,11 @@ struct AnParams {
 };
 
 hipError_t launch_analyse(const AnParams &p, hipStream_t stream);
+// Slices of many pictures: tasks[t] = picture << 16 | slice (t < ntasks) with ps[picture] in device
+// memory; `workers` workgroups take tasks from a queue counted in *head (0 at the launch).
+hipError_t launch_analyse_batch(const AnParams *ps, const int *tasks, int ntasks, int *head, int workers,
+                                hipStream_t stream);
+// workgroups of the batched analysis resident on the device at once (0 on error)
+int analyse_batch_workers();
 
 }  // namespace ia

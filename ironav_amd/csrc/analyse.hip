@@ -1997,10 +1997,9 @@ struct Coder {
   }
 };
 
-__global__ __launch_bounds__(64) void analyse_kernel(AnParams p) {
-  __shared__ Lds lds;
+// Slice s of the picture p describes, its macroblocks in raster order.
+__device__ void analyse_slice(const AnParams &p, int s, Lds &lds) {
   Coder c(p, lds);
-  const int s = blockIdx.x;
   c.row0 = s * p.mb_h / p.nslices;
   c.row1 = (s + 1) * p.mb_h / p.nslices;
   c.slice_id = s + 1;
@@ -2026,7 +2025,43 @@ __global__ __launch_bounds__(64) void analyse_kernel(AnParams p) {
     }
 }
 
+__global__ __launch_bounds__(64) void analyse_kernel(AnParams p) {
+  __shared__ Lds lds;
+  analyse_slice(p, blockIdx.x, lds);
+}
+
+// Many pictures at once: a pool of workgroups takes (picture, slice) tasks from a queue, tasks[t] =
+// picture << 16 | slice for t < ntasks (longest first), ps[picture] its parameters. *head counts the
+// tasks taken and must be 0 at the launch; a worker leaves when the queue is empty.
+__global__ __launch_bounds__(64) void analyse_batch_kernel(const AnParams *ps, const int *tasks, int ntasks, int *head) {
+  __shared__ Lds lds;
+  __shared__ int next;
+  for (;;) {
+    __syncthreads();  // the previous task is done with `next` and the LDS
+    if (threadIdx.x == 0) next = atomicAdd(head, 1);
+    __syncthreads();
+    const int t = next;
+    if (t >= ntasks) return;
+    const int e = __builtin_amdgcn_readfirstlane(tasks[t]);
+    analyse_slice(ps[e >> 16], e & 0xffff, lds);
+  }
+}
+
 }  // namespace
+
+hipError_t launch_analyse_batch(const AnParams *ps, const int *tasks, int ntasks, int *head, int workers,
+                                hipStream_t stream) {
+  hipLaunchKernelGGL(analyse_batch_kernel, dim3(workers), dim3(64), 0, stream, ps, tasks, ntasks, head);
+  return hipGetLastError();
+}
+
+int analyse_batch_workers() {
+  int dev = 0, per = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, analyse_batch_kernel, 64, 0) != hipSuccess) return 0;
+  return (per > 0 ? per : 1) * prop.multiProcessorCount;
+}
 
 hipError_t launch_analyse(const AnParams &p, hipStream_t stream) {
   hipLaunchKernelGGL(analyse_kernel, dim3(p.nslices), dim3(64), 0, stream, p);

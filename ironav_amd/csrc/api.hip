@@ -6,67 +6,103 @@
 #include <cmath>
 #include <string>
 
-#include "analyse.h"
-#include "proposal.h"
+#include "ctx.h"
 
-using ia::Geom;
+namespace iah {
 
-struct IaSlot {
-  uint8_t *hp = nullptr;      // 4 half-sample planes, sw x sh each
-  uint8_t *cb = nullptr, *cr = nullptr;
-  ia_mbinfo *mbs = nullptr;   // macroblock records of the reference picture
-  bool loaded = false;
-};
+std::string &create_error() {
+  static thread_local std::string e;
+  return e;
+}
 
-struct ia_ctx {
-  ia_config cfg;
-  Geom g;
-  int device = 0;
-  hipStream_t own = nullptr, stream = nullptr;
-  uint8_t *src[3] = {nullptr, nullptr, nullptr};
-  uint8_t *tmp_luma = nullptr;
-  IaSlot slots[16];
-  ia_fsresult *fs = nullptr;
-  int fs_nref = 0, fs_slots[16] = {0};
-  bool fs_valid = false;
-  int fs_variant = -1;
-  // analysis outputs and working state
-  uint8_t *rec[3] = {nullptr, nullptr, nullptr};
-  ia_mbcode *codes = nullptr;
-  ia_mbstats *stats = nullptr;
-  ia_mbinfo *mbs = nullptr;
-  ia::MbState *state = nullptr;
-  uint32_t *entropy = nullptr;
-  bool an_valid = false;
-  // learned proposal
-  ia::CnnWeights *wts = nullptr;
-  bool wts_loaded = false;
-  int *fb = nullptr;           // [0], [1] counts (alternating calls), [2..] entries
-  int fb_gen = 0;              // the count the next learned search uses
-  float *feat = nullptr, *cnn_out = nullptr;
-  int16_t *centres = nullptr;
-  std::string err;
-};
+int check_refs(ia_ctx *c, const int *ref_slots, int nref) {
+  if (nref < 1 || nref > c->cfg.nrefs || !ref_slots) return fail(c, IA_EINVAL, "nref must be within 1..nrefs");
+  for (int r = 0; r < nref; r++) {
+    if (ref_slots[r] < 0 || ref_slots[r] >= c->cfg.nrefs) return fail(c, IA_EINVAL, "reference slot out of range");
+    if (!c->slots[ref_slots[r]].loaded) return fail(c, IA_ESTATE, "reference slot is empty");
+  }
+  return IA_OK;
+}
+
+void search_args(ia_ctx *c, const int *ref_slots, int nref, const uint8_t *src_y, ia::FsArgs *a) {
+  memset(a, 0, sizeof *a);
+  a->src_y = src_y ? src_y : c->src[0];
+  for (int r = 0; r < nref; r++) a->ref_g[r] = c->slots[ref_slots[r]].hp;
+  a->out = c->fs;
+  a->nref = nref;
+  a->mb_w = c->g.mb_w;
+  a->mb_h = c->g.mb_h;
+  a->range = c->cfg.search_range;
+}
+
+static int chroma_qp(int qp) {
+  static const uint8_t T[22] = {29, 30, 31, 32, 32, 33, 34, 34, 35, 35, 36, 36, 37, 37, 37, 38, 38, 38, 39, 39, 39, 39};
+  const int q = qp < 0 ? 0 : (qp > 51 ? 51 : qp);
+  return q < 30 ? q : T[q - 30];
+}
+
+int analysis_params(ia_ctx *c, int qp, int kind, const int *ref_slots, int nref, const uint8_t *const *src,
+                    bool need_fs, ia::AnParams *pp) {
+  ia::AnParams &p = *pp;
+  if (qp < 0 || qp > 51) return fail(c, IA_EINVAL, "qp must be within 0..51");
+  if (kind != IA_PIC_IDR && kind != IA_PIC_P) return fail(c, IA_EINVAL, "kind must be IA_PIC_IDR or IA_PIC_P");
+  memset(&p, 0, sizeof p);
+  const bool pslice = kind == IA_PIC_P;
+  if (pslice) {
+    int rc = check_refs(c, ref_slots, nref);
+    if (rc) return rc;
+  } else {
+    nref = 0;
+  }
+  if (pslice && c->cfg.search_mode != IA_SEARCH_DIAMOND) {
+    if (need_fs) {
+      if (!c->fs_valid || c->fs_nref != nref) return fail(c, IA_ESTATE, "no exhaustive-search results for these references");
+      for (int r = 0; r < nref; r++)
+        if (c->fs_slots[r] != ref_slots[r]) return fail(c, IA_ESTATE, "exhaustive-search results are for other references");
+    }
+    p.fs = c->fs;
+  }
+  for (int k = 0; k < 3; k++) {
+    p.src[k] = src && src[k] ? src[k] : c->src[k];
+    p.rec[k] = c->rec[k];
+  }
+  for (int r = 0; r < nref; r++) {
+    const IaSlot &sl = c->slots[ref_slots[r]];
+    p.hp[r] = sl.hp;
+    p.rcb[r] = sl.cb;
+    p.rcr[r] = sl.cr;
+  }
+  p.rmbs0 = nref > 0 ? c->slots[ref_slots[0]].mbs : nullptr;
+  p.codes = c->codes;
+  p.stats = c->stats;
+  p.mbs = c->mbs;
+  p.state = c->state;
+  p.entropy = c->entropy;
+  p.mb_w = c->g.mb_w;
+  p.mb_h = c->g.mb_h;
+  p.nslices = c->cfg.slices < c->g.mb_h ? c->cfg.slices : c->g.mb_h;
+  p.nref = nref;
+  p.qp_slice = qp;
+  p.qp = qp;  // + 6 * (bit_depth - 8) at 8 bits
+  p.qpc = chroma_qp(qp);
+  // lambda = max(1, round(2^((qp - 12) / 6))), enc/frame.rs:164-166
+  p.lambda = (int)round(pow(2.0, ((double)p.qp - 12.0) / 6.0));
+  if (p.lambda < 1) p.lambda = 1;
+  p.p_slice = pslice;
+  p.sub8 = c->cfg.sub8x8 && qp <= 20;
+  p.use_cabac = c->cfg.cabac != 0;
+  p.transform8 = c->cfg.cabac && c->cfg.transform8;
+  p.lam2 = 1.6 * pow(2.0, ((double)p.qp - 12.0) / 3.0);
+  p.skip_thresh = 256.0 * 0.625 * pow(2.0, (double)p.qp / 6.0);
+  return IA_OK;
+}
+
+}  // namespace iah
+
+using iah::fail;
+using iah::hip_fail;
 
 namespace {
-
-thread_local std::string g_create_error;
-
-int fail(ia_ctx *c, int code, const std::string &msg) {
-  if (c) c->err = msg;
-  else g_create_error = msg;
-  return code;
-}
-
-int hip_fail(ia_ctx *c, hipError_t e, const char *what) {
-  return fail(c, IA_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-#define IA_HIP(c, call)                                    \
-  do {                                                     \
-    hipError_t e_ = (call);                                \
-    if (e_ != hipSuccess) return hip_fail((c), e_, #call); \
-  } while (0)
 
 int check_config(const ia_config *cfg, std::string &why) {
   if (cfg->width < 16 || cfg->height < 16 || cfg->width > 8192 || cfg->height > 8192 || (cfg->width & 1) || (cfg->height & 1)) {
@@ -193,7 +229,7 @@ void ia_destroy(ia_ctx *c) {
   delete c;
 }
 
-const char *ia_last_error(const ia_ctx *c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+const char *ia_last_error(const ia_ctx *c) { return c ? c->err.c_str() : iah::create_error().c_str(); }
 
 int ia_set_stream(ia_ctx *c, void *hip_stream) {
   if (!c) return IA_EINVAL;
@@ -240,18 +276,9 @@ int ia_upload_source(ia_ctx *c, const uint8_t *y, const uint8_t *cb, const uint8
   return IA_OK;
 }
 
-static int check_refs(ia_ctx *c, const int *ref_slots, int nref) {
-  if (nref < 1 || nref > c->cfg.nrefs || !ref_slots) return fail(c, IA_EINVAL, "nref must be within 1..nrefs");
-  for (int r = 0; r < nref; r++) {
-    if (ref_slots[r] < 0 || ref_slots[r] >= c->cfg.nrefs) return fail(c, IA_EINVAL, "reference slot out of range");
-    if (!c->slots[ref_slots[r]].loaded) return fail(c, IA_ESTATE, "reference slot is empty");
-  }
-  return IA_OK;
-}
-
 int ia_motion_search(ia_ctx *c, const int *ref_slots, int nref) {
   if (!c) return IA_EINVAL;
-  int rc = check_refs(c, ref_slots, nref);
+  int rc = iah::check_refs(c, ref_slots, nref);
   if (rc) return rc;
   if (c->cfg.search_mode == IA_SEARCH_LEARNED) {
     if (!c->wts_loaded) return fail(c, IA_ESTATE, "the proposal's weights are not loaded");
@@ -275,14 +302,7 @@ int ia_motion_search(ia_ctx *c, const int *ref_slots, int nref) {
     p.fallback_ratio = c->cfg.fallback_ratio;
     IA_HIP(c, ia::launch_proposal(p, c->stream));
     ia::FsArgs a;
-    memset(&a, 0, sizeof a);
-    a.src_y = c->src[0];
-    for (int r = 0; r < nref; r++) a.ref_g[r] = c->slots[ref_slots[r]].hp;
-    a.out = c->fs;
-    a.nref = nref;
-    a.mb_w = c->g.mb_w;
-    a.mb_h = c->g.mb_h;
-    a.range = c->cfg.search_range;
+    iah::search_args(c, ref_slots, nref, nullptr, &a);
     if (fallback) {
       IA_HIP(c, ia::launch_fullsearch_list(a, c->fb + 2, c->fb + gen, c->fb + (gen ^ 1), c->stream));
       c->fb_gen = gen ^ 1;
@@ -293,14 +313,7 @@ int ia_motion_search(ia_ctx *c, const int *ref_slots, int nref) {
     return IA_OK;
   }
   ia::FsArgs a;
-  memset(&a, 0, sizeof a);
-  a.src_y = c->src[0];
-  for (int r = 0; r < nref; r++) a.ref_g[r] = c->slots[ref_slots[r]].hp;
-  a.out = c->fs;
-  a.nref = nref;
-  a.mb_w = c->g.mb_w;
-  a.mb_h = c->g.mb_h;
-  a.range = c->cfg.search_range;
+  iah::search_args(c, ref_slots, nref, nullptr, &a);
   IA_HIP(c, ia::launch_fullsearch(a, c->fs_variant, c->stream));
   c->fs_nref = nref;
   for (int r = 0; r < nref; r++) c->fs_slots[r] = ref_slots[r];
@@ -318,63 +331,12 @@ int ia_read_motion(ia_ctx *c, ia_fsresult *out) {
   return IA_OK;
 }
 
-static int chroma_qp(int qp) {
-  static const uint8_t T[22] = {29, 30, 31, 32, 32, 33, 34, 34, 35, 35, 36, 36, 37, 37, 37, 38, 38, 38, 39, 39, 39, 39};
-  const int q = qp < 0 ? 0 : (qp > 51 ? 51 : qp);
-  return q < 30 ? q : T[q - 30];
-}
 
 int ia_analyse_uploaded(ia_ctx *c, int qp, int kind, const int *ref_slots, int nref) {
   if (!c) return IA_EINVAL;
-  if (qp < 0 || qp > 51) return fail(c, IA_EINVAL, "qp must be within 0..51");
-  if (kind != IA_PIC_IDR && kind != IA_PIC_P) return fail(c, IA_EINVAL, "kind must be IA_PIC_IDR or IA_PIC_P");
   ia::AnParams p;
-  memset(&p, 0, sizeof p);
-  const bool pslice = kind == IA_PIC_P;
-  if (pslice) {
-    int rc = check_refs(c, ref_slots, nref);
-    if (rc) return rc;
-  } else {
-    nref = 0;
-  }
-  if (pslice && c->cfg.search_mode != IA_SEARCH_DIAMOND) {
-    if (!c->fs_valid || c->fs_nref != nref) return fail(c, IA_ESTATE, "no exhaustive-search results for these references");
-    for (int r = 0; r < nref; r++)
-      if (c->fs_slots[r] != ref_slots[r]) return fail(c, IA_ESTATE, "exhaustive-search results are for other references");
-    p.fs = c->fs;
-  }
-  for (int k = 0; k < 3; k++) {
-    p.src[k] = c->src[k];
-    p.rec[k] = c->rec[k];
-  }
-  for (int r = 0; r < nref; r++) {
-    const IaSlot &sl = c->slots[ref_slots[r]];
-    p.hp[r] = sl.hp;
-    p.rcb[r] = sl.cb;
-    p.rcr[r] = sl.cr;
-  }
-  p.rmbs0 = nref > 0 ? c->slots[ref_slots[0]].mbs : nullptr;
-  p.codes = c->codes;
-  p.stats = c->stats;
-  p.mbs = c->mbs;
-  p.state = c->state;
-  p.entropy = c->entropy;
-  p.mb_w = c->g.mb_w;
-  p.mb_h = c->g.mb_h;
-  p.nslices = c->cfg.slices < c->g.mb_h ? c->cfg.slices : c->g.mb_h;
-  p.nref = nref;
-  p.qp_slice = qp;
-  p.qp = qp;  // + 6 * (bit_depth - 8) at 8 bits
-  p.qpc = chroma_qp(qp);
-  // lambda = max(1, round(2^((qp - 12) / 6))), enc/frame.rs:164-166
-  p.lambda = (int)round(pow(2.0, ((double)p.qp - 12.0) / 6.0));
-  if (p.lambda < 1) p.lambda = 1;
-  p.p_slice = pslice;
-  p.sub8 = c->cfg.sub8x8 && qp <= 20;
-  p.use_cabac = c->cfg.cabac != 0;
-  p.transform8 = c->cfg.cabac && c->cfg.transform8;
-  p.lam2 = 1.6 * pow(2.0, ((double)p.qp - 12.0) / 3.0);
-  p.skip_thresh = 256.0 * 0.625 * pow(2.0, (double)p.qp / 6.0);
+  int rc = iah::analysis_params(c, qp, kind, ref_slots, nref, nullptr, true, &p);
+  if (rc) return rc;
   IA_HIP(c, ia::launch_analyse(p, c->stream));
   c->an_valid = true;
   return IA_OK;
@@ -456,7 +418,7 @@ static int ensure_tool_buffers(ia_ctx *c) {
 
 int ia_proposal_features(ia_ctx *c, const int *ref_slots, int nref, float *features, int16_t *centres) {
   if (!c) return IA_EINVAL;
-  int rc = check_refs(c, ref_slots, nref);
+  int rc = iah::check_refs(c, ref_slots, nref);
   if (rc) return rc;
   if (nref > 3) return fail(c, IA_EUNSUPPORTED, "the learned proposal takes at most 3 references");
   rc = ensure_tool_buffers(c);

@@ -1,0 +1,334 @@
+// batch.hip — many analysis contexts (streams of any sizes) through one set of launches per frame
+// step, optionally captured once as a hipGraph and replayed.
+//
+// A step enqueues, on the batch's stream:
+//   1. the copy of the per-frame table (AnParams, FsArgs, half-sample and copy jobs per stream) from
+//      pinned host memory to the device;
+//   2. the reset of the analysis queue (a one-word kernel: no memset nodes in the graph);
+//   3. the exhaustive search of every (stream, macroblock, reference), one launch per tile shape
+//      (streams of IA_SEARCH_FULL; the autotuner sets each context's shape);
+//   4. the analysis: a pool of workgroups taking (stream, slice) tasks from a queue, longest first;
+//   5. the promotion of each stream's reconstruction to a reference slot: its half-sample planes
+//      (one batched launch) and its chroma planes and macroblock records (one batched copy).
+// Every kernel reads its per-stream arguments from the device table, so one captured graph serves
+// every frame: the grids cover the largest case (all references of every stream) and the blocks of
+// a stream's unused references, or of a stream that promotes nothing, return at once.
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "ctx.h"
+
+namespace {
+
+struct CopyJob {
+  uint8_t *dst;                // null: nothing to copy this step
+  const uint8_t *src;
+  long long bytes;
+  int first, pad;              // first block of the job in the launch
+};
+
+constexpr int CP_THREADS = 256, CP_UNITS = 4;                            // 16-byte units per thread
+constexpr long long CP_CHUNK = (long long)CP_THREADS * CP_UNITS * 16;  // bytes per block
+
+// Block b copies chunk b - first of the job whose chunks hold it (jobs by `first`).
+__global__ __launch_bounds__(CP_THREADS) void copy_batch_kernel(const CopyJob *jobs, int njobs) {
+  const int b = blockIdx.x;
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].first <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const CopyJob &j = jobs[lo];
+  if (!j.dst) return;
+  const long long base = (long long)(b - j.first) * CP_CHUNK;
+#pragma unroll
+  for (int k = 0; k < CP_UNITS; k++) {
+    const long long o = base + ((long long)k * CP_THREADS + threadIdx.x) * 16;
+    if (o + 16 <= j.bytes) {
+      *reinterpret_cast<uint4 *>(j.dst + o) = *reinterpret_cast<const uint4 *>(j.src + o);
+    } else {
+      for (long long q = o; q < j.bytes && q < o + 16; q++) j.dst[q] = j.src[q];
+    }
+  }
+}
+
+__global__ void reset_word(int *p) {
+  if (threadIdx.x == 0) *p = 0;
+}
+
+size_t align16(size_t n) { return (n + 15) & ~(size_t)15; }
+
+}  // namespace
+
+struct ia_batch {
+  std::vector<ia_ctx *> ctx;
+  int n = 0;
+  hipStream_t stream = nullptr;
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  // the per-frame table, pinned on the host and mirrored on the device
+  size_t tab_bytes = 0, off_fs = 0, off_hp = 0, off_cp = 0;
+  uint8_t *h_tab = nullptr, *d_tab = nullptr;
+  // fixed tables: analysis tasks, queue head, search launch members
+  int *d_fixed = nullptr;
+  int ntasks = 0, workers = 0, hp_blocks = 0, cp_blocks = 0;
+  struct Group {
+    int variant, max_range, nmem, blocks;
+    size_t offs, sidx;  // word offsets into d_fixed
+  };
+  std::vector<Group> groups;
+  size_t head = 0, tasks = 0;  // word offsets into d_fixed
+  std::string err;
+
+  ia::AnParams *an_h() { return reinterpret_cast<ia::AnParams *>(h_tab); }
+  ia::FsArgs *fs_h() { return reinterpret_cast<ia::FsArgs *>(h_tab + off_fs); }
+  ia::HpJob *hp_h() { return reinterpret_cast<ia::HpJob *>(h_tab + off_hp); }
+  CopyJob *cp_h() { return reinterpret_cast<CopyJob *>(h_tab + off_cp); }
+};
+
+namespace {
+
+int bfail(ia_batch *b, int code, const std::string &msg) {
+  b->err = msg;
+  return code;
+}
+
+#define IA_BHIP(b, call)                                                                       \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess) return bfail((b), IA_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int enqueue(ia_batch *b) {
+  hipStream_t st = b->stream;
+  IA_BHIP(b, hipMemcpyAsync(b->d_tab, b->h_tab, b->tab_bytes, hipMemcpyHostToDevice, st));
+  int *fixed = b->d_fixed;
+  hipLaunchKernelGGL(reset_word, dim3(1), dim3(64), 0, st, fixed + b->head);
+  IA_BHIP(b, hipGetLastError());
+  const ia::FsArgs *fs_d = reinterpret_cast<const ia::FsArgs *>(b->d_tab + b->off_fs);
+  for (const auto &g : b->groups)
+    IA_BHIP(b, ia::launch_fullsearch_batch(fs_d, fixed + g.offs, fixed + g.sidx, g.nmem, g.blocks, g.variant,
+                                           g.max_range, st));
+  IA_BHIP(b, ia::launch_analyse_batch(reinterpret_cast<const ia::AnParams *>(b->d_tab), fixed + b->tasks, b->ntasks,
+                                      fixed + b->head, b->workers, st));
+  IA_BHIP(b, ia::launch_halfpel_batch(reinterpret_cast<const ia::HpJob *>(b->d_tab + b->off_hp), b->n, b->hp_blocks,
+                                      st));
+  hipLaunchKernelGGL(copy_batch_kernel, dim3(b->cp_blocks), dim3(CP_THREADS), 0, st,
+                     reinterpret_cast<const CopyJob *>(b->d_tab + b->off_cp), 3 * b->n);
+  IA_BHIP(b, hipGetLastError());
+  return IA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ia_batch_create(ia_batch **out, ia_ctx *const *ctxs, int n, int use_graph) {
+  if (!out || !ctxs || n < 1 || n > 65535) return IA_EINVAL;
+  *out = nullptr;
+  for (int i = 0; i < n; i++) {
+    if (!ctxs[i]) return IA_EINVAL;
+    if (ctxs[i]->cfg.search_mode == IA_SEARCH_LEARNED) return IA_EUNSUPPORTED;
+    for (int j = 0; j < i; j++)
+      if (ctxs[j] == ctxs[i]) return IA_EINVAL;
+  }
+  ia_batch *b = new ia_batch();
+  b->ctx.assign(ctxs, ctxs + n);
+  b->n = n;
+  b->use_graph = use_graph != 0;
+  // the per-frame table
+  b->off_fs = align16((size_t)n * sizeof(ia::AnParams));
+  b->off_hp = b->off_fs + align16((size_t)n * sizeof(ia::FsArgs));
+  b->off_cp = b->off_hp + align16((size_t)n * sizeof(ia::HpJob));
+  b->tab_bytes = b->off_cp + align16((size_t)3 * n * sizeof(CopyJob));
+  // analysis tasks, longest slice first (ties by stream, then slice)
+  struct Task {
+    int mbs, s, k;
+  };
+  std::vector<Task> tv;
+  for (int s = 0; s < n; s++) {
+    const ia_ctx *c = ctxs[s];
+    const int ns = std::min(c->cfg.slices, c->g.mb_h);
+    for (int k = 0; k < ns; k++)
+      tv.push_back({((k + 1) * c->g.mb_h / ns - k * c->g.mb_h / ns) * c->g.mb_w, s, k});
+  }
+  std::stable_sort(tv.begin(), tv.end(), [](const Task &x, const Task &y) { return x.mbs > y.mbs; });
+  std::vector<int> fixed;
+  b->head = fixed.size();
+  fixed.push_back(0);
+  b->tasks = fixed.size();
+  for (const Task &t : tv) fixed.push_back(t.s << 16 | t.k);
+  b->ntasks = (int)tv.size();
+  // the exhaustive search: one launch per (tile shape, window class) of the full-search streams
+  for (int s = 0; s < n; s++) {
+    const ia_ctx *c = ctxs[s];
+    if (c->cfg.search_mode != IA_SEARCH_FULL) continue;
+    const int v = c->fs_variant >= 0 && c->fs_variant < ia::fullsearch_variants() ? c->fs_variant
+                                                                                   : ia::fullsearch_default_variant();
+    const int wide = c->cfg.search_range > 32;
+    ia_batch::Group *g = nullptr;
+    for (auto &x : b->groups)
+      if (x.variant == v && (x.max_range > 32) == (wide != 0)) g = &x;
+    if (!g) {
+      b->groups.push_back({v, 0, 0, 0, 0, 0});
+      g = &b->groups.back();
+    }
+    g->max_range = std::max(g->max_range, c->cfg.search_range);
+  }
+  for (auto &g : b->groups) {
+    std::vector<int> offs, sidx;
+    int blocks = 0;
+    for (int s = 0; s < n; s++) {
+      const ia_ctx *c = ctxs[s];
+      if (c->cfg.search_mode != IA_SEARCH_FULL) continue;
+      const int v = c->fs_variant >= 0 && c->fs_variant < ia::fullsearch_variants() ? c->fs_variant
+                                                                                     : ia::fullsearch_default_variant();
+      if (v != g.variant || (c->cfg.search_range > 32) != (g.max_range > 32)) continue;
+      const int nmb = c->g.mb_w * c->g.mb_h, nmbp = (nmb + 7) / 8 * 8;
+      offs.push_back(blocks);
+      sidx.push_back(s);
+      blocks += nmbp * c->cfg.nrefs;
+    }
+    g.nmem = (int)offs.size();
+    g.blocks = blocks;
+    g.offs = fixed.size();
+    fixed.insert(fixed.end(), offs.begin(), offs.end());
+    g.sidx = fixed.size();
+    fixed.insert(fixed.end(), sidx.begin(), sidx.end());
+  }
+  // promotion launches: tiles of each stream's half-sample planes, chunks of its copies
+  int hp_blocks = 0, cp_blocks = 0;
+  for (int s = 0; s < n; s++) {
+    const Geom &g = ctxs[s]->g;
+    hp_blocks += ((g.sw + ia::HP_TW - 1) / ia::HP_TW) * ((g.sh + ia::HP_TH - 1) / ia::HP_TH);
+    const long long clen = (long long)g.cw * g.ch, mlen = (long long)g.mb_w * g.mb_h * sizeof(ia_mbinfo);
+    cp_blocks += 2 * (int)((clen + CP_CHUNK - 1) / CP_CHUNK) + (int)((mlen + CP_CHUNK - 1) / CP_CHUNK);
+  }
+  b->hp_blocks = hp_blocks;
+  b->cp_blocks = cp_blocks;
+  const int workers = ia::analyse_batch_workers();
+  b->workers = std::max(1, std::min(workers > 0 ? workers : 1024, b->ntasks));
+  bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
+            hipHostMalloc((void **)&b->h_tab, b->tab_bytes, hipHostMallocDefault) == hipSuccess &&
+            hipMalloc((void **)&b->d_tab, b->tab_bytes) == hipSuccess &&
+            hipMalloc((void **)&b->d_fixed, fixed.size() * sizeof(int)) == hipSuccess &&
+            hipMemcpy(b->d_fixed, fixed.data(), fixed.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+  if (!ok) {
+    ia_batch_destroy(b);
+    return IA_ENOMEM;
+  }
+  memset(b->h_tab, 0, b->tab_bytes);
+  *out = b;
+  return IA_OK;
+}
+
+void ia_batch_destroy(ia_batch *b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->h_tab) (void)hipHostFree(b->h_tab);
+  (void)hipFree(b->d_tab);
+  (void)hipFree(b->d_fixed);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+const char *ia_batch_last_error(const ia_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int ia_batch_step(ia_batch *b, const ia_frame *frames) {
+  if (!b || !frames) return IA_EINVAL;
+  // the previous step has read the table; the contexts' own streams (uploads) are done
+  IA_BHIP(b, hipStreamSynchronize(b->stream));
+  for (ia_ctx *c : b->ctx) IA_BHIP(b, hipStreamSynchronize(c->stream));
+  for (int s = 0; s < b->n; s++) {
+    ia_ctx *c = b->ctx[s];
+    const ia_frame &f = frames[s];
+    if (f.promote_slot < -1 || f.promote_slot >= c->cfg.nrefs)
+      return bfail(b, IA_EINVAL, "stream " + std::to_string(s) + ": promote_slot out of range");
+    const uint8_t *src[3] = {f.y, f.cb, f.cr};
+    const int nref = f.kind == IA_PIC_P ? f.nref : 0;
+    int rc = iah::analysis_params(c, f.qp, f.kind, f.ref_slots, nref, src, false, &b->an_h()[s]);
+    if (rc) return bfail(b, rc, "stream " + std::to_string(s) + ": " + c->err);
+    ia::FsArgs &fa = b->fs_h()[s];
+    if (c->cfg.search_mode == IA_SEARCH_FULL) {
+      iah::search_args(c, f.ref_slots, nref, f.y, &fa);
+    } else {
+      memset(&fa, 0, sizeof fa);
+      fa.mb_w = c->g.mb_w;
+      fa.mb_h = c->g.mb_h;
+    }
+    const Geom &g = c->g;
+    ia::HpJob &hj = b->hp_h()[s];
+    hj.luma = c->rec[0];
+    hj.planes = f.promote_slot >= 0 ? c->slots[f.promote_slot].hp : nullptr;
+    hj.pw = g.pw;
+    hj.ph = g.ph;
+    hj.tiles_x = (g.sw + ia::HP_TW - 1) / ia::HP_TW;
+    CopyJob *cj = b->cp_h() + 3 * s;
+    const long long clen = (long long)g.cw * g.ch, mlen = (long long)g.mb_w * g.mb_h * sizeof(ia_mbinfo);
+    IaSlot *ps = f.promote_slot >= 0 ? &c->slots[f.promote_slot] : nullptr;
+    cj[0] = {ps ? ps->cb : nullptr, c->rec[1], clen, 0, 0};
+    cj[1] = {ps ? ps->cr : nullptr, c->rec[2], clen, 0, 0};
+    cj[2] = {ps ? reinterpret_cast<uint8_t *>(ps->mbs) : nullptr, reinterpret_cast<const uint8_t *>(c->mbs), mlen, 0, 0};
+  }
+  // first blocks of the promotion jobs (the same every step)
+  int hfirst = 0, cfirst = 0;
+  for (int s = 0; s < b->n; s++) {
+    const Geom &g = b->ctx[s]->g;
+    b->hp_h()[s].first = hfirst;
+    hfirst += ((g.sw + ia::HP_TW - 1) / ia::HP_TW) * ((g.sh + ia::HP_TH - 1) / ia::HP_TH);
+    for (int k = 0; k < 3; k++) {
+      CopyJob &j = b->cp_h()[3 * s + k];
+      j.first = cfirst;
+      cfirst += (int)((j.bytes + CP_CHUNK - 1) / CP_CHUNK);
+    }
+  }
+  if (b->use_graph) {
+    if (!b->exec) {
+      IA_BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      int rc = enqueue(b);
+      hipGraph_t graph = nullptr;
+      hipError_t e = hipStreamEndCapture(b->stream, &graph);
+      if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      if (e != hipSuccess) return bfail(b, IA_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      b->graph = graph;
+      IA_BHIP(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    IA_BHIP(b, hipGraphLaunch(b->exec, b->stream));
+  } else {
+    int rc = enqueue(b);
+    if (rc) return rc;
+  }
+  // the contexts' state after the step
+  for (int s = 0; s < b->n; s++) {
+    ia_ctx *c = b->ctx[s];
+    const ia_frame &f = frames[s];
+    const int nref = f.kind == IA_PIC_P ? f.nref : 0;
+    if (c->cfg.search_mode == IA_SEARCH_FULL && nref > 0) {
+      c->fs_nref = nref;
+      for (int r = 0; r < nref; r++) c->fs_slots[r] = f.ref_slots[r];
+      c->fs_valid = true;
+    } else {
+      c->fs_valid = false;
+    }
+    c->an_valid = true;
+    if (f.promote_slot >= 0) c->slots[f.promote_slot].loaded = true;
+  }
+  return IA_OK;
+}
+
+int ia_batch_sync(ia_batch *b) {
+  if (!b) return IA_EINVAL;
+  IA_BHIP(b, hipStreamSynchronize(b->stream));
+  return IA_OK;
+}
+
+}  // extern "C"

@@ -46,5 +46,21 @@ struct FsArgs {
 hipError_t launch_fullsearch(const FsArgs &a, int variant, hipStream_t stream);
 int fullsearch_variants();
 const char *fullsearch_variant_name(int v);
+int fullsearch_default_variant();
+// The search of many pictures in one launch (args in device memory): member k of the launch
+// (offs[k] <= block < offs[k+1], each member's block count 8 * ceil(nmb / 8) * its most references)
+// searches picture sidx[k]; all members' ranges are at most max_range and share the tile `variant`.
+hipError_t launch_fullsearch_batch(const FsArgs *args, const int *offs, const int *sidx, int nmem, int nblocks,
+                                   int variant, int max_range, hipStream_t stream);
+
+// One picture's half-sample build in a batched launch: tiles [first, first + tiles) of the launch
+// build `planes` from `luma`; a null `planes` skips the picture this time.
+struct HpJob {
+  const uint8_t *luma;
+  uint8_t *planes;
+  int pw, ph, tiles_x, first;
+};
+constexpr int HP_TW = 64, HP_TH = 16;  // tile of the half-sample kernels
+hipError_t launch_halfpel_batch(const HpJob *jobs, int njobs, int nblocks, hipStream_t stream);
 
 }  // namespace ia

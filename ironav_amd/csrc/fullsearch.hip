@@ -225,6 +225,31 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   search_one<K, NT, RAWS>(a, rows_lds, mb, blockIdx.y, raw);
 }
 
+// Many pictures at once. Workgroup b belongs to member k of the launch (offs[k] <= b < offs[k+1];
+// each member's count a multiple of 8, so a block's XCD is that of its index within the member),
+// which searches picture sidx[k] with args[sidx[k]]: x = b - offs[k] = ref * nmbp + i, nmbp = the
+// macroblocks rounded up to a multiple of 8, i mapped to a macroblock by XCD band as above.
+// Workgroups past the picture's macroblocks or its references of this step return at once.
+template <int K, int NT, int RAWS, int WPE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void fullsearch_batch_kernel(
+    const FsArgs *args, const int *offs, const int *sidx, int nmem, int rows_lds) {
+  extern __shared__ uint32_t raw[];
+  const int b = blockIdx.x;
+  int lo = 0, hi = nmem - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (offs[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const FsArgs &a = args[sidx[lo]];
+  const int nmb = a.mb_w * a.mb_h, nmbp = (nmb + NXCD - 1) / NXCD * NXCD;
+  const int x = b - offs[lo], r = x / nmbp, i = x - r * nmbp;
+  if (r >= a.nref) return;
+  const int band = nmbp / NXCD, mb = (i % NXCD) * band + i / NXCD;
+  if (mb >= nmb) return;
+  search_one<K, NT, RAWS>(a, rows_lds, mb, r, raw);
+}
+
 // Block i searches entry list[i] = mb * 4 + ref when i < *count and returns at once otherwise (the
 // grid covers every possible entry: the count is known on the device only). Block 0 clears *reset.
 template <int K, int NT, int RAWS, int WPE>
@@ -243,10 +268,13 @@ struct Variant {
   int k, nt;
   void (*fn32)(FsArgs, int);  // ranges up to 32
   void (*fn64)(FsArgs, int);  // ranges up to 64
+  void (*bfn32)(const FsArgs *, const int *, const int *, int, int);  // many pictures, ranges up to 32
+  void (*bfn64)(const FsArgs *, const int *, const int *, int, int);
 };
 
-#define IA_FS_VARIANT(K, NT, W)                                                                         \
-  {"k" #K "_t" #NT "_w" #W, K, NT, fullsearch_kernel<K, NT, raws_for(32), W>, fullsearch_kernel<K, NT, raws_for(64), W>}
+#define IA_FS_VARIANT(K, NT, W)                                                                          \
+  {"k" #K "_t" #NT "_w" #W, K, NT, fullsearch_kernel<K, NT, raws_for(32), W>, fullsearch_kernel<K, NT, raws_for(64), W>, \
+   fullsearch_batch_kernel<K, NT, raws_for(32), W>, fullsearch_batch_kernel<K, NT, raws_for(64), W>}
 const Variant VARIANTS[] = {
     IA_FS_VARIANT(4, 256, 3), IA_FS_VARIANT(6, 256, 3), IA_FS_VARIANT(8, 256, 2), IA_FS_VARIANT(4, 128, 3),
     IA_FS_VARIANT(6, 128, 3), IA_FS_VARIANT(6, 256, 2),
@@ -274,6 +302,22 @@ hipError_t launch_fullsearch(const FsArgs &a, int variant, hipStream_t stream) {
   const size_t shmem = (size_t)rows_lds * raws * sizeof(uint32_t);
   dim3 grid(a.mb_w * a.mb_h, a.nref);
   hipLaunchKernelGGL(small ? v.fn32 : v.fn64, grid, dim3(v.nt), shmem, stream, a, rows_lds);
+  return hipGetLastError();
+}
+
+int fullsearch_default_variant() { return DEFAULT_VARIANT; }
+
+hipError_t launch_fullsearch_batch(const FsArgs *args, const int *offs, const int *sidx, int nmem, int nblocks,
+                                   int variant, int max_range, hipStream_t stream) {
+  if (variant < 0 || variant >= NVARIANTS) variant = DEFAULT_VARIANT;
+  if (max_range < 1 || max_range > RMAX || nmem < 1 || nblocks < 1) return hipErrorInvalidValue;
+  const Variant &v = VARIANTS[variant];
+  const bool small = max_range <= 32;
+  const int raws = small ? raws_for(32) : raws_for(64);
+  const int rows_lds = 2 * max_range + 1 + v.k + 15;
+  const size_t shmem = (size_t)rows_lds * raws * sizeof(uint32_t);
+  hipLaunchKernelGGL(small ? v.bfn32 : v.bfn64, dim3(nblocks), dim3(v.nt), shmem, stream, args, offs, sidx, nmem,
+                     rows_lds);
   return hipGetLastError();
 }
 

@@ -15,22 +15,28 @@
 namespace ia {
 namespace {
 
-constexpr int TW = 64, TH = 16;
+constexpr int TW = HP_TW, TH = HP_TH;
 
 __device__ __forceinline__ int tap6(int a, int b, int c, int d, int e, int f) {
   return a - 5 * b + 20 * c + 20 * d - 5 * e + f;
 }
 
-__global__ __launch_bounds__(256) void halfpel_kernel(const uint8_t *__restrict__ luma, int pw, int ph,
-                                                       uint8_t *__restrict__ planes) {
-  const int sw = pw + 2 * PAD, sh = ph + 2 * PAD;
-  const size_t n = (size_t)sw * sh;
-  const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+struct HpLds {
   // G over rows y0-2 .. y0+TH+2 and columns x0-2 .. x0+TW+2 (TH+5 x TW+5), fetched at clamped
   // coordinates: equal to G inside the plane, and only feeding zeroed outputs outside it.
-  __shared__ int16_t gt[TH + 5][TW + 8];
+  int16_t gt[TH + 5][TW + 8];
   // unrounded horizontal sums over the same rows, columns x0 .. x0+TW-1
-  __shared__ int32_t b1t[TH + 5][TW];
+  int32_t b1t[TH + 5][TW];
+};
+
+// Tile (bx, by) of the four planes of `luma` (256 threads).
+__device__ __forceinline__ void halfpel_tile(HpLds &L, const uint8_t *__restrict__ luma, int pw, int ph,
+                                             uint8_t *__restrict__ planes, int bx, int by) {
+  const int sw = pw + 2 * PAD, sh = ph + 2 * PAD;
+  const size_t n = (size_t)sw * sh;
+  const int x0 = bx * TW, y0 = by * TH;
+  auto &gt = L.gt;
+  auto &b1t = L.b1t;
   const int tid = threadIdx.x;
   for (int i = tid; i < (TH + 5) * (TW + 5); i += 256) {
     const int r = i / (TW + 5), c = i % (TW + 5);
@@ -78,7 +84,34 @@ __global__ __launch_bounds__(256) void halfpel_kernel(const uint8_t *__restrict_
   *reinterpret_cast<uint32_t *>(planes + 3 * n + o) = oj;
 }
 
+__global__ __launch_bounds__(256) void halfpel_kernel(const uint8_t *__restrict__ luma, int pw, int ph,
+                                                       uint8_t *__restrict__ planes) {
+  __shared__ HpLds lds;
+  halfpel_tile(lds, luma, pw, ph, planes, blockIdx.x, blockIdx.y);
+}
+
+// Many pictures: block b builds tile b - first of the job whose tiles hold it (jobs by `first`).
+__global__ __launch_bounds__(256) void halfpel_batch_kernel(const HpJob *jobs, int njobs) {
+  __shared__ HpLds lds;
+  const int b = blockIdx.x;
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].first <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const HpJob &j = jobs[lo];
+  if (!j.planes) return;
+  const int t = b - j.first;
+  halfpel_tile(lds, j.luma, j.pw, j.ph, j.planes, t % j.tiles_x, t / j.tiles_x);
+}
+
 }  // namespace
+
+hipError_t launch_halfpel_batch(const HpJob *jobs, int njobs, int nblocks, hipStream_t stream) {
+  hipLaunchKernelGGL(halfpel_batch_kernel, dim3(nblocks), dim3(256), 0, stream, jobs, njobs);
+  return hipGetLastError();
+}
 
 hipError_t launch_halfpel(const uint8_t *luma, int pw, int ph, uint8_t *planes, hipStream_t stream) {
   const int sw = pw + 2 * PAD, sh = ph + 2 * PAD;

@@ -188,6 +188,19 @@ class Analyser:
                                               abi.ptr(rcb), abi.ptr(rcr), abi.ptr(mbs)))
         return Analysis(codes, stats, Picture(ry, rcb, rcr), mbs, self.mb_w, self.mb_h, self.rows)
 
+    def read_analysis(self):
+        """The results of the last analysis of this context (e.g. after a batch step)."""
+        n = self.mb_w * self.mb_h
+        codes = np.zeros(n, dtype=abi.MBCODE_DTYPE)
+        stats = np.zeros(n, dtype=abi.MBSTATS_DTYPE)
+        mbs = np.zeros(n, dtype=abi.MBINFO_DTYPE)
+        ry = np.empty((self.ph, self.pw), dtype=np.uint8)
+        rcb = np.empty((self.ph // 2, self.pw // 2), dtype=np.uint8)
+        rcr = np.empty_like(rcb)
+        self._ok(library().ia_read_analysis(self._h, abi.ptr(codes), abi.ptr(stats), abi.ptr(ry), abi.ptr(rcb),
+                                            abi.ptr(rcr), abi.ptr(mbs)))
+        return Analysis(codes, stats, Picture(ry, rcb, rcr), mbs, self.mb_w, self.mb_h, self.rows)
+
     def load_proposal_weights(self, params):
         """Loads the learned proposal's CNN parameters (abi.CNN_PARAMS float32, see proposal.py)."""
         params = np.ascontiguousarray(params, dtype=np.float32).ravel()
@@ -221,3 +234,61 @@ class Analyser:
 def fullsearch_variants():
     lib = library()
     return [lib.ia_fullsearch_variant_name(i).decode() for i in range(lib.ia_fullsearch_variants())]
+
+
+class Batch:
+    """Streams of any sizes (Analysers) through one set of launches per frame step; with use_graph the
+    step is one captured hipGraph, replayed every frame."""
+
+    def __init__(self, analysers, use_graph=True):
+        self.analysers = list(analysers)
+        n = len(self.analysers)
+        arr = (C.c_void_p * n)(*[a._h.value for a in self.analysers])
+        h = C.c_void_p()
+        rc = library().ia_batch_create(C.byref(h), arr, n, int(use_graph))
+        if rc != abi.IA_OK:
+            if rc == abi.IA_EINVAL:
+                raise ValueError("ia_batch_create: invalid arguments")
+            raise AnalysisError(f"ia_batch_create failed ({rc})")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().ia_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def step(self, frames):
+        """frames[i] for analyser i: dict(qp, kind, refs=[slots newest first], promote=slot or -1,
+        planes=(y, cb, cr) device addresses or None for the analyser's uploaded source)."""
+        if len(frames) != len(self.analysers):
+            raise ValueError("one frame per stream")
+        arr = (abi.Frame * len(frames))()
+        for f, d in zip(arr, frames):
+            refs = list(d.get("refs", ()))
+            f.qp, f.kind, f.nref = d["qp"], d["kind"], len(refs)
+            for i, r in enumerate(refs):
+                f.ref_slots[i] = r
+            f.promote_slot = d.get("promote", -1)
+            planes = d.get("planes")
+            if planes is not None:
+                f.y, f.cb, f.cr = planes
+        rc = library().ia_batch_step(self._h, arr)
+        if rc != abi.IA_OK:
+            msg = library().ia_batch_last_error(self._h).decode(errors="replace")
+            if rc == abi.IA_EINVAL:
+                raise ValueError(msg)
+            raise AnalysisError(f"error {rc}: {msg}")
+
+    def sync(self):
+        rc = library().ia_batch_sync(self._h)
+        if rc != abi.IA_OK:
+            raise AnalysisError(library().ia_batch_last_error(self._h).decode(errors="replace"))

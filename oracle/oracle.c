@@ -2240,4 +2240,5 @@ void or_abi_sizes(int32_t *out) {
   out[2] = (int32_t)sizeof(ia_mbcode);
   out[3] = (int32_t)sizeof(ia_mbstats);
   out[4] = (int32_t)sizeof(ia_fsresult);
+  out[5] = (int32_t)sizeof(ia_frame);
 }

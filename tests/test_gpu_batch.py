@@ -1,0 +1,126 @@
+"""The batched path on the GPU: streams of different sizes and settings through one set of launches
+per frame step (eager and as a replayed hipGraph) give, per stream and per frame, exactly what the
+single-context path gives (itself bit-exact with the oracle), including the on-device promotion of
+each reconstruction to a reference."""
+import numpy as np
+import pytest
+
+from ironav_amd import abi, clipgen
+from ironav_amd.h264a import Analyser, Batch, Picture
+
+pytestmark = pytest.mark.gpu
+
+# (width, height, seed, search range, nrefs, slices, cabac, transform8)
+STREAMS = [(64, 48, 1, 16, 2, 1, 1, 1), (96, 64, 2, 16, 2, 2, 1, 1), (160, 96, 3, 32, 3, 3, 1, 0),
+           (48, 32, 4, 8, 1, 2, 0, 0), (128, 128, 5, 16, 2, 4, 1, 1)]
+FRAMES = 4
+
+
+def clip(w, h, seed):
+    return [Picture(*clipgen.pad_planes(*f)) for f in clipgen.make_clip(clipgen.ClipSpec(w, h, FRAMES, seed))]
+
+
+def schedule(nrefs, t):
+    """IDR at 0, then P pictures against the newest min(t, nrefs) references; each picture becomes a
+    reference in slot t % nrefs."""
+    refs = [(t - 1 - k) % nrefs for k in range(min(t, nrefs))]
+    return (25 if t == 0 else 28 + (t % 3) - 1), (abi.PIC_IDR if t == 0 else abi.PIC_P), refs, t % nrefs
+
+
+def analyser(w, h, rng, nrefs, slices, cabac, t8):
+    return Analyser(w, h, nrefs=nrefs, slices=slices, cabac=bool(cabac), transform8=bool(t8),
+                    search_mode=abi.SEARCH_FULL, search_range=rng)
+
+
+def single_runs():
+    out = []
+    for (w, h, seed, rng, nrefs, slices, cabac, t8) in STREAMS:
+        pics = clip(w, h, seed)
+        res = []
+        with analyser(w, h, rng, nrefs, slices, cabac, t8) as an:
+            for t in range(FRAMES):
+                qp, kind, refs, promote = schedule(nrefs, t)
+                r = an.analyse(pics[t], qp, kind, refs)
+                an.set_reference(promote, r.recon, r.mbs)
+                res.append(r)
+        out.append(res)
+    return out
+
+
+def batch_runs(use_graph, device_planes=False):
+    import torch
+
+    clips = [clip(w, h, seed) for (w, h, seed, *_rest) in STREAMS]
+    ans = [analyser(w, h, rng, nrefs, slices, cabac, t8) for (w, h, seed, rng, nrefs, slices, cabac, t8) in STREAMS]
+    out = [[] for _ in STREAMS]
+    keep = []
+    with Batch(ans, use_graph=use_graph) as b:
+        for t in range(FRAMES):
+            frames = []
+            for i, an in enumerate(ans):
+                qp, kind, refs, promote = schedule(STREAMS[i][4], t)
+                d = dict(qp=qp, kind=kind, refs=refs, promote=promote)
+                pic = clips[i][t]
+                if device_planes:
+                    planes = [torch.from_numpy(p).cuda() for p in (pic.y, pic.cb, pic.cr)]
+                    keep.append(planes)
+                    d["planes"] = tuple(p.data_ptr() for p in planes)
+                else:
+                    an.upload(pic)
+                frames.append(d)
+            torch.cuda.synchronize()
+            b.step(frames)
+            b.sync()
+            for i, an in enumerate(ans):
+                out[i].append(an.read_analysis())
+    for an in ans:
+        an.close()
+    return out
+
+
+def same(a, b):
+    return (np.array_equal(a.codes, b.codes) and np.array_equal(a.stats, b.stats) and np.array_equal(a.mbs, b.mbs)
+            and np.array_equal(a.recon.y, b.recon.y) and np.array_equal(a.recon.cb, b.recon.cb)
+            and np.array_equal(a.recon.cr, b.recon.cr))
+
+
+@pytest.fixture(scope="module")
+def single():
+    return single_runs()
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_matches_single_streams(single, use_graph):
+    got = batch_runs(use_graph)
+    for i in range(len(STREAMS)):
+        for t in range(FRAMES):
+            assert same(got[i][t], single[i][t]), (STREAMS[i], t)
+
+
+def test_batch_graph_from_device_planes(single):
+    """Pictures handed as device planes (no upload); the graph replayed every frame."""
+    got = batch_runs(True, device_planes=True)
+    for i in range(len(STREAMS)):
+        for t in range(FRAMES):
+            assert same(got[i][t], single[i][t]), (STREAMS[i], t)
+
+
+def test_single_stream_chain_matches_oracle(oracle):
+    """The single-context chain the batch is compared with, pinned to the oracle for one stream."""
+    w, h, seed, rng, nrefs, slices, cabac, t8 = STREAMS[1]
+    pics = clip(w, h, seed)
+    with analyser(w, h, rng, nrefs, slices, cabac, t8) as an:
+        refs_host = {}
+        for t in range(3):
+            qp, kind, refs, promote = schedule(nrefs, t)
+            if kind == abi.PIC_P:
+                an.upload(pics[t])
+                an.motion_search(refs)
+                fs = an.read_motion(len(refs))
+            r = an.analyse(pics[t], qp, kind, refs)
+            want = oracle.analyse_picture(an.cfg, pics[t], qp, kind, [refs_host[s] for s in refs],
+                                          fs if kind == abi.PIC_P else None)
+            assert np.array_equal(r.codes, want[0]) and np.array_equal(r.stats, want[1]), t
+            assert np.array_equal(r.recon.y, want[2][0]) and np.array_equal(r.mbs, want[3]), t
+            an.set_reference(promote, r.recon, r.mbs)
+            refs_host[promote] = (r.recon, r.mbs)

@@ -289,8 +289,8 @@ def test_intra_nxn_directional_symmetries(oracle):
 
 def test_abi_struct_layouts(oracle):
     """The ctypes mirror of include/ironav_h264a.h lays the structs out as the C compiler does."""
-    sizes = np.zeros(5, dtype=np.int32)
+    sizes = np.zeros(6, dtype=np.int32)
     oracle.lib().or_abi_sizes(sizes.ctypes.data)
     assert list(sizes) == [C.sizeof(abi.Config), C.sizeof(abi.MbInfo), C.sizeof(abi.MbCode), C.sizeof(abi.MbStats),
-                           C.sizeof(abi.FsResult)]
+                           C.sizeof(abi.FsResult), C.sizeof(abi.Frame)]
     assert abi.MBCODE_DTYPE.itemsize == C.sizeof(abi.MbCode)
