@@ -94,11 +94,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (the counters' scale for the traffic figures)")
+    ap.add_argument("--config", choices=["learned", "batched"], default="learned",
+                    help="learned: the learned proposal vs the exhaustive search at 1080p (the metric); "
+                         "batched: many streams of mixed sizes through the batched analysis")
+    ap.add_argument("--streams", type=int, default=64, help="batched: streams")
+    ap.add_argument("--frames", type=int, default=30, help="batched: frames per stream")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (streams or slice bands would "
               "shard across GPUs with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.config == "batched":
+        return bench_batched(args)
 
     import numpy as np
     import torch
@@ -243,6 +250,117 @@ def main():
                      "note": "launch with fallback off: cost volume, CNN, top-8 and refinement of every "
                              "(macroblock, reference)"},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def digest(res):
+    import hashlib
+
+    h = hashlib.blake2b(digest_size=16)
+    for a in (res.codes, res.stats, res.mbs, res.recon.y, res.recon.cb, res.recon.cr):
+        h.update(a.tobytes())
+    return h.hexdigest()
+
+
+def bench_batched(args):
+    """Streams of mixed sizes (640x360 .. 3840x2160, each its own seed, objects +-24 px/picture and a
+    global pan of +-4) through the batched analysis: per frame step one exhaustive search (+-32) of
+    every (stream, macroblock, reference), the queued slice workers of the exact analysis and the
+    on-device promotion of every reconstruction to a reference, replayed as one hipGraph. Pictures
+    are generated on the device and stay resident; the timed region is the P frames 2..F-1 (frame 0
+    is the IDR, frame 1 the first P and the graph's capture). Each stream's results at frames 0 and 1
+    are compared with the single-stream path's (digests)."""
+    import numpy as np
+    import torch
+
+    from ironav_amd import abi, autotune, clipgen
+    from ironav_amd.h264a import Analyser, Batch, Picture
+
+    sizes = [(640, 360), (854, 480), (1280, 720), (1920, 1080), (3840, 2160)]
+    rng = np.random.default_rng(64)
+    order = [sizes[i % len(sizes)] for i in range(args.streams)]
+    rng.shuffle(order)
+    specs = [clipgen.ClipSpec(w, h, args.frames, 6400 + i, max_motion=24, pan=4, noise_sigma=2.0)
+             for i, (w, h) in enumerate(order)]
+    nmbs = [((w + 15) // 16) * ((h + 15) // 16) for w, h in order]
+    torch.cuda.init()
+    t0 = time.perf_counter()
+    frames = [list(clipgen.make_clip_device(sp)) for sp in specs]
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    # slices: tasks of about equal length, one per resident analysis workgroup
+    workers = torch.cuda.get_device_properties(0).multi_processor_count * 4
+    target = max(1, -(-sum(nmbs) // workers))
+    slices = [min(max(1, round(n / target)), (h + 15) // 16) for n, (w, h) in zip(nmbs, order)]
+    nrefs = 3
+    table = autotune.load()
+    ans = []
+    for (w, h), ns in zip(order, slices):
+        an = Analyser(w, h, nrefs=nrefs, slices=ns, search_mode=abi.SEARCH_FULL, search_range=32, sub8x8=True)
+        autotune.apply(an, table)
+        ans.append(an)
+
+    def frame_args(i, t):
+        y, cb, cr = frames[i][t]
+        refs = [(t - 1 - k) % nrefs for k in range(min(t, nrefs))]
+        return dict(qp=25 if t == 0 else 28, kind=abi.PIC_IDR if t == 0 else abi.PIC_P, refs=refs,
+                    promote=t % nrefs, planes=(y.data_ptr(), cb.data_ptr(), cr.data_ptr()))
+
+    want = [[None, None] for _ in ans]
+    with Batch(ans, use_graph=True) as b:
+        for t in range(min(2, args.frames)):
+            b.step([frame_args(i, t) for i in range(len(ans))])
+            b.sync()
+            for i, an in enumerate(ans):
+                want[i][t] = digest(an.read_analysis())
+        b.sync()
+        t1 = time.perf_counter()
+        for t in range(2, args.frames):
+            b.step([frame_args(i, t) for i in range(len(ans))])
+        b.sync()
+        dt = time.perf_counter() - t1
+    steps = max(0, args.frames - 2)
+    # per-stream bit-exactness of frames 0 and 1 against the single-stream path
+    equal = 0
+    for i, ((w, h), ns) in enumerate(zip(order, slices)):
+        with Analyser(w, h, nrefs=nrefs, slices=ns, search_mode=abi.SEARCH_FULL, search_range=32, sub8x8=True) as an:
+            autotune.apply(an, table)
+            pics = [Picture(*(p.cpu().numpy() for p in frames[i][t])) for t in range(min(2, args.frames))]
+            ok = True
+            for t, pic in enumerate(pics):
+                a = frame_args(i, t)
+                r = an.analyse(pic, a["qp"], a["kind"], a["refs"])
+                an.set_reference(a["promote"], r.recon, r.mbs)
+                ok = ok and digest(r) == want[i][t]
+            equal += ok
+    for an in ans:
+        an.close()
+    mb_per_step = sum(nmbs)
+    ms = dt * 1000.0 / max(1, steps)
+    out = {
+        "metric": "MB/s batched analysis: streams of mixed sizes, +-32 exhaustive search + exact analysis",
+        "value": round(mb_per_step * steps / dt, 1) if steps else None,
+        "unit": "MB/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 2,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": f"synthetic (device-generated clips, {gen_s:.1f} s; resident in HBM)",
+        "config": {"workload": f"{args.streams} streams x {args.frames} frames, sizes {sorted(set(order))}, IPPP "
+                               "(IDR QP 25, P QP 28), 3 references, +-32 exhaustive search, CABAC + 8x8 transform, "
+                               "one hipGraph per frame step",
+                   "macroblocks_per_step": mb_per_step,
+                   "streams_per_size": {f"{w}x{h}": order.count((w, h)) for w, h in sizes},
+                   "slices": {f"{w}x{h}": s for (w, h), s in zip(order, slices)},
+                   "analysis_workers": workers,
+                   "tile_shapes": {f"{w}x{h}": autotune.choose(w, h, 32, table) for w, h in sizes}},
+        "per_stream_bitexact": {"frames_checked": [0, 1], "streams_equal": equal, "streams": len(ans),
+                                "against": "the single-stream path (itself bit-exact with the oracle)"},
     }
     print(json.dumps(out))
 

@@ -102,6 +102,59 @@ def make_clip(spec: ClipSpec):
         yield y, cb, cr
 
 
+def make_clip_device(spec: ClipSpec, device="cuda"):
+    """The same kind of clips made on a device with torch, for workloads too large to generate on the
+    host: the first scene and its motion are make_clip's (the same seeded draws); the noise comes
+    from torch's generator on the device, and a scene after a cut from a generator seeded with
+    (seed, cut frame), so frames equal make_clip's up to the noise until the first cut. Yields planes
+    padded to whole macroblocks (as pad_planes) as contiguous uint8 tensors on the device."""
+    import torch
+
+    if spec.width % 2 or spec.height % 2:
+        raise ValueError("width and height must be even")
+    rng = np.random.Generator(np.random.PCG64(spec.seed))
+    scene = _scene(rng, spec)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(spec.seed)
+    lut = torch.tensor(_LUT, dtype=torch.int64, device=device)
+    W, H = spec.width, spec.height
+    pw, ph = (W + 15) // 16 * 16, (H + 15) // 16 * 16
+    ys = torch.arange(H, dtype=torch.int64, device=device).view(H, 1)
+    xs = torch.arange(W, dtype=torch.int64, device=device).view(1, W)
+    t0 = 0
+    for t in range(spec.frames):
+        if t in spec.cuts:
+            scene = _scene(np.random.Generator(np.random.PCG64([spec.seed, t])), spec)
+            t0 = t
+        tt = t - t0
+        waves, rects, (panx, pany) = scene
+        bx, by = xs + panx * tt, ys + pany * tt
+        acc = torch.full((H, W), 128 << _LUT_BITS, dtype=torch.int64, device=device)
+        for fx, fy, phs, amp in waves:
+            acc += amp * lut[((fx * bx + fy * by + phs) & 0xFFFF) >> (16 - _LUT_BITS)]
+        img = acc >> _LUT_BITS
+        for rw, rh, x0, y0, vx, vy, tfx, tfy, base, amp in rects:
+            u = torch.remainder(xs - (x0 + vx * tt), W).expand(H, W)
+            v = torch.remainder(ys - (y0 + vy * tt), H).expand(H, W)
+            inside = (u < rw) & (v < rh)
+            phase = ((tfx * u + tfy * v) & 0xFFFF) >> (16 - _LUT_BITS)
+            checker = (((u >> 3) ^ (v >> 3)) & 1) * 24 - 12
+            tex = base + ((amp * lut[phase]) >> _LUT_BITS) + checker
+            img = torch.where(inside, tex, img)
+        noise = torch.round(torch.randn((H, W), generator=gen, device=device) * spec.noise_sigma).to(torch.int64)
+        y = torch.clamp(img + noise, 0, 255).to(torch.int32)
+        avg = (y[0::2, 0::2] + y[1::2, 0::2] + y[0::2, 1::2] + y[1::2, 1::2] + 2) >> 2
+        cb = torch.clamp((avg >> 1) + 72, 0, 255)
+        cr = torch.clamp(184 - (avg >> 1), 0, 255)
+
+        def pad(a, h2, w2):
+            a = torch.cat([a, a[:, -1:].expand(a.shape[0], w2 - a.shape[1])], 1) if w2 > a.shape[1] else a
+            a = torch.cat([a, a[-1:, :].expand(h2 - a.shape[0], a.shape[1])], 0) if h2 > a.shape[0] else a
+            return a.to(torch.uint8).contiguous()
+
+        yield pad(y, ph, pw), pad(cb, ph // 2, pw // 2), pad(cr, ph // 2, pw // 2)
+
+
 def pad_planes(y, cb, cr):
     """Pads planes to whole macroblocks by repeating the last column and row."""
     h, w = y.shape

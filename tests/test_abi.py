@@ -61,3 +61,16 @@ def test_clip_generator_is_deterministic_and_pinned():
     y, cb, cr = clipgen.pad_planes(*next(clipgen.make_clip(clipgen.ClipSpec(40, 26, 1, 5))))
     assert y.shape == (32, 48) and cb.shape == (16, 24)
     assert np.array_equal(y[26:, :40], np.repeat(y[25:26, :40], 6, axis=0))
+
+
+def test_device_clip_generator_matches_host_without_noise():
+    """make_clip_device (torch, here on the CPU) draws the same scene and motion as make_clip: with
+    no noise and no cut the padded planes are identical."""
+    torch = pytest.importorskip("torch")
+    for spec in (clipgen.ClipSpec(96, 64, 3, 11, noise_sigma=0.0),
+                 clipgen.ClipSpec(90, 50, 3, 12, noise_sigma=0.0, pan=3, max_motion=20)):
+        host = [clipgen.pad_planes(*f) for f in clipgen.make_clip(spec)]
+        dev = list(clipgen.make_clip_device(spec, device="cpu"))
+        for fh, fd in zip(host, dev):
+            for a, b in zip(fh, fd):
+                assert isinstance(b, torch.Tensor) and np.array_equal(a, b.numpy())
