@@ -232,7 +232,9 @@ int ia_read_fallback_count(ia_ctx *ctx, int *count);
  * travel in a table copied from pinned host memory by the graph's first node. Each context's
  * results are read with ia_read_analysis / ia_read_motion after ia_batch_sync. While a batch
  * exists its contexts take no other calls but uploads, reads and reference loads between steps.
- * Contexts with IA_SEARCH_LEARNED are not batched (IA_EUNSUPPORTED). */
+ * Contexts with IA_SEARCH_LEARNED must share their CNN weights (ia_load_proposal_weights with the
+ * same parameters); their proposal runs as one persistent launch over all their macroblocks and
+ * their fallbacks as one list (ia_batch_fallback_count). */
 typedef struct ia_batch ia_batch;
 
 typedef struct ia_frame {
@@ -253,6 +255,9 @@ const char *ia_batch_last_error(const ia_batch *b);
 int ia_batch_step(ia_batch *b, const ia_frame *frames);
 /* Waits for the batch's stream. */
 int ia_batch_sync(ia_batch *b);
+/* The number of (stream, macroblock, reference) triples the last step sent from the learned proposal
+ * to the exhaustive search. Synchronous. */
+int ia_batch_fallback_count(ia_batch *b, int *count);
 
 /* Tile shapes of the full-search kernel, for the autotuner (no counterpart in the reference, whose
  * search is a scalar loop): the number of compiled shapes, each one's name, and the shape a context

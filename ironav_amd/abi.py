@@ -103,6 +103,7 @@ PROTOTYPES = {
     "ia_batch_last_error": (C.c_char_p, [vp]),
     "ia_batch_step": (C.c_int, [vp, C.POINTER(Frame)]),
     "ia_batch_sync": (C.c_int, [vp]),
+    "ia_batch_fallback_count": (C.c_int, [vp, C.POINTER(C.c_int)]),
 }
 
 

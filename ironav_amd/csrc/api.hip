@@ -401,10 +401,15 @@ int ia_load_proposal_weights(ia_ctx *c, const float *params, int n) {
     for (int co = 0; co < COUT[l]; co++) w->b[l][co] = q[co];
     q += COUT[l];
   }
+  // FNV-1a of the converted weights: a batch's learned streams must share theirs
+  uint64_t h = 1469598103934665603ull;
+  const uint8_t *bytes = reinterpret_cast<const uint8_t *>(w);
+  for (size_t i = 0; i < sizeof *w; i++) h = (h ^ bytes[i]) * 1099511628211ull;
   hipError_t e = hipMemcpy(c->wts, w, sizeof *w, hipMemcpyHostToDevice);
   delete w;
   if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(weights)");
   c->wts_loaded = true;
+  c->wts_hash = h;
   return IA_OK;
 }
 

@@ -6,7 +6,9 @@
 //      pinned host memory to the device;
 //   2. the reset of the analysis queue (a one-word kernel: no memset nodes in the graph);
 //   3. the exhaustive search of every (stream, macroblock, reference), one launch per tile shape
-//      (streams of IA_SEARCH_FULL; the autotuner sets each context's shape);
+//      (streams of IA_SEARCH_FULL; the autotuner sets each context's shape); for the streams of
+//      IA_SEARCH_LEARNED, the proposal over all their macroblocks (one persistent launch; the CNN in
+//      the same graph) and the exhaustive search of the batch's one fallback list;
 //   4. the analysis: a pool of workgroups taking (stream, slice) tasks from a queue, longest first;
 //   5. the promotion of each stream's reconstruction to a reference slot: its half-sample planes
 //      (one batched launch) and its chroma planes and macroblock records (one batched copy).
@@ -16,6 +18,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "ctx.h"
@@ -55,8 +58,9 @@ __global__ __launch_bounds__(CP_THREADS) void copy_batch_kernel(const CopyJob *j
   }
 }
 
-__global__ void reset_word(int *p) {
+__global__ void reset_words(int *p, int *q) {
   if (threadIdx.x == 0) *p = 0;
+  if (threadIdx.x == 1 && q) *q = 0;
 }
 
 size_t align16(size_t n) { return (n + 15) & ~(size_t)15; }
@@ -71,7 +75,7 @@ struct ia_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   // the per-frame table, pinned on the host and mirrored on the device
-  size_t tab_bytes = 0, off_fs = 0, off_hp = 0, off_cp = 0;
+  size_t tab_bytes = 0, off_fs = 0, off_hp = 0, off_cp = 0, off_pp = 0;
   uint8_t *h_tab = nullptr, *d_tab = nullptr;
   // fixed tables: analysis tasks, queue head, search launch members
   int *d_fixed = nullptr;
@@ -82,12 +86,18 @@ struct ia_batch {
   };
   std::vector<Group> groups;
   size_t head = 0, tasks = 0;  // word offsets into d_fixed
+  // learned streams: proposal launch members (word offsets into d_fixed), the batch's fallback list
+  size_t prop_offs = 0, prop_sidx = 0;
+  int prop_nmem = 0, prop_total = 0, fb_max_range = 0, fb_max_entries = 0;
+  int *d_fb = nullptr;  // [0] count, [1..] entries picture << 20 | mb << 2 | ref
+  uint64_t wts_hash = 0;
   std::string err;
 
   ia::AnParams *an_h() { return reinterpret_cast<ia::AnParams *>(h_tab); }
   ia::FsArgs *fs_h() { return reinterpret_cast<ia::FsArgs *>(h_tab + off_fs); }
   ia::HpJob *hp_h() { return reinterpret_cast<ia::HpJob *>(h_tab + off_hp); }
   CopyJob *cp_h() { return reinterpret_cast<CopyJob *>(h_tab + off_cp); }
+  ia::PropArgs *pp_h() { return reinterpret_cast<ia::PropArgs *>(h_tab + off_pp); }
 };
 
 namespace {
@@ -107,12 +117,18 @@ int enqueue(ia_batch *b) {
   hipStream_t st = b->stream;
   IA_BHIP(b, hipMemcpyAsync(b->d_tab, b->h_tab, b->tab_bytes, hipMemcpyHostToDevice, st));
   int *fixed = b->d_fixed;
-  hipLaunchKernelGGL(reset_word, dim3(1), dim3(64), 0, st, fixed + b->head);
+  hipLaunchKernelGGL(reset_words, dim3(1), dim3(64), 0, st, fixed + b->head, b->d_fb);
   IA_BHIP(b, hipGetLastError());
   const ia::FsArgs *fs_d = reinterpret_cast<const ia::FsArgs *>(b->d_tab + b->off_fs);
   for (const auto &g : b->groups)
     IA_BHIP(b, ia::launch_fullsearch_batch(fs_d, fixed + g.offs, fixed + g.sidx, g.nmem, g.blocks, g.variant,
                                            g.max_range, st));
+  if (b->prop_nmem > 0) {
+    IA_BHIP(b, ia::launch_proposal_batch(reinterpret_cast<const ia::PropArgs *>(b->d_tab + b->off_pp),
+                                         fixed + b->prop_offs, fixed + b->prop_sidx, b->prop_nmem, b->prop_total, st));
+    if (b->fb_max_entries > 0)
+      IA_BHIP(b, ia::launch_fullsearch_list_batch(fs_d, b->d_fb + 1, b->d_fb, b->fb_max_range, b->fb_max_entries, st));
+  }
   IA_BHIP(b, ia::launch_analyse_batch(reinterpret_cast<const ia::AnParams *>(b->d_tab), fixed + b->tasks, b->ntasks,
                                       fixed + b->head, b->workers, st));
   IA_BHIP(b, ia::launch_halfpel_batch(reinterpret_cast<const ia::HpJob *>(b->d_tab + b->off_hp), b->n, b->hp_blocks,
@@ -132,9 +148,12 @@ int ia_batch_create(ia_batch **out, ia_ctx *const *ctxs, int n, int use_graph) {
   *out = nullptr;
   for (int i = 0; i < n; i++) {
     if (!ctxs[i]) return IA_EINVAL;
-    if (ctxs[i]->cfg.search_mode == IA_SEARCH_LEARNED) return IA_EUNSUPPORTED;
     for (int j = 0; j < i; j++)
       if (ctxs[j] == ctxs[i]) return IA_EINVAL;
+  }
+  if (n > 2047) {
+    for (int i = 0; i < n; i++)
+      if (ctxs[i]->cfg.search_mode == IA_SEARCH_LEARNED) return IA_EINVAL;  // fallback entries tag 11 bits
   }
   ia_batch *b = new ia_batch();
   b->ctx.assign(ctxs, ctxs + n);
@@ -144,7 +163,8 @@ int ia_batch_create(ia_batch **out, ia_ctx *const *ctxs, int n, int use_graph) {
   b->off_fs = align16((size_t)n * sizeof(ia::AnParams));
   b->off_hp = b->off_fs + align16((size_t)n * sizeof(ia::FsArgs));
   b->off_cp = b->off_hp + align16((size_t)n * sizeof(ia::HpJob));
-  b->tab_bytes = b->off_cp + align16((size_t)3 * n * sizeof(CopyJob));
+  b->off_pp = b->off_cp + align16((size_t)3 * n * sizeof(CopyJob));
+  b->tab_bytes = b->off_pp + align16((size_t)n * sizeof(ia::PropArgs));
   // analysis tasks, longest slice first (ties by stream, then slice)
   struct Task {
     int mbs, s, k;
@@ -200,6 +220,27 @@ int ia_batch_create(ia_batch **out, ia_ctx *const *ctxs, int n, int use_graph) {
     g.sidx = fixed.size();
     fixed.insert(fixed.end(), sidx.begin(), sidx.end());
   }
+  // the learned streams: one proposal launch over all their macroblocks; the fallback list holds at
+  // most every (macroblock, reference) of them
+  {
+    std::vector<int> offs, sidx;
+    int total = 0;
+    for (int s = 0; s < n; s++) {
+      const ia_ctx *c = ctxs[s];
+      if (c->cfg.search_mode != IA_SEARCH_LEARNED) continue;
+      offs.push_back(total);
+      sidx.push_back(s);
+      total += c->g.mb_w * c->g.mb_h;
+      b->fb_max_range = std::max(b->fb_max_range, c->cfg.search_range);
+      if (std::isfinite(c->cfg.fallback_ratio)) b->fb_max_entries += c->g.mb_w * c->g.mb_h * std::min(c->cfg.nrefs, 3);
+    }
+    b->prop_nmem = (int)offs.size();
+    b->prop_total = total;
+    b->prop_offs = fixed.size();
+    fixed.insert(fixed.end(), offs.begin(), offs.end());
+    b->prop_sidx = fixed.size();
+    fixed.insert(fixed.end(), sidx.begin(), sidx.end());
+  }
   // promotion launches: tiles of each stream's half-sample planes, chunks of its copies
   int hp_blocks = 0, cp_blocks = 0;
   for (int s = 0; s < n; s++) {
@@ -216,6 +257,7 @@ int ia_batch_create(ia_batch **out, ia_ctx *const *ctxs, int n, int use_graph) {
             hipHostMalloc((void **)&b->h_tab, b->tab_bytes, hipHostMallocDefault) == hipSuccess &&
             hipMalloc((void **)&b->d_tab, b->tab_bytes) == hipSuccess &&
             hipMalloc((void **)&b->d_fixed, fixed.size() * sizeof(int)) == hipSuccess &&
+            hipMalloc((void **)&b->d_fb, (1 + (size_t)b->fb_max_entries) * sizeof(int)) == hipSuccess &&
             hipMemcpy(b->d_fixed, fixed.data(), fixed.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
   if (!ok) {
     ia_batch_destroy(b);
@@ -234,6 +276,7 @@ void ia_batch_destroy(ia_batch *b) {
   if (b->h_tab) (void)hipHostFree(b->h_tab);
   (void)hipFree(b->d_tab);
   (void)hipFree(b->d_fixed);
+  (void)hipFree(b->d_fb);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
@@ -255,7 +298,27 @@ int ia_batch_step(ia_batch *b, const ia_frame *frames) {
     int rc = iah::analysis_params(c, f.qp, f.kind, f.ref_slots, nref, src, false, &b->an_h()[s]);
     if (rc) return bfail(b, rc, "stream " + std::to_string(s) + ": " + c->err);
     ia::FsArgs &fa = b->fs_h()[s];
-    if (c->cfg.search_mode == IA_SEARCH_FULL) {
+    ia::PropArgs &pa = b->pp_h()[s];
+    memset(&pa, 0, sizeof pa);
+    if (c->cfg.search_mode == IA_SEARCH_LEARNED) {
+      if (!c->wts_loaded) return bfail(b, IA_ESTATE, "stream " + std::to_string(s) + ": the proposal's weights are not loaded");
+      if (nref > 3) return bfail(b, IA_EUNSUPPORTED, "stream " + std::to_string(s) + ": the learned proposal takes at most 3 references");
+      if (b->wts_hash == 0) b->wts_hash = c->wts_hash;
+      if (c->wts_hash != b->wts_hash) return bfail(b, IA_ESTATE, "the learned streams of a batch must share their CNN weights");
+      pa.src_y = f.y ? f.y : c->src[0];
+      for (int r = 0; r < nref; r++) pa.ref_g[r] = c->slots[f.ref_slots[r]].hp;
+      pa.rmbs0 = nref > 0 ? c->slots[f.ref_slots[0]].mbs : nullptr;
+      pa.wts = c->wts;
+      pa.out = c->fs;
+      pa.fb_count = b->d_fb;
+      pa.fb_list = std::isfinite(c->cfg.fallback_ratio) ? b->d_fb + 1 : nullptr;
+      pa.fb_tag = s << 20;
+      pa.mb_w = c->g.mb_w;
+      pa.mb_h = c->g.mb_h;
+      pa.nref = nref;
+      pa.fallback_ratio = c->cfg.fallback_ratio;
+    }
+    if (c->cfg.search_mode != IA_SEARCH_DIAMOND) {
       iah::search_args(c, f.ref_slots, nref, f.y, &fa);
     } else {
       memset(&fa, 0, sizeof fa);
@@ -312,7 +375,7 @@ int ia_batch_step(ia_batch *b, const ia_frame *frames) {
     ia_ctx *c = b->ctx[s];
     const ia_frame &f = frames[s];
     const int nref = f.kind == IA_PIC_P ? f.nref : 0;
-    if (c->cfg.search_mode == IA_SEARCH_FULL && nref > 0) {
+    if (c->cfg.search_mode != IA_SEARCH_DIAMOND && nref > 0) {
       c->fs_nref = nref;
       for (int r = 0; r < nref; r++) c->fs_slots[r] = f.ref_slots[r];
       c->fs_valid = true;
@@ -322,6 +385,14 @@ int ia_batch_step(ia_batch *b, const ia_frame *frames) {
     c->an_valid = true;
     if (f.promote_slot >= 0) c->slots[f.promote_slot].loaded = true;
   }
+  return IA_OK;
+}
+
+int ia_batch_fallback_count(ia_batch *b, int *count) {
+  if (!b || !count) return IA_EINVAL;
+  IA_BHIP(b, hipStreamSynchronize(b->stream));
+  *count = 0;
+  if (b->d_fb) IA_BHIP(b, hipMemcpy(count, b->d_fb, sizeof(int), hipMemcpyDeviceToHost));
   return IA_OK;
 }
 

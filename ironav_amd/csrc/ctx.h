@@ -39,6 +39,7 @@ struct ia_ctx {
   // learned proposal
   ia::CnnWeights *wts = nullptr;
   bool wts_loaded = false;
+  uint64_t wts_hash = 0;       // of the loaded weights (a batch's learned streams share one set)
   int *fb = nullptr;           // [0], [1] counts (alternating calls), [2..] entries
   int fb_gen = 0;              // the count the next learned search uses
   float *feat = nullptr, *cnn_out = nullptr;

@@ -250,6 +250,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   search_one<K, NT, RAWS>(a, rows_lds, mb, r, raw);
 }
 
+// The fallback list of a batch: entries list[i] = picture << 20 | mb << 2 | ref (i < *count) with the
+// picture's arguments args[picture]; the blocks walk the list grid-stride.
+template <int K, int NT, int RAWS, int WPE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void fullsearch_list_batch_kernel(
+    const FsArgs *args, int rows_lds, const int *list, const int *count) {
+  extern __shared__ uint32_t raw[];
+  const int n = *count;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int e = list[i];
+    search_one<K, NT, RAWS>(args[e >> 20], rows_lds, (e >> 2) & 0x3ffff, e & 3, raw);
+    __syncthreads();  // the next entry restages the window and the reduction scratch
+  }
+}
+
 // Block i searches entry list[i] = mb * 4 + ref when i < *count and returns at once otherwise (the
 // grid covers every possible entry: the count is known on the device only). Block 0 clears *reset.
 template <int K, int NT, int RAWS, int WPE>
@@ -318,6 +332,20 @@ hipError_t launch_fullsearch_batch(const FsArgs *args, const int *offs, const in
   const size_t shmem = (size_t)rows_lds * raws * sizeof(uint32_t);
   hipLaunchKernelGGL(small ? v.bfn32 : v.bfn64, dim3(nblocks), dim3(v.nt), shmem, stream, args, offs, sidx, nmem,
                      rows_lds);
+  return hipGetLastError();
+}
+
+hipError_t launch_fullsearch_list_batch(const FsArgs *args, const int *list, const int *count, int max_range,
+                                        int max_entries, hipStream_t stream) {
+  if (max_range < 1 || max_range > RMAX || max_entries < 1) return hipErrorInvalidValue;
+  const bool small = max_range <= 32;
+  const int raws = small ? raws_for(32) : raws_for(64);
+  const int rows_lds = 2 * max_range + 1 + LIST_K + 15;
+  const size_t shmem = (size_t)rows_lds * raws * sizeof(uint32_t);
+  dim3 grid(min(max_entries, 4096));
+  hipLaunchKernelGGL(small ? (fullsearch_list_batch_kernel<LIST_K, LIST_NT, raws_for(32), LIST_W>)
+                           : (fullsearch_list_batch_kernel<LIST_K, LIST_NT, raws_for(64), LIST_W>),
+                     grid, dim3(LIST_NT), shmem, stream, args, rows_lds, list, count);
   return hipGetLastError();
 }
 

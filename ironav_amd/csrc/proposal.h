@@ -24,7 +24,8 @@ struct PropArgs {
   const ia_mbinfo *rmbs0;       // reference 0's macroblock records (temporal predictors), may be null
   const CnnWeights *wts;
   ia_fsresult *out;             // [nref][mb][IA_NPART]
-  int *fb_list;                 // fallback (mb * 4 + ref) entries; null: no fallback
+  int *fb_list;                 // fallback (fb_tag | mb * 4 + ref) entries; null: no fallback
+  int fb_tag;                   // added to the entries (the picture's index in a batch, << 20)
   int *fb_count;
   float *feat;                  // optional: CNN input features [mb][81][CNN_IN] (for training)
   float *cnn_out;               // optional: CNN outputs [mb][81][CNN_OUT]
@@ -36,8 +37,16 @@ struct PropArgs {
 };
 
 hipError_t launch_proposal(const PropArgs &a, hipStream_t stream);
+// Many pictures in one launch (args in device memory): macroblock g < total of the launch is
+// macroblock g - offs[k] of picture sidx[k] (offs[k] <= g < offs[k+1]); all share args[sidx[0]].wts.
+hipError_t launch_proposal_batch(const PropArgs *args, const int *offs, const int *sidx, int nmem, int total,
+                                 hipStream_t stream);
 // the exhaustive search over the (mb, ref) entries of a fallback list; the search also sets *reset
 // to 0 (the count the next list will be gathered in)
 hipError_t launch_fullsearch_list(const FsArgs &a, const int *list, const int *count, int *reset, hipStream_t stream);
+// the fallback list of a batch: entries picture << 20 | mb << 2 | ref with args[picture] in device
+// memory (ranges at most max_range; at most max_entries entries)
+hipError_t launch_fullsearch_list_batch(const FsArgs *args, const int *list, const int *count, int max_range,
+                                        int max_entries, hipStream_t stream);
 
 }  // namespace ia

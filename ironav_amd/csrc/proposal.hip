@@ -230,13 +230,10 @@ __device__ __forceinline__ void win_row16(const uint32_t (*win)[WST], int off, i
   r3 = __builtin_amdgcn_alignbyte(w4, w3, sh);
 }
 
-// Persistent: each workgroup loads the weights once, then its waves take macroblocks in turn.
-__global__ __launch_bounds__(64 * WAVES) void proposal_kernel(PropArgs a) {
-  extern __shared__ uint8_t lds_raw[];
-  PropLds &L = *reinterpret_cast<PropLds *>(lds_raw);
-  const int tid = threadIdx.x, wave = tid >> 6;
+// The CNN's weights into the workgroup's LDS (all threads of the workgroup).
+__device__ __forceinline__ void load_weights(PropLds &L, const CnnWeights *w, int tid) {
   {
-    const CnnWeights &W = *a.wts;
+    const CnnWeights &W = *w;
     // layer 0: taps 9..11 zero; layers 1, 2 copied as 16-byte vectors; layer 3: the first 16 outputs
     for (int i = tid; i < 12 * CNN_CH; i += 64 * WAVES) {
       const int t = i / CNN_CH, co = i % CNN_CH;
@@ -254,260 +251,307 @@ __global__ __launch_bounds__(64 * WAVES) void proposal_kernel(PropArgs a) {
     }
     for (int i = tid; i < CNN_LAYERS * CNN_CH; i += 64 * WAVES) (&L.b[0][0])[i] = (&W.b[0][0])[i];
   }
+}
+
+// One macroblock of the picture `a` describes, by one wave (its buffers V, the weights in L).
+__device__ void propose_mb(PropLds &L, WaveLds &V, const PropArgs &a, int mb, int tid) {
+  if (a.nref < 1) return;  // an IDR picture of a batch: nothing to search
+  const int nmb = a.mb_w * a.mb_h;
+  const int pw = a.mb_w * 16, ph = a.mb_h * 16, sw = pw + 2 * PAD;
+  // the lane index afresh in each iteration (an empty asm the compiler cannot see through), so
+  // that lane-derived addresses are recomputed here rather than hoisted out of the loop and held
+  // in registers for its whole length
+  int lane = tid & 63;
+  asm volatile("" : "+v"(lane));
+  const int mbx = mb % a.mb_w, mby = mb / a.mb_w;
+  const int x0 = mbx * 16, y0 = mby * 16;
+  const int lox = -(x0 + 24), hix = pw - x0 + 8, loy = -(y0 + 24), hiy = ph - y0 + 8;
+
+  // the macroblock's samples (lane i loads dword i: row i / 4, columns 4 (i % 4)..+3) and reference
+  // 0's records of this macroblock, the left and the above one (lane k < 3), all in flight at once
+  const uint32_t sv = ld_u32(a.src_y + (size_t)(y0 + (lane >> 2)) * pw + x0 + (lane & 3) * 4);
+  int tkind = 0, tmvw = 0;
+  if (a.rmbs0 && lane < 3) {
+    const int nx = lane == 1 ? mbx - 1 : mbx, ny = lane == 2 ? mby - 1 : mby;
+    if (nx >= 0 && ny >= 0) {
+      const ia_mbinfo *m = a.rmbs0 + ny * a.mb_w + nx;
+      tkind = m->kind;
+      tmvw = *reinterpret_cast<const int *>(&m->mv[5][0]);
+    }
+  }
+  // temporal predictors: those records' vectors (quarter samples) when inter coded
+  int tmv[3][2], tok[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    tok[k] = __builtin_amdgcn_readlane(tkind, k) == 5;
+    const int w = tok[k] ? __builtin_amdgcn_readlane(tmvw, k) : 0;
+    tmv[k][0] = (int16_t)(w & 0xffff);
+    tmv[k][1] = (int16_t)((uint32_t)w >> 16);
+  }
+  // grid centres: the co-located vector scaled by the reference's distance, to a multiple of 4
+  // samples, clamped so the grid lies within the macroblock's vectors
+  int cx[3], cy[3];
+  for (int r = 0; r < 3; r++) {
+    const int qx = (r + 1) * tmv[0][0], qy = (r + 1) * tmv[0][1];
+    const int px = (int)floorf((qx + 8) / 16.0f) * 4, py = (int)floorf((qy + 8) / 16.0f) * 4;
+    cx[r] = min(max(px, lox + 16), hix - 16);
+    cy[r] = min(max(py, loy + 16), hiy - 16);
+  }
+  if (lane == 0 && a.centre)
+    for (int r = 0; r < 3; r++) {
+      a.centre[(mb * 3 + r) * 2] = (int16_t)cx[r];
+      a.centre[(mb * 3 + r) * 2 + 1] = (int16_t)cy[r];
+    }
+  const int nref = a.feat_in ? 0 : a.nref;
+  // the reference windows, held in registers until the refinement (the CNN reuses their LDS)
+  uint32_t wreg[3][WIN / 4];
+  int woff[3] = {0, 0, 0};
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+    if (r < nref) woff[r] = load_window(lane, a.ref_g[r], sw, x0 + cx[r] - 18 + PAD, y0 + cy[r] - 18 + PAD, wreg[r]);
+  // the samples as scalars
+  uint32_t src[64];
+#pragma unroll
+  for (int i = 0; i < 64; i++) src[i] = __builtin_amdgcn_readlane(sv, i);
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+    if (r < nref) put_window(lane, wreg[r], V.win[r]);
+  for (int i = lane; i < PCELLS * 8 / 8; i += 64) reinterpret_cast<uint4 *>(&V.x0[0][0])[i] = make_uint4(0, 0, 0, 0);
+  wave_sync();
+
+  // features
+  if (a.feat_in) {
+    for (int i = lane; i < CNN_CELLS * CNN_IN; i += 64)
+      V.x0[padpos(i / CNN_IN)][i % CNN_IN] = f2bf(a.feat_in[(size_t)mb * CNN_CELLS * CNN_IN + i]);
+  } else {
+    // cost volume from the staged windows: the nref * 81 (reference, cell) pairs spread over the lanes
+    for (int t = lane; t < nref * CNN_CELLS; t += 64) {
+      const int r = t >= 2 * CNN_CELLS ? 2 : t >= CNN_CELLS ? 1 : 0, c = t - r * CNN_CELLS;
+      const int off = r == 2 ? woff[2] : r == 1 ? woff[1] : woff[0];
+      const int wxp = 4 * (c % CNN_GRID) + 2, wyp = 4 * (c / CNN_GRID) + 2;  // window offset of the cell
+      uint32_t s = 0;
+#pragma unroll
+      for (int row = 0; row < 16; row++) {
+        uint32_t r0, r1, r2, r3;
+        win_row16(V.win[r], off, wxp, wyp + row, r0, r1, r2, r3);
+        s = __builtin_amdgcn_sad_u8(src[row * 4 + 0], r0, s);
+        s = __builtin_amdgcn_sad_u8(src[row * 4 + 1], r1, s);
+        s = __builtin_amdgcn_sad_u8(src[row * 4 + 2], r2, s);
+        s = __builtin_amdgcn_sad_u8(src[row * 4 + 3], r3, s);
+      }
+      const float f = __log2f(1.0f + (float)s) * (1.0f / 16.0f);
+      V.x0[padpos(c)][r] = f2bf(f);
+      if (a.feat) a.feat[((size_t)mb * CNN_CELLS + c) * CNN_IN + r] = f;
+    }
+    for (int c = lane; c < CNN_CELLS; c += 64) {
+      const int gx = c % CNN_GRID, gy = c / CNN_GRID;
+      float fv[CNN_IN] = {0, 0, 0, 0, 0, 0, 1.0f, 0};
+      for (int k = 0; k < 3; k++) {
+        if (!tok[k]) continue;
+        const int kx = (int)floorf((tmv[k][0] - 4 * cx[0] + 8) / 16.0f) + 4;
+        const int ky = (int)floorf((tmv[k][1] - 4 * cy[0] + 8) / 16.0f) + 4;
+        if (kx == gx && ky == gy) fv[3 + k] = 1.0f;
+      }
+      for (int ch = 3; ch < CNN_IN; ch++) {
+        V.x0[padpos(c)][ch] = f2bf(fv[ch]);
+        if (a.feat) a.feat[((size_t)mb * CNN_CELLS + c) * CNN_IN + ch] = fv[ch];
+      }
+      if (a.feat)
+        for (int r = a.nref; r < 3; r++) a.feat[((size_t)mb * CNN_CELLS + c) * CNN_IN + r] = 0.f;
+    }
+  }
+  wave_sync();
+  if (a.features_only) return;
+
+  // the CNN (the windows' space now holds the activations, borders zero)
+  for (int i = lane; i < PCELLS * AST / 8; i += 64) reinterpret_cast<uint4 *>(&V.act[0][0])[i] = make_uint4(0, 0, 0, 0);
+  wave_sync();
+  conv_layer<true, 2, true>(lane, &V.x0[0][0], 8, &L.w0[0][0][0], L.b[0], &V.act[0][0], nullptr);
+  wave_sync();
+  conv_layer<false, 2, true>(lane, &V.act[0][0], AST, &L.w1[0][0][0], L.b[1], &V.act[0][0], nullptr);
+  wave_sync();
+  conv_layer<false, 2, true>(lane, &V.act[0][0], AST, &L.w2[0][0][0], L.b[2], &V.act[0][0], nullptr);
+  wave_sync();
+  conv_layer<false, 1, false>(lane, &V.act[0][0], AST, &L.w3[0][0][0], L.b[3], nullptr, &V.out[0][0]);
+  wave_sync();
+  if (a.cnn_out)
+    for (int i = lane; i < CNN_CELLS * CNN_OUT; i += 64) a.cnn_out[(size_t)mb * CNN_CELLS * CNN_OUT + i] = (&V.out[0][0])[i];
+  if (a.feat_in) return;
+
+  // the windows again, for the refinement (the activations are done with)
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+    if (r < nref) put_window(lane, wreg[r], V.win[r]);
+  wave_sync();
+
+  // the 8 best cells of each reference's score map (ties to the lower cell index), the three
+  // references' selections interleaved; lane k < 8 stores the k-th
+  {
+    uint32_t k0[3], k1[3];
+    int mine[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      k0[r] = (ford(V.out[lane][r]) & ~0x7fu) | (uint32_t)(127 - lane);
+      k1[r] = lane + 64 < CNN_CELLS ? (ford(V.out[lane + 64][r]) & ~0x7fu) | (uint32_t)(127 - lane - 64) : 0u;
+      mine[r] = 0;
+    }
+#pragma unroll
+    for (int k = 0; k < CNN_TOPK; k++)
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+        if (r < nref) {
+          const int cell = 127 - (int)(wave_max(max(k0[r], k1[r])) & 0x7f);
+          if (cell == lane) k0[r] = 0;
+          if (cell == lane + 64) k1[r] = 0;
+          if (lane == k) mine[r] = cell;
+        }
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+      if (r < nref && lane < CNN_TOPK) V.cand[r][lane] = mine[r];
+    wave_sync();
+  }
+
+  // exact refinement of all 41 partitions around the selected cells, +-2 samples. Lanes 21r..21r+20
+  // take reference r; its 40 strips (cell rank k, column x of the cell's 5x5 block) go two per
+  // lane, and a strip walks its column's 5 displacements (each staged row read once feeds every
+  // candidate that uses it). Lane 63 idles.
+  uint32_t run[IA_NPART];
+#pragma unroll
+  for (int p = 0; p < IA_NPART; p++) run[p] = 0xffffffffu;
+  const int rg = lane / 21;
+  if (rg < nref) {
+    const int off = rg == 2 ? woff[2] : rg == 1 ? woff[1] : woff[0];
+    const int crx = rg == 2 ? cx[2] : rg == 1 ? cx[1] : cx[0];
+    const int cry = rg == 2 ? cy[2] : rg == 1 ? cy[1] : cy[0];
+    for (int strip = lane - 21 * rg; strip < 5 * CNN_TOPK; strip += 21) {
+      const int k = strip / 5, x = strip - 5 * k;
+      const int cell = V.cand[rg][k];
+      const int ox = 4 * (cell % CNN_GRID) + x, oy = 4 * (cell / CNN_GRID);  // window offset of the strip top
+      const int dx = crx - 18 + ox, dy0 = cry - 18 + oy;
+      uint32_t acc[5][16];
+#pragma unroll
+      for (int i = 0; i < 5; i++)
+#pragma unroll
+        for (int b = 0; b < 16; b++) acc[i][b] = (uint32_t)(k * 25 + i * 5 + x);  // order: rank, then raster
+#pragma unroll
+      for (int y = 0; y < 20; y++) {
+        uint32_t r0, r1, r2, r3;
+        win_row16(V.win[rg], off, ox, oy + y, r0, r1, r2, r3);
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+          const int row = y - i;
+          if (row >= 0 && row < 16) {
+            const int by = row >> 2;
+            acc[i][by * 4 + 0] = __builtin_amdgcn_sad_hi_u8(src[row * 4 + 0], r0, acc[i][by * 4 + 0]);
+            acc[i][by * 4 + 1] = __builtin_amdgcn_sad_hi_u8(src[row * 4 + 1], r1, acc[i][by * 4 + 1]);
+            acc[i][by * 4 + 2] = __builtin_amdgcn_sad_hi_u8(src[row * 4 + 2], r2, acc[i][by * 4 + 2]);
+            acc[i][by * 4 + 3] = __builtin_amdgcn_sad_hi_u8(src[row * 4 + 3], r3, acc[i][by * 4 + 3]);
+          }
+        }
+      }
+      const bool xin = dx >= lox && dx <= hix;
+#pragma unroll
+      for (int i = 0; i < 5; i++) {
+        if (xin && dy0 + i >= loy && dy0 + i <= hiy) {
+          uint32_t key[IA_NPART];
+          partition_keys(acc[i], key);
+#pragma unroll
+          for (int p = 0; p < IA_NPART; p++) run[p] = min(run[p], key[p]);
+        }
+      }
+    }
+  }
+  // the minima over each reference's lanes by a transpose through LDS (over the windows, now read):
+  // T[lane][p] = run[p]; output o = 41 r + p takes the minimum of column p over rows 21r..21r+20
+  wave_sync();
+  uint32_t *T = &V.win[0][0][0];
+#pragma unroll
+  for (int p = 0; p < IA_NPART; p++) T[lane * IA_NPART + p] = run[p];
+  wave_sync();
+  for (int o = lane; o < nref * IA_NPART; o += 64) {
+    const int r = o >= 2 * IA_NPART ? 2 : o >= IA_NPART ? 1 : 0, p = o - IA_NPART * r;
+    uint32_t v = 0xffffffffu;
+#pragma unroll
+    for (int l = 0; l < 21; l++) v = min(v, T[(21 * r + l) * IA_NPART + p]);
+    const int lg = p == 0 ? 4 : p < 5 ? 3 : p < 9 ? 2 : p < 25 ? 1 : 0;  // log2 of the 4x4 blocks summed
+    ia_fsresult res;
+    if (v == 0xffffffffu) {
+      res.mv[0] = res.mv[1] = 0;
+      res.sad = 0xffffffffu;
+    } else {
+      const int idx = (int)((v & 0xffffu) >> lg), k = idx / 25, j = idx % 25;
+      const int cell = V.cand[r][k];
+      const int crx = r == 2 ? cx[2] : r == 1 ? cx[1] : cx[0];
+      const int cry = r == 2 ? cy[2] : r == 1 ? cy[1] : cy[0];
+      res.mv[0] = (int16_t)((crx + 4 * (cell % CNN_GRID - 4) + (j % 5 - 2)) * 4);
+      res.mv[1] = (int16_t)((cry + 4 * (cell / CNN_GRID - 4) + (j / 5 - 2)) * 4);
+      res.sad = v >> 16;
+    }
+    a.out[((size_t)r * nmb + mb) * IA_NPART + p] = res;
+    // fallback when the refined 16x16 cost exceeds the ratio times the predicted one
+    if (p == 0 && a.fb_list) {
+      const float pred = exp2f(V.out[V.cand[r][0]][3]);
+      if (res.sad == 0xffffffffu || (float)res.sad > a.fallback_ratio * pred) {
+        const int slot = atomicAdd(a.fb_count, 1);
+        a.fb_list[slot] = a.fb_tag | (mb * 4 + r);
+      }
+    }
+  }
+  wave_sync();  // the next macroblock restages the windows and features
+}
+
+// Persistent: each workgroup loads the weights once, then its waves take macroblocks in turn.
+__global__ __launch_bounds__(64 * WAVES) void proposal_kernel(PropArgs a) {
+  extern __shared__ uint8_t lds_raw[];
+  PropLds &L = *reinterpret_cast<PropLds *>(lds_raw);
+  const int tid = threadIdx.x, wave = tid >> 6;
+  load_weights(L, a.wts, tid);
   __syncthreads();
   WaveLds &V = L.wv[wave];
   const int nmb = a.mb_w * a.mb_h;
-  const int pw = a.mb_w * 16, ph = a.mb_h * 16, sw = pw + 2 * PAD;
   // workgroups go to the 8 XCDs in turn (each with its own L2): renumber them so that those on one
   // XCD take consecutive chunks of macroblocks, whose windows overlap
   const int nxcd = 8, per = gridDim.x / nxcd;
   const int vb = (int)blockIdx.x < per * nxcd ? (int)(blockIdx.x % nxcd) * per + (int)blockIdx.x / nxcd : (int)blockIdx.x;
-  for (int mb = vb * WAVES + wave; mb < nmb; mb += gridDim.x * WAVES) {
-    // the lane index afresh in each iteration (an empty asm the compiler cannot see through), so
-    // that lane-derived addresses are recomputed here rather than hoisted out of the loop and held
-    // in registers for its whole length
-    int lane = tid & 63;
-    asm volatile("" : "+v"(lane));
-    const int mbx = mb % a.mb_w, mby = mb / a.mb_w;
-    const int x0 = mbx * 16, y0 = mby * 16;
-    const int lox = -(x0 + 24), hix = pw - x0 + 8, loy = -(y0 + 24), hiy = ph - y0 + 8;
+  for (int mb = vb * WAVES + wave; mb < nmb; mb += gridDim.x * WAVES) propose_mb(L, V, a, mb, tid);
+}
 
-    // the macroblock's samples (lane i loads dword i: row i / 4, columns 4 (i % 4)..+3) and reference
-    // 0's records of this macroblock, the left and the above one (lane k < 3), all in flight at once
-    const uint32_t sv = ld_u32(a.src_y + (size_t)(y0 + (lane >> 2)) * pw + x0 + (lane & 3) * 4);
-    int tkind = 0, tmvw = 0;
-    if (a.rmbs0 && lane < 3) {
-      const int nx = lane == 1 ? mbx - 1 : mbx, ny = lane == 2 ? mby - 1 : mby;
-      if (nx >= 0 && ny >= 0) {
-        const ia_mbinfo *m = a.rmbs0 + ny * a.mb_w + nx;
-        tkind = m->kind;
-        tmvw = *reinterpret_cast<const int *>(&m->mv[5][0]);
-      }
+// Many pictures in one persistent launch: macroblock g of the launch is macroblock g - offs[k] of
+// member k (offs[k] <= g < offs[k+1]), whose picture is args[sidx[k]]. All members share the CNN
+// weights of args[sidx[0]].
+__global__ __launch_bounds__(64 * WAVES) void proposal_batch_kernel(const PropArgs *args, const int *offs,
+                                                                    const int *sidx, int nmem, int total) {
+  extern __shared__ uint8_t lds_raw[];
+  PropLds &L = *reinterpret_cast<PropLds *>(lds_raw);
+  const int tid = threadIdx.x, wave = tid >> 6;
+  load_weights(L, args[sidx[0]].wts, tid);
+  __syncthreads();
+  WaveLds &V = L.wv[wave];
+  const int nxcd = 8, per = gridDim.x / nxcd;
+  const int vb = (int)blockIdx.x < per * nxcd ? (int)(blockIdx.x % nxcd) * per + (int)blockIdx.x / nxcd : (int)blockIdx.x;
+  for (int g = vb * WAVES + wave; g < total; g += gridDim.x * WAVES) {
+    const int gu = __builtin_amdgcn_readfirstlane(g);
+    int lo = 0, hi = nmem - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (offs[mid] <= gu) lo = mid;
+      else hi = mid - 1;
     }
-    // temporal predictors: those records' vectors (quarter samples) when inter coded
-    int tmv[3][2], tok[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      tok[k] = __builtin_amdgcn_readlane(tkind, k) == 5;
-      const int w = tok[k] ? __builtin_amdgcn_readlane(tmvw, k) : 0;
-      tmv[k][0] = (int16_t)(w & 0xffff);
-      tmv[k][1] = (int16_t)((uint32_t)w >> 16);
-    }
-    // grid centres: the co-located vector scaled by the reference's distance, to a multiple of 4
-    // samples, clamped so the grid lies within the macroblock's vectors
-    int cx[3], cy[3];
-    for (int r = 0; r < 3; r++) {
-      const int qx = (r + 1) * tmv[0][0], qy = (r + 1) * tmv[0][1];
-      const int px = (int)floorf((qx + 8) / 16.0f) * 4, py = (int)floorf((qy + 8) / 16.0f) * 4;
-      cx[r] = min(max(px, lox + 16), hix - 16);
-      cy[r] = min(max(py, loy + 16), hiy - 16);
-    }
-    if (lane == 0 && a.centre)
-      for (int r = 0; r < 3; r++) {
-        a.centre[(mb * 3 + r) * 2] = (int16_t)cx[r];
-        a.centre[(mb * 3 + r) * 2 + 1] = (int16_t)cy[r];
-      }
-    const int nref = a.feat_in ? 0 : a.nref;
-    // the reference windows, held in registers until the refinement (the CNN reuses their LDS)
-    uint32_t wreg[3][WIN / 4];
-    int woff[3] = {0, 0, 0};
-#pragma unroll
-    for (int r = 0; r < 3; r++)
-      if (r < nref) woff[r] = load_window(lane, a.ref_g[r], sw, x0 + cx[r] - 18 + PAD, y0 + cy[r] - 18 + PAD, wreg[r]);
-    // the samples as scalars
-    uint32_t src[64];
-#pragma unroll
-    for (int i = 0; i < 64; i++) src[i] = __builtin_amdgcn_readlane(sv, i);
-#pragma unroll
-    for (int r = 0; r < 3; r++)
-      if (r < nref) put_window(lane, wreg[r], V.win[r]);
-    for (int i = lane; i < PCELLS * 8 / 8; i += 64) reinterpret_cast<uint4 *>(&V.x0[0][0])[i] = make_uint4(0, 0, 0, 0);
-    wave_sync();
-
-    // features
-    if (a.feat_in) {
-      for (int i = lane; i < CNN_CELLS * CNN_IN; i += 64)
-        V.x0[padpos(i / CNN_IN)][i % CNN_IN] = f2bf(a.feat_in[(size_t)mb * CNN_CELLS * CNN_IN + i]);
-    } else {
-      // cost volume from the staged windows: the nref * 81 (reference, cell) pairs spread over the lanes
-      for (int t = lane; t < nref * CNN_CELLS; t += 64) {
-        const int r = t >= 2 * CNN_CELLS ? 2 : t >= CNN_CELLS ? 1 : 0, c = t - r * CNN_CELLS;
-        const int off = r == 2 ? woff[2] : r == 1 ? woff[1] : woff[0];
-        const int wxp = 4 * (c % CNN_GRID) + 2, wyp = 4 * (c / CNN_GRID) + 2;  // window offset of the cell
-        uint32_t s = 0;
-#pragma unroll
-        for (int row = 0; row < 16; row++) {
-          uint32_t r0, r1, r2, r3;
-          win_row16(V.win[r], off, wxp, wyp + row, r0, r1, r2, r3);
-          s = __builtin_amdgcn_sad_u8(src[row * 4 + 0], r0, s);
-          s = __builtin_amdgcn_sad_u8(src[row * 4 + 1], r1, s);
-          s = __builtin_amdgcn_sad_u8(src[row * 4 + 2], r2, s);
-          s = __builtin_amdgcn_sad_u8(src[row * 4 + 3], r3, s);
-        }
-        const float f = __log2f(1.0f + (float)s) * (1.0f / 16.0f);
-        V.x0[padpos(c)][r] = f2bf(f);
-        if (a.feat) a.feat[((size_t)mb * CNN_CELLS + c) * CNN_IN + r] = f;
-      }
-      for (int c = lane; c < CNN_CELLS; c += 64) {
-        const int gx = c % CNN_GRID, gy = c / CNN_GRID;
-        float fv[CNN_IN] = {0, 0, 0, 0, 0, 0, 1.0f, 0};
-        for (int k = 0; k < 3; k++) {
-          if (!tok[k]) continue;
-          const int kx = (int)floorf((tmv[k][0] - 4 * cx[0] + 8) / 16.0f) + 4;
-          const int ky = (int)floorf((tmv[k][1] - 4 * cy[0] + 8) / 16.0f) + 4;
-          if (kx == gx && ky == gy) fv[3 + k] = 1.0f;
-        }
-        for (int ch = 3; ch < CNN_IN; ch++) {
-          V.x0[padpos(c)][ch] = f2bf(fv[ch]);
-          if (a.feat) a.feat[((size_t)mb * CNN_CELLS + c) * CNN_IN + ch] = fv[ch];
-        }
-        if (a.feat)
-          for (int r = a.nref; r < 3; r++) a.feat[((size_t)mb * CNN_CELLS + c) * CNN_IN + r] = 0.f;
-      }
-    }
-    wave_sync();
-    if (a.features_only) continue;
-
-    // the CNN (the windows' space now holds the activations, borders zero)
-    for (int i = lane; i < PCELLS * AST / 8; i += 64) reinterpret_cast<uint4 *>(&V.act[0][0])[i] = make_uint4(0, 0, 0, 0);
-    wave_sync();
-    conv_layer<true, 2, true>(lane, &V.x0[0][0], 8, &L.w0[0][0][0], L.b[0], &V.act[0][0], nullptr);
-    wave_sync();
-    conv_layer<false, 2, true>(lane, &V.act[0][0], AST, &L.w1[0][0][0], L.b[1], &V.act[0][0], nullptr);
-    wave_sync();
-    conv_layer<false, 2, true>(lane, &V.act[0][0], AST, &L.w2[0][0][0], L.b[2], &V.act[0][0], nullptr);
-    wave_sync();
-    conv_layer<false, 1, false>(lane, &V.act[0][0], AST, &L.w3[0][0][0], L.b[3], nullptr, &V.out[0][0]);
-    wave_sync();
-    if (a.cnn_out)
-      for (int i = lane; i < CNN_CELLS * CNN_OUT; i += 64) a.cnn_out[(size_t)mb * CNN_CELLS * CNN_OUT + i] = (&V.out[0][0])[i];
-    if (a.feat_in) continue;
-
-    // the windows again, for the refinement (the activations are done with)
-#pragma unroll
-    for (int r = 0; r < 3; r++)
-      if (r < nref) put_window(lane, wreg[r], V.win[r]);
-    wave_sync();
-
-    // the 8 best cells of each reference's score map (ties to the lower cell index), the three
-    // references' selections interleaved; lane k < 8 stores the k-th
-    {
-      uint32_t k0[3], k1[3];
-      int mine[3];
-#pragma unroll
-      for (int r = 0; r < 3; r++) {
-        k0[r] = (ford(V.out[lane][r]) & ~0x7fu) | (uint32_t)(127 - lane);
-        k1[r] = lane + 64 < CNN_CELLS ? (ford(V.out[lane + 64][r]) & ~0x7fu) | (uint32_t)(127 - lane - 64) : 0u;
-        mine[r] = 0;
-      }
-#pragma unroll
-      for (int k = 0; k < CNN_TOPK; k++)
-#pragma unroll
-        for (int r = 0; r < 3; r++)
-          if (r < nref) {
-            const int cell = 127 - (int)(wave_max(max(k0[r], k1[r])) & 0x7f);
-            if (cell == lane) k0[r] = 0;
-            if (cell == lane + 64) k1[r] = 0;
-            if (lane == k) mine[r] = cell;
-          }
-      wave_sync();
-#pragma unroll
-      for (int r = 0; r < 3; r++)
-        if (r < nref && lane < CNN_TOPK) V.cand[r][lane] = mine[r];
-      wave_sync();
-    }
-
-    // exact refinement of all 41 partitions around the selected cells, +-2 samples. Lanes 21r..21r+20
-    // take reference r; its 40 strips (cell rank k, column x of the cell's 5x5 block) go two per
-    // lane, and a strip walks its column's 5 displacements (each staged row read once feeds every
-    // candidate that uses it). Lane 63 idles.
-    uint32_t run[IA_NPART];
-#pragma unroll
-    for (int p = 0; p < IA_NPART; p++) run[p] = 0xffffffffu;
-    const int rg = lane / 21;
-    if (rg < nref) {
-      const int off = rg == 2 ? woff[2] : rg == 1 ? woff[1] : woff[0];
-      const int crx = rg == 2 ? cx[2] : rg == 1 ? cx[1] : cx[0];
-      const int cry = rg == 2 ? cy[2] : rg == 1 ? cy[1] : cy[0];
-      for (int strip = lane - 21 * rg; strip < 5 * CNN_TOPK; strip += 21) {
-        const int k = strip / 5, x = strip - 5 * k;
-        const int cell = V.cand[rg][k];
-        const int ox = 4 * (cell % CNN_GRID) + x, oy = 4 * (cell / CNN_GRID);  // window offset of the strip top
-        const int dx = crx - 18 + ox, dy0 = cry - 18 + oy;
-        uint32_t acc[5][16];
-#pragma unroll
-        for (int i = 0; i < 5; i++)
-#pragma unroll
-          for (int b = 0; b < 16; b++) acc[i][b] = (uint32_t)(k * 25 + i * 5 + x);  // order: rank, then raster
-#pragma unroll
-        for (int y = 0; y < 20; y++) {
-          uint32_t r0, r1, r2, r3;
-          win_row16(V.win[rg], off, ox, oy + y, r0, r1, r2, r3);
-#pragma unroll
-          for (int i = 0; i < 5; i++) {
-            const int row = y - i;
-            if (row >= 0 && row < 16) {
-              const int by = row >> 2;
-              acc[i][by * 4 + 0] = __builtin_amdgcn_sad_hi_u8(src[row * 4 + 0], r0, acc[i][by * 4 + 0]);
-              acc[i][by * 4 + 1] = __builtin_amdgcn_sad_hi_u8(src[row * 4 + 1], r1, acc[i][by * 4 + 1]);
-              acc[i][by * 4 + 2] = __builtin_amdgcn_sad_hi_u8(src[row * 4 + 2], r2, acc[i][by * 4 + 2]);
-              acc[i][by * 4 + 3] = __builtin_amdgcn_sad_hi_u8(src[row * 4 + 3], r3, acc[i][by * 4 + 3]);
-            }
-          }
-        }
-        const bool xin = dx >= lox && dx <= hix;
-#pragma unroll
-        for (int i = 0; i < 5; i++) {
-          if (xin && dy0 + i >= loy && dy0 + i <= hiy) {
-            uint32_t key[IA_NPART];
-            partition_keys(acc[i], key);
-#pragma unroll
-            for (int p = 0; p < IA_NPART; p++) run[p] = min(run[p], key[p]);
-          }
-        }
-      }
-    }
-    // the minima over each reference's lanes by a transpose through LDS (over the windows, now read):
-    // T[lane][p] = run[p]; output o = 41 r + p takes the minimum of column p over rows 21r..21r+20
-    wave_sync();
-    uint32_t *T = &V.win[0][0][0];
-#pragma unroll
-    for (int p = 0; p < IA_NPART; p++) T[lane * IA_NPART + p] = run[p];
-    wave_sync();
-    for (int o = lane; o < nref * IA_NPART; o += 64) {
-      const int r = o >= 2 * IA_NPART ? 2 : o >= IA_NPART ? 1 : 0, p = o - IA_NPART * r;
-      uint32_t v = 0xffffffffu;
-#pragma unroll
-      for (int l = 0; l < 21; l++) v = min(v, T[(21 * r + l) * IA_NPART + p]);
-      const int lg = p == 0 ? 4 : p < 5 ? 3 : p < 9 ? 2 : p < 25 ? 1 : 0;  // log2 of the 4x4 blocks summed
-      ia_fsresult res;
-      if (v == 0xffffffffu) {
-        res.mv[0] = res.mv[1] = 0;
-        res.sad = 0xffffffffu;
-      } else {
-        const int idx = (int)((v & 0xffffu) >> lg), k = idx / 25, j = idx % 25;
-        const int cell = V.cand[r][k];
-        const int crx = r == 2 ? cx[2] : r == 1 ? cx[1] : cx[0];
-        const int cry = r == 2 ? cy[2] : r == 1 ? cy[1] : cy[0];
-        res.mv[0] = (int16_t)((crx + 4 * (cell % CNN_GRID - 4) + (j % 5 - 2)) * 4);
-        res.mv[1] = (int16_t)((cry + 4 * (cell / CNN_GRID - 4) + (j / 5 - 2)) * 4);
-        res.sad = v >> 16;
-      }
-      a.out[((size_t)r * nmb + mb) * IA_NPART + p] = res;
-      // fallback when the refined 16x16 cost exceeds the ratio times the predicted one
-      if (p == 0 && a.fb_list) {
-        const float pred = exp2f(V.out[V.cand[r][0]][3]);
-        if (res.sad == 0xffffffffu || (float)res.sad > a.fallback_ratio * pred) {
-          const int slot = atomicAdd(a.fb_count, 1);
-          a.fb_list[slot] = mb * 4 + r;
-        }
-      }
-    }
-    wave_sync();  // the next macroblock restages the windows and features
+    propose_mb(L, V, args[sidx[lo]], gu - offs[lo], tid);
   }
 }
 
 }  // namespace
+
+hipError_t launch_proposal_batch(const PropArgs *args, const int *offs, const int *sidx, int nmem, int total,
+                                 hipStream_t stream) {
+  if (nmem < 1 || total < 1) return hipErrorInvalidValue;
+  dim3 grid(min((total + WAVES - 1) / WAVES, 256));
+  hipLaunchKernelGGL(proposal_batch_kernel, grid, dim3(64 * WAVES), sizeof(PropLds), stream, args, offs, sidx, nmem,
+                     total);
+  return hipGetLastError();
+}
 
 hipError_t launch_proposal(const PropArgs &a, hipStream_t stream) {
   const int nmb = a.mb_w * a.mb_h;

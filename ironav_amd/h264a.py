@@ -288,6 +288,13 @@ class Batch:
                 raise ValueError(msg)
             raise AnalysisError(f"error {rc}: {msg}")
 
+    def fallback_count(self):
+        c = C.c_int()
+        rc = library().ia_batch_fallback_count(self._h, C.byref(c))
+        if rc != abi.IA_OK:
+            raise AnalysisError(library().ia_batch_last_error(self._h).decode(errors="replace"))
+        return c.value
+
     def sync(self):
         rc = library().ia_batch_sync(self._h)
         if rc != abi.IA_OK:

@@ -124,3 +124,57 @@ def test_single_stream_chain_matches_oracle(oracle):
             assert np.array_equal(r.recon.y, want[2][0]) and np.array_equal(r.mbs, want[3]), t
             an.set_reference(promote, r.recon, r.mbs)
             refs_host[promote] = (r.recon, r.mbs)
+
+
+# learned streams (trained weights), one with every pair falling back, beside a full-search stream
+LSTREAMS = [(96, 64, 21, 16, 3, 2, 1.1), (160, 96, 22, 32, 2, 3, 1.1), (64, 48, 23, 16, 2, 1, 0.0),
+            (128, 96, 24, 32, 3, 2, float("inf"))]
+
+
+def learned_analyser(w, h, rng, nrefs, slices, ratio, params):
+    an = Analyser(w, h, nrefs=nrefs, slices=slices, search_mode=abi.SEARCH_LEARNED, search_range=rng,
+                  fallback_ratio=ratio)
+    an.load_proposal_weights(params)
+    return an
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_with_learned_streams_matches_single_streams(use_graph):
+    from ironav_amd import proposal
+
+    params = proposal.load_weights()
+    clips = [clip(w, h, seed) for (w, h, seed, *_r) in LSTREAMS] + [clip(*STREAMS[1][:3])]
+    # single-context runs
+    want = []
+    for i, spec in enumerate(LSTREAMS + [STREAMS[1]]):
+        an = learned_analyser(*spec[:2], spec[3], spec[4], spec[5], spec[6], params) if i < len(LSTREAMS) \
+            else analyser(*spec[:2], *spec[3:])
+        res = []
+        with an:
+            for t in range(FRAMES):
+                qp, kind, refs, promote = schedule(an.cfg.nrefs, t)
+                r = an.analyse(clips[i][t], qp, kind, refs)
+                an.set_reference(promote, r.recon, r.mbs)
+                res.append(r)
+        want.append(res)
+    # the batch
+    ans = [learned_analyser(*s[:2], s[3], s[4], s[5], s[6], params) for s in LSTREAMS] + [analyser(*STREAMS[1][:2], *STREAMS[1][3:])]
+    with Batch(ans, use_graph=use_graph) as b:
+        for t in range(FRAMES):
+            frames = []
+            for i, an in enumerate(ans):
+                qp, kind, refs, promote = schedule(an.cfg.nrefs, t)
+                an.upload(clips[i][t])
+                frames.append(dict(qp=qp, kind=kind, refs=refs, promote=promote))
+            b.step(frames)
+            b.sync()
+            nfb = b.fallback_count()
+            if t == 0:
+                assert nfb == 0
+            else:
+                # the stream with ratio 0 sends every (macroblock, reference)
+                assert nfb >= 12 * min(t, 2)
+            for i, an in enumerate(ans):
+                assert same(an.read_analysis(), want[i][t]), (i, t)
+    for an in ans:
+        an.close()
