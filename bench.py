@@ -322,11 +322,14 @@ def bench_batched(args):
         t1 = time.perf_counter()
         for t in range(2, args.frames):
             b.step([frame_args(i, t) for i in range(len(ans))])
+            if t % 5 == 0:
+                print(f"[batched] frame {t}/{args.frames}", file=sys.stderr, flush=True)
         b.sync()
         dt = time.perf_counter() - t1
     steps = max(0, args.frames - 2)
     # per-stream bit-exactness of frames 0 and 1 against the single-stream path
     equal = 0
+    print(f"[batched] timed {dt:.2f} s; checking streams against the single-stream path", file=sys.stderr, flush=True)
     for i, ((w, h), ns) in enumerate(zip(order, slices)):
         with Analyser(w, h, nrefs=nrefs, slices=ns, search_mode=abi.SEARCH_FULL, search_range=32, sub8x8=True) as an:
             autotune.apply(an, table)
@@ -430,6 +433,9 @@ def bench_learned_batched(args):
                     t_steps += 1
                 if mode == abi.SEARCH_LEARNED and any(f["kind"] == abi.PIC_P for f in fr):
                     fb += b.fallback_count()
+                if t % 10 == 0:
+                    print(f"[learned-batched] {'learned' if mode == abi.SEARCH_LEARNED else 'exhaustive'} frame {t}/{nfr}",
+                          file=sys.stderr, flush=True)
                 for an in ans:
                     an._ok(library().ia_read_analysis(an._h, None, abi.ptr(stats), None, None, None, None))
                     cost += [stats["rd_cost"].sum(), stats["inter_cost"][stats["inter_cost"] >= 0].sum()]
