@@ -30,6 +30,16 @@ constexpr int NXCD = 8;  // MI355X: accelerator complex dies, each with its own 
 
 __device__ __forceinline__ uint32_t ld_u32(const uint8_t *p) { return *reinterpret_cast<const uint32_t *>(p); }
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// v_min3_u32 written out: the compiler otherwise builds a tree over the candidates of a strip and
+// merges it into the running minimum, one instruction more per partition and strip.
+__device__ __forceinline__ uint32_t min3u(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t d;
+  asm("v_min3_u32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+  return d;
+}
+
 // The 41 partition words of one candidate from its 16 4x4 words (raster order), numbered as
 // IA_NPART documents. A word is (SAD << 16) + (4x4 blocks summed) * row, so one add per partition.
 __device__ __forceinline__ void partition_keys(const uint32_t *b, uint32_t *key) {
@@ -60,6 +70,12 @@ __device__ __forceinline__ uint32_t dpp_min(uint32_t v) {
   return min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)0xffffffff, (int)v, CTRL, ROW_MASK, BANK_MASK, false));
 }
 
+template <int CTRL, int ROW_MASK, int BANK_MASK>
+__device__ __forceinline__ void dpp_min_all(uint32_t *m) {
+#pragma unroll
+  for (int p = 0; p < IA_NPART; p++) m[p] = dpp_min<CTRL, ROW_MASK, BANK_MASK>(m[p]);
+}
+
 // Minimum over the 64 lanes of a wave: an inclusive scan by DPP (row shifts, then row broadcasts),
 // whose last lane holds the whole wave's minimum.
 __device__ __forceinline__ uint32_t wave_min(uint32_t v) {
@@ -73,14 +89,23 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
-// Dwords per staged row: the window's first byte sits up to 3 bytes into the first dword, and the
-// last candidate column reads 5 dwords from its own.
-constexpr int raws_for(int rmax) { return ((3 + 2 * rmax + 16) >> 2) + 2; }
-// The kernels come in two window sizes: ranges up to 32 and up to 64.
-constexpr int rmax_of(int raws) { return raws == raws_for(32) ? 32 : 64; }
+// Two staged layouts, by the largest range a kernel serves:
+//  - ranges up to 32, "shifted": staged row dword e holds the plane's 4 bytes from byte al + e on (al
+//    the window's first dword boundary), so every byte alignment is stored once and a candidate's 4
+//    dwords of a row are e = b0, b0 + 4, b0 + 8, b0 + 12: no realignment in the loop, and 64 lanes
+//    at consecutive columns read 64 consecutive dwords. The last candidate column starts at byte
+//    b0 <= 3 + 64, so a row holds 80 dwords (20 per alignment).
+//  - ranges up to 64: aligned dwords (a row of 4x fewer dwords keeps the larger window within LDS);
+//    the loop realigns 5 dwords into 4 with v_alignbyte.
+constexpr int row_dwords(int rmax) { return rmax == 32 ? 4 * (((3 + 2 * rmax) >> 2) + 4) : ((3 + 2 * rmax + 16) >> 2) + 2; }
+constexpr int RW32 = row_dwords(32), RW64 = row_dwords(64);
+constexpr int rmax_of(int rw) { return rw == RW32 ? 32 : 64; }
+// Staged rows a K-row strip walk reads: strips start at rows 0, K, ... below the window's last row
+// 2 * range and read K + 15 rows each.
+inline int rows_for(int k, int range) { return (2 * range + k) / k * k + 15; }
 
-// K: candidates per strip (rows of one column); NT: threads; RAWS: dwords per staged window row;
-// WPE: waves per SIMD the register allocation must allow.
+// K: candidates per strip (rows of one column); NT: threads; RAWS: dwords per staged window row
+// (row_dwords of the layout); WPE: waves per SIMD the register allocation must allow.
 template <int K, int NT, int RAWS>
 __device__ __forceinline__ void search_one(const FsArgs &a, int rows_lds, int mb, int r, uint32_t *raw) {
   const int mbx = mb % a.mb_w, mby = mb / a.mb_w;
@@ -92,19 +117,37 @@ __device__ __forceinline__ void search_one(const FsArgs &a, int rows_lds, int mb
   const int S = (NR + K - 1) / K;
   // a lane may visit several columns of one strip (when C > NT): its keys order them by row, then
   // by the lane's column step c / NT, so that ties resolve in raster order and decode exactly
+  constexpr bool SHIFTED = rmax_of(RAWS) == 32;
   constexpr bool MULTI = 2 * rmax_of(RAWS) + 1 > NT;  // can a lane visit two columns of a row?
+  // are a lane's passes more than a wave's width plus a window row apart (see the reduction)?
+  constexpr bool LANE_ORDER = !MULTI && 2 * rmax_of(RAWS) + 1 <= NT - 63;
   const int TQ = MULTI ? (C + NT - 1) / NT : 1;
   const uint8_t *g = a.ref_g[r];
   const int tid = threadIdx.x;
 
-  // stage the window's rows as aligned dwords: row rr holds plane bytes from `al` on, where the
-  // window's first column sits `wxo` bytes in
+  // stage the window's rows; the window's first column sits `wxo` bytes past the dword boundary `al`
   const int wx = x0 + dxlo + PAD, wy = y0 + dylo + PAD;
   const int al = wx & ~3, wxo = wx - al;
-  for (int e = tid; e < rows_lds * RAWS; e += NT) {
-    const int rr = e / RAWS, k = e - rr * RAWS;
-    const int gy = min(wy + rr, sh - 1);
-    raw[e] = ld_u32(g + (size_t)gy * sw + min(al + 4 * k, sw - 4));
+  if constexpr (SHIFTED) {
+    // item (row, d): the aligned plane dwords d and d + 1 give the row's dwords 4d .. 4d + 3
+    constexpr int ND = RAWS / 4;
+    for (int e = tid; e < rows_lds * ND; e += NT) {
+      const int rr = e / ND, d = e - rr * ND;
+      const uint8_t *row = g + (size_t)min(wy + rr, sh - 1) * sw;
+      const uint32_t lo = ld_u32(row + min(al + 4 * d, sw - 4)), hi = ld_u32(row + min(al + 4 * d + 4, sw - 4));
+      u32x4 v;
+      v.x = lo;
+      v.y = __builtin_amdgcn_alignbyte(hi, lo, 1);
+      v.z = __builtin_amdgcn_alignbyte(hi, lo, 2);
+      v.w = __builtin_amdgcn_alignbyte(hi, lo, 3);
+      *reinterpret_cast<u32x4 *>(raw + (size_t)rr * RAWS + 4 * d) = v;
+    }
+  } else {
+    for (int e = tid; e < rows_lds * RAWS; e += NT) {
+      const int rr = e / RAWS, k = e - rr * RAWS;
+      const int gy = min(wy + rr, sh - 1);
+      raw[e] = ld_u32(g + (size_t)gy * sw + min(al + 4 * k, sw - 4));
+    }
   }
   // the macroblock's samples, uniform across the workgroup
   uint32_t src[64];
@@ -121,7 +164,7 @@ __device__ __forceinline__ void search_one(const FsArgs &a, int rows_lds, int mb
     const int s = q / C, c = q - s * C;
     const int b0 = wxo + c;                 // byte of the candidate's first sample in a staged row
     const uint32_t sh8 = (uint32_t)(b0 & 3);
-    const uint32_t *col = raw + (b0 >> 2) + s * K * RAWS;
+    const uint32_t *col = raw + (SHIFTED ? b0 : b0 >> 2) + s * K * RAWS;
     const int i0 = s * K;
     uint32_t acc[K][16];
 #pragma unroll
@@ -131,9 +174,14 @@ __device__ __forceinline__ void search_one(const FsArgs &a, int rows_lds, int mb
 #pragma unroll
     for (int j = 0; j < K + 15; j++) {
       const uint32_t *rp = col + j * RAWS;
-      const uint32_t w0 = rp[0], w1 = rp[1], w2 = rp[2], w3 = rp[3], w4 = rp[4];
-      const uint32_t r0 = __builtin_amdgcn_alignbyte(w1, w0, sh8), r1 = __builtin_amdgcn_alignbyte(w2, w1, sh8);
-      const uint32_t r2 = __builtin_amdgcn_alignbyte(w3, w2, sh8), r3 = __builtin_amdgcn_alignbyte(w4, w3, sh8);
+      uint32_t r0, r1, r2, r3;
+      if constexpr (SHIFTED) {
+        r0 = rp[0], r1 = rp[4], r2 = rp[8], r3 = rp[12];
+      } else {
+        const uint32_t w0 = rp[0], w1 = rp[1], w2 = rp[2], w3 = rp[3], w4 = rp[4];
+        r0 = __builtin_amdgcn_alignbyte(w1, w0, sh8), r1 = __builtin_amdgcn_alignbyte(w2, w1, sh8);
+        r2 = __builtin_amdgcn_alignbyte(w3, w2, sh8), r3 = __builtin_amdgcn_alignbyte(w4, w3, sh8);
+      }
 #pragma unroll
       for (int k = 0; k < K; k++) {
         const int rr = j - k;
@@ -155,7 +203,7 @@ __device__ __forceinline__ void search_one(const FsArgs &a, int rows_lds, int mb
         if (k + 1 < K) {
           partition_keys(acc[k + 1], kb);
 #pragma unroll
-          for (int p = 0; p < IA_NPART; p++) run[p] = min(run[p], min(ka[p], kb[p]));
+          for (int p = 0; p < IA_NPART; p++) run[p] = min3u(run[p], ka[p], kb[p]);
         } else {
 #pragma unroll
           for (int p = 0; p < IA_NPART; p++) run[p] = min(run[p], ka[p]);
@@ -175,24 +223,57 @@ __device__ __forceinline__ void search_one(const FsArgs &a, int rows_lds, int mb
     }
   }
 
-  // rewrite each running minimum as (SAD << 15) | raster index, then reduce over the workgroup
+  // reduce over the wave, then over the workgroup as (SAD << 15) | raster index
   __shared__ uint32_t red[NT / 64][IA_NPART];
   const int lane = tid & 63, wave = tid >> 6;
+  if constexpr (LANE_ORDER) {
+    // The low half of a running minimum is the absolute row times the blocks summed, so keys compare
+    // across lanes up to the column; lanes of one wave that met the same row did so in the same pass
+    // (two passes of a lane are NT > C + 62 positions apart), where a lower lane holds a lower column.
+    // So the wave's least key, taken in its lowest lane, is the wave's winner; it is decoded once,
+    // from uniform values. The DPP steps run over all partitions at once.
+    const int wave_u = __builtin_amdgcn_readfirstlane(wave);  // the decode then runs on the scalar unit
+    uint32_t m[IA_NPART];
 #pragma unroll
-  for (int p = 0; p < IA_NPART; p++) {
-    const int lg = p == 0 ? 4 : p < 5 ? 3 : p < 9 ? 2 : p < 25 ? 1 : 0;  // log2 of the 4x4 blocks summed
-    const uint32_t v = run[p];
-    uint32_t gk = 0xffffffffu;
-    if (v != 0xffffffffu) {
-      const uint32_t sad = v >> 16;
-      const int o = (int)((v & 0xffffu) >> lg), i = o / TQ;
-      const int s = i / K;
-      const int t = s * C > tid ? (s * C - tid + NT - 1) / NT : 0;
-      const int c = tid + t * NT - s * C + (o - i * TQ) * NT;  // the lane's first column of the strip, stepped
-      gk = (sad << 15) | (uint32_t)(i * C + c);
+    for (int p = 0; p < IA_NPART; p++) m[p] = run[p];
+    dpp_min_all<0x111, 0xf, 0xf>(m);
+    dpp_min_all<0x112, 0xf, 0xf>(m);
+    dpp_min_all<0x113, 0xf, 0xf>(m);
+    dpp_min_all<0x114, 0xf, 0xe>(m);
+    dpp_min_all<0x118, 0xf, 0xc>(m);
+    dpp_min_all<0x142, 0xa, 0xf>(m);
+    dpp_min_all<0x143, 0xc, 0xf>(m);
+#pragma unroll
+    for (int p = 0; p < IA_NPART; p++) {
+      const int lg = p == 0 ? 4 : p < 5 ? 3 : p < 9 ? 2 : p < 25 ? 1 : 0;  // log2 of the 4x4 blocks summed
+      const uint32_t wm = (uint32_t)__builtin_amdgcn_readlane((int)m[p], 63);
+      uint32_t gk = 0xffffffffu;
+      if (wm != 0xffffffffu) {
+        const int tl = wave_u * 64 + __ffsll((unsigned long long)__ballot(run[p] == wm)) - 1;
+        const int i = (int)((wm & 0xffffu) >> lg), s = i / K;
+        const int t = s * C > tl ? (s * C - tl + NT - 1) / NT : 0;
+        gk = ((wm >> 16) << 15) | (uint32_t)(i * C + tl + t * NT - s * C);
+      }
+      if (lane == 0) red[wave][p] = gk;
     }
-    gk = wave_min(gk);
-    if (lane == 0) red[wave][p] = gk;
+  } else {
+    // each lane rewrites its minima as (SAD << 15) | raster index first
+#pragma unroll
+    for (int p = 0; p < IA_NPART; p++) {
+      const int lg = p == 0 ? 4 : p < 5 ? 3 : p < 9 ? 2 : p < 25 ? 1 : 0;
+      const uint32_t v = run[p];
+      uint32_t gk = 0xffffffffu;
+      if (v != 0xffffffffu) {
+        const uint32_t sad = v >> 16;
+        const int o = (int)((v & 0xffffu) >> lg), i = o / TQ;
+        const int s = i / K;
+        const int t = s * C > tid ? (s * C - tid + NT - 1) / NT : 0;
+        const int c = tid + t * NT - s * C + (o - i * TQ) * NT;  // the lane's first column of the strip, stepped
+        gk = (sad << 15) | (uint32_t)(i * C + c);
+      }
+      gk = wave_min(gk);
+      if (lane == 0) red[wave][p] = gk;
+    }
   }
   __syncthreads();
   if (tid < IA_NPART) {
@@ -219,7 +300,7 @@ __device__ __forceinline__ void search_one(const FsArgs &a, int rows_lds, int mb
 template <int K, int NT, int RAWS, int WPE>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void fullsearch_kernel(FsArgs a,
                                                                                                  int rows_lds) {
-  extern __shared__ uint32_t raw[];
+  extern __shared__ __align__(16) uint32_t raw[];
   const int x = blockIdx.x, band = gridDim.x / NXCD;
   const int mb = x < band * NXCD ? (x % NXCD) * band + x / NXCD : x;
   search_one<K, NT, RAWS>(a, rows_lds, mb, blockIdx.y, raw);
@@ -233,7 +314,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 template <int K, int NT, int RAWS, int WPE>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void fullsearch_batch_kernel(
     const FsArgs *args, const int *offs, const int *sidx, int nmem, int rows_lds) {
-  extern __shared__ uint32_t raw[];
+  extern __shared__ __align__(16) uint32_t raw[];
   const int b = blockIdx.x;
   int lo = 0, hi = nmem - 1;
   while (lo < hi) {
@@ -255,7 +336,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 template <int K, int NT, int RAWS, int WPE>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void fullsearch_list_batch_kernel(
     const FsArgs *args, int rows_lds, const int *list, const int *count) {
-  extern __shared__ uint32_t raw[];
+  extern __shared__ __align__(16) uint32_t raw[];
   const int n = *count;
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     const int e = list[i];
@@ -269,7 +350,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 template <int K, int NT, int RAWS, int WPE>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void fullsearch_list_kernel(
     FsArgs a, int rows_lds, const int *list, const int *count, int *reset) {
-  extern __shared__ uint32_t raw[];
+  extern __shared__ __align__(16) uint32_t raw[];
   if (blockIdx.x == 0 && threadIdx.x == 0) *reset = 0;
   if ((int)blockIdx.x >= *count) return;
   const int e = list[blockIdx.x];
@@ -287,8 +368,8 @@ struct Variant {
 };
 
 #define IA_FS_VARIANT(K, NT, W)                                                                          \
-  {"k" #K "_t" #NT "_w" #W, K, NT, fullsearch_kernel<K, NT, raws_for(32), W>, fullsearch_kernel<K, NT, raws_for(64), W>, \
-   fullsearch_batch_kernel<K, NT, raws_for(32), W>, fullsearch_batch_kernel<K, NT, raws_for(64), W>}
+  {"k" #K "_t" #NT "_w" #W, K, NT, fullsearch_kernel<K, NT, RW32, W>, fullsearch_kernel<K, NT, RW64, W>, \
+   fullsearch_batch_kernel<K, NT, RW32, W>, fullsearch_batch_kernel<K, NT, RW64, W>}
 const Variant VARIANTS[] = {
     IA_FS_VARIANT(4, 256, 3), IA_FS_VARIANT(6, 256, 3), IA_FS_VARIANT(8, 256, 2), IA_FS_VARIANT(4, 128, 3),
     IA_FS_VARIANT(6, 128, 3), IA_FS_VARIANT(6, 256, 2),
@@ -310,9 +391,8 @@ hipError_t launch_fullsearch(const FsArgs &a, int variant, hipStream_t stream) {
   if (a.range < 1 || a.range > RMAX || a.nref < 1 || a.nref > 16) return hipErrorInvalidValue;
   const Variant &v = VARIANTS[variant];
   const bool small = a.range <= 32;
-  const int raws = small ? raws_for(32) : raws_for(64);
-  // rows a strip may read: the last strip starts below row NR-1 and reads K+15 rows
-  const int rows_lds = 2 * a.range + 1 + v.k + 15;
+  const int raws = small ? RW32 : RW64;
+  const int rows_lds = rows_for(v.k, a.range);
   const size_t shmem = (size_t)rows_lds * raws * sizeof(uint32_t);
   dim3 grid(a.mb_w * a.mb_h, a.nref);
   hipLaunchKernelGGL(small ? v.fn32 : v.fn64, grid, dim3(v.nt), shmem, stream, a, rows_lds);
@@ -327,8 +407,8 @@ hipError_t launch_fullsearch_batch(const FsArgs *args, const int *offs, const in
   if (max_range < 1 || max_range > RMAX || nmem < 1 || nblocks < 1) return hipErrorInvalidValue;
   const Variant &v = VARIANTS[variant];
   const bool small = max_range <= 32;
-  const int raws = small ? raws_for(32) : raws_for(64);
-  const int rows_lds = 2 * max_range + 1 + v.k + 15;
+  const int raws = small ? RW32 : RW64;
+  const int rows_lds = rows_for(v.k, max_range);
   const size_t shmem = (size_t)rows_lds * raws * sizeof(uint32_t);
   hipLaunchKernelGGL(small ? v.bfn32 : v.bfn64, dim3(nblocks), dim3(v.nt), shmem, stream, args, offs, sidx, nmem,
                      rows_lds);
@@ -339,12 +419,12 @@ hipError_t launch_fullsearch_list_batch(const FsArgs *args, const int *list, con
                                         int max_entries, hipStream_t stream) {
   if (max_range < 1 || max_range > RMAX || max_entries < 1) return hipErrorInvalidValue;
   const bool small = max_range <= 32;
-  const int raws = small ? raws_for(32) : raws_for(64);
-  const int rows_lds = 2 * max_range + 1 + LIST_K + 15;
+  const int raws = small ? RW32 : RW64;
+  const int rows_lds = rows_for(LIST_K, max_range);
   const size_t shmem = (size_t)rows_lds * raws * sizeof(uint32_t);
   dim3 grid(min(max_entries, 4096));
-  hipLaunchKernelGGL(small ? (fullsearch_list_batch_kernel<LIST_K, LIST_NT, raws_for(32), LIST_W>)
-                           : (fullsearch_list_batch_kernel<LIST_K, LIST_NT, raws_for(64), LIST_W>),
+  hipLaunchKernelGGL(small ? (fullsearch_list_batch_kernel<LIST_K, LIST_NT, RW32, LIST_W>)
+                           : (fullsearch_list_batch_kernel<LIST_K, LIST_NT, RW64, LIST_W>),
                      grid, dim3(LIST_NT), shmem, stream, args, rows_lds, list, count);
   return hipGetLastError();
 }
@@ -353,12 +433,12 @@ hipError_t launch_fullsearch_list(const FsArgs &a, const int *list, const int *c
                                   hipStream_t stream) {
   if (a.range < 1 || a.range > RMAX || a.nref < 1 || a.nref > 4) return hipErrorInvalidValue;
   const bool small = a.range <= 32;
-  const int raws = small ? raws_for(32) : raws_for(64);
-  const int rows_lds = 2 * a.range + 1 + LIST_K + 15;
+  const int raws = small ? RW32 : RW64;
+  const int rows_lds = rows_for(LIST_K, a.range);
   const size_t shmem = (size_t)rows_lds * raws * sizeof(uint32_t);
   dim3 grid(a.mb_w * a.mb_h * a.nref);
-  hipLaunchKernelGGL(small ? (fullsearch_list_kernel<LIST_K, LIST_NT, raws_for(32), LIST_W>)
-                           : (fullsearch_list_kernel<LIST_K, LIST_NT, raws_for(64), LIST_W>),
+  hipLaunchKernelGGL(small ? (fullsearch_list_kernel<LIST_K, LIST_NT, RW32, LIST_W>)
+                           : (fullsearch_list_kernel<LIST_K, LIST_NT, RW64, LIST_W>),
                      grid, dim3(LIST_NT), shmem, stream, a, rows_lds, list, count, reset);
   return hipGetLastError();
 }
