@@ -88,6 +88,13 @@ __device__ __forceinline__ void partition_keys(const uint32_t *b, uint32_t *key)
   key[0] = key[1] + key[2];
 }
 
+// v_min3_u32 written out (the compiler otherwise reduces the pair first, one instruction more)
+__device__ __forceinline__ uint32_t min3u(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t d;
+  asm("v_min3_u32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+  return d;
+}
+
 // a float's order as an unsigned integer
 __device__ __forceinline__ uint32_t ford(float f) {
   const uint32_t u = __float_as_uint(f);
@@ -143,13 +150,17 @@ __device__ void conv_layer(int lane, const uint16_t *in, int ist, const uint16_t
   // the whole layer's output stays in registers until every lane has read the input, so the output
   // may overwrite it. Operands: A = the weights (rows = output channels), B = the activations
   // (columns = cells), so D gives each lane 4 consecutive output channels of one cell.
+  // the accumulators start at the bias (this lane's 4 output channels of each tile)
+  f32x4 bv[NT];
+#pragma unroll
+  for (int nt = 0; nt < NT; nt++) bv[nt] = *reinterpret_cast<const f32x4 *>(bias + nt * 16 + 4 * g);
   f32x4 acc[6][NT];
 #pragma unroll
   for (int mt = 0; mt < 6; mt++) {
     const int p = min(mt * 16 + m, CNN_CELLS - 1);  // this lane's B column (grid cell)
     const int pp = padpos(p);
 #pragma unroll
-    for (int nt = 0; nt < NT; nt++) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int nt = 0; nt < NT; nt++) acc[mt][nt] = bv[nt];
 #pragma unroll
     for (int ks = 0; ks < KSTEPS; ks++) {
       bf16x8 x;
@@ -170,12 +181,11 @@ __device__ void conv_layer(int lane, const uint16_t *in, int ist, const uint16_t
 #pragma unroll
   for (int nt = 0; nt < NT; nt++) {
     const int c0 = nt * 16 + 4 * g;
-    const f32x4 bv = *reinterpret_cast<const f32x4 *>(bias + c0);
 #pragma unroll
     for (int mt = 0; mt < 6; mt++) {
       const int q = mt * 16 + m;
       if (q < CNN_CELLS) {
-        f32x4 v = acc[mt][nt] + bv;
+        f32x4 v = acc[mt][nt];
         if (RELU) {
 #pragma unroll
           for (int i = 0; i < 4; i++) v[i] = fmaxf(v[i], 0.f);
@@ -215,6 +225,19 @@ __device__ __forceinline__ void put_window(int lane, const uint32_t *w, uint32_t
   uint32_t *q = &win[lane >> 4][lane & 15];
 #pragma unroll
   for (int k = 0; k < WIN / 4; k++) q[k * 4 * WST] = w[k];
+}
+
+// The window shifted by `sh` bytes (dword q of a row holds the row's bytes 4q + sh .. 4q + sh + 3):
+// the cost volume's cells share one byte alignment per reference, so it reads aligned dwords. The
+// next dword of a row sits in the next lane of the same 16-lane DPP row (dwords 0..14 are exact;
+// the row's last dword is never read).
+__device__ __forceinline__ void put_window_shifted(int lane, const uint32_t *w, uint32_t (*win)[WST], uint32_t sh) {
+  uint32_t *q = &win[lane >> 4][lane & 15];
+#pragma unroll
+  for (int k = 0; k < WIN / 4; k++) {
+    const uint32_t nx = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)w[k], 0x101, 0xf, 0xf, false);  // row_shl:1
+    q[k * 4 * WST] = __builtin_amdgcn_alignbyte(nx, w[k], sh);
+  }
 }
 
 // the 4 dwords of 16 window samples at window column x of row y
@@ -313,9 +336,10 @@ __device__ void propose_mb(PropLds &L, WaveLds &V, const PropArgs &a, int mb, in
   uint32_t src[64];
 #pragma unroll
   for (int i = 0; i < 64; i++) src[i] = __builtin_amdgcn_readlane(sv, i);
+  // for the cost volume, each window shifted to its cells' byte alignment
 #pragma unroll
   for (int r = 0; r < 3; r++)
-    if (r < nref) put_window(lane, wreg[r], V.win[r]);
+    if (r < nref) put_window_shifted(lane, wreg[r], V.win[r], (uint32_t)((woff[r] + 2) & 3));
   for (int i = lane; i < PCELLS * 8 / 8; i += 64) reinterpret_cast<uint4 *>(&V.x0[0][0])[i] = make_uint4(0, 0, 0, 0);
   wave_sync();
 
@@ -328,12 +352,13 @@ __device__ void propose_mb(PropLds &L, WaveLds &V, const PropArgs &a, int mb, in
     for (int t = lane; t < nref * CNN_CELLS; t += 64) {
       const int r = t >= 2 * CNN_CELLS ? 2 : t >= CNN_CELLS ? 1 : 0, c = t - r * CNN_CELLS;
       const int off = r == 2 ? woff[2] : r == 1 ? woff[1] : woff[0];
-      const int wxp = 4 * (c % CNN_GRID) + 2, wyp = 4 * (c / CNN_GRID) + 2;  // window offset of the cell
+      // the cell's first sample is window byte off + 2 + 4 gx: dword (off + 2) / 4 + gx of the shifted rows
+      const int q0 = ((off + 2) >> 2) + c % CNN_GRID, wyp = 4 * (c / CNN_GRID) + 2;
       uint32_t s = 0;
 #pragma unroll
       for (int row = 0; row < 16; row++) {
-        uint32_t r0, r1, r2, r3;
-        win_row16(V.win[r], off, wxp, wyp + row, r0, r1, r2, r3);
+        const uint32_t *wp = &V.win[r][wyp + row][q0];
+        const uint32_t r0 = wp[0], r1 = wp[1], r2 = wp[2], r3 = wp[3];
         s = __builtin_amdgcn_sad_u8(src[row * 4 + 0], r0, s);
         s = __builtin_amdgcn_sad_u8(src[row * 4 + 1], r1, s);
         s = __builtin_amdgcn_sad_u8(src[row * 4 + 2], r2, s);
@@ -451,13 +476,27 @@ __device__ void propose_mb(PropLds &L, WaveLds &V, const PropArgs &a, int mb, in
         }
       }
       const bool xin = dx >= lox && dx <= hix;
+      // candidates 0+1 and 2+3 by v_min3 when both lie within the macroblock's vectors, else one by one
 #pragma unroll
-      for (int i = 0; i < 5; i++) {
-        if (xin && dy0 + i >= loy && dy0 + i <= hiy) {
-          uint32_t key[IA_NPART];
-          partition_keys(acc[i], key);
+      for (int i = 0; i < 5; i += 2) {
+        const bool in0 = xin && dy0 + i >= loy && dy0 + i <= hiy;
+        const bool in1 = i + 1 < 5 && xin && dy0 + i + 1 >= loy && dy0 + i + 1 <= hiy;
+        if (i + 1 < 5 && in0 && in1) {
+          uint32_t ka[IA_NPART], kb[IA_NPART];
+          partition_keys(acc[i], ka);
+          partition_keys(acc[i + 1], kb);
 #pragma unroll
-          for (int p = 0; p < IA_NPART; p++) run[p] = min(run[p], key[p]);
+          for (int p = 0; p < IA_NPART; p++) run[p] = min3u(run[p], ka[p], kb[p]);
+        } else {
+#pragma unroll
+          for (int j = 0; j < 2; j++) {
+            if (i + j < 5 && (j == 0 ? in0 : in1)) {
+              uint32_t key[IA_NPART];
+              partition_keys(acc[i + j], key);
+#pragma unroll
+              for (int p = 0; p < IA_NPART; p++) run[p] = min(run[p], key[p]);
+            }
+          }
         }
       }
     }

@@ -189,3 +189,108 @@ def test_fallback_count_per_call_and_fallback_off(oracle):
         got = an.read_motion(2)
         # the proposal's own winners (its grid is not bounded by the exhaustive search's range)
         assert np.all(got["sad"][:, :, 0] < 0xFFFFFFFF)
+
+
+def _sad(src, gplane, x0, y0, w, h, dx, dy):
+    """SAD of the w x h block at (x0, y0) against the G plane (48 samples of border) at (dx, dy)."""
+    blk = gplane[y0 + dy + 48:y0 + dy + 48 + h, x0 + dx + 48:x0 + dx + 48 + w].astype(np.int32)
+    return int(np.abs(src[y0:y0 + h, x0:x0 + w].astype(np.int32) - blk).sum())
+
+
+def test_cost_volume_features_match_numpy():
+    """The CNN's input features, restated in numpy: the grid centres from reference 0's co-located
+    vector (scaled by the reference's distance, to a multiple of 4 samples, clamped inside the
+    macroblock's vectors); channels 0..2 log2(1 + SAD16) / 16 at the 81 cells 4 samples apart (the SAD
+    recomputed from the G plane and recovered from the feature); channels 3..5 the cells of the
+    co-located, left and above vectors; channel 6 one, channel 7 zero."""
+    w, h, nref = 208, 144, 3
+    mb_w, mb_h = 13, 9
+    pics = frames(w, h, nref + 1, 31, max_motion=12)
+    rng = np.random.default_rng(5)
+    n = mb_w * mb_h
+    mbs = np.zeros(n, dtype=abi.MBINFO_DTYPE)
+    mbs["kind"] = np.where(rng.random(n) < 0.7, 5, 1)
+    mbs["mv"][:, 5, 0] = rng.integers(-90, 91, n)
+    mbs["mv"][:, 5, 1] = rng.integers(-90, 91, n)
+    with Analyser(w, h, nrefs=3, search_mode=abi.SEARCH_LEARNED, search_range=32) as an:
+        an.load_proposal_weights(proposal.init_params(15))
+        for r in range(nref):
+            an.set_reference(r, pics[r], mbs if r == 0 else None)
+        an.upload(pics[nref])
+        feat, ctr = an.proposal_features(list(range(nref)))
+        G = [an.read_halfpel(r)[0] for r in range(nref)]
+    src = pics[nref].y
+    ctr = ctr.reshape(n, 3, 2)
+    for mb in range(n):
+        mbx, mby = mb % mb_w, mb // mb_w
+        x0, y0 = mbx * 16, mby * 16
+        lox, hix, loy, hiy = -(x0 + 24), w - x0 + 8, -(y0 + 24), h - y0 + 8
+        recs = [(mbx, mby), (mbx - 1, mby), (mbx, mby - 1)]
+        tmv = []
+        for nx, ny in recs:
+            m = mbs[ny * mb_w + nx] if nx >= 0 and ny >= 0 else None
+            tmv.append((int(m["mv"][5][0]), int(m["mv"][5][1])) if m is not None and m["kind"] == 5 else None)
+        c0 = tmv[0] or (0, 0)
+        for r in range(nref):
+            px = int(np.floor(((r + 1) * c0[0] + 8) / 16)) * 4
+            py = int(np.floor(((r + 1) * c0[1] + 8) / 16)) * 4
+            cx, cy = min(max(px, lox + 16), hix - 16), min(max(py, loy + 16), hiy - 16)
+            assert (int(ctr[mb, r, 0]), int(ctr[mb, r, 1])) == (cx, cy), (mb, r)
+            for c in range(81):
+                gx, gy = c % 9, c // 9
+                s = _sad(src, G[r], x0, y0, 16, 16, cx + 4 * (gx - 4), cy + 4 * (gy - 4))
+                f = float(feat[mb, c, r])
+                assert abs(f - np.log2(1.0 + s) / 16) <= 2e-6, (mb, r, c)
+                assert abs(2.0 ** (16 * f) - 1 - s) <= 1e-4 * s + 0.05, (mb, r, c, s)
+        cx0, cy0 = int(ctr[mb, 0, 0]), int(ctr[mb, 0, 1])
+        for c in range(81):
+            gx, gy = c % 9, c // 9
+            for k in range(3):
+                hit = tmv[k] is not None and (int(np.floor((tmv[k][0] - 4 * cx0 + 8) / 16)) + 4,
+                                              int(np.floor((tmv[k][1] - 4 * cy0 + 8) / 16)) + 4) == (gx, gy)
+                assert feat[mb, c, 3 + k] == (1.0 if hit else 0.0), (mb, c, k)
+        assert np.all(feat[mb, :, 6] == 1.0) and np.all(feat[mb, :, 7] == 0.0)
+
+
+def test_learned_refinement_is_locally_optimal():
+    """Every position of a selected cell's 5x5 block is evaluated, so each returned partition winner's
+    SAD is at most the SAD at every position that lies in all the blocks (of grid cells around the
+    reference's centre) that contain the returned vector; and it is exact at the returned vector."""
+    params = proposal.init_params(11)
+    w, h, nref = 128, 96, 3
+    pics = frames(w, h, nref + 1, 17, max_motion=8)
+    with Analyser(w, h, nrefs=3, search_mode=abi.SEARCH_LEARNED, search_range=32, fallback_ratio=1e9) as an:
+        an.load_proposal_weights(params)
+        for r in range(nref):
+            an.set_reference(r, pics[r])
+        an.upload(pics[nref])
+        an.motion_search(list(range(nref)))
+        got = an.read_motion(nref)
+        _, ctr = an.proposal_features(list(range(nref)))
+        G = [an.read_halfpel(r)[0] for r in range(nref)]
+    src = pics[nref].y
+    mb_w = w // 16
+    ctr = ctr.reshape(-1, 3, 2)
+    checked = 0
+    for r in range(nref):
+        for mb in range(0, (w // 16) * (h // 16), 3):
+            mbx, mby = mb % mb_w, mb // mb_w
+            cx, cy = int(ctr[mb, r, 0]), int(ctr[mb, r, 1])
+            lox, hix, loy, hiy = -(mbx * 16 + 24), w - mbx * 16 + 8, -(mby * 16 + 24), h - mby * 16 + 8
+            for p, (x, y, pw_, ph_) in enumerate(abi.PARTITIONS):
+                dx, dy = (int(v) // 4 for v in got[r, mb]["mv"][p])
+                x0, y0 = mbx * 16 + x * 4, mby * 16 + y * 4
+                sad = int(got[r, mb]["sad"][p])
+                assert sad == _sad(src, G[r], x0, y0, pw_ * 4, ph_ * 4, dx, dy), (r, mb, p)
+                # grid cells whose 5x5 block holds (dx, dy)
+                cells = [(gx, gy) for gx in range(9) for gy in range(9)
+                         if abs(dx - (cx + 4 * (gx - 4))) <= 2 and abs(dy - (cy + 4 * (gy - 4))) <= 2]
+                assert cells, (r, mb, p)
+                xs = set.intersection(*[set(range(cx + 4 * (gx - 4) - 2, cx + 4 * (gx - 4) + 3)) for gx, _ in cells])
+                ys = set.intersection(*[set(range(cy + 4 * (gy - 4) - 2, cy + 4 * (gy - 4) + 3)) for _, gy in cells])
+                for ex in xs:
+                    for ey in ys:
+                        if lox <= ex <= hix and loy <= ey <= hiy:
+                            assert sad <= _sad(src, G[r], x0, y0, pw_ * 4, ph_ * 4, ex, ey), (r, mb, p, ex, ey)
+                            checked += 1
+    assert checked > 1000
