@@ -31,6 +31,7 @@ constexpr int NXCD = 8;  // MI355X: accelerator complex dies, each with its own 
 __device__ __forceinline__ uint32_t ld_u32(const uint8_t *p) { return *reinterpret_cast<const uint32_t *>(p); }
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(4))) const uint32_t CU32;  // constant address space: scalar loads
 
 // v_min3_u32 written out: the compiler otherwise builds a tree over the candidates of a strip and
 // merges it into the running minimum, one instruction more per partition and strip.
@@ -76,12 +77,11 @@ __device__ __forceinline__ void dpp_min_all(uint32_t *m) {
   for (int p = 0; p < IA_NPART; p++) m[p] = dpp_min<CTRL, ROW_MASK, BANK_MASK>(m[p]);
 }
 
-// Minimum over the 64 lanes of a wave: an inclusive scan by DPP (row shifts, then row broadcasts),
-// whose last lane holds the whole wave's minimum.
+// Minimum over the 64 lanes of a wave: an inclusive scan by DPP (row shifts of the running minimum by
+// 1, 2, 4 and 8 lanes, then row broadcasts), whose last lane holds the whole wave's minimum.
 __device__ __forceinline__ uint32_t wave_min(uint32_t v) {
   v = dpp_min<0x111, 0xf, 0xf>(v);
   v = dpp_min<0x112, 0xf, 0xf>(v);
-  v = dpp_min<0x113, 0xf, 0xf>(v);
   v = dpp_min<0x114, 0xf, 0xe>(v);
   v = dpp_min<0x118, 0xf, 0xc>(v);
   v = dpp_min<0x142, 0xa, 0xf>(v);
@@ -150,10 +150,11 @@ __device__ __forceinline__ void search_one(const FsArgs &a, int rows_lds, int mb
     }
   }
   // the macroblock's samples, uniform across the workgroup
+  // (scalar loads: the picture is read-only while the kernel runs)
   uint32_t src[64];
-  const uint8_t *sp = a.src_y + (size_t)y0 * pw + x0;
+  const CU32 *sp = (const CU32 *)(a.src_y + (size_t)y0 * pw + x0);
 #pragma unroll
-  for (int i = 0; i < 64; i++) src[i] = __builtin_amdgcn_readfirstlane(ld_u32(sp + (size_t)(i >> 2) * pw + (i & 3) * 4));
+  for (int i = 0; i < 64; i++) src[i] = sp[(i >> 2) * (pw >> 2) + (i & 3)];
   __syncthreads();
 
   uint32_t run[IA_NPART];
@@ -238,7 +239,6 @@ __device__ __forceinline__ void search_one(const FsArgs &a, int rows_lds, int mb
     for (int p = 0; p < IA_NPART; p++) m[p] = run[p];
     dpp_min_all<0x111, 0xf, 0xf>(m);
     dpp_min_all<0x112, 0xf, 0xf>(m);
-    dpp_min_all<0x113, 0xf, 0xf>(m);
     dpp_min_all<0x114, 0xf, 0xe>(m);
     dpp_min_all<0x118, 0xf, 0xc>(m);
     dpp_min_all<0x142, 0xa, 0xf>(m);
