@@ -28,6 +28,14 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
+// LDS-qualified types: the CNN layer is a function of its own, and generic pointers there would make
+// its loads and stores FLAT instructions
+typedef __attribute__((address_space(3))) uint16_t LdsU16;
+typedef __attribute__((address_space(3))) float LdsF32;
+typedef __attribute__((address_space(3))) bf16x8 LdsBf16x8;
+typedef __attribute__((address_space(3))) f32x4 LdsF32x4;
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) u32x2 LdsU32x2;
 
 constexpr int WAVES = 8;
 constexpr int PG = CNN_GRID + 2;   // padded grid side (one cell of zeros around)
@@ -129,9 +137,9 @@ __device__ __forceinline__ int tapoff(int t) { return (t / 3 - 1) * PG + (t % 3 
 
 // One 3x3 convolution layer on a wave's 9x9 grid. IN0: the first layer (8-channel input); NT: 16-wide
 // output tiles; RELU: activation. Reads `in` (stride `ist`), writes bf16 to `outb` or f32 to `outf`.
-template <bool IN0, int NT, bool RELU>
-__device__ void conv_layer(int lane, const uint16_t *in, int ist, const uint16_t *w, const float *bias, uint16_t *outb,
-                           float *outf) {
+template <bool IN0, int NT, bool RELU, bool OUTF>
+__device__ void conv_layer(int lane, const LdsU16 *in, int ist, const LdsU16 *w, const LdsF32 *bias, LdsU16 *outb,
+                           LdsF32 *outf) {
   const int m = lane & 15, g = lane >> 4;
   constexpr int KSTEPS = IN0 ? 3 : 9;
   // the lane's B fragments of every K step and output tile, held in registers for the layer
@@ -142,9 +150,9 @@ __device__ void conv_layer(int lane, const uint16_t *in, int ist, const uint16_t
     for (int nt = 0; nt < NT; nt++) {
       if (IN0) {
         const int t = ks * 4 + g;  // this lane's tap; taps 9..11 are zero
-        bw[ks][nt] = t < 9 ? *reinterpret_cast<const bf16x8 *>(w + ((size_t)t * CNN_CH + nt * 16 + m) * 8) : bf16x8{};
+        bw[ks][nt] = t < 9 ? *(const LdsBf16x8 *)(w + (t * CNN_CH + nt * 16 + m) * 8) : bf16x8{};
       } else {
-        bw[ks][nt] = *reinterpret_cast<const bf16x8 *>(w + ((size_t)ks * (NT * 16) + nt * 16 + m) * CNN_CH + g * 8);
+        bw[ks][nt] = *(const LdsBf16x8 *)(w + (ks * (NT * 16) + nt * 16 + m) * CNN_CH + g * 8);
       }
     }
   // the whole layer's output stays in registers until every lane has read the input, so the output
@@ -153,7 +161,7 @@ __device__ void conv_layer(int lane, const uint16_t *in, int ist, const uint16_t
   // the accumulators start at the bias (this lane's 4 output channels of each tile)
   f32x4 bv[NT];
 #pragma unroll
-  for (int nt = 0; nt < NT; nt++) bv[nt] = *reinterpret_cast<const f32x4 *>(bias + nt * 16 + 4 * g);
+  for (int nt = 0; nt < NT; nt++) bv[nt] = *(const LdsF32x4 *)(bias + nt * 16 + 4 * g);
   f32x4 acc[6][NT];
 #pragma unroll
   for (int mt = 0; mt < 6; mt++) {
@@ -166,9 +174,9 @@ __device__ void conv_layer(int lane, const uint16_t *in, int ist, const uint16_t
       bf16x8 x;
       if (IN0) {
         const int t = ks * 4 + g;
-        x = t < 9 ? *reinterpret_cast<const bf16x8 *>(in + (size_t)(pp + tapoff(t)) * ist) : bf16x8{};
+        x = t < 9 ? *(const LdsBf16x8 *)(in + (pp + tapoff(t)) * ist) : bf16x8{};
       } else {
-        x = *reinterpret_cast<const bf16x8 *>(in + (size_t)(pp + tapoff(ks)) * ist + g * 8);
+        x = *(const LdsBf16x8 *)(in + (pp + tapoff(ks)) * ist + g * 8);
       }
       // one activation fragment feeds every output tile
 #pragma unroll
@@ -190,15 +198,15 @@ __device__ void conv_layer(int lane, const uint16_t *in, int ist, const uint16_t
 #pragma unroll
           for (int i = 0; i < 4; i++) v[i] = fmaxf(v[i], 0.f);
         }
-        if (outb) {
+        if constexpr (!OUTF) {
           const bf16x2 lo = __builtin_convertvector(f32x2{v[0], v[1]}, bf16x2);
           const bf16x2 hi = __builtin_convertvector(f32x2{v[2], v[3]}, bf16x2);
-          uint2 w2;
+          u32x2 w2;
           w2.x = __builtin_bit_cast(uint32_t, lo);
           w2.y = __builtin_bit_cast(uint32_t, hi);
-          *reinterpret_cast<uint2 *>(outb + (size_t)padpos(q) * AST + c0) = w2;
+          *(LdsU32x2 *)(outb + padpos(q) * AST + c0) = w2;
         }
-        if (outf && c0 < CNN_OUT) *reinterpret_cast<f32x4 *>(outf + q * CNN_OUT + c0) = v;
+        if (OUTF && c0 < CNN_OUT) *(LdsF32x4 *)(outf + q * CNN_OUT + c0) = v;
       }
     }
   }
@@ -391,13 +399,17 @@ __device__ void propose_mb(PropLds &L, WaveLds &V, const PropArgs &a, int mb, in
   // the CNN (the windows' space now holds the activations, borders zero)
   for (int i = lane; i < PCELLS * AST / 8; i += 64) reinterpret_cast<uint4 *>(&V.act[0][0])[i] = make_uint4(0, 0, 0, 0);
   wave_sync();
-  conv_layer<true, 2, true>(lane, &V.x0[0][0], 8, &L.w0[0][0][0], L.b[0], &V.act[0][0], nullptr);
+  conv_layer<true, 2, true, false>(lane, (const LdsU16 *)&V.x0[0][0], 8, (const LdsU16 *)&L.w0[0][0][0],
+                                   (const LdsF32 *)L.b[0], (LdsU16 *)&V.act[0][0], nullptr);
   wave_sync();
-  conv_layer<false, 2, true>(lane, &V.act[0][0], AST, &L.w1[0][0][0], L.b[1], &V.act[0][0], nullptr);
+  conv_layer<false, 2, true, false>(lane, (const LdsU16 *)&V.act[0][0], AST, (const LdsU16 *)&L.w1[0][0][0],
+                                    (const LdsF32 *)L.b[1], (LdsU16 *)&V.act[0][0], nullptr);
   wave_sync();
-  conv_layer<false, 2, true>(lane, &V.act[0][0], AST, &L.w2[0][0][0], L.b[2], &V.act[0][0], nullptr);
+  conv_layer<false, 2, true, false>(lane, (const LdsU16 *)&V.act[0][0], AST, (const LdsU16 *)&L.w2[0][0][0],
+                                    (const LdsF32 *)L.b[2], (LdsU16 *)&V.act[0][0], nullptr);
   wave_sync();
-  conv_layer<false, 1, false>(lane, &V.act[0][0], AST, &L.w3[0][0][0], L.b[3], nullptr, &V.out[0][0]);
+  conv_layer<false, 1, false, true>(lane, (const LdsU16 *)&V.act[0][0], AST, (const LdsU16 *)&L.w3[0][0][0],
+                                    (const LdsF32 *)L.b[3], nullptr, (LdsF32 *)&V.out[0][0]);
   wave_sync();
   if (a.cnn_out)
     for (int i = lane; i < CNN_CELLS * CNN_OUT; i += 64) a.cnn_out[(size_t)mb * CNN_CELLS * CNN_OUT + i] = (&V.out[0][0])[i];
