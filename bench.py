@@ -279,7 +279,7 @@ def bench_batched(args):
     import torch
 
     from ironav_amd import abi, autotune, clipgen
-    from ironav_amd.h264a import Analyser, Batch, Picture
+    from ironav_amd.h264a import Analyser, Batch, Picture, batch_workers
 
     sizes = [(640, 360), (854, 480), (1280, 720), (1920, 1080), (3840, 2160)]
     rng = np.random.default_rng(64)
@@ -294,7 +294,7 @@ def bench_batched(args):
     torch.cuda.synchronize()
     gen_s = time.perf_counter() - t0
     # slices: tasks of about equal length, one per resident analysis workgroup
-    workers = torch.cuda.get_device_properties(0).multi_processor_count * 4
+    workers = batch_workers() or torch.cuda.get_device_properties(0).multi_processor_count * 4
     target = max(1, -(-sum(nmbs) // workers))
     slices = [min(max(1, round(n / target)), (h + 15) // 16) for n, (w, h) in zip(nmbs, order)]
     nrefs = 3
@@ -384,7 +384,7 @@ def bench_learned_batched(args):
     import torch
 
     from ironav_amd import abi, autotune, clipgen, proposal
-    from ironav_amd.h264a import Analyser, Batch, library
+    from ironav_amd.h264a import Analyser, Batch, batch_workers, library
 
     nst = args.streams if args.streams != 64 else 8
     nfr = args.frames if args.frames != 30 else 120
@@ -397,7 +397,7 @@ def bench_learned_batched(args):
     torch.cuda.synchronize()
     gen_s = time.perf_counter() - t0
     nmb = ((w + 15) // 16) * ((h + 15) // 16)
-    workers = torch.cuda.get_device_properties(0).multi_processor_count * 4
+    workers = batch_workers() or torch.cuda.get_device_properties(0).multi_processor_count * 4
     slices = min((h + 15) // 16, max(1, workers // nst))
     params = proposal.load_weights()
     table = autotune.load()

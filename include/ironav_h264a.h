@@ -258,6 +258,10 @@ int ia_batch_sync(ia_batch *b);
 /* The number of (stream, macroblock, reference) triples the last step sent from the learned proposal
  * to the exhaustive search. Synchronous. */
 int ia_batch_fallback_count(ia_batch *b, int *count);
+/* The analysis workgroups resident at once on the current device (one wave each): a batch's
+ * (stream, slice) tasks run that many at a time, so slices of about equal length, about this many
+ * of them over all streams, keep every worker busy. 0 if the device cannot be queried. */
+int ia_batch_workers(void);
 
 /* Tile shapes of the full-search kernel, for the autotuner (no counterpart in the reference, whose
  * search is a scalar loop): the number of compiled shapes, each one's name, and the shape a context

@@ -104,6 +104,7 @@ PROTOTYPES = {
     "ia_batch_step": (C.c_int, [vp, C.POINTER(Frame)]),
     "ia_batch_sync": (C.c_int, [vp]),
     "ia_batch_fallback_count": (C.c_int, [vp, C.POINTER(C.c_int)]),
+    "ia_batch_workers": (C.c_int, []),
 }
 
 

@@ -396,6 +396,8 @@ int ia_batch_fallback_count(ia_batch *b, int *count) {
   return IA_OK;
 }
 
+int ia_batch_workers(void) { return ia::analyse_batch_workers(); }
+
 int ia_batch_sync(ia_batch *b) {
   if (!b) return IA_EINVAL;
   IA_BHIP(b, hipStreamSynchronize(b->stream));

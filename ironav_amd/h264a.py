@@ -24,6 +24,11 @@ _LIB = None
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libironav_h264a.so")
 
 
+def batch_workers():
+    """Analysis workgroups resident at once on the current device (the size of a batch's task pool)."""
+    return int(library().ia_batch_workers())
+
+
 def library():
     """The loaded gfx950 library; raises if it has not been built."""
     global _LIB
