@@ -132,6 +132,13 @@ struct PropLds {
   WaveLds wv[WAVES];
 };
 
+// The 16-byte chunk of weight row r (64 bytes: 32 input channels) that holds input channels
+// 8g..8g+7. Sixteen lanes read chunk g of sixteen consecutive rows at once; unswizzled, rows 64 bytes
+// apart put those reads on the same LDS banks (8-way conflicts with 128 bytes of banks). Rotating the
+// chunk by (r >> 2) + 2 ((r >> 1) & 1) spreads any 8 consecutive rows over 128 bytes and any 16 over
+// 256 bytes.
+__device__ __forceinline__ int wswz(int r, int g) { return (g + (r >> 2) + 2 * ((r >> 1) & 1)) & 3; }
+
 __device__ __forceinline__ int padpos(int p) { return (p / CNN_GRID + 1) * PG + (p % CNN_GRID) + 1; }
 __device__ __forceinline__ int tapoff(int t) { return (t / 3 - 1) * PG + (t % 3 - 1); }
 
@@ -152,7 +159,8 @@ __device__ void conv_layer(int lane, const LdsU16 *in, int ist, const LdsU16 *w,
         const int t = ks * 4 + g;  // this lane's tap; taps 9..11 are zero
         bw[ks][nt] = t < 9 ? *(const LdsBf16x8 *)(w + (t * CNN_CH + nt * 16 + m) * 8) : bf16x8{};
       } else {
-        bw[ks][nt] = *(const LdsBf16x8 *)(w + (ks * (NT * 16) + nt * 16 + m) * CNN_CH + g * 8);
+        const int r = ks * (NT * 16) + nt * 16 + m;  // weight row: tap, output channel
+        bw[ks][nt] = *(const LdsBf16x8 *)(w + r * CNN_CH + wswz(r, g) * 8);
       }
     }
   // the whole layer's output stays in registers until every lane has read the input, so the output
@@ -272,13 +280,16 @@ __device__ __forceinline__ void load_weights(PropLds &L, const CnnWeights *w, in
       if (t < 9) v = *reinterpret_cast<const uint4 *>(&W.w[0][t][co][0]);
       *reinterpret_cast<uint4 *>(&L.w0[t][co][0]) = v;
     }
+    // rows of 32 input channels: chunk g (8 channels) of row r goes to chunk wswz(r, g)
     for (int i = tid; i < 9 * CNN_CH * CNN_CH / 8; i += 64 * WAVES) {
-      reinterpret_cast<uint4 *>(&L.w1[0][0][0])[i] = reinterpret_cast<const uint4 *>(&W.w[1][0][0][0])[i];
-      reinterpret_cast<uint4 *>(&L.w2[0][0][0])[i] = reinterpret_cast<const uint4 *>(&W.w[2][0][0][0])[i];
+      const int r = i >> 2, d = r * 4 + wswz(r, i & 3);
+      reinterpret_cast<uint4 *>(&L.w1[0][0][0])[d] = reinterpret_cast<const uint4 *>(&W.w[1][0][0][0])[i];
+      reinterpret_cast<uint4 *>(&L.w2[0][0][0])[d] = reinterpret_cast<const uint4 *>(&W.w[2][0][0][0])[i];
     }
     for (int i = tid; i < 9 * 16 * CNN_CH / 8; i += 64 * WAVES) {
-      const int t = i / (16 * CNN_CH / 8), r = i % (16 * CNN_CH / 8);
-      reinterpret_cast<uint4 *>(&L.w3[t][0][0])[r] = reinterpret_cast<const uint4 *>(&W.w[3][t][0][0])[r];
+      const int t = i / (16 * CNN_CH / 8), c = i % (16 * CNN_CH / 8), r = t * 16 + (c >> 2);
+      reinterpret_cast<uint4 *>(&L.w3[0][0][0])[r * 4 + wswz(r, c & 3)] =
+          reinterpret_cast<const uint4 *>(&W.w[3][t][0][0])[c];
     }
     for (int i = tid; i < CNN_LAYERS * CNN_CH; i += 64 * WAVES) (&L.b[0][0])[i] = (&W.b[0][0])[i];
   }
