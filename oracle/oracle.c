@@ -9,6 +9,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -1880,9 +1881,9 @@ static void keep(coder_t *c, keep_t *k, double j, const mbcode_t *code, int mbx,
 /* enc/frame.rs:1462-1513 */
 static void decide_rd(coder_t *c, int mbx, int mby, int i16, const inter_t *inter, mbcode_t *out) {
   const double lam2 = 1.6 * pow(2.0, ((double)qpp_(c) - 12.0) / 3.0);
-  static keep_t k;
+  static _Thread_local keep_t k;
   k.have = 0;
-  static mbcode_t code;
+  static _Thread_local mbcode_t code;
   if (inter) {
     int16_t sm[2];
     skip_mv(c, mbx, mby, sm);
@@ -1971,7 +1972,7 @@ static void code_mb(coder_t *c, int mbx, int mby, mbcode_t *out) {
   }
   /* SATD decisions (CAVLC), enc/frame.rs:760-789 */
   int try_i4 = !have_inter || best_intra < inter.cost;
-  static mbcode_t i4code;
+  static _Thread_local mbcode_t i4code;
   int have_i4 = 0;
   if (try_i4) {
     int cost = code_i4(c, mbx, mby, &i4code) + lam * 6;
@@ -2156,16 +2157,80 @@ static int chroma_qp(int qp, int offset, int bd) {
   return qpi < 30 ? qpi : OR_CHROMA_QP[qpi - 30];
 }
 
-int or_analyse_picture(const ia_config *cfg, const uint8_t *y, const uint8_t *cb, const uint8_t *cr, int qp,
-                       int kind, const uint8_t *const *ref_y, const uint8_t *const *ref_cb,
-                       const uint8_t *const *ref_cr, const ia_mbinfo *const *ref_mbs, int nref,
-                       const ia_fsresult *fs, ia_mbcode *codes, ia_mbstats *stats, uint8_t *recon_y,
-                       uint8_t *recon_cb, uint8_t *recon_cr, ia_mbinfo *mbs) {
+/* One slice of a picture: the per-slice coder of enc/frame.rs:381-487 over its rows. */
+typedef struct {
+  const ia_config *cfg;
+  uint16_t **src, **pic;
+  refpic_t *refs;
+  int np, qp, mb_w, mb_h, nslices, threads, first;
+  mbstate_t *state;
+  const ia_fsresult *fs;
+  ia_mbcode *codes;
+  ia_mbstats *stats;
+  ia_mbinfo *mbs;
+} slice_job_t;
+
+static void code_slice(const slice_job_t *j, int s, mbcode_t *code) {
+  coder_t c;
+  memset(&c, 0, sizeof c);
+  c.src[0] = j->src[0]; c.src[1] = j->src[1]; c.src[2] = j->src[2];
+  c.bd = 8;
+  c.mb_w = j->mb_w;
+  c.mb_h = j->mb_h;
+  c.qp = j->qp;
+  c.qpc = chroma_qp(j->qp, 0, 8);
+  c.lambda = or_lambda(j->qp);
+  c.row0 = s * j->mb_h / j->nslices;
+  c.row1 = (s + 1) * j->mb_h / j->nslices;
+  c.slice_id = s + 1;
+  c.p_slice = j->np > 0;
+  c.nref = j->np;
+  c.plist = j->refs;
+  c.sub8 = j->cfg->sub8x8 && j->qp <= 20;
+  c.use_cabac = j->cfg->cabac;
+  c.transform8 = j->cfg->cabac && j->cfg->transform8;
+  c.pic[0] = j->pic[0]; c.pic[1] = j->pic[1]; c.pic[2] = j->pic[2];
+  c.mbs = j->mbs;
+  c.state = j->state + (size_t)c.row0 * j->mb_w;
+  c.fs = j->fs;
+  c.search_mode = j->cfg->search_mode;
+  cabac_init(c.cabac, !c.p_slice, j->qp);
+  for (int mby = c.row0; mby < c.row1; mby++)
+    for (int mbx = 0; mbx < j->mb_w; mbx++) {
+      const int idx = mby * j->mb_w + mbx;
+      ia_mbstats dummy;
+      c.st = j->stats ? &j->stats[idx] : &dummy;
+      code_mb(&c, mbx, mby, code);
+      if (c.use_cabac) {
+        bins_t live = {c.cabac, 0};
+        mb_bins(&c, &live, code, mbx, mby);
+      }
+      finish_mb(&c, code, mbx, mby);
+      if (j->codes) export_code(code, &j->codes[idx]);
+    }
+}
+
+/* Worker t codes slices t, t + threads, ... (slices share nothing but the read-only references
+ * and source; each writes its own rows of the picture, records and state). */
+static void *slice_worker(void *arg) {
+  const slice_job_t *j = arg;
+  mbcode_t *code = malloc(sizeof *code);
+  for (int s = j->first; s < j->nslices; s += j->threads) code_slice(j, s, code);
+  free(code);
+  return NULL;
+}
+
+int or_analyse_picture_mt(const ia_config *cfg, const uint8_t *y, const uint8_t *cb, const uint8_t *cr, int qp,
+                          int kind, const uint8_t *const *ref_y, const uint8_t *const *ref_cb,
+                          const uint8_t *const *ref_cr, const ia_mbinfo *const *ref_mbs, int nref,
+                          const ia_fsresult *fs, ia_mbcode *codes, ia_mbstats *stats, uint8_t *recon_y,
+                          uint8_t *recon_cb, uint8_t *recon_cr, ia_mbinfo *mbs, int threads) {
   const int mb_w = (cfg->width + 15) / 16, mb_h = (cfg->height + 15) / 16, pw = mb_w * 16, ph = mb_h * 16;
   const size_t ylen = (size_t)pw * ph, clen = ylen / 4;
   if (cfg->bit_depth != 8 || cfg->fast || qp < 0 || qp > 51 || (kind == IA_PIC_P && (nref < 1 || nref > 16))) return -1;
   entropy_init();
   const int nslices = clampi(cfg->slices, 1, mb_h);
+  threads = clampi(threads, 1, nslices);
   uint16_t *src[3] = {widen(y, ylen), widen(cb, clen), widen(cr, clen)};
   uint16_t *pic[3] = {calloc(ylen, 2), calloc(clen, 2), calloc(clen, 2)};
   refpic_t refs[16];
@@ -2181,46 +2246,22 @@ int or_analyse_picture(const ia_config *cfg, const uint8_t *y, const uint8_t *cb
     refs[r].mbs = ref_mbs ? ref_mbs[r] : NULL;
   }
   mbstate_t *state = calloc((size_t)mb_w * mb_h, sizeof *state);
-  static mbcode_t code;
-  for (int s = 0; s < nslices; s++) {
-    coder_t c;
-    memset(&c, 0, sizeof c);
-    c.src[0] = src[0]; c.src[1] = src[1]; c.src[2] = src[2];
-    c.bd = 8;
-    c.mb_w = mb_w;
-    c.mb_h = mb_h;
-    c.qp = qp;
-    c.qpc = chroma_qp(qp, 0, 8);
-    c.lambda = or_lambda(qp);
-    c.row0 = s * mb_h / nslices;
-    c.row1 = (s + 1) * mb_h / nslices;
-    c.slice_id = s + 1;
-    c.p_slice = np > 0;
-    c.nref = np;
-    c.plist = refs;
-    c.sub8 = cfg->sub8x8 && qp <= 20;
-    c.use_cabac = cfg->cabac;
-    c.transform8 = cfg->cabac && cfg->transform8;
-    c.pic[0] = pic[0]; c.pic[1] = pic[1]; c.pic[2] = pic[2];
-    c.mbs = mbs;
-    c.state = state + (size_t)c.row0 * mb_w;
-    c.fs = fs;
-    c.search_mode = cfg->search_mode;
-    cabac_init(c.cabac, !c.p_slice, qp);
-    for (int mby = c.row0; mby < c.row1; mby++)
-      for (int mbx = 0; mbx < mb_w; mbx++) {
-        const int idx = mby * mb_w + mbx;
-        ia_mbstats dummy;
-        c.st = stats ? &stats[idx] : &dummy;
-        code_mb(&c, mbx, mby, &code);
-        if (c.use_cabac) {
-          bins_t live = {c.cabac, 0};
-          mb_bins(&c, &live, &code, mbx, mby);
-        }
-        finish_mb(&c, &code, mbx, mby);
-        if (codes) export_code(&code, &codes[idx]);
-      }
+  slice_job_t jobs[64];
+  pthread_t tids[64];
+  if (threads > 64) threads = 64;
+  int started = 0, rc = 0;
+  for (int t = 0; t < threads; t++) {
+    jobs[t] = (slice_job_t){cfg, src, pic, refs, np, qp, mb_w, mb_h, nslices, threads, t, state, fs, codes, stats, mbs};
+    if (threads == 1) {
+      slice_worker(&jobs[t]);
+    } else if (pthread_create(&tids[t], NULL, slice_worker, &jobs[t]) == 0) {
+      started++;
+    } else {
+      rc = -2;  /* the slices of this worker are left uncoded */
+      break;
+    }
   }
+  for (int t = 0; t < started; t++) pthread_join(tids[t], NULL);
   for (size_t i = 0; i < ylen; i++) recon_y[i] = (uint8_t)pic[0][i];
   for (size_t i = 0; i < clen; i++) { recon_cb[i] = (uint8_t)pic[1][i]; recon_cr[i] = (uint8_t)pic[2][i]; }
   for (int r = 0; r < np; r++) {
@@ -2230,7 +2271,16 @@ int or_analyse_picture(const ia_config *cfg, const uint8_t *y, const uint8_t *cb
   }
   for (int k = 0; k < 3; k++) { free(src[k]); free(pic[k]); }
   free(state);
-  return 0;
+  return rc;
+}
+
+int or_analyse_picture(const ia_config *cfg, const uint8_t *y, const uint8_t *cb, const uint8_t *cr, int qp,
+                       int kind, const uint8_t *const *ref_y, const uint8_t *const *ref_cb,
+                       const uint8_t *const *ref_cr, const ia_mbinfo *const *ref_mbs, int nref,
+                       const ia_fsresult *fs, ia_mbcode *codes, ia_mbstats *stats, uint8_t *recon_y,
+                       uint8_t *recon_cb, uint8_t *recon_cr, ia_mbinfo *mbs) {
+  return or_analyse_picture_mt(cfg, y, cb, cr, qp, kind, ref_y, ref_cb, ref_cr, ref_mbs, nref, fs, codes, stats,
+                               recon_y, recon_cb, recon_cr, mbs, 1);
 }
 
 /* Sizes of the ABI structs as this compiler lays them out, for the binding tests. */

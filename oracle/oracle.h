@@ -87,6 +87,15 @@ int or_analyse_picture(const ia_config *cfg, const uint8_t *y, const uint8_t *cb
                        const ia_fsresult *fs, ia_mbcode *codes, ia_mbstats *stats, uint8_t *recon_y,
                        uint8_t *recon_cb, uint8_t *recon_cr, ia_mbinfo *mbs);
 
+/* The same with the slices coded on `threads` host threads (slice s on thread s mod threads), as the
+ * reference's encoder codes each slice on a thread of its own (enc/mod.rs:623-627). Returns -2 if a
+ * thread could not be started. */
+int or_analyse_picture_mt(const ia_config *cfg, const uint8_t *y, const uint8_t *cb, const uint8_t *cr, int qp,
+                          int kind, const uint8_t *const *ref_y, const uint8_t *const *ref_cb,
+                          const uint8_t *const *ref_cr, const ia_mbinfo *const *ref_mbs, int nref,
+                          const ia_fsresult *fs, ia_mbcode *codes, ia_mbstats *stats, uint8_t *recon_y,
+                          uint8_t *recon_cb, uint8_t *recon_cr, ia_mbinfo *mbs, int threads);
+
 #ifdef __cplusplus
 }
 #endif

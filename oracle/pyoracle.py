@@ -55,6 +55,8 @@ def lib():
             "or_abi_sizes": (None, [vp]),
             "or_analyse_picture": (i, [C.POINTER(abi.Config), vp, vp, vp, i, i, vp, vp, vp, vp, i, vp, vp, vp, vp, vp,
                                        vp, vp]),
+            "or_analyse_picture_mt": (i, [C.POINTER(abi.Config), vp, vp, vp, i, i, vp, vp, vp, vp, i, vp, vp, vp, vp,
+                                          vp, vp, vp, i]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -97,9 +99,9 @@ def fullsearch(src_y, ref_ys, mb_w, mb_h, rng):
     return out
 
 
-def analyse_picture(cfg, pic, qp, kind, refs=(), fs=None):
+def analyse_picture(cfg, pic, qp, kind, refs=(), fs=None, threads=1):
     """refs: sequence of (Picture, mbinfo array or None), list-0 order. Returns
-    (codes, stats, (ry, rcb, rcr), mbs)."""
+    (codes, stats, (ry, rcb, rcr), mbs). threads > 1 codes the slices on that many host threads."""
     mb_w, mb_h = (cfg.width + 15) // 16, (cfg.height + 15) // 16
     n = mb_w * mb_h
     codes = np.zeros(n, dtype=abi.MBCODE_DTYPE)
@@ -114,11 +116,11 @@ def analyse_picture(cfg, pic, qp, kind, refs=(), fs=None):
         return C.cast(a, C.c_void_p)
 
     ref_mbs = [None if m is None else np.ascontiguousarray(m, dtype=abi.MBINFO_DTYPE) for _, m in refs]
-    rc = lib().or_analyse_picture(C.byref(cfg), P(pic.y), P(pic.cb), P(pic.cr), qp, kind,
-                                  arr([r.y for r, _ in refs]), arr([r.cb for r, _ in refs]),
-                                  arr([r.cr for r, _ in refs]), arr(ref_mbs) if refs else None, len(refs),
-                                  P(fs) if fs is not None else None, P(codes), P(stats), P(ry), P(rcb), P(rcr),
-                                  P(mbs))
+    rc = lib().or_analyse_picture_mt(C.byref(cfg), P(pic.y), P(pic.cb), P(pic.cr), qp, kind,
+                                     arr([r.y for r, _ in refs]), arr([r.cb for r, _ in refs]),
+                                     arr([r.cr for r, _ in refs]), arr(ref_mbs) if refs else None, len(refs),
+                                     P(fs) if fs is not None else None, P(codes), P(stats), P(ry), P(rcb), P(rcr),
+                                     P(mbs), threads)
     if rc != 0:
         raise ValueError("oracle rejected the arguments")
     return codes, stats, (ry, rcb, rcr), mbs

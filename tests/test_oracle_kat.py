@@ -294,3 +294,25 @@ def test_abi_struct_layouts(oracle):
     assert list(sizes) == [C.sizeof(abi.Config), C.sizeof(abi.MbInfo), C.sizeof(abi.MbCode), C.sizeof(abi.MbStats),
                            C.sizeof(abi.FsResult), C.sizeof(abi.Frame)]
     assert abi.MBCODE_DTYPE.itemsize == C.sizeof(abi.MbCode)
+
+
+def test_threaded_slices_equal_one_thread(oracle):
+    """The picture driver with slices on host threads (the CPU baseline's form, one slice per thread as
+    enc/mod.rs:623-627 runs them) codes every slice exactly as the single-threaded driver."""
+    from ironav_amd import clipgen
+    from ironav_amd.h264a import Picture
+
+    w, h = 96, 64
+    pics = [Picture(*clipgen.pad_planes(*f)) for f in clipgen.make_clip(clipgen.ClipSpec(w, h, 3, 41, max_motion=6))]
+    cfg = abi.Config(w, h, 8, 1, 1, 0, 0, 2, 4, abi.SEARCH_DIAMOND, 16, 1.1)
+    digests = []
+    for threads in (1, 3):
+        refs, out = [], []
+        for t, pic in enumerate(pics):
+            kind = abi.PIC_IDR if t == 0 else abi.PIC_P
+            codes, stats, rec, mbs = oracle.analyse_picture(cfg, pic, 25 if t == 0 else 28, kind, refs[:2],
+                                                            threads=threads)
+            out.append((codes.tobytes(), stats.tobytes(), b"".join(r.tobytes() for r in rec), mbs.tobytes()))
+            refs.insert(0, (Picture(*rec), mbs))
+        digests.append(out)
+    assert digests[0] == digests[1]
