@@ -107,6 +107,7 @@ struct Lds {
   uint8_t cc[2][9][12];
   uint8_t cabac[1024];  // live context states of the slice
   uint8_t tmp[1024];    // a copy counted on for a rate estimate
+  uint32_t cab[256];    // per (state, LPS?): bit cost in 1/256 bits << 8 | next state (enc/cabac.rs:32-40,162-173)
   MbState nb[4];        // left, top, top-right, top-left neighbours (valid when nb_ok)
   int nb_ok[4];
   ia_mbcode code;       // the candidate being coded
@@ -1456,15 +1457,12 @@ struct Coder {
   struct Bins {
     uint8_t *state;
     uint32_t bits;
-    const uint32_t *ent;
+    const uint32_t *cab;  // Lds::cab: one LDS read gives a decision's cost and next state
     __device__ void decision(int ctx, int bin) {
       const uint8_t s = state[ctx];
-      bits += ent[(s >> 1) * 2 + (bin != (s & 1))];
-      const int p = s >> 1, mps = s & 1;
-      uint8_t ns;
-      if (bin != mps) ns = (uint8_t)((IA_TRANS_LPS[p] << 1) | (p == 0 ? !mps : mps));
-      else ns = (uint8_t)(((p < 62 ? p + 1 : p) << 1) | mps);
-      state[ctx] = ns;
+      const uint32_t e = cab[s * 2 + (bin != (s & 1))];
+      bits += e >> 8;
+      state[ctx] = (uint8_t)(e & 0xffu);
     }
     __device__ void bypass(int) { bits += 256; }
     __device__ void terminate(int bin) { bits += bin ? 7 * 256 : 1; }
@@ -1719,7 +1717,7 @@ struct Coder {
     sync();
     uint32_t bits = 0;
     if (lane() == 0) {
-      Bins b{L.tmp, 0, P.entropy};
+      Bins b{L.tmp, 0, L.cab};
       mb_bins(b, L.code);
       L.ival[0] = (int)b.bits;
     }
@@ -1988,7 +1986,7 @@ struct Coder {
       P.stats[idx] = st;
       // the chosen macroblock's bins adapt the live contexts as the writer would
       if (P.use_cabac) {
-        Bins b{L.cabac, 0, P.entropy};
+        Bins b{L.cabac, 0, L.cab};
         mb_bins(b, L.code);
       }
     }
@@ -2011,6 +2009,13 @@ __device__ void analyse_slice(const AnParams &p, int s, Lds &lds) {
       const int n = p.p_slice ? IA_CABAC_INIT_P[i][1] : IA_CABAC_INIT_I[i][1];
       const int pre = clampi(((m * qp) >> 4) + n, 1, 126);
       lds.cabac[i] = pre <= 63 ? (uint8_t)((63 - pre) << 1) : (uint8_t)(((pre - 64) << 1) | 1);
+    }
+    // a decision's cost and next state per (state, LPS?): state s = pStateIdx << 1 | valMPS
+    for (int i = threadIdx.x; i < 256; i += 64) {
+      const int st = i >> 1, lps = i & 1, ps = st >> 1, mps = st & 1;
+      const uint32_t ns = lps ? (uint32_t)((IA_TRANS_LPS[ps] << 1) | (ps == 0 ? !mps : mps))
+                              : (uint32_t)(((ps < 62 ? ps + 1 : ps) << 1) | mps);
+      lds.cab[i] = (p.entropy[ps * 2 + lps] << 8) | ns;
     }
   }
   __syncthreads();
