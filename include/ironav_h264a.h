@@ -115,6 +115,9 @@ typedef struct ia_mbstats {
   int32_t inter_cost;      /* best_inter's cost (enc/frame.rs:839); -1 when not searched */
   int32_t inter_part;      /* best_inter's partitioning; -1 when not searched */
   double rd_cost;          /* J of the kept candidate (enc/frame.rs:1453-1457); 0 when decided before RD */
+  int32_t bits;            /* CABAC: the kept macroblock's rate in 1/256 bits, counted on the live contexts
+                              as the writer codes it (enc/frame.rs:2020-2024, enc/cabac.rs:150-191); 0 with
+                              CAVLC */
 } ia_mbstats;
 
 /* One exhaustive-search winner: integer-sample vector in quarter samples, and its SAD. */

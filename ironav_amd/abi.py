@@ -51,7 +51,7 @@ class MbCode(C.Structure):
 class MbStats(C.Structure):
     _fields_ = [
         ("skip_sad", C.c_int32), ("i16_cost", C.c_int32), ("inter_cost", C.c_int32), ("inter_part", C.c_int32),
-        ("rd_cost", C.c_double),
+        ("rd_cost", C.c_double), ("bits", C.c_int32),
     ]
 
 

@@ -1,0 +1,118 @@
+"""Rate-distortion of the learned proposal against the exhaustive search: the Bjontegaard delta rate.
+
+Each search mode codes the same generator clip IPPP at four QPs, closed loop (each chain's
+references are its own reconstructions). The rate of a picture is the sum over its macroblocks of
+`ia_mbstats.bits`: the CABAC bins of each kept macroblock counted on the live contexts, as the
+reference's writer codes them (`CabacCost`'s model, enc/cabac.rs:150-191; the headers and the
+arithmetic coder's termination are not counted, and are the same for both modes). Distortion is the
+PSNR of the reconstruction (before deblocking, as both chains predict from it) over the picture's
+visible area, luma and chroma weighted 4:1:1.
+
+BD-rate (Bjontegaard): log rate interpolated over PSNR per mode (piecewise cubic Hermite, as JVET's
+common test conditions compute it; the single-cubic form of VCEG-M33 beside it), the mean difference
+over the PSNR range both curves cover, as a percentage: negative = the learned proposal needs fewer
+bits for the same quality.
+
+    python -m ironav_amd.bdrate                  # on the GPU: 1080p (+-32) and 2160p-type (+-64) clips
+"""
+import json
+import sys
+
+import numpy as np
+
+QPS = (22, 27, 32, 37)
+
+
+def bd_rate(rate_a, psnr_a, rate_b, psnr_b, method="pchip"):
+    """Bjontegaard delta rate of curve b against curve a, in percent: log rate interpolated over PSNR,
+    its mean difference over the PSNR range both curves cover. "pchip": piecewise cubic Hermite
+    interpolation (the form JVET's common test conditions use; stable when a curve bends sharply);
+    "cubic": one cubic polynomial per curve (VCEG-M33)."""
+    pa, pb = np.asarray(psnr_a, float), np.asarray(psnr_b, float)
+    la, lb = np.log(np.asarray(rate_a, float)), np.log(np.asarray(rate_b, float))
+    lo, hi = max(pa.min(), pb.min()), min(pa.max(), pb.max())
+    if method == "cubic":
+        ia, ib = np.polyint(np.polyfit(pa, la, 3)), np.polyint(np.polyfit(pb, lb, 3))
+        da = np.polyval(ia, hi) - np.polyval(ia, lo)
+        db = np.polyval(ib, hi) - np.polyval(ib, lo)
+    else:
+        from scipy.interpolate import PchipInterpolator
+
+        oa, ob = np.argsort(pa), np.argsort(pb)
+        da = PchipInterpolator(pa[oa], la[oa]).integrate(lo, hi)
+        db = PchipInterpolator(pb[ob], lb[ob]).integrate(lo, hi)
+    return float((np.exp((db - da) / (hi - lo)) - 1.0) * 100.0)
+
+
+def psnr(src, rec, w, h):
+    """PSNR over the visible area, luma and chroma weighted 4:1:1 (MSE averaged with those weights)."""
+    def mse(a, b, ww, hh):
+        d = a[:hh, :ww].astype(np.float64) - b[:hh, :ww].astype(np.float64)
+        return float(np.mean(d * d))
+    m = (4 * mse(src.y, rec.y, w, h) + mse(src.cb, rec.cb, w // 2, h // 2) + mse(src.cr, rec.cr, w // 2, h // 2)) / 6
+    return 10.0 * np.log10(255.0 * 255.0 / max(m, 1e-10))
+
+
+def code_chain(pics, w, h, qp, mode, search_range, params=None, nrefs=3, slices=None):
+    """Codes the pictures IPPP (IDR at qp - 3, as the reference's driver) with one search mode; returns
+    (bits, mean PSNR) over the P pictures."""
+    from . import abi
+    from .h264a import Analyser
+
+    slices = slices or (h + 15) // 16
+    bits, ps = 0.0, []
+    with Analyser(w, h, nrefs=nrefs, slices=slices, search_mode=mode, search_range=search_range,
+                  fallback_ratio=1.1) as an:
+        if params is not None:
+            an.load_proposal_weights(params)
+        refs = []
+        for t, pic in enumerate(pics):
+            n = min(len(refs), nrefs)
+            for s in range(n):
+                an.set_reference(s, refs[s][0], refs[s][1])
+            kind = abi.PIC_IDR if t == 0 else abi.PIC_P
+            res = an.analyse(pic, max(0, qp - 3) if t == 0 else qp, kind, list(range(n)))
+            if t > 0:
+                bits += float(res.stats["bits"].sum()) / 256.0
+                ps.append(psnr(pic, res.recon, w, h))
+            refs.insert(0, (res.recon, res.mbs))
+    return bits, float(np.mean(ps))
+
+
+def run(name, spec, search_range, log=print):
+    from . import abi, clipgen, proposal
+    from .h264a import Picture
+
+    pics = [Picture(*(p.cpu().numpy() for p in f)) for f in clipgen.make_clip_device(spec)]
+    params = proposal.load_weights()
+    curves = {}
+    for label, mode in (("exhaustive", abi.SEARCH_FULL), ("learned", abi.SEARCH_LEARNED)):
+        pts = []
+        for qp in QPS:
+            b, p = code_chain(pics, spec.width, spec.height, qp, mode, search_range,
+                              params if mode == abi.SEARCH_LEARNED else None)
+            pts.append((b, p))
+            log(f"[bdrate] {name} {label} qp {qp}: {b / max(1, len(pics) - 1):.0f} bits/picture, {p:.3f} dB")
+        curves[label] = pts
+    ra, pa = zip(*curves["exhaustive"])
+    rb, pb = zip(*curves["learned"])
+    return {"clip": name, "qps": list(QPS), "pictures": len(pics), "search_range": search_range,
+            "exhaustive": {"bits": [round(x, 1) for x in ra], "psnr": [round(x, 4) for x in pa]},
+            "learned": {"bits": [round(x, 1) for x in rb], "psnr": [round(x, 4) for x in pb]},
+            "bd_rate_percent": round(bd_rate(ra, pa, rb, pb), 3),
+            "bd_rate_percent_cubic": round(bd_rate(ra, pa, rb, pb, "cubic"), 3)}
+
+
+def main():
+    from . import clipgen
+
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    s1 = clipgen.workload_1080p()
+    s1.frames = 8
+    print(json.dumps(run("1920x1080 workload clip (objects +-12, noise 2), 8 pictures", s1, 32, log)), flush=True)
+    s2 = clipgen.ClipSpec(3840, 2160, 8, 8000, max_motion=64, noise_sigma=4.0)
+    print(json.dumps(run("3840x2160 (objects +-64, noise 4), 8 pictures", s2, 64, log)), flush=True)
+
+
+if __name__ == "__main__":
+    main()

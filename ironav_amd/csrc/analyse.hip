@@ -1983,12 +1983,15 @@ struct Coder {
       finish_mb(L.code, ms, info);
       P.state[idx] = ms;
       P.mbs[idx] = info;
-      P.stats[idx] = st;
-      // the chosen macroblock's bins adapt the live contexts as the writer would
+      // the chosen macroblock's bins adapt the live contexts as the writer would; their count is its rate
+      ia_mbstats out = st;
+      out.bits = 0;
       if (P.use_cabac) {
         Bins b{L.cabac, 0, L.cab};
         mb_bins(b, L.code);
+        out.bits = (int32_t)b.bits;
       }
+      P.stats[idx] = out;
     }
     __threadfence_block();
     sync();

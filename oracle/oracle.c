@@ -2201,9 +2201,11 @@ static void code_slice(const slice_job_t *j, int s, mbcode_t *code) {
       ia_mbstats dummy;
       c.st = j->stats ? &j->stats[idx] : &dummy;
       code_mb(&c, mbx, mby, code);
+      c.st->bits = 0;
       if (c.use_cabac) {
         bins_t live = {c.cabac, 0};
         mb_bins(&c, &live, code, mbx, mby);
+        c.st->bits = (int32_t)live.bits;
       }
       finish_mb(&c, code, mbx, mby);
       if (j->codes) export_code(code, &j->codes[idx]);

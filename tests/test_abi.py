@@ -74,3 +74,18 @@ def test_device_clip_generator_matches_host_without_noise():
         for fh, fd in zip(host, dev):
             for a, b in zip(fh, fd):
                 assert isinstance(b, torch.Tensor) and np.array_equal(a, b.numpy())
+
+
+def test_bd_rate_closed_forms():
+    """Bjontegaard delta rate: equal curves give 0; a curve needing 10% more bits at every PSNR gives
+    +10% (the cubic fit of log rate is exact for a constant offset); 10% fewer gives -10%."""
+    from ironav_amd.bdrate import bd_rate
+
+    psnr = [33.0, 35.5, 38.2, 41.0]
+    rate = [1.0e5, 2.1e5, 4.6e5, 9.9e5]
+    for method in ("pchip", "cubic"):
+        assert abs(bd_rate(rate, psnr, rate, psnr, method)) < 1e-9
+        assert abs(bd_rate(rate, psnr, [r * 1.1 for r in rate], psnr, method) - 10.0) < 1e-6
+        assert abs(bd_rate(rate, psnr, [r * 0.9 for r in rate], psnr, method) + 10.0) < 1e-6
+        # a curve shifted by +0.5 dB at the same rates needs fewer bits at equal quality
+        assert bd_rate(rate, psnr, rate, [p + 0.5 for p in psnr], method) < 0
