@@ -80,16 +80,18 @@ def labels(fs, centres, nref):
     return cell, cost
 
 
-def make_dataset(n_clips=200, width=320, height=192, frames=6, seed=20000, search_range=32, log=None):
+def make_dataset(n_clips=200, width=320, height=192, frames=6, seed=20000, search_range=32, motion=(4, 25),
+                 pan=(0, 5), noise=(1.0, 4.0), log=None):
     """Features, target cells and costs from generator clips: each clip coded IPPP (analysis on the
-    GPU, so reference 0 carries the encoder's own vectors), each P picture searched exhaustively."""
+    GPU, so reference 0 carries the encoder's own vectors), each P picture searched exhaustively.
+    Per clip: object motion bound, pan and noise sigma drawn from [motion), [pan) and [noise)."""
     feats, cells, costs = [], [], []
     with Analyser(width, height, nrefs=3, slices=max(1, (height + 15) // 16), search_mode=abi.SEARCH_FULL,
                   search_range=search_range) as an:
         for k in range(n_clips):
             rng = np.random.default_rng(seed + k)
-            spec = clipgen.ClipSpec(width, height, frames, seed + k, max_motion=int(rng.integers(4, 25)),
-                                    pan=int(rng.integers(0, 5)), noise_sigma=float(rng.uniform(1.0, 4.0)))
+            spec = clipgen.ClipSpec(width, height, frames, seed + k, max_motion=int(rng.integers(*motion)),
+                                    pan=int(rng.integers(*pan)), noise_sigma=float(rng.uniform(*noise)))
             pics = [Picture(*clipgen.pad_planes(*f)) for f in clipgen.make_clip(spec)]
             refs = []
             for t, pic in enumerate(pics):
@@ -198,13 +200,16 @@ def train(feat, cell, cost, *, epochs=8, batch=512, lr=2e-3, cost_w=0.05, seed=0
 
 
 def main():
-    """Builds the dataset from 200 generator clips, trains, and writes ironav_amd/weights/proposal.npz
-    (or the path in $IA_WEIGHTS_OUT)."""
+    """Builds the dataset from 300 generator clips (200 over the generator's range, 100 with noise
+    sigma 3-5), trains, and writes ironav_amd/weights/proposal.npz (or the path in $IA_WEIGHTS_OUT)."""
     import time
 
     t0 = time.time()
     log = lambda m: print(f"[{time.time() - t0:7.1f}s] {m}", flush=True)
-    feat, cell, cost = make_dataset(log=log)
+    # the generator's range of content, then as many again of heavy noise (sigma 3-5), where the cost
+    # head would otherwise predict too low a SAD
+    parts = [make_dataset(log=log), make_dataset(n_clips=100, seed=30000, noise=(3.0, 5.0), log=log)]
+    feat, cell, cost = (np.concatenate(x) for x in zip(*parts))
     log(f"dataset: {len(feat)} samples, {np.mean(cell >= 0):.3f} of targets on the grid")
     params = train(feat, cell, cost, log=log)
     out = os.environ.get("IA_WEIGHTS_OUT", WEIGHTS)
