@@ -6,20 +6,22 @@
 // least SAD and its displacement, ties to the first in raster order of (dy, dx).
 //
 // Design (integer VALU bound; SAD is not a dense contraction, so no MFMA):
-//  - The reference window is staged once in LDS as aligned dwords (coalesced loads, clipped at the
-//    plane's edges). A candidate's 16 samples of a row are 5 aligned dwords realigned in registers
-//    by v_alignbyte; 4 neighbouring lanes read the same dwords, so a wave touches 17 banks at most.
+//  - The reference window is staged once in LDS (coalesced loads, clipped at the plane's edges). For
+//    ranges up to 32 every byte alignment of a row is stored once, so a candidate's 16 samples of a
+//    row are 4 dwords read as they are; for ranges up to 64 (a 4x larger window) rows are aligned
+//    dwords and a candidate's row is 5 of them realigned by v_alignbyte.
 //  - A lane owns one column of displacements and walks a strip of K rows of it: each reference row
 //    loaded from LDS feeds the K candidates that use it (one row is read once per K candidates).
-//  - The 64 source dwords of the macroblock are uniform: they sit in scalar registers.
+//  - The 64 source dwords of the macroblock are uniform: scalar loads put them in scalar registers.
 //  - v_sad_hi_u8 accumulates each 4x4 block's SAD into the high half of a 32-bit word whose low half
 //    holds the candidate's row index in the window. Partition SADs are sums of 4x4 words, so every
 //    partition's word is already a sort key (SAD, then row) at one add per partition, and a running
-//    minimum per partition costs one v_min (v_min3 for two candidates) per candidate.
+//    minimum per partition costs one v_min3 per two candidates.
 //  - A lane's candidates are taken in increasing row order, and within one row (possible when the
 //    window is wider than the workgroup) in increasing column order, so the low half orders them as
-//    row * TQ + column step, TQ being the most columns of one row a lane visits. At the end the keys
-//    are rewritten as (SAD, raster index) and reduced over the workgroup.
+//    row * TQ + column step, TQ being the most columns of one row a lane visits. At the end the
+//    wave's least key per partition is found by DPP and decoded to (SAD, raster index) once, and the
+//    waves' results meet in LDS.
 #include "common.h"
 
 namespace ia {
