@@ -100,7 +100,9 @@ def main():
                          "batched: many streams of mixed sizes through the batched analysis; "
                          "learned-batched: 2160p streams with large motion and scene cuts, the learned "
                          "proposal inside the batched step vs the exhaustive search")
-    ap.add_argument("--slices", type=int, default=68, help="plain: slices per picture")
+    ap.add_argument("--slices", type=int, default=0, help="plain: slices per picture (0: one per macroblock row)")
+    ap.add_argument("--size", choices=["1080p", "720p"], default="1080p", help="plain: picture size")
+    ap.add_argument("--search", choices=["full", "diamond"], default="full", help="plain: motion search")
     ap.add_argument("--threads", type=int, default=16, help="plain: host threads of the CPU oracle")
     ap.add_argument("--streams", type=int, default=64, help="batched: streams")
     ap.add_argument("--frames", type=int, default=30, help="batched: frames per stream")
@@ -273,12 +275,13 @@ def digest(res):
 
 
 def bench_plain(args):
-    """The plain GPU path on the 1080p workload (BASELINE's bit-exact configuration: +-32 exhaustive
-    search, sub-8x8 partitions on (active at QP <= 20), 8x8 transform, CABAC, 3 references): one P
-    picture per step, exhaustive search + exact analysis of every macroblock, inputs resident. The
-    same picture's analysis by the CPU oracle (its exhaustive search with one thread per reference,
-    then the slices on `--threads` host threads, as the reference's encoder codes each slice on a
-    thread) is timed beside it and compared with the GPU's results bit for bit."""
+    """The plain GPU path (exhaustive or diamond search + exact analysis) on one P picture per step,
+    inputs resident, against the CPU oracle on host threads, bit-checked on the whole picture.
+    --size 1080p (default; BASELINE's bit-exact configuration): +-32 exhaustive search, sub-8x8 on,
+    8x8 transform, CABAC, 3 references. --size 720p (BASELINE's CPU-runnable configuration): the
+    reference's diamond search (--search diamond) or +-16 exhaustive search. The oracle runs its
+    exhaustive search (if any) with one thread per reference, then the slices on `--threads` host
+    threads, as the reference's encoder codes each slice on a thread of its own."""
     import concurrent.futures
     import numpy as np
     import torch
@@ -286,14 +289,18 @@ def bench_plain(args):
     from ironav_amd import abi, clipgen
     from ironav_amd.h264a import Analyser, Picture
 
-    width, height, nref, qp = 1920, 1080, 3, 28
-    spec = clipgen.workload_1080p()
+    big = args.size == "1080p"
+    width, height, nref, qp = (1920, 1080, 3, 28) if big else (1280, 720, 3, 28)
+    rng = 32 if big else 16
+    full = args.search == "full"
+    mode = abi.SEARCH_FULL if full else abi.SEARCH_DIAMOND
+    slices = args.slices or (height + 15) // 16
+    spec = clipgen.workload_1080p() if big else clipgen.workload_720p()
     spec.frames = 5
     pics = [Picture(*clipgen.pad_planes(*f)) for f in clipgen.make_clip(spec)]
     torch.cuda.init()
     stream = torch.cuda.Stream()
-    an = Analyser(width, height, nrefs=nref, slices=args.slices, search_mode=abi.SEARCH_FULL, search_range=32,
-                  sub8x8=True)
+    an = Analyser(width, height, nrefs=nref, slices=slices, search_mode=mode, search_range=rng, sub8x8=True)
     an.set_stream(stream.cuda_stream)
     refs = []
     for t in range(4):
@@ -308,7 +315,8 @@ def bench_plain(args):
     slots = list(range(nref))
 
     def step():
-        an.motion_search(slots)
+        if full:
+            an.motion_search(slots)
         an.analyse_uploaded(qp, abi.PIC_P, slots)
 
     for _ in range(args.warmup):
@@ -323,6 +331,7 @@ def bench_plain(args):
     ms = e0.elapsed_time(e1) / args.steps
     got = an.read_analysis()
     mbs = an.mb_w * an.mb_h
+    search = f"+-{rng} exhaustive search" if full else "the reference's diamond search"
     print(f"[plain] GPU {ms:.2f} ms per picture; CPU oracle next", file=sys.stderr, flush=True)
 
     cpu, exact = None, None
@@ -331,25 +340,30 @@ def bench_plain(args):
 
         pyoracle.build()
         t0 = time.perf_counter()
-        with concurrent.futures.ThreadPoolExecutor(max_workers=nref) as ex:
-            parts = list(ex.map(lambda r: pyoracle.fullsearch(pics[4].y, [refs[r][0].y], an.mb_w, an.mb_h, 32),
-                                range(nref)))
-        fs = np.ascontiguousarray(np.concatenate(parts))
+        fs = None
+        if full:
+            with concurrent.futures.ThreadPoolExecutor(max_workers=nref) as ex:
+                parts = list(ex.map(lambda r: pyoracle.fullsearch(pics[4].y, [refs[r][0].y], an.mb_w, an.mb_h, rng),
+                                    range(nref)))
+            fs = np.ascontiguousarray(np.concatenate(parts))
         t1 = time.perf_counter()
         codes, stats, rec, mbinfo = pyoracle.analyse_picture(an.cfg, pics[4], qp, abi.PIC_P,
                                                              [(refs[r][0], refs[r][1]) for r in range(nref)], fs,
                                                              threads=args.threads)
         t2 = time.perf_counter()
         exact = bool(np.array_equal(codes, got.codes) and np.array_equal(stats, got.stats)
-                     and np.array_equal(mbinfo, got.mbs) and all(np.array_equal(a, b) for a, b in zip(rec, (got.recon.y, got.recon.cb, got.recon.cr))))
+                     and np.array_equal(mbinfo, got.mbs)
+                     and all(np.array_equal(a, b) for a, b in zip(rec, (got.recon.y, got.recon.cb, got.recon.cr))))
         cores = os.cpu_count() or 0
-        cpu = {"value": round(mbs / (t2 - t0), 1), "unit": "MB/s", "cores": max(nref, min(args.threads, args.slices)),
+        used = min(args.threads, slices)
+        cpu = {"value": round(mbs / (t2 - t0), 1), "unit": "MB/s", "cores": max(nref if full else 1, used),
                "kind": "port",
-               "sample": f"the oracle's analysis of the same P picture (8160 macroblocks): exhaustive search "
-                         f"{t1 - t0:.1f} s on {nref} threads (one per reference), then {args.slices} slices on "
-                         f"{min(args.threads, args.slices)} threads {t2 - t1:.1f} s; host has {cores} logical CPUs"}
+               "sample": f"the oracle's analysis of the same P picture ({mbs} macroblocks): "
+                         + (f"exhaustive search {t1 - t0:.1f} s on {nref} threads (one per reference), then " if full
+                            else "")
+                         + f"{slices} slices on {used} threads {t2 - t1:.2f} s; host has {cores} logical CPUs"}
     out = {
-        "metric": "MB/s plain analysis @1080p: +-32 exhaustive search + exact code_mb, GPU vs CPU oracle",
+        "metric": f"MB/s plain analysis @{args.size}: {search} + exact code_mb, GPU vs CPU oracle",
         "value": round(mbs / (ms / 1000.0), 1),
         "unit": "MB/s",
         "n_gpus": 1,
@@ -360,10 +374,10 @@ def bench_plain(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u8",
-        "data": "synthetic (seeded generator clip 1920x1080 4:2:0; references coded IPPP by the analysis; inputs "
-                "resident in HBM)",
-        "config": {"workload": f"P picture at 1920x1080, QP {qp}, 3 references, +-32 exhaustive search, sub-8x8 "
-                               f"partitions on, 8x8 transform, CABAC, {args.slices} slices",
+        "data": f"synthetic (seeded generator clip {width}x{height} 4:2:0; references coded IPPP by the analysis; "
+                "inputs resident in HBM)",
+        "config": {"workload": f"P picture at {width}x{height}, QP {qp}, 3 references, {search}, sub-8x8 partitions "
+                               f"on, 8x8 transform, CABAC, {slices} slices",
                    "macroblocks_per_step": mbs},
         "bitexact_vs_oracle": exact,
         "cpu_baseline": cpu,
