@@ -60,7 +60,9 @@ struct HpJob {
   uint8_t *planes;
   int pw, ph, tiles_x, first;
 };
-constexpr int HP_TW = 64, HP_TH = 16;  // tile of the half-sample kernels
+// Strip of the half-sample kernels: one wave, 256 columns x HP_TH rows. A single picture's launch
+// has at most about HP_WAVES waves; a wave builds one or more strips of one column band.
+constexpr int HP_TW = 256, HP_TH = 4, HP_WAVES = 8192;
 hipError_t launch_halfpel_batch(const HpJob *jobs, int njobs, int nblocks, hipStream_t stream);
 
 }  // namespace ia

@@ -19,7 +19,7 @@ def frames(w, h, n, seed, **kw):
     return [Picture(*clipgen.pad_planes(*f)) for f in clipgen.make_clip(spec)]
 
 
-@pytest.mark.parametrize("w,h", [(16, 16), (200, 120), (1920, 1080)])
+@pytest.mark.parametrize("w,h", [(16, 16), (200, 120), (1280, 720), (1920, 1080), (3840, 2160)])
 def test_halfpel_planes_bit_exact(oracle, w, h):
     pics = frames(w, h, 1, 21)
     with Analyser(w, h, nrefs=1) as an:
