@@ -163,6 +163,19 @@ int ia_set_reference(ia_ctx *ctx, int slot, const uint8_t *y, const uint8_t *cb,
  * of border) to `out`. For checking. */
 int ia_read_halfpel(ia_ctx *ctx, int slot, uint8_t *out);
 
+/* Makes the last analysed picture the reference in `slot` without a host round trip: its
+ * reconstruction and macroblock records are copied on the device; with `deblock` non-zero the copy
+ * is loop-filtered first, as the encoder filters each picture before keeping it
+ * (enc/mod.rs:653-654, h264/deblock.rs:608-695: frame macroblocks, 4:2:0, filtering across slice
+ * edges, zero offsets); then its half-sample planes are built (enc/mod.rs:660). Replaces the
+ * reference's deblock_picture + HalfPel::new + reference list push (enc/mod.rs:653-667). The
+ * analysis outputs (ia_read_analysis) stay the reconstruction before filtering. Synchronous. */
+int ia_promote_reference(ia_ctx *ctx, int slot, int deblock);
+
+/* Copies reference `slot`: its luma (mb_w*16 x mb_h*16), chroma and records; any pointer may be
+ * NULL. Synchronous. */
+int ia_read_reference(ia_ctx *ctx, int slot, uint8_t *y, uint8_t *cb, uint8_t *cr, ia_mbinfo *mbs);
+
 /* Uploads the picture to analyse (host, padded layout) to the device, on the context's stream. */
 int ia_upload_source(ia_ctx *ctx, const uint8_t *y, const uint8_t *cb, const uint8_t *cr);
 

@@ -84,6 +84,8 @@ PROTOTYPES = {
     "ia_set_stream": (C.c_int, [vp, vp]),
     "ia_set_reference": (C.c_int, [vp, C.c_int, vp, vp, vp, vp]),
     "ia_read_halfpel": (C.c_int, [vp, C.c_int, vp]),
+    "ia_promote_reference": (C.c_int, [vp, C.c_int, C.c_int]),
+    "ia_read_reference": (C.c_int, [vp, C.c_int, vp, vp, vp, vp]),
     "ia_upload_source": (C.c_int, [vp, vp, vp, vp]),
     "ia_motion_search": (C.c_int, [vp, ip, C.c_int]),
     "ia_read_motion": (C.c_int, [vp, vp]),

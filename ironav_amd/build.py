@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libironav_h264a.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["api.hip", "batch.hip", "halfpel.hip", "fullsearch.hip", "analyse.hip", "proposal.hip"]
+SOURCES = ["api.hip", "batch.hip", "halfpel.hip", "deblock.hip", "fullsearch.hip", "analyse.hip", "proposal.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950",

@@ -175,8 +175,10 @@ int ia_create(ia_ctx **out, const ia_config *cfg) {
        alloc((void **)&c->state, nmb * sizeof(ia::MbState)) == hipSuccess &&
        alloc((void **)&c->entropy, 128 * sizeof(uint32_t)) == hipSuccess &&
        alloc((void **)&c->wts, sizeof(ia::CnnWeights)) == hipSuccess &&
-       alloc((void **)&c->fb, (2 + 4 * nmb) * sizeof(int)) == hipSuccess;
-  ok = ok && hipMemset(c->fb, 0, 2 * sizeof(int)) == hipSuccess;
+       alloc((void **)&c->fb, (2 + 4 * nmb) * sizeof(int)) == hipSuccess &&
+       alloc((void **)&c->db, (1 + (size_t)g.mb_h) * sizeof(int)) == hipSuccess;
+  ok = ok && hipMemset(c->fb, 0, 2 * sizeof(int)) == hipSuccess &&
+       hipMemset(c->db, 0, (1 + (size_t)g.mb_h) * sizeof(int)) == hipSuccess;
   if (ok) {
     // -log2 of the MPS/LPS probability per CABAC state in 1/256 bits (enc/cabac.rs:162-173), computed
     // with the C library's pow/log2 as the reference computes it with Rust's (which call the same)
@@ -214,6 +216,7 @@ void ia_destroy(ia_ctx *c) {
   (void)hipFree(c->entropy);
   (void)hipFree(c->wts);
   (void)hipFree(c->fb);
+  (void)hipFree(c->db);
   (void)hipFree(c->feat);
   (void)hipFree(c->cnn_out);
   (void)hipFree(c->centres);
@@ -252,6 +255,50 @@ int ia_set_reference(ia_ctx *c, int slot, const uint8_t *y, const uint8_t *cb, c
   IA_HIP(c, hipStreamSynchronize(c->stream));
   s.loaded = true;
   c->fs_valid = false;
+  return IA_OK;
+}
+
+int ia_promote_reference(ia_ctx *c, int slot, int deblock) {
+  if (!c) return IA_EINVAL;
+  if (slot < 0 || slot >= c->cfg.nrefs) return fail(c, IA_EINVAL, "bad reference slot");
+  if (!c->an_valid) return fail(c, IA_ESTATE, "no analysed picture to promote");
+  const Geom &g = c->g;
+  const size_t ylen = (size_t)g.pw * g.ph, clen = (size_t)g.cw * g.ch, nmb = (size_t)g.mb_w * g.mb_h;
+  IaSlot &s = c->slots[slot];
+  s.loaded = false;
+  IA_HIP(c, hipMemcpyAsync(c->tmp_luma, c->rec[0], ylen, hipMemcpyDeviceToDevice, c->stream));
+  IA_HIP(c, hipMemcpyAsync(s.cb, c->rec[1], clen, hipMemcpyDeviceToDevice, c->stream));
+  IA_HIP(c, hipMemcpyAsync(s.cr, c->rec[2], clen, hipMemcpyDeviceToDevice, c->stream));
+  IA_HIP(c, hipMemcpyAsync(s.mbs, c->mbs, nmb * sizeof(ia_mbinfo), hipMemcpyDeviceToDevice, c->stream));
+  if (deblock) IA_HIP(c, ia::launch_deblock(c->tmp_luma, s.cb, s.cr, c->mbs, g.mb_w, g.mb_h, c->db + 1, c->db, c->stream));
+  IA_HIP(c, ia::launch_halfpel(c->tmp_luma, g.pw, g.ph, s.hp, c->stream));
+  int stalled = 0;
+  IA_HIP(c, hipMemcpyAsync(&stalled, c->db, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  IA_HIP(c, hipStreamSynchronize(c->stream));
+  if (stalled) {
+    IA_HIP(c, hipMemset(c->db, 0, (1 + (size_t)g.mb_h) * sizeof(int)));
+    return fail(c, IA_EDEVICE, "the loop filter's rows stalled");
+  }
+  s.loaded = true;
+  c->fs_valid = false;
+  return IA_OK;
+}
+
+int ia_read_reference(ia_ctx *c, int slot, uint8_t *y, uint8_t *cb, uint8_t *cr, ia_mbinfo *mbs) {
+  if (!c) return IA_EINVAL;
+  if (slot < 0 || slot >= c->cfg.nrefs) return fail(c, IA_EINVAL, "bad reference slot");
+  if (!c->slots[slot].loaded) return fail(c, IA_ESTATE, "reference slot is empty");
+  const Geom &g = c->g;
+  const IaSlot &s = c->slots[slot];
+  const size_t clen = (size_t)g.cw * g.ch, nmb = (size_t)g.mb_w * g.mb_h;
+  // the luma is plane G's inside (its border repeats the picture's edges)
+  if (y)
+    IA_HIP(c, hipMemcpy2DAsync(y, g.pw, s.hp + (size_t)ia::PAD * g.sw + ia::PAD, g.sw, g.pw, g.ph, hipMemcpyDeviceToHost,
+                               c->stream));
+  if (cb) IA_HIP(c, hipMemcpyAsync(cb, s.cb, clen, hipMemcpyDeviceToHost, c->stream));
+  if (cr) IA_HIP(c, hipMemcpyAsync(cr, s.cr, clen, hipMemcpyDeviceToHost, c->stream));
+  if (mbs) IA_HIP(c, hipMemcpyAsync(mbs, s.mbs, nmb * sizeof(ia_mbinfo), hipMemcpyDeviceToHost, c->stream));
+  IA_HIP(c, hipStreamSynchronize(c->stream));
   return IA_OK;
 }
 

@@ -65,4 +65,19 @@ struct HpJob {
 constexpr int HP_TW = 256, HP_TH = 4, HP_WAVES = 8192;
 hipError_t launch_halfpel_batch(const HpJob *jobs, int njobs, int nblocks, hipStream_t stream);
 
+// The loop filter of one picture in place (one wave per macroblock row): planes y (stride mb_w*16),
+// cb and cr (stride mb_w*8) from the picture's records. prog: mb_h counters, 0 before the launch and
+// 0 again after it; err: set non-zero if a row stalled waiting for the one above.
+hipError_t launch_deblock(uint8_t *y, uint8_t *cb, uint8_t *cr, const ia_mbinfo *mbs, int mb_w, int mb_h, int *prog,
+                          int *err, hipStream_t stream);
+// One picture's loop filter in a batched launch: blocks [first, first + mb_h) filter its rows; a null
+// `y` skips the picture this time.
+struct DbJob {
+  uint8_t *y, *cb, *cr;
+  const ia_mbinfo *mbs;
+  int *prog;
+  int mb_w, mb_h, first;
+};
+hipError_t launch_deblock_batch(const DbJob *jobs, int njobs, int nblocks, int *err, hipStream_t stream);
+
 }  // namespace ia

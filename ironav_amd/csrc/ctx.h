@@ -44,6 +44,8 @@ struct ia_ctx {
   int fb_gen = 0;              // the count the next learned search uses
   float *feat = nullptr, *cnn_out = nullptr;
   int16_t *centres = nullptr;
+  // loop filter: [0] stall flag, [1 .. mb_h] row progress (both 0 between launches)
+  int *db = nullptr;
   std::string err;
 };
 

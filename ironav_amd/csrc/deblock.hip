@@ -1,0 +1,386 @@
+// deblock.hip — the encoder's loop filter on gfx950.
+//
+// Filters a reconstructed picture in place as the encoder does before the picture becomes a
+// reference (enc/mod.rs:653-654; h264/deblock.rs:232-421 and :608-695): frame macroblocks, 4:2:0,
+// 8-bit samples, every slice filtered across slice edges with zero alpha/beta and chroma QP offsets
+// (the encoder's slice parameters), no I_PCM or lossless macroblocks.
+//
+// Macroblocks are filtered in raster order, each its vertical edges and then its horizontal ones,
+// and a macroblock's edges read what the macroblocks left of and above it wrote. One wave filters
+// one macroblock row from left to right. Macroblock (x, y) starts once row y-1 has finished
+// macroblock x+1, whose left edge changes the right columns of (x, y-1): the wavefront the
+// reference's `deblock_picture_par` runs on threads (h264/deblock.rs:583-603). Rows publish their
+// progress with an agent-scope release store on a counter per row and read it with an acquire load
+// (rows of one picture run on different CUs and XCDs); a row re-reads its counter only when the last
+// value it saw is not far enough ahead.
+//
+// Within a macroblock, lanes 0-15 hold the 16 luma lines, lanes 16-23 the Cb lines and 24-31 the Cr
+// lines. For the vertical edges a lane holds its row (the macroblock's samples and the 4 columns left
+// of it) in registers; an LDS tile transposes the rows into columns (with the 4 rows above) for the
+// horizontal edges, and back for the stores. Lanes 0-31 first derive the 32 boundary strengths
+// (direction x edge x 4-line segment) of the macroblock, one each, and the filtering lanes gather
+// theirs with cross-lane reads.
+#include "common.h"
+#include "h264_tables.h"
+
+namespace ia {
+namespace {
+
+constexpr int SPIN_LIMIT = 1 << 20;  // polls of the row above before the kernel reports a stall
+
+__device__ __forceinline__ int iabs(int v) { return v < 0 ? -v : v; }
+__device__ __forceinline__ int clip3(int lo, int hi, int v) { return min(max(v, lo), hi); }
+
+// Whether the transform block holding 4x4 block (bx, by) has coefficients (h264/deblock.rs:336-344).
+__device__ __forceinline__ bool coded(const ia_mbinfo &m, int bx, int by) {
+  if (m.t8x8) {
+    const int x0 = bx & 2, y0 = by & 2;
+    return (m.nnz[y0 * 4 + x0] | m.nnz[y0 * 4 + x0 + 1] | m.nnz[(y0 + 1) * 4 + x0] | m.nnz[(y0 + 1) * 4 + x0 + 1]) != 0;
+  }
+  return m.nnz[by * 4 + bx] != 0;
+}
+
+__device__ __forceinline__ bool far(const int16_t *a, const int16_t *b) {
+  return iabs(a[0] - b[0]) >= 4 || iabs(a[1] - b[1]) >= 4;
+}
+
+// Different references, a different number of vectors, or vectors a whole sample apart
+// (h264/deblock.rs:352-383). List 1 is unused in P pictures: reference -1, vector 0.
+__device__ bool motion_differs(const ia_mbinfo &p, int pbx, int pby, const ia_mbinfo &q, int qbx, int qby) {
+  const int16_t zero[2] = {0, 0};
+  const int p8 = (pby >> 1) * 2 + (pbx >> 1), q8 = (qby >> 1) * 2 + (qbx >> 1);
+  const int pr0 = p.ref_id[p8], qr0 = q.ref_id[q8];
+  const int16_t *pm0 = p.mv[pby * 4 + pbx], *qm0 = q.mv[qby * 4 + qbx];
+  const int np = pr0 >= 0, nq = qr0 >= 0;
+  if (np != nq) return true;
+  if (np == 1) return pr0 != qr0 || far(pm0, qm0);
+  // no vector on either side (pr0 == qr0 == -1 with list 1 unused): the reference's last case
+  return (far(pm0, qm0) || far(zero, zero)) && (far(pm0, zero) || far(zero, qm0));
+}
+
+__device__ bool uniform_motion(const ia_mbinfo &m) {
+  for (int i = 1; i < 4; i++)
+    if (m.ref_idx[i] != m.ref_idx[0] || m.ref_id[i] != m.ref_id[0]) return false;
+  for (int i = 1; i < 16; i++)
+    if (m.mv[i][0] != m.mv[0][0] || m.mv[i][1] != m.mv[0][1]) return false;
+  return true;
+}
+
+// Boundary strength of segment i of edge e (0: the macroblock edge) in direction dir (0: vertical
+// edges) of macroblock m; o is the macroblock across a macroblock edge (h264/deblock.rs:391-408).
+__device__ int strength(const ia_mbinfo &m, const ia_mbinfo &o, int dir, int e, int i) {
+  if (m.kind != IA_KIND_INTER || o.kind != IA_KIND_INTER) return e == 0 ? 4 : 3;
+  const int qbx = dir == 0 ? e : i, qby = dir == 0 ? i : e;
+  const int pbx = dir == 0 ? (e == 0 ? 3 : e - 1) : i, pby = dir == 0 ? i : (e == 0 ? 3 : e - 1);
+  if (coded(m, qbx, qby) || coded(o, pbx, pby)) return 2;
+  return motion_differs(o, pbx, pby, m, qbx, qby) ? 1 : 0;
+}
+
+// alpha, beta and tc0 of strength b at average QP qpav (offsets 0: indexA = indexB = qpav).
+struct Thr {
+  int alpha, beta, tc0;
+};
+__device__ __forceinline__ Thr thresholds(int qpav, int b) {
+  const int ia = clip3(0, 51, qpav);
+  Thr t;
+  t.alpha = IA_ALPHA[ia];
+  t.beta = IA_BETA[ia];
+  t.tc0 = b > 0 && b < 4 ? max((int)IA_TC0[ia][b], 0) : 0;
+  return t;
+}
+
+__device__ __forceinline__ int chroma_qp(int qp) {
+  const int q = clip3(0, 51, qp);
+  return q < 30 ? q : IA_CHROMA_QP[q - 30];
+}
+
+// One luma sample line across an edge, p3..p0 | q0..q3, strength b > 0 (h264/deblock.rs:279-334).
+__device__ __forceinline__ void filter_luma(int &p3, int &p2, int &p1, int &p0, int &q0, int &q1, int &q2, int &q3,
+                                            int b, const Thr &t) {
+  if (t.alpha == 0 || t.beta == 0) return;
+  const int P0 = p0, Q0 = q0, P1 = p1, Q1 = q1, P2 = p2, Q2 = q2;
+  if (iabs(P0 - Q0) >= t.alpha || iabs(P1 - P0) >= t.beta || iabs(Q1 - Q0) >= t.beta) return;
+  const int ap = iabs(P2 - P0), aq = iabs(Q2 - Q0);
+  if (b < 4) {
+    const int tc = t.tc0 + (ap < t.beta) + (aq < t.beta);
+    const int delta = clip3(-tc, tc, (((Q0 - P0) * 4) + (P1 - Q1) + 4) >> 3);
+    p0 = clip3(0, 255, P0 + delta);
+    q0 = clip3(0, 255, Q0 - delta);
+    if (ap < t.beta) p1 = P1 + clip3(-t.tc0, t.tc0, (P2 + ((P0 + Q0 + 1) >> 1) - (2 * P1)) >> 1);
+    if (aq < t.beta) q1 = Q1 + clip3(-t.tc0, t.tc0, (Q2 + ((P0 + Q0 + 1) >> 1) - (2 * Q1)) >> 1);
+    return;
+  }
+  const bool small = iabs(P0 - Q0) < (t.alpha >> 2) + 2;
+  if (ap < t.beta && small) {
+    p0 = (P2 + 2 * P1 + 2 * P0 + 2 * Q0 + Q1 + 4) >> 3;
+    p1 = (P2 + P1 + P0 + Q0 + 2) >> 2;
+    p2 = (2 * p3 + 3 * P2 + P1 + P0 + Q0 + 4) >> 3;
+  } else {
+    p0 = (2 * P1 + P0 + Q1 + 2) >> 2;
+  }
+  if (aq < t.beta && small) {
+    q0 = (P1 + 2 * P0 + 2 * Q0 + 2 * Q1 + Q2 + 4) >> 3;
+    q1 = (P0 + Q0 + Q1 + Q2 + 2) >> 2;
+    q2 = (2 * q3 + 3 * Q2 + Q1 + Q0 + P0 + 4) >> 3;
+  } else {
+    q0 = (2 * Q1 + Q0 + P1 + 2) >> 2;
+  }
+}
+
+// One chroma sample line across an edge, p1 p0 | q0 q1 (h264/deblock.rs:279-298).
+__device__ __forceinline__ void filter_chroma(int &p1, int &p0, int &q0, int &q1, int b, const Thr &t) {
+  if (t.alpha == 0 || t.beta == 0) return;
+  const int P0 = p0, Q0 = q0, P1 = p1, Q1 = q1;
+  if (iabs(P0 - Q0) >= t.alpha || iabs(P1 - P0) >= t.beta || iabs(Q1 - Q0) >= t.beta) return;
+  if (b < 4) {
+    const int tc = t.tc0 + 1;
+    const int delta = clip3(-tc, tc, (((Q0 - P0) * 4) + (P1 - Q1) + 4) >> 3);
+    p0 = clip3(0, 255, P0 + delta);
+    q0 = clip3(0, 255, Q0 - delta);
+  } else {
+    p0 = (2 * P1 + P0 + Q1 + 2) >> 2;
+    q0 = (2 * Q1 + Q0 + P1 + 2) >> 2;
+  }
+  (void)p1;
+  (void)q1;
+}
+
+__device__ __forceinline__ uint32_t pack4(const int *v) {
+  return (uint32_t)v[0] | ((uint32_t)v[1] << 8) | ((uint32_t)v[2] << 16) | ((uint32_t)v[3] << 24);
+}
+__device__ __forceinline__ void unpack4(uint32_t w, int *v) {
+#pragma unroll
+  for (int k = 0; k < 4; k++) v[k] = (w >> (8 * k)) & 0xff;
+}
+
+// Tiles: luma rows y0-4 .. y0+15 with columns x0-4 .. x0+15 at 12 .. 31 (the macroblock's own 16
+// columns 16-aligned); chroma rows cy0-2 .. cy0+7 with columns cx0-4 .. cx0+7 at 4 .. 15.
+constexpr int LX = 12, CX = 4;
+struct DbLds {
+  __attribute__((aligned(16))) uint8_t y[20][32];
+  __attribute__((aligned(16))) uint8_t c[2][10][16];
+  uint8_t bs[32];  // strengths [dir][e][segment]
+};
+
+// Macroblock row `row` of one picture (one wave). prog[r]: macroblocks of row r finished (the
+// counters start at 0 and are 0 again when the launch ends); err: set when a row waited too long.
+__device__ void deblock_row(DbLds &L, uint8_t *__restrict__ py, uint8_t *__restrict__ pcb, uint8_t *__restrict__ pcr,
+                            const ia_mbinfo *__restrict__ mbs, int mb_w, int mb_h, int row, int *prog, int *err) {
+  const int lane = threadIdx.x;
+  const int ls = mb_w * 16, cs = mb_w * 8;
+  const int y0 = row * 16, cy0 = row * 8;
+  const bool luma = lane < 16, chroma = lane >= 16 && lane < 32;
+  const int cl = (lane - 16) & 7;            // chroma line
+  uint8_t *cpl = lane < 24 ? pcb : pcr;      // chroma plane of lanes 16-31
+  const int comp = lane < 24 ? 0 : 1;
+  int left[4] = {0, 0, 0, 0};                // the 4 samples left of the macroblock on the lane's line
+  int seen = 0;                              // last progress of the row above this wave read
+  for (int x = 0; x < mb_w; x++) {
+    const int x0 = x * 16, cx0 = x * 8;
+    // the macroblock's own samples: untouched by the filtering of any earlier macroblock
+    int s[20];  // luma lane: columns x0-4 .. x0+15 of row y0+lane; chroma lane: columns cx0-4 .. cx0+7
+    if (luma) {
+      const uint4 v = *reinterpret_cast<const uint4 *>(py + (size_t)(y0 + lane) * ls + x0);
+      unpack4(v.x, s + 4);
+      unpack4(v.y, s + 8);
+      unpack4(v.z, s + 12);
+      unpack4(v.w, s + 16);
+    } else if (chroma) {
+      const uint2 v = *reinterpret_cast<const uint2 *>(cpl + (size_t)(cy0 + cl) * cs + cx0);
+      unpack4(v.x, s + 4);
+      unpack4(v.y, s + 8);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) s[k] = left[k];
+    const int idx = row * mb_w + x;
+    const ia_mbinfo &m = mbs[idx];
+    const bool live = m.slice != 0;
+    const bool left_ok = live && x > 0 && mbs[idx - 1].slice != 0;
+    const bool top_ok = live && row > 0 && mbs[idx - mb_w].slice != 0;
+    const ia_mbinfo &ml = x > 0 ? mbs[idx - 1] : m;
+    const ia_mbinfo &mt = row > 0 ? mbs[idx - mb_w] : m;
+    // boundary strengths, one per lane 0-31: dir = lane >> 4, edge e = (lane >> 2) & 3, segment lane & 3
+    if (lane < 32 && live) {
+      const int dir = lane >> 4, e = (lane >> 2) & 3, i = lane & 3;
+      bool still = m.kind == IA_KIND_INTER && uniform_motion(m);
+      for (int k = 0; k < 16 && still; k++) still = m.nnz[k] == 0;
+      L.bs[lane] = (uint8_t)(e > 0 && still ? 0 : strength(m, e == 0 ? (dir == 0 ? ml : mt) : m, dir, e, i));
+    }
+    // wait for macroblock x+1 of the row above (the last one at the row's end); after a wait that
+    // timed out the row waits no more (the launch reports the stall and its output is not used)
+    if (row > 0) {
+      const int need = min(x + 2, mb_w);
+      if (seen < need) {
+        if (lane == 0) {
+          int v = __hip_atomic_load(prog + row - 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+          for (int n = 0; v < need; n++) {
+            if (n == SPIN_LIMIT) {
+              __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              v = mb_w;
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            v = __hip_atomic_load(prog + row - 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+          }
+          seen = v;
+        }
+        seen = __shfl(seen, 0);
+      }
+    }
+    __syncthreads();  // strengths in LDS; the row above's stores visible to every lane
+    // the rows above: luma y0-4 .. y0-1 into tile rows 0-3, chroma cy0-2 .. cy0-1 into rows 0-1
+    if (top_ok) {
+      if (lane < 4) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(py + (size_t)(y0 - 4 + lane) * ls + x0);
+        *reinterpret_cast<uint4 *>(&L.y[lane][LX + 4]) = v;
+      } else if (lane >= 16 && lane < 20) {
+        const int c = (lane - 16) >> 1, r = (lane - 16) & 1;
+        const uint2 v = *reinterpret_cast<const uint2 *>((c ? pcr : pcb) + (size_t)(cy0 - 2 + r) * cs + cx0);
+        *reinterpret_cast<uint2 *>(&L.c[c][r][CX + 4]) = v;
+      }
+    }
+    if (live) {
+      // vertical edges, on the lane's row in registers
+      if (luma) {
+        const int seg = lane >> 2;
+        const int qpl = (ml.qp + m.qp + 1) >> 1;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          if ((e == 0 && !left_ok) || (m.t8x8 && (e & 1))) continue;
+          const int b = L.bs[e * 4 + seg];
+          if (b == 0) continue;
+          const Thr t = thresholds(e == 0 ? qpl : m.qp, b);
+          const int q = 4 + 4 * e;
+          filter_luma(s[q - 4], s[q - 3], s[q - 2], s[q - 1], s[q], s[q + 1], s[q + 2], s[q + 3], b, t);
+        }
+      } else if (chroma) {
+        const int seg = cl >> 1;
+        const int qpc = (chroma_qp(ml.qp) + chroma_qp(m.qp) + 1) >> 1, qpm = chroma_qp(m.qp);
+#pragma unroll
+        for (int e = 0; e < 4; e += 2) {
+          if (e == 0 && !left_ok) continue;
+          const int b = L.bs[e * 4 + seg];
+          if (b == 0) continue;
+          const Thr t = thresholds(e == 0 ? qpc : qpm, b);
+          const int q = 4 + 2 * e;
+          filter_chroma(s[q - 2], s[q - 1], s[q], s[q + 1], b, t);
+        }
+      }
+    }
+    // rows into the tile
+    if (luma) {
+#pragma unroll
+      for (int k = 0; k < 5; k++) *reinterpret_cast<uint32_t *>(&L.y[4 + lane][LX + 4 * k]) = pack4(s + 4 * k);
+    } else if (chroma) {
+#pragma unroll
+      for (int k = 0; k < 3; k++) *reinterpret_cast<uint32_t *>(&L.c[comp][2 + cl][CX + 4 * k]) = pack4(s + 4 * k);
+    }
+    __syncthreads();
+    if (live) {
+      // horizontal edges, on the lane's column
+      if (luma) {
+        int v[20];
+#pragma unroll
+        for (int r = 0; r < 20; r++) v[r] = L.y[r][LX + 4 + lane];
+        const int seg = lane >> 2;
+        const int qpt = (mt.qp + m.qp + 1) >> 1;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          if ((e == 0 && !top_ok) || (m.t8x8 && (e & 1))) continue;
+          const int b = L.bs[16 + e * 4 + seg];
+          if (b == 0) continue;
+          const Thr t = thresholds(e == 0 ? qpt : m.qp, b);
+          const int q = 4 + 4 * e;
+          filter_luma(v[q - 4], v[q - 3], v[q - 2], v[q - 1], v[q], v[q + 1], v[q + 2], v[q + 3], b, t);
+        }
+#pragma unroll
+        for (int r = 1; r < 20; r++) L.y[r][LX + 4 + lane] = (uint8_t)v[r];
+      } else if (chroma) {
+        int v[10];
+#pragma unroll
+        for (int r = 0; r < 10; r++) v[r] = L.c[comp][r][CX + 4 + cl];
+        const int seg = cl >> 1;
+        const int qpc = (chroma_qp(mt.qp) + chroma_qp(m.qp) + 1) >> 1, qpm = chroma_qp(m.qp);
+#pragma unroll
+        for (int e = 0; e < 4; e += 2) {
+          if (e == 0 && !top_ok) continue;
+          const int b = L.bs[16 + e * 4 + seg];
+          if (b == 0) continue;
+          const Thr t = thresholds(e == 0 ? qpc : qpm, b);
+          const int q = 2 + 2 * e;
+          filter_chroma(v[q - 2], v[q - 1], v[q], v[q + 1], b, t);
+        }
+#pragma unroll
+        for (int r = 1; r < 10; r++) L.c[comp][r][CX + 4 + cl] = (uint8_t)v[r];
+      }
+    }
+    __syncthreads();
+    // stores: the lane's row (its 4 left samples belong to the macroblock on the left), the rows above
+    if (luma) {
+      uint32_t w[5];
+#pragma unroll
+      for (int k = 0; k < 5; k++) w[k] = *reinterpret_cast<const uint32_t *>(&L.y[4 + lane][LX + 4 * k]);
+      uint8_t *rp = py + (size_t)(y0 + lane) * ls + x0;
+      *reinterpret_cast<uint4 *>(rp) = make_uint4(w[1], w[2], w[3], w[4]);
+      if (x > 0) *reinterpret_cast<uint32_t *>(rp - 4) = w[0];
+      unpack4(w[4], left);
+      if (top_ok && lane >= 1 && lane < 4)
+        *reinterpret_cast<uint4 *>(py + (size_t)(y0 - 4 + lane) * ls + x0) = *reinterpret_cast<const uint4 *>(&L.y[lane][LX + 4]);
+    } else if (chroma) {
+      uint32_t w[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) w[k] = *reinterpret_cast<const uint32_t *>(&L.c[comp][2 + cl][CX + 4 * k]);
+      uint8_t *rp = cpl + (size_t)(cy0 + cl) * cs + cx0;
+      *reinterpret_cast<uint2 *>(rp) = make_uint2(w[1], w[2]);
+      if (x > 0) *reinterpret_cast<uint32_t *>(rp - 4) = w[0];
+      unpack4(w[2], left);
+      if (top_ok && cl == 0)
+        *reinterpret_cast<uint2 *>(cpl + (size_t)(cy0 - 1) * cs + cx0) = *reinterpret_cast<const uint2 *>(&L.c[comp][1][CX + 4]);
+    }
+    __syncthreads();  // every lane's stores issued before the progress is published
+    if (lane == 0) __hip_atomic_store(prog + row, x + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  // the counter of the row above is read no more: clear it for the next launch (and the last row its own)
+  if (lane == 0) {
+    if (row > 0) __hip_atomic_store(prog + row - 1, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (row == mb_h - 1) __hip_atomic_store(prog + row, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+__global__ __launch_bounds__(64) void deblock_kernel(uint8_t *py, uint8_t *pcb, uint8_t *pcr, const ia_mbinfo *mbs,
+                                                      int mb_w, int mb_h, int *prog, int *err) {
+  __shared__ DbLds lds;
+  deblock_row(lds, py, pcb, pcr, mbs, mb_w, mb_h, blockIdx.x, prog, err);
+}
+
+// Many pictures: block b filters row b - first of the job whose rows hold it (jobs by `first`).
+__global__ __launch_bounds__(64) void deblock_batch_kernel(const DbJob *jobs, int njobs, int *err) {
+  __shared__ DbLds lds;
+  const int b = blockIdx.x;
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].first <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const DbJob &j = jobs[lo];
+  if (!j.y) return;
+  deblock_row(lds, j.y, j.cb, j.cr, j.mbs, j.mb_w, j.mb_h, b - j.first, j.prog, err);
+}
+
+}  // namespace
+
+hipError_t launch_deblock(uint8_t *y, uint8_t *cb, uint8_t *cr, const ia_mbinfo *mbs, int mb_w, int mb_h, int *prog,
+                          int *err, hipStream_t stream) {
+  if (mb_w < 1 || mb_h < 1 || !y || !cb || !cr || !mbs || !prog || !err) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(deblock_kernel, dim3(mb_h), dim3(64), 0, stream, y, cb, cr, mbs, mb_w, mb_h, prog, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_deblock_batch(const DbJob *jobs, int njobs, int nblocks, int *err, hipStream_t stream) {
+  if (njobs < 1 || nblocks < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(deblock_batch_kernel, dim3(nblocks), dim3(64), 0, stream, jobs, njobs, err);
+  return hipGetLastError();
+}
+
+}  // namespace ia

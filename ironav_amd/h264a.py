@@ -149,6 +149,20 @@ class Analyser:
         self._ok(library().ia_set_reference(self._h, slot, abi.ptr(pic.y), abi.ptr(pic.cb), abi.ptr(pic.cr),
                                             abi.ptr(mbs)))
 
+    def promote_reference(self, slot, deblock=True):
+        """Makes the last analysed picture the reference in `slot` on the device: loop-filtered first
+        when `deblock` (as the encoder keeps its references), then its half-sample planes built."""
+        self._ok(library().ia_promote_reference(self._h, slot, 1 if deblock else 0))
+
+    def read_reference(self, slot):
+        """(Picture, macroblock records) of reference `slot`."""
+        y = np.empty((self.ph, self.pw), dtype=np.uint8)
+        cb = np.empty((self.ph // 2, self.pw // 2), dtype=np.uint8)
+        cr = np.empty_like(cb)
+        mbs = np.empty(self.mb_w * self.mb_h, dtype=abi.MBINFO_DTYPE)
+        self._ok(library().ia_read_reference(self._h, slot, abi.ptr(y), abi.ptr(cb), abi.ptr(cr), abi.ptr(mbs)))
+        return Picture(y, cb, cr), mbs
+
     def read_halfpel(self, slot):
         sw, sh = self.pw + 96, self.ph + 96
         out = np.empty((4, sh, sw), dtype=np.uint8)

@@ -9,6 +9,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <stddef.h>
 #include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
@@ -2283,6 +2284,183 @@ int or_analyse_picture(const ia_config *cfg, const uint8_t *y, const uint8_t *cb
                        uint8_t *recon_cb, uint8_t *recon_cr, ia_mbinfo *mbs) {
   return or_analyse_picture_mt(cfg, y, cb, cr, qp, kind, ref_y, ref_cb, ref_cr, ref_mbs, nref, fs, codes, stats,
                                recon_y, recon_cb, recon_cr, mbs, 1);
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * The loop filter as the encoder runs it on each reconstructed picture before it becomes a reference
+ * (enc/mod.rs:653-654): frame macroblocks (no MBAFF), 4:2:0, 8-bit samples, every slice with
+ * disable_deblocking_filter_idc 0, zero alpha/beta offsets and zero chroma QP offset, no I_PCM or
+ * lossless macroblocks. Follows h264/deblock.rs:232-421 and :608-695.
+ * ------------------------------------------------------------------------------------------- */
+
+/* Whether the transform block holding 4x4 block (bx, by) has coefficients (h264/deblock.rs:336-344). */
+static int db_coded(const ia_mbinfo *m, int bx, int by) {
+  if (m->t8x8) {
+    const int x0 = bx & 2, y0 = by & 2;
+    return m->nnz[y0 * 4 + x0] | m->nnz[y0 * 4 + x0 + 1] | m->nnz[(y0 + 1) * 4 + x0] | m->nnz[(y0 + 1) * 4 + x0 + 1];
+  }
+  return m->nnz[by * 4 + bx] > 0;
+}
+
+/* Vectors a whole sample or more apart (h264/deblock.rs:347-349, frame macroblocks: mvy = 4). */
+static int db_far(const int16_t *a, const int16_t *b) {
+  return abs(a[0] - b[0]) >= 4 || abs(a[1] - b[1]) >= 4;
+}
+
+/* Different references, a different number of vectors, or vectors a sample apart
+ * (h264/deblock.rs:352-383); list 1 is unused in P pictures (reference -1, vector 0). */
+static int db_motion_differs(const ia_mbinfo *p, int pbx, int pby, const ia_mbinfo *q, int qbx, int qby) {
+  static const int16_t zero[2] = {0, 0};
+  const int p8 = (pby >> 1) * 2 + (pbx >> 1), q8 = (qby >> 1) * 2 + (qbx >> 1);
+  const int32_t pr[2] = {p->ref_id[p8], -1}, qr[2] = {q->ref_id[q8], -1};
+  const int16_t *pm[2] = {p->mv[pby * 4 + pbx], zero}, *qm[2] = {q->mv[qby * 4 + qbx], zero};
+  const int np = (pr[0] >= 0) + (pr[1] >= 0), nq = (qr[0] >= 0) + (qr[1] >= 0);
+  if (np != nq) return 1;
+  if (np == 1) {
+    const int lp = pr[0] >= 0 ? 0 : 1, lq = qr[0] >= 0 ? 0 : 1;
+    return pr[lp] != qr[lq] || db_far(pm[lp], qm[lq]);
+  }
+  if (!((pr[0] == qr[0] && pr[1] == qr[1]) || (pr[0] == qr[1] && pr[1] == qr[0]))) return 1;
+  if (pr[0] != pr[1]) {
+    if (pr[0] == qr[0]) return db_far(pm[0], qm[0]) || db_far(pm[1], qm[1]);
+    return db_far(pm[0], qm[1]) || db_far(pm[1], qm[0]);
+  }
+  return (db_far(pm[0], qm[0]) || db_far(pm[1], qm[1])) && (db_far(pm[0], qm[1]) || db_far(pm[1], qm[0]));
+}
+
+/* Every 4x4 block with the macroblock's first vector and every 8x8 block with its first reference
+ * (h264/deblock.rs:385-389, list 0; list 1 is uniformly unused). */
+static int db_uniform_motion(const ia_mbinfo *m) {
+  for (int i = 1; i < 4; i++)
+    if (m->ref_idx[i] != m->ref_idx[0] || m->ref_id[i] != m->ref_id[0]) return 0;
+  for (int i = 1; i < 16; i++)
+    if (m->mv[i][0] != m->mv[0][0] || m->mv[i][1] != m->mv[0][1]) return 0;
+  return 1;
+}
+
+/* Boundary strengths of the four 4-sample segments of edge e (0 = the macroblock edge) in direction
+ * dir (0: vertical edges, 1: horizontal); `o` is the macroblock across a macroblock edge
+ * (h264/deblock.rs:391-408). */
+static void db_strength(const ia_mbinfo *m, const ia_mbinfo *o, int dir, int e, uint8_t *bs) {
+  if (m->kind != IA_KIND_INTER || o->kind != IA_KIND_INTER) {
+    for (int i = 0; i < 4; i++) bs[i] = e == 0 ? 4 : 3;
+    return;
+  }
+  for (int i = 0; i < 4; i++) {
+    int qbx, qby, pbx, pby;
+    if (dir == 0) qbx = e, qby = i, pbx = e == 0 ? 3 : e - 1, pby = i;
+    else qbx = i, qby = e, pbx = i, pby = e == 0 ? 3 : e - 1;
+    bs[i] = db_coded(m, qbx, qby) || db_coded(o, pbx, pby) ? 2 : db_motion_differs(o, pbx, pby, m, qbx, qby) ? 1 : 0;
+  }
+}
+
+/* One sample line across an edge; q is the first sample past it, xs the step across it
+ * (h264/deblock.rs:279-334, 8-bit). */
+static void db_filter_line(uint8_t *pl, ptrdiff_t q, ptrdiff_t xs, int b, int tc0, int alpha, int beta, int chroma) {
+  const int p0 = pl[q - xs], q0 = pl[q], p1 = pl[q - 2 * xs], q1 = pl[q + xs];
+  if (abs(p0 - q0) >= alpha || abs(p1 - p0) >= beta || abs(q1 - q0) >= beta) return;
+  if (chroma) {
+    if (b < 4) {
+      const int tc = tc0 + 1;
+      const int delta = clampi((((q0 - p0) * 4) + (p1 - q1) + 4) >> 3, -tc, tc);
+      pl[q - xs] = (uint8_t)clampi(p0 + delta, 0, 255);
+      pl[q] = (uint8_t)clampi(q0 - delta, 0, 255);
+    } else {
+      pl[q - xs] = (uint8_t)((2 * p1 + p0 + q1 + 2) >> 2);
+      pl[q] = (uint8_t)((2 * q1 + q0 + p1 + 2) >> 2);
+    }
+    return;
+  }
+  const int p2 = pl[q - 3 * xs], q2 = pl[q + 2 * xs];
+  const int ap = abs(p2 - p0), aq = abs(q2 - q0);
+  if (b < 4) {
+    const int tc = tc0 + (ap < beta) + (aq < beta);
+    const int delta = clampi((((q0 - p0) * 4) + (p1 - q1) + 4) >> 3, -tc, tc);
+    pl[q - xs] = (uint8_t)clampi(p0 + delta, 0, 255);
+    pl[q] = (uint8_t)clampi(q0 - delta, 0, 255);
+    if (ap < beta) pl[q - 2 * xs] = (uint8_t)(p1 + clampi((p2 + ((p0 + q0 + 1) >> 1) - (2 * p1)) >> 1, -tc0, tc0));
+    if (aq < beta) pl[q + xs] = (uint8_t)(q1 + clampi((q2 + ((p0 + q0 + 1) >> 1) - (2 * q1)) >> 1, -tc0, tc0));
+    return;
+  }
+  const int small = abs(p0 - q0) < (alpha >> 2) + 2;
+  if (ap < beta && small) {
+    const int p3 = pl[q - 4 * xs];
+    pl[q - xs] = (uint8_t)((p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3);
+    pl[q - 2 * xs] = (uint8_t)((p2 + p1 + p0 + q0 + 2) >> 2);
+    pl[q - 3 * xs] = (uint8_t)((2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3);
+  } else {
+    pl[q - xs] = (uint8_t)((2 * p1 + p0 + q1 + 2) >> 2);
+  }
+  if (aq < beta && small) {
+    const int q3 = pl[q + 3 * xs];
+    pl[q] = (uint8_t)((p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3);
+    pl[q + xs] = (uint8_t)((p0 + q0 + q1 + q2 + 2) >> 2);
+    pl[q + 2 * xs] = (uint8_t)((2 * q3 + 3 * q2 + q1 + q0 + p0 + 4) >> 3);
+  } else {
+    pl[q] = (uint8_t)((2 * q1 + q0 + p1 + 2) >> 2);
+  }
+}
+
+/* An edge of segments bs[0..3] of `per_bs` lines each, starting at pos, lines ys apart
+ * (h264/deblock.rs:232-276 without I_PCM restoring). */
+static void db_filter_edge(uint8_t *pl, ptrdiff_t pos, ptrdiff_t xs, ptrdiff_t ys, int per_bs, const uint8_t *bs,
+                           int qpav, int chroma) {
+  const int ia = clampi(qpav, 0, 51), ib = clampi(qpav, 0, 51);
+  const int alpha = OR_ALPHA[ia], beta = OR_BETA[ib];
+  if (alpha == 0 || beta == 0) return;
+  for (int s = 0; s < 4; s++) {
+    if (bs[s] == 0) continue;
+    const int tc0 = bs[s] < 4 ? (OR_TC0[ia][bs[s]] > 0 ? OR_TC0[ia][bs[s]] : 0) : 0;
+    for (int k = s * per_bs; k < (s + 1) * per_bs; k++) db_filter_line(pl, pos + k * ys, xs, bs[s], tc0, alpha, beta, chroma);
+  }
+}
+
+/* Macroblock (mbx, mby): strengths once, then luma and chroma edges (h264/deblock.rs:608-695). */
+static void db_mb(uint8_t *y, uint8_t *cb, uint8_t *cr, const ia_mbinfo *mbs, int mb_w, int mbx, int mby) {
+  const ptrdiff_t ls = (ptrdiff_t)mb_w * 16, cs = (ptrdiff_t)mb_w * 8;
+  const int idx = mby * mb_w + mbx;
+  const ia_mbinfo *m = &mbs[idx];
+  if (m->slice == 0) return;
+  /* every slice filters across slice edges (disable_deblocking_filter_idc 0) */
+  const int left_ok = mbx > 0 && mbs[idx - 1].slice != 0, top_ok = mby > 0 && mbs[idx - mb_w].slice != 0;
+  const ia_mbinfo *edge_mb[2] = {mbx > 0 ? &mbs[idx - 1] : m, mby > 0 ? &mbs[idx - mb_w] : m};
+  uint8_t bs[2][4][4];
+  memset(bs, 0, sizeof bs);
+  int still = m->kind == IA_KIND_INTER && db_uniform_motion(m);
+  for (int i = 0; i < 16 && still; i++) still = m->nnz[i] == 0;
+  for (int dir = 0; dir < 2; dir++)
+    for (int e = 0; e < 4; e++) {
+      if (e > 0 && still) continue;
+      db_strength(m, e == 0 ? edge_mb[dir] : m, dir, e, bs[dir][e]);
+    }
+  const int x0 = mbx * 16, y0 = mby * 16;
+  for (int dir = 0; dir < 2; dir++)
+    for (int e = 0; e < 4; e++) {
+      if (e == 0 && !(dir == 0 ? left_ok : top_ok)) continue;
+      if ((m->t8x8 && (e & 1)) || !(bs[dir][e][0] | bs[dir][e][1] | bs[dir][e][2] | bs[dir][e][3])) continue;
+      const ia_mbinfo *o = e == 0 ? edge_mb[dir] : m;
+      const int qpav = (o->qp + m->qp + 1) >> 1;
+      if (dir == 0) db_filter_edge(y, y0 * ls + x0 + e * 4, 1, ls, 4, bs[dir][e], qpav, 0);
+      else db_filter_edge(y, (y0 + e * 4) * ls + x0, ls, 1, 4, bs[dir][e], qpav, 0);
+    }
+  /* chroma 4:2:0: vertical edges 0 and 4, horizontal edges 0 and 4 (luma edges 0 and 2) */
+  uint8_t *cpl[2] = {cb, cr};
+  for (int comp = 0; comp < 2; comp++)
+    for (int dir = 0; dir < 2; dir++)
+      for (int e = 0; e < 4; e += 2) {
+        if ((e == 0 && !(dir == 0 ? left_ok : top_ok)) || !(bs[dir][e][0] | bs[dir][e][1] | bs[dir][e][2] | bs[dir][e][3]))
+          continue;
+        const ia_mbinfo *o = e == 0 ? edge_mb[dir] : m;
+        const int qpav = (chroma_qp(o->qp, 0, 8) + chroma_qp(m->qp, 0, 8) + 1) >> 1;
+        const int cx0 = mbx * 8, cy0 = mby * 8;
+        if (dir == 0) db_filter_edge(cpl[comp], cy0 * cs + cx0 + e * 2, 1, cs, 2, bs[dir][e], qpav, 1);
+        else db_filter_edge(cpl[comp], (cy0 + e * 2) * cs + cx0, cs, 1, 2, bs[dir][e], qpav, 1);
+      }
+}
+
+void or_deblock_picture(uint8_t *y, uint8_t *cb, uint8_t *cr, const ia_mbinfo *mbs, int mb_w, int mb_h) {
+  for (int mby = 0; mby < mb_h; mby++)
+    for (int mbx = 0; mbx < mb_w; mbx++) db_mb(y, cb, cr, mbs, mb_w, mbx, mby);
 }
 
 /* Sizes of the ABI structs as this compiler lays them out, for the binding tests. */

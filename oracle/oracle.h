@@ -74,6 +74,12 @@ void or_predict_chroma(int mode, const int32_t *top8, const int32_t *left16, int
 void or_fullsearch(const uint8_t *src_y, const uint8_t *const *ref_y, int nref, int mb_w, int mb_h, int range,
                    ia_fsresult *out);
 
+/* The loop filter the encoder applies to each reconstruction before it becomes a reference
+ * (enc/mod.rs:653-654, h264/deblock.rs:410-421 and :608-695): frame macroblocks, 4:2:0, 8-bit,
+ * filtering across slice edges with zero offsets. In place on padded planes y (stride mb_w*16),
+ * cb and cr (stride mb_w*8), from the picture's macroblock records. */
+void or_deblock_picture(uint8_t *y, uint8_t *cb, uint8_t *cr, const ia_mbinfo *mbs, int mb_w, int mb_h);
+
 /* sizeof of ia_config, ia_mbinfo, ia_mbcode, ia_mbstats and ia_fsresult, for the binding tests. */
 void or_abi_sizes(int32_t *out);
 

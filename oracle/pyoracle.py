@@ -53,6 +53,7 @@ def lib():
             "or_predict_chroma": (None, [i, vp, vp, C.c_int32, i, i, vp, i, i]),
             "or_fullsearch": (None, [vp, vp, i, i, i, i, vp]),
             "or_abi_sizes": (None, [vp]),
+            "or_deblock_picture": (None, [vp, vp, vp, vp, i, i]),
             "or_analyse_picture": (i, [C.POINTER(abi.Config), vp, vp, vp, i, i, vp, vp, vp, vp, i, vp, vp, vp, vp, vp,
                                        vp, vp]),
             "or_analyse_picture_mt": (i, [C.POINTER(abi.Config), vp, vp, vp, i, i, vp, vp, vp, vp, i, vp, vp, vp, vp,
@@ -75,6 +76,20 @@ def halfpel_build(luma16, bd=8):
     out = np.empty((4, ph + 96, pw + 96), dtype=np.uint16)
     lib().or_halfpel_build(P(luma16), pw, ph, bd, P(out))
     return out
+
+
+def deblock_picture(y, cb, cr, mbs, mb_w, mb_h):
+    """The encoder's loop filter (h264/deblock.rs:608-695) on copies of padded 8-bit planes y
+    (mb_h*16 x mb_w*16), cb and cr (mb_h*8 x mb_w*8) with the picture's macroblock records; returns the
+    filtered (y, cb, cr)."""
+    y = np.ascontiguousarray(y, dtype=np.uint8).copy()
+    cb = np.ascontiguousarray(cb, dtype=np.uint8).copy()
+    cr = np.ascontiguousarray(cr, dtype=np.uint8).copy()
+    assert y.shape == (mb_h * 16, mb_w * 16) and cb.shape == cr.shape == (mb_h * 8, mb_w * 8)
+    mbs = np.ascontiguousarray(mbs, dtype=abi.MBINFO_DTYPE)
+    assert mbs.size == mb_w * mb_h
+    lib().or_deblock_picture(P(y), P(cb), P(cr), P(mbs), mb_w, mb_h)
+    return y, cb, cr
 
 
 def halfpel_predict(planes, pw, ph, x, y, mv, w, h):
