@@ -94,12 +94,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (the counters' scale for the traffic figures)")
-    ap.add_argument("--config", choices=["learned", "plain", "batched", "learned-batched"], default="learned",
+    ap.add_argument("--config", choices=["learned", "plain", "batched", "learned-batched", "deblock"], default="learned",
                     help="learned: the learned proposal vs the exhaustive search at 1080p (the metric); "
                          "plain: the exhaustive search + exact analysis at 1080p against the CPU oracle; "
                          "batched: many streams of mixed sizes through the batched analysis; "
                          "learned-batched: 2160p streams with large motion and scene cuts, the learned "
-                         "proposal inside the batched step vs the exhaustive search")
+                         "proposal inside the batched step vs the exhaustive search; "
+                         "deblock: the loop filter and reference promotion of a 1080p P picture against the "
+                         "CPU oracle's filter")
     ap.add_argument("--slices", type=int, default=0, help="plain: slices per picture (0: one per macroblock row)")
     ap.add_argument("--size", choices=["1080p", "720p"], default="1080p", help="plain: picture size")
     ap.add_argument("--search", choices=["full", "diamond"], default="full", help="plain: motion search")
@@ -117,6 +119,8 @@ def main():
         return bench_learned_batched(args)
     if args.config == "plain":
         return bench_plain(args)
+    if args.config == "deblock":
+        return bench_deblock(args)
 
     import numpy as np
     import torch
@@ -380,6 +384,100 @@ def bench_plain(args):
                                f"on, 8x8 transform, CABAC, {slices} slices",
                    "macroblocks_per_step": mbs},
         "bitexact_vs_oracle": exact,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def bench_deblock(args):
+    """The encoder's step after the analysis of a picture, on the GPU: the loop filter of the 1080p
+    workload's P picture (its own reconstruction and records, device-resident) and its promotion to a
+    reference (copies, filter, half-sample planes), one promotion per step; against the CPU oracle's
+    loop filter of the same picture on one host thread, bit-checked."""
+    import numpy as np
+    import torch
+
+    from ironav_amd import abi, clipgen
+    from ironav_amd.h264a import Analyser, Picture
+
+    width, height, nref, qp = 1920, 1080, 3, 28
+    spec = clipgen.workload_1080p()
+    spec.frames = 3
+    pics = [Picture(*clipgen.pad_planes(*f)) for f in clipgen.make_clip(spec)]
+    torch.cuda.init()
+    an = Analyser(width, height, nrefs=nref, slices=68, search_mode=abi.SEARCH_FULL, search_range=32)
+    order = []
+    results = []
+    for t, pic in enumerate(pics):
+        kind = abi.PIC_IDR if t == 0 else abi.PIC_P
+        slots = order[:nref]
+        if kind == abi.PIC_P:
+            an.upload(pic)
+            an.motion_search(slots)
+        results.append(an.analyse(pic, qp - 3 if t == 0 else qp, kind, slots))
+        slot = len(order)
+        an.promote_reference(slot)
+        order.insert(0, slot)
+    res = results[-1]  # the last P picture's analysis: each step promotes it into the spare slot
+    spare = nref - 1
+
+    def timed(steps):
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            an.promote_reference(spare)
+        return (time.perf_counter() - t0) * 1000.0 / steps
+
+    timed(args.warmup)
+    ms = timed(args.steps)
+    got, _ = an.read_reference(spare)
+    # the IDR picture (every edge of strength 3 or 4): its records and reconstruction analysed again
+    an.analyse(pics[0], qp - 3, abi.PIC_IDR, [])
+    timed(max(1, args.warmup // 2))
+    ms_idr = timed(max(1, args.steps // 2))
+    got_idr, _ = an.read_reference(spare)
+    mbs = an.mb_w * an.mb_h
+    print(f"[deblock] GPU promotion {ms:.3f} ms per P picture, {ms_idr:.3f} ms per IDR picture; CPU oracle next",
+          file=sys.stderr, flush=True)
+    cpu, exact, idr = None, None, {"ms_per_promotion": round(ms_idr, 4)}
+    if not args.no_cpu:
+        from oracle import pyoracle
+
+        pyoracle.build()
+        reps = 3
+
+        def oracle_filter(r):
+            t1 = time.perf_counter()
+            for _ in range(reps):
+                out = pyoracle.deblock_picture(r.recon.y, r.recon.cb, r.recon.cr, r.mbs, an.mb_w, an.mb_h)
+            return out, (time.perf_counter() - t1) / reps
+
+        want, cpu_s = oracle_filter(res)
+        want_idr, cpu_idr_s = oracle_filter(results[0])
+        exact = all(np.array_equal(a, b) for a, b in zip(want, (got.y, got.cb, got.cr)))
+        exact = exact and all(np.array_equal(a, b) for a, b in zip(want_idr, (got_idr.y, got_idr.cb, got_idr.cr)))
+        cpu = {"value": round(mbs / cpu_s, 1), "unit": "MB/s", "cores": 1, "kind": "port",
+               "sample": f"the oracle's loop filter of the same P picture ({mbs} macroblocks), {reps} times, "
+                         f"{cpu_s * 1000:.1f} ms each, one thread"}
+        idr["cpu_oracle_ms"] = round(cpu_idr_s * 1000, 2)
+    out = {
+        "metric": "MB/s loop filter + reference promotion @1080p (P picture, device-resident), GPU vs CPU oracle",
+        "value": round(mbs / (ms / 1000.0), 1),
+        "unit": "MB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded generator clip 1920x1080 4:2:0 coded IPP by the analysis at QP 28; "
+                "the step filters the last P picture's reconstruction)",
+        "config": {"workload": "1920x1080 P picture: copies of its reconstruction and records to a reference "
+                               "slot, the loop filter, the half-sample planes (one synchronous call, "
+                               "host round trip included)", "macroblocks_per_step": mbs},
+        "bitexact_vs_oracle": exact,
+        "idr_picture": idr,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -259,9 +259,11 @@ typedef struct ia_frame {
   int nref;               /* P pictures: references used, 1..nrefs */
   int ref_slots[16];      /* P pictures: slots, newest first (as plist, enc/mod.rs:559) */
   int promote_slot;       /* -1, or the slot that receives this picture's reconstruction and records
-                             as a reference when the step is done (not deblocked) */
+                             as a reference when the step is done */
   const uint8_t *y, *cb, *cr;  /* the picture's padded planes in device memory, or null: the context's
                                   uploaded source (ia_upload_source) */
+  int promote_deblock;    /* non-zero: the promoted reference is loop-filtered first, as the encoder
+                             keeps its references (enc/mod.rs:653-654); 0: kept as reconstructed */
 } ia_frame;
 
 int ia_batch_create(ia_batch **out, ia_ctx *const *ctxs, int n, int use_graph);

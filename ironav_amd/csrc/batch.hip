@@ -10,8 +10,10 @@
 //      IA_SEARCH_LEARNED, the proposal over all their macroblocks (one persistent launch; the CNN in
 //      the same graph) and the exhaustive search of the batch's one fallback list;
 //   4. the analysis: a pool of workgroups taking (stream, slice) tasks from a queue, longest first;
-//   5. the promotion of each stream's reconstruction to a reference slot: its half-sample planes
-//      (one batched launch) and its chroma planes and macroblock records (one batched copy).
+//   5. the promotion of each stream's reconstruction to a reference slot: its planes and macroblock
+//      records copied (one batched copy; the luma to the context's scratch plane), the loop filter
+//      of the copies (one launch over every promoted stream's macroblock rows, when the frame asks
+//      for it, as the encoder filters its references), then the half-sample planes (one launch).
 // Every kernel reads its per-stream arguments from the device table, so one captured graph serves
 // every frame: the grids cover the largest case (all references of every stream) and the blocks of
 // a stream's unused references, or of a stream that promotes nothing, return at once.
@@ -75,11 +77,11 @@ struct ia_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   // the per-frame table, pinned on the host and mirrored on the device
-  size_t tab_bytes = 0, off_fs = 0, off_hp = 0, off_cp = 0, off_pp = 0;
+  size_t tab_bytes = 0, off_fs = 0, off_hp = 0, off_cp = 0, off_db = 0, off_pp = 0;
   uint8_t *h_tab = nullptr, *d_tab = nullptr;
   // fixed tables: analysis tasks, queue head, search launch members
   int *d_fixed = nullptr;
-  int ntasks = 0, workers = 0, hp_blocks = 0, cp_blocks = 0;
+  int ntasks = 0, workers = 0, hp_blocks = 0, cp_blocks = 0, db_blocks = 0;
   struct Group {
     int variant, max_range, nmem, blocks;
     size_t offs, sidx;  // word offsets into d_fixed
@@ -90,6 +92,7 @@ struct ia_batch {
   size_t prop_offs = 0, prop_sidx = 0;
   int prop_nmem = 0, prop_total = 0, fb_max_range = 0, fb_max_entries = 0;
   int *d_fb = nullptr;  // [0] count, [1..] entries picture << 20 | mb << 2 | ref
+  int *d_dberr = nullptr;  // set when a loop filter row stalled
   uint64_t wts_hash = 0;
   std::string err;
 
@@ -97,6 +100,7 @@ struct ia_batch {
   ia::FsArgs *fs_h() { return reinterpret_cast<ia::FsArgs *>(h_tab + off_fs); }
   ia::HpJob *hp_h() { return reinterpret_cast<ia::HpJob *>(h_tab + off_hp); }
   CopyJob *cp_h() { return reinterpret_cast<CopyJob *>(h_tab + off_cp); }
+  ia::DbJob *db_h() { return reinterpret_cast<ia::DbJob *>(h_tab + off_db); }
   ia::PropArgs *pp_h() { return reinterpret_cast<ia::PropArgs *>(h_tab + off_pp); }
 };
 
@@ -131,11 +135,13 @@ int enqueue(ia_batch *b) {
   }
   IA_BHIP(b, ia::launch_analyse_batch(reinterpret_cast<const ia::AnParams *>(b->d_tab), fixed + b->tasks, b->ntasks,
                                       fixed + b->head, b->workers, st));
+  hipLaunchKernelGGL(copy_batch_kernel, dim3(b->cp_blocks), dim3(CP_THREADS), 0, st,
+                     reinterpret_cast<const CopyJob *>(b->d_tab + b->off_cp), 4 * b->n);
+  IA_BHIP(b, hipGetLastError());
+  IA_BHIP(b, ia::launch_deblock_batch(reinterpret_cast<const ia::DbJob *>(b->d_tab + b->off_db), b->n, b->db_blocks,
+                                      b->d_dberr, st));
   IA_BHIP(b, ia::launch_halfpel_batch(reinterpret_cast<const ia::HpJob *>(b->d_tab + b->off_hp), b->n, b->hp_blocks,
                                       st));
-  hipLaunchKernelGGL(copy_batch_kernel, dim3(b->cp_blocks), dim3(CP_THREADS), 0, st,
-                     reinterpret_cast<const CopyJob *>(b->d_tab + b->off_cp), 3 * b->n);
-  IA_BHIP(b, hipGetLastError());
   return IA_OK;
 }
 
@@ -163,7 +169,8 @@ int ia_batch_create(ia_batch **out, ia_ctx *const *ctxs, int n, int use_graph) {
   b->off_fs = align16((size_t)n * sizeof(ia::AnParams));
   b->off_hp = b->off_fs + align16((size_t)n * sizeof(ia::FsArgs));
   b->off_cp = b->off_hp + align16((size_t)n * sizeof(ia::HpJob));
-  b->off_pp = b->off_cp + align16((size_t)3 * n * sizeof(CopyJob));
+  b->off_db = b->off_cp + align16((size_t)4 * n * sizeof(CopyJob));
+  b->off_pp = b->off_db + align16((size_t)n * sizeof(ia::DbJob));
   b->tab_bytes = b->off_pp + align16((size_t)n * sizeof(ia::PropArgs));
   // analysis tasks, longest slice first (ties by stream, then slice)
   struct Task {
@@ -241,16 +248,21 @@ int ia_batch_create(ia_batch **out, ia_ctx *const *ctxs, int n, int use_graph) {
     b->prop_sidx = fixed.size();
     fixed.insert(fixed.end(), sidx.begin(), sidx.end());
   }
-  // promotion launches: tiles of each stream's half-sample planes, chunks of its copies
-  int hp_blocks = 0, cp_blocks = 0;
+  // promotion launches: strips of each stream's half-sample planes, chunks of its copies, rows of
+  // its loop filter
+  int hp_blocks = 0, cp_blocks = 0, db_blocks = 0;
   for (int s = 0; s < n; s++) {
     const Geom &g = ctxs[s]->g;
     hp_blocks += ((g.sw + ia::HP_TW - 1) / ia::HP_TW) * ((g.sh + ia::HP_TH - 1) / ia::HP_TH);
-    const long long clen = (long long)g.cw * g.ch, mlen = (long long)g.mb_w * g.mb_h * sizeof(ia_mbinfo);
-    cp_blocks += 2 * (int)((clen + CP_CHUNK - 1) / CP_CHUNK) + (int)((mlen + CP_CHUNK - 1) / CP_CHUNK);
+    const long long ylen = (long long)g.pw * g.ph, clen = (long long)g.cw * g.ch,
+                    mlen = (long long)g.mb_w * g.mb_h * sizeof(ia_mbinfo);
+    cp_blocks += (int)((ylen + CP_CHUNK - 1) / CP_CHUNK) + 2 * (int)((clen + CP_CHUNK - 1) / CP_CHUNK) +
+                 (int)((mlen + CP_CHUNK - 1) / CP_CHUNK);
+    db_blocks += g.mb_h;
   }
   b->hp_blocks = hp_blocks;
   b->cp_blocks = cp_blocks;
+  b->db_blocks = db_blocks;
   const int workers = ia::analyse_batch_workers();
   b->workers = std::max(1, std::min(workers > 0 ? workers : 1024, b->ntasks));
   bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
@@ -258,6 +270,8 @@ int ia_batch_create(ia_batch **out, ia_ctx *const *ctxs, int n, int use_graph) {
             hipMalloc((void **)&b->d_tab, b->tab_bytes) == hipSuccess &&
             hipMalloc((void **)&b->d_fixed, fixed.size() * sizeof(int)) == hipSuccess &&
             hipMalloc((void **)&b->d_fb, (1 + (size_t)b->fb_max_entries) * sizeof(int)) == hipSuccess &&
+            hipMalloc((void **)&b->d_dberr, sizeof(int)) == hipSuccess &&
+            hipMemset(b->d_dberr, 0, sizeof(int)) == hipSuccess &&
             hipMemcpy(b->d_fixed, fixed.data(), fixed.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
   if (!ok) {
     ia_batch_destroy(b);
@@ -277,6 +291,7 @@ void ia_batch_destroy(ia_batch *b) {
   (void)hipFree(b->d_tab);
   (void)hipFree(b->d_fixed);
   (void)hipFree(b->d_fb);
+  (void)hipFree(b->d_dberr);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
@@ -326,27 +341,41 @@ int ia_batch_step(ia_batch *b, const ia_frame *frames) {
       fa.mb_h = c->g.mb_h;
     }
     const Geom &g = c->g;
+    IaSlot *ps = f.promote_slot >= 0 ? &c->slots[f.promote_slot] : nullptr;
+    // the luma goes through the context's scratch plane (filtered there), the rest to the slot
     ia::HpJob &hj = b->hp_h()[s];
-    hj.luma = c->rec[0];
-    hj.planes = f.promote_slot >= 0 ? c->slots[f.promote_slot].hp : nullptr;
+    hj.luma = c->tmp_luma;
+    hj.planes = ps ? ps->hp : nullptr;
     hj.pw = g.pw;
     hj.ph = g.ph;
     hj.tiles_x = (g.sw + ia::HP_TW - 1) / ia::HP_TW;
-    CopyJob *cj = b->cp_h() + 3 * s;
-    const long long clen = (long long)g.cw * g.ch, mlen = (long long)g.mb_w * g.mb_h * sizeof(ia_mbinfo);
-    IaSlot *ps = f.promote_slot >= 0 ? &c->slots[f.promote_slot] : nullptr;
-    cj[0] = {ps ? ps->cb : nullptr, c->rec[1], clen, 0, 0};
-    cj[1] = {ps ? ps->cr : nullptr, c->rec[2], clen, 0, 0};
-    cj[2] = {ps ? reinterpret_cast<uint8_t *>(ps->mbs) : nullptr, reinterpret_cast<const uint8_t *>(c->mbs), mlen, 0, 0};
+    CopyJob *cj = b->cp_h() + 4 * s;
+    const long long ylen = (long long)g.pw * g.ph, clen = (long long)g.cw * g.ch,
+                    mlen = (long long)g.mb_w * g.mb_h * sizeof(ia_mbinfo);
+    cj[0] = {ps ? c->tmp_luma : nullptr, c->rec[0], ylen, 0, 0};
+    cj[1] = {ps ? ps->cb : nullptr, c->rec[1], clen, 0, 0};
+    cj[2] = {ps ? ps->cr : nullptr, c->rec[2], clen, 0, 0};
+    cj[3] = {ps ? reinterpret_cast<uint8_t *>(ps->mbs) : nullptr, reinterpret_cast<const uint8_t *>(c->mbs), mlen, 0, 0};
+    ia::DbJob &dj = b->db_h()[s];
+    const bool filt = ps && f.promote_deblock;
+    dj.y = filt ? c->tmp_luma : nullptr;
+    dj.cb = filt ? ps->cb : nullptr;
+    dj.cr = filt ? ps->cr : nullptr;
+    dj.mbs = c->mbs;
+    dj.prog = c->db + 1;
+    dj.mb_w = g.mb_w;
+    dj.mb_h = g.mb_h;
   }
   // first blocks of the promotion jobs (the same every step)
-  int hfirst = 0, cfirst = 0;
+  int hfirst = 0, cfirst = 0, dfirst = 0;
   for (int s = 0; s < b->n; s++) {
     const Geom &g = b->ctx[s]->g;
     b->hp_h()[s].first = hfirst;
     hfirst += ((g.sw + ia::HP_TW - 1) / ia::HP_TW) * ((g.sh + ia::HP_TH - 1) / ia::HP_TH);
-    for (int k = 0; k < 3; k++) {
-      CopyJob &j = b->cp_h()[3 * s + k];
+    b->db_h()[s].first = dfirst;
+    dfirst += g.mb_h;
+    for (int k = 0; k < 4; k++) {
+      CopyJob &j = b->cp_h()[4 * s + k];
       j.first = cfirst;
       cfirst += (int)((j.bytes + CP_CHUNK - 1) / CP_CHUNK);
     }
@@ -401,6 +430,12 @@ int ia_batch_workers(void) { return ia::analyse_batch_workers(); }
 int ia_batch_sync(ia_batch *b) {
   if (!b) return IA_EINVAL;
   IA_BHIP(b, hipStreamSynchronize(b->stream));
+  int stalled = 0;
+  IA_BHIP(b, hipMemcpy(&stalled, b->d_dberr, sizeof(int), hipMemcpyDeviceToHost));
+  if (stalled) {
+    IA_BHIP(b, hipMemset(b->d_dberr, 0, sizeof(int)));
+    return bfail(b, IA_EDEVICE, "the loop filter's rows stalled");
+  }
   return IA_OK;
 }
 

@@ -40,35 +40,27 @@ __device__ __forceinline__ bool coded(const ia_mbinfo &m, int bx, int by) {
   return m.nnz[by * 4 + bx] != 0;
 }
 
-__device__ __forceinline__ bool far(const int16_t *a, const int16_t *b) {
-  return iabs(a[0] - b[0]) >= 4 || iabs(a[1] - b[1]) >= 4;
-}
+// Vectors a whole sample or more apart (frame macroblocks: the vertical limit is 4 quarter samples).
+__device__ __forceinline__ bool far(int ax, int ay, int bx, int by) { return iabs(ax - bx) >= 4 || iabs(ay - by) >= 4; }
 
 // Different references, a different number of vectors, or vectors a whole sample apart
-// (h264/deblock.rs:352-383). List 1 is unused in P pictures: reference -1, vector 0.
-__device__ bool motion_differs(const ia_mbinfo &p, int pbx, int pby, const ia_mbinfo &q, int qbx, int qby) {
-  const int16_t zero[2] = {0, 0};
+// (h264/deblock.rs:347-383). List 1 is unused in P pictures: reference -1, vector 0.
+__device__ __forceinline__ bool motion_differs(const ia_mbinfo &p, int pbx, int pby, const ia_mbinfo &q, int qbx,
+                                               int qby) {
   const int p8 = (pby >> 1) * 2 + (pbx >> 1), q8 = (qby >> 1) * 2 + (qbx >> 1);
   const int pr0 = p.ref_id[p8], qr0 = q.ref_id[q8];
-  const int16_t *pm0 = p.mv[pby * 4 + pbx], *qm0 = q.mv[qby * 4 + qbx];
+  const int px = p.mv[pby * 4 + pbx][0], py = p.mv[pby * 4 + pbx][1];
+  const int qx = q.mv[qby * 4 + qbx][0], qy = q.mv[qby * 4 + qbx][1];
   const int np = pr0 >= 0, nq = qr0 >= 0;
   if (np != nq) return true;
-  if (np == 1) return pr0 != qr0 || far(pm0, qm0);
-  // no vector on either side (pr0 == qr0 == -1 with list 1 unused): the reference's last case
-  return (far(pm0, qm0) || far(zero, zero)) && (far(pm0, zero) || far(zero, qm0));
-}
-
-__device__ bool uniform_motion(const ia_mbinfo &m) {
-  for (int i = 1; i < 4; i++)
-    if (m.ref_idx[i] != m.ref_idx[0] || m.ref_id[i] != m.ref_id[0]) return false;
-  for (int i = 1; i < 16; i++)
-    if (m.mv[i][0] != m.mv[0][0] || m.mv[i][1] != m.mv[0][1]) return false;
-  return true;
+  if (np == 1) return pr0 != qr0 || far(px, py, qx, qy);
+  // no vector on either side (list 1's vectors are 0): the reference's last case
+  return far(px, py, qx, qy) && (far(px, py, 0, 0) || far(0, 0, qx, qy));
 }
 
 // Boundary strength of segment i of edge e (0: the macroblock edge) in direction dir (0: vertical
 // edges) of macroblock m; o is the macroblock across a macroblock edge (h264/deblock.rs:391-408).
-__device__ int strength(const ia_mbinfo &m, const ia_mbinfo &o, int dir, int e, int i) {
+__device__ __forceinline__ int strength(const ia_mbinfo &m, const ia_mbinfo &o, int dir, int e, int i) {
   if (m.kind != IA_KIND_INTER || o.kind != IA_KIND_INTER) return e == 0 ? 4 : 3;
   const int qbx = dir == 0 ? e : i, qby = dir == 0 ? i : e;
   const int pbx = dir == 0 ? (e == 0 ? 3 : e - 1) : i, pby = dir == 0 ? i : (e == 0 ? 3 : e - 1);
@@ -76,23 +68,9 @@ __device__ int strength(const ia_mbinfo &m, const ia_mbinfo &o, int dir, int e, 
   return motion_differs(o, pbx, pby, m, qbx, qby) ? 1 : 0;
 }
 
-// alpha, beta and tc0 of strength b at average QP qpav (offsets 0: indexA = indexB = qpav).
 struct Thr {
   int alpha, beta, tc0;
 };
-__device__ __forceinline__ Thr thresholds(int qpav, int b) {
-  const int ia = clip3(0, 51, qpav);
-  Thr t;
-  t.alpha = IA_ALPHA[ia];
-  t.beta = IA_BETA[ia];
-  t.tc0 = b > 0 && b < 4 ? max((int)IA_TC0[ia][b], 0) : 0;
-  return t;
-}
-
-__device__ __forceinline__ int chroma_qp(int qp) {
-  const int q = clip3(0, 51, qp);
-  return q < 30 ? q : IA_CHROMA_QP[q - 30];
-}
 
 // One luma sample line across an edge, p3..p0 | q0..q3, strength b > 0 (h264/deblock.rs:279-334).
 __device__ __forceinline__ void filter_luma(int &p3, int &p2, int &p1, int &p0, int &q0, int &q1, int &q2, int &q3,
@@ -159,12 +137,31 @@ constexpr int LX = 12, CX = 4;
 struct DbLds {
   __attribute__((aligned(16))) uint8_t y[20][32];
   __attribute__((aligned(16))) uint8_t c[2][10][16];
-  uint8_t bs[32];  // strengths [dir][e][segment]
+  uint8_t bs[32];     // strengths [dir][e][segment]
+  ia_mbinfo rm[2];    // records of the row's macroblocks x (at x & 1) and x - 1
+  ia_mbinfo rt[2];    // record of the macroblock above x (at x & 1)
+  uint8_t alpha[52], beta[52], qpc[52];  // the filter's tables (and chroma QP per QP), copied once
+  int8_t tc0[52][4];
 };
+// alpha, beta and tc0 of strength b at average QP qpav (offsets 0: indexA = indexB = qpav), from the
+// LDS copies of the tables.
+__device__ __forceinline__ Thr thresholds(const DbLds &L, int qpav, int b) {
+  const int ia = clip3(0, 51, qpav);
+  Thr t;
+  t.alpha = L.alpha[ia];
+  t.beta = L.beta[ia];
+  t.tc0 = b > 0 && b < 4 ? max((int)L.tc0[ia][b], 0) : 0;
+  return t;
+}
+
+__device__ __forceinline__ int chroma_qp(const DbLds &L, int qp) { return L.qpc[clip3(0, 51, qp)]; }
+
+constexpr int REC_DWORDS = sizeof(ia_mbinfo) / 4;
+static_assert(sizeof(ia_mbinfo) % 4 == 0 && REC_DWORDS <= 32, "a record is read as dwords by lanes 0-31 / 32-63");
 
 // Macroblock row `row` of one picture (one wave). prog[r]: macroblocks of row r finished (the
 // counters start at 0 and are 0 again when the launch ends); err: set when a row waited too long.
-__device__ void deblock_row(DbLds &L, uint8_t *__restrict__ py, uint8_t *__restrict__ pcb, uint8_t *__restrict__ pcr,
+__device__ __forceinline__ void deblock_row(DbLds &L, uint8_t *__restrict__ py, uint8_t *__restrict__ pcb, uint8_t *__restrict__ pcr,
                             const ia_mbinfo *__restrict__ mbs, int mb_w, int mb_h, int row, int *prog, int *err) {
   const int lane = threadIdx.x;
   const int ls = mb_w * 16, cs = mb_w * 8;
@@ -175,35 +172,68 @@ __device__ void deblock_row(DbLds &L, uint8_t *__restrict__ py, uint8_t *__restr
   const int comp = lane < 24 ? 0 : 1;
   int left[4] = {0, 0, 0, 0};                // the 4 samples left of the macroblock on the lane's line
   int seen = 0;                              // last progress of the row above this wave read
+  // A macroblock's records and own samples are not changed by the filter: the wave fetches those of
+  // macroblock x+1 while it filters x (records into LDS, samples into registers).
+  const uint32_t *grec = reinterpret_cast<const uint32_t *>(mbs);
+  // a lane's dword of the records of macroblock x (lanes 0-31) or of the one above it (32-63)
+  auto fetch_record = [&](int x) -> uint32_t {
+    const int k = lane & 31;
+    if (k >= REC_DWORDS || (lane >= 32 && row == 0)) return 0u;
+    return grec[(size_t)((lane < 32 ? row : row - 1) * mb_w + x) * REC_DWORDS + k];
+  };
+  auto store_record = [&](int x, uint32_t v) {
+    const int k = lane & 31;
+    if (k < REC_DWORDS) reinterpret_cast<uint32_t *>(lane < 32 ? &L.rm[x & 1] : &L.rt[x & 1])[k] = v;
+  };
+  auto fetch_samples = [&](int x, uint4 &v) {
+    if (luma) v = *reinterpret_cast<const uint4 *>(py + (size_t)(y0 + lane) * ls + x * 16);
+    else if (chroma) {
+      const uint2 c = *reinterpret_cast<const uint2 *>(cpl + (size_t)(cy0 + cl) * cs + x * 8);
+      v.x = c.x;
+      v.y = c.y;
+    }
+  };
+  for (int q = lane; q < 52; q += 64) {
+    L.alpha[q] = IA_ALPHA[q];
+    L.beta[q] = IA_BETA[q];
+    L.qpc[q] = (uint8_t)(q < 30 ? q : IA_CHROMA_QP[q - 30]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) L.tc0[q][k] = IA_TC0[q][k];
+  }
+  uint4 nv = make_uint4(0, 0, 0, 0);
+  uint32_t nrec = fetch_record(0);
+  fetch_samples(0, nv);
+  store_record(0, nrec);
+  __syncthreads();
   for (int x = 0; x < mb_w; x++) {
     const int x0 = x * 16, cx0 = x * 8;
-    // the macroblock's own samples: untouched by the filtering of any earlier macroblock
+    // the macroblock's own samples, fetched during the previous macroblock
     int s[20];  // luma lane: columns x0-4 .. x0+15 of row y0+lane; chroma lane: columns cx0-4 .. cx0+7
-    if (luma) {
-      const uint4 v = *reinterpret_cast<const uint4 *>(py + (size_t)(y0 + lane) * ls + x0);
-      unpack4(v.x, s + 4);
-      unpack4(v.y, s + 8);
-      unpack4(v.z, s + 12);
-      unpack4(v.w, s + 16);
-    } else if (chroma) {
-      const uint2 v = *reinterpret_cast<const uint2 *>(cpl + (size_t)(cy0 + cl) * cs + cx0);
-      unpack4(v.x, s + 4);
-      unpack4(v.y, s + 8);
-    }
+    unpack4(nv.x, s + 4);
+    unpack4(nv.y, s + 8);
+    unpack4(nv.z, s + 12);
+    unpack4(nv.w, s + 16);
 #pragma unroll
     for (int k = 0; k < 4; k++) s[k] = left[k];
-    const int idx = row * mb_w + x;
-    const ia_mbinfo &m = mbs[idx];
+    if (x + 1 < mb_w) {
+      nrec = fetch_record(x + 1);
+      fetch_samples(x + 1, nv);
+    }
+    const ia_mbinfo &m = L.rm[x & 1];
+    const ia_mbinfo &ml = x > 0 ? L.rm[(x - 1) & 1] : m;
+    const ia_mbinfo &mt = row > 0 ? L.rt[x & 1] : m;
     const bool live = m.slice != 0;
-    const bool left_ok = live && x > 0 && mbs[idx - 1].slice != 0;
-    const bool top_ok = live && row > 0 && mbs[idx - mb_w].slice != 0;
-    const ia_mbinfo &ml = x > 0 ? mbs[idx - 1] : m;
-    const ia_mbinfo &mt = row > 0 ? mbs[idx - mb_w] : m;
+    const bool left_ok = live && x > 0 && ml.slice != 0;
+    const bool top_ok = live && row > 0 && mt.slice != 0;
     // boundary strengths, one per lane 0-31: dir = lane >> 4, edge e = (lane >> 2) & 3, segment lane & 3
+    // an inter macroblock that moves as a whole and has no coefficients leaves its inner edges
+    // unfiltered (h264/deblock.rs:628-634, :385-389): lanes 0-15 check a 4x4 block each, 16-19 an 8x8 block
+    bool same = true;
+    if (lane < 16) same = m.nnz[lane] == 0 && m.mv[lane][0] == m.mv[0][0] && m.mv[lane][1] == m.mv[0][1];
+    else if (lane < 20) same = m.ref_idx[lane - 16] == m.ref_idx[0] && m.ref_id[lane - 16] == m.ref_id[0];
+    const bool still = m.kind == IA_KIND_INTER && __all(same);
     if (lane < 32 && live) {
       const int dir = lane >> 4, e = (lane >> 2) & 3, i = lane & 3;
-      bool still = m.kind == IA_KIND_INTER && uniform_motion(m);
-      for (int k = 0; k < 16 && still; k++) still = m.nnz[k] == 0;
       L.bs[lane] = (uint8_t)(e > 0 && still ? 0 : strength(m, e == 0 ? (dir == 0 ? ml : mt) : m, dir, e, i));
     }
     // wait for macroblock x+1 of the row above (the last one at the row's end); after a wait that
@@ -249,19 +279,19 @@ __device__ void deblock_row(DbLds &L, uint8_t *__restrict__ py, uint8_t *__restr
           if ((e == 0 && !left_ok) || (m.t8x8 && (e & 1))) continue;
           const int b = L.bs[e * 4 + seg];
           if (b == 0) continue;
-          const Thr t = thresholds(e == 0 ? qpl : m.qp, b);
+          const Thr t = thresholds(L, e == 0 ? qpl : m.qp, b);
           const int q = 4 + 4 * e;
           filter_luma(s[q - 4], s[q - 3], s[q - 2], s[q - 1], s[q], s[q + 1], s[q + 2], s[q + 3], b, t);
         }
       } else if (chroma) {
         const int seg = cl >> 1;
-        const int qpc = (chroma_qp(ml.qp) + chroma_qp(m.qp) + 1) >> 1, qpm = chroma_qp(m.qp);
+        const int qpc = (chroma_qp(L, ml.qp) + chroma_qp(L, m.qp) + 1) >> 1, qpm = chroma_qp(L, m.qp);
 #pragma unroll
         for (int e = 0; e < 4; e += 2) {
           if (e == 0 && !left_ok) continue;
           const int b = L.bs[e * 4 + seg];
           if (b == 0) continue;
-          const Thr t = thresholds(e == 0 ? qpc : qpm, b);
+          const Thr t = thresholds(L, e == 0 ? qpc : qpm, b);
           const int q = 4 + 2 * e;
           filter_chroma(s[q - 2], s[q - 1], s[q], s[q + 1], b, t);
         }
@@ -289,7 +319,7 @@ __device__ void deblock_row(DbLds &L, uint8_t *__restrict__ py, uint8_t *__restr
           if ((e == 0 && !top_ok) || (m.t8x8 && (e & 1))) continue;
           const int b = L.bs[16 + e * 4 + seg];
           if (b == 0) continue;
-          const Thr t = thresholds(e == 0 ? qpt : m.qp, b);
+          const Thr t = thresholds(L, e == 0 ? qpt : m.qp, b);
           const int q = 4 + 4 * e;
           filter_luma(v[q - 4], v[q - 3], v[q - 2], v[q - 1], v[q], v[q + 1], v[q + 2], v[q + 3], b, t);
         }
@@ -300,13 +330,13 @@ __device__ void deblock_row(DbLds &L, uint8_t *__restrict__ py, uint8_t *__restr
 #pragma unroll
         for (int r = 0; r < 10; r++) v[r] = L.c[comp][r][CX + 4 + cl];
         const int seg = cl >> 1;
-        const int qpc = (chroma_qp(mt.qp) + chroma_qp(m.qp) + 1) >> 1, qpm = chroma_qp(m.qp);
+        const int qpc = (chroma_qp(L, mt.qp) + chroma_qp(L, m.qp) + 1) >> 1, qpm = chroma_qp(L, m.qp);
 #pragma unroll
         for (int e = 0; e < 4; e += 2) {
           if (e == 0 && !top_ok) continue;
           const int b = L.bs[16 + e * 4 + seg];
           if (b == 0) continue;
-          const Thr t = thresholds(e == 0 ? qpc : qpm, b);
+          const Thr t = thresholds(L, e == 0 ? qpc : qpm, b);
           const int q = 2 + 2 * e;
           filter_chroma(v[q - 2], v[q - 1], v[q], v[q + 1], b, t);
         }
@@ -339,6 +369,9 @@ __device__ void deblock_row(DbLds &L, uint8_t *__restrict__ py, uint8_t *__restr
     }
     __syncthreads();  // every lane's stores issued before the progress is published
     if (lane == 0) __hip_atomic_store(prog + row, x + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    // macroblock x+1's records replace x-1's (read no more)
+    if (x + 1 < mb_w) store_record(x + 1, nrec);
+    __syncthreads();
   }
   // the counter of the row above is read no more: clear it for the next launch (and the last row its own)
   if (lane == 0) {

@@ -287,6 +287,7 @@ class Batch:
 
     def step(self, frames):
         """frames[i] for analyser i: dict(qp, kind, refs=[slots newest first], promote=slot or -1,
+        deblock=True (the promoted reference loop-filtered first, as the encoder keeps it),
         planes=(y, cb, cr) device addresses or None for the analyser's uploaded source)."""
         if len(frames) != len(self.analysers):
             raise ValueError("one frame per stream")
@@ -297,6 +298,7 @@ class Batch:
             for i, r in enumerate(refs):
                 f.ref_slots[i] = r
             f.promote_slot = d.get("promote", -1)
+            f.promote_deblock = 1 if d.get("deblock", True) else 0
             planes = d.get("planes")
             if planes is not None:
                 f.y, f.cb, f.cr = planes

@@ -1,7 +1,7 @@
 """The batched path on the GPU: streams of different sizes and settings through one set of launches
 per frame step (eager and as a replayed hipGraph) give, per stream and per frame, exactly what the
 single-context path gives (itself bit-exact with the oracle), including the on-device promotion of
-each reconstruction to a reference."""
+each reconstruction to a reference, loop-filtered (as the encoder keeps its references) or not."""
 import numpy as np
 import pytest
 
@@ -32,7 +32,7 @@ def analyser(w, h, rng, nrefs, slices, cabac, t8):
                     search_mode=abi.SEARCH_FULL, search_range=rng)
 
 
-def single_runs():
+def single_runs(deblock=True):
     out = []
     for (w, h, seed, rng, nrefs, slices, cabac, t8) in STREAMS:
         pics = clip(w, h, seed)
@@ -41,13 +41,13 @@ def single_runs():
             for t in range(FRAMES):
                 qp, kind, refs, promote = schedule(nrefs, t)
                 r = an.analyse(pics[t], qp, kind, refs)
-                an.set_reference(promote, r.recon, r.mbs)
+                an.promote_reference(promote, deblock=deblock)
                 res.append(r)
         out.append(res)
     return out
 
 
-def batch_runs(use_graph, device_planes=False):
+def batch_runs(use_graph, device_planes=False, deblock=True):
     import torch
 
     clips = [clip(w, h, seed) for (w, h, seed, *_rest) in STREAMS]
@@ -59,7 +59,7 @@ def batch_runs(use_graph, device_planes=False):
             frames = []
             for i, an in enumerate(ans):
                 qp, kind, refs, promote = schedule(STREAMS[i][4], t)
-                d = dict(qp=qp, kind=kind, refs=refs, promote=promote)
+                d = dict(qp=qp, kind=kind, refs=refs, promote=promote, deblock=deblock)
                 pic = clips[i][t]
                 if device_planes:
                     planes = [torch.from_numpy(p).cuda() for p in (pic.y, pic.cb, pic.cr)]
@@ -105,8 +105,19 @@ def test_batch_graph_from_device_planes(single):
             assert same(got[i][t], single[i][t]), (STREAMS[i], t)
 
 
+def test_batch_without_loop_filter():
+    """Promotion without the loop filter (frames' promote_deblock 0) against the single-context
+    promotion without it."""
+    want = single_runs(deblock=False)
+    got = batch_runs(True, deblock=False)
+    for i in range(len(STREAMS)):
+        for t in range(FRAMES):
+            assert same(got[i][t], want[i][t]), (STREAMS[i], t)
+
+
 def test_single_stream_chain_matches_oracle(oracle):
-    """The single-context chain the batch is compared with, pinned to the oracle for one stream."""
+    """The single-context chain the batch is compared with, pinned to the oracle for one stream:
+    every reference loop-filtered (the oracle's filter of the oracle's reconstruction)."""
     w, h, seed, rng, nrefs, slices, cabac, t8 = STREAMS[1]
     pics = clip(w, h, seed)
     with analyser(w, h, rng, nrefs, slices, cabac, t8) as an:
@@ -122,8 +133,9 @@ def test_single_stream_chain_matches_oracle(oracle):
                                           fs if kind == abi.PIC_P else None)
             assert np.array_equal(r.codes, want[0]) and np.array_equal(r.stats, want[1]), t
             assert np.array_equal(r.recon.y, want[2][0]) and np.array_equal(r.mbs, want[3]), t
-            an.set_reference(promote, r.recon, r.mbs)
-            refs_host[promote] = (r.recon, r.mbs)
+            an.promote_reference(promote)
+            refs_host[promote] = (Picture(*oracle.deblock_picture(want[2][0], want[2][1], want[2][2], want[3],
+                                                                  an.mb_w, an.mb_h)), want[3])
 
 
 # learned streams (trained weights), one with every pair falling back, beside a full-search stream
@@ -154,7 +166,7 @@ def test_batch_with_learned_streams_matches_single_streams(use_graph):
             for t in range(FRAMES):
                 qp, kind, refs, promote = schedule(an.cfg.nrefs, t)
                 r = an.analyse(clips[i][t], qp, kind, refs)
-                an.set_reference(promote, r.recon, r.mbs)
+                an.promote_reference(promote)
                 res.append(r)
         want.append(res)
     # the batch
