@@ -72,55 +72,46 @@ struct Thr {
   int alpha, beta, tc0;
 };
 
-// One luma sample line across an edge, p3..p0 | q0..q3, strength b > 0 (h264/deblock.rs:279-334).
-__device__ __forceinline__ void filter_luma(int &p3, int &p2, int &p1, int &p0, int &q0, int &q1, int &q2, int &q3,
-                                            int b, const Thr &t) {
-  if (t.alpha == 0 || t.beta == 0) return;
-  const int P0 = p0, Q0 = q0, P1 = p1, Q1 = q1, P2 = p2, Q2 = q2;
-  if (iabs(P0 - Q0) >= t.alpha || iabs(P1 - P0) >= t.beta || iabs(Q1 - Q0) >= t.beta) return;
-  const int ap = iabs(P2 - P0), aq = iabs(Q2 - Q0);
-  if (b < 4) {
-    const int tc = t.tc0 + (ap < t.beta) + (aq < t.beta);
-    const int delta = clip3(-tc, tc, (((Q0 - P0) * 4) + (P1 - Q1) + 4) >> 3);
-    p0 = clip3(0, 255, P0 + delta);
-    q0 = clip3(0, 255, Q0 - delta);
-    if (ap < t.beta) p1 = P1 + clip3(-t.tc0, t.tc0, (P2 + ((P0 + Q0 + 1) >> 1) - (2 * P1)) >> 1);
-    if (aq < t.beta) q1 = Q1 + clip3(-t.tc0, t.tc0, (Q2 + ((P0 + Q0 + 1) >> 1) - (2 * Q1)) >> 1);
-    return;
-  }
-  const bool small = iabs(P0 - Q0) < (t.alpha >> 2) + 2;
-  if (ap < t.beta && small) {
-    p0 = (P2 + 2 * P1 + 2 * P0 + 2 * Q0 + Q1 + 4) >> 3;
-    p1 = (P2 + P1 + P0 + Q0 + 2) >> 2;
-    p2 = (2 * p3 + 3 * P2 + P1 + P0 + Q0 + 4) >> 3;
-  } else {
-    p0 = (2 * P1 + P0 + Q1 + 2) >> 2;
-  }
-  if (aq < t.beta && small) {
-    q0 = (P1 + 2 * P0 + 2 * Q0 + 2 * Q1 + Q2 + 4) >> 3;
-    q1 = (P0 + Q0 + Q1 + Q2 + 2) >> 2;
-    q2 = (2 * q3 + 3 * Q2 + Q1 + Q0 + P0 + 4) >> 3;
-  } else {
-    q0 = (2 * Q1 + Q0 + P1 + 2) >> 2;
-  }
+// Orders the LDS and memory accesses of the lanes of a one-wave workgroup: the wave executes its
+// LDS accesses in order, so only the compiler must not move accesses across this point.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One chroma sample line across an edge, p1 p0 | q0 q1 (h264/deblock.rs:279-298).
-__device__ __forceinline__ void filter_chroma(int &p1, int &p0, int &q0, int &q1, int b, const Thr &t) {
-  if (t.alpha == 0 || t.beta == 0) return;
-  const int P0 = p0, Q0 = q0, P1 = p1, Q1 = q1;
-  if (iabs(P0 - Q0) >= t.alpha || iabs(P1 - P0) >= t.beta || iabs(Q1 - Q0) >= t.beta) return;
-  if (b < 4) {
-    const int tc = t.tc0 + 1;
-    const int delta = clip3(-tc, tc, (((Q0 - P0) * 4) + (P1 - Q1) + 4) >> 3);
-    p0 = clip3(0, 255, P0 + delta);
-    q0 = clip3(0, 255, Q0 - delta);
-  } else {
-    p0 = (2 * P1 + P0 + Q1 + 2) >> 2;
-    q0 = (2 * Q1 + Q0 + P1 + 2) >> 2;
-  }
-  (void)p1;
-  (void)q1;
+// One sample line across an edge, p3..p0 | q0..q3, strength b (0: no filtering), without branches
+// so that luma and chroma lanes share the instructions (h264/deblock.rs:279-334). Chroma lines use
+// p1..q1 only: their p2/q2 updates are off (as if |p2 - p0| and |q2 - q0| were at least beta), tc is
+// tc0 + 1, and strength 4 takes the 3-tap p0/q0 formulas, which are luma's outside its strong case.
+__device__ __forceinline__ void filter_line(int &p3, int &p2, int &p1, int &p0, int &q0, int &q1, int &q2, int &q3,
+                                            int b, int alpha, int beta, int tc0, bool chroma) {
+  const int P0 = p0, Q0 = q0, P1 = p1, Q1 = q1, P2 = p2, Q2 = q2;
+  const bool on = b > 0 && iabs(P0 - Q0) < alpha && iabs(P1 - P0) < beta && iabs(Q1 - Q0) < beta;
+  const bool ap = !chroma && iabs(P2 - P0) < beta, aq = !chroma && iabs(Q2 - Q0) < beta;
+  // strength 1-3
+  const int tc = chroma ? tc0 + 1 : tc0 + ap + aq;
+  const int delta = clip3(-tc, tc, ((Q0 - P0) * 4 + (P1 - Q1) + 4) >> 3);
+  const int avg = (P0 + Q0 + 1) >> 1;
+  const int n_p0 = clip3(0, 255, P0 + delta), n_q0 = clip3(0, 255, Q0 - delta);
+  const int n_p1 = ap ? P1 + clip3(-tc0, tc0, (P2 + avg - 2 * P1) >> 1) : P1;
+  const int n_q1 = aq ? Q1 + clip3(-tc0, tc0, (Q2 + avg - 2 * Q1) >> 1) : Q1;
+  // strength 4
+  const bool small = iabs(P0 - Q0) < (alpha >> 2) + 2;
+  const bool sp = ap && small, sq = aq && small;
+  const int s_p0 = sp ? (P2 + 2 * P1 + 2 * P0 + 2 * Q0 + Q1 + 4) >> 3 : (2 * P1 + P0 + Q1 + 2) >> 2;
+  const int s_p1 = sp ? (P2 + P1 + P0 + Q0 + 2) >> 2 : P1;
+  const int s_p2 = sp ? (2 * p3 + 3 * P2 + P1 + P0 + Q0 + 4) >> 3 : P2;
+  const int s_q0 = sq ? (P1 + 2 * P0 + 2 * Q0 + 2 * Q1 + Q2 + 4) >> 3 : (2 * Q1 + Q0 + P1 + 2) >> 2;
+  const int s_q1 = sq ? (P0 + Q0 + Q1 + Q2 + 2) >> 2 : Q1;
+  const int s_q2 = sq ? (2 * q3 + 3 * Q2 + Q1 + Q0 + P0 + 4) >> 3 : Q2;
+  const bool strong = b == 4;
+  p0 = on ? (strong ? s_p0 : n_p0) : P0;
+  q0 = on ? (strong ? s_q0 : n_q0) : Q0;
+  p1 = on ? (strong ? s_p1 : n_p1) : P1;
+  q1 = on ? (strong ? s_q1 : n_q1) : Q1;
+  p2 = on && strong ? s_p2 : P2;
+  q2 = on && strong ? s_q2 : Q2;
 }
 
 __device__ __forceinline__ uint32_t pack4(const int *v) {
@@ -150,7 +141,8 @@ __device__ __forceinline__ Thr thresholds(const DbLds &L, int qpav, int b) {
   Thr t;
   t.alpha = L.alpha[ia];
   t.beta = L.beta[ia];
-  t.tc0 = b > 0 && b < 4 ? max((int)L.tc0[ia][b], 0) : 0;
+  t.tc0 = max((int)L.tc0[ia][b & 3], 0);  // read for every strength; used for 1-3 only
+  if (t.beta == 0) t.alpha = 0;           // no filtering: alpha 0 makes |p0 - q0| < alpha false
   return t;
 }
 
@@ -209,16 +201,24 @@ __device__ __forceinline__ void deblock_row(DbLds &L, uint8_t *__restrict__ py, 
     const int x0 = x * 16, cx0 = x * 8;
     // the macroblock's own samples, fetched during the previous macroblock
     int s[20];  // luma lane: columns x0-4 .. x0+15 of row y0+lane; chroma lane: columns cx0-4 .. cx0+7
-    unpack4(nv.x, s + 4);
-    unpack4(nv.y, s + 8);
-    unpack4(nv.z, s + 12);
-    unpack4(nv.w, s + 16);
+    // luma lanes: s[4 + k] = column x0 + k. Chroma lanes: columns cx0 .. cx0+7 at s[4, 5, 10 .. 15],
+    // so that the chroma edges at cx0 and cx0+4 sit where the luma edges at x0 and x0+8 do
+    if (chroma) {
+      int t[8];
+      unpack4(nv.x, t);
+      unpack4(nv.y, t + 4);
+      s[4] = t[0], s[5] = t[1], s[6] = s[7] = s[8] = s[9] = 0;
+#pragma unroll
+      for (int k = 2; k < 8; k++) s[8 + k] = t[k];
+      s[16] = s[17] = s[18] = s[19] = 0;
+    } else {
+      unpack4(nv.x, s + 4);
+      unpack4(nv.y, s + 8);
+      unpack4(nv.z, s + 12);
+      unpack4(nv.w, s + 16);
+    }
 #pragma unroll
     for (int k = 0; k < 4; k++) s[k] = left[k];
-    if (x + 1 < mb_w) {
-      nrec = fetch_record(x + 1);
-      fetch_samples(x + 1, nv);
-    }
     const ia_mbinfo &m = L.rm[x & 1];
     const ia_mbinfo &ml = x > 0 ? L.rm[(x - 1) & 1] : m;
     const ia_mbinfo &mt = row > 0 ? L.rt[x & 1] : m;
@@ -257,7 +257,7 @@ __device__ __forceinline__ void deblock_row(DbLds &L, uint8_t *__restrict__ py, 
         seen = __shfl(seen, 0);
       }
     }
-    __syncthreads();  // strengths in LDS; the row above's stores visible to every lane
+    wave_sync();  // strengths in LDS; the row above's stores visible to every lane (the acquire is the wave's)
     // the rows above: luma y0-4 .. y0-1 into tile rows 0-3, chroma cy0-2 .. cy0-1 into rows 0-1
     if (top_ok) {
       if (lane < 4) {
@@ -269,82 +269,80 @@ __device__ __forceinline__ void deblock_row(DbLds &L, uint8_t *__restrict__ py, 
         *reinterpret_cast<uint2 *>(&L.c[c][r][CX + 4]) = v;
       }
     }
-    if (live) {
-      // vertical edges, on the lane's row in registers
-      if (luma) {
-        const int seg = lane >> 2;
-        const int qpl = (ml.qp + m.qp + 1) >> 1;
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-          if ((e == 0 && !left_ok) || (m.t8x8 && (e & 1))) continue;
-          const int b = L.bs[e * 4 + seg];
-          if (b == 0) continue;
-          const Thr t = thresholds(L, e == 0 ? qpl : m.qp, b);
-          const int q = 4 + 4 * e;
-          filter_luma(s[q - 4], s[q - 3], s[q - 2], s[q - 1], s[q], s[q + 1], s[q + 2], s[q + 3], b, t);
-        }
-      } else if (chroma) {
-        const int seg = cl >> 1;
-        const int qpc = (chroma_qp(L, ml.qp) + chroma_qp(L, m.qp) + 1) >> 1, qpm = chroma_qp(L, m.qp);
-#pragma unroll
-        for (int e = 0; e < 4; e += 2) {
-          if (e == 0 && !left_ok) continue;
-          const int b = L.bs[e * 4 + seg];
-          if (b == 0) continue;
-          const Thr t = thresholds(L, e == 0 ? qpc : qpm, b);
-          const int q = 4 + 2 * e;
-          filter_chroma(s[q - 2], s[q - 1], s[q], s[q + 1], b, t);
-        }
-      }
+    // fetch macroblock x+1's records and samples now: they arrive while x is filtered (the release
+    // at the end of x waits for every load of the wave)
+    if (x + 1 < mb_w) {
+      nrec = fetch_record(x + 1);
+      fetch_samples(x + 1, nv);
     }
-    // rows into the tile
+    // per lane: the strength segment of its line, and the QPs of its macroblock edge and inner edges
+    const int seg = luma ? lane >> 2 : cl >> 1;
+    const int qpm = luma ? m.qp : chroma_qp(L, m.qp);
+    const int qpv = luma ? (ml.qp + m.qp + 1) >> 1 : (chroma_qp(L, ml.qp) + chroma_qp(L, m.qp) + 1) >> 1;
+    const int qph = luma ? (mt.qp + m.qp + 1) >> 1 : (chroma_qp(L, mt.qp) + chroma_qp(L, m.qp) + 1) >> 1;
+    const bool line = (luma || chroma) && live;
+    // the strength of edge e for the lane (0: not filtered): luma skips the odd inner edges of an
+    // 8x8-transform macroblock; chroma has edges 0 and 2 only (its sample 0 and 4)
+    auto edge_bs = [&](int dir, int e, bool ok0) -> int {
+      const bool use = line && (e > 0 || ok0) && (luma ? !(m.t8x8 && (e & 1)) : (e & 1) == 0);
+      return use ? L.bs[dir * 16 + e * 4 + seg] : 0;
+    };
+    // vertical edges, on the lane's row in registers
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int b = edge_bs(0, e, left_ok);
+      const Thr t = thresholds(L, e == 0 ? qpv : qpm, b);
+      const int q = 4 + 4 * e;
+      if (__any(b != 0))
+        filter_line(s[q - 4], s[q - 3], s[q - 2], s[q - 1], s[q], s[q + 1], s[q + 2], s[q + 3], b, t.alpha, t.beta,
+                    t.tc0, chroma);
+    }
+    // rows into the tiles (chroma lanes gather their spread samples back)
     if (luma) {
 #pragma unroll
       for (int k = 0; k < 5; k++) *reinterpret_cast<uint32_t *>(&L.y[4 + lane][LX + 4 * k]) = pack4(s + 4 * k);
     } else if (chroma) {
-#pragma unroll
-      for (int k = 0; k < 3; k++) *reinterpret_cast<uint32_t *>(&L.c[comp][2 + cl][CX + 4 * k]) = pack4(s + 4 * k);
+      const int u[4] = {s[4], s[5], s[10], s[11]};
+      *reinterpret_cast<uint32_t *>(&L.c[comp][2 + cl][CX]) = pack4(s);
+      *reinterpret_cast<uint32_t *>(&L.c[comp][2 + cl][CX + 4]) = pack4(u);
+      *reinterpret_cast<uint32_t *>(&L.c[comp][2 + cl][CX + 8]) = pack4(s + 12);
     }
-    __syncthreads();
-    if (live) {
-      // horizontal edges, on the lane's column
-      if (luma) {
-        int v[20];
+    wave_sync();
+    // horizontal edges, on the lane's column: luma rows y0-4 .. y0+15 at v[0 .. 19]; chroma rows
+    // cy0-2 .. cy0+7 at v[2 .. 5, 10 .. 15] (its edges at v[4] and v[12], as luma's edges 0 and 2)
+    int v[20];
+    if (luma) {
 #pragma unroll
-        for (int r = 0; r < 20; r++) v[r] = L.y[r][LX + 4 + lane];
-        const int seg = lane >> 2;
-        const int qpt = (mt.qp + m.qp + 1) >> 1;
+      for (int r = 0; r < 20; r++) v[r] = L.y[r][LX + 4 + lane];
+    } else {
+      const uint8_t(*ct)[16] = L.c[comp];
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-          if ((e == 0 && !top_ok) || (m.t8x8 && (e & 1))) continue;
-          const int b = L.bs[16 + e * 4 + seg];
-          if (b == 0) continue;
-          const Thr t = thresholds(L, e == 0 ? qpt : m.qp, b);
-          const int q = 4 + 4 * e;
-          filter_luma(v[q - 4], v[q - 3], v[q - 2], v[q - 1], v[q], v[q + 1], v[q + 2], v[q + 3], b, t);
-        }
+      for (int r = 0; r < 20; r++) v[r] = 0;
+      if (chroma) {
+        v[2] = ct[0][CX + 4 + cl], v[3] = ct[1][CX + 4 + cl], v[4] = ct[2][CX + 4 + cl], v[5] = ct[3][CX + 4 + cl];
 #pragma unroll
-        for (int r = 1; r < 20; r++) L.y[r][LX + 4 + lane] = (uint8_t)v[r];
-      } else if (chroma) {
-        int v[10];
-#pragma unroll
-        for (int r = 0; r < 10; r++) v[r] = L.c[comp][r][CX + 4 + cl];
-        const int seg = cl >> 1;
-        const int qpc = (chroma_qp(L, mt.qp) + chroma_qp(L, m.qp) + 1) >> 1, qpm = chroma_qp(L, m.qp);
-#pragma unroll
-        for (int e = 0; e < 4; e += 2) {
-          if (e == 0 && !top_ok) continue;
-          const int b = L.bs[16 + e * 4 + seg];
-          if (b == 0) continue;
-          const Thr t = thresholds(L, e == 0 ? qpc : qpm, b);
-          const int q = 2 + 2 * e;
-          filter_chroma(v[q - 2], v[q - 1], v[q], v[q + 1], b, t);
-        }
-#pragma unroll
-        for (int r = 1; r < 10; r++) L.c[comp][r][CX + 4 + cl] = (uint8_t)v[r];
+        for (int r = 4; r < 10; r++) v[6 + r] = ct[r][CX + 4 + cl];
       }
     }
-    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int b = edge_bs(1, e, top_ok);
+      const Thr t = thresholds(L, e == 0 ? qph : qpm, b);
+      const int q = 4 + 4 * e;
+      if (__any(b != 0))
+        filter_line(v[q - 4], v[q - 3], v[q - 2], v[q - 1], v[q], v[q + 1], v[q + 2], v[q + 3], b, t.alpha, t.beta,
+                    t.tc0, chroma);
+    }
+    if (luma) {
+#pragma unroll
+      for (int r = 1; r < 20; r++) L.y[r][LX + 4 + lane] = (uint8_t)v[r];
+    } else if (chroma) {
+      uint8_t(*ct)[16] = L.c[comp];
+      ct[1][CX + 4 + cl] = (uint8_t)v[3], ct[2][CX + 4 + cl] = (uint8_t)v[4], ct[3][CX + 4 + cl] = (uint8_t)v[5];
+#pragma unroll
+      for (int r = 4; r < 10; r++) ct[r][CX + 4 + cl] = (uint8_t)v[6 + r];
+    }
+    wave_sync();
     // stores: the lane's row (its 4 left samples belong to the macroblock on the left), the rows above
     if (luma) {
       uint32_t w[5];
@@ -367,11 +365,11 @@ __device__ __forceinline__ void deblock_row(DbLds &L, uint8_t *__restrict__ py, 
       if (top_ok && cl == 0)
         *reinterpret_cast<uint2 *>(cpl + (size_t)(cy0 - 1) * cs + cx0) = *reinterpret_cast<const uint2 *>(&L.c[comp][1][CX + 4]);
     }
-    __syncthreads();  // every lane's stores issued before the progress is published
+    wave_sync();  // every lane's stores before the progress is published (the release waits for them)
     if (lane == 0) __hip_atomic_store(prog + row, x + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     // macroblock x+1's records replace x-1's (read no more)
     if (x + 1 < mb_w) store_record(x + 1, nrec);
-    __syncthreads();
+    wave_sync();
   }
   // the counter of the row above is read no more: clear it for the next launch (and the last row its own)
   if (lane == 0) {
