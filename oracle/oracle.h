@@ -17,6 +17,9 @@
  *  - se/ue bits: Exp-Golomb lengths (enc/bitw.rs:81-89) against the codes' definition.
  *  - full search: brute force in numpy on small pictures.
  *  - intra predictors: closed forms of H.264 8.3 on small edge sets (numpy).
+ *  - loop filter (or_deblock_picture): the reference holds no vectors for it; pinned by what H.264 8.7
+ *    fixes (identities, the strength-4 mirror symmetry, p0 + q0 conservation, transposition, reach);
+ *    partially pinned.
  *  - motion_search / partition decision / code_mb / decide_rd / mb_bins: parity unpinned at the level of
  *    exact decisions (the reference commits no encoder outputs and cannot be built here); checked by
  *    invariants (reconstruction = prediction + dequantised residual, costs recomputed from outputs,
