@@ -89,29 +89,35 @@ __device__ __forceinline__ void filter_line(int &p3, int &p2, int &p1, int &p0, 
   const int P0 = p0, Q0 = q0, P1 = p1, Q1 = q1, P2 = p2, Q2 = q2;
   const bool on = b > 0 && iabs(P0 - Q0) < alpha && iabs(P1 - P0) < beta && iabs(Q1 - Q0) < beta;
   const bool ap = !chroma && iabs(P2 - P0) < beta, aq = !chroma && iabs(Q2 - Q0) < beta;
-  // strength 1-3
-  const int tc = chroma ? tc0 + 1 : tc0 + ap + aq;
-  const int delta = clip3(-tc, tc, ((Q0 - P0) * 4 + (P1 - Q1) + 4) >> 3);
-  const int avg = (P0 + Q0 + 1) >> 1;
-  const int n_p0 = clip3(0, 255, P0 + delta), n_q0 = clip3(0, 255, Q0 - delta);
-  const int n_p1 = ap ? P1 + clip3(-tc0, tc0, (P2 + avg - 2 * P1) >> 1) : P1;
-  const int n_q1 = aq ? Q1 + clip3(-tc0, tc0, (Q2 + avg - 2 * Q1) >> 1) : Q1;
-  // strength 4
-  const bool small = iabs(P0 - Q0) < (alpha >> 2) + 2;
-  const bool sp = ap && small, sq = aq && small;
-  const int s_p0 = sp ? (P2 + 2 * P1 + 2 * P0 + 2 * Q0 + Q1 + 4) >> 3 : (2 * P1 + P0 + Q1 + 2) >> 2;
-  const int s_p1 = sp ? (P2 + P1 + P0 + Q0 + 2) >> 2 : P1;
-  const int s_p2 = sp ? (2 * p3 + 3 * P2 + P1 + P0 + Q0 + 4) >> 3 : P2;
-  const int s_q0 = sq ? (P1 + 2 * P0 + 2 * Q0 + 2 * Q1 + Q2 + 4) >> 3 : (2 * Q1 + Q0 + P1 + 2) >> 2;
-  const int s_q1 = sq ? (P0 + Q0 + Q1 + Q2 + 2) >> 2 : Q1;
-  const int s_q2 = sq ? (2 * q3 + 3 * Q2 + Q1 + Q0 + P0 + 4) >> 3 : Q2;
   const bool strong = b == 4;
-  p0 = on ? (strong ? s_p0 : n_p0) : P0;
-  q0 = on ? (strong ? s_q0 : n_q0) : Q0;
-  p1 = on ? (strong ? s_p1 : n_p1) : P1;
-  q1 = on ? (strong ? s_q1 : n_q1) : Q1;
-  p2 = on && strong ? s_p2 : P2;
-  q2 = on && strong ? s_q2 : Q2;
+  int n0 = P0, m0 = Q0, n1 = P1, m1 = Q1, n2 = P2, m2 = Q2;
+  // each form only where some lane of the wave takes it (a wave-uniform branch)
+  if (__any(on && !strong)) {  // strength 1-3
+    const int tc = chroma ? tc0 + 1 : tc0 + ap + aq;
+    const int delta = clip3(-tc, tc, ((Q0 - P0) * 4 + (P1 - Q1) + 4) >> 3);
+    const int avg = (P0 + Q0 + 1) >> 1;
+    if (!strong) {
+      n0 = clip3(0, 255, P0 + delta);
+      m0 = clip3(0, 255, Q0 - delta);
+      n1 = ap ? P1 + clip3(-tc0, tc0, (P2 + avg - 2 * P1) >> 1) : P1;
+      m1 = aq ? Q1 + clip3(-tc0, tc0, (Q2 + avg - 2 * Q1) >> 1) : Q1;
+    }
+  }
+  if (__any(on && strong)) {  // strength 4
+    const bool small = iabs(P0 - Q0) < (alpha >> 2) + 2;
+    const bool sp = ap && small, sq = aq && small;
+    if (strong) {
+      n0 = sp ? (P2 + 2 * P1 + 2 * P0 + 2 * Q0 + Q1 + 4) >> 3 : (2 * P1 + P0 + Q1 + 2) >> 2;
+      n1 = sp ? (P2 + P1 + P0 + Q0 + 2) >> 2 : P1;
+      n2 = sp ? (2 * p3 + 3 * P2 + P1 + P0 + Q0 + 4) >> 3 : P2;
+      m0 = sq ? (P1 + 2 * P0 + 2 * Q0 + 2 * Q1 + Q2 + 4) >> 3 : (2 * Q1 + Q0 + P1 + 2) >> 2;
+      m1 = sq ? (P0 + Q0 + Q1 + Q2 + 2) >> 2 : Q1;
+      m2 = sq ? (2 * q3 + 3 * Q2 + Q1 + Q0 + P0 + 4) >> 3 : Q2;
+    }
+  }
+  if (on) {
+    p0 = n0, q0 = m0, p1 = n1, q1 = m1, p2 = n2, q2 = m2;
+  }
 }
 
 __device__ __forceinline__ uint32_t pack4(const int *v) {

@@ -3,7 +3,8 @@
 The 1920x1080 generator clip: pictures 0-3 are coded IPPP by the analysis (so reference 0 carries the
 encoder's vectors), picture 4 is searched against the 3 references, `--launches` times, by the
 exhaustive search (`--mode full`, +-32) or the learned proposal with the fallback off
-(`--mode learned`: the proposal kernel alone).
+(`--mode learned`: the proposal kernel alone), or on at 1.1x (`--mode learned-fallback`: the
+proposal, then the exhaustive search of its fallback list).
 
     rocprofv3 --pmc ... -- python3 ironav_amd/probes/search_loop.py --mode full --launches 5
 """
@@ -16,7 +17,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspa
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--mode", choices=["full", "learned"], default="full")
+    ap.add_argument("--mode", choices=["full", "learned", "learned-fallback"], default="full")
     ap.add_argument("--launches", type=int, default=5)
     args = ap.parse_args()
 
@@ -40,7 +41,7 @@ def main():
         an = Analyser(1920, 1080, nrefs=3, slices=68, search_mode=abi.SEARCH_FULL, search_range=32)
     else:
         an = Analyser(1920, 1080, nrefs=3, slices=68, search_mode=abi.SEARCH_LEARNED, search_range=32,
-                      fallback_ratio=float("inf"))
+                      fallback_ratio=1.1 if args.mode == "learned-fallback" else float("inf"))
         an.load_proposal_weights(proposal.load_weights())
     for s in range(3):
         an.set_reference(s, refs[s][0], refs[s][1])
