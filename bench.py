@@ -44,7 +44,7 @@ PROP_ABSDIFF_PER_MB_REF = (81 + 8 * 25) * 256
 # dispatch), each scaled by the counters' reading for a 256 MiB device copy: FETCH_SIZE reads
 # 131,092.6 KiB for it (scale 2.0), WRITE_SIZE 262,144 KiB (scale 1.0); profiles/traffic_1080p_3ref_pmc.md
 TRAFFIC_PROPOSAL = round((9268.9 * 262144 / 131092.8 + 7841.5) * 1024)     # proposal_kernel
-TRAFFIC_FULLSEARCH = round((7746.1 * 262144 / 131092.8 + 10149.7) * 1024)  # fullsearch_kernel
+TRAFFIC_FULLSEARCH = round((6905.7 * 262144 / 131094.2 + 7841.2) * 1024)   # fullsearch_kernel<5,128,..>
 
 
 def window_positions(mb_w, mb_h, rng):

@@ -374,11 +374,11 @@ struct Variant {
    fullsearch_batch_kernel<K, NT, RW32, W>, fullsearch_batch_kernel<K, NT, RW64, W>}
 const Variant VARIANTS[] = {
     IA_FS_VARIANT(4, 256, 3), IA_FS_VARIANT(6, 256, 3), IA_FS_VARIANT(8, 256, 2), IA_FS_VARIANT(4, 128, 3),
-    IA_FS_VARIANT(6, 128, 3), IA_FS_VARIANT(6, 256, 2),
+    IA_FS_VARIANT(6, 128, 3), IA_FS_VARIANT(6, 256, 2), IA_FS_VARIANT(5, 128, 3), IA_FS_VARIANT(5, 256, 3),
 };
 #undef IA_FS_VARIANT
 constexpr int NVARIANTS = sizeof(VARIANTS) / sizeof(VARIANTS[0]);
-constexpr int DEFAULT_VARIANT = 4;  // k6_t128_w3: fastest on MI355X at 1080p +-32 (measured)
+constexpr int DEFAULT_VARIANT = 6;  // k5_t128_w3: fastest on MI355X at 1080p +-32 (measured; 65 rows = 13 strips)
 // the list kernel's tile: a fallback list is short (a few percent of the (mb, ref) pairs), so wide
 // workgroups shorten the one round it takes (fastest measured at 1080p +-32, 3 references)
 constexpr int LIST_K = 6, LIST_NT = 256, LIST_W = 3;
