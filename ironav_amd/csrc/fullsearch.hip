@@ -379,9 +379,10 @@ const Variant VARIANTS[] = {
 #undef IA_FS_VARIANT
 constexpr int NVARIANTS = sizeof(VARIANTS) / sizeof(VARIANTS[0]);
 constexpr int DEFAULT_VARIANT = 6;  // k5_t128_w3: fastest on MI355X at 1080p +-32 (measured; 65 rows = 13 strips)
-// the list kernel's tile: a fallback list is short (a few percent of the (mb, ref) pairs), so wide
-// workgroups shorten the one round it takes (fastest measured at 1080p +-32, 3 references)
-constexpr int LIST_K = 6, LIST_NT = 256, LIST_W = 3;
+// the list kernel's tile: a fallback list is short (a few percent of the (mb, ref) pairs), so wider
+// workgroups than the whole-picture search's shorten the one round it takes (fastest measured at
+// 1080p +-32, 3 references: 1.3% ahead of K=6 with 256 threads)
+constexpr int LIST_K = 5, LIST_NT = 192, LIST_W = 3;
 
 }  // namespace
 

@@ -99,9 +99,9 @@ def test_training_kernels_match_torch_autograd():
     assert np.abs(got - want).max() <= 1e-3 * np.abs(want).max(), np.abs(got - want).max()
 
 
-def _learned(w, h, nref, ratio, params, seed=7, **clip):
+def _learned(w, h, nref, ratio, params, seed=7, search_range=32, **clip):
     pics = frames(w, h, nref + 2, seed, **clip)
-    with Analyser(w, h, nrefs=3, slices=2, search_mode=abi.SEARCH_LEARNED, search_range=32,
+    with Analyser(w, h, nrefs=3, slices=2, search_mode=abi.SEARCH_LEARNED, search_range=search_range,
                   fallback_ratio=ratio) as an:
         an.load_proposal_weights(params)
         # reference 0 carries the encoder's vectors: analyse the pictures before
@@ -139,13 +139,16 @@ def test_learned_search_sads_are_exact():
                 assert -(mbx * 16 + 24) <= dx <= 128 - mbx * 16 + 8 and -(mby * 16 + 24) <= dy <= 96 - mby * 16 + 8
 
 
-def test_learned_search_falls_back_to_full_search(oracle):
+@pytest.mark.parametrize("w,h,rng", [(96, 64, 32), (208, 160, 32), (288, 224, 64)])
+def test_learned_search_falls_back_to_full_search(oracle, w, h, rng):
     """With a fallback ratio of 0 every (macroblock, reference) goes to the exhaustive search through
-    the fallback list: the results are the oracle's exhaustive search exactly."""
+    the fallback list: the results are the oracle's exhaustive search exactly (clipped windows at the
+    small size; whole +-32 windows, 65 rows, inside the larger pictures; +-64 windows)."""
     params = proposal.init_params(12)
-    src, refs, got, fb = _learned(96, 64, 2, 0.0, params)
-    assert fb == 24 * 2
-    want = oracle.fullsearch(src.y, [r.y for r in refs], 6, 4, 32)
+    src, refs, got, fb = _learned(w, h, 2, 0.0, params, search_range=rng, max_motion=rng // 2)
+    mb_w, mb_h = w // 16, h // 16
+    assert fb == mb_w * mb_h * 2
+    want = oracle.fullsearch(src.y, [r.y for r in refs], mb_w, mb_h, rng)
     assert np.array_equal(got["sad"], want["sad"]) and np.array_equal(got["mv"], want["mv"])
 
 
