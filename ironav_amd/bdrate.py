@@ -1,12 +1,13 @@
 """Rate-distortion of the learned proposal against the exhaustive search: the Bjontegaard delta rate.
 
-Each search mode codes the same generator clip IPPP at four QPs, closed loop (each chain's
-references are its own reconstructions). The rate of a picture is the sum over its macroblocks of
+Each search mode codes the same generator clip IPPP at four QPs, closed loop: each chain's
+references are its own reconstructions, loop-filtered on the device as the encoder keeps them
+(`ia_promote_reference`, enc/mod.rs:653-667). The rate of a picture is the sum over its macroblocks of
 `ia_mbstats.bits`: the CABAC bins of each kept macroblock counted on the live contexts, as the
 reference's writer codes them (`CabacCost`'s model, enc/cabac.rs:150-191; the headers and the
 arithmetic coder's termination are not counted, and are the same for both modes). Distortion is the
-PSNR of the reconstruction (before deblocking, as both chains predict from it) over the picture's
-visible area, luma and chroma weighted 4:1:1.
+PSNR of the loop-filtered picture (the encoder's output and reference) over the picture's visible
+area, luma and chroma weighted 4:1:1.
 
 BD-rate (Bjontegaard): log rate interpolated over PSNR per mode (piecewise cubic Hermite, as JVET's
 common test conditions compute it; the single-cubic form of VCEG-M33 beside it), the mean difference
@@ -65,17 +66,16 @@ def code_chain(pics, w, h, qp, mode, search_range, params=None, nrefs=3, slices=
                   fallback_ratio=1.1) as an:
         if params is not None:
             an.load_proposal_weights(params)
-        refs = []
+        order = []  # reference slots, newest picture first
         for t, pic in enumerate(pics):
-            n = min(len(refs), nrefs)
-            for s in range(n):
-                an.set_reference(s, refs[s][0], refs[s][1])
             kind = abi.PIC_IDR if t == 0 else abi.PIC_P
-            res = an.analyse(pic, max(0, qp - 3) if t == 0 else qp, kind, list(range(n)))
+            res = an.analyse(pic, max(0, qp - 3) if t == 0 else qp, kind, order[:nrefs])
+            slot = order.pop() if len(order) == nrefs else len(order)
+            an.promote_reference(slot, deblock=True)
+            order.insert(0, slot)
             if t > 0:
                 bits += float(res.stats["bits"].sum()) / 256.0
-                ps.append(psnr(pic, res.recon, w, h))
-            refs.insert(0, (res.recon, res.mbs))
+                ps.append(psnr(pic, an.read_reference(slot)[0], w, h))
     return bits, float(np.mean(ps))
 
 
