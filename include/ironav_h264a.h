@@ -58,7 +58,7 @@ typedef struct ia_config {
                               (enc/frame.rs:757-758); 0: decisions by SATD (CAVLC) */
   int32_t transform8;      /* transform_8x8_mode_flag; used with cabac only (enc/frame.rs:409) */
   int32_t sub8x8;          /* P_8x8 sub-partitions, active at qp <= 20 (enc/frame.rs:200,436) */
-  int32_t fast;            /* `realtime()` decisions; must be 0 */
+  int32_t fast;            /* `realtime()` decisions (enc/frame.rs:348): shorter search, 16x16 only, early P_Skip */
   int32_t nrefs;           /* reference slots, 1..16 */
   int32_t slices;          /* slice count, 1..mb_h (fixed: it changes the decisions) */
   int32_t search_mode;     /* IA_SEARCH_* */

@@ -35,6 +35,10 @@ struct AnParams {
   int p_slice, sub8, use_cabac, transform8;
   double lam2;                  // 1.6 * 2^((QP'Y - 12) / 3)
   double skip_thresh;           // 256 * 0.625 * 2^(QP'Y / 6)
+  int fast;                     // realtime decisions (enc/frame.rs:348)
+  double fast_skip_y;           // 256 * 0.05 * 2^(QP'Y / 6): luma SAD bound of the untransformed P_Skip
+  double fast_skip_c;           // 128 * 0.05 * 2^(QP'Y / 6): chroma SAD bound of the same
+  double fast_flat;             // 256 * 0.8 * 2^(QP'Y / 6): Intra16x16 cost below which Intra4x4 is not tried
 };
 
 hipError_t launch_analyse(const AnParams &p, hipStream_t stream);

@@ -413,7 +413,8 @@ struct Coder {
       int cost = sad_int(r, bxp, byp, w, h, cands[k][0], cands[k][1]) + lam * mvd_bits(cands[k][0], cands[k][1], mvp[0], mvp[1]);
       if (cost < best_cost) { bestx = cands[k][0]; besty = cands[k][1]; best_cost = cost; }
     }
-    for (int si = 0; si < 3; si++) {
+    // the realtime decisions skip the 16-sample step (enc/frame.rs:973)
+    for (int si = P.fast ? 1 : 0; si < 3; si++) {
       const int step = 16 >> si, iters = step == 4 ? 32 : 4;
       for (int it = 0; it < iters; it++) {
         int moved = 0;
@@ -426,7 +427,14 @@ struct Coder {
         if (!moved) break;
       }
     }
-    // sub-sample refinement by SATD: 8 half samples around, then 4 quarter samples
+    // realtime: stop at an integer vector that already matches (enc/frame.rs:990-992)
+    if (P.fast && best_cost < w * h / 8 + lam) {
+      out[0] = (int16_t)bestx;
+      out[1] = (int16_t)besty;
+      return best_cost - lam * mvd_bits(bestx, besty, mvp[0], mvp[1]);
+    }
+    // sub-sample refinement by SATD: 8 half samples around, then 4 quarter samples (realtime coding
+    // keeps to the half samples, enc/frame.rs:1006)
     auto cost_sub = [&](int vx, int vy) {
       luma_pred(r, x0, y0, w, h, vx, vy, L.pred, 16);
       sync();
@@ -437,7 +445,7 @@ struct Coder {
     best_cost = cost_sub(bestx, besty);
     const int SQ[8][2] = {{-1, -1}, {0, -1}, {1, -1}, {-1, 0}, {1, 0}, {-1, 1}, {0, 1}, {1, 1}};
     const int DM[4][2] = {{0, -1}, {-1, 0}, {1, 0}, {0, 1}};
-    for (int pass = 0; pass < 2; pass++) {
+    for (int pass = 0; pass < (P.fast ? 1 : 2); pass++) {
       const int step = pass == 0 ? 2 : 1, np = pass == 0 ? 8 : 4;
       const int cx0 = bestx, cy0 = besty;
       for (int k = 0; k < np; k++) {
@@ -525,6 +533,7 @@ struct Coder {
     for (int i = 0; i < 16; i++) { best.mvs[i][0] = mv[0]; best.mvs[i][1] = mv[1]; }
     for (int k = 0; k < 4; k++) { best.refs[k] = (uint8_t)r; best.sub[k] = 0; }
     best.cost = cost + lam * (mvd_bits(mv[0], mv[1], mvp[0], mvp[1]) + 1) + rb;
+    if (P.fast) return;  // realtime: 16x16 only (enc/frame.rs:861-863)
     if (lane() == 0)
       for (int i = 0; i < 16; i++) { L.st_mvs[i][0] = 0; L.st_mvs[i][1] = 0; L.st_refs[i] = 0; }
     sync();
@@ -1808,6 +1817,30 @@ struct Coder {
       const int sad = wave_sum(s);
       sync();
       st->skip_sad = sad;
+      if (P.fast && (double)sad < P.fast_skip_y) {
+        // realtime: P_Skip without a transform when luma and chroma are within a twentieth of a
+        // quantiser step of the skip prediction (enc/frame.rs:731-746)
+        int16_t mvs[16][2];
+        for (int i = 0; i < 16; i++) { mvs[i][0] = smv[0]; mvs[i][1] = smv[1]; }
+        const uint8_t z4[4] = {0, 0, 0, 0};
+        inter_pred(0, z4, mvs, z4);
+        sync();
+        int cs = 0;
+        for (int i = lane(); i < 128; i += 64) cs += iabs((int)L.src_c[i >> 6][i & 63] - (int)L.cpred[i >> 6][i & 63]);
+        const int csad = wave_sum(cs);
+        if ((double)csad < P.fast_skip_c) {
+          code_new(K_INTER);
+          if (lane() == 0) {
+            L.code.skip = 1;
+            for (int i = 0; i < 16; i++) { L.code.mvs[i][0] = smv[0]; L.code.mvs[i][1] = smv[1]; }
+          }
+          for (int i = lane(); i < 256; i += 64) cyv(i & 15, i >> 4) = L.pred[i];
+          for (int i = lane(); i < 128; i += 64) ccv(i >> 6, (i & 63) & 7, (i & 63) >> 3) = L.cpred[i >> 6][i & 63];
+          sync();
+          return;
+        }
+        sync();
+      }
       if ((double)sad < P.skip_thresh) {
         int16_t mvs[16][2];
         for (int i = 0; i < 16; i++) { mvs[i][0] = smv[0]; mvs[i][1] = smv[1]; }
@@ -1835,7 +1868,8 @@ struct Coder {
       return;
     }
     // SATD decisions (CAVLC), enc/frame.rs:760-789
-    const int try_i4 = !P.p_slice || best_intra < inter.cost;
+    // realtime: flat intra macroblocks stay Intra16x16 (enc/frame.rs:763-765)
+    const int try_i4 = (!P.p_slice || best_intra < inter.cost) && !(P.fast && !P.p_slice && (double)i16c < P.fast_flat);
     int have_i4 = 0;
     if (try_i4) {
       const int cost = code_i4() + lam * 6;

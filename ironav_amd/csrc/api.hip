@@ -94,6 +94,11 @@ int analysis_params(ia_ctx *c, int qp, int kind, const int *ref_slots, int nref,
   p.transform8 = c->cfg.cabac && c->cfg.transform8;
   p.lam2 = 1.6 * pow(2.0, ((double)p.qp - 12.0) / 3.0);
   p.skip_thresh = 256.0 * 0.625 * pow(2.0, (double)p.qp / 6.0);
+  const double step = pow(2.0, (double)p.qp / 6.0);
+  p.fast = c->cfg.fast != 0;
+  p.fast_skip_y = 256.0 * 0.05 * step;
+  p.fast_skip_c = 128.0 * 0.05 * step;
+  p.fast_flat = 256.0 * 0.8 * step;
   return IA_OK;
 }
 
@@ -111,10 +116,6 @@ int check_config(const ia_config *cfg, std::string &why) {
   }
   if (cfg->bit_depth != 8) {
     why = "only 8-bit pictures are supported on the GPU";
-    return IA_EUNSUPPORTED;
-  }
-  if (cfg->fast) {
-    why = "the fast (realtime) decisions are not implemented";
     return IA_EUNSUPPORTED;
   }
   if (cfg->nrefs < 1 || cfg->nrefs > 16) {

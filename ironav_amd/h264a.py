@@ -98,12 +98,20 @@ def slice_rows(mb_h, slices):
     return [(i * mb_h // n, (i + 1) * mb_h // n) for i in range(n)]
 
 
+def realtime(height, threads):
+    """The Analyser settings of the encoder's live mode (enc/mod.rs:249-254, `realtime()`): the
+    realtime decisions, one reference, and a slice per host thread up to one per two macroblock rows.
+    Pass them as keyword arguments: Analyser(w, h, **realtime(h, threads))."""
+    mb_h = (height + 15) // 16
+    return dict(fast=True, nrefs=1, slices=max(1, min(threads, mb_h // 2)))
+
+
 class Analyser:
     """One encoder's analysis context on the current HIP device."""
 
     def __init__(self, width, height, *, nrefs=3, slices=1, cabac=True, transform8=True, sub8x8=False,
-                 search_mode=abi.SEARCH_DIAMOND, search_range=32, fallback_ratio=1.1):
-        self.cfg = abi.Config(width, height, 8, int(cabac), int(transform8), int(sub8x8), 0, nrefs, slices,
+                 search_mode=abi.SEARCH_DIAMOND, search_range=32, fallback_ratio=1.1, fast=False):
+        self.cfg = abi.Config(width, height, 8, int(cabac), int(transform8), int(sub8x8), int(fast), nrefs, slices,
                               search_mode, search_range, fallback_ratio)
         lib = library()
         h = C.c_void_p()

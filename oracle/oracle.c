@@ -805,6 +805,7 @@ typedef struct {
   uint8_t cabac[1024];
   const ia_fsresult *fs;
   int search_mode;
+  int fast;              /* realtime decisions, enc/frame.rs:348 */
   ia_mbstats *st;
 } coder_t;
 
@@ -1035,8 +1036,9 @@ static int motion_search(const coder_t *c, int r, int mbx, int mby, int bx, int 
     int cost = COST_INT(cands[k]);
     if (cost < best_cost) { best[0] = cands[k][0]; best[1] = cands[k][1]; best_cost = cost; }
   }
+  /* the realtime decisions skip the 16-sample step (enc/frame.rs:973) */
   static const int STEPS[3] = {16, 8, 4};
-  for (int si = 0; si < 3; si++) {
+  for (int si = c->fast ? 1 : 0; si < 3; si++) {
     int step = STEPS[si];
     int iters = step == 4 ? 32 : 4;
     for (int it = 0; it < iters; it++) {
@@ -1051,13 +1053,20 @@ static int motion_search(const coder_t *c, int r, int mbx, int mby, int bx, int 
       if (!moved) break;
     }
   }
-  /* sub-sample refinement by SATD: 8 half samples around, then 4 quarter samples (:994-1017) */
+  /* realtime: stop at an integer vector that already matches (enc/frame.rs:990-992) */
+  if (c->fast && best_cost < w * h / 8 + lam) {
+    out[0] = best[0];
+    out[1] = best[1];
+    return best_cost - lam * mvd_bits(best, mvp);
+  }
+  /* sub-sample refinement by SATD: 8 half samples around, then 4 quarter samples (:994-1017);
+     realtime coding keeps to the half samples (:1006) */
   uint16_t pred[256];
 #define COST_SUB(mv) (luma_pred(c, r, x0, y0, w, h, (mv), pred), satd_block(c, 0, x0, y0, w, h, pred) + lam * mvd_bits((mv), mvp))
   best_cost = COST_SUB(best);
   static const int SQ[8][2] = {{-1, -1}, {0, -1}, {1, -1}, {-1, 0}, {1, 0}, {-1, 1}, {0, 1}, {1, 1}};
   static const int DM[4][2] = {{0, -1}, {-1, 0}, {1, 0}, {0, 1}};
-  for (int pass = 0; pass < 2; pass++) {
+  for (int pass = 0; pass < (c->fast ? 1 : 2); pass++) {
     int step = pass == 0 ? 2 : 1, np = pass == 0 ? 8 : 4;
     const int (*pts)[2] = pass == 0 ? SQ : DM;
     int16_t centre[2] = {best[0], best[1]};
@@ -1133,6 +1142,11 @@ static void best_inter_ref(const coder_t *c, int mbx, int mby, int r, inter_t *o
   best.part = 0;
   for (int i = 0; i < 16; i++) { best.mvs[i][0] = mv[0]; best.mvs[i][1] = mv[1]; }
   best.cost = cost + lam * (mvd_bits(mv, mvp) + 1) + rb;
+  if (c->fast) {  /* realtime: 16x16 only (enc/frame.rs:861-863) */
+    *out = best;
+    memcpy(out->refs, refs, 4);
+    return;
+  }
   substate_t st;
   memset(&st, 0, sizeof st);
   uint8_t sub[4] = {0, 0, 0, 0};
@@ -1944,6 +1958,27 @@ static void code_mb(coder_t *c, int mbx, int mby, mbcode_t *out) {
     for (int i = 0; i < 256; i++) sad += absi((int)c->src[0][(size_t)(y0 + i / 16) * ls + x0 + i % 16] - pred[i]);
     st->skip_sad = sad;
     double step = pow(2.0, (double)qpp_(c) / 6.0);
+    if (c->fast && (double)sad < 256.0 * 0.05 * step) {
+      /* realtime: P_Skip without a transform when luma and chroma are within a twentieth of a
+         quantiser step of the skip prediction (enc/frame.rs:731-746) */
+      int16_t mvs[16][2];
+      for (int i = 0; i < 16; i++) { mvs[i][0] = smv[0]; mvs[i][1] = smv[1]; }
+      static const uint8_t Z4[4] = {0, 0, 0, 0};
+      uint16_t rec[384], (*cpred)[64] = (uint16_t(*)[64])(rec + 256);
+      inter_pred(c, mbx, mby, 0, Z4, (const int16_t(*)[2])mvs, Z4, rec, cpred);
+      const int cs = cs_(c);
+      int csad = 0;
+      for (int k = 0; k < 2; k++)
+        for (int i = 0; i < 64; i++) csad += absi((int)c->src[1 + k][(size_t)(mby * 8 + i / 8) * cs + mbx * 8 + i % 8] - cpred[k][i]);
+      if ((double)csad < 128.0 * 0.05 * step) {
+        mbcode_new(out, K_INTER);
+        out->skip = 1;
+        memcpy(out->mvs, mvs, sizeof mvs);
+        memcpy(rec, pred, sizeof pred);
+        restore_mb(c, mbx, mby, rec);
+        return;
+      }
+    }
     if ((double)sad < 256.0 * 0.625 * step) {
       int16_t mvs[16][2];
       for (int i = 0; i < 16; i++) { mvs[i][0] = smv[0]; mvs[i][1] = smv[1]; }
@@ -1973,6 +2008,8 @@ static void code_mb(coder_t *c, int mbx, int mby, mbcode_t *out) {
   }
   /* SATD decisions (CAVLC), enc/frame.rs:760-789 */
   int try_i4 = !have_inter || best_intra < inter.cost;
+  /* realtime: flat intra macroblocks stay Intra16x16 (enc/frame.rs:763-765) */
+  if (c->fast && !have_inter && (double)i16c < 256.0 * 0.8 * pow(2.0, (double)qpp_(c) / 6.0)) try_i4 = 0;
   static _Thread_local mbcode_t i4code;
   int have_i4 = 0;
   if (try_i4) {
@@ -2195,6 +2232,7 @@ static void code_slice(const slice_job_t *j, int s, mbcode_t *code) {
   c.state = j->state + (size_t)c.row0 * j->mb_w;
   c.fs = j->fs;
   c.search_mode = j->cfg->search_mode;
+  c.fast = j->cfg->fast != 0;
   cabac_init(c.cabac, !c.p_slice, j->qp);
   for (int mby = c.row0; mby < c.row1; mby++)
     for (int mbx = 0; mbx < j->mb_w; mbx++) {
@@ -2230,7 +2268,7 @@ int or_analyse_picture_mt(const ia_config *cfg, const uint8_t *y, const uint8_t 
                           uint8_t *recon_cb, uint8_t *recon_cr, ia_mbinfo *mbs, int threads) {
   const int mb_w = (cfg->width + 15) / 16, mb_h = (cfg->height + 15) / 16, pw = mb_w * 16, ph = mb_h * 16;
   const size_t ylen = (size_t)pw * ph, clen = ylen / 4;
-  if (cfg->bit_depth != 8 || cfg->fast || qp < 0 || qp > 51 || (kind == IA_PIC_P && (nref < 1 || nref > 16))) return -1;
+  if (cfg->bit_depth != 8 || qp < 0 || qp > 51 || (kind == IA_PIC_P && (nref < 1 || nref > 16))) return -1;
   entropy_init();
   const int nslices = clampi(cfg->slices, 1, mb_h);
   threads = clampi(threads, 1, nslices);

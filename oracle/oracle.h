@@ -20,6 +20,9 @@
  *  - loop filter (or_deblock_picture): the reference holds no vectors for it; pinned by what H.264 8.7
  *    fixes (identities, the strength-4 mirror symmetry, p0 + q0 conservation, transposition, reach);
  *    partially pinned.
+ *  - realtime decisions (coder_t.fast): the early P_Skip, 16x16 only, half samples only and the
+ *    flat-intra rule by what they must do to a picture (tests/test_oracle_realtime.py); the shorter
+ *    diamond and the integer stop parity unpinned.
  *  - motion_search / partition decision / code_mb / decide_rd / mb_bins: parity unpinned at the level of
  *    exact decisions (the reference commits no encoder outputs and cannot be built here); checked by
  *    invariants (reconstruction = prediction + dequantised residual, costs recomputed from outputs,

@@ -40,7 +40,6 @@ def test_create_rejects_bad_configs_before_touching_a_device():
     bad = [
         abi.Config(15, 16, 8, 1, 1, 0, 0, 1, 1, 0, 16, 1.1),    # odd width
         abi.Config(64, 64, 10, 1, 1, 0, 0, 1, 1, 0, 16, 1.1),   # 10-bit
-        abi.Config(64, 64, 8, 1, 1, 0, 1, 1, 1, 0, 16, 1.1),    # fast
         abi.Config(64, 64, 8, 1, 1, 0, 0, 0, 1, 0, 16, 1.1),    # nrefs 0
         abi.Config(64, 64, 8, 1, 1, 0, 0, 1, 5, 0, 16, 1.1),    # more slices than MB rows
         abi.Config(64, 64, 8, 1, 1, 0, 0, 1, 1, 0, 65, 1.1),    # range

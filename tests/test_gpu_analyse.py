@@ -43,10 +43,10 @@ def compare(tag, got, want):
 
 
 def run_chain(oracle, w, h, nframes, qp, *, nrefs=3, slices=1, cabac=True, transform8=True, sub8x8=False,
-              search_mode=abi.SEARCH_DIAMOND, search_range=16, seed=1, **clip):
+              search_mode=abi.SEARCH_DIAMOND, search_range=16, seed=1, fast=False, **clip):
     pics = frames(w, h, nframes, seed, **clip)
     cfg = dict(nrefs=nrefs, slices=slices, cabac=cabac, transform8=transform8, sub8x8=sub8x8,
-               search_mode=search_mode, search_range=search_range)
+               search_mode=search_mode, search_range=search_range, fast=fast)
     with Analyser(w, h, **cfg) as an:
         refs = []  # (Picture, mbinfo), most recent first
         for t, pic in enumerate(pics):
@@ -122,3 +122,29 @@ def test_1080p_workload_chain(oracle, qp):
     slices, exhaustive search +-32 as the extra candidate, 8x8 transform, CABAC; bit-exact."""
     run_chain(oracle, 1920, 1080, 3, qp, nrefs=3, slices=8, search_mode=abi.SEARCH_FULL, search_range=32,
               seed=1080 + qp)
+
+
+@pytest.mark.parametrize("cabac,qp", [(True, 26), (False, 26), (False, 38)])
+def test_realtime_decisions(oracle, cabac, qp):
+    """The live mode's decisions (enc/frame.rs:731-765, 861-863, 973, 990-992, 1006): early P_Skip
+    without a transform, 16x16 only, the shorter diamond, the integer stop, half samples only, and
+    Intra16x16 alone for flat macroblocks of CAVLC I pictures."""
+    run_chain(oracle, 128, 96, 4, qp, nrefs=1, slices=2, cabac=cabac, transform8=cabac, fast=True, seed=51 + qp,
+              max_motion=8)
+
+
+def test_realtime_still_scene_and_full_search(oracle):
+    """A still scene takes the untransformed P_Skip in most macroblocks; the exhaustive winners as the
+    extra candidate of the shorter search."""
+    got = run_chain(oracle, 96, 64, 3, 30, nrefs=1, slices=1, fast=True, seed=60, max_motion=0, noise_sigma=0.0)
+    assert (got[0]["skip"] == 1).mean() > 0.5
+    run_chain(oracle, 128, 96, 3, 22, nrefs=2, slices=2, fast=True, search_mode=abi.SEARCH_FULL, search_range=24,
+              seed=61, max_motion=10)
+
+
+def test_realtime_1080p(oracle):
+    """The live mode at 1920x1080: IDR + 2 P pictures, one reference, 34 slices (one per two
+    macroblock rows, as `realtime()` sets them on a many-core host), CABAC; bit-exact."""
+    from ironav_amd.h264a import realtime
+    kw = realtime(1080, 64)
+    run_chain(oracle, 1920, 1080, 3, 28, nrefs=kw["nrefs"], slices=kw["slices"], fast=True, seed=1081)
