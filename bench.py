@@ -106,6 +106,9 @@ def main():
     ap.add_argument("--size", choices=["1080p", "720p"], default="1080p", help="plain: picture size")
     ap.add_argument("--search", choices=["full", "diamond"], default="full", help="plain: motion search")
     ap.add_argument("--threads", type=int, default=16, help="plain: host threads of the CPU oracle")
+    ap.add_argument("--realtime", action="store_true",
+                    help="plain, batched: the encoder's live mode (realtime decisions, one reference; batched: "
+                         "the diamond search)")
     ap.add_argument("--streams", type=int, default=64, help="batched: streams")
     ap.add_argument("--frames", type=int, default=30, help="batched: frames per stream")
     args = ap.parse_args()
@@ -295,6 +298,8 @@ def bench_plain(args):
 
     big = args.size == "1080p"
     width, height, nref, qp = (1920, 1080, 3, 28) if big else (1280, 720, 3, 28)
+    if args.realtime:
+        nref = 1  # enc/mod.rs:249-254
     rng = 32 if big else 16
     full = args.search == "full"
     mode = abi.SEARCH_FULL if full else abi.SEARCH_DIAMOND
@@ -304,7 +309,8 @@ def bench_plain(args):
     pics = [Picture(*clipgen.pad_planes(*f)) for f in clipgen.make_clip(spec)]
     torch.cuda.init()
     stream = torch.cuda.Stream()
-    an = Analyser(width, height, nrefs=nref, slices=slices, search_mode=mode, search_range=rng, sub8x8=True)
+    an = Analyser(width, height, nrefs=nref, slices=slices, search_mode=mode, search_range=rng, sub8x8=True,
+                  fast=args.realtime)
     an.set_stream(stream.cuda_stream)
     refs = []
     for t in range(4):
@@ -336,6 +342,8 @@ def bench_plain(args):
     got = an.read_analysis()
     mbs = an.mb_w * an.mb_h
     search = f"+-{rng} exhaustive search" if full else "the reference's diamond search"
+    if args.realtime:
+        search += " with the realtime decisions"
     print(f"[plain] GPU {ms:.2f} ms per picture; CPU oracle next", file=sys.stderr, flush=True)
 
     cpu, exact = None, None
@@ -380,7 +388,7 @@ def bench_plain(args):
         "dtype": "u8",
         "data": f"synthetic (seeded generator clip {width}x{height} 4:2:0; references coded IPPP by the analysis; "
                 "inputs resident in HBM)",
-        "config": {"workload": f"P picture at {width}x{height}, QP {qp}, 3 references, {search}, sub-8x8 partitions "
+        "config": {"workload": f"P picture at {width}x{height}, QP {qp}, {nref} reference(s), {search}, sub-8x8 partitions "
                                f"on, 8x8 transform, CABAC, {slices} slices",
                    "macroblocks_per_step": mbs},
         "bitexact_vs_oracle": exact,
@@ -513,11 +521,14 @@ def bench_batched(args):
     workers = batch_workers() or torch.cuda.get_device_properties(0).multi_processor_count * 4
     target = max(1, -(-sum(nmbs) // workers))
     slices = [min(max(1, round(n / target)), (h + 15) // 16) for n, (w, h) in zip(nmbs, order)]
-    nrefs = 3
+    # the live mode (enc/mod.rs:249-254): realtime decisions, one reference, the reference's diamond search
+    nrefs = 1 if args.realtime else 3
+    mode = abi.SEARCH_DIAMOND if args.realtime else abi.SEARCH_FULL
     table = autotune.load()
     ans = []
     for (w, h), ns in zip(order, slices):
-        an = Analyser(w, h, nrefs=nrefs, slices=ns, search_mode=abi.SEARCH_FULL, search_range=32, sub8x8=True)
+        an = Analyser(w, h, nrefs=nrefs, slices=ns, search_mode=mode, search_range=32, sub8x8=True,
+                      fast=args.realtime)
         autotune.apply(an, table)
         ans.append(an)
 
@@ -547,22 +558,25 @@ def bench_batched(args):
     equal = 0
     print(f"[batched] timed {dt:.2f} s; checking streams against the single-stream path", file=sys.stderr, flush=True)
     for i, ((w, h), ns) in enumerate(zip(order, slices)):
-        with Analyser(w, h, nrefs=nrefs, slices=ns, search_mode=abi.SEARCH_FULL, search_range=32, sub8x8=True) as an:
+        with Analyser(w, h, nrefs=nrefs, slices=ns, search_mode=mode, search_range=32, sub8x8=True,
+                      fast=args.realtime) as an:
             autotune.apply(an, table)
             pics = [Picture(*(p.cpu().numpy() for p in frames[i][t])) for t in range(min(2, args.frames))]
             ok = True
             for t, pic in enumerate(pics):
                 a = frame_args(i, t)
                 r = an.analyse(pic, a["qp"], a["kind"], a["refs"])
-                an.set_reference(a["promote"], r.recon, r.mbs)
+                an.promote_reference(a["promote"])  # loop filter on, as the batch promotes
                 ok = ok and digest(r) == want[i][t]
             equal += ok
     for an in ans:
         an.close()
     mb_per_step = sum(nmbs)
     ms = dt * 1000.0 / max(1, steps)
+    search = ("the reference's diamond search with the realtime decisions" if args.realtime
+              else "+-32 exhaustive search")
     out = {
-        "metric": "MB/s batched analysis: streams of mixed sizes, +-32 exhaustive search + exact analysis",
+        "metric": f"MB/s batched analysis: streams of mixed sizes, {search} + exact analysis",
         "value": round(mb_per_step * steps / dt, 1) if steps else None,
         "unit": "MB/s",
         "n_gpus": 1,
@@ -575,7 +589,7 @@ def bench_batched(args):
         "dtype": "u8",
         "data": f"synthetic (device-generated clips, {gen_s:.1f} s; resident in HBM)",
         "config": {"workload": f"{args.streams} streams x {args.frames} frames, sizes {sorted(set(order))}, IPPP "
-                               "(IDR QP 25, P QP 28), 3 references, +-32 exhaustive search, CABAC + 8x8 transform, "
+                               f"(IDR QP 25, P QP 28), {nrefs} reference(s), {search}, CABAC + 8x8 transform, "
                                "one hipGraph per frame step",
                    "macroblocks_per_step": mb_per_step,
                    "streams_per_size": {f"{w}x{h}": order.count((w, h)) for w, h in sizes},
