@@ -1490,9 +1490,15 @@ struct Coder {
   };
 
   __device__ static void write_block(Bins &c, int cat, int cbf_ctx, const int16_t *levels, int max) {
-    int total = 0, last = -1;
-    for (int i = 0; i < max; i++)
-      if (levels[i] != 0) { total++; last = i; }
+    // the block's nonzero levels as a mask, read four at a time (independent reads share one wait)
+    uint64_t nz = 0;
+    int j = 0;
+    for (; j + 4 <= max; j += 4) {
+      const int16_t l0 = levels[j], l1 = levels[j + 1], l2 = levels[j + 2], l3 = levels[j + 3];
+      nz |= (uint64_t)((l0 != 0) | ((l1 != 0) << 1) | ((l2 != 0) << 2) | ((l3 != 0) << 3)) << j;
+    }
+    for (; j < max; j++) nz |= (uint64_t)(levels[j] != 0) << j;
+    const int total = __builtin_popcountll(nz), last = nz ? 63 - __builtin_clzll(nz) : -1;
     if (cbf_ctx >= 0) c.decision(cbf_ctx, total > 0);
     if (total == 0) return;
     for (int i = 0; i < max - 1; i++) {
@@ -1500,18 +1506,20 @@ struct Coder {
       if (cat == 3) { sig = min(i, 2); lst = min(i, 2); }
       else if (cat == 5) { sig = IA_SIG_OFF_8X8[0][i]; lst = IA_LAST_OFF_8X8[i]; }
       else { sig = i; lst = i; }
-      const int l = levels[i];
-      c.decision(IA_SIG_OFF[0][cat] + sig, l != 0);
-      if (l != 0) {
+      const int nzi = (int)((nz >> i) & 1);
+      c.decision(IA_SIG_OFF[0][cat] + sig, nzi);
+      if (nzi) {
         c.decision(IA_LAST_OFF[0][cat] + lst, i == last);
         if (i == last) break;
       }
     }
     const int base = IA_ABS_OFF[cat];
     int eq1 = 0, gt1 = 0;
-    for (int i = last; i >= 0; i--) {
+    // the nonzero levels from the last to the first
+    for (uint64_t m = nz; m != 0;) {
+      const int i = 63 - __builtin_clzll(m);
+      m &= ~(1ull << i);
       const int l = levels[i];
-      if (l == 0) continue;
       const uint32_t a = (uint32_t)iabs(l);
       const int inc0 = gt1 != 0 ? 0 : min(1 + eq1, 4);
       if (a == 1) {
