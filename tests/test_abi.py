@@ -88,3 +88,11 @@ def test_bd_rate_closed_forms():
         assert abs(bd_rate(rate, psnr, [r * 0.9 for r in rate], psnr, method) + 10.0) < 1e-6
         # a curve shifted by +0.5 dB at the same rates needs fewer bits at equal quality
         assert bd_rate(rate, psnr, rate, [p + 0.5 for p in psnr], method) < 0
+
+
+def test_realtime_settings_follow_the_encoders_live_mode():
+    """h264a.realtime mirrors enc/mod.rs:249-254: the realtime decisions, one reference, a slice per
+    thread up to one per two macroblock rows (at least one)."""
+    assert h264a.realtime(1080, 64) == {"fast": True, "nrefs": 1, "slices": 34}
+    assert h264a.realtime(1080, 16)["slices"] == 16
+    assert h264a.realtime(16, 8)["slices"] == 1 and h264a.realtime(48, 1)["slices"] == 1
