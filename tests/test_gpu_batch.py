@@ -27,17 +27,18 @@ def schedule(nrefs, t):
     return (25 if t == 0 else 28 + (t % 3) - 1), (abi.PIC_IDR if t == 0 else abi.PIC_P), refs, t % nrefs
 
 
-def analyser(w, h, rng, nrefs, slices, cabac, t8):
+def analyser(w, h, rng, nrefs, slices, cabac, t8, fast=False):
+    # the live mode searches as the reference's does (diamond); the default streams add the exhaustive winner
     return Analyser(w, h, nrefs=nrefs, slices=slices, cabac=bool(cabac), transform8=bool(t8),
-                    search_mode=abi.SEARCH_FULL, search_range=rng)
+                    search_mode=abi.SEARCH_DIAMOND if fast else abi.SEARCH_FULL, search_range=rng, fast=fast)
 
 
-def single_runs(deblock=True):
+def single_runs(deblock=True, fast=False):
     out = []
     for (w, h, seed, rng, nrefs, slices, cabac, t8) in STREAMS:
         pics = clip(w, h, seed)
         res = []
-        with analyser(w, h, rng, nrefs, slices, cabac, t8) as an:
+        with analyser(w, h, rng, nrefs, slices, cabac, t8, fast) as an:
             for t in range(FRAMES):
                 qp, kind, refs, promote = schedule(nrefs, t)
                 r = an.analyse(pics[t], qp, kind, refs)
@@ -47,11 +48,12 @@ def single_runs(deblock=True):
     return out
 
 
-def batch_runs(use_graph, device_planes=False, deblock=True):
+def batch_runs(use_graph, device_planes=False, deblock=True, fast=False):
     import torch
 
     clips = [clip(w, h, seed) for (w, h, seed, *_rest) in STREAMS]
-    ans = [analyser(w, h, rng, nrefs, slices, cabac, t8) for (w, h, seed, rng, nrefs, slices, cabac, t8) in STREAMS]
+    ans = [analyser(w, h, rng, nrefs, slices, cabac, t8, fast)
+           for (w, h, seed, rng, nrefs, slices, cabac, t8) in STREAMS]
     out = [[] for _ in STREAMS]
     keep = []
     with Batch(ans, use_graph=use_graph) as b:
@@ -110,6 +112,16 @@ def test_batch_without_loop_filter():
     promotion without it."""
     want = single_runs(deblock=False)
     got = batch_runs(True, deblock=False)
+    for i in range(len(STREAMS)):
+        for t in range(FRAMES):
+            assert same(got[i][t], want[i][t]), (STREAMS[i], t)
+
+
+def test_batch_live_mode_streams():
+    """Streams in the live mode (realtime decisions, diamond search; CABAC and CAVLC), replayed as a
+    graph, against the single-context path in the same mode."""
+    want = single_runs(fast=True)
+    got = batch_runs(True, fast=True)
     for i in range(len(STREAMS)):
         for t in range(FRAMES):
             assert same(got[i][t], want[i][t]), (STREAMS[i], t)
