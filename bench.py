@@ -113,8 +113,8 @@ def main():
     ap.add_argument("--frames", type=int, default=30, help="batched: frames per stream")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (streams or slice bands would "
-              "shard across GPUs with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (whole streams, or slice bands with a "
+              "halo of reference rows, would shard across GPUs). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     if args.config == "batched":
         return bench_batched(args)
