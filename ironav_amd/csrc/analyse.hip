@@ -58,6 +58,8 @@ struct Rect {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 __device__ __forceinline__ int iabs(int v) { return v < 0 ? -v : v; }
+// v << s on the two's-complement bits (Rust's i32 `<<`, which the dequantisation of negative levels uses)
+__device__ __forceinline__ int shl(int v, int s) { return (int)((uint32_t)v << s); }
 
 // ---- wave reductions -------------------------------------------------------------------------------
 
@@ -699,7 +701,7 @@ struct Coder {
       if (skip_dc && p == 0) continue;
       const int l = quant(c[p], IA_MF[m][class4(p)], f, qbits);
       lv[k] = (int16_t)l;
-      co[p] = (l * IA_V[m][class4(p)]) << (qp / 6);
+      co[p] = shl(l * IA_V[m][class4(p)], qp / 6);
     }
     return c[0];
   }
@@ -743,7 +745,7 @@ struct Coder {
       lv[k] = (int16_t)l;
       if (l != 0) {
         const int v = l * 16 * IA_DEQUANT8_COEFF_INIT[m][class8(p)];
-        co[p] = qp >= 36 ? v << (qp / 6 - 6) : (v + (1 << (5 - qp / 6))) >> (6 - qp / 6);
+        co[p] = qp >= 36 ? shl(v, qp / 6 - 6) : (v + (1 << (5 - qp / 6))) >> (6 - qp / 6);
       }
     }
   }
@@ -1058,7 +1060,7 @@ struct Coder {
       hadamard4(dl, ff);
       const int lsd = 16 * IA_V[m][0];
       for (int b = 0; b < 16; b++)
-        L.co[b][0] = qp >= 36 ? (ff[b] * lsd) << (qp / 6 - 6) : (ff[b] * lsd + (1 << (5 - qp / 6))) >> (6 - qp / 6);
+        L.co[b][0] = qp >= 36 ? shl(ff[b] * lsd, qp / 6 - 6) : (ff[b] * lsd + (1 << (5 - qp / 6))) >> (6 - qp / 6);
     }
     sync();
     for (int i = lane(); i < 256; i += 64) cyv(i & 15, i >> 4) = L.pred[i];
@@ -1111,7 +1113,7 @@ struct Coder {
       for (int ci = 0; ci < 2; ci++) {
         const int16_t *l = L.code.cdc[ci];
         const int f[4] = {l[0] + l[1] + l[2] + l[3], l[0] - l[1] + l[2] - l[3], l[0] + l[1] - l[2] - l[3], l[0] - l[1] - l[2] + l[3]};
-        for (int blk = 0; blk < 4; blk++) L.co[ci * 4 + blk][0] = cbpc > 0 ? ((f[blk] * lsd) << (qp / 6)) >> 5 : 0;
+        for (int blk = 0; blk < 4; blk++) L.co[ci * 4 + blk][0] = cbpc > 0 ? shl(f[blk] * lsd, qp / 6) >> 5 : 0;
       }
     }
     sync();

@@ -22,6 +22,8 @@ static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ?
 static inline int mini(int a, int b) { return a < b ? a : b; }
 static inline int maxi(int a, int b) { return a > b ? a : b; }
 static inline int absi(int a) { return a < 0 ? -a : a; }
+/* v << s on the two's-complement bits, as Rust's i32 `<<` shifts negative dequantised levels */
+static inline int shl(int v, int s) { return (int)((uint32_t)v << s); }
 
 /* ---------------------------------------------------------------------------------------------
  * Bit lengths and cost helpers (enc/bitw.rs:81-89, enc/frame.rs:164-171)
@@ -263,7 +265,7 @@ void or_quant8(const int32_t *src, const int32_t *pred, int qp, int intra, int32
     lv[k] = l;
     if (l != 0) {
       int v = l * 16 * OR_DEQUANT8_COEFF_INIT[m][class8(p)];
-      co[p] = qp >= 36 ? v << (qp / 6 - 6) : (v + (1 << (5 - qp / 6))) >> (6 - qp / 6);
+      co[p] = qp >= 36 ? shl(v, qp / 6 - 6) : (v + (1 << (5 - qp / 6))) >> (6 - qp / 6);
     }
   }
 }
@@ -1257,7 +1259,7 @@ static void quant_block(const int32_t *src, const int32_t *pred, int qp, int int
     if (skip_dc && p == 0) continue;
     int l = or_quant(c[p], OR_MF[m][class4(p)], f, qbits);
     lv[k] = l;
-    co[p] = (l * OR_V[m][class4(p)]) << (qp / 6);
+    co[p] = shl(l * OR_V[m][class4(p)], qp / 6);
   }
   *dc = c[0];
 }
@@ -1384,7 +1386,7 @@ static void code_i16(coder_t *c, int mbx, int mby, int mode, mbcode_t *code) {
   for (int b = 0; b < 16; b++) {
     int32_t co[16];
     memcpy(co, cos[b], sizeof co);
-    co[0] = qp >= 36 ? (ff[b] * lsd) << (qp / 6 - 6) : (ff[b] * lsd + (1 << (5 - qp / 6))) >> (6 - qp / 6);
+    co[0] = qp >= 36 ? shl(ff[b] * lsd, qp / 6 - 6) : (ff[b] * lsd + (1 << (5 - qp / 6))) >> (6 - qp / 6);
     if (count_nz(co, 16)) {
       int bx = b % 4, by = b / 4;
       or_idct4_add(co, c->pic[0] + (size_t)(y0 + by * 4) * ls + x0 + bx * 4, ls, maxv_(c));
@@ -1435,7 +1437,7 @@ static void code_chroma(coder_t *c, mbcode_t *code, int mbx, int mby, const uint
     for (int blk = 0; blk < 4; blk++) {
       int32_t co[16];
       memcpy(co, cos[ci][blk], sizeof co);
-      co[0] = cbpc > 0 ? ((f[blk] * lsd) << (qp / 6)) >> 5 : 0;
+      co[0] = cbpc > 0 ? shl(f[blk] * lsd, qp / 6) >> 5 : 0;
       code->cnnz[ci][blk] = (uint8_t)count_nz(code->cac[ci][blk], 16);
       if (count_nz(co, 16)) {
         int bx = blk & 1, by = blk >> 1;
