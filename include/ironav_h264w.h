@@ -1,0 +1,62 @@
+/* ironav_h264w.h — C ABI of the host side around the GPU analysis: the encoder driver's picture-type
+ * decisions and the bitstream back end that writes the analysed macroblocks.
+ *
+ * The GPU analysis (ironav_h264a.h) returns one `ia_mbcode` per macroblock. What nervosys/IronAV's
+ * encoder does around its slice coders stays on the CPU, as in the reference:
+ *  - the picture type of each input picture: first picture, keyint, forced, or a scene cut
+ *    (`H264Encoder::encode`, crates/ironav-codec/src/h264/enc/mod.rs:700-717, with `change_threads`,
+ *    crates/ironav-codec/src/change.rs:57-95);
+ *  - the bitstream: SPS, PPS, slice headers and CABAC slice data written from the macroblock codes
+ *    (`write_mb_cabac` and `mb_bins`, enc/frame.rs:2020-2279; `CabacWriter`, enc/cabac.rs:42-146;
+ *    `BitWriter` and NAL escaping, enc/bitw.rs; `sps`/`pps`/`slice_header`, enc/mod.rs:296-400).
+ *
+ * Conventions as in ironav_h264a.h: every function returns IW_OK (0) or a negative IW_E* code; the
+ * caller owns every buffer; an object is not thread safe. Pictures are 8-bit 4:2:0 padded to whole
+ * macroblocks (luma stride mb_w*16).
+ */
+#ifndef IRONAV_H264W_H
+#define IRONAV_H264W_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  IW_OK = 0,
+  IW_EINVAL = -1,   /* an argument is out of range */
+  IW_ENOMEM = -3,   /* an allocation failed */
+  IW_ESPACE = -6    /* the output buffer is too small */
+};
+
+/* ---- picture types (enc/mod.rs:700-717, change.rs) ---- */
+
+/* The low-resolution luma of a padded picture: the mean of each 4x4 block (rounded down), (mb_w*4) x
+ * (mb_h*4) values (`LowRes::new` as `H264Encoder::low_res` calls it, enc/mod.rs:686-692,
+ * change.rs:18-25). */
+int iw_lowres(const uint8_t *y, int mb_w, int mb_h, int32_t *out);
+
+/* How well `cur` is predicted from `prev` (two low-resolution pictures lw x lh): `inter` sums each
+ * 8x8 block's least SAD against `prev` over displacements within +-4, `intra` its absolute error
+ * against its own mean (`change_threads`, change.rs:57-89); the same sums for any `threads`. */
+int iw_change(const int32_t *prev, const int32_t *cur, int lw, int lh, int threads, int64_t *inter,
+              int64_t *intra);
+
+typedef struct iw_gop iw_gop;
+
+/* The picture-type state of one encoder: picture count, last IDR picture and the previous picture's
+ * low-resolution luma. keyint 0 takes the reference's default (250, enc/mod.rs:27). */
+int iw_gop_create(iw_gop **out, int width, int height, uint64_t keyint, int threads);
+void iw_gop_destroy(iw_gop *g);
+
+/* The type of the next input picture (padded luma y): *kind = 0 (IDR: first picture, keyint reached,
+ * `force_idr`, or a scene cut: inter > 0.8 x intra) or 1 (P). inter/intra (may be NULL) receive the
+ * scene-cut measure against the previous picture (0 for the first). */
+int iw_gop_next(iw_gop *g, const uint8_t *y, int force_idr, int *kind, int64_t *inter, int64_t *intra);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

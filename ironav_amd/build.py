@@ -56,6 +56,32 @@ def build(verbose=False):
     return OUT
 
 
+WRITER_OUT = os.path.join(HERE, "libironav_h264w.so")
+WRITER_SOURCES = ["gop.c"]
+
+
+def build_writer():
+    """The host side around the analysis (include/ironav_h264w.h): picture-type decisions and the
+    bitstream back end, plain C for the CPU, ironav_amd/libironav_h264w.so."""
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    cc = os.environ.get("CC", "gcc")
+    objs = []
+    for src in WRITER_SOURCES:
+        obj = os.path.join(objdir, src.replace(".c", ".w.o"))
+        p = subprocess.run([cc, "-std=c11", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter", "-pthread",
+                            "-c", os.path.join(CSRC, src), "-o", obj], capture_output=True, text=True)
+        if p.returncode != 0:
+            raise RuntimeError(f"{cc} failed on {src}:\n{p.stderr}")
+        objs.append(obj)
+    tmp = WRITER_OUT + ".tmp"
+    p = subprocess.run([cc, "-shared", "-pthread", "-o", tmp, *objs], capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError(f"link failed:\n{p.stderr}")
+    os.replace(tmp, WRITER_OUT)
+    return WRITER_OUT
+
+
 TRAIN_OUT = os.path.join(HERE, "libironav_train.so")
 
 
@@ -91,4 +117,5 @@ def build_probes():
 if __name__ == "__main__":
     print(build(verbose=True))
     print(build_train())
+    print(build_writer())
     print(build_probes())
