@@ -2,7 +2,7 @@
 the oracle.
 
 The kernel source (ironav_amd/csrc/analyse.hip) is compiled with g++ and tests/hostwave/hostwave.h,
-which runs each 64-lane workgroup as 64 host threads with the wave's cross-lane operations as
+which runs each 64-lane workgroup as 64 fibers on one host thread with the cross-lane operations as
 barriers. Every index the kernel computes into its private arrays, its LDS fields and the global
 buffers is checked: a sanitizer report or a diverged wave fails the run. The outputs must also be
 bit-exact with the oracle, so the emulation runs the same code paths the GPU runs.
