@@ -94,15 +94,24 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (the counters' scale for the traffic figures)")
-    ap.add_argument("--config", choices=["learned", "plain", "batched", "learned-batched", "deblock"], default="learned",
+    ap.add_argument("--config", choices=["learned", "plain", "batched", "learned-batched", "deblock", "search-loop",
+                                         "halfpel-loop", "issue-rates"], default="learned",
                     help="learned: the learned proposal vs the exhaustive search at 1080p (the metric); "
                          "plain: the exhaustive search + exact analysis at 1080p against the CPU oracle; "
                          "batched: many streams of mixed sizes through the batched analysis; "
                          "learned-batched: 2160p streams with large motion and scene cuts, the learned "
                          "proposal inside the batched step vs the exhaustive search; "
                          "deblock: the loop filter and reference promotion of a 1080p P picture against the "
-                         "CPU oracle's filter")
-    ap.add_argument("--slices", type=int, default=0, help="plain: slices per picture (0: one per macroblock row)")
+                         "CPU oracle's filter; search-loop: the 1080p picture's motion search (--mode) "
+                         "launched --launches times, for counter and trace runs; halfpel-loop: the half-sample "
+                         "planes of a 1080p reference built --launches times beside two torch kernels that "
+                         "move the same bytes; issue-rates: the VALU issue rates behind the SAD peak")
+    ap.add_argument("--mode", choices=["full", "learned", "learned-fallback"], default="full",
+                    help="search-loop: exhaustive search, the proposal alone (fallback off), or with the fallback")
+    ap.add_argument("--launches", type=int, default=5, help="search-loop, halfpel-loop: launches")
+    ap.add_argument("--slices", type=int, default=0,
+                    help="plain: slices per picture (0: one per macroblock row, or the live mode's rule with "
+                         "--realtime; -1: the default encoder's rule min(threads, mb_h/8))")
     ap.add_argument("--size", choices=["1080p", "720p"], default="1080p", help="plain: picture size")
     ap.add_argument("--search", choices=["full", "diamond"], default="full", help="plain: motion search")
     ap.add_argument("--threads", type=int, default=16, help="plain: host threads of the CPU oracle")
@@ -124,6 +133,12 @@ def main():
         return bench_plain(args)
     if args.config == "deblock":
         return bench_deblock(args)
+    if args.config == "search-loop":
+        return bench_search_loop(args)
+    if args.config == "halfpel-loop":
+        return bench_halfpel_loop(args)
+    if args.config == "issue-rates":
+        return bench_issue_rates(args)
 
     import numpy as np
     import torch
@@ -272,6 +287,112 @@ def main():
     print(json.dumps(out))
 
 
+def workload_references(pics, nref=3):
+    """Pictures 0-3 of the 1080p workload clip coded IPPP by the analysis (IDR 25, P 28, 68 slices,
+    exhaustive search): the references of picture 4, newest first, with the encoder's vectors."""
+    from ironav_amd import abi
+    from ironav_amd.h264a import Analyser
+
+    refs = []
+    with Analyser(1920, 1080, nrefs=nref, slices=68, search_mode=abi.SEARCH_FULL, search_range=32) as coder:
+        for t in range(4):
+            n = min(len(refs), nref)
+            for s in range(n):
+                coder.set_reference(s, refs[s][0], refs[s][1])
+            res = coder.analyse(pics[t], 25 if t == 0 else 28, abi.PIC_IDR if t == 0 else abi.PIC_P, list(range(n)))
+            refs.insert(0, (res.recon, res.mbs))
+    return refs
+
+
+def bench_search_loop(args):
+    """The motion search of the 1080p workload picture, launched --launches times for counter and
+    trace runs (rocprofv3 --pmc / --kernel-trace): the exhaustive search (--mode full, +-32), the
+    learned proposal with the fallback off (learned: proposal_kernel alone) or on at 1.1x
+    (learned-fallback: the proposal, then the exhaustive search of its fallback list)."""
+    import torch  # noqa: F401  (the HIP runtime torch loads is the one the library binds to)
+
+    from ironav_amd import abi, clipgen, proposal
+    from ironav_amd.h264a import Analyser, Picture
+
+    spec = clipgen.workload_1080p()
+    spec.frames = 5
+    pics = [Picture(*clipgen.pad_planes(*f)) for f in clipgen.make_clip(spec)]
+    refs = workload_references(pics)
+    if args.mode == "full":
+        an = Analyser(1920, 1080, nrefs=3, slices=68, search_mode=abi.SEARCH_FULL, search_range=args.range)
+        an.set_fullsearch_variant(args.variant)
+    else:
+        an = Analyser(1920, 1080, nrefs=3, slices=68, search_mode=abi.SEARCH_LEARNED, search_range=args.range,
+                      fallback_ratio=1.1 if args.mode == "learned-fallback" else float("inf"))
+        an.load_proposal_weights(proposal.load_weights())
+    with an:
+        for s in range(3):
+            an.set_reference(s, refs[s][0], refs[s][1])
+        an.upload(pics[4])
+        t0 = time.perf_counter()
+        for _ in range(args.launches):
+            an.motion_search([0, 1, 2])
+        an.sync()
+        dt = (time.perf_counter() - t0) * 1000.0 / args.launches
+    print(json.dumps({"config": "search-loop", "mode": args.mode, "launches": args.launches,
+                      "wall_ms_per_launch": round(dt, 4)}), flush=True)
+
+
+def bench_halfpel_loop(args):
+    """The half-sample planes of one 1920x1080 reference, built --launches times (each set_reference
+    uploads the luma and launches halfpel_kernel once); beside them two torch kernels that move the
+    same bytes give the floors at this size: a device copy of 5.82 MB (11.6 MB of traffic, the
+    kernel's algorithmic bytes) and a fill of the four planes (9.55 MB written)."""
+    import torch
+
+    from ironav_amd import clipgen
+    from ironav_amd.h264a import Analyser, Picture
+
+    w, h = 1920, 1080
+    pic = Picture(*clipgen.pad_planes(*next(iter(clipgen.make_clip(clipgen.ClipSpec(w, h, 1, 21))))))
+    with Analyser(w, h, nrefs=1) as an:
+        for _ in range(args.launches):
+            an.set_reference(0, pic)
+        an.sync()
+    pw, ph = pic.y.shape[1], pic.y.shape[0]
+    planes = 4 * (pw + 96) * (ph + 96)
+    src = torch.empty((pw * ph + planes) // 2, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    fill = torch.empty(planes, dtype=torch.uint8, device="cuda")
+    for _ in range(args.launches):
+        dst.copy_(src)
+        fill.fill_(7)
+    torch.cuda.synchronize()
+    print(json.dumps({"config": "halfpel-loop", "launches": args.launches, "copy_bytes": src.numel(),
+                      "fill_bytes": planes}), flush=True)
+
+
+def bench_issue_rates(args):
+    """The VALU issue rates that bound the SAD kernels (ironav_amd/csrc/issue_rates.hip): lane-ops per
+    second and per CU-clock at 2.4 GHz for v_sad_u8, v_sad_hi_u8, the quad-SAD forms, v_min3_u32 and
+    v_add_u32; 4 absolute differences per v_sad lane-op give the SAD peak."""
+    import ctypes as C
+
+    import torch  # noqa: F401
+
+    lib = C.CDLL(os.path.join(ROOT, "ironav_amd", "libironav_probe.so"))
+    lib.ia_probe_issue_rates.restype = C.c_int
+    lib.ia_probe_issue_rates.argtypes = [C.POINTER(C.c_double), C.c_int]
+    lib.ia_probe_issue_rate_name.restype = C.c_char_p
+    lib.ia_probe_issue_rate_name.argtypes = [C.c_int]
+    out = (C.c_double * 8)()
+    n = lib.ia_probe_issue_rates(out, 8)
+    if n < 0:
+        raise RuntimeError("issue-rate measurement failed")
+    rates = {}
+    for k in range(n):
+        rates[lib.ia_probe_issue_rate_name(k).decode()] = {
+            "G_lane_ops_per_s": round(out[k] / 1e9, 1),
+            "per_CU_clock": round(out[k] / MI355X_CU / (MI355X_CLK_GHZ * 1e9), 3)}
+    print(json.dumps({"config": "issue-rates", "rates": rates,
+                      "sad_peak_T_absdiff_per_s": round(VALU_ABSDIFF_PEAK_T, 1)}), flush=True)
+
+
 def digest(res):
     import hashlib
 
@@ -294,7 +415,7 @@ def bench_plain(args):
     import torch
 
     from ironav_amd import abi, clipgen
-    from ironav_amd.h264a import Analyser, Picture
+    from ironav_amd.h264a import Analyser, Picture, realtime
 
     big = args.size == "1080p"
     width, height, nref, qp = (1920, 1080, 3, 28) if big else (1280, 720, 3, 28)
@@ -303,7 +424,16 @@ def bench_plain(args):
     rng = 32 if big else 16
     full = args.search == "full"
     mode = abi.SEARCH_FULL if full else abi.SEARCH_DIAMOND
-    slices = args.slices or (height + 15) // 16
+    mb_h = (height + 15) // 16
+    if args.slices > 0:
+        slices = args.slices
+    elif args.realtime:
+        # the live mode's own slicing: a slice per host thread, up to one per two rows (enc/mod.rs:252)
+        slices = realtime(height, args.threads)["slices"]
+    elif args.slices < 0:
+        slices = max(1, min(args.threads, mb_h // 8))  # the default encoder's rule (enc/mod.rs:192)
+    else:
+        slices = mb_h  # one slice per macroblock row
     spec = clipgen.workload_1080p() if big else clipgen.workload_720p()
     spec.frames = 5
     pics = [Picture(*clipgen.pad_planes(*f)) for f in clipgen.make_clip(spec)]

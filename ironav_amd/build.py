@@ -99,17 +99,20 @@ def build_train():
     return TRAIN_OUT
 
 
-PROBE_SRC = os.path.join(HERE, "probes", "issue_rates.hip")
-PROBE_OUT = os.path.join(HERE, "probes", "issue_rates")
+PROBE_OUT = os.path.join(HERE, "libironav_probe.so")
 
 
-def build_probes():
-    """The issue-rate probe (ironav_amd/probes/issue_rates): the VALU rates behind the SAD rooflines."""
+def build_probe():
+    """The VALU issue-rate measurement behind the SAD rooflines (`bench.py --config issue-rates`),
+    ironav_amd/libironav_probe.so."""
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    obj = os.path.join(objdir, "issue_rates.o")
+    _compile("issue_rates.hip", obj)
     tmp = PROBE_OUT + ".tmp"
-    p = subprocess.run([HIPCC, *[f for f in FLAGS if f not in ("-fPIC",)], PROBE_SRC, "-o", tmp],
-                       capture_output=True, text=True)
+    p = subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, obj], capture_output=True, text=True)
     if p.returncode != 0:
-        raise RuntimeError(f"hipcc failed on {PROBE_SRC}:\n{p.stderr}")
+        raise RuntimeError(f"link failed:\n{p.stderr}")
     os.replace(tmp, PROBE_OUT)
     return PROBE_OUT
 
@@ -118,4 +121,4 @@ if __name__ == "__main__":
     print(build(verbose=True))
     print(build_train())
     print(build_writer())
-    print(build_probes())
+    print(build_probe())

@@ -433,7 +433,11 @@ int ia_batch_sync(ia_batch *b) {
   int stalled = 0;
   IA_BHIP(b, hipMemcpy(&stalled, b->d_dberr, sizeof(int), hipMemcpyDeviceToHost));
   if (stalled) {
+    // A row that gave up waiting cleared the counter of the row above it, which may have published
+    // later: reset every stream's row counters (outside the graph, the step is over) so that the next
+    // step's rows wait as they should, as the single-context promotion does after a stall.
     IA_BHIP(b, hipMemset(b->d_dberr, 0, sizeof(int)));
+    for (ia_ctx *c : b->ctx) IA_BHIP(b, hipMemset(c->db, 0, (1 + (size_t)c->g.mb_h) * sizeof(int)));
     return bfail(b, IA_EDEVICE, "the loop filter's rows stalled");
   }
   return IA_OK;

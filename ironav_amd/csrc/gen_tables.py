@@ -12,7 +12,17 @@ encoder's forward quantisation multipliers), so they are read from its sources r
     crates/ironav-codec/src/h264/mb.rs       CHROMA_QP
     crates/ironav-codec/src/h264/enc/frame.rs MF, V, MF8
 
-Usage: python gen_tables.py <reference-root> <out.h> <PREFIX>
+With a fourth argument `decoder` it writes the tables of the oracle's decoder instead (all three
+CABAC_INIT_PB sets, the 4x4 dequantisation scales, the default scaling lists and the scans):
+
+    crates/ironav-codec/src/h264/tables.rs   CABAC_INIT_I, CABAC_INIT_PB, SIG_OFF, LAST_OFF, ABS_OFF,
+                                             SIG_OFF_8X8, LAST_OFF_8X8, LPS_RANGE, DEQUANT4_COEFF_INIT,
+                                             DEQUANT8_COEFF_INIT, DEQUANT8_COEFF_INIT_SCAN, DEFAULT_SCALING4,
+                                             DEFAULT_SCALING8, ZIGZAG_DIRECT, ALPHA, BETA, TC0
+    crates/ironav-codec/src/h264/cabac.rs    TRANS_LPS
+    crates/ironav-codec/src/h264/ps.rs       ZIGZAG4
+
+Usage: python gen_tables.py <reference-root> <out.h> <PREFIX> [decoder]
 The generated headers are committed; this script only reruns when the tables must be regenerated.
 """
 import ast
@@ -28,6 +38,16 @@ WANT = {
     "crates/ironav-codec/src/h264/ps.rs": ["ZIGZAG4"],
     "crates/ironav-codec/src/h264/mb.rs": ["CHROMA_QP"],
     "crates/ironav-codec/src/h264/enc/frame.rs": ["MF", "V", "MF8"],
+}
+
+WANT_DECODER = {
+    "crates/ironav-codec/src/h264/tables.rs": [
+        "CABAC_INIT_I", "CABAC_INIT_PB", "SIG_OFF", "LAST_OFF", "ABS_OFF", "SIG_OFF_8X8", "LAST_OFF_8X8",
+        "LPS_RANGE", "DEQUANT4_COEFF_INIT", "DEQUANT8_COEFF_INIT", "DEQUANT8_COEFF_INIT_SCAN", "DEFAULT_SCALING4",
+        "DEFAULT_SCALING8", "ZIGZAG_DIRECT", "ALPHA", "BETA", "TC0",
+    ],
+    "crates/ironav-codec/src/h264/cabac.rs": ["TRANS_LPS"],
+    "crates/ironav-codec/src/h264/ps.rs": ["ZIGZAG4"],
 }
 
 CTYPE = {"u8": "uint8_t", "i8": "int8_t", "u16": "uint16_t", "i32": "int32_t"}
@@ -75,6 +95,7 @@ def emit(name, base, value, prefix):
 
 def main():
     root, out, prefix = sys.argv[1], sys.argv[2], sys.argv[3]
+    decoder = len(sys.argv) > 4 and sys.argv[4] == "decoder"
     parts = [
         "/* Generated by ironav_amd/csrc/gen_tables.py from nervosys/IronAV's h264 sources: numeric tables of",
         " * ITU-T H.264 (and the encoder's forward quantisation multipliers) that must match bit for bit. */",
@@ -83,11 +104,11 @@ def main():
         "#include <stdint.h>",
         "",
     ]
-    for rel, names in WANT.items():
+    for rel, names in (WANT_DECODER if decoder else WANT).items():
         text = open(f"{root}/{rel}").read()
         for n in names:
             base, value = parse(text, n)
-            if n == "CABAC_INIT_PB":
+            if n == "CABAC_INIT_PB" and not decoder:
                 value = value[0]  # cabac_init_idc 0 is the only one the encoder uses
                 n = "CABAC_INIT_P"
             parts.append(emit(n, base, value, prefix))

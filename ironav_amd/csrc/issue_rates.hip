@@ -1,7 +1,8 @@
-// issue_rates.hip — the VALU issue rates that bound the SAD kernels, measured on the device.
+// issue_rates.hip — the VALU issue rates that bound the SAD kernels, measured on the device
+// (libironav_probe.so, run by `bench.py --config issue-rates`).
 //
 // Each kernel runs 8 independent chains of one instruction per lane in a long loop over 4096
-// workgroups of 256 threads and prints lane-operations per second and per CU-clock (at 2.4 GHz).
+// workgroups of 256 threads; the result is lane-operations per second.
 // A full-rate wave64 VALU instruction occupies a 16-lane SIMD for 4 clocks, i.e. 64 lane-ops per
 // CU-clock over the 4 SIMDs; v_sad_u8 does 4 absolute differences per lane-op and v_qsad_pk_u16_u8 /
 // v_mqsad_u32_u8 16 (4 adjacent 4-sample SADs).
@@ -89,26 +90,35 @@ __global__ void k_add(uint32_t *out, uint32_t s) {
 }
 
 typedef void (*kfn)(uint32_t *, uint32_t);
-int main() {
+
+// The issue rate of each probed instruction in lane-operations per second, in the order of
+// ia_probe_issue_rate_name: v_sad_u8, v_sad_hi_u8, v_qsad_pk_u16_u8, v_mqsad_u32_u8, v_min3_u32,
+// v_add_u32. Returns the count written (at most n), or -1 if a HIP call failed.
+extern "C" int ia_probe_issue_rates(double *out, int n) {
   const int blocks = 256 * 16, threads = 256;
-  uint32_t *out;
-  if (hipMalloc(&out, (size_t)blocks * threads * 4) != hipSuccess) return 1;  // one word per thread
-  struct { const char *n; kfn f; } ks[] = {{"v_sad_u8", k_sad}, {"v_sad_hi_u8", k_sadhi}, {"v_qsad_pk_u16_u8", k_qsad},
-                                           {"v_mqsad_u32_u8", k_mqsad}, {"v_min3_u32", k_min3}, {"v_add_u32", k_add}};
+  const kfn ks[] = {k_sad, k_sadhi, k_qsad, k_mqsad, k_min3, k_add};
+  const int nk = (int)(sizeof ks / sizeof ks[0]);
+  uint32_t *buf;
+  if (hipMalloc(&buf, (size_t)blocks * threads * 4) != hipSuccess) return -1;  // one word per thread
   hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return 1;
-  for (auto &k : ks) {
-    hipLaunchKernelGGL(k.f, dim3(blocks), dim3(threads), 0, 0, out, 12345u);
-    (void)hipEventRecord(e0, 0);
-    for (int rep = 0; rep < 5; rep++) hipLaunchKernelGGL(k.f, dim3(blocks), dim3(threads), 0, 0, out, 777u + rep);
-    (void)hipEventRecord(e1, 0);
-    if (hipEventSynchronize(e1) != hipSuccess) return 1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return -1;
+  int w = 0;
+  for (int k = 0; k < nk && w < n; k++) {
+    hipLaunchKernelGGL(ks[k], dim3(blocks), dim3(threads), 0, 0, buf, 12345u);
+    if (hipEventRecord(e0, 0) != hipSuccess) return -1;
+    for (int rep = 0; rep < 5; rep++) hipLaunchKernelGGL(ks[k], dim3(blocks), dim3(threads), 0, 0, buf, 777u + rep);
+    if (hipEventRecord(e1, 0) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return -1;
     float ms = 0;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    const double ops = 5.0 * blocks * threads * (double)ITERS * CH;
-    printf("%-18s %8.3f ms  %8.1f G lane-ops/s  (%.3f per CU-clock at 2.4 GHz)\n", k.n, ms, ops / (ms * 1e6),
-           ops / (ms * 1e-3) / 256 / 2.4e9);
+    if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1;
+    out[w++] = 5.0 * blocks * threads * (double)ITERS * CH / (ms * 1e-3);
   }
-  (void)hipFree(out);
-  return 0;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(buf);
+  return w;
+}
+
+extern "C" const char *ia_probe_issue_rate_name(int k) {
+  static const char *names[] = {"v_sad_u8", "v_sad_hi_u8", "v_qsad_pk_u16_u8", "v_mqsad_u32_u8", "v_min3_u32", "v_add_u32"};
+  return k >= 0 && k < 6 ? names[k] : nullptr;
 }
