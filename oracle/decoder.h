@@ -1,0 +1,62 @@
+/* decoder.h — the verifier: a C restatement of nervosys/IronAV's H.264 decoder for the subset the
+ * encoder's streams use.
+ *
+ * TEST INFRASTRUCTURE ONLY. Nothing in the product (ironav_amd/) includes, links or calls this code;
+ * tests/ (and bench.py's rate measurement) load it to check that bitstreams decode to the pictures the
+ * encoder reconstructed (crates/ironav-codec/src/h264/enc/mod.rs:812-825 is the reference's test of
+ * the same property) and to pin the oracle's reconstruction primitives against the reference's
+ * decoder fixtures (libx264 streams with FFmpeg's decoded pictures,
+ * crates/ironav-engine/tests/data/h264_*, crates/ironav-engine/tests/e2e.rs:1991-2036).
+ *
+ * Restated from the reference decoder: crates/ironav-codec/src/h264/mod.rs (NAL dispatch, picture
+ * boundaries), ps.rs (SPS, PPS, slice header), bits.rs (emulation prevention, Exp-Golomb),
+ * cabac.rs (the arithmetic decoder), mb.rs (CABAC macroblock layer, motion vector prediction,
+ * residual, intra and inter reconstruction), dpb.rs (order counts, reference lists, marking),
+ * deblock.rs (through the oracle's loop filter). The subset: progressive frames, 8-bit 4:2:0,
+ * I and P slices, CABAC, the 8x8 transform, scaling matrices, explicit weighted prediction, I_PCM,
+ * several slices and reference pictures. Anything else (CAVLC, B slices, MBAFF, field pictures,
+ * other chroma formats or bit depths) is rejected with OD_EUNSUPPORTED.
+ */
+#ifndef IRONAV_ORACLE_DECODER_H
+#define IRONAV_ORACLE_DECODER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../include/ironav_h264a.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OD_OK = 0, OD_EINVAL = -1, OD_EBITSTREAM = -2, OD_EUNSUPPORTED = -3, OD_ENOMEM = -4 };
+
+typedef struct od_dec od_dec;
+
+od_dec *od_create(void);
+void od_destroy(od_dec *d);
+/* The text of the last error. */
+const char *od_error(const od_dec *d);
+
+/* One NAL unit (header byte first, emulation prevention bytes still in). */
+int od_decode_nal(od_dec *d, const uint8_t *nal, size_t len);
+/* A byte stream of NAL units with start codes (Annex B). */
+int od_decode_annexb(od_dec *d, const uint8_t *data, size_t len);
+/* Finishes the picture being decoded; after it every decoded picture is available. */
+int od_flush(od_dec *d);
+
+/* Decoded pictures, in output order (order count within each IDR period) after od_flush. */
+int od_frame_count(const od_dec *d);
+/* Cropped size of frame k and its size in macroblocks. */
+int od_frame_size(const od_dec *d, int k, int *width, int *height, int *mb_w, int *mb_h);
+/* The cropped planes of frame k (y: width x height, cb/cr: ceil(width/2) x ceil(height/2)). */
+int od_frame_read(const od_dec *d, int k, uint8_t *y, uint8_t *cb, uint8_t *cr);
+/* The whole planes of frame k (mb_w*16 x mb_h*16 luma, half that chroma), loop-filtered, and its
+ * macroblock records (mb_w*mb_h; kind, qp, cbp, skip, t8x8, nnz, vectors and references); any
+ * pointer may be NULL. */
+int od_frame_read_full(const od_dec *d, int k, uint8_t *y, uint8_t *cb, uint8_t *cr, ia_mbinfo *mbs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

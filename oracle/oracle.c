@@ -2329,8 +2329,10 @@ int or_analyse_picture(const ia_config *cfg, const uint8_t *y, const uint8_t *cb
 /* ---------------------------------------------------------------------------------------------
  * The loop filter as the encoder runs it on each reconstructed picture before it becomes a reference
  * (enc/mod.rs:653-654): frame macroblocks (no MBAFF), 4:2:0, 8-bit samples, every slice with
- * disable_deblocking_filter_idc 0, zero alpha/beta offsets and zero chroma QP offset, no I_PCM or
- * lossless macroblocks. Follows h264/deblock.rs:232-421 and :608-695.
+ * disable_deblocking_filter_idc 0, zero alpha/beta offsets and zero chroma QP offset; with per-slice
+ * settings (the decoder's pictures) any idc, offsets and chroma QP offsets. I_PCM macroblocks carry
+ * QP 0 in their records; lossless macroblocks are not handled. Follows h264/deblock.rs:232-421 and
+ * :608-695.
  * ------------------------------------------------------------------------------------------- */
 
 /* Whether the transform block holding 4x4 block (bx, by) has coefficients (h264/deblock.rs:336-344). */
@@ -2444,8 +2446,8 @@ static void db_filter_line(uint8_t *pl, ptrdiff_t q, ptrdiff_t xs, int b, int tc
 /* An edge of segments bs[0..3] of `per_bs` lines each, starting at pos, lines ys apart
  * (h264/deblock.rs:232-276 without I_PCM restoring). */
 static void db_filter_edge(uint8_t *pl, ptrdiff_t pos, ptrdiff_t xs, ptrdiff_t ys, int per_bs, const uint8_t *bs,
-                           int qpav, int chroma) {
-  const int ia = clampi(qpav, 0, 51), ib = clampi(qpav, 0, 51);
+                           int qpav, int chroma, int alpha_offset, int beta_offset) {
+  const int ia = clampi(qpav + alpha_offset, 0, 51), ib = clampi(qpav + beta_offset, 0, 51);
   const int alpha = OR_ALPHA[ia], beta = OR_BETA[ib];
   if (alpha == 0 || beta == 0) return;
   for (int s = 0; s < 4; s++) {
@@ -2456,13 +2458,19 @@ static void db_filter_edge(uint8_t *pl, ptrdiff_t pos, ptrdiff_t xs, ptrdiff_t y
 }
 
 /* Macroblock (mbx, mby): strengths once, then luma and chroma edges (h264/deblock.rs:608-695). */
-static void db_mb(uint8_t *y, uint8_t *cb, uint8_t *cr, const ia_mbinfo *mbs, int mb_w, int mbx, int mby) {
+static void db_mb(uint8_t *y, uint8_t *cb, uint8_t *cr, const ia_mbinfo *mbs, int mb_w, int mbx, int mby,
+                  const or_slice_filter *sp) {
   const ptrdiff_t ls = (ptrdiff_t)mb_w * 16, cs = (ptrdiff_t)mb_w * 8;
   const int idx = mby * mb_w + mbx;
   const ia_mbinfo *m = &mbs[idx];
   if (m->slice == 0) return;
-  /* every slice filters across slice edges (disable_deblocking_filter_idc 0) */
-  const int left_ok = mbx > 0 && mbs[idx - 1].slice != 0, top_ok = mby > 0 && mbs[idx - mb_w].slice != 0;
+  /* the filter settings of the slice holding this macroblock (h264/deblock.rs:608-620): idc 1 leaves
+     its edges alone, idc 2 keeps to edges inside the slice, idc 0 filters across slice edges too */
+  const or_slice_filter zero = {0, 0, 0, {0, 0}};
+  const or_slice_filter *f = sp ? &sp[m->slice - 1] : &zero;
+  if (f->disable_idc == 1) return;
+  const int left_ok = mbx > 0 && mbs[idx - 1].slice != 0 && (f->disable_idc != 2 || mbs[idx - 1].slice == m->slice);
+  const int top_ok = mby > 0 && mbs[idx - mb_w].slice != 0 && (f->disable_idc != 2 || mbs[idx - mb_w].slice == m->slice);
   const ia_mbinfo *edge_mb[2] = {mbx > 0 ? &mbs[idx - 1] : m, mby > 0 ? &mbs[idx - mb_w] : m};
   uint8_t bs[2][4][4];
   memset(bs, 0, sizeof bs);
@@ -2480,8 +2488,8 @@ static void db_mb(uint8_t *y, uint8_t *cb, uint8_t *cr, const ia_mbinfo *mbs, in
       if ((m->t8x8 && (e & 1)) || !(bs[dir][e][0] | bs[dir][e][1] | bs[dir][e][2] | bs[dir][e][3])) continue;
       const ia_mbinfo *o = e == 0 ? edge_mb[dir] : m;
       const int qpav = (o->qp + m->qp + 1) >> 1;
-      if (dir == 0) db_filter_edge(y, y0 * ls + x0 + e * 4, 1, ls, 4, bs[dir][e], qpav, 0);
-      else db_filter_edge(y, (y0 + e * 4) * ls + x0, ls, 1, 4, bs[dir][e], qpav, 0);
+      if (dir == 0) db_filter_edge(y, y0 * ls + x0 + e * 4, 1, ls, 4, bs[dir][e], qpav, 0, f->alpha_offset, f->beta_offset);
+      else db_filter_edge(y, (y0 + e * 4) * ls + x0, ls, 1, 4, bs[dir][e], qpav, 0, f->alpha_offset, f->beta_offset);
     }
   /* chroma 4:2:0: vertical edges 0 and 4, horizontal edges 0 and 4 (luma edges 0 and 2) */
   uint8_t *cpl[2] = {cb, cr};
@@ -2491,16 +2499,24 @@ static void db_mb(uint8_t *y, uint8_t *cb, uint8_t *cr, const ia_mbinfo *mbs, in
         if ((e == 0 && !(dir == 0 ? left_ok : top_ok)) || !(bs[dir][e][0] | bs[dir][e][1] | bs[dir][e][2] | bs[dir][e][3]))
           continue;
         const ia_mbinfo *o = e == 0 ? edge_mb[dir] : m;
-        const int qpav = (chroma_qp(o->qp, 0, 8) + chroma_qp(m->qp, 0, 8) + 1) >> 1;
+        const int off = f->chroma_qp_offset[comp];
+        const int qpav = (chroma_qp(o->qp, off, 8) + chroma_qp(m->qp, off, 8) + 1) >> 1;
         const int cx0 = mbx * 8, cy0 = mby * 8;
-        if (dir == 0) db_filter_edge(cpl[comp], cy0 * cs + cx0 + e * 2, 1, cs, 2, bs[dir][e], qpav, 1);
-        else db_filter_edge(cpl[comp], (cy0 + e * 2) * cs + cx0, cs, 1, 2, bs[dir][e], qpav, 1);
+        if (dir == 0)
+          db_filter_edge(cpl[comp], cy0 * cs + cx0 + e * 2, 1, cs, 2, bs[dir][e], qpav, 1, f->alpha_offset, f->beta_offset);
+        else
+          db_filter_edge(cpl[comp], (cy0 + e * 2) * cs + cx0, cs, 1, 2, bs[dir][e], qpav, 1, f->alpha_offset, f->beta_offset);
       }
 }
 
 void or_deblock_picture(uint8_t *y, uint8_t *cb, uint8_t *cr, const ia_mbinfo *mbs, int mb_w, int mb_h) {
+  or_deblock_picture_params(y, cb, cr, mbs, mb_w, mb_h, NULL);
+}
+
+void or_deblock_picture_params(uint8_t *y, uint8_t *cb, uint8_t *cr, const ia_mbinfo *mbs, int mb_w, int mb_h,
+                               const or_slice_filter *sp) {
   for (int mby = 0; mby < mb_h; mby++)
-    for (int mbx = 0; mbx < mb_w; mbx++) db_mb(y, cb, cr, mbs, mb_w, mbx, mby);
+    for (int mbx = 0; mbx < mb_w; mbx++) db_mb(y, cb, cr, mbs, mb_w, mbx, mby, sp);
 }
 
 /* Sizes of the ABI structs as this compiler lays them out, for the binding tests. */

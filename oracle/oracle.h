@@ -85,6 +85,13 @@ void or_fullsearch(const uint8_t *src_y, const uint8_t *const *ref_y, int nref, 
  * filtering across slice edges with zero offsets. In place on padded planes y (stride mb_w*16),
  * cb and cr (stride mb_w*8), from the picture's macroblock records. */
 void or_deblock_picture(uint8_t *y, uint8_t *cb, uint8_t *cr, const ia_mbinfo *mbs, int mb_w, int mb_h);
+/* The same with the settings of each slice (indexed by ia_mbinfo.slice - 1): disable_deblocking_filter_idc,
+ * FilterOffsetA/B (slice_alpha/beta_c0_offset_div2 * 2) and the chroma QP offsets of Cb and Cr. */
+typedef struct or_slice_filter {
+  int disable_idc, alpha_offset, beta_offset, chroma_qp_offset[2];
+} or_slice_filter;
+void or_deblock_picture_params(uint8_t *y, uint8_t *cb, uint8_t *cr, const ia_mbinfo *mbs, int mb_w, int mb_h,
+                               const or_slice_filter *sp);
 
 /* sizeof of ia_config, ia_mbinfo, ia_mbcode, ia_mbstats and ia_fsresult, for the binding tests. */
 void or_abi_sizes(int32_t *out);
