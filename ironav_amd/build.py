@@ -57,7 +57,7 @@ def build(verbose=False):
 
 
 WRITER_OUT = os.path.join(HERE, "libironav_h264w.so")
-WRITER_SOURCES = ["gop.c"]
+WRITER_SOURCES = ["gop.c", "writer.c"]
 
 
 def build_writer():

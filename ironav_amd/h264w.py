@@ -1,6 +1,10 @@
 """Host side around the GPU analysis: the encoder driver's picture-type decisions and the bitstream
 back end (include/ironav_h264w.h, ironav_amd/libironav_h264w.so).
 
+`Stream` writes each analysed picture (the `ia_mbcode`s of ia_read_analysis) as an Annex B access
+unit, as the reference encoder's `code_picture` writes its slices with `write_mb_cabac`
+(crates/ironav-codec/src/h264/enc/mod.rs:539-652, enc/frame.rs:2020-2279).
+
 `Gop` restates the picture-type rule of nervosys/IronAV's `H264Encoder::encode`
 (crates/ironav-codec/src/h264/enc/mod.rs:700-717): IDR for the first picture, after `keyint`
 pictures, when forced, or at a scene cut measured by `change_threads`
@@ -16,7 +20,12 @@ from . import abi
 _LIB = None
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libironav_h264w.so")
 
-IW_OK, IW_EINVAL, IW_ENOMEM, IW_ESPACE = 0, -1, -3, -6
+IW_OK, IW_EINVAL, IW_ENOMEM, IW_EUNSUPPORTED, IW_ESPACE = 0, -1, -3, -5, -6
+
+
+class Params(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("fps_num", C.c_int32), ("fps_den", C.c_int32),
+                ("cabac", C.c_int32), ("transform8", C.c_int32), ("nrefs", C.c_int32), ("slices", C.c_int32)]
 
 
 def library():
@@ -33,6 +42,13 @@ def library():
             "iw_gop_create": (i, [C.POINTER(vp), i, i, C.c_uint64, i]),
             "iw_gop_destroy": (None, [vp]),
             "iw_gop_next": (i, [vp, vp, i, C.POINTER(C.c_int), i64p, i64p]),
+            "iw_stream_create": (i, [C.POINTER(vp), C.POINTER(Params)]),
+            "iw_stream_destroy": (None, [vp]),
+            "iw_write_picture": (i, [vp, i, i, i, vp, C.POINTER(C.c_size_t)]),
+            "iw_read_picture": (i, [vp, vp, C.c_size_t]),
+            "iw_level": (i, [i, i, i, i]),
+            "iw_put_nal": (i, [i, i, vp, C.c_size_t, vp, C.c_size_t, C.POINTER(C.c_size_t)]),
+            "iw_cabac_encode_bins": (i, [vp, vp, vp, i, i, i, vp, C.c_size_t, C.POINTER(C.c_size_t)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -46,6 +62,8 @@ def _check(rc):
         return
     if rc == IW_EINVAL:
         raise ValueError("invalid argument")
+    if rc == IW_EUNSUPPORTED:
+        raise NotImplementedError("the bitstream back end writes CABAC streams only")
     raise RuntimeError(f"ironav_h264w error {rc}")
 
 
@@ -98,3 +116,70 @@ class Gop:
         k = C.c_int()
         _check(library().iw_gop_next(self._h, abi.ptr(y), int(bool(force_idr)), C.byref(k), None, None))
         return abi.PIC_IDR if k.value == 0 else abi.PIC_P
+
+
+def level(width, height, fps_num=0, fps_den=0):
+    """level_idc the SPS carries for this size and rate."""
+    return int(library().iw_level(width, height, fps_num, fps_den))
+
+
+def put_nal(ref_idc, nal_type, rbsp):
+    """A NAL unit with its start code and emulation prevention bytes."""
+    rbsp = bytes(rbsp)
+    n = C.c_size_t()
+    _check(library().iw_put_nal(ref_idc, nal_type, rbsp, len(rbsp), None, 0, C.byref(n)))
+    out = C.create_string_buffer(n.value)
+    _check(library().iw_put_nal(ref_idc, nal_type, rbsp, len(rbsp), out, n.value, C.byref(n)))
+    return out.raw[:n.value]
+
+
+def cabac_encode_bins(kinds, ctx, vals, intra, qp):
+    """The CABAC coder alone: bins (kind 0 decision, 1 bypass, 2 terminate 0) then a terminate 1."""
+    kinds = np.ascontiguousarray(kinds, dtype=np.uint8)
+    ctx = np.ascontiguousarray(ctx, dtype=np.uint16)
+    vals = np.ascontiguousarray(vals, dtype=np.uint8)
+    n = C.c_size_t()
+    L = library()
+    _check(L.iw_cabac_encode_bins(abi.ptr(kinds), abi.ptr(ctx), abi.ptr(vals), len(kinds), int(intra), qp, None, 0,
+                                  C.byref(n)))
+    out = C.create_string_buffer(max(1, n.value))
+    _check(L.iw_cabac_encode_bins(abi.ptr(kinds), abi.ptr(ctx), abi.ptr(vals), len(kinds), int(intra), qp, out,
+                                  n.value, C.byref(n)))
+    return out.raw[:n.value]
+
+
+class Stream:
+    """One IPPP output stream: each analysed picture written as an Annex B access unit."""
+
+    def __init__(self, width, height, *, nrefs=3, slices=1, transform8=True, cabac=True, fps=(0, 0)):
+        p = Params(width, height, fps[0], fps[1], int(cabac), int(transform8), nrefs, slices)
+        h = C.c_void_p()
+        _check(library().iw_stream_create(C.byref(h), C.byref(p)))
+        self._h = h
+        self.mb_w, self.mb_h = (width + 15) // 16, (height + 15) // 16
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().iw_stream_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def write(self, kind, qp, codes, nref=0):
+        """The access unit of a picture of `kind` (abi.PIC_IDR / abi.PIC_P against `nref` references)."""
+        codes = np.ascontiguousarray(codes, dtype=abi.MBCODE_DTYPE)
+        if codes.shape != (self.mb_w * self.mb_h,):
+            raise ValueError("one code per macroblock")
+        n = C.c_size_t()
+        L = library()
+        _check(L.iw_write_picture(self._h, 0 if kind == abi.PIC_IDR else 1, qp, nref, abi.ptr(codes), C.byref(n)))
+        out = C.create_string_buffer(max(1, n.value))
+        _check(L.iw_read_picture(self._h, out, n.value))
+        return out.raw[:n.value]

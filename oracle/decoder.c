@@ -1787,3 +1787,27 @@ int od_frame_read_full(const od_dec *d, int k, uint8_t *y, uint8_t *cb, uint8_t 
   if (mbs) memcpy(mbs, f->mbs, nmb * sizeof(ia_mbinfo));
   return OD_OK;
 }
+
+int od_cabac_decode_bins(const uint8_t *data, size_t n, const uint8_t *kinds, const uint16_t *ctx, int count, int intra,
+                         int qp, uint8_t *vals, int *final_terminate, size_t *bytes_read) {
+  if (!data || !kinds || !ctx || !vals || count < 0) return OD_EINVAL;
+  br_t b = {data, n, 0};
+  cabac_t c;
+  memset(&c, 0, sizeof c);
+  c.b = &b;
+  cabac_init_contexts(&c, intra, 0, qp);
+  cabac_init_engine(&c);
+  for (int i = 0; i < count; i++) {
+    if (kinds[i] == 0) {
+      if (ctx[i] >= 1024) return OD_EINVAL;
+      vals[i] = (uint8_t)decision(&c, ctx[i]);
+    } else if (kinds[i] == 1) {
+      vals[i] = (uint8_t)bypass(&c);
+    } else {
+      vals[i] = (uint8_t)terminate(&c);
+    }
+  }
+  if (final_terminate) *final_terminate = terminate(&c);
+  if (bytes_read) *bytes_read = (b.pos + 7) / 8;
+  return OD_OK;
+}

@@ -56,6 +56,12 @@ int od_frame_read(const od_dec *d, int k, uint8_t *y, uint8_t *cb, uint8_t *cr);
  * pointer may be NULL. */
 int od_frame_read_full(const od_dec *d, int k, uint8_t *y, uint8_t *cb, uint8_t *cr, ia_mbinfo *mbs);
 
+/* The arithmetic decoder on its own: `count` bins of kinds[i] (0 decision on context ctx[i], 1 bypass,
+ * 2 terminate) from contexts initialised as an I (intra) or P (cabac_init_idc 0) slice's at `qp`; then
+ * one more terminate bin into *final_terminate and the bytes consumed into *bytes_read. */
+int od_cabac_decode_bins(const uint8_t *data, size_t n, const uint8_t *kinds, const uint16_t *ctx, int count, int intra,
+                         int qp, uint8_t *vals, int *final_terminate, size_t *bytes_read);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,6 +28,7 @@ def _lib():
             "od_frame_size": (i, [vp, i, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(i)]),
             "od_frame_read": (i, [vp, i, vp, vp, vp]),
             "od_frame_read_full": (i, [vp, i, vp, vp, vp, vp]),
+            "od_cabac_decode_bins": (i, [vp, sz, vp, vp, i, i, i, vp, C.POINTER(i), C.POINTER(sz)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -93,6 +94,21 @@ class Decoder:
                                                mbs.ctypes.data))
             out.append(Frame(y, cb, cr, (fy, fcb, fcr), mbs))
         return out
+
+
+def cabac_decode_bins(data, kinds, ctx, intra, qp):
+    """(values, final terminate bin, bytes read) of the bins `kinds`/`ctx` decoded from `data`."""
+    L = _lib()
+    kinds = np.ascontiguousarray(kinds, dtype=np.uint8)
+    ctx = np.ascontiguousarray(ctx, dtype=np.uint16)
+    vals = np.zeros(len(kinds), np.uint8)
+    fin, nread = C.c_int(), C.c_size_t()
+    b = bytes(data)
+    rc = L.od_cabac_decode_bins(b, len(b), kinds.ctypes.data, ctx.ctypes.data, len(kinds), int(intra), qp,
+                                vals.ctypes.data, C.byref(fin), C.byref(nread))
+    if rc != 0:
+        raise DecodeError(f"od_cabac_decode_bins: {rc}")
+    return vals, fin.value, nread.value
 
 
 def decode_annexb(data):
