@@ -2,12 +2,13 @@
 
 Each search mode codes the same generator clip IPPP at four QPs, closed loop: each chain's
 references are its own reconstructions, loop-filtered on the device as the encoder keeps them
-(`ia_promote_reference`, enc/mod.rs:653-667). The rate of a picture is the sum over its macroblocks of
-`ia_mbstats.bits`: the CABAC bins of each kept macroblock counted on the live contexts, as the
-reference's writer codes them (`CabacCost`'s model, enc/cabac.rs:150-191; the headers and the
-arithmetic coder's termination are not counted, and are the same for both modes). Distortion is the
-PSNR of the loop-filtered picture (the encoder's output and reference) over the picture's visible
-area, luma and chroma weighted 4:1:1.
+(`ia_promote_reference`, enc/mod.rs:653-667). The rate of a picture is the size of its access unit as
+the host back end writes it (libironav_h264w: slice headers and CABAC slice data of the analysed
+macroblock codes, enc/frame.rs:2020-2279); the model rate the analysis counted
+(`ia_mbstats.bits`, `CabacCost`, enc/cabac.rs:150-191) is reported beside it. Distortion is the PSNR
+of the loop-filtered picture over the picture's visible area, luma and chroma weighted 4:1:1: the
+picture a decoder rebuilds from the written stream (the GPU tests decode such chains and compare them
+with these references byte for byte).
 
 BD-rate (Bjontegaard): log rate interpolated over PSNR per mode (piecewise cubic Hermite, as JVET's
 common test conditions compute it; the single-cubic form of VCEG-M33 beside it), the mean difference
@@ -56,27 +57,30 @@ def psnr(src, rec, w, h):
 
 def code_chain(pics, w, h, qp, mode, search_range, params=None, nrefs=3, slices=None):
     """Codes the pictures IPPP (IDR at qp - 3, as the reference's driver) with one search mode; returns
-    (bits, mean PSNR) over the P pictures."""
-    from . import abi
+    (written bits, model bits, mean PSNR) over the P pictures."""
+    from . import abi, h264w
     from .h264a import Analyser
 
     slices = slices or (h + 15) // 16
-    bits, ps = 0.0, []
+    bits, model, ps = 0.0, 0.0, []
     with Analyser(w, h, nrefs=nrefs, slices=slices, search_mode=mode, search_range=search_range,
-                  fallback_ratio=1.1) as an:
+                  fallback_ratio=1.1) as an, h264w.Stream(w, h, nrefs=nrefs, slices=slices) as wr:
         if params is not None:
             an.load_proposal_weights(params)
         order = []  # reference slots, newest picture first
         for t, pic in enumerate(pics):
             kind = abi.PIC_IDR if t == 0 else abi.PIC_P
-            res = an.analyse(pic, max(0, qp - 3) if t == 0 else qp, kind, order[:nrefs])
+            q = max(0, qp - 3) if t == 0 else qp
+            res = an.analyse(pic, q, kind, order[:nrefs])
+            au = wr.write(kind, q, res.codes, len(order[:nrefs]))
             slot = order.pop() if len(order) == nrefs else len(order)
             an.promote_reference(slot, deblock=True)
             order.insert(0, slot)
             if t > 0:
-                bits += float(res.stats["bits"].sum()) / 256.0
+                bits += 8.0 * len(au)
+                model += float(res.stats["bits"].sum()) / 256.0
                 ps.append(psnr(pic, an.read_reference(slot)[0], w, h))
-    return bits, float(np.mean(ps))
+    return bits, model, float(np.mean(ps))
 
 
 def run(name, spec, search_range, log=print):
@@ -85,21 +89,27 @@ def run(name, spec, search_range, log=print):
 
     pics = [Picture(*(p.cpu().numpy() for p in f)) for f in clipgen.make_clip_device(spec)]
     params = proposal.load_weights()
-    curves = {}
+    curves, models = {}, {}
     for label, mode in (("exhaustive", abi.SEARCH_FULL), ("learned", abi.SEARCH_LEARNED)):
-        pts = []
+        pts, mods = [], []
         for qp in QPS:
-            b, p = code_chain(pics, spec.width, spec.height, qp, mode, search_range,
-                              params if mode == abi.SEARCH_LEARNED else None)
+            b, m, p = code_chain(pics, spec.width, spec.height, qp, mode, search_range,
+                                 params if mode == abi.SEARCH_LEARNED else None)
             pts.append((b, p))
-            log(f"[bdrate] {name} {label} qp {qp}: {b / max(1, len(pics) - 1):.0f} bits/picture, {p:.3f} dB")
+            mods.append(m)
+            log(f"[bdrate] {name} {label} qp {qp}: {b / max(1, len(pics) - 1):.0f} bits/picture written "
+                f"({m / max(1, len(pics) - 1):.0f} modelled), {p:.3f} dB")
         curves[label] = pts
+        models[label] = mods
     ra, pa = zip(*curves["exhaustive"])
     rb, pb = zip(*curves["learned"])
     return {"clip": name, "qps": list(QPS), "pictures": len(pics), "search_range": search_range,
-            "exhaustive": {"bits": [round(x, 1) for x in ra], "psnr": [round(x, 4) for x in pa]},
-            "learned": {"bits": [round(x, 1) for x in rb], "psnr": [round(x, 4) for x in pb]},
+            "exhaustive": {"bits": [round(x, 1) for x in ra], "model_bits": [round(x, 1) for x in models["exhaustive"]],
+                           "psnr": [round(x, 4) for x in pa]},
+            "learned": {"bits": [round(x, 1) for x in rb], "model_bits": [round(x, 1) for x in models["learned"]],
+                        "psnr": [round(x, 4) for x in pb]},
             "bd_rate_percent": round(bd_rate(ra, pa, rb, pb), 3),
+            "bd_rate_percent_model_rate": round(bd_rate(models["exhaustive"], pa, models["learned"], pb), 3),
             "bd_rate_percent_cubic": round(bd_rate(ra, pa, rb, pb, "cubic"), 3)}
 
 
