@@ -288,6 +288,17 @@ int ia_fullsearch_variants(void);
 const char *ia_fullsearch_variant_name(int variant);
 int ia_set_fullsearch_variant(ia_ctx *ctx, int variant);
 
+/* The loop filter of ia_promote_reference (deblock 1) on a caller's picture, allocating nothing. No
+ * counterpart in the reference: it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops that forbid allocation). y (mb_w*16 x mb_h*16), cb, cr (mb_w*8 x mb_h*8)
+ * and the picture's records `mbs` (mb_w*mb_h) are device memory; the planes are filtered in place,
+ * asynchronously on `stream` (a hipStream_t, NULL for the default stream). *stalled (device memory, may
+ * be NULL) receives 1 if a row's wait for the row above gave up, else 0. The row counters live in a
+ * static device array of 1 + 512 words: mb_h must be at most 512 (IA_EINVAL beyond). Every call on a
+ * GPU shares that GPU's array, so calls that overlap on one GPU race. */
+int ia_deblock_picture_noalloc(uint8_t *y, uint8_t *cb, uint8_t *cr, const ia_mbinfo *mbs, int mb_w, int mb_h,
+                               int *stalled, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -108,6 +108,7 @@ PROTOTYPES = {
     "ia_batch_sync": (C.c_int, [vp]),
     "ia_batch_fallback_count": (C.c_int, [vp, C.POINTER(C.c_int)]),
     "ia_batch_workers": (C.c_int, []),
+    "ia_deblock_picture_noalloc": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, vp, vp]),
 }
 
 

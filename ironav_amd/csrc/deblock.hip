@@ -20,6 +20,8 @@
 // horizontal edges, and back for the stores. Lanes 0-31 first derive the 32 boundary strengths
 // (direction x edge x 4-line segment) of the macroblock, one each, and the filtering lanes gather
 // theirs with cross-lane reads.
+#include <mutex>
+
 #include "common.h"
 #include "h264_tables.h"
 
@@ -421,3 +423,62 @@ hipError_t launch_deblock_batch(const DbJob *jobs, int njobs, int nblocks, int *
 }
 
 }  // namespace ia
+
+// ---- the loop filter without allocation ------------------------------------------------------------
+//
+// For callers that cannot allocate device memory (code under graph capture, tight serving loops): the
+// row counters and the stall flag live in a file-scope device array instead of a context's buffers.
+// Every call on a GPU shares that GPU's array, so calls that overlap on one GPU race.
+
+namespace ia {
+namespace {
+
+constexpr int NOALLOC_MAX_ROWS = 512;  // pictures up to 8192 samples high
+
+// [0] the stall flag, [1 + r] the progress counter of macroblock row r
+__device__ __align__(256) int scratch[1 + NOALLOC_MAX_ROWS];
+
+// the array's device address per device, from hipGetSymbolAddress on the first call there
+constexpr int NOALLOC_MAX_DEVICES = 64;
+int *g_scratch[NOALLOC_MAX_DEVICES];
+std::mutex g_scratch_mu;
+
+__global__ void zero_words(int *p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+__global__ void copy_word(const int *src, int *dst) {
+  if (threadIdx.x == 0) *dst = *src;
+}
+
+}  // namespace
+
+}  // namespace ia
+
+extern "C" int ia_deblock_picture_noalloc(uint8_t *y, uint8_t *cb, uint8_t *cr, const ia_mbinfo *mbs, int mb_w, int mb_h,
+                                          int *stalled, void *stream) {
+  if (!y || !cb || !cr || !mbs || mb_w < 1 || mb_h < 1 || mb_w > 512) return IA_EINVAL;
+  if (mb_h > ia::NOALLOC_MAX_ROWS) return IA_EINVAL;  // beyond the static scratch array
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= ia::NOALLOC_MAX_DEVICES) return IA_EDEVICE;
+  int *p;
+  {
+    std::lock_guard<std::mutex> lock(ia::g_scratch_mu);
+    if (!ia::g_scratch[dev]) {
+      void *addr = nullptr;
+      if (hipGetSymbolAddress(&addr, HIP_SYMBOL(ia::scratch)) != hipSuccess || !addr) return IA_EDEVICE;
+      ia::g_scratch[dev] = static_cast<int *>(addr);
+    }
+    p = ia::g_scratch[dev];
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  // the counters and the flag start at zero (a fill kernel, so that a captured graph replays it)
+  hipLaunchKernelGGL(ia::zero_words, dim3(1), dim3(256), 0, st, p, 1 + mb_h);
+  if (hipGetLastError() != hipSuccess) return IA_EDEVICE;
+  if (ia::launch_deblock(y, cb, cr, mbs, mb_w, mb_h, p + 1, p, st) != hipSuccess) return IA_EDEVICE;
+  if (stalled) {
+    hipLaunchKernelGGL(ia::copy_word, dim3(1), dim3(64), 0, st, p, stalled);
+    if (hipGetLastError() != hipSuccess) return IA_EDEVICE;
+  }
+  return IA_OK;
+}
