@@ -199,6 +199,10 @@ struct od_dec {
   /* order counts and references */
   int prev_poc_msb, prev_poc_lsb, prev_frame_num, prev_frame_num_offset, prev_ref_has_mmco5;
   int idr_seq, next_id;
+  /* what the decoded syntax exercised: counts of Intra4x4 / Intra8x8 modes, Intra16x16 modes, chroma
+     modes, P partitionings, sub-partitionings, P_Skip, I_PCM, 8x8-transform inter macroblocks,
+     reference indexes above 0 */
+  int64_t stats[OD_NSTATS];
   frame_t frames[OD_MAX_FRAMES];
   int nframes;
 };
@@ -1323,6 +1327,7 @@ static int decode_pcm(sdec_t *t) {
       for (int x = 0; x < 8; x++) *cpix(t, c, t->mbx * 8 + x, t->mby * 8 + y) = (uint16_t)br_u(b, 8);
   cabac_init_engine(&t->c);
   dmb_t *m = t->cur;
+  t->d->stats[OD_STAT_PCM]++;
   m->kind = K_PCM;
   m->cbp = 0x2f;
   memset(m->nz, 16, sizeof m->nz);
@@ -1339,6 +1344,7 @@ static int decode_skip(sdec_t *t) {
   dmb_t *m = t->cur;
   m->kind = K_INTER;
   m->skip = 1;
+  t->d->stats[OD_STAT_SKIP]++;
   m->cbp = 0;
   for (int i = 0; i < 4; i++) m->ref[i] = 0;
   int16_t mv[2] = {0, 0};
@@ -1400,6 +1406,7 @@ static int decode_mb(sdec_t *t, int islice) {
         const int mode = syn[n][0] ? pred : (syn[n][1] < pred ? syn[n][1] : syn[n][1] + 1);
         if (m->t8) {
           for (int j = 0; j < 4; j++) m->modes[(by + (j >> 1)) * 4 + bx + (j & 1)] = (uint8_t)mode;
+          t->d->stats[OD_STAT_I8 + mode]++;
           intra_nxn(t, 1, bx, by, mode);
           const int b8 = n;
           if ((m->cbp >> b8) & 1) {
@@ -1408,6 +1415,7 @@ static int decode_mb(sdec_t *t, int islice) {
           }
         } else {
           m->modes[by * 4 + bx] = (uint8_t)mode;
+          t->d->stats[OD_STAT_I4 + mode]++;
           intra_nxn(t, 0, bx, by, mode);
           int32_t *c = t->coef[by * 4 + bx];
           dequant4(t, c, 0, t->qp, 0);
@@ -1416,6 +1424,7 @@ static int decode_mb(sdec_t *t, int islice) {
           if (any) or_idct4_add(c, ypix(t, t->mbx * 16 + bx * 4, t->mby * 16 + by * 4), t->mb_w * 16, 255);
         }
       }
+      t->d->stats[OD_STAT_CHROMA + m->chroma_mode]++;
       intra_chroma_pred(t);
       add_chroma_residual(t);
       return OD_OK;
@@ -1423,6 +1432,7 @@ static int decode_mb(sdec_t *t, int islice) {
     /* Intra16x16: mb_type 1..24 */
     m->kind = K_I16;
     t->i16_mode = (mt - 1) % 4;
+    t->d->stats[OD_STAT_I16 + t->i16_mode]++;
     const int cc = ((mt - 1) / 4) % 3, luma = mt >= 13;
     m->chroma_mode = (uint8_t)read_chroma_mode(t);
     m->cbp = (uint8_t)((luma ? 15 : 0) | cc << 4);
@@ -1446,6 +1456,7 @@ static int decode_mb(sdec_t *t, int islice) {
       for (int y = 0; y < 16; y++)
         for (int x = 0; x < 16; x++) *ypix(t, x0 + x, y0 + y) = out[y * 16 + x];
     }
+    t->d->stats[OD_STAT_CHROMA + m->chroma_mode]++;
     intra16_dc(t);
     add_luma_residual(t, 1);
     intra_chroma_pred(t);
@@ -1457,8 +1468,12 @@ static int decode_mb(sdec_t *t, int islice) {
   m->chroma_mode = 0;
   const int part = mt;
   int sub[4] = {0, 0, 0, 0};
+  t->d->stats[OD_STAT_PART + part]++;
   if (part == 3)
-    for (int k = 0; k < 4; k++) sub[k] = read_sub_mb_type(t);
+    for (int k = 0; k < 4; k++) {
+      sub[k] = read_sub_mb_type(t);
+      t->d->stats[OD_STAT_SUB + sub[k]]++;
+    }
   const int na = t->h->num_ref_idx_active;
   int refs[4] = {0, 0, 0, 0};
   static const int np[4] = {1, 2, 2, 4};
@@ -1467,6 +1482,7 @@ static int decode_mb(sdec_t *t, int islice) {
   for (int k = 0; k < np[part]; k++) {
     if (na > 1) refs[k] = read_ref_idx(t, pr[part][k][0], pr[part][k][1]);
     if (refs[k] >= na) return fail(t->d, OD_EBITSTREAM, "ref_idx %d of %d", refs[k], na);
+    if (refs[k] > 0) t->d->stats[OD_STAT_REF_GT0]++;
     /* the partition's 8x8 blocks carry its reference for the next ref_idx contexts */
     for (int y8 = pr[part][k][1] / 2; y8 <= (pr[part][k][1] + pr[part][k][3] - 1) / 2; y8++)
       for (int x8 = pr[part][k][0] / 2; x8 <= (pr[part][k][0] + pr[part][k][2] - 1) / 2; x8++)
@@ -1507,6 +1523,7 @@ static int decode_mb(sdec_t *t, int islice) {
   m->cbp = (uint8_t)read_cbp(t);
   if ((m->cbp & 15) && t->q->transform8 && (part != 3 || (sub[0] | sub[1] | sub[2] | sub[3]) == 0))
     m->t8 = (uint8_t)read_t8_flag(t);
+  if (m->t8) t->d->stats[OD_STAT_INTER_T8]++;
   int qd = 0;
   if (m->cbp) qd = read_qp_delta(t);
   t->prev_qp_delta_nz = qd != 0;
@@ -1810,4 +1827,10 @@ int od_cabac_decode_bins(const uint8_t *data, size_t n, const uint8_t *kinds, co
   if (final_terminate) *final_terminate = terminate(&c);
   if (bytes_read) *bytes_read = (b.pos + 7) / 8;
   return OD_OK;
+}
+
+int od_stats(const od_dec *d, int64_t *out, int n) {
+  if (!d || !out || n < 0) return OD_EINVAL;
+  for (int i = 0; i < n && i < OD_NSTATS; i++) out[i] = d->stats[i];
+  return OD_NSTATS;
 }

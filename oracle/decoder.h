@@ -56,6 +56,16 @@ int od_frame_read(const od_dec *d, int k, uint8_t *y, uint8_t *cb, uint8_t *cr);
  * pointer may be NULL. */
 int od_frame_read_full(const od_dec *d, int k, uint8_t *y, uint8_t *cb, uint8_t *cr, ia_mbinfo *mbs);
 
+/* What the decoded syntax exercised, as counts: Intra4x4 modes 0-8, Intra8x8 modes 0-8, Intra16x16
+ * modes 0-3, chroma modes 0-3, P partitionings 0-3 (16x16, 16x8, 8x16, 8x8), sub-partitionings 0-3,
+ * P_Skip, I_PCM, inter macroblocks with the 8x8 transform, reference indexes above 0. Returns
+ * OD_NSTATS and fills min(n, OD_NSTATS) counts. */
+enum {
+  OD_STAT_I4 = 0, OD_STAT_I8 = 9, OD_STAT_I16 = 18, OD_STAT_CHROMA = 22, OD_STAT_PART = 26, OD_STAT_SUB = 30,
+  OD_STAT_SKIP = 34, OD_STAT_PCM = 35, OD_STAT_INTER_T8 = 36, OD_STAT_REF_GT0 = 37, OD_NSTATS = 38
+};
+int od_stats(const od_dec *d, int64_t *out, int n);
+
 /* The arithmetic decoder on its own: `count` bins of kinds[i] (0 decision on context ctx[i], 1 bypass,
  * 2 terminate) from contexts initialised as an I (intra) or P (cabac_init_idc 0) slice's at `qp`; then
  * one more terminate bin into *final_terminate and the bytes consumed into *bytes_read. */

@@ -10,23 +10,30 @@
  * Parity pins (what each part is checked against, in tests/):
  *  - halfpel build/predict: the decoder's interpolation (h264/inter.rs:466-547) restated separately from
  *    H.264 8.4.2.2.1 here (or_luma_mc_ref), over the vector grid of enc/subpel.rs:160-193 at 8 and 10 bits.
+ *  - or_luma_mc_ref, or_chroma_mc, the nine 4x4/8x8 intra modes with the 8x8 reference filter, the
+ *    16x16 and chroma modes, the inverse transforms and the loop filter (or_deblock_picture_params):
+ *    the oracle's decoder (decoder.c) calls them unchanged and decodes the reference decoder's 13 CABAC
+ *    fixtures (libx264 streams) to FFmpeg's pictures byte for byte, every mode and partitioning occurring
+ *    (tests/test_oracle_decoder.py).
  *  - fdct4/quant/idct4, quant8/idct8: the reference's round-trip tests (enc/frame.rs:2443-2479), and
  *    closed forms (the transforms are the H.264 matrices; tests check them against numpy matrix products).
  *  - decimate_score: the known answers of enc/frame.rs:2482-2492.
  *  - entropy table: its closed form (enc/cabac.rs:162-173) recomputed in numpy.
  *  - se/ue bits: Exp-Golomb lengths (enc/bitw.rs:81-89) against the codes' definition.
  *  - full search: brute force in numpy on small pictures.
- *  - intra predictors: closed forms of H.264 8.3 on small edge sets (numpy).
- *  - loop filter (or_deblock_picture): the reference holds no vectors for it; pinned by what H.264 8.7
- *    fixes (identities, the strength-4 mirror symmetry, p0 + q0 conservation, transposition, reach);
- *    partially pinned.
+ *  - loop filter: also what H.264 8.7 fixes (identities, symmetries, reach; tests/test_oracle_deblock.py).
  *  - realtime decisions (coder_t.fast): the early P_Skip, 16x16 only, half samples only and the
  *    flat-intra rule by what they must do to a picture (tests/test_oracle_realtime.py); the shorter
  *    diamond and the integer stop parity unpinned.
- *  - motion_search / partition decision / code_mb / decide_rd / mb_bins: parity unpinned at the level of
- *    exact decisions (the reference commits no encoder outputs and cannot be built here); checked by
- *    invariants (reconstruction = prediction + dequantised residual, costs recomputed from outputs,
- *    mvd = mv - predictor).
+ *  - the reconstruction of or_analyse_picture (prediction, dequantisation, inverse transforms), its
+ *    vectors against their predictors (mvd = mv - the 8.4.1.3 predictor, P_Skip vectors by 8.4.1.1) and
+ *    the CABAC bin sequence of mb_bins: the analysis of IPPP chains is written as a bitstream by the
+ *    product's back end and decoded by decoder.c, which recomputes every vector from its mvd and its
+ *    own predictor; the decoded pictures equal the loop-filtered reconstruction and the decoded records
+ *    the analysis', and the written size is within 2% of the rate mb_bins counts (tests/test_writer.py).
+ *  - motion_search's candidate order and diamond, best_inter_ref/sub_search, code_mb's thresholds and
+ *    decide_rd's order and costs: parity unpinned (the reference commits no encoder outputs and cannot be
+ *    built here; any decision the syntax allows decodes correctly). Restated line by line.
  */
 #ifndef IRONAV_ORACLE_H
 #define IRONAV_ORACLE_H

@@ -29,6 +29,7 @@ def _lib():
             "od_frame_read": (i, [vp, i, vp, vp, vp]),
             "od_frame_read_full": (i, [vp, i, vp, vp, vp, vp]),
             "od_cabac_decode_bins": (i, [vp, sz, vp, vp, i, i, i, vp, C.POINTER(i), C.POINTER(sz)]),
+            "od_stats": (i, [vp, vp, i]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -74,6 +75,15 @@ class Decoder:
     def annexb(self, data):
         b = bytes(data)
         self._ok(self.L.od_decode_annexb(self.h, b, len(b)))
+
+    def stats(self):
+        """Counts of the syntax the decoded pictures exercised (decoder.h, od_stats)."""
+        out = np.zeros(64, np.int64)
+        n = self.L.od_stats(self.h, out.ctypes.data, 64)
+        out = out[:n]
+        return {"i4_modes": out[0:9], "i8_modes": out[9:18], "i16_modes": out[18:22], "chroma_modes": out[22:26],
+                "partitions": out[26:30], "sub_partitions": out[30:34], "skip": int(out[34]), "pcm": int(out[35]),
+                "inter_t8": int(out[36]), "ref_gt0": int(out[37])}
 
     def frames(self):
         self._ok(self.L.od_flush(self.h))
@@ -204,11 +214,12 @@ def mp4_nals(path):
     return nals
 
 
-def decode_mp4(path):
+def decode_mp4(path, with_stats=False):
     d = Decoder()
     try:
         for n in mp4_nals(path):
             d.nal(n)
-        return d.frames()
+        frames = d.frames()
+        return (frames, d.stats()) if with_stats else frames
     finally:
         d.close()

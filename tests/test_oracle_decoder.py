@@ -35,3 +35,20 @@ def test_fixture_decodes_to_ffmpeg_pictures(oracle, name):
     for k, f in enumerate(frames):
         got = np.concatenate([f.y.ravel(), f.cb.ravel(), f.cr.ravel()])
         assert np.array_equal(got, want[k * size:(k + 1) * size]), f"{name}: picture {k} differs from FFmpeg's"
+
+
+def test_fixtures_exercise_every_prediction_form(oracle):
+    """Across the fixtures, every Intra4x4 and Intra8x8 mode (the 8x8 reference filter with them), every
+    Intra16x16 and chroma mode, every P partitioning and sub-partitioning, P_Skip, I_PCM, the 8x8
+    transform in inter macroblocks and reference indexes above 0 occur: the oracle's primitives for
+    each are pinned by FFmpeg's pictures."""
+    from oracle import pydecoder
+
+    tot = None
+    for name in STREAMS:
+        _, st = pydecoder.decode_mp4(os.path.join(HERE, f"h264_{name}.mp4"), with_stats=True)
+        tot = st if tot is None else {k: tot[k] + st[k] for k in st}
+    for k in ("i4_modes", "i8_modes", "i16_modes", "chroma_modes", "partitions", "sub_partitions"):
+        assert np.all(np.asarray(tot[k]) > 0), (k, tot[k])
+    for k in ("skip", "pcm", "inter_t8", "ref_gt0"):
+        assert tot[k] > 0, k
