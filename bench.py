@@ -20,8 +20,10 @@ work against the packed-u8 SAD issue peak, and its CNN is BF16 MFMA work, report
 the fraction of the launch that both would take at their peaks. full_search.roofline is the
 exhaustive-search kernel's. traffic is the HBM bytes per launch that rocprofv3's FETCH_SIZE and
 WRITE_SIZE counters read (scaled by what they read for a copy of known size, --copy-calibration).
-cpu_baseline is the oracle's scalar exhaustive search of the same picture against one reference on
-one host core, scaled to three references.
+cpu_baseline is the reference's CPU path for the same picture: the oracle's analysis (the reference's
+diamond search and code_mb) on the host's cores, one thread per slice with the encoder's slice count.
+end_to_end times search + exact analysis per picture for both searches; chain_cost_gap measures the
+cost gap over the clip's first 60 pictures coded closed loop.
 """
 import argparse
 import json
@@ -92,6 +94,8 @@ def main():
     ap.add_argument("--range", type=int, default=32)
     ap.add_argument("--variant", type=int, default=-1, help="full-search tile shape (-1: default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--chain", type=int, default=60,
+                    help="learned: pictures of the chain the cost gap is measured over (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (the counters' scale for the traffic figures)")
     ap.add_argument("--config", choices=["learned", "plain", "batched", "learned-batched", "deblock", "search-loop",
@@ -229,17 +233,12 @@ def main():
     # the time both kinds of work would take at their peaks, over the launch's time
     at_peak_ms = (prop_absdiff / (VALU_ABSDIFF_PEAK_T * 1e12) + cnn_flop / (MFMA_BF16_PEAK_T * 1e12)) * 1000.0
 
+    end_to_end = end_to_end_block(stream, refs, pics[4], params, args)
+    chain = chain_gap_block(stream, params, args) if args.chain > 0 else None
+
     cpu = None
     if not args.no_cpu:
-        from oracle import pyoracle
-
-        pyoracle.build()
-        t = time.perf_counter()
-        pyoracle.fullsearch(pics[4].y, [refs[0][0].y], mb_w, mb_h, args.range)
-        dt = time.perf_counter() - t
-        cpu = {"value": round(mbs / (dt * nref), 1), "unit": "MB/s", "cores": 1, "kind": "port",
-               "sample": f"oracle scalar exhaustive search of the 1080p picture against 1 reference "
-                         f"(+-{args.range}, 41 partitions), {dt:.1f} s, scaled to {nref} references"}
+        cpu = cpu_analysis_baseline(pics[4], refs, args)
 
     out = {
         "metric": "MB/s motion analysis @1080p: learned proposal vs full search at <=1% cost gap",
@@ -282,9 +281,149 @@ def main():
                      "frac_both_at_peak": round(at_peak_ms / ms_prop, 4),
                      "note": "launch with fallback off: cost volume, CNN, top-8 and refinement of every "
                              "(macroblock, reference)"},
+        "end_to_end": end_to_end,
+        "chain_cost_gap": chain,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def cpu_analysis_baseline(pic, refs, args):
+    """The reference's CPU path for the same picture: the oracle's analysis (the reference's diamond
+    search and code_mb with its rate-distortion decisions, 3 references, CABAC + 8x8 transform) of the
+    1080p P picture, its slices on host threads as the reference's encoder runs them, one thread per
+    slice with the encoder's slice count min(threads, mb_h/8) (enc/mod.rs:192, :623-627); best of 3."""
+    from ironav_amd import abi
+    from oracle import pyoracle
+
+    pyoracle.build()
+    nproc = os.cpu_count() or 1
+    mb_h = (pic.y.shape[0] + 15) // 16
+    slices = max(1, min(nproc, mb_h // 8))
+    cfg = abi.Config(pic.y.shape[1], 1080, 8, 1, 1, 0, 0, 3, slices, abi.SEARCH_DIAMOND, 32, 1.1)
+    best = None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        pyoracle.analyse_picture(cfg, pic, 28, abi.PIC_P, refs[:3], None, threads=slices)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    mbs = (pic.y.shape[1] // 16) * mb_h
+    return {"value": round(mbs / best, 1), "unit": "MB/s", "cores": slices, "kind": "port",
+            "sample": f"the oracle's analysis of the 1080p P picture ({mbs} macroblocks; the reference's diamond "
+                      f"search + code_mb, 3 references, CABAC + 8x8 transform), {slices} slices on {slices} threads "
+                      f"(min(nproc, mb_h/8)); best of 3, {best * 1000:.1f} ms; host: {nproc} logical CPUs, "
+                      f"{cpu_model()}"}
+
+
+def end_to_end_block(stream, refs, pic, params, args):
+    """Search + exact analysis of the 1080p P picture per step, learned against exhaustive, inputs
+    resident: the motion search, then code_mb over every slice with the search's winners as the extra
+    candidates; at one slice per macroblock row (68) and at the encoder's slice rule (8)."""
+    import torch
+
+    from ironav_amd import abi
+    from ironav_amd.h264a import Analyser
+
+    out = {}
+    for slices in (68, 8):
+        row = {}
+        for label, mode in (("learned", abi.SEARCH_LEARNED), ("exhaustive", abi.SEARCH_FULL)):
+            with Analyser(1920, 1080, nrefs=3, slices=slices, search_mode=mode, search_range=args.range,
+                          fallback_ratio=1.1) as an:
+                an.set_stream(stream.cuda_stream)
+                if mode == abi.SEARCH_LEARNED:
+                    an.load_proposal_weights(params)
+                for s in range(3):
+                    an.set_reference(s, refs[s][0], refs[s][1])
+                an.upload(pic)
+                an.sync()
+
+                def step():
+                    an.motion_search([0, 1, 2])
+                    an.analyse_uploaded(28, abi.PIC_P, [0, 1, 2])
+
+                steps = 5 if slices == 68 else 3
+                ms, wall = timed(stream, step, steps, 1)
+                torch.cuda.synchronize()
+                row[label] = {"ms_per_picture": round(max(ms, wall), 3),
+                              "MB_per_s": round(8160 / (max(ms, wall) / 1000.0), 1)}
+        row["speedup"] = round(row["exhaustive"]["ms_per_picture"] / row["learned"]["ms_per_picture"], 3)
+        out[f"slices_{slices}"] = row
+    out["note"] = ("motion search + exact analysis (code_mb with the CABAC rate model) per 1080p P picture "
+                   "against 3 references; the analysis is the same for both and bounds the step")
+    return out
+
+
+def chain_gap_block(stream, params, args):
+    """The cost gap over a chain: the workload clip's first --chain pictures (made on the device, the
+    host generator's scenes and motion with device noise) coded IPPP closed loop with the learned search,
+    references loop-filtered on the device; every P picture also searched exhaustively against the same
+    references. Gaps are means over all P pictures: the best reference's 16x16 SAD, and the encoder's own
+    inter and RD costs after the exact analysis with either result as its extra candidate."""
+    import numpy as np
+
+    from ironav_amd import abi, clipgen
+    from ironav_amd.h264a import Analyser, Picture
+
+    spec = clipgen.workload_1080p()
+    spec.frames = args.chain
+    common = dict(nrefs=3, slices=68, search_range=args.range, fallback_ratio=1.1)
+    sad_l = sad_f = inter_l = inter_f = rd_l = rd_f = 0.0
+    fb = pairs = 0
+    per_pic = []
+    with Analyser(1920, 1080, search_mode=abi.SEARCH_LEARNED, **common) as lrn, \
+            Analyser(1920, 1080, search_mode=abi.SEARCH_FULL, **common) as ful:
+        lrn.load_proposal_weights(params)
+        order = []
+        for t, planes in enumerate(clipgen.make_clip_device(spec)):
+            pic = Picture(*(p.cpu().numpy() for p in planes))
+            kind = abi.PIC_IDR if t == 0 else abi.PIC_P
+            refs = order[:3]
+            if kind == abi.PIC_P:
+                # the exhaustive search and analysis against the learned chain's references
+                for s in refs:
+                    ref, mbs = lrn.read_reference(s)
+                    ful.set_reference(s, ref, mbs)
+                ful.upload(pic)
+                ful.motion_search(refs)
+                fs = ful.read_motion(len(refs))
+                ful.analyse_uploaded(28, abi.PIC_P, refs)
+                sf = read_stats(ful, 8160)
+            res = lrn.analyse(pic, 25 if t == 0 else 28, kind, refs)
+            if kind == abi.PIC_P:
+                lr = lrn.read_motion(len(refs))
+                fb += lrn.fallback_count()
+                pairs += 8160 * len(refs)
+                bl = lr["sad"][:, :, 0].min(axis=0).astype(np.float64).sum()
+                bf = fs["sad"][:, :, 0].min(axis=0).astype(np.float64).sum()
+                sad_l += bl
+                sad_f += bf
+                both = (res.stats["inter_cost"] >= 0) & (sf["inter_cost"] >= 0)
+                inter_l += float(res.stats["inter_cost"][both].sum())
+                inter_f += float(sf["inter_cost"][both].sum())
+                rd_l += float(res.stats["rd_cost"].sum())
+                rd_f += float(sf["rd_cost"].sum())
+                per_pic.append(bl / bf - 1.0)
+            slot = order.pop() if len(order) == 3 else len(order)
+            lrn.promote_reference(slot, deblock=True)
+            order.insert(0, slot)
+    return {"pictures": args.chain, "p_pictures": len(per_pic),
+            "sad16_best_reference": round(sad_l / sad_f - 1.0, 5),
+            "sad16_best_reference_worst_picture": round(max(per_pic), 5) if per_pic else None,
+            "encoder_inter_cost": round(inter_l / inter_f - 1.0, 5), "encoder_rd_cost": round(rd_l / rd_f - 1.0, 5),
+            "fallback_rate": round(fb / max(1, pairs), 4),
+            "note": "closed loop with the learned search (references loop-filtered on the device); the exhaustive "
+                    "search of each P picture against the same references; QP 28, IDR 25"}
 
 
 def workload_references(pics, nref=3):
