@@ -82,6 +82,27 @@ __device__ __forceinline__ int wave_sum(int v) {
   return __builtin_amdgcn_readlane(v, 63);
 }
 
+// Sum over each group of 4 consecutive lanes, the same in the group's lanes (DPP quad permutations).
+__device__ __forceinline__ int quad_sum(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, false);  // quad_perm [1, 0, 3, 2]
+  v += __builtin_amdgcn_update_dpp(0, v, 0x4e, 0xf, 0xf, false);  // quad_perm [2, 3, 0, 1]
+  return v;
+}
+
+// Sum over each aligned group of n lanes (n = 1, 2, 4, 8 or 16), the same in the group's lanes: the
+// quad permutations exchange lanes 1 and 2 apart, the half-row mirror pairs each lane with one of the
+// other quad of its 8, the row mirror with one of the other 8 of its 16.
+__device__ __forceinline__ int group_sum(int v, int n) {
+  if (n >= 2) v += __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, false);   // quad_perm [1, 0, 3, 2]
+  if (n >= 4) v += __builtin_amdgcn_update_dpp(0, v, 0x4e, 0xf, 0xf, false);   // quad_perm [2, 3, 0, 1]
+  if (n >= 8) v += __builtin_amdgcn_update_dpp(0, v, 0x141, 0xf, 0xf, false);  // row_half_mirror
+  if (n >= 16) v += __builtin_amdgcn_update_dpp(0, v, 0x140, 0xf, 0xf, false); // row_mirror
+  return v;
+}
+
+// lanes whose predicate holds, counted (the same in every lane)
+__device__ __forceinline__ int count_lanes(bool pred) { return __popcll(__ballot(pred)); }
+
 __device__ __forceinline__ int lane() { return threadIdx.x; }
 
 __device__ __forceinline__ void sync() { __syncthreads(); }
@@ -375,6 +396,44 @@ struct Coder {
     return 25 + 4 * b8 + (y & 1) * 2 + (x & 1);
   }
 
+  // The costs SATD + lambda * mvd_bits that the sub-sample refinement gives n candidate vectors of the
+  // w x h partition at (bxp, byp) of the macroblock (picture position (x0, y0)) against reference r,
+  // computed together: a lane takes one 4x4 block of one candidate's quarter-sample prediction
+  // (enc/subpel.rs:122-145, as luma_pred forms it), and each candidate's blocks are summed within its
+  // group of lanes.
+  __device__ void cost_sub_batch(int r, int x0, int y0, int bxp, int byp, int w, int h, const int (*cand)[2], int n,
+                                 const int16_t *mvp, int *costs) const {
+    const int pw = P.mb_w * 16, ph = P.mb_h * 16, sw = pw + 2 * PAD, sh = ph + 2 * PAD;
+    const size_t plane = (size_t)sw * sh;
+    const int nbx = w >> 2, nb = nbx * (h >> 2), per = 64 / nb, lnb = __builtin_ctz(nb);
+    const int lam = P.lambda;
+    for (int base = 0; base < n; base += per) {
+      const int g = base + (lane() >> lnb), blk = lane() & (nb - 1);
+      int sv = 0;
+      if (g < n) {
+        const int vx = cand[g][0], vy = cand[g][1];
+        const int fx = vx & 3, fy = vy & 3;
+        const int ix = clampi(x0 + (vx >> 2), -w - 3, pw + 1), iy = clampi(y0 + (vy >> 2), -h - 3, ph + 1);
+        const uint8_t *m = QMIX[fy][fx];
+        const int ox = (blk % nbx) * 4, oy = (blk / nbx) * 4;
+        const uint8_t *a = P.hp[r] + m[0] * plane + (size_t)(iy + oy + m[2] + PAD) * sw + (ix + ox + m[1] + PAD);
+        const uint8_t *b = P.hp[r] + m[3] * plane + (size_t)(iy + oy + m[5] + PAD) * sw + (ix + ox + m[4] + PAD);
+        int d[16];
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+          for (int c = 0; c < 4; c++) {
+            const int pv = ((int)a[(size_t)rr * sw + c] + (int)b[(size_t)rr * sw + c] + 1) >> 1;
+            d[rr * 4 + c] = (int)L.src_y[(byp + oy + rr) * 16 + bxp + ox + c] - pv;
+          }
+        sv = satd4(d);
+      }
+      sv = group_sum(sv, nb);
+      for (int k = 0; k < per && base + k < n; k++)
+        costs[base + k] = __builtin_amdgcn_readlane(sv, k * nb) + lam * mvd_bits(cand[base + k][0], cand[base + k][1], mvp[0], mvp[1]);
+    }
+  }
+
   // ---- motion search (enc/frame.rs:949-1019) ----
 
   __device__ int motion_search(int r, int bx, int by, int bw, int bh, const int16_t *mvp, const int16_t (*extra)[2],
@@ -436,26 +495,35 @@ struct Coder {
       return best_cost - lam * mvd_bits(bestx, besty, mvp[0], mvp[1]);
     }
     // sub-sample refinement by SATD: 8 half samples around, then 4 quarter samples (realtime coding
-    // keeps to the half samples, enc/frame.rs:1006)
-    auto cost_sub = [&](int vx, int vy) {
-      luma_pred(r, x0, y0, w, h, vx, vy, L.pred, 16);
-      sync();
-      int c = satd(0, bxp, byp, w, h, L.pred, 16) + lam * mvd_bits(vx, vy, mvp[0], mvp[1]);
-      sync();
-      return c;
-    };
-    best_cost = cost_sub(bestx, besty);
+    // keeps to the half samples, enc/frame.rs:1006). Each pass probes points around a centre fixed for
+    // the pass, so its candidates' costs are computed together and then taken in the reference's order
+    // with its strict minimum; the first pass carries the centre's own cost.
     const int SQ[8][2] = {{-1, -1}, {0, -1}, {1, -1}, {-1, 0}, {1, 0}, {-1, 1}, {0, 1}, {1, 1}};
     const int DM[4][2] = {{0, -1}, {-1, 0}, {1, 0}, {0, 1}};
     for (int pass = 0; pass < (P.fast ? 1 : 2); pass++) {
       const int step = pass == 0 ? 2 : 1, np = pass == 0 ? 8 : 4;
       const int cx0 = bestx, cy0 = besty;
+      int cand[9][2], cost[9];
+      int n = 0;
+      if (pass == 0) {
+        cand[0][0] = cx0;
+        cand[0][1] = cy0;
+        n = 1;
+      }
       for (int k = 0; k < np; k++) {
         const int ddx = pass == 0 ? SQ[k][0] : DM[k][0], ddy = pass == 0 ? SQ[k][1] : DM[k][1];
-        int cx = clampi(cx0 + ddx * step, lo_x, hi_x), cy = clampi(cy0 + ddy * step, lo_y, hi_y);
-        int cost = cost_sub(cx, cy);
-        if (cost < best_cost) { bestx = cx; besty = cy; best_cost = cost; }
+        cand[n][0] = clampi(cx0 + ddx * step, lo_x, hi_x);
+        cand[n][1] = clampi(cy0 + ddy * step, lo_y, hi_y);
+        n++;
       }
+      cost_sub_batch(r, x0, y0, bxp, byp, w, h, cand, n, mvp, cost);
+      int k0 = 0;
+      if (pass == 0) {
+        best_cost = cost[0];
+        k0 = 1;
+      }
+      for (int k = k0; k < n; k++)
+        if (cost[k] < best_cost) { bestx = cand[k][0]; besty = cand[k][1]; best_cost = cost[k]; }
     }
     out[0] = (int16_t)bestx;
     out[1] = (int16_t)besty;
@@ -724,32 +792,6 @@ struct Coder {
     o[7] = (a4 >> 2) - a7;
   }
 
-  // enc/frame.rs:88-105 on one 8x8 residual (raster, in place -> coefficients)
-  __device__ static void quant8(int *d, int qp, int intra, int16_t *lv, int32_t *co) {
-    int t[8], o[8];
-    for (int r = 0; r < 8; r++) {
-      for (int k = 0; k < 8; k++) t[k] = d[r * 8 + k];
-      fdct8_1d(t, o);
-      for (int k = 0; k < 8; k++) d[r * 8 + k] = o[k];
-    }
-    for (int c = 0; c < 8; c++) {
-      for (int r = 0; r < 8; r++) t[r] = d[r * 8 + c];
-      fdct8_1d(t, o);
-      for (int r = 0; r < 8; r++) d[r * 8 + c] = o[r];
-    }
-    const int m = qp % 6, qbits = 16 + qp / 6, f = (1 << qbits) / (intra ? 3 : 6);
-    for (int k = 0; k < 64; k++) co[k] = 0;
-    for (int k = 0; k < 64; k++) {
-      const int p = ZIG8[k];
-      const int l = quant(d[p], IA_MF8[m][class8(p)], f, qbits);
-      lv[k] = (int16_t)l;
-      if (l != 0) {
-        const int v = l * 16 * IA_DEQUANT8_COEFF_INIT[m][class8(p)];
-        co[p] = qp >= 36 ? shl(v, qp / 6 - 6) : (v + (1 << (5 - qp / 6))) >> (6 - qp / 6);
-      }
-    }
-  }
-
   __device__ static int wadd(int a, int b) { return (int)((uint32_t)a + (uint32_t)b); }
   __device__ static int wsub(int a, int b) { return (int)((uint32_t)a - (uint32_t)b); }
 
@@ -793,23 +835,6 @@ struct Coder {
     o[5] = wsub(b4, b3);
     o[6] = wsub(b2, b5);
     o[7] = wsub(b0, b7);
-  }
-
-  __device__ static void idct8_add(const int32_t *c, uint8_t *dst, int stride) {
-    int t[64], d[8], hh[8];
-    for (int i = 0; i < 8; i++) {
-      int rr[8];
-      for (int k = 0; k < 8; k++) rr[k] = c[i * 8 + k];
-      row8(rr, t + i * 8);
-    }
-    for (int j = 0; j < 8; j++) {
-      for (int i = 0; i < 8; i++) d[i] = t[i * 8 + j];
-      row8(d, hh);
-      for (int i = 0; i < 8; i++) {
-        uint8_t *p = dst + i * stride + j;
-        *p = (uint8_t)clampi(*p + (wadd(hh[i], 32) >> 6), 0, 255);
-      }
-    }
   }
 
   __device__ static void hadamard4(const int32_t *c, int32_t *o) {
@@ -863,6 +888,74 @@ struct Coder {
     int k = 0;
     for (int i = 0; i < n; i++) k += v[i] != 0;
     return k;
+  }
+
+  // ---- 8x8 transforms split over the lanes (the arithmetic of quant8 / idct8_add, enc/frame.rs:88-105,
+  // h264/idct.rs:55-74) ----
+
+  // Blocks b0..b0+nb-1 of L.co8 hold raster residuals: forward transform (a lane per row, then per
+  // column), then quantisation and dequantisation (a lane per coefficient). Levels land in lv[b] in
+  // zig-zag order, the dequantised coefficients back in L.co8[b] (raster); nz[b - b0] the nonzero levels.
+  __device__ void quant8_par(int b0, int nb, int qp, int intra, int16_t (*lv)[64], int *nz) {
+    const int l = lane();
+    if (l < nb * 8) {
+      int32_t *row = L.co8[b0 + (l >> 3)] + (l & 7) * 8;
+      int t[8], o[8];
+      for (int k = 0; k < 8; k++) t[k] = row[k];
+      fdct8_1d(t, o);
+      for (int k = 0; k < 8; k++) row[k] = o[k];
+    }
+    sync();
+    if (l < nb * 8) {
+      int32_t *col = L.co8[b0 + (l >> 3)] + (l & 7);
+      int t[8], o[8];
+      for (int k = 0; k < 8; k++) t[k] = col[k * 8];
+      fdct8_1d(t, o);
+      for (int k = 0; k < 8; k++) col[k * 8] = o[k];
+    }
+    sync();
+    const int m = qp % 6, qbits = 16 + qp / 6, f = (1 << qbits) / (intra ? 3 : 6);
+    const int pz = ZIG8[l], cls = class8(pz), mf = IA_MF8[m][cls], dq = 16 * IA_DEQUANT8_COEFF_INIT[m][cls];
+    for (int b = b0; b < b0 + nb; b++) {
+      const int lvl = quant(L.co8[b][pz], mf, f, qbits);
+      lv[b][l] = (int16_t)lvl;
+      int co = 0;
+      if (lvl != 0) {
+        const int v = lvl * dq;
+        co = qp >= 36 ? shl(v, qp / 6 - 6) : (v + (1 << (5 - qp / 6))) >> (6 - qp / 6);
+      }
+      L.co8[b][pz] = co;  // this lane alone reads and writes coefficient pz
+      nz[b - b0] = count_lanes(lvl != 0);
+    }
+    sync();
+  }
+
+  // the inverse transform of L.co8[b] added to the canvas at (x, y) for the blocks whose bit is set in
+  // `mask` (b = 0..3 at (b & 1) * 8, (b >> 1) * 8): a lane per row, then per column
+  __device__ void idct8_add_par(unsigned mask) {
+    const int l = lane(), b = l >> 3;
+    const bool on = l < 32 && ((mask >> b) & 1);
+    if (on) {
+      int32_t *row = L.co8[b] + (l & 7) * 8;
+      int t[8], o[8];
+      for (int k = 0; k < 8; k++) t[k] = row[k];
+      row8(t, o);
+      for (int k = 0; k < 8; k++) row[k] = o[k];
+    }
+    sync();
+    if (on) {
+      const int j = l & 7;
+      const int32_t *col = L.co8[b] + j;
+      int d[8], hh[8];
+      for (int i = 0; i < 8; i++) d[i] = col[i * 8];
+      row8(d, hh);
+      uint8_t *dst = &cyv((b & 1) * 8 + j, (b >> 1) * 8);
+      for (int i = 0; i < 8; i++) {
+        uint8_t *q = dst + i * 28;
+        *q = (uint8_t)clampi(*q + (wadd(hh[i], 32) >> 6), 0, 255);
+      }
+    }
+    sync();
   }
 
   // ---- coding of the candidates (enc/frame.rs:1050-1918) ----
@@ -1135,31 +1228,34 @@ struct Coder {
   // enc/frame.rs:1254-1304 (the luma against L.pred with the 8x8 transform)
   __device__ void code_luma8() {
     const int qp = P.qp;
-    if (lane() < 4) {
-      const int i8 = lane(), bx = (i8 & 1) * 8, by = (i8 >> 1) * 8;
-      int d[64];
-      for (int k = 0; k < 64; k++) {
-        const int o = (by + k / 8) * 16 + bx + k % 8;
-        d[k] = (int)L.src_y[o] - (int)L.pred[o];
-      }
-      quant8(d, qp, 0, L.code.luma8[i8], L.co8[i8]);
-      L.score[i8] = decimate_score8(L.code.luma8[i8]);
+    // residuals of the four 8x8 blocks (block b = i >> 6, raster sample i & 63)
+    for (int i = lane(); i < 256; i += 64) {
+      const int b = i >> 6, k = i & 63, o = ((b >> 1) * 8 + (k >> 3)) * 16 + (b & 1) * 8 + (k & 7);
+      L.co8[b][k] = (int)L.src_y[o] - (int)L.pred[o];
     }
     sync();
+    int nz[4];
+    quant8_par(0, 4, qp, 0, L.code.luma8, nz);
+    if (lane() < 4) L.score[lane()] = decimate_score8(L.code.luma8[lane()]);
     for (int i = lane(); i < 256; i += 64) cyv(i & 15, i >> 4) = L.pred[i];
     sync();
     const int total = L.score[0] + L.score[1] + L.score[2] + L.score[3];
+    unsigned coded = 0;
+    int n[4];
+    for (int b = 0; b < 4; b++) {
+      const bool kept = !(L.score[b] < 4 || total < 6);
+      n[b] = kept ? nz[b] : 0;
+      if (!kept)
+        for (int k = lane(); k < 64; k += 64) L.code.luma8[b][k] = 0;
+      if (n[b] > 0) coded |= 1u << b;
+    }
+    sync();
+    if (coded) idct8_add_par(coded);
     if (lane() < 4) {
       const int i8 = lane();
-      int n = 0;
-      if (L.score[i8] < 4 || total < 6) {
-        for (int k = 0; k < 64; k++) L.code.luma8[i8][k] = 0;
-      } else {
-        n = count_nz(L.code.luma8[i8], 64);
-        if (n > 0) idct8_add(L.co8[i8], &cyv((i8 & 1) * 8, (i8 >> 1) * 8), 28);
-        for (int j = 0; j < 4; j++) L.code.nnz[((i8 >> 1) * 2 + (j >> 1)) * 4 + (i8 & 1) * 2 + (j & 1)] = (uint8_t)n;
-      }
-      L.ival[i8] = n;
+      if (!(L.score[i8] < 4 || total < 6))
+        for (int j = 0; j < 4; j++) L.code.nnz[((i8 >> 1) * 2 + (j >> 1)) * 4 + (i8 & 1) * 2 + (j & 1)] = (uint8_t)n[i8];
+      L.ival[i8] = n[i8];
     }
     sync();
     if (lane() == 0) {
@@ -1307,39 +1403,39 @@ struct Coder {
         left[7] = (pl[6] + 3 * pl[7] + 2) >> 2;
       }
       const int predicted = pred_i4_mode(L.code.modes, bx * 2, by * 2);
-      int c = 0x7fffffff;
-      const int m = lane();
-      if (m < 9 && mode_ok(m, have_top, have_left, have_tl)) {
-        int s = 0;
-        for (int q = 0; q < 4; q++) {
-          const int ox = (q & 1) * 4, oy = (q >> 1) * 4;
-          int d[16];
-          for (int i = 0; i < 16; i++) {
-            const int xx = ox + (i & 3), yy = oy + (i >> 2);
-            d[i] = (int)L.src_y[(y + yy) * 16 + x + xx] - nxn_sample(8, m, xx, yy, top, left, tl, have_top, have_left);
-          }
-          s += satd4(d);
+      // lane 4m + q < 36: the SATD of quadrant q of mode m's prediction; the quad's sum is the mode's
+      const int m = lane() >> 2, qd = lane() & 3;
+      const bool ok = lane() < 36 && mode_ok(m, have_top, have_left, have_tl);
+      int s = 0;
+      if (ok) {
+        const int ox = (qd & 1) * 4, oy = (qd >> 1) * 4;
+        int d[16];
+        for (int i = 0; i < 16; i++) {
+          const int xx = ox + (i & 3), yy = oy + (i >> 2);
+          d[i] = (int)L.src_y[(y + yy) * 16 + x + xx] - nxn_sample(8, m, xx, yy, top, left, tl, have_top, have_left);
         }
-        c = s + P.lambda * (m == predicted ? 1 : 4);
+        s = satd4(d);
       }
+      s = quad_sum(s);
+      const int c = ok ? s + P.lambda * (m == predicted ? 1 : 4) : 0x7fffffff;
       int best_m = 2, best_c = 0x7fffffff;
       for (int k = 0; k < 9; k++) {
-        int ck = __builtin_amdgcn_readlane(c, k);
+        int ck = __builtin_amdgcn_readlane(c, 4 * k);
         if (ck < best_c) { best_c = ck; best_m = k; }
       }
+      // the prediction into the canvas and the residual, a lane per sample
+      {
+        const int i = lane();
+        const int pv = nxn_sample(8, best_m, i & 7, i >> 3, top, left, tl, have_top, have_left);
+        L.co8[i8][i] = (int)L.src_y[(y + (i >> 3)) * 16 + x + (i & 7)] - pv;
+        cyv(x + (i & 7), y + (i >> 3)) = (uint8_t)pv;
+      }
+      sync();
+      int n;
+      quant8_par(i8, 1, qp, 1, L.code.luma8, &n);
+      if (n > 0) idct8_add_par(1u << i8);
       if (lane() == 0) {
-        int d[64];
-        for (int i = 0; i < 64; i++) {
-          const int pv = nxn_sample(8, best_m, i & 7, i >> 3, top, left, tl, have_top, have_left);
-          d[i] = (int)L.src_y[(y + (i >> 3)) * 16 + x + (i & 7)] - pv;
-          cyv(x + (i & 7), y + (i >> 3)) = (uint8_t)pv;
-        }
-        quant8(d, qp, 1, L.code.luma8[i8], L.co8[i8]);
-        const int n = count_nz(L.code.luma8[i8], 64);
-        if (n > 0) {
-          idct8_add(L.co8[i8], &cyv(x, y), 28);
-          L.code.cbp |= (uint8_t)(1 << i8);
-        }
+        if (n > 0) L.code.cbp |= (uint8_t)(1 << i8);
         for (int j = 0; j < 4; j++) {
           const int b = (by * 2 + (j >> 1)) * 4 + bx * 2 + (j & 1);
           L.code.modes[b] = (uint8_t)best_m;

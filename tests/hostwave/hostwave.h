@@ -133,7 +133,8 @@ inline T readlane(T v, int k) {
 }
 
 // __builtin_amdgcn_update_dpp(old, src, ctrl, row_mask, bank_mask, bound_ctrl) for the controls the
-// kernels use: row_shr:1..15 (0x111..0x11f), row_bcast:15 (0x142), row_bcast:31 (0x143). A lane
+// kernels use: quad_perm (0x00..0xff), row_shr:1..15 (0x111..0x11f), row_mirror (0x140),
+// row_half_mirror (0x141), row_bcast:15 (0x142), row_bcast:31 (0x143). A lane
 // whose row or bank is masked off, or whose source lane is outside the row (shifts) or below the
 // broadcast source, keeps `old` (bound_ctrl 0) or takes 0 (bound_ctrl 1).
 inline int update_dpp(int old, int src, int ctrl, int row_mask, int bank_mask, bool bound_ctrl) {
@@ -142,9 +143,15 @@ inline int update_dpp(int old, int src, int ctrl, int row_mask, int bank_mask, b
   const int l = cur_lane(), row = l >> 4, bank = (l >> 2) & 3;
   if (!((row_mask >> row) & 1) || !((bank_mask >> bank) & 1)) return old;
   int from = -1;
-  if (ctrl >= 0x111 && ctrl <= 0x11f) {
+  if (ctrl >= 0 && ctrl <= 0xff) {
+    from = (l & ~3) | ((ctrl >> (2 * (l & 3))) & 3);  // quad_perm
+  } else if (ctrl >= 0x111 && ctrl <= 0x11f) {
     const int n = ctrl - 0x110;
     if ((l & 15) >= n) from = l - n;
+  } else if (ctrl == 0x140) {
+    from = (l & ~15) | (15 - (l & 15));  // row_mirror
+  } else if (ctrl == 0x141) {
+    from = (l & ~7) | (7 - (l & 7));  // row_half_mirror
   } else if (ctrl == 0x142) {
     if (row >= 1) from = row * 16 - 1;
   } else if (ctrl == 0x143) {
@@ -155,6 +162,15 @@ inline int update_dpp(int old, int src, int ctrl, int row_mask, int bank_mask, b
   }
   if (from < 0) return bound_ctrl ? 0 : old;
   return (int)all[from];
+}
+
+// __ballot(pred): the lanes whose predicate holds, as a 64-bit mask
+inline uint64_t ballot(int pred) {
+  int64_t all[kLanes];
+  exchange(pred != 0, all);
+  uint64_t m = 0;
+  for (int i = 0; i < kLanes; i++) m |= (uint64_t)(all[i] != 0) << i;
+  return m;
 }
 
 struct Thunk {
@@ -231,6 +247,8 @@ inline void run_workgroup(unsigned block, void (*fn)(void *), void *arg) {
 #define __builtin_amdgcn_update_dpp(old, src, ctrl, rm, bm, bc) hostwave::update_dpp((old), (src), (ctrl), (rm), (bm), (bc))
 
 inline int __clz(int x) { return x ? __builtin_clz((unsigned)x) : 32; }
+inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+#define __ballot(p) hostwave::ballot((p))
 inline int atomicAdd(int *p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 
 template <class A, class B>
