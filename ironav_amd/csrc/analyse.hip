@@ -147,16 +147,21 @@ struct Lds {
   int16_t st_mvs[16][2];  // sub-partition search state (enc/frame.rs:866)
   int8_t st_refs[16];
   int16_t mv8[16][2];
+  uint64_t nzm[2][16];  // masks of the nonzero levels and of the levels above 1 (count_bins)
+  uint64_t pos8[24];    // 8x8 scan positions of each significance (0-14) and last (15-23) context
 };
+
+// The workgroup's working set. Declared at namespace scope so that every function that touches it
+// addresses it as LDS (ds_read/ds_write) rather than through generic pointers.
+__shared__ Lds L;
 
 // ---- the slice coder ---------------------------------------------------------------------------
 
 struct Coder {
   const AnParams &P;
-  Lds &L;
   int mbx, mby, row0, row1, slice_id;
 
-  __device__ Coder(const AnParams &p, Lds &l) : P(p), L(l) {}
+  __device__ Coder(const AnParams &p) : P(p) {}
 
   __device__ int ls() const { return P.mb_w * 16; }
   __device__ int cs() const { return P.mb_w * 8; }
@@ -1559,206 +1564,191 @@ struct Coder {
     return (long long)wave_sum(s);  // at most 384 * 255^2 < 2^31
   }
 
-  // ---- CABAC bins (enc/frame.rs:2028-2279, enc/cabac.rs:150-246), counted or applied ----
+  // ---- CABAC bins (enc/frame.rs:2028-2279, enc/cabac.rs:150-246), counted over the lanes ----
+  //
+  // A macroblock's decision bins fall into chains that share no context: the header syntax
+  // (mb_skip_flag, mb_type, sub_mb_type, ref_idx, coded_block_pattern, transform_size_8x8_flag,
+  // intra_chroma_pred_mode, mb_qp_delta), the Intra4x4/8x8 mode flags, each component of the motion
+  // vector differences, and per residual block category its coded_block_flags, each significance
+  // and last position (for 8x8 blocks each significance context and each last context) and each
+  // level context. A lane takes one chain and codes its decisions in the writer's order, so every
+  // context passes through the states the writer's does and the costs summed over the lanes are the
+  // writer's. Bypass and terminate bins cost the same wherever they are counted.
 
-  struct Bins {
-    uint8_t *state;
-    uint32_t bits;
-    const uint32_t *cab;  // Lds::cab: one LDS read gives a decision's cost and next state
-    __device__ void decision(int ctx, int bin) {
-      const uint8_t s = state[ctx];
-      const uint32_t e = cab[s * 2 + (bin != (s & 1))];
-      bits += e >> 8;
-      state[ctx] = (uint8_t)(e & 0xffu);
-    }
-    __device__ void bypass(int) { bits += 256; }
-    __device__ void terminate(int bin) { bits += bin ? 7 * 256 : 1; }
-    __device__ void exp_golomb(uint32_t v, uint32_t k) {
-      while (v >= (1u << k)) {
-        bypass(1);
-        v -= 1u << k;
-        k++;
+  // per block category: coefficients per block, blocks per macroblock, significance/last lanes
+  // (positions 0..max-2; for 8x8 blocks 15 significance and 9 last contexts), level contexts
+  __device__ static int cat_max(int cat) { return cat == 0 || cat == 2 ? 16 : cat == 3 ? 4 : cat == 5 ? 64 : 15; }
+  __device__ static int cat_blocks(int cat) { return cat == 0 ? 1 : cat == 3 ? 2 : cat == 4 ? 8 : cat == 5 ? 4 : 16; }
+  __device__ static int cat_sig_jobs(int cat) { return cat == 5 ? 24 : cat_max(cat) - 1; }
+  __device__ static int cat_jobs(int cat) { return (cat != 5) + cat_sig_jobs(cat) + (cat == 3 ? 9 : 10); }
+
+  // one decision on a state held in a register: its cost in 1/256 bits, the next state
+  __device__ void dec(uint32_t &s, int bin, uint32_t &bits) const {
+    const uint32_t e = L.cab[s * 2 + (bin != (int)(s & 1))];
+    bits += e >> 8;
+    s = e & 0xffu;
+  }
+  __device__ void dec_at(uint8_t *st, int ctx, int bin, uint32_t &bits) const {
+    uint32_t s = st[ctx];
+    dec(s, bin, bits);
+    st[ctx] = (uint8_t)s;
+  }
+  // bins of the k-th order Exp-Golomb code of v (enc/cabac.rs:221-246): a unary prefix of t ones,
+  // its terminating zero and k + t suffix bits, t = floor(log2(v / 2^k + 1))
+  __device__ static int eg_bins(uint32_t v, int k) {
+    const int t = 31 - __clz((int)((v >> k) + 1));
+    return 2 * t + 1 + k;
+  }
+
+  // Masks (bit = scan position) of the residual blocks that are coded: L.nzm[0][w] their nonzero
+  // levels, L.nzm[1][w] the levels above 1 in magnitude. Words 0-3 hold the luma 4x4 blocks 4w..4w+3
+  // in coding order (16 bits each), 4-7 the luma 8x8 blocks, 8-9 the chroma AC blocks (component,
+  // block) four to a word, 10 the luma DC block (bits 0-15) and the chroma DC blocks (bits 16-19,
+  // 20-23); 11-15 stay zero. A lane per coefficient, a ballot per word.
+  __device__ void nz_masks() {
+    const ia_mbcode &c = L.code;
+    const int l = lane(), cbpl = c.cbp & 15, cbpc = c.cbp >> 4, i16 = c.kind == K_I16, t8 = c.t8x8;
+    auto put = [&](int w, int v) {
+      const uint64_t nz = __ballot(v != 0), g1 = __ballot(v > 1 || v < -1);
+      if (l == 0) {
+        L.nzm[0][w] = nz;
+        L.nzm[1][w] = g1;
       }
-      bypass(0);
-      while (k > 0) {
-        k--;
-        bypass((v >> k) & 1);
+    };
+    for (int w = 0; w < 4; w++) {
+      int v = 0;
+      if (!t8 && cbpl) {
+        const int n = 4 * w + (l >> 4), pos = l & 15;
+        int bx, by;
+        block_pos(n, &bx, &by);
+        const int16_t *blk = c.luma[by * 4 + bx];
+        if ((cbpl >> (n >> 2)) & 1) v = i16 ? (pos < 15 ? blk[pos + 1] : 0) : blk[pos];
+      }
+      put(w, v);
+    }
+    for (int b = 0; b < 4; b++) put(4 + b, t8 && ((cbpl >> b) & 1) ? c.luma8[b][l] : 0);
+    for (int w = 0; w < 2; w++) {
+      const int pos = l & 15;
+      put(8 + w, cbpc == 2 && pos < 15 ? c.cac[w][l >> 4][pos + 1] : 0);
+    }
+    {
+      int v = 0;
+      if (l < 16) v = i16 ? c.dc[l] : 0;
+      else if (l < 24) v = cbpc > 0 ? c.cdc[(l - 16) >> 2][(l - 16) & 3] : 0;
+      put(10, v);
+    }
+    if (l < 10) L.nzm[l / 5][11 + l % 5] = 0;
+  }
+
+  // The mask words of a category (four from its first: luma 4x4 0, 8x8 4, chroma AC 8, DC 10) and the
+  // mask of its block b from them.
+  __device__ static int cat_word(int cat) { return cat == 0 || cat == 3 ? 10 : cat == 4 ? 8 : cat == 5 ? 4 : 0; }
+  struct Words {
+    uint64_t w0, w1, w2, w3;
+    __device__ uint64_t word(int i) const { return i == 0 ? w0 : i == 1 ? w1 : i == 2 ? w2 : w3; }
+    __device__ uint64_t block(int cat, int b) const {
+      switch (cat) {
+        case 0: return w0 & 0xffffu;
+        case 3: return (w0 >> (16 + 4 * b)) & 15u;
+        case 5: return word(b);
+        default: return (word(b >> 2) >> (16 * (b & 3))) & 0xffffu;
       }
     }
   };
+  __device__ Words cat_words(int set, int cat) const {
+    const uint64_t *w = &L.nzm[set][cat_word(cat)];
+    return Words{w[0], w[1], w[2], w[3]};
+  }
 
-  __device__ static void write_block(Bins &c, int cat, int cbf_ctx, const int16_t *levels, int max) {
-    // the block's nonzero levels as a mask, read four at a time (independent reads share one wait)
-    uint64_t nz = 0;
-    int j = 0;
-    for (; j + 4 <= max; j += 4) {
-      const int16_t l0 = levels[j], l1 = levels[j + 1], l2 = levels[j + 2], l3 = levels[j + 3];
-      nz |= (uint64_t)((l0 != 0) | ((l1 != 0) << 1) | ((l2 != 0) << 2) | ((l3 != 0) << 3)) << j;
-    }
-    for (; j < max; j++) nz |= (uint64_t)(levels[j] != 0) << j;
-    const int total = __builtin_popcountll(nz), last = nz ? 63 - __builtin_clzll(nz) : -1;
-    if (cbf_ctx >= 0) c.decision(cbf_ctx, total > 0);
-    if (total == 0) return;
-    for (int i = 0; i < max - 1; i++) {
-      int sig, lst;
-      if (cat == 3) { sig = min(i, 2); lst = min(i, 2); }
-      else if (cat == 5) { sig = IA_SIG_OFF_8X8[0][i]; lst = IA_LAST_OFF_8X8[i]; }
-      else { sig = i; lst = i; }
-      const int nzi = (int)((nz >> i) & 1);
-      c.decision(IA_SIG_OFF[0][cat] + sig, nzi);
-      if (nzi) {
-        c.decision(IA_LAST_OFF[0][cat] + lst, i == last);
-        if (i == last) break;
-      }
-    }
-    const int base = IA_ABS_OFF[cat];
-    int eq1 = 0, gt1 = 0;
-    // the nonzero levels from the last to the first
-    for (uint64_t m = nz; m != 0;) {
-      const int i = 63 - __builtin_clzll(m);
-      m &= ~(1ull << i);
-      const int l = levels[i];
-      const uint32_t a = (uint32_t)iabs(l);
-      const int inc0 = gt1 != 0 ? 0 : min(1 + eq1, 4);
-      if (a == 1) {
-        c.decision(base + inc0, 0);
-        eq1++;
-      } else {
-        c.decision(base + inc0, 1);
-        const int incn = base + 5 + min(gt1, 4 - (cat == 3));
-        for (int k = 0; k < min((int)a - 2, 13); k++) c.decision(incn, 1);
-        if (a < 15) c.decision(incn, 0);
-        else c.exp_golomb(a - 15, 0);
-        gt1++;
-      }
-      c.bypass(l < 0);
+  // the level at scan position i of block b of a category
+  __device__ int blk_level(int cat, int b, int i) const {
+    const ia_mbcode &c = L.code;
+    int bx, by;
+    block_pos(b, &bx, &by);
+    switch (cat) {
+      case 0: return c.dc[i];
+      case 1: return c.luma[by * 4 + bx][i + 1];
+      case 2: return c.luma[by * 4 + bx][i];
+      case 3: return c.cdc[b][i];
+      case 4: return c.cac[b >> 2][b & 3][i + 1];
+      default: return c.luma8[b][i];
     }
   }
 
-  __device__ static void mvd_bins(Bins &c, int comp, int d, uint32_t sum) {
-    const int base = comp == 0 ? 40 : 47;
-    const int inc = sum < 3 ? 0 : (sum > 32 ? 2 : 1);
-    const uint32_t a = (uint32_t)iabs(d);
-    c.decision(base + inc, a > 0);
-    if (a > 0) {
-      int ctx = base + 3;
-      for (uint32_t k = 1; k < (a < 9 ? a : 9); k++) {
-        c.decision(ctx, 1);
-        if (ctx < base + 6) ctx++;
-      }
-      if (a < 9) c.decision(ctx, 0);
-      else c.exp_golomb(a - 9, 3);
-      c.bypass(d < 0);
-    }
-  }
-
-  __device__ void mb_bins(Bins &b, const ia_mbcode &code) const {
+  // mb_skip_flag, mb_type, transform_size_8x8_flag, intra_chroma_pred_mode, sub_mb_type, ref_idx,
+  // coded_block_pattern and mb_qp_delta (enc/frame.rs:2028-2110, :2150-2200)
+  __device__ uint32_t header_bins(uint8_t *st) const {
+    const ia_mbcode &code = L.code;
     const MbState *left = nb(-1, 0), *top = nb(0, -1);
     const int kind = code.kind;
+    uint32_t bits = 0;
     if (P.p_slice) {
-      b.decision(11 + (left && !left->skip) + (top && !top->skip), code.skip);
-      if (code.skip) return;
+      dec_at(st, 11 + (left && !left->skip) + (top && !top->skip), code.skip, bits);
+      if (code.skip) return bits;
     }
     const int cbpl = code.cbp & 15, cbpc = code.cbp >> 4, prefix_ok = P.p_slice, base = P.p_slice ? 17 : 3;
     if (kind == K_INTER) {
       const int b1 = code.part == 1 || code.part == 2, b2 = code.part == 1 || code.part == 3;
-      b.decision(14, 0);
-      b.decision(15, b1);
-      b.decision(b1 ? 17 : 16, b2);
+      dec_at(st, 14, 0, bits);
+      dec_at(st, 15, b1, bits);
+      dec_at(st, b1 ? 17 : 16, b2, bits);
     } else {
       int first;
       if (prefix_ok) {
-        b.decision(14, 1);
+        dec_at(st, 14, 1, bits);
         first = 17;
       } else {
         first = 3 + (left && !(left->kind == K_I4 || left->kind == K_I8)) + (top && !(top->kind == K_I4 || top->kind == K_I8));
       }
       if (kind == K_I4 || kind == K_I8) {
-        b.decision(first, 0);
-        if (P.transform8) b.decision(399 + (left && left->t8x8) + (top && top->t8x8), kind == K_I8);
+        dec_at(st, first, 0, bits);
+        if (P.transform8) dec_at(st, 399 + (left && left->t8x8) + (top && top->t8x8), kind == K_I8, bits);
       } else {
-        b.decision(first, 1);
-        b.terminate(0);
+        dec_at(st, first, 1, bits);
+        bits += 1;  // terminate bin 0
         const int b1 = prefix_ok ? base + 1 : base + 3, b2 = prefix_ok ? base + 2 : base + 4, b3 = prefix_ok ? base + 3 : base + 6;
-        b.decision(b1, cbpl != 0);
-        b.decision(b2, cbpc != 0);
-        if (cbpc != 0) b.decision(prefix_ok ? base + 2 : base + 5, cbpc == 2);
-        b.decision(b3, (code.i16_mode >> 1) == 1);
-        b.decision(prefix_ok ? base + 3 : base + 7, (code.i16_mode & 1) == 1);
-      }
-    }
-    if (kind == K_I4 || kind == K_I8) {
-      const int t8 = kind == K_I8;
-      uint8_t so_far[16];
-      for (int i = 0; i < 16; i++) so_far[i] = 2;
-      for (int n = 0; n < (t8 ? 4 : 16); n++) {
-        int bx, by;
-        if (t8) { bx = (n & 1) * 2; by = (n >> 1) * 2; }
-        else block_pos(n, &bx, &by);
-        const int mode = code.modes[by * 4 + bx];
-        const int pred = pred_i4_mode(so_far, bx, by);
-        so_far[by * 4 + bx] = (uint8_t)mode;
-        if (t8) {
-          so_far[by * 4 + bx + 1] = (uint8_t)mode;
-          so_far[(by + 1) * 4 + bx] = (uint8_t)mode;
-          so_far[(by + 1) * 4 + bx + 1] = (uint8_t)mode;
-        }
-        b.decision(68, mode == pred);
-        if (mode != pred) {
-          const int rem = mode < pred ? mode : mode - 1;
-          for (int k = 0; k < 3; k++) b.decision(69, (rem >> k) & 1);
-        }
+        dec_at(st, b1, cbpl != 0, bits);
+        dec_at(st, b2, cbpc != 0, bits);
+        if (cbpc != 0) dec_at(st, prefix_ok ? base + 2 : base + 5, cbpc == 2, bits);
+        dec_at(st, b3, (code.i16_mode >> 1) == 1, bits);
+        dec_at(st, prefix_ok ? base + 3 : base + 7, (code.i16_mode & 1) == 1, bits);
       }
     }
     if (kind != K_INTER) {
       const int m = code.chroma_mode;
       const int ca = left && !left->inter && left->chroma_mode != 0, cb = top && !top->inter && top->chroma_mode != 0;
-      b.decision(64 + ca + cb, m > 0);
+      dec_at(st, 64 + ca + cb, m > 0, bits);
       if (m > 0) {
-        b.decision(67, m > 1);
-        if (m > 1) b.decision(67, m > 2);
+        dec_at(st, 67, m > 1, bits);
+        if (m > 1) dec_at(st, 67, m > 2, bits);
       }
     } else {
       if (code.part == 3) {
         for (int k = 0; k < 4; k++) {
           const int s = code.sub[k];
-          if (s == 0) b.decision(21, 1);
-          else if (s == 1) { b.decision(21, 0); b.decision(22, 0); }
-          else if (s == 2) { b.decision(21, 0); b.decision(22, 1); b.decision(23, 1); }
-          else { b.decision(21, 0); b.decision(22, 1); b.decision(23, 0); }
+          dec_at(st, 21, s == 0, bits);
+          if (s != 0) {
+            dec_at(st, 22, s != 1, bits);
+            if (s != 1) dec_at(st, 23, s == 2, bits);
+          }
         }
       }
       if (P.nref > 1) {
-        uint8_t grid[4] = {0, 0, 0, 0};
-        Rect parts[4];
-        int np;
-        if (code.part == 3) {
-          np = 4;
-          for (int k = 0; k < 4; k++) parts[k] = Rect{(k & 1) * 2, (k >> 1) * 2, 2, 2};
-        } else {
-          np = NPARTS[code.part];
-          for (int k = 0; k < np; k++) parts[k] = Rect{PARTS[code.part][k][0], PARTS[code.part][k][1], PARTS[code.part][k][2], PARTS[code.part][k][3]};
-        }
+        uint32_t grid = 0;  // reference index per 8x8 block so far, a byte each
+        const int np = code.part == 3 ? 4 : NPARTS[code.part];
         for (int k = 0; k < np; k++) {
-          const int bx = parts[k].x, by = parts[k].y, w = parts[k].w, h = parts[k].h;
-          const int a = bx > 0 ? grid[(by / 2) * 2 + (bx - 1) / 2] > 0 : (left && left->refs[(by / 2) * 2 + 1] > 0);
-          const int bb = by > 0 ? grid[((by - 1) / 2) * 2 + bx / 2] > 0 : (top && top->refs[2 + bx / 2] > 0);
+          int bx, by, w, h;
+          if (code.part == 3) { bx = (k & 1) * 2; by = (k >> 1) * 2; w = h = 2; }
+          else { bx = PARTS[code.part][k][0]; by = PARTS[code.part][k][1]; w = PARTS[code.part][k][2]; h = PARTS[code.part][k][3]; }
+          const int a = bx > 0 ? ((grid >> (8 * ((by / 2) * 2 + (bx - 1) / 2))) & 0xff) > 0 : (left && left->refs[(by / 2) * 2 + 1] > 0);
+          const int bb = by > 0 ? ((grid >> (8 * (((by - 1) / 2) * 2 + bx / 2))) & 0xff) > 0 : (top && top->refs[2 + bx / 2] > 0);
           const int r = code.refs[k], cbase = 54 + a + 2 * bb;
-          for (int i = 0; i <= r; i++) b.decision(i == 0 ? cbase : (i == 1 ? 58 : 59), i < r);
+          for (int i = 0; i <= r; i++) dec_at(st, i == 0 ? cbase : (i == 1 ? 58 : 59), i < r, bits);
           for (int y8 = by / 2; y8 <= (by + h - 1) / 2; y8++)
-            for (int x8 = bx / 2; x8 <= (bx + w - 1) / 2; x8++) grid[y8 * 2 + x8] = code.refs[k];
-        }
-      }
-      Rect rs[16];
-      const int n = rects(code.part, code.sub, rs);
-      uint8_t grid[16][2];
-      mvd_grid(rs, n, code.mvds, grid);
-      for (int k = 0; k < n; k++) {
-        const int bx = rs[k].x, by = rs[k].y;
-        for (int comp = 0; comp < 2; comp++) {
-          const int a = bx > 0 ? grid[by * 4 + bx - 1][comp] : (left ? left->mvd[by * 4 + 3][comp] : 0);
-          const int bb = by > 0 ? grid[(by - 1) * 4 + bx][comp] : (top ? top->mvd[12 + bx][comp] : 0);
-          mvd_bins(b, comp, code.mvds[k][comp], (uint32_t)(a + bb));
+            for (int x8 = bx / 2; x8 <= (bx + w - 1) / 2; x8++) {
+              const int sh = 8 * (y8 * 2 + x8);
+              grid = (grid & ~(0xffu << sh)) | ((uint32_t)r << sh);
+            }
         }
       }
     }
@@ -1768,54 +1758,281 @@ struct Coder {
         const int cond_a = (b8 & 1) ? ((luma >> (b8 - 1)) & 1) == 0 : (left && ((left->cbp >> (b8 + 1)) & 1) == 0);
         const int cond_b = b8 >= 2 ? ((luma >> (b8 - 2)) & 1) == 0 : (top && ((top->cbp >> (b8 + 2)) & 1) == 0);
         const int bit = (cbpl >> b8) & 1;
-        b.decision(73 + cond_a + 2 * cond_b, bit);
+        dec_at(st, 73 + cond_a + 2 * cond_b, bit, bits);
         luma |= bit << b8;
       }
       int ca = left && (left->cbp >> 4) != 0, cb = top && (top->cbp >> 4) != 0;
-      b.decision(77 + ca + 2 * cb, cbpc != 0);
+      dec_at(st, 77 + ca + 2 * cb, cbpc != 0, bits);
       if (cbpc != 0) {
         ca = left && (left->cbp >> 4) == 2;
         cb = top && (top->cbp >> 4) == 2;
-        b.decision(81 + ca + 2 * cb, cbpc == 2);
+        dec_at(st, 81 + ca + 2 * cb, cbpc == 2, bits);
       }
     }
     if (kind == K_INTER && P.transform8 && cbpl != 0 && (code.part != 3 || (code.sub[0] | code.sub[1] | code.sub[2] | code.sub[3]) == 0))
-      b.decision(399 + (left && left->t8x8) + (top && top->t8x8), code.t8x8);
-    if (code.cbp != 0 || kind == K_I16) {
-      b.decision(60, 0);
-      const int un = kind != K_INTER;
-      if (kind == K_I16) {
-        const int da = left ? (left->cbf_dc & 1) != 0 : un, db = top ? (top->cbf_dc & 1) != 0 : un;
-        write_block(b, 0, 85 + da + 2 * db, code.dc, 16);
+      dec_at(st, 399 + (left && left->t8x8) + (top && top->t8x8), code.t8x8, bits);
+    if (code.cbp != 0 || kind == K_I16) dec_at(st, 60, 0, bits);
+    return bits;
+  }
+
+  // prev_intra_pred_mode_flag and rem_intra_pred_mode of the Intra4x4/8x8 blocks (contexts 68, 69);
+  // the modes coded so far four bits each in a register
+  __device__ uint32_t mode_bins(uint8_t *st) const {
+    const ia_mbcode &code = L.code;
+    if (code.kind != K_I4 && code.kind != K_I8) return 0;
+    const MbState *left = nb(-1, 0), *top = nb(0, -1);
+    const int t8 = code.kind == K_I8;
+    uint32_t s68 = st[68], s69 = st[69], bits = 0;
+    uint64_t so_far = 0x2222222222222222ull;
+    for (int n = 0; n < (t8 ? 4 : 16); n++) {
+      int bx, by;
+      if (t8) { bx = (n & 1) * 2; by = (n >> 1) * 2; }
+      else block_pos(n, &bx, &by);
+      const int mode = code.modes[by * 4 + bx];
+      int a = -1, b = -1;
+      if (bx > 0) a = (int)((so_far >> (4 * (by * 4 + bx - 1))) & 15);
+      else if (left) a = left->i4 ? left->modes[by * 4 + 3] : 2;
+      if (by > 0) b = (int)((so_far >> (4 * ((by - 1) * 4 + bx))) & 15);
+      else if (top) b = top->i4 ? top->modes[12 + bx] : 2;
+      const int pred = a >= 0 && b >= 0 ? min(a, b) : 2;
+      const uint64_t m = (uint64_t)mode;
+      const int i = by * 4 + bx;
+      so_far = (so_far & ~(15ull << (4 * i))) | (m << (4 * i));
+      if (t8) {
+        so_far = (so_far & ~(15ull << (4 * (i + 1)))) | (m << (4 * (i + 1)));
+        so_far = (so_far & ~(15ull << (4 * (i + 4)))) | (m << (4 * (i + 4)));
+        so_far = (so_far & ~(15ull << (4 * (i + 5)))) | (m << (4 * (i + 5)));
       }
-      const int cat = kind == K_I16 ? 1 : 2, cbase = kind == K_I16 ? 85 + 4 : 85 + 8;
-      for (int i8 = 0; i8 < 4; i8++)
-        if (code.t8x8 && (cbpl & (1 << i8))) write_block(b, 5, -1, code.luma8[i8], 64);
-      for (int n = 0; n < 16; n++) {
-        int bx, by;
-        block_pos(n, &bx, &by);
-        if (code.t8x8 || !(cbpl & (1 << (n / 4)))) continue;
-        const int a = bx > 0 ? code.nnz[by * 4 + bx - 1] > 0 : (left ? left->nnz[by * 4 + 3] > 0 : un);
-        const int bb = by > 0 ? code.nnz[(by - 1) * 4 + bx] > 0 : (top ? top->nnz[12 + bx] > 0 : un);
-        const int16_t *blk = code.luma[by * 4 + bx];
-        if (cat == 1) write_block(b, cat, cbase + a + 2 * bb, blk + 1, 15);
-        else write_block(b, cat, cbase + a + 2 * bb, blk, 16);
+      dec(s68, mode == pred, bits);
+      if (mode != pred) {
+        const int rem = mode < pred ? mode : mode - 1;
+        for (int k = 0; k < 3; k++) dec(s69, (rem >> k) & 1, bits);
       }
-      if (cbpc > 0)
-        for (int comp = 0; comp < 2; comp++) {
-          const int da = left ? (left->cbf_dc & (2 << comp)) != 0 : un, db = top ? (top->cbf_dc & (2 << comp)) != 0 : un;
-          write_block(b, 3, 85 + 12 + da + 2 * db, code.cdc[comp], 4);
-        }
-      if (cbpc == 2)
-        for (int comp = 0; comp < 2; comp++)
-          for (int blk = 0; blk < 4; blk++) {
-            const int bx = blk & 1, by = blk >> 1;
-            const uint8_t *cur = code.cnnz[comp];
-            const int a = bx > 0 ? cur[by * 2] > 0 : (left ? left->cnnz[comp][by * 2 + 1] > 0 : un);
-            const int bb = by > 0 ? cur[bx] > 0 : (top ? top->cnnz[comp][2 + bx] > 0 : un);
-            write_block(b, 4, 85 + 16 + a + 2 * bb, code.cac[comp][blk] + 1, 15);
-          }
     }
+    st[68] = (uint8_t)s68;
+    st[69] = (uint8_t)s69;
+    return bits;
+  }
+
+  // one component of the motion vector differences, partitions in coding order (enc/frame.rs:2110-2150);
+  // min(|mvd|, 64) of the 4x4 blocks coded so far a byte each in two registers
+  __device__ uint32_t mvd_comp_bins(uint8_t *st, int comp) const {
+    const ia_mbcode &code = L.code;
+    if (code.kind != K_INTER || code.skip) return 0;
+    const MbState *left = nb(-1, 0), *top = nb(0, -1);
+    const int base = comp == 0 ? 40 : 47, part = code.part;
+    uint64_t g[2] = {0, 0};
+    uint32_t bits = 0;
+    int k = 0;
+    for (int q = 0; q < (part == 3 ? 4 : NPARTS[part]); q++) {
+      const int sub = part == 3 ? code.sub[q] : 0;
+      for (int s = 0; s < (part == 3 ? NSUBS[sub] : 1); s++, k++) {
+        int bx, by, w, h;
+        if (part == 3) {
+          bx = (q & 1) * 2 + SUBS[sub][s][0];
+          by = (q >> 1) * 2 + SUBS[sub][s][1];
+          w = SUBS[sub][s][2];
+          h = SUBS[sub][s][3];
+        } else {
+          bx = PARTS[part][q][0];
+          by = PARTS[part][q][1];
+          w = PARTS[part][q][2];
+          h = PARTS[part][q][3];
+        }
+        auto at = [&](int i) { return (int)((g[i >> 3] >> (8 * (i & 7))) & 0xff); };
+        const int a = bx > 0 ? at(by * 4 + bx - 1) : (left ? left->mvd[by * 4 + 3][comp] : 0);
+        const int bb = by > 0 ? at((by - 1) * 4 + bx) : (top ? top->mvd[12 + bx][comp] : 0);
+        const uint32_t sum = (uint32_t)(a + bb);
+        const int d = code.mvds[k][comp];
+        const uint32_t ad = (uint32_t)iabs(d);
+        dec_at(st, base + (sum < 3 ? 0 : (sum > 32 ? 2 : 1)), ad > 0, bits);
+        if (ad > 0) {
+          int ctx = base + 3;
+          for (uint32_t t = 1; t < (ad < 9 ? ad : 9); t++) {
+            dec_at(st, ctx, 1, bits);
+            if (ctx < base + 6) ctx++;
+          }
+          if (ad < 9) dec_at(st, ctx, 0, bits);
+          else bits += 256u * (uint32_t)eg_bins(ad - 9, 3);
+          bits += 256;  // sign
+        }
+        const uint64_t v = (uint64_t)min((int)ad, 64);
+        for (int y = by; y < by + h; y++)
+          for (int x = bx; x < bx + w; x++) {
+            const int i = y * 4 + x, sh = 8 * (i & 7);
+            if (i < 8) g[0] = (g[0] & ~(0xffull << sh)) | (v << sh);
+            else g[1] = (g[1] & ~(0xffull << sh)) | (v << sh);
+          }
+      }
+    }
+    return bits;
+  }
+
+  // coded_block_flag of each coded block of a category (contexts 85 + 4 cat + a + 2 b), the four
+  // states packed in a register
+  __device__ uint32_t cbf_bins(uint8_t *st, int cat) const {
+    const ia_mbcode &code = L.code;
+    const MbState *left = nb(-1, 0), *top = nb(0, -1);
+    const int un = code.kind != K_INTER, cbpl = code.cbp & 15, ctx0 = 85 + 4 * cat;
+    const Words nzw = cat_words(0, cat);
+    uint32_t pk = (uint32_t)st[ctx0] | ((uint32_t)st[ctx0 + 1] << 8) | ((uint32_t)st[ctx0 + 2] << 16) | ((uint32_t)st[ctx0 + 3] << 24);
+    uint32_t bits = 0;
+    for (int b = 0; b < cat_blocks(cat); b++) {
+      int a, bb;
+      if (cat == 0) {
+        a = left ? (left->cbf_dc & 1) != 0 : un;
+        bb = top ? (top->cbf_dc & 1) != 0 : un;
+      } else if (cat == 1 || cat == 2) {
+        if (code.t8x8 || !((cbpl >> (b >> 2)) & 1)) continue;
+        int bx, by;
+        block_pos(b, &bx, &by);
+        a = bx > 0 ? code.nnz[by * 4 + bx - 1] > 0 : (left ? left->nnz[by * 4 + 3] > 0 : un);
+        bb = by > 0 ? code.nnz[(by - 1) * 4 + bx] > 0 : (top ? top->nnz[12 + bx] > 0 : un);
+      } else if (cat == 3) {
+        a = left ? (left->cbf_dc & (2 << b)) != 0 : un;
+        bb = top ? (top->cbf_dc & (2 << b)) != 0 : un;
+      } else {
+        const int comp = b >> 2, bx = b & 1, by = (b >> 1) & 1;
+        const uint8_t *cur = code.cnnz[comp];
+        a = bx > 0 ? cur[by * 2] > 0 : (left ? left->cnnz[comp][by * 2 + 1] > 0 : un);
+        bb = by > 0 ? cur[bx] > 0 : (top ? top->cnnz[comp][2 + bx] > 0 : un);
+      }
+      const int sh = 8 * (a + 2 * bb);
+      uint32_t s = (pk >> sh) & 0xffu;
+      dec(s, nzw.block(cat, b) != 0, bits);
+      pk = (pk & ~(0xffu << sh)) | (s << sh);
+    }
+    for (int i = 0; i < 4; i++) st[ctx0 + i] = (uint8_t)(pk >> (8 * i));
+    return bits;
+  }
+
+  // significance map lane j of a category: positions j of every coded block (significant_coeff_flag
+  // and last_significant_coeff_flag on contexts sig + j, last + j); for 8x8 blocks the positions of
+  // significance context j (j < 15) or of last context j - 15
+  __device__ uint32_t sig_bins(uint8_t *st, int cat, int j) const {
+    const int max = cat_max(cat);
+    uint64_t smask, tmask;
+    int sctx, lctx;
+    if (cat == 5) {
+      smask = j < 15 ? L.pos8[j] : 0;
+      tmask = j < 15 ? 0 : L.pos8[j];
+      sctx = IA_SIG_OFF[0][5] + j;
+      lctx = IA_LAST_OFF[0][5] + j - 15;
+    } else {
+      smask = tmask = 1ull << j;
+      sctx = IA_SIG_OFF[0][cat] + j;
+      lctx = IA_LAST_OFF[0][cat] + j;
+    }
+    const Words nzw = cat_words(0, cat);
+    uint32_t ss = smask ? st[sctx] : 0, ts = tmask ? st[lctx] : 0, bits = 0;
+    for (int b = 0; b < cat_blocks(cat); b++) {
+      const uint64_t m = nzw.block(cat, b);
+      if (m == 0) continue;
+      const int last = 63 - __builtin_clzll(m);
+      uint64_t q = (smask | tmask) & ((2ull << min(last, max - 2)) - 1);
+      while (q) {
+        const int p = __builtin_ctzll(q);
+        q &= q - 1;
+        const int nzp = (int)((m >> p) & 1);
+        if ((smask >> p) & 1) dec(ss, nzp, bits);
+        if (((tmask >> p) & 1) && nzp) dec(ts, p == last, bits);
+      }
+    }
+    if (smask) st[sctx] = (uint8_t)ss;
+    if (tmask) st[lctx] = (uint8_t)ts;
+    return bits;
+  }
+
+  // Level context k of a category (coeff_abs_level_minus1). The levels of a block are coded from the
+  // last to the first; while no level above 1 has been coded, the r-th takes its first bin on context
+  // min(1 + r, 4), after the first such level on context 0; the first bin is 1 for the levels above 1
+  // only. The j-th level above 1 codes its further bins on context 5 + min(j, 4) (5 + min(j, 3) for
+  // chroma DC). So the masks of nonzero levels and of levels above 1 give each context its bins; only
+  // the levels above 1 are read. Lane k = 0 also counts the bypass bins of the signs, the lanes of the
+  // further bins the Exp-Golomb suffixes of their levels.
+  __device__ uint32_t level_bins(uint8_t *st, int cat, int k) const {
+    const int ctx = IA_ABS_OFF[cat] + k, gmax = 4 - (cat == 3);
+    const Words nzw = cat_words(0, cat), g1w = cat_words(1, cat);
+    uint32_t s = st[ctx], bits = 0;
+    for (int b = 0; b < cat_blocks(cat); b++) {
+      const uint64_t m = nzw.block(cat, b);
+      if (m == 0) continue;
+      const uint64_t g = g1w.block(cat, b);
+      const int n = __popcll(m);
+      // levels coded before the first level above 1 (all of them without one)
+      const int hg = g ? 63 - __builtin_clzll(g) : -1;
+      const int r1 = g ? __popcll(m >> hg) - 1 : n;  // its index from the last level
+      if (k == 0) {
+        bits += 256u * (uint32_t)n;  // signs
+        // the levels after the first level above 1, from the last to the first
+        for (uint64_t q = g ? m & ((1ull << hg) - 1) : 0; q != 0;) {
+          const int i = 63 - __builtin_clzll(q);
+          q &= ~(1ull << i);
+          dec(s, (int)((g >> i) & 1), bits);
+        }
+      } else if (k < 4) {
+        if (k - 1 < n && k - 1 <= r1) dec(s, k - 1 == r1, bits);
+      } else if (k == 4) {
+        for (int r = 3; r < min(r1, n); r++) dec(s, 0, bits);
+        if (r1 >= 3 && r1 < n) dec(s, 1, bits);
+      } else {
+        // the levels above 1 whose further bins use this context: the (k - 5)-th, or from the
+        // gmax-th on
+        int j = 0;
+        for (uint64_t q = g; q != 0; j++) {
+          const int i = 63 - __builtin_clzll(q);
+          q &= ~(1ull << i);
+          if (5 + min(j, gmax) != k) continue;
+          const int a = iabs(blk_level(cat, b, i));
+          for (int t = 0; t < min(a - 2, 13); t++) dec(s, 1, bits);
+          if (a < 15) dec(s, 0, bits);
+          else bits += 256u * (uint32_t)eg_bins((uint32_t)(a - 15), 0);
+        }
+      }
+    }
+    st[ctx] = (uint8_t)s;
+    return bits;
+  }
+
+  // The rate of the macroblock in L.code in 1/256 bits, coded on the context states st (which it
+  // leaves as the writer would); the same in every lane.
+  __device__ uint32_t count_bins(uint8_t *st) {
+    const ia_mbcode &code = L.code;
+    nz_masks();
+    sync();
+    const int cbpl = code.cbp & 15, cbpc = code.cbp >> 4, i16 = code.kind == K_I16, t8 = code.t8x8;
+    // the categories coded, in a fixed order, four bits each
+    uint32_t cats = 0;
+    int ncat = 0, total = 4;
+    auto add = [&](int cat) {
+      cats |= (uint32_t)cat << (4 * ncat++);
+      total += cat_jobs(cat);
+    };
+    if (i16) add(0);
+    if (cbpl && !t8) add(i16 ? 1 : 2);
+    if (cbpl && t8) add(5);
+    if (cbpc > 0) add(3);
+    if (cbpc == 2) add(4);
+    uint32_t bits = 0;
+    for (int j = lane(); j < total; j += 64) {
+      if (j == 0) bits += header_bins(st);
+      else if (j == 1) bits += mode_bins(st);
+      else if (j < 4) bits += mvd_comp_bins(st, j - 2);
+      else {
+        int t = j - 4, ci = 0;
+        while (t >= cat_jobs((cats >> (4 * ci)) & 15)) t -= cat_jobs((cats >> (4 * ci++)) & 15);
+        const int cat = (cats >> (4 * ci)) & 15;
+        if (cat != 5 && t == 0) bits += cbf_bins(st, cat);
+        else {
+          t -= cat != 5;
+          if (t < cat_sig_jobs(cat)) bits += sig_bins(st, cat, t);
+          else bits += level_bins(st, cat, t - cat_sig_jobs(cat));
+        }
+      }
+    }
+    const uint32_t total_bits = (uint32_t)wave_sum((int)bits);
+    sync();
+    return total_bits;
   }
 
   __device__ static void mvd_grid(const Rect *rs, int n, const int16_t (*mvds)[2], uint8_t (*out)[2]) {
@@ -1828,18 +2045,14 @@ struct Coder {
 
   // J of the candidate in L.code whose reconstruction is in the canvas (enc/frame.rs:1453-1457)
   __device__ double rd_cost(double lam2) {
-    for (int i = lane(); i < 1024; i += 64) L.tmp[i] = L.cabac[i];
-    sync();
-    uint32_t bits = 0;
-    if (lane() == 0) {
-      Bins b{L.tmp, 0, L.cab};
-      mb_bins(b, L.code);
-      L.ival[0] = (int)b.bits;
+    {
+      const uint32_t *s = reinterpret_cast<const uint32_t *>(L.cabac);
+      uint32_t *d = reinterpret_cast<uint32_t *>(L.tmp);
+      for (int i = lane(); i < 256; i += 64) d[i] = s[i];
     }
     sync();
-    bits = (uint32_t)L.ival[0];
+    const uint32_t bits = count_bins(L.tmp);
     const long long ssd = ssd_mb();
-    sync();
     return (double)ssd + lam2 * ((double)bits / 256.0);
   }
 
@@ -2117,20 +2330,16 @@ struct Coder {
     const uint32_t *s = reinterpret_cast<const uint32_t *>(&L.code);
     uint32_t *d = reinterpret_cast<uint32_t *>(&P.codes[idx]);
     for (int i = lane(); i < (int)(sizeof(ia_mbcode) / 4); i += 64) d[i] = s[i];
+    // the chosen macroblock's bins adapt the live contexts as the writer would; their count is its rate
+    const uint32_t bits = P.use_cabac ? count_bins(L.cabac) : 0;
     if (lane() == 0) {
       MbState ms;
       ia_mbinfo info;
       finish_mb(L.code, ms, info);
       P.state[idx] = ms;
       P.mbs[idx] = info;
-      // the chosen macroblock's bins adapt the live contexts as the writer would; their count is its rate
       ia_mbstats out = st;
-      out.bits = 0;
-      if (P.use_cabac) {
-        Bins b{L.cabac, 0, L.cab};
-        mb_bins(b, L.code);
-        out.bits = (int32_t)b.bits;
-      }
+      out.bits = (int32_t)bits;
       P.stats[idx] = out;
     }
     __threadfence_block();
@@ -2139,8 +2348,8 @@ struct Coder {
 };
 
 // Slice s of the picture p describes, its macroblocks in raster order.
-__device__ void analyse_slice(const AnParams &p, int s, Lds &lds) {
-  Coder c(p, lds);
+__device__ void analyse_slice(const AnParams &p, int s) {
+  Coder c(p);
   c.row0 = s * p.mb_h / p.nslices;
   c.row1 = (s + 1) * p.mb_h / p.nslices;
   c.slice_id = s + 1;
@@ -2151,14 +2360,22 @@ __device__ void analyse_slice(const AnParams &p, int s, Lds &lds) {
       const int m = p.p_slice ? IA_CABAC_INIT_P[i][0] : IA_CABAC_INIT_I[i][0];
       const int n = p.p_slice ? IA_CABAC_INIT_P[i][1] : IA_CABAC_INIT_I[i][1];
       const int pre = clampi(((m * qp) >> 4) + n, 1, 126);
-      lds.cabac[i] = pre <= 63 ? (uint8_t)((63 - pre) << 1) : (uint8_t)(((pre - 64) << 1) | 1);
+      L.cabac[i] = pre <= 63 ? (uint8_t)((63 - pre) << 1) : (uint8_t)(((pre - 64) << 1) | 1);
+    }
+    // the 8x8 scan positions (0-62) that code on each significance and each last context
+    if (threadIdx.x < 24) {
+      const int j = threadIdx.x;
+      uint64_t m = 0;
+      for (int q = 0; q < 63; q++)
+        if ((j < 15 ? IA_SIG_OFF_8X8[0][q] : IA_LAST_OFF_8X8[q] + 15) == j) m |= 1ull << q;
+      L.pos8[j] = m;
     }
     // a decision's cost and next state per (state, LPS?): state s = pStateIdx << 1 | valMPS
     for (int i = threadIdx.x; i < 256; i += 64) {
       const int st = i >> 1, lps = i & 1, ps = st >> 1, mps = st & 1;
       const uint32_t ns = lps ? (uint32_t)((IA_TRANS_LPS[ps] << 1) | (ps == 0 ? !mps : mps))
                               : (uint32_t)(((ps < 62 ? ps + 1 : ps) << 1) | mps);
-      lds.cab[i] = (p.entropy[ps * 2 + lps] << 8) | ns;
+      L.cab[i] = (p.entropy[ps * 2 + lps] << 8) | ns;
     }
   }
   __syncthreads();
@@ -2178,8 +2395,7 @@ __device__ void analyse_slice(const AnParams &p, int s, Lds &lds) {
 // private segment grows by 64 bytes per lane). The kernel is latency-bound, so the added waves are
 // nearly free throughput: 1.9x on 64 streams of mixed sizes.
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void analyse_kernel(AnParams p) {
-  __shared__ Lds lds;
-  analyse_slice(p, blockIdx.x, lds);
+  analyse_slice(p, blockIdx.x);
 }
 
 // Many pictures at once: a pool of workgroups takes (picture, slice) tasks from a queue, tasks[t] =
@@ -2188,7 +2404,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void an
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void analyse_batch_kernel(const AnParams *ps,
                                                                                                     const int *tasks,
                                                                                                     int ntasks, int *head) {
-  __shared__ Lds lds;
   __shared__ int next;
   for (;;) {
     __syncthreads();  // the previous task is done with `next` and the LDS
@@ -2197,7 +2412,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void an
     const int t = next;
     if (t >= ntasks) return;
     const int e = __builtin_amdgcn_readfirstlane(tasks[t]);
-    analyse_slice(ps[e >> 16], e & 0xffff, lds);
+    analyse_slice(ps[e >> 16], e & 0xffff);
   }
 }
 
