@@ -62,15 +62,23 @@ template <int CTRL, int ROW_MASK, int BANK_MASK>
 __device__ __forceinline__ uint32_t dpp_max(uint32_t v) {
   return max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, BANK_MASK, false));
 }
-// wave-wide maximum (the row-shift sequence suits idempotent operations)
-__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
-  v = dpp_max<0x111, 0xf, 0xf>(v);
-  v = dpp_max<0x112, 0xf, 0xf>(v);
-  v = dpp_max<0x114, 0xf, 0xf>(v);
-  v = dpp_max<0x118, 0xf, 0xf>(v);
-  v = dpp_max<0x142, 0xa, 0xf>(v);
-  v = dpp_max<0x143, 0xc, 0xf>(v);
-  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+// three independent wave-wide maxima (the row-shift sequence suits idempotent operations), their DPP
+// steps interleaved so that each step's latency is covered by the other two chains
+__device__ __forceinline__ void wave_max3(uint32_t &a, uint32_t &b, uint32_t &c) {
+#define IA_MAX3_STEP(CTRL, RM)       \
+  a = dpp_max<CTRL, RM, 0xf>(a);     \
+  b = dpp_max<CTRL, RM, 0xf>(b);     \
+  c = dpp_max<CTRL, RM, 0xf>(c);
+  IA_MAX3_STEP(0x111, 0xf)
+  IA_MAX3_STEP(0x112, 0xf)
+  IA_MAX3_STEP(0x114, 0xf)
+  IA_MAX3_STEP(0x118, 0xf)
+  IA_MAX3_STEP(0x142, 0xa)
+  IA_MAX3_STEP(0x143, 0xc)
+#undef IA_MAX3_STEP
+  a = (uint32_t)__builtin_amdgcn_readlane((int)a, 63);
+  b = (uint32_t)__builtin_amdgcn_readlane((int)b, 63);
+  c = (uint32_t)__builtin_amdgcn_readlane((int)c, 63);
 }
 
 // 41 partition words from 16 raster 4x4 words (as in the exhaustive search)
@@ -170,27 +178,34 @@ __device__ void conv_layer(int lane, const LdsU16 *in, int ist, const LdsU16 *w,
   f32x4 bv[NT];
 #pragma unroll
   for (int nt = 0; nt < NT; nt++) bv[nt] = *(const LdsF32x4 *)(bias + nt * 16 + 4 * g);
+  // K steps outermost: the 6 x NT accumulators are independent, so consecutive MFMAs never wait on
+  // each other; each accumulator still sums its K steps in order
   f32x4 acc[6][NT];
+  int pp[6];
 #pragma unroll
   for (int mt = 0; mt < 6; mt++) {
-    const int p = min(mt * 16 + m, CNN_CELLS - 1);  // this lane's B column (grid cell)
-    const int pp = padpos(p);
+    pp[mt] = padpos(min(mt * 16 + m, CNN_CELLS - 1));  // this lane's B column (grid cell) of tile mt
 #pragma unroll
     for (int nt = 0; nt < NT; nt++) acc[mt][nt] = bv[nt];
+  }
 #pragma unroll
-    for (int ks = 0; ks < KSTEPS; ks++) {
-      bf16x8 x;
+  for (int ks = 0; ks < KSTEPS; ks++) {
+    bf16x8 x[6];
+#pragma unroll
+    for (int mt = 0; mt < 6; mt++) {
       if (IN0) {
         const int t = ks * 4 + g;
-        x = t < 9 ? *(const LdsBf16x8 *)(in + (pp + tapoff(t)) * ist) : bf16x8{};
+        x[mt] = t < 9 ? *(const LdsBf16x8 *)(in + (pp[mt] + tapoff(t)) * ist) : bf16x8{};
       } else {
-        x = *(const LdsBf16x8 *)(in + (pp + tapoff(ks)) * ist + g * 8);
+        x[mt] = *(const LdsBf16x8 *)(in + (pp[mt] + tapoff(ks)) * ist + g * 8);
       }
-      // one activation fragment feeds every output tile
+    }
+    // one activation fragment feeds every output tile
+#pragma unroll
+    for (int mt = 0; mt < 6; mt++)
 #pragma unroll
       for (int nt = 0; nt < NT; nt++)
-        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[ks][nt], x, acc[mt][nt], 0, 0, 0);
-    }
+        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[ks][nt], x[mt], acc[mt][nt], 0, 0, 0);
   }
   wave_sync();
   // D: this lane holds output channels nt*16 + 4g + i, i = 0..3, of cell mt*16 + m
@@ -443,16 +458,20 @@ __device__ void propose_mb(PropLds &L, WaveLds &V, const PropArgs &a, int mb, in
       k1[r] = lane + 64 < CNN_CELLS ? (ford(V.out[lane + 64][r]) & ~0x7fu) | (uint32_t)(127 - lane - 64) : 0u;
       mine[r] = 0;
     }
+    // the three references' selections side by side (those beyond nref are not stored)
 #pragma unroll
-    for (int k = 0; k < CNN_TOPK; k++)
+    for (int k = 0; k < CNN_TOPK; k++) {
+      uint32_t m0 = max(k0[0], k1[0]), m1 = max(k0[1], k1[1]), m2 = max(k0[2], k1[2]);
+      wave_max3(m0, m1, m2);
+      const uint32_t mm[3] = {m0, m1, m2};
 #pragma unroll
-      for (int r = 0; r < 3; r++)
-        if (r < nref) {
-          const int cell = 127 - (int)(wave_max(max(k0[r], k1[r])) & 0x7f);
-          if (cell == lane) k0[r] = 0;
-          if (cell == lane + 64) k1[r] = 0;
-          if (lane == k) mine[r] = cell;
-        }
+      for (int r = 0; r < 3; r++) {
+        const int cell = 127 - (int)(mm[r] & 0x7f);
+        if (cell == lane) k0[r] = 0;
+        if (cell == lane + 64) k1[r] = 0;
+        if (lane == k) mine[r] = cell;
+      }
+    }
     wave_sync();
 #pragma unroll
     for (int r = 0; r < 3; r++)
