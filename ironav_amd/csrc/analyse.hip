@@ -149,6 +149,7 @@ struct Lds {
   int16_t mv8[16][2];
   uint64_t nzm[2][16];  // masks of the nonzero levels and of the levels above 1 (count_bins)
   uint64_t pos8[24];    // 8x8 scan positions of each significance (0-14) and last (15-23) context
+  int etop[16], eleft[8], etl;  // the edges of the Intra4x4 block being coded
 };
 
 // The workgroup's working set. Declared at namespace scope so that every function that touches it
@@ -1049,6 +1050,7 @@ struct Coder {
     code_new(K_I4);
     const int ta = up(), la = mbx > 0, tla = mbx > 0 && ta, tra = ta && mbx + 1 < P.mb_w;
     const int qp = P.qp;
+    const int l = lane();
     int cost = 0;
     for (int n = 0; n < 16; n++) {
       int bx, by;
@@ -1063,48 +1065,111 @@ struct Coder {
       if (by == 0) have_tr = bx < 3 ? ta : tra;
       else if (bx == 3) have_tr = 0;
       else have_tr = zorder(bx + 1, by - 1) < zorder(bx, by);
-      int top[16] = {}, left[8] = {}, tl = 0;
-      if (have_top) {
-        for (int i = 0; i < 4; i++) top[i] = cyv(x + i, y - 1);
-        for (int i = 4; i < 8; i++) top[i] = have_tr ? cyv(x + i, y - 1) : top[3];
-      }
-      if (have_left)
-        for (int j = 0; j < 4; j++) left[j] = cyv(x - 1, y + j);
-      if (have_tl) tl = cyv(x - 1, y - 1);
+      // the block's edges, shared in LDS: lanes 0-7 the row above (4-7 above-right, or sample 3
+      // repeated), 8-11 the column to the left, 12 the corner; zero where missing
+      if (l < 8) L.etop[l] = have_top ? cyv(x + (l < 4 || have_tr ? l : 3), y - 1) : 0;
+      else if (l < 12) L.eleft[l - 8] = have_left ? cyv(x - 1, y + l - 8) : 0;
+      else if (l == 12) L.etl = have_tl ? cyv(x - 1, y - 1) : 0;
+      sync();
+      const int tl = L.etl;
       const int predicted = pred_i4_mode(L.code.modes, bx, by);
-      // lane m < 9: mode m's prediction and SATD
-      int c = 0x7fffffff;
-      const int m = lane();
-      if (m < 9 && mode_ok(m, have_top, have_left, have_tl)) {
-        int d[16];
-        for (int i = 0; i < 16; i++)
-          d[i] = (int)L.src_y[(y + (i >> 2)) * 16 + x + (i & 3)] - nxn_sample(4, m, i & 3, i >> 2, top, left, tl, have_top, have_left);
-        c = satd4(d) + P.lambda * (m == predicted ? 1 : 4);
+      // the nine modes' SATDs, four modes a pass: lane 16 g + i takes sample i (row i >> 2, column
+      // i & 3) of mode 4 pass + g; the horizontal Hadamard within the lane's quad (a row), the
+      // vertical one through LDS, the absolute values summed over the 16 lanes (enc/frame.rs:641-659)
+      int cpass[3];
+#pragma unroll
+      for (int pass = 0; pass < 3; pass++) {
+        const int m = 4 * pass + (l >> 4), i = l & 15, xx = i & 3, yy = i >> 2;
+        const bool ok = m < 9 && mode_ok(m, have_top, have_left, have_tl);
+        int v = 0;
+        if (ok) v = (int)L.src_y[(y + yy) * 16 + x + xx] - nxn_sample(4, m, xx, yy, L.etop, L.eleft, tl, have_top, have_left);
+        int p = __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, false);  // quad_perm [1, 0, 3, 2]
+        v = (xx & 1) ? p - v : v + p;
+        p = __builtin_amdgcn_update_dpp(0, v, 0x4e, 0xf, 0xf, false);  // quad_perm [2, 3, 0, 1]
+        v = (xx & 2) ? p - v : v + p;
+        L.ival[l] = v;
+        sync();
+        const int *col = &L.ival[(l & ~15) | xx];
+        const int a0 = col[0], a1 = col[4], a2 = col[8], a3 = col[12];
+        const int s0 = a0 + a1, s1 = a2 + a3, d0 = a0 - a1, d1 = a2 - a3;
+        const int o = yy == 0 ? s0 + s1 : yy == 1 ? s0 - s1 : yy == 2 ? d0 - d1 : d0 + d1;
+        const int sum = group_sum(iabs(o), 16) >> 1;
+        cpass[pass] = ok ? sum + P.lambda * (m == predicted ? 1 : 4) : 0x7fffffff;
+        sync();
       }
       // the first mode of least cost (strict <, in mode order)
       int best_m = 2, best_c = 0x7fffffff;
+#pragma unroll
       for (int k = 0; k < 9; k++) {
-        int ck = __builtin_amdgcn_readlane(c, k);
+        const int ck = __builtin_amdgcn_readlane(cpass[k >> 2], 16 * (k & 3));
         if (ck < best_c) { best_c = ck; best_m = k; }
       }
       cost += best_c;
-      // lane 0 codes the block: prediction, transform, quantisation, reconstruction
-      if (lane() == 0) {
-        int d[16];
-        for (int i = 0; i < 16; i++) {
-          const int pv = nxn_sample(4, best_m, i & 3, i >> 2, top, left, tl, have_top, have_left);
-          d[i] = (int)L.src_y[(y + (i >> 2)) * 16 + x + (i & 3)] - pv;
-          cyv(x + (i & 3), y + (i >> 2)) = (uint8_t)pv;
-        }
-        int16_t *lv = L.code.luma[by * 4 + bx];
-        int32_t co[16];
-        quant_block(d, qp, 1, 0, lv, co);
-        int nz = count_nz(lv, 16);
-        if (nz) idct4_add(co, &cyv(x, y), 28);
+      // the block coded with it (enc/frame.rs:1025-1046, h264/idct.rs:8-32): lane i < 16 predicts
+      // sample i into the canvas and quantises coefficient i of the scan; the transforms take a lane
+      // per row, then per column, through L.co of the block
+      int32_t *co = L.co[by * 4 + bx];
+      if (l < 16) {
+        const int pv = nxn_sample(4, best_m, l & 3, l >> 2, L.etop, L.eleft, tl, have_top, have_left);
+        co[l] = (int)L.src_y[(y + (l >> 2)) * 16 + x + (l & 3)] - pv;
+        cyv(x + (l & 3), y + (l >> 2)) = (uint8_t)pv;
+      }
+      sync();
+      if (l < 4) {  // rows
+        int32_t *a = co + 4 * l;
+        const int s03 = a[0] + a[3], d03 = a[0] - a[3], s12 = a[1] + a[2], d12 = a[1] - a[2];
+        a[0] = s03 + s12;
+        a[1] = 2 * d03 + d12;
+        a[2] = s03 - s12;
+        a[3] = d03 - 2 * d12;
+      }
+      sync();
+      if (l < 4) {  // columns
+        int32_t *a = co + l;
+        const int s03 = a[0] + a[12], d03 = a[0] - a[12], s12 = a[4] + a[8], d12 = a[4] - a[8];
+        a[0] = s03 + s12;
+        a[4] = 2 * d03 + d12;
+        a[8] = s03 - s12;
+        a[12] = d03 - 2 * d12;
+      }
+      sync();
+      int lvl = 0, pz = 0;
+      if (l < 16) {
+        const int qbits = 15 + qp / 6, f = (1 << qbits) / 3, m = qp % 6;
+        pz = ZIG4[l];
+        lvl = quant(co[pz], IA_MF[m][class4(pz)], f, qbits);
+        L.code.luma[by * 4 + bx][l] = (int16_t)lvl;
+      }
+      const int nz = count_lanes(lvl != 0);
+      sync();
+      if (l < 16) co[pz] = shl(lvl * IA_V[qp % 6][class4(pz)], qp / 6);
+      if (l == 0) {
         L.code.nnz[by * 4 + bx] = (uint8_t)nz;
         L.code.modes[by * 4 + bx] = (uint8_t)best_m;
       }
       sync();
+      if (nz) {
+        if (l < 4) {  // rows
+          int32_t *d = co + 4 * l;
+          const int e0 = wadd(d[0], d[2]), e1 = wsub(d[0], d[2]), e2 = wsub(d[1] >> 1, d[3]), e3 = wadd(d[1], d[3] >> 1);
+          d[0] = wadd(e0, e3);
+          d[1] = wadd(e1, e2);
+          d[2] = wsub(e1, e2);
+          d[3] = wsub(e0, e3);
+        }
+        sync();
+        if (l < 4) {  // columns, added to the prediction
+          const int32_t *t = co + l;
+          const int g0 = wadd(t[0], t[8]), g1 = wsub(t[0], t[8]), g2 = wsub(t[4] >> 1, t[12]), g3 = wadd(t[4], t[12] >> 1);
+          const int hh[4] = {wadd(g0, g3), wadd(g1, g2), wsub(g1, g2), wsub(g0, g3)};
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            uint8_t &q = cyv(x + l, y + i);
+            q = (uint8_t)clampi(q + (wadd(hh[i], 32) >> 6), 0, 255);
+          }
+        }
+        sync();
+      }
     }
     if (lane() == 0) {
       int cbp = 0;
@@ -1995,12 +2060,16 @@ struct Coder {
   }
 
   // The rate of the macroblock in L.code in 1/256 bits, coded on the context states st (which it
-  // leaves as the writer would); the same in every lane.
-  __device__ uint32_t count_bins(uint8_t *st) {
+  // leaves as the writer would); the same in every lane. The chroma residual's bins (coded_block_flags
+  // and levels of the chroma DC and AC blocks) are counted when *chroma < 0, their cost then stored
+  // there; otherwise *chroma is taken as their cost (the same chroma coded before on the same
+  // states) and their contexts are left alone.
+  __device__ uint32_t count_bins(uint8_t *st, int *chroma) {
     const ia_mbcode &code = L.code;
     nz_masks();
     sync();
     const int cbpl = code.cbp & 15, cbpc = code.cbp >> 4, i16 = code.kind == K_I16, t8 = code.t8x8;
+    const int count_chroma = *chroma < 0;
     // the categories coded, in a fixed order, four bits each
     uint32_t cats = 0;
     int ncat = 0, total = 4;
@@ -2011,9 +2080,9 @@ struct Coder {
     if (i16) add(0);
     if (cbpl && !t8) add(i16 ? 1 : 2);
     if (cbpl && t8) add(5);
-    if (cbpc > 0) add(3);
-    if (cbpc == 2) add(4);
-    uint32_t bits = 0;
+    if (count_chroma && cbpc > 0) add(3);
+    if (count_chroma && cbpc == 2) add(4);
+    uint32_t bits = 0, cbits = 0;
     for (int j = lane(); j < total; j += 64) {
       if (j == 0) bits += header_bins(st);
       else if (j == 1) bits += mode_bins(st);
@@ -2022,15 +2091,21 @@ struct Coder {
         int t = j - 4, ci = 0;
         while (t >= cat_jobs((cats >> (4 * ci)) & 15)) t -= cat_jobs((cats >> (4 * ci++)) & 15);
         const int cat = (cats >> (4 * ci)) & 15;
-        if (cat != 5 && t == 0) bits += cbf_bins(st, cat);
-        else {
+        uint32_t b;
+        if (cat != 5 && t == 0) {
+          b = cbf_bins(st, cat);
+        } else {
           t -= cat != 5;
-          if (t < cat_sig_jobs(cat)) bits += sig_bins(st, cat, t);
-          else bits += level_bins(st, cat, t - cat_sig_jobs(cat));
+          if (t < cat_sig_jobs(cat)) b = sig_bins(st, cat, t);
+          else b = level_bins(st, cat, t - cat_sig_jobs(cat));
         }
+        bits += b;
+        if (cat == 3 || cat == 4) cbits += b;
       }
     }
-    const uint32_t total_bits = (uint32_t)wave_sum((int)bits);
+    uint32_t total_bits = (uint32_t)wave_sum((int)bits);
+    if (count_chroma) *chroma = wave_sum((int)cbits);
+    else total_bits += (uint32_t)*chroma;
     sync();
     return total_bits;
   }
@@ -2044,14 +2119,15 @@ struct Coder {
   }
 
   // J of the candidate in L.code whose reconstruction is in the canvas (enc/frame.rs:1453-1457)
-  __device__ double rd_cost(double lam2) {
+  // *chroma: the chroma residual's rate when known (see count_bins), else -1
+  __device__ double rd_cost(double lam2, int *chroma) {
     {
       const uint32_t *s = reinterpret_cast<const uint32_t *>(L.cabac);
       uint32_t *d = reinterpret_cast<uint32_t *>(L.tmp);
       for (int i = lane(); i < 256; i += 64) d[i] = s[i];
     }
     sync();
-    const uint32_t bits = count_bins(L.tmp);
+    const uint32_t bits = count_bins(L.tmp, chroma);
     const long long ssd = ssd_mb();
     return (double)ssd + lam2 * ((double)bits / 256.0);
   }
@@ -2070,6 +2146,57 @@ struct Coder {
       save_rec(L.keep_rec);
     }
     sync();
+  }
+
+  // The chroma of a candidate (mode, levels, nonzero counts, coded pattern) kept in L.i4, which the
+  // rate-distortion decision does not otherwise use: every intra candidate codes the same chroma
+  // (enc/frame.rs:1807-1852 depends on the chroma samples and edges only), so the candidates after
+  // the first take it from there; their reconstructed chroma is already in the canvas.
+  static_assert(offsetof(ia_mbcode, cac) == offsetof(ia_mbcode, cdc) + 16 && sizeof(ia_mbcode) == offsetof(ia_mbcode, cac) + 256,
+                "the chroma levels end the code, DC first");
+  __device__ void save_chroma() {
+    const uint32_t *s = reinterpret_cast<const uint32_t *>(&L.code);
+    uint32_t *d = reinterpret_cast<uint32_t *>(&L.i4);
+    constexpr int cnnz = offsetof(ia_mbcode, cnnz) / 4, cdc = offsetof(ia_mbcode, cdc) / 4;
+    // cnnz: 2 words; cdc and cac: 4 + 64 words
+    if (lane() < 2) d[cnnz + lane()] = s[cnnz + lane()];
+    for (int i = lane(); i < 68; i += 64) d[cdc + i] = s[cdc + i];
+    if (lane() == 0) {
+      L.i4.chroma_mode = L.code.chroma_mode;
+      L.i4.cbp = L.code.cbp & 0xf0;
+    }
+    sync();
+  }
+  __device__ void restore_chroma() {
+    const uint32_t *s = reinterpret_cast<const uint32_t *>(&L.i4);
+    uint32_t *d = reinterpret_cast<uint32_t *>(&L.code);
+    constexpr int cnnz = offsetof(ia_mbcode, cnnz) / 4, cdc = offsetof(ia_mbcode, cdc) / 4;
+    if (lane() < 2) d[cnnz + lane()] = s[cnnz + lane()];
+    for (int i = lane(); i < 68; i += 64) d[cdc + i] = s[cdc + i];
+    if (lane() == 0) {
+      L.code.chroma_mode = L.i4.chroma_mode;
+      L.code.cbp = (uint8_t)((L.code.cbp & 15) | L.i4.cbp);
+    }
+    sync();
+  }
+
+  // The inter candidate in L.code again with the 8x8 transform: its vectors, prediction (L.pred) and
+  // chroma stay those of the 4x4 transform's (enc/frame.rs:1201-1252 predicts and codes the chroma
+  // alike for both), so only the luma residual is coded again.
+  __device__ void recode_inter_t8() {
+    uint32_t *lv = reinterpret_cast<uint32_t *>(&L.code.luma[0][0]);
+    uint32_t *lv8 = reinterpret_cast<uint32_t *>(&L.code.luma8[0][0]);
+    for (int i = lane(); i < 128; i += 64) {
+      lv[i] = 0;
+      lv8[i] = 0;
+    }
+    if (lane() < 16) L.code.nnz[lane()] = 0;
+    if (lane() == 0) {
+      L.code.cbp &= 0xf0;
+      L.code.t8x8 = 0;
+    }
+    sync();
+    code_luma8();
   }
 
   __device__ void decide_rd(int i16, const Inter *inter, ia_mbstats *st) {
@@ -2092,25 +2219,29 @@ struct Coder {
       for (int i = lane(); i < 256; i += 64) cyv(i & 15, i >> 4) = L.pred[i];
       for (int i = lane(); i < 128; i += 64) ccv(i >> 6, (i & 63) & 7, (i & 63) >> 3) = L.cpred[i >> 6][i & 63];
       sync();
-      keep(rd_cost(lam2), &have, &best_j);
+      int none = -1;
+      keep(rd_cost(lam2, &none), &have, &best_j);
+      int inter_chroma = -1;
       code_inter(inter->part, inter->mvs, inter->refs, inter->sub, 0);
       if (!(L.code.cbp == 0 && inter->part == 0 && inter->mvs[0][0] == sm[0] && inter->mvs[0][1] == sm[1] && inter->refs[0] == 0))
-        keep(rd_cost(lam2), &have, &best_j);
+        keep(rd_cost(lam2, &inter_chroma), &have, &best_j);
       if (P.transform8 && (inter->sub[0] | inter->sub[1] | inter->sub[2] | inter->sub[3]) == 0) {
-        code_inter(inter->part, inter->mvs, inter->refs, inter->sub, 1);
-        if (L.code.t8x8) keep(rd_cost(lam2), &have, &best_j);
+        recode_inter_t8();
+        if (L.code.t8x8) keep(rd_cost(lam2, &inter_chroma), &have, &best_j);
       }
     }
+    int intra_chroma = -1;
     code_i16(i16);
     code_intra_chroma();
-    keep(rd_cost(lam2), &have, &best_j);
+    save_chroma();
+    keep(rd_cost(lam2, &intra_chroma), &have, &best_j);
     code_i4();
-    code_intra_chroma();
-    keep(rd_cost(lam2), &have, &best_j);
+    restore_chroma();
+    keep(rd_cost(lam2, &intra_chroma), &have, &best_j);
     if (P.transform8) {
       code_i8();
-      code_intra_chroma();
-      keep(rd_cost(lam2), &have, &best_j);
+      restore_chroma();
+      keep(rd_cost(lam2, &intra_chroma), &have, &best_j);
     }
     restore_rec(L.keep_rec);
     copy_code(&L.code, &L.keep);
@@ -2331,7 +2462,8 @@ struct Coder {
     uint32_t *d = reinterpret_cast<uint32_t *>(&P.codes[idx]);
     for (int i = lane(); i < (int)(sizeof(ia_mbcode) / 4); i += 64) d[i] = s[i];
     // the chosen macroblock's bins adapt the live contexts as the writer would; their count is its rate
-    const uint32_t bits = P.use_cabac ? count_bins(L.cabac) : 0;
+    int chroma = -1;
+    const uint32_t bits = P.use_cabac ? count_bins(L.cabac, &chroma) : 0;
     if (lane() == 0) {
       MbState ms;
       ia_mbinfo info;
