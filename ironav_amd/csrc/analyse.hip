@@ -303,17 +303,49 @@ struct Coder {
   // ---- samples ----
 
   // SAD of the w x h source block at (bx, by) of the macroblock against reference r's integer plane
-  // at the integer part of (mx, my) (enc/frame.rs:662-671)
+  // at the integer part of (mx, my) (enc/frame.rs:662-671): a lane per four samples of a row, read as
+  // two aligned dwords and byte-aligned (the plane's border covers the dword past a block's row)
   __device__ int sad_int(int r, int bx, int by, int w, int h, int mx, int my) const {
+    int sad;
+    const int pos[1][2] = {{mx, my}};
+    sad_batch(r, bx, by, w, h, pos, 1, &sad);
+    return sad;
+  }
+
+  // The SADs of n <= 8 integer vectors (quarter-sample units, multiples of 4) of one w x h block,
+  // several at a time: a group of w * h / 4 lanes per vector, a lane per four samples of a row
+  // (v_sad_u8 on the dword of the source and the byte-aligned dword of the reference), the group's
+  // sums reduced within it.
+  __device__ void sad_batch(int r, int bx, int by, int w, int h, const int (*pos)[2], int n, int *sads) const {
     const int sw = P.mb_w * 16 + 2 * PAD;
-    const uint8_t *g = P.hp[r] + (size_t)(mby * 16 + by + (my >> 2) + PAD) * sw + (mbx * 16 + bx + (mx >> 2) + PAD);
-    int s = 0;
-    const int lw = __builtin_ctz(w);
-    for (int i = lane(); i < w * h; i += 64) {
-      int rr = i >> lw, c = i & (w - 1);
-      s += iabs((int)L.src_y[(by + rr) * 16 + bx + c] - (int)g[(size_t)rr * sw + c]);
+    const int lwd = __builtin_ctz(w) - 2, nd = (w * h) >> 2, lnd = __builtin_ctz(nd), per = 64 >> lnd;
+    const uint8_t *g = P.hp[r] + (size_t)(mby * 16 + by + PAD) * sw + (mbx * 16 + bx + PAD);
+    const int i = lane() & (nd - 1), rr = i >> lwd, q = i & ((1 << lwd) - 1);
+    const uint32_t src = *reinterpret_cast<const uint32_t *>(&L.src_y[(by + rr) * 16 + bx + 4 * q]);
+    for (int c0 = 0; c0 < n; c0 += per) {
+      const int c = c0 + (lane() >> lnd);
+      uint32_t v = 0;
+      if (c < n) {
+        int px = 0, py = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+          if (k == c) { px = pos[k][0] >> 2; py = pos[k][1] >> 2; }
+        const uint8_t *a = g + (ptrdiff_t)(rr + py) * sw + 4 * q + px;
+        const uint32_t *d = reinterpret_cast<const uint32_t *>((uintptr_t)a & ~(uintptr_t)3);
+        const uint32_t sh = (uint32_t)((uintptr_t)a & 3);
+        v = __builtin_amdgcn_sad_u8(src, __builtin_amdgcn_alignbyte(d[1], d[0], sh), 0u);
+      }
+      int t = (int)v;
+      if (nd == 64) {
+        t = wave_sum(t);
+        sads[c0] = t;
+      } else {
+        t = group_sum(t, nd < 16 ? nd : 16);
+        for (int k = 0; k < per && c0 + k < n; k++)
+          sads[c0 + k] = nd == 32 ? __builtin_amdgcn_readlane(t, 32 * k) + __builtin_amdgcn_readlane(t, 32 * k + 16)
+                                  : __builtin_amdgcn_readlane(t, k << lnd);
+      }
     }
-    return wave_sum(s);
   }
 
   // quarter-sample prediction of the w x h block at picture (x, y) from reference r into out
@@ -476,9 +508,20 @@ struct Coder {
     }
     const int bxp = bx * 4, byp = by * 4;
     int bestx = 0, besty = 0, best_cost = 0x7fffffff;
-    for (int k = 0; k < nc; k++) {
-      int cost = sad_int(r, bxp, byp, w, h, cands[k][0], cands[k][1]) + lam * mvd_bits(cands[k][0], cands[k][1], mvp[0], mvp[1]);
-      if (cost < best_cost) { bestx = cands[k][0]; besty = cands[k][1]; best_cost = cost; }
+    // the candidates' costs eight at a time, taken in order (the first least cost)
+    for (int k0 = 0; k0 < nc; k0 += 8) {
+      int pos[8][2], sads[8];
+      const int n = min(nc - k0, 8);
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        pos[k][0] = k < n ? cands[k0 + k][0] : 0;
+        pos[k][1] = k < n ? cands[k0 + k][1] : 0;
+      }
+      sad_batch(r, bxp, byp, w, h, pos, n, sads);
+      for (int k = 0; k < n; k++) {
+        const int cost = sads[k] + lam * mvd_bits(pos[k][0], pos[k][1], mvp[0], mvp[1]);
+        if (cost < best_cost) { bestx = pos[k][0]; besty = pos[k][1]; best_cost = cost; }
+      }
     }
     // the realtime decisions skip the 16-sample step (enc/frame.rs:973)
     for (int si = P.fast ? 1 : 0; si < 3; si++) {
@@ -1445,53 +1488,70 @@ struct Coder {
       else if (bx > 0) have_tl = ta;
       else have_tl = tla;
       const int have_tr = i8 == 0 ? ta : i8 == 1 ? tra : i8 == 2 ? 1 : 0;
-      int pt[16] = {}, pl[8] = {}, ptl = 0;
-      if (have_top) {
-        for (int i = 0; i < 8; i++) pt[i] = cyv(x + i, y - 1);
-        for (int i = 8; i < 16; i++) pt[i] = have_tr ? cyv(x + i, y - 1) : pt[7];
-      }
-      if (have_left)
-        for (int j = 0; j < 8; j++) pl[j] = cyv(x - 1, y + j);
-      if (have_tl) ptl = cyv(x - 1, y - 1);
-      // reference-sample smoothing (h264/intra.rs:130-159)
-      int top[16], left[8], tl = ptl;
-      for (int i = 0; i < 16; i++) top[i] = pt[i];
-      for (int j = 0; j < 8; j++) left[j] = pl[j];
-      if (have_top) {
-        top[0] = have_tl ? (ptl + 2 * pt[0] + pt[1] + 2) >> 2 : (3 * pt[0] + pt[1] + 2) >> 2;
-        for (int i = 1; i < 15; i++) top[i] = (pt[i - 1] + 2 * pt[i] + pt[i + 1] + 2) >> 2;
-        top[15] = (pt[14] + 3 * pt[15] + 2) >> 2;
-      }
-      if (have_tl) {
-        if (have_top && have_left) tl = (pt[0] + 2 * ptl + pl[0] + 2) >> 2;
-        else if (have_top) tl = (3 * ptl + pt[0] + 2) >> 2;
-        else if (have_left) tl = (3 * ptl + pl[0] + 2) >> 2;
-      }
-      if (have_left) {
-        left[0] = have_tl ? (ptl + 2 * pl[0] + pl[1] + 2) >> 2 : (3 * pl[0] + pl[1] + 2) >> 2;
-        for (int j = 1; j < 7; j++) left[j] = (pl[j - 1] + 2 * pl[j] + pl[j + 1] + 2) >> 2;
-        left[7] = (pl[6] + 3 * pl[7] + 2) >> 2;
-      }
-      const int predicted = pred_i4_mode(L.code.modes, bx * 2, by * 2);
-      // lane 4m + q < 36: the SATD of quadrant q of mode m's prediction; the quad's sum is the mode's
-      const int m = lane() >> 2, qd = lane() & 3;
-      const bool ok = lane() < 36 && mode_ok(m, have_top, have_left, have_tl);
-      int s = 0;
-      if (ok) {
-        const int ox = (qd & 1) * 4, oy = (qd >> 1) * 4;
-        int d[16];
-        for (int i = 0; i < 16; i++) {
-          const int xx = ox + (i & 3), yy = oy + (i >> 2);
-          d[i] = (int)L.src_y[(y + yy) * 16 + x + xx] - nxn_sample(8, m, xx, yy, top, left, tl, have_top, have_left);
+      // the edges, smoothed (h264/intra.rs:130-159), into LDS: lane i < 16 the row above (8-15 above
+      // right, or sample 7 repeated), 16-23 the column to the left, 24 the corner; zero where missing
+      {
+        const int l = lane();
+        auto pt = [&](int i) { return have_top ? (int)cyv(x + (i < 8 || have_tr ? i : 7), y - 1) : 0; };
+        auto pl = [&](int j) { return have_left ? (int)cyv(x - 1, y + j) : 0; };
+        const int ptl = have_tl ? (int)cyv(x - 1, y - 1) : 0;
+        if (l < 16) {
+          int v = 0;
+          if (have_top) {
+            if (l == 0) v = have_tl ? (ptl + 2 * pt(0) + pt(1) + 2) >> 2 : (3 * pt(0) + pt(1) + 2) >> 2;
+            else if (l == 15) v = (pt(14) + 3 * pt(15) + 2) >> 2;
+            else v = (pt(l - 1) + 2 * pt(l) + pt(l + 1) + 2) >> 2;
+          }
+          L.etop[l] = v;
+        } else if (l < 24) {
+          const int j = l - 16;
+          int v = 0;
+          if (have_left) {
+            if (j == 0) v = have_tl ? (ptl + 2 * pl(0) + pl(1) + 2) >> 2 : (3 * pl(0) + pl(1) + 2) >> 2;
+            else if (j == 7) v = (pl(6) + 3 * pl(7) + 2) >> 2;
+            else v = (pl(j - 1) + 2 * pl(j) + pl(j + 1) + 2) >> 2;
+          }
+          L.eleft[j] = v;
+        } else if (l == 24) {
+          int v = ptl;
+          if (have_tl) {
+            if (have_top && have_left) v = (pt(0) + 2 * ptl + pl(0) + 2) >> 2;
+            else if (have_top) v = (3 * ptl + pt(0) + 2) >> 2;
+            else if (have_left) v = (3 * ptl + pl(0) + 2) >> 2;
+          }
+          L.etl = v;
         }
-        s = satd4(d);
       }
-      s = quad_sum(s);
-      const int c = ok ? s + P.lambda * (m == predicted ? 1 : 4) : 0x7fffffff;
+      sync();
+      const int *top = L.etop, *left = L.eleft, tl = L.etl;
+      const int predicted = pred_i4_mode(L.code.modes, bx * 2, by * 2);
+      // the nine modes' SATDs, a mode a pass (no divergence): lane 16 q + i takes sample i of the
+      // 4x4 quadrant q; the horizontal Hadamard within the lane's quad, the vertical one through LDS,
+      // each quadrant's SATD summed over its 16 lanes, the mode's over the quadrants
       int best_m = 2, best_c = 0x7fffffff;
-      for (int k = 0; k < 9; k++) {
-        int ck = __builtin_amdgcn_readlane(c, 4 * k);
-        if (ck < best_c) { best_c = ck; best_m = k; }
+      {
+        const int l = lane(), qd = l >> 4, i = l & 15, xx = (qd & 1) * 4 + (i & 3), yy = (qd >> 1) * 4 + (i >> 2);
+        const int sv = L.src_y[(y + yy) * 16 + x + xx];
+        for (int m = 0; m < 9; m++) {
+          if (!mode_ok(m, have_top, have_left, have_tl)) continue;
+          int v = sv - nxn_sample(8, m, xx, yy, top, left, tl, have_top, have_left);
+          int p = __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, false);  // quad_perm [1, 0, 3, 2]
+          v = (i & 1) ? p - v : v + p;
+          p = __builtin_amdgcn_update_dpp(0, v, 0x4e, 0xf, 0xf, false);  // quad_perm [2, 3, 0, 1]
+          v = (i & 2) ? p - v : v + p;
+          L.ival[l] = v;
+          sync();
+          const int *col = &L.ival[(l & ~15) | (i & 3)];
+          const int a0 = col[0], a1 = col[4], a2 = col[8], a3 = col[12];
+          const int s0 = a0 + a1, s1 = a2 + a3, d0 = a0 - a1, d1 = a2 - a3;
+          const int r = i >> 2, o = r == 0 ? s0 + s1 : r == 1 ? s0 - s1 : r == 2 ? d0 - d1 : d0 + d1;
+          const int q = group_sum(iabs(o), 16) >> 1;
+          const int sat = __builtin_amdgcn_readlane(q, 0) + __builtin_amdgcn_readlane(q, 16) + __builtin_amdgcn_readlane(q, 32) +
+                          __builtin_amdgcn_readlane(q, 48);
+          const int c = sat + P.lambda * (m == predicted ? 1 : 4);
+          if (c < best_c) { best_c = c; best_m = m; }
+          sync();
+        }
       }
       // the prediction into the canvas and the residual, a lane per sample
       {
@@ -1519,30 +1579,34 @@ struct Coder {
   // enc/frame.rs:1807-1852
   __device__ void code_intra_chroma() {
     const int la = mbx > 0, ta = up();
-    // lane (mode, c) < 8: the chroma prediction's SATD of component c under mode m
-    int cst = 0;
-    const int m = lane() >> 1, ci = lane() & 1;
-    int okm = 0;
-    if (lane() < 8) {
-      okm = m == 0 ? 1 : m == 1 ? la : m == 2 ? ta : (la && ta);
-      if (okm) {
-        for (int q = 0; q < 4; q++) {
-          const int ox = (q & 1) * 4, oy = (q >> 1) * 4;
-          int d[16];
-          for (int i = 0; i < 16; i++) {
-            const int xx = ox + (i & 3), yy = oy + (i >> 2);
-            d[i] = (int)L.src_c[ci][yy * 8 + xx] - chroma_sample(ci, m, xx, yy, ta, la);
-          }
-          cst += satd4(d);
-        }
-      }
-    }
+    // each mode's SATD over both components, a (mode, component) a pass over the 64 samples (no
+    // divergence): lane 16 q + i takes sample i of 4x4 block q; Hadamard as for the luma blocks
     int best_m = 0, best_c = 0x7fffffff;
-    for (int mm = 0; mm < 4; mm++) {
-      const int ok = mm == 0 ? 1 : mm == 1 ? la : mm == 2 ? ta : (la && ta);
-      if (!ok) continue;
-      const int cost = P.lambda * (mm == 0 ? 1 : 3) + __builtin_amdgcn_readlane(cst, 2 * mm) + __builtin_amdgcn_readlane(cst, 2 * mm + 1);
-      if (cost < best_c) { best_c = cost; best_m = mm; }
+    {
+      const int l = lane(), qd = l >> 4, i = l & 15, xx = (qd & 1) * 4 + (i & 3), yy = (qd >> 1) * 4 + (i >> 2);
+      for (int mm = 0; mm < 4; mm++) {
+        const int ok = mm == 0 ? 1 : mm == 1 ? la : mm == 2 ? ta : (la && ta);
+        if (!ok) continue;
+        int cost = P.lambda * (mm == 0 ? 1 : 3);
+        for (int ci = 0; ci < 2; ci++) {
+          int v = (int)L.src_c[ci][yy * 8 + xx] - chroma_sample(ci, mm, xx, yy, ta, la);
+          int p = __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, false);  // quad_perm [1, 0, 3, 2]
+          v = (i & 1) ? p - v : v + p;
+          p = __builtin_amdgcn_update_dpp(0, v, 0x4e, 0xf, 0xf, false);  // quad_perm [2, 3, 0, 1]
+          v = (i & 2) ? p - v : v + p;
+          L.ival[l] = v;
+          sync();
+          const int *col = &L.ival[(l & ~15) | (i & 3)];
+          const int a0 = col[0], a1 = col[4], a2 = col[8], a3 = col[12];
+          const int s0 = a0 + a1, s1 = a2 + a3, d0 = a0 - a1, d1 = a2 - a3;
+          const int r = i >> 2, o = r == 0 ? s0 + s1 : r == 1 ? s0 - s1 : r == 2 ? d0 - d1 : d0 + d1;
+          const int q = group_sum(iabs(o), 16) >> 1;
+          cost += __builtin_amdgcn_readlane(q, 0) + __builtin_amdgcn_readlane(q, 16) + __builtin_amdgcn_readlane(q, 32) +
+                  __builtin_amdgcn_readlane(q, 48);
+          sync();
+        }
+        if (cost < best_c) { best_c = cost; best_m = mm; }
+      }
     }
     for (int i = lane(); i < 128; i += 64) {
       const int c2 = i >> 6, o = i & 63;

@@ -248,6 +248,23 @@ inline void run_workgroup(unsigned block, void (*fn)(void *), void *arg) {
 
 inline int __clz(int x) { return x ? __builtin_clz((unsigned)x) : 32; }
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+
+namespace hostwave {
+// v_alignbyte_b32: the 4 bytes at byte offset s & 3 of the 8-byte value hi:lo
+inline uint32_t alignbyte(uint32_t hi, uint32_t lo, uint32_t s) {
+  return (uint32_t)(((((uint64_t)hi) << 32) | lo) >> (8 * (s & 3)));
+}
+// v_sad_u8: acc plus the sum of the absolute differences of the four bytes of a and b
+inline uint32_t sad_u8(uint32_t a, uint32_t b, uint32_t acc) {
+  for (int k = 0; k < 4; k++) {
+    const int x = (int)((a >> (8 * k)) & 0xff), y = (int)((b >> (8 * k)) & 0xff);
+    acc += (uint32_t)(x > y ? x - y : y - x);
+  }
+  return acc;
+}
+}  // namespace hostwave
+#define __builtin_amdgcn_alignbyte(hi, lo, s) hostwave::alignbyte((hi), (lo), (s))
+#define __builtin_amdgcn_sad_u8(a, b, acc) hostwave::sad_u8((a), (b), (acc))
 #define __ballot(p) hostwave::ballot((p))
 inline int atomicAdd(int *p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 
