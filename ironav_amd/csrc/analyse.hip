@@ -121,6 +121,19 @@ __device__ __forceinline__ int16_t median3(int16_t a, int16_t b, int16_t c) {
 
 // ---- per-slice working set in LDS --------------------------------------------------------------
 
+// The result of one reference's search, and the motion search's state per group of 16 lanes: the
+// sub-partition search's vectors and references (enc/frame.rs:866) and each group's result.
+struct SearchRes {
+  int32_t cost, part;
+  int16_t mvs[16][2];
+  uint8_t refs[4], sub[4];
+};
+struct SearchLds {
+  int16_t st_mvs[4][16][2];
+  int8_t st_refs[4][16];
+  SearchRes res[4];
+};
+
 struct Lds {
   uint8_t src_y[256];
   uint8_t src_c[2][64];
@@ -129,7 +142,10 @@ struct Lds {
   uint8_t cy[17][28];
   uint8_t cc[2][9][12];
   uint8_t cabac[1024];  // live context states of the slice
-  uint8_t tmp[1024];    // a copy counted on for a rate estimate
+  union {
+    uint8_t tmp[1024];  // a copy counted on for a rate estimate
+    SearchLds ms;       // the motion search's per-group state (not live at the same time)
+  };
   uint32_t cab[256];    // per (state, LPS?): bit cost in 1/256 bits << 8 | next state (enc/cabac.rs:32-40,162-173)
   MbState nb[4];        // left, top, top-right, top-left neighbours (valid when nb_ok)
   int nb_ok[4];
@@ -144,8 +160,6 @@ struct Lds {
   int32_t dcs[16];
   int32_t score[16];
   int32_t ival[64];
-  int16_t st_mvs[16][2];  // sub-partition search state (enc/frame.rs:866)
-  int8_t st_refs[16];
   int16_t mv8[16][2];
   uint64_t nzm[2][16];  // masks of the nonzero levels and of the levels above 1 (count_bins)
   uint64_t pos8[24];    // 8x8 scan positions of each significance (0-14) and last (15-23) context
@@ -302,50 +316,35 @@ struct Coder {
 
   // ---- samples ----
 
-  // SAD of the w x h source block at (bx, by) of the macroblock against reference r's integer plane
-  // at the integer part of (mx, my) (enc/frame.rs:662-671): a lane per four samples of a row, read as
-  // two aligned dwords and byte-aligned (the plane's border covers the dword past a block's row)
-  __device__ int sad_int(int r, int bx, int by, int w, int h, int mx, int my) const {
-    int sad;
-    const int pos[1][2] = {{mx, my}};
-    sad_batch(r, bx, by, w, h, pos, 1, &sad);
-    return sad;
-  }
+  // ---- motion search: a reference per group of 16 lanes ----
+  //
+  // best_inter searches its references independently (enc/frame.rs:840-848), so up to four searches
+  // run at once, reference c0 + g on lanes 16 g .. 16 g + 15 (the group g). Every value the search
+  // code holds is the same within a group; its reductions stay within the group's DPP row and its LDS
+  // state is per group.
 
-  // The SADs of n <= 8 integer vectors (quarter-sample units, multiples of 4) of one w x h block,
-  // several at a time: a group of w * h / 4 lanes per vector, a lane per four samples of a row
-  // (v_sad_u8 on the dword of the source and the byte-aligned dword of the reference), the group's
-  // sums reduced within it.
-  __device__ void sad_batch(int r, int bx, int by, int w, int h, const int (*pos)[2], int n, int *sads) const {
+  __device__ static int glane() { return lane() & 15; }
+  __device__ static int group() { return lane() >> 4; }
+  // the value v of lane src (ds_bpermute; each lane its own source)
+  __device__ static int bperm(int v, int src) { return __builtin_amdgcn_ds_bpermute(src << 2, v); }
+
+  // SAD of the w x h source block at (bx, by) of the macroblock against reference r's integer plane
+  // at the integer part of (mx, my) (enc/frame.rs:662-671), over the group: its lanes take the block's
+  // dwords (four samples of a row) in turn, v_sad_u8 of the source dword and the reference read as
+  // two aligned dwords byte-aligned (the plane's border covers the dword past a block's row).
+  __device__ int sad_g(int r, int bx, int by, int w, int h, int mx, int my) const {
     const int sw = P.mb_w * 16 + 2 * PAD;
-    const int lwd = __builtin_ctz(w) - 2, nd = (w * h) >> 2, lnd = __builtin_ctz(nd), per = 64 >> lnd;
-    const uint8_t *g = P.hp[r] + (size_t)(mby * 16 + by + PAD) * sw + (mbx * 16 + bx + PAD);
-    const int i = lane() & (nd - 1), rr = i >> lwd, q = i & ((1 << lwd) - 1);
-    const uint32_t src = *reinterpret_cast<const uint32_t *>(&L.src_y[(by + rr) * 16 + bx + 4 * q]);
-    for (int c0 = 0; c0 < n; c0 += per) {
-      const int c = c0 + (lane() >> lnd);
-      uint32_t v = 0;
-      if (c < n) {
-        int px = 0, py = 0;
-#pragma unroll
-        for (int k = 0; k < 8; k++)
-          if (k == c) { px = pos[k][0] >> 2; py = pos[k][1] >> 2; }
-        const uint8_t *a = g + (ptrdiff_t)(rr + py) * sw + 4 * q + px;
-        const uint32_t *d = reinterpret_cast<const uint32_t *>((uintptr_t)a & ~(uintptr_t)3);
-        const uint32_t sh = (uint32_t)((uintptr_t)a & 3);
-        v = __builtin_amdgcn_sad_u8(src, __builtin_amdgcn_alignbyte(d[1], d[0], sh), 0u);
-      }
-      int t = (int)v;
-      if (nd == 64) {
-        t = wave_sum(t);
-        sads[c0] = t;
-      } else {
-        t = group_sum(t, nd < 16 ? nd : 16);
-        for (int k = 0; k < per && c0 + k < n; k++)
-          sads[c0 + k] = nd == 32 ? __builtin_amdgcn_readlane(t, 32 * k) + __builtin_amdgcn_readlane(t, 32 * k + 16)
-                                  : __builtin_amdgcn_readlane(t, k << lnd);
-      }
+    const int lwd = __builtin_ctz(w) - 2, nd = (w * h) >> 2;
+    const uint8_t *g = P.hp[r] + (size_t)(mby * 16 + by + (my >> 2) + PAD) * sw + (mbx * 16 + bx + (mx >> 2) + PAD);
+    uint32_t v = 0;
+    for (int i = glane(); i < nd; i += 16) {
+      const int rr = i >> lwd, q = i & ((1 << lwd) - 1);
+      const uint32_t src = *reinterpret_cast<const uint32_t *>(&L.src_y[(by + rr) * 16 + bx + 4 * q]);
+      const uint8_t *a = g + (size_t)rr * sw + 4 * q;
+      const uint32_t *d = reinterpret_cast<const uint32_t *>((uintptr_t)a & ~(uintptr_t)3);
+      v = __builtin_amdgcn_sad_u8(src, __builtin_amdgcn_alignbyte(d[1], d[0], (uint32_t)((uintptr_t)a & 3)), v);
     }
+    return group_sum((int)v, 16);
   }
 
   // quarter-sample prediction of the w x h block at picture (x, y) from reference r into out
@@ -434,22 +433,25 @@ struct Coder {
     return 25 + 4 * b8 + (y & 1) * 2 + (x & 1);
   }
 
-  // The costs SATD + lambda * mvd_bits that the sub-sample refinement gives n candidate vectors of the
-  // w x h partition at (bxp, byp) of the macroblock (picture position (x0, y0)) against reference r,
-  // computed together: a lane takes one 4x4 block of one candidate's quarter-sample prediction
-  // (enc/subpel.rs:122-145, as luma_pred forms it), and each candidate's blocks are summed within its
-  // group of lanes.
-  __device__ void cost_sub_batch(int r, int x0, int y0, int bxp, int byp, int w, int h, const int (*cand)[2], int n,
-                                 const int16_t *mvp, int *costs) const {
+  // The costs SATD + lambda * mvd_bits that the sub-sample refinement gives n <= 9 candidate vectors
+  // of the w x h partition at (bxp, byp) of the macroblock (picture position (x0, y0)) against
+  // reference r, over the group: a lane takes one 4x4 block of one candidate's quarter-sample
+  // prediction (enc/subpel.rs:122-145, as luma_pred forms it), 16 / blocks candidates at a time, and
+  // each candidate's blocks are summed within its lanes.
+  __device__ void cost_sub_g(int r, int x0, int y0, int bxp, int byp, int w, int h, const int (*cand)[2], int n,
+                             const int16_t *mvp, int *costs) const {
     const int pw = P.mb_w * 16, ph = P.mb_h * 16, sw = pw + 2 * PAD, sh = ph + 2 * PAD;
     const size_t plane = (size_t)sw * sh;
-    const int nbx = w >> 2, nb = nbx * (h >> 2), per = 64 / nb, lnb = __builtin_ctz(nb);
-    const int lam = P.lambda;
+    const int nbx = w >> 2, nb = nbx * (h >> 2), lnb = __builtin_ctz(nb), per = 16 >> lnb;
+    const int lam = P.lambda, gb = lane() & ~15;
     for (int base = 0; base < n; base += per) {
-      const int g = base + (lane() >> lnb), blk = lane() & (nb - 1);
+      const int gi = base + (glane() >> lnb), blk = glane() & (nb - 1);
       int sv = 0;
-      if (g < n) {
-        const int vx = cand[g][0], vy = cand[g][1];
+      if (gi < n) {
+        int vx = 0, vy = 0;
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+          if (k == gi) { vx = cand[k][0]; vy = cand[k][1]; }
         const int fx = vx & 3, fy = vy & 3;
         const int ix = clampi(x0 + (vx >> 2), -w - 3, pw + 1), iy = clampi(y0 + (vy >> 2), -h - 3, ph + 1);
         const uint8_t *m = QMIX[fy][fx];
@@ -468,11 +470,19 @@ struct Coder {
       }
       sv = group_sum(sv, nb);
       for (int k = 0; k < per && base + k < n; k++)
-        costs[base + k] = __builtin_amdgcn_readlane(sv, k * nb) + lam * mvd_bits(cand[base + k][0], cand[base + k][1], mvp[0], mvp[1]);
+        costs[base + k] = bperm(sv, gb + (k << lnb)) + lam * mvd_bits(cand[base + k][0], cand[base + k][1], mvp[0], mvp[1]);
     }
   }
 
   // ---- motion search (enc/frame.rs:949-1019) ----
+  //
+  // The four groups run the same code: their control flow is kept uniform (each loop runs while any
+  // group still needs it, and a group that is done only stops taking the results), so every DPP
+  // reduction is reached by all lanes. The geometry of a search is the same in all groups; the
+  // reference, predictors and candidates are the group's.
+
+  // any lane's predicate holds (the same in every lane)
+  __device__ static bool any_lane(bool p) { return __ballot(p) != 0; }
 
   __device__ int motion_search(int r, int bx, int by, int bw, int bh, const int16_t *mvp, const int16_t (*extra)[2],
                                int nextra, int16_t *out) {
@@ -508,48 +518,36 @@ struct Coder {
     }
     const int bxp = bx * 4, byp = by * 4;
     int bestx = 0, besty = 0, best_cost = 0x7fffffff;
-    // the candidates' costs eight at a time, taken in order (the first least cost)
-    for (int k0 = 0; k0 < nc; k0 += 8) {
-      int pos[8][2], sads[8];
-      const int n = min(nc - k0, 8);
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        pos[k][0] = k < n ? cands[k0 + k][0] : 0;
-        pos[k][1] = k < n ? cands[k0 + k][1] : 0;
-      }
-      sad_batch(r, bxp, byp, w, h, pos, n, sads);
-      for (int k = 0; k < n; k++) {
-        const int cost = sads[k] + lam * mvd_bits(pos[k][0], pos[k][1], mvp[0], mvp[1]);
-        if (cost < best_cost) { bestx = pos[k][0]; besty = pos[k][1]; best_cost = cost; }
-      }
+    for (int k = 0; any_lane(k < nc); k++) {
+      const bool on = k < nc;
+      const int cx = on ? cands[k][0] : 0, cy = on ? cands[k][1] : 0;
+      const int cost = sad_g(r, bxp, byp, w, h, cx, cy) + lam * mvd_bits(cx, cy, mvp[0], mvp[1]);
+      if (on && cost < best_cost) { bestx = cx; besty = cy; best_cost = cost; }
     }
     // the realtime decisions skip the 16-sample step (enc/frame.rs:973)
     for (int si = P.fast ? 1 : 0; si < 3; si++) {
       const int step = 16 >> si, iters = step == 4 ? 32 : 4;
-      for (int it = 0; it < iters; it++) {
+      bool going = true;  // this group's search at this step has not yet stopped
+      for (int it = 0; it < iters && any_lane(going); it++) {
         int moved = 0;
         for (int d = 0; d < 4; d++) {
           int dx = d == 0 ? -step : d == 1 ? step : 0, dy = d == 2 ? -step : d == 3 ? step : 0;
           int cx = clampi(bestx + dx, lo_x, hi_x), cy = clampi(besty + dy, lo_y, hi_y);
-          int cost = sad_int(r, bxp, byp, w, h, cx, cy) + lam * mvd_bits(cx, cy, mvp[0], mvp[1]);
-          if (cost < best_cost) { bestx = cx; besty = cy; best_cost = cost; moved = 1; }
+          int cost = sad_g(r, bxp, byp, w, h, cx, cy) + lam * mvd_bits(cx, cy, mvp[0], mvp[1]);
+          if (going && cost < best_cost) { bestx = cx; besty = cy; best_cost = cost; moved = 1; }
         }
-        if (!moved) break;
+        if (!moved) going = false;
       }
     }
     // realtime: stop at an integer vector that already matches (enc/frame.rs:990-992)
-    if (P.fast && best_cost < w * h / 8 + lam) {
-      out[0] = (int16_t)bestx;
-      out[1] = (int16_t)besty;
-      return best_cost - lam * mvd_bits(bestx, besty, mvp[0], mvp[1]);
-    }
+    const bool stop = P.fast && best_cost < w * h / 8 + lam;
     // sub-sample refinement by SATD: 8 half samples around, then 4 quarter samples (realtime coding
     // keeps to the half samples, enc/frame.rs:1006). Each pass probes points around a centre fixed for
     // the pass, so its candidates' costs are computed together and then taken in the reference's order
     // with its strict minimum; the first pass carries the centre's own cost.
     const int SQ[8][2] = {{-1, -1}, {0, -1}, {1, -1}, {-1, 0}, {1, 0}, {-1, 1}, {0, 1}, {1, 1}};
     const int DM[4][2] = {{0, -1}, {-1, 0}, {1, 0}, {0, 1}};
-    for (int pass = 0; pass < (P.fast ? 1 : 2); pass++) {
+    for (int pass = 0; pass < (P.fast ? 1 : 2) && any_lane(!stop); pass++) {
       const int step = pass == 0 ? 2 : 1, np = pass == 0 ? 8 : 4;
       const int cx0 = bestx, cy0 = besty;
       int cand[9][2], cost[9];
@@ -565,30 +563,32 @@ struct Coder {
         cand[n][1] = clampi(cy0 + ddy * step, lo_y, hi_y);
         n++;
       }
-      cost_sub_batch(r, x0, y0, bxp, byp, w, h, cand, n, mvp, cost);
-      int k0 = 0;
-      if (pass == 0) {
-        best_cost = cost[0];
-        k0 = 1;
+      cost_sub_g(r, x0, y0, bxp, byp, w, h, cand, n, mvp, cost);
+      if (!stop) {
+        int k0 = 0;
+        if (pass == 0) {
+          best_cost = cost[0];
+          k0 = 1;
+        }
+        for (int k = k0; k < n; k++)
+          if (cost[k] < best_cost) { bestx = cand[k][0]; besty = cand[k][1]; best_cost = cost[k]; }
       }
-      for (int k = k0; k < n; k++)
-        if (cost[k] < best_cost) { bestx = cand[k][0]; besty = cand[k][1]; best_cost = cost[k]; }
     }
     out[0] = (int16_t)bestx;
     out[1] = (int16_t)besty;
     return best_cost - lam * mvd_bits(bestx, besty, mvp[0], mvp[1]);
   }
 
-  // enc/frame.rs:911-940; the search state lives in L.st_*
+  // enc/frame.rs:911-940; the search state lives in the group's L.ms.st_*
   __device__ int sub_search(int r, int k, const int16_t *extra, uint32_t *done, int *cost_out) {
-    const int lam = P.lambda, ox = (k & 1) * 2, oy = (k >> 1) * 2;
+    const int lam = P.lambda, ox = (k & 1) * 2, oy = (k >> 1) * 2, g = group();
     int best_s = -1, best_cost = 0;
     int16_t bmv[4][2] = {};
     uint32_t bdone = 0;
     for (int s = 0; s < (P.sub8 ? 4 : 1); s++) {
       int16_t tm[16][2];
       int8_t tr[16];
-      for (int i = 0; i < 16; i++) { tm[i][0] = L.st_mvs[i][0]; tm[i][1] = L.st_mvs[i][1]; tr[i] = L.st_refs[i]; }
+      for (int i = 0; i < 16; i++) { tm[i][0] = L.ms.st_mvs[g][i][0]; tm[i][1] = L.ms.st_mvs[g][i][1]; tr[i] = L.ms.st_refs[g][i]; }
       uint32_t td = *done;
       int cost = lam * (s == 0 ? 0 : 2);
       int16_t sm[4][2];
@@ -615,18 +615,17 @@ struct Coder {
         for (int q = 0; q < NSUBS[s]; q++) { bmv[q][0] = sm[q][0]; bmv[q][1] = sm[q][1]; }
       }
     }
-    // paint the kept sub-partitioning's vectors into the state
-    for (int q = 0; q < NSUBS[best_s]; q++) {
-      const int x = ox + SUBS[best_s][q][0], y = oy + SUBS[best_s][q][1], w = SUBS[best_s][q][2], h = SUBS[best_s][q][3];
-      for (int yy = y; yy < y + h; yy++)
-        for (int xx = x; xx < x + w; xx++) {
-          if (lane() == 0) {
-            L.st_mvs[yy * 4 + xx][0] = bmv[q][0];
-            L.st_mvs[yy * 4 + xx][1] = bmv[q][1];
-            L.st_refs[yy * 4 + xx] = (int8_t)r;
+    // paint the kept sub-partitioning's vectors into the group's state
+    if (glane() == 0)
+      for (int q = 0; q < NSUBS[best_s]; q++) {
+        const int x = ox + SUBS[best_s][q][0], y = oy + SUBS[best_s][q][1], w = SUBS[best_s][q][2], h = SUBS[best_s][q][3];
+        for (int yy = y; yy < y + h; yy++)
+          for (int xx = x; xx < x + w; xx++) {
+            L.ms.st_mvs[g][yy * 4 + xx][0] = bmv[q][0];
+            L.ms.st_mvs[g][yy * 4 + xx][1] = bmv[q][1];
+            L.ms.st_refs[g][yy * 4 + xx] = (int8_t)r;
           }
-        }
-    }
+      }
     sync();
     *done = bdone;
     *cost_out = best_cost;
@@ -640,9 +639,9 @@ struct Coder {
     int cost;
   };
 
-  // enc/frame.rs:852-906
+  // enc/frame.rs:852-906, for the group's reference r
   __device__ void best_inter_ref(int r, Inter &best) {
-    const int lam = P.lambda, rb = lam * ref_bits(r);
+    const int lam = P.lambda, rb = lam * ref_bits(r), g = group();
     uint8_t refs[4] = {(uint8_t)r, (uint8_t)r, (uint8_t)r, (uint8_t)r};
     int16_t mvp[2], mv[2], col[1][2];
     mv_pred(r, mvp);
@@ -653,64 +652,98 @@ struct Coder {
     for (int k = 0; k < 4; k++) { best.refs[k] = (uint8_t)r; best.sub[k] = 0; }
     best.cost = cost + lam * (mvd_bits(mv[0], mv[1], mvp[0], mvp[1]) + 1) + rb;
     if (P.fast) return;  // realtime: 16x16 only (enc/frame.rs:861-863)
-    if (lane() == 0)
-      for (int i = 0; i < 16; i++) { L.st_mvs[i][0] = 0; L.st_mvs[i][1] = 0; L.st_refs[i] = 0; }
+    if (glane() == 0)
+      for (int i = 0; i < 16; i++) { L.ms.st_mvs[g][i][0] = 0; L.ms.st_mvs[g][i][1] = 0; L.ms.st_refs[g][i] = 0; }
     sync();
     uint32_t done = 0;
     uint8_t sub[4] = {0, 0, 0, 0};
     int cost8 = lam * 9;
-    for (int k = 0; k < 4; k++) {
+    bool alive = true;  // this group's 8x8 total is still below its best
+    for (int k = 0; k < 4 && any_lane(alive); k++) {
       int cc;
-      sub[k] = (uint8_t)sub_search(r, k, mv, &done, &cc);
-      cost8 += cc + rb;
-      if (cost8 >= best.cost) break;
+      const int sk = sub_search(r, k, mv, &done, &cc);
+      if (alive) {
+        sub[k] = (uint8_t)sk;
+        cost8 += cc + rb;
+        if (cost8 >= best.cost) alive = false;
+      }
     }
-    if (cost8 < best.cost) {
+    const bool use8 = alive && cost8 < best.cost;
+    if (!any_lane(use8)) return;
+    if (use8) {
       Rect rs[16];
       int n = rects(3, sub, rs);
       best.part = 3;
       for (int i = 0; i < 16; i++) { best.mvs[i][0] = 0; best.mvs[i][1] = 0; }
-      for (int k = 0; k < n; k++) { best.mvs[k][0] = L.st_mvs[rs[k].y * 4 + rs[k].x][0]; best.mvs[k][1] = L.st_mvs[rs[k].y * 4 + rs[k].x][1]; }
+      for (int k = 0; k < n; k++) {
+        best.mvs[k][0] = L.ms.st_mvs[g][rs[k].y * 4 + rs[k].x][0];
+        best.mvs[k][1] = L.ms.st_mvs[g][rs[k].y * 4 + rs[k].x][1];
+      }
       for (int k = 0; k < 4; k++) best.sub[k] = sub[k];
       best.cost = cost8;
-      int16_t at8[4][2];
-      for (int b = 0; b < 4; b++) {
-        at8[b][0] = L.st_mvs[(b / 2) * 8 + (b % 2) * 2][0];
-        at8[b][1] = L.st_mvs[(b / 2) * 8 + (b % 2) * 2][1];
-      }
-      for (int part = 1; part < 3; part++) {
-        int16_t mvs[16][2] = {};
-        int pc = lam * 3;
-        const uint8_t z4[4] = {0, 0, 0, 0};
-        for (int k = 0; k < 2; k++) {
-          int16_t ps[16][2] = {}, m[2], cands[2][2];
-          part_mvps(part, z4, mvs, refs, ps);
-          int b0 = part == 1 ? 2 * k : k, b1 = part == 1 ? 2 * k + 1 : k + 2;
-          cands[0][0] = at8[b0][0]; cands[0][1] = at8[b0][1];
-          cands[1][0] = at8[b1][0]; cands[1][1] = at8[b1][1];
-          const uint8_t *pr = PARTS[part][k];
-          int cc = motion_search(r, pr[0], pr[1], pr[2], pr[3], ps[k], (const int16_t(*)[2])cands, 2, m);
+    }
+    int16_t at8[4][2];
+    for (int b = 0; b < 4; b++) {
+      at8[b][0] = L.ms.st_mvs[g][(b / 2) * 8 + (b % 2) * 2][0];
+      at8[b][1] = L.ms.st_mvs[g][(b / 2) * 8 + (b % 2) * 2][1];
+    }
+    for (int part = 1; part < 3; part++) {
+      int16_t mvs[16][2] = {};
+      int pc = lam * 3;
+      bool palive = use8;
+      const uint8_t z4[4] = {0, 0, 0, 0};
+      for (int k = 0; k < 2 && any_lane(palive); k++) {
+        int16_t ps[16][2] = {}, m[2], cands[2][2];
+        part_mvps(part, z4, mvs, refs, ps);
+        int b0 = part == 1 ? 2 * k : k, b1 = part == 1 ? 2 * k + 1 : k + 2;
+        cands[0][0] = at8[b0][0]; cands[0][1] = at8[b0][1];
+        cands[1][0] = at8[b1][0]; cands[1][1] = at8[b1][1];
+        const uint8_t *pr = PARTS[part][k];
+        int cc = motion_search(r, pr[0], pr[1], pr[2], pr[3], ps[k], (const int16_t(*)[2])cands, 2, m);
+        if (palive) {
           mvs[k][0] = m[0];
           mvs[k][1] = m[1];
           pc += cc + lam * mvd_bits(m[0], m[1], ps[k][0], ps[k][1]) + rb;
-          if (pc >= best.cost) break;
+          if (pc >= best.cost) palive = false;
         }
-        if (pc < best.cost) {
-          best.part = part;
-          for (int i = 0; i < 16; i++) { best.mvs[i][0] = mvs[i][0]; best.mvs[i][1] = mvs[i][1]; }
-          for (int k = 0; k < 4; k++) best.sub[k] = 0;
-          best.cost = pc;
-        }
+      }
+      if (palive && pc < best.cost) {
+        best.part = part;
+        for (int i = 0; i < 16; i++) { best.mvs[i][0] = mvs[i][0]; best.mvs[i][1] = mvs[i][1]; }
+        for (int k = 0; k < 4; k++) best.sub[k] = 0;
+        best.cost = pc;
       }
     }
     for (int k = 0; k < 4; k++) best.refs[k] = (uint8_t)r;
   }
 
+  // the references four at a time, reference c0 + g in group g (a group beyond the last reference
+  // repeats the chunk's first and is not taken); their results through LDS, then taken in reference
+  // order, the first least cost (enc/frame.rs:840-848)
   __device__ void best_inter(Inter &out) {
-    for (int r = 0; r < P.nref; r++) {
+    const int g = group();
+    for (int c0 = 0; c0 < P.nref; c0 += 4) {
+      const int r = c0 + g < P.nref ? c0 + g : c0;
       Inter t;
       best_inter_ref(r, t);
-      if (r == 0 || t.cost < out.cost) out = t;
+      if (glane() == 0) {
+        SearchRes &q = L.ms.res[g];
+        q.cost = t.cost;
+        q.part = t.part;
+        for (int i = 0; i < 16; i++) { q.mvs[i][0] = t.mvs[i][0]; q.mvs[i][1] = t.mvs[i][1]; }
+        for (int k = 0; k < 4; k++) { q.refs[k] = t.refs[k]; q.sub[k] = t.sub[k]; }
+      }
+      sync();
+      for (int k = 0; k < 4 && c0 + k < P.nref; k++) {
+        const SearchRes &q = L.ms.res[k];
+        if (c0 + k == 0 || q.cost < out.cost) {
+          out.cost = q.cost;
+          out.part = q.part;
+          for (int i = 0; i < 16; i++) { out.mvs[i][0] = q.mvs[i][0]; out.mvs[i][1] = q.mvs[i][1]; }
+          for (int j = 0; j < 4; j++) { out.refs[j] = q.refs[j]; out.sub[j] = q.sub[j]; }
+        }
+      }
+      sync();
     }
   }
 

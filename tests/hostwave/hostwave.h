@@ -263,6 +263,15 @@ inline uint32_t sad_u8(uint32_t a, uint32_t b, uint32_t acc) {
   return acc;
 }
 }  // namespace hostwave
+namespace hostwave {
+// ds_bpermute_b32: the value v of lane (addr / 4) mod 64, for each lane its own
+inline int bpermute(int addr, int v) {
+  int64_t all[kLanes];
+  exchange(v, all);
+  return (int)all[(addr >> 2) & (kLanes - 1)];
+}
+}  // namespace hostwave
+#define __builtin_amdgcn_ds_bpermute(addr, v) hostwave::bpermute((addr), (v))
 #define __builtin_amdgcn_alignbyte(hi, lo, s) hostwave::alignbyte((hi), (lo), (s))
 #define __builtin_amdgcn_sad_u8(a, b, acc) hostwave::sad_u8((a), (b), (acc))
 #define __ballot(p) hostwave::ballot((p))
