@@ -164,6 +164,9 @@ struct Lds {
   uint64_t nzm[2][16];  // masks of the nonzero levels and of the levels above 1 (count_bins)
   uint64_t pos8[24];    // 8x8 scan positions of each significance (0-14) and last (15-23) context
   int etop[16], eleft[8], etl;  // the edges of the Intra4x4 block being coded
+  uint32_t kept_bits;           // the kept candidate's rate
+  uint8_t kept_st[440];         // the context states (0-439, those the slice codes on) after its bins
+  uint8_t chroma_st[64];        // the chroma residual's context states after the last full count
 };
 
 // The workgroup's working set. Declared at namespace scope so that every function that touches it
@@ -175,6 +178,7 @@ __shared__ Lds L;
 struct Coder {
   const AnParams &P;
   int mbx, mby, row0, row1, slice_id;
+  int rd_kept;  // the macroblock was decided by rate-distortion: its rate and states are in L.kept_*
 
   __device__ Coder(const AnParams &p) : P(p) {}
 
@@ -2217,16 +2221,31 @@ struct Coder {
 
   // J of the candidate in L.code whose reconstruction is in the canvas (enc/frame.rs:1453-1457)
   // *chroma: the chroma residual's rate when known (see count_bins), else -1
-  __device__ double rd_cost(double lam2, int *chroma) {
+  // The chroma residual's contexts, 61 in four runs: the coded_block_flags of chroma DC and AC
+  // (85 + 4 cat, cats 3 and 4), then for each of significance, last and level the chroma DC
+  // contexts followed by the chroma AC ones.
+  __device__ static int chroma_ctx(int i) {
+    return i < 8 ? 85 + 12 + i : i < 25 ? IA_SIG_OFF[0][3] + i - 8 : i < 42 ? IA_LAST_OFF[0][3] + i - 25 : IA_ABS_OFF[3] + i - 42;
+  }
+
+  // J of the candidate in L.code; its rate in *bits and the context states after its bins in L.tmp.
+  // When its chroma is that of the last full count (*chroma >= 0), the chroma contexts start at the
+  // states that count left them in, so L.tmp still ends as the writer's would.
+  __device__ double rd_cost(double lam2, int *chroma, uint32_t *bits) {
+    const int full = *chroma < 0;
     {
       const uint32_t *s = reinterpret_cast<const uint32_t *>(L.cabac);
       uint32_t *d = reinterpret_cast<uint32_t *>(L.tmp);
       for (int i = lane(); i < 256; i += 64) d[i] = s[i];
     }
     sync();
-    const uint32_t bits = count_bins(L.tmp, chroma);
+    if (!full && lane() < 61) L.tmp[chroma_ctx(lane())] = L.chroma_st[lane()];
+    sync();
+    *bits = count_bins(L.tmp, chroma);
+    if (full && lane() < 61) L.chroma_st[lane()] = L.tmp[chroma_ctx(lane())];
     const long long ssd = ssd_mb();
-    return (double)ssd + lam2 * ((double)bits / 256.0);
+    sync();
+    return (double)ssd + lam2 * ((double)*bits / 256.0);
   }
 
   __device__ void copy_code(ia_mbcode *dst, const ia_mbcode *src) {
@@ -2235,12 +2254,18 @@ struct Coder {
     for (int i = lane(); i < (int)(sizeof(ia_mbcode) / 4); i += 64) d[i] = s[i];
   }
 
-  __device__ void keep(double j, int *have, double *best_j) {
+  // the candidate just costed replaces the kept one when it is the first or its J is lower: its code,
+  // reconstruction, rate and final context states
+  __device__ void keep(double j, uint32_t bits, int *have, double *best_j) {
     if (!*have || j < *best_j) {
       *have = 1;
       *best_j = j;
       copy_code(&L.keep, &L.code);
       save_rec(L.keep_rec);
+      const uint32_t *s = reinterpret_cast<const uint32_t *>(L.tmp);
+      uint32_t *d = reinterpret_cast<uint32_t *>(L.kept_st);
+      for (int i = lane(); i < 110; i += 64) d[i] = s[i];
+      if (lane() == 0) L.kept_bits = bits;
     }
     sync();
   }
@@ -2296,6 +2321,12 @@ struct Coder {
     code_luma8();
   }
 
+  __device__ void rd_keep(double lam2, int *chroma, int *have, double *best_j) {
+    uint32_t b;
+    const double j = rd_cost(lam2, chroma, &b);
+    keep(j, b, have, best_j);
+  }
+
   __device__ void decide_rd(int i16, const Inter *inter, ia_mbstats *st) {
     const double lam2 = P.lam2;
     int have = 0;
@@ -2317,38 +2348,40 @@ struct Coder {
       for (int i = lane(); i < 128; i += 64) ccv(i >> 6, (i & 63) & 7, (i & 63) >> 3) = L.cpred[i >> 6][i & 63];
       sync();
       int none = -1;
-      keep(rd_cost(lam2, &none), &have, &best_j);
+      rd_keep(lam2, &none, &have, &best_j);
       int inter_chroma = -1;
       code_inter(inter->part, inter->mvs, inter->refs, inter->sub, 0);
       if (!(L.code.cbp == 0 && inter->part == 0 && inter->mvs[0][0] == sm[0] && inter->mvs[0][1] == sm[1] && inter->refs[0] == 0))
-        keep(rd_cost(lam2, &inter_chroma), &have, &best_j);
+        rd_keep(lam2, &inter_chroma, &have, &best_j);
       if (P.transform8 && (inter->sub[0] | inter->sub[1] | inter->sub[2] | inter->sub[3]) == 0) {
         recode_inter_t8();
-        if (L.code.t8x8) keep(rd_cost(lam2, &inter_chroma), &have, &best_j);
+        if (L.code.t8x8) rd_keep(lam2, &inter_chroma, &have, &best_j);
       }
     }
     int intra_chroma = -1;
     code_i16(i16);
     code_intra_chroma();
     save_chroma();
-    keep(rd_cost(lam2, &intra_chroma), &have, &best_j);
+    rd_keep(lam2, &intra_chroma, &have, &best_j);
     code_i4();
     restore_chroma();
-    keep(rd_cost(lam2, &intra_chroma), &have, &best_j);
+    rd_keep(lam2, &intra_chroma, &have, &best_j);
     if (P.transform8) {
       code_i8();
       restore_chroma();
-      keep(rd_cost(lam2, &intra_chroma), &have, &best_j);
+      rd_keep(lam2, &intra_chroma, &have, &best_j);
     }
     restore_rec(L.keep_rec);
     copy_code(&L.code, &L.keep);
     sync();
     st->rd_cost = best_j;
+    rd_kept = 1;
   }
 
   // enc/frame.rs:715-790; the decision lands in L.code and the canvas
   __device__ void code_mb(ia_mbstats *st) {
     const int lam = P.lambda;
+    rd_kept = 0;
     st->skip_sad = -1;
     st->i16_cost = -1;
     st->inter_cost = -1;
@@ -2559,8 +2592,19 @@ struct Coder {
     uint32_t *d = reinterpret_cast<uint32_t *>(&P.codes[idx]);
     for (int i = lane(); i < (int)(sizeof(ia_mbcode) / 4); i += 64) d[i] = s[i];
     // the chosen macroblock's bins adapt the live contexts as the writer would; their count is its rate
-    int chroma = -1;
-    const uint32_t bits = P.use_cabac ? count_bins(L.cabac, &chroma) : 0;
+    uint32_t bits = 0;
+    if (P.use_cabac) {
+      if (rd_kept) {
+        // counted when it was costed: take its rate and final states
+        const uint32_t *s = reinterpret_cast<const uint32_t *>(L.kept_st);
+        uint32_t *d = reinterpret_cast<uint32_t *>(L.cabac);
+        for (int i = lane(); i < 110; i += 64) d[i] = s[i];
+        bits = L.kept_bits;
+      } else {
+        int chroma = -1;
+        bits = count_bins(L.cabac, &chroma);
+      }
+    }
     if (lane() == 0) {
       MbState ms;
       ia_mbinfo info;
