@@ -767,7 +767,8 @@ def bench_batched(args):
     on-device promotion of every reconstruction to a reference, replayed as one hipGraph. Pictures
     are generated on the device and stay resident; the timed region is the P frames 2..F-1 (frame 0
     is the IDR, frame 1 the first P and the graph's capture). Each stream's results at frames 0 and 1
-    are compared with the single-stream path's (digests)."""
+    and at the last timed frame are compared with the single-stream path's coding the same chain
+    (digests)."""
     import numpy as np
     import torch
 
@@ -822,22 +823,31 @@ def bench_batched(args):
                 print(f"[batched] frame {t}/{args.frames}", file=sys.stderr, flush=True)
         b.sync()
         dt = time.perf_counter() - t1
+        last = args.frames - 1
+        if last >= 2:
+            for i, an in enumerate(ans):
+                want[i].append(digest(an.read_analysis()))
     steps = max(0, args.frames - 2)
-    # per-stream bit-exactness of frames 0 and 1 against the single-stream path
+    # per-stream bit-exactness of frames 0, 1 and the last timed frame against the single-stream path
+    # coding the same chain
+    checked = [0, 1] + ([last] if last >= 2 else [])
     equal = 0
     print(f"[batched] timed {dt:.2f} s; checking streams against the single-stream path", file=sys.stderr, flush=True)
     for i, ((w, h), ns) in enumerate(zip(order, slices)):
         with Analyser(w, h, nrefs=nrefs, slices=ns, search_mode=mode, search_range=32, sub8x8=True,
                       fast=args.realtime) as an:
             autotune.apply(an, table)
-            pics = [Picture(*(p.cpu().numpy() for p in frames[i][t])) for t in range(min(2, args.frames))]
             ok = True
-            for t, pic in enumerate(pics):
+            for t in range(args.frames):
+                pic = Picture(*(p.cpu().numpy() for p in frames[i][t]))
                 a = frame_args(i, t)
                 r = an.analyse(pic, a["qp"], a["kind"], a["refs"])
                 an.promote_reference(a["promote"])  # loop filter on, as the batch promotes
-                ok = ok and digest(r) == want[i][t]
+                if t in checked:
+                    ok = ok and digest(r) == want[i][checked.index(t)]
             equal += ok
+        if i % 16 == 15:
+            print(f"[batched] checked {i + 1}/{len(ans)} streams", file=sys.stderr, flush=True)
     for an in ans:
         an.close()
     mb_per_step = sum(nmbs)
@@ -865,24 +875,27 @@ def bench_batched(args):
                    "slices": {f"{w}x{h}": s for (w, h), s in zip(order, slices)},
                    "analysis_workers": workers,
                    "tile_shapes": {f"{w}x{h}": autotune.choose(w, h, 32, table) for w, h in sizes}},
-        "per_stream_bitexact": {"frames_checked": [0, 1], "streams_equal": equal, "streams": len(ans),
+        "per_stream_bitexact": {"frames_checked": checked, "streams_equal": equal, "streams": len(ans),
                                 "against": "the single-stream path (itself bit-exact with the oracle)"},
     }
     print(json.dumps(out))
 
 
 def bench_learned_batched(args):
-    """Streams at 3840x2160 (objects +-64 px/picture, noise sigma 4, two scene cuts per stream, each
-    coded as an IDR) through the batched step twice: with the learned proposal (CNN, +-2 refinement,
+    """Streams at 3840x2160 (objects +-64 px/picture, noise sigma 4, two scene cuts per stream) through
+    the batched step twice: with the learned proposal (CNN, +-2 refinement,
     fallback to the +-64 exhaustive search at 1.1x the predicted cost; one persistent proposal
     launch for all streams, in the same hipGraph as the analysis) and with the +-64 exhaustive
-    search. Both chains code their own references. Reported: MB/s of each whole step (search,
-    analysis, reference promotion), the encoder's summed costs of the learned chain against the
-    exhaustive chain's (closed loop), and the fallback rate."""
+    search. Both chains code their own references. Each picture's type comes from the encoder's own
+    decision (libironav_h264w's iw_gop: first picture, keyint, or a scene cut by the low-resolution
+    change measure, enc/mod.rs:700-717, change.rs:57-95), taken on the host for every picture before
+    the timed steps; the generator's cut positions are only reported beside the detected ones.
+    Reported: MB/s of each whole step (search, analysis, reference promotion), the encoder's summed
+    costs of the learned chain against the exhaustive chain's (closed loop), and the fallback rate."""
     import numpy as np
     import torch
 
-    from ironav_amd import abi, autotune, clipgen, proposal
+    from ironav_amd import abi, autotune, clipgen, h264w, proposal
     from ironav_amd.h264a import Analyser, Batch, batch_workers, library
 
     nst = args.streams if args.streams != 64 else 8
@@ -895,6 +908,13 @@ def bench_learned_batched(args):
     frames = [list(clipgen.make_clip_device(sp)) for sp in specs]
     torch.cuda.synchronize()
     gen_s = time.perf_counter() - t0
+    # the picture types: the encoder's scene-cut and keyint decisions on each stream's luma
+    t0 = time.perf_counter()
+    idrs = []
+    for i in range(nst):
+        with h264w.Gop(w, h, threads=os.cpu_count() or 1) as g:
+            idrs.append(tuple(t for t in range(nfr) if g.next(frames[i][t][0].cpu().numpy()) == abi.PIC_IDR))
+    gop_s = time.perf_counter() - t0
     nmb = ((w + 15) // 16) * ((h + 15) // 16)
     workers = batch_workers() or torch.cuda.get_device_properties(0).multi_processor_count * 4
     slices = min((h + 15) // 16, max(1, workers // nst))
@@ -902,7 +922,7 @@ def bench_learned_batched(args):
     table = autotune.load()
 
     def schedule(i, t):
-        last_idr = max([0] + [c for c in cuts[i] if c <= t])
+        last_idr = max(c for c in idrs[i] if c <= t)
         idr = t == last_idr
         refs = [] if idr else [(t - 1 - k) % nrefs for k in range(min(t - last_idr, nrefs))]
         y, cb, cr = frames[i][t]
@@ -944,7 +964,7 @@ def bench_learned_batched(args):
 
     cl, fb, tl, n_l = chain(abi.SEARCH_LEARNED)
     cf, _, tf, n_f = chain(abi.SEARCH_FULL)
-    p_pairs = sum(min(t - max([0] + [c for c in cuts[i] if c <= t]), nrefs)
+    p_pairs = sum(min(t - max(c for c in idrs[i] if c <= t), nrefs)
                   for i in range(nst) for t in range(nfr)) * nmb
     mbs = nst * nmb
     out = {
@@ -961,7 +981,7 @@ def bench_learned_batched(args):
         "dtype": "u8",
         "data": f"synthetic (device-generated clips, {gen_s:.1f} s; resident in HBM)",
         "config": {"workload": f"{nst} streams x {nfr} frames at 3840x2160, objects +-64 px/picture, noise sigma 4, "
-                               f"scene cuts {list(cuts)} (IDR), 3 references, learned proposal (fallback 1.1x, "
+                               f"generator scene cuts {list(cuts)}, 3 references, learned proposal (fallback 1.1x, "
                                f"+-64 exhaustive), slices {slices}, one hipGraph per frame step",
                    "macroblocks_per_step": mbs},
         "exhaustive_chain": {"MB_per_s": round(mbs * n_f / tf, 1), "ms_per_step": round(tf * 1000.0 / n_f, 3)},
@@ -969,6 +989,9 @@ def bench_learned_batched(args):
         "encoder_cost_gap_closed_loop": {"rd_cost": round(float(cl[0] / cf[0] - 1.0), 5),
                                          "inter_cost": round(float(cl[1] / cf[1] - 1.0), 5)},
         "fallback_rate": round(fb / max(1, p_pairs), 4),
+        "idr_pictures": {"detected": [list(x) for x in idrs], "generator_cuts": [list(c) for c in cuts],
+                         "detector": "iw_gop (scene cut: inter > 0.8 x intra on the 4x4-mean low-resolution luma; "
+                                     "keyint 250)", "host_seconds": round(gop_s, 2)},
     }
     print(json.dumps(out))
 
