@@ -2,7 +2,7 @@
  * encoder's streams use.
  *
  * TEST INFRASTRUCTURE ONLY. Nothing in the product (ironav_amd/) includes, links or calls this code;
- * tests/ (and bench.py's rate measurement) load it to check that bitstreams decode to the pictures the
+ * only tests/ load it, to check that bitstreams decode to the pictures the
  * encoder reconstructed (crates/ironav-codec/src/h264/enc/mod.rs:812-825 is the reference's test of
  * the same property) and to pin the oracle's reconstruction primitives against the reference's
  * decoder fixtures (libx264 streams with FFmpeg's decoded pictures,
