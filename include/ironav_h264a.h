@@ -48,8 +48,8 @@ enum { IA_KIND_NONE = 0, IA_KIND_I4X4 = 1, IA_KIND_I8X8 = 2, IA_KIND_I16X16 = 3,
  *                     ia_load_proposal_weights). */
 enum { IA_SEARCH_DIAMOND = 0, IA_SEARCH_FULL = 1, IA_SEARCH_LEARNED = 2 };
 
-/* Picture kinds of `code_picture` (enc/mod.rs:539): 0 IDR (I slices), 1 P. */
-enum { IA_PIC_IDR = 0, IA_PIC_P = 1 };
+/* Picture kinds of `code_picture` (enc/mod.rs:539): 0 IDR (I slices), 1 P, 2 B (ia_analyse_b). */
+enum { IA_PIC_IDR = 0, IA_PIC_P = 1, IA_PIC_B = 2 };
 
 typedef struct ia_config {
   int32_t width, height;   /* luma size in samples (even); mb_w = ceil(width/16), mb_h = ceil(height/16) */
@@ -67,8 +67,9 @@ typedef struct ia_config {
                               a non-finite ratio (infinity) turns the fallback off */
 } ia_config;
 
-/* What later pictures and the loop filter read about a coded macroblock: the P/I subset of
- * h264/mb.rs:34-68 `MbInfo` that `finish_mb` (enc/frame.rs:2281-2339) fills. */
+/* What later pictures and the loop filter read about a coded macroblock: the frame-macroblock subset
+ * of h264/mb.rs:34-68 `MbInfo` that `finish_mb` (enc/frame.rs:2281-2339) fills. List 1 is used by B
+ * pictures only; P and I pictures leave it at vector 0, reference -1. */
 typedef struct ia_mbinfo {
   uint16_t slice;          /* 1-based slice id */
   uint8_t kind;            /* IA_KIND_* */
@@ -80,11 +81,18 @@ typedef struct ia_mbinfo {
   uint8_t nnz[16];         /* luma coefficient counts, raster 4x4 blocks */
   int16_t mv[16][2];       /* list 0 vectors per raster 4x4 block, quarter samples */
   int8_t ref_idx[4];       /* list 0 reference index per 8x8 block, -1 when unused */
-  int32_t ref_id[4];       /* 1 + ref_idx, -1 when unused */
+  int32_t ref_id[4];       /* 1 + ref_idx, -1 when unused (B pictures: 1, the list 0 picture) */
+  int16_t mv1[16][2];      /* list 1 vectors per raster 4x4 block (B pictures) */
+  int8_t ref_idx1[4];      /* list 1 reference index per 8x8 block, -1 when unused */
+  int32_t ref_id1[4];      /* 2 (the list 1 picture) where list 1 is used, else -1 */
+  uint8_t bdirect;         /* B_Skip or B_Direct_16x16 */
+  uint8_t direct8;         /* B_8x8: mask of the 8x8 blocks coded B_Direct_8x8 (15 for bdirect) */
+  uint8_t reserved1[2];
 } ia_mbinfo;
 
-/* A macroblock's decisions and levels, ready to write: the P/I subset of `MbCode`
- * (enc/frame.rs:275-310). Levels fit int16 at 8 bits. */
+/* A macroblock's decisions and levels, ready to write: `MbCode` (enc/frame.rs:275-310). Levels fit
+ * int16 at 8 bits. In B pictures `part` is 0..3 (B_8x8 has no sub-partitions) and the motion is in
+ * bdir/buse/bmv/bmvd; `skip` marks B_Skip, and part 0 with bdir[0] 0 B_Direct_16x16. */
 typedef struct ia_mbcode {
   uint8_t kind;            /* IA_KIND_* */
   uint8_t skip;            /* P_Skip */
@@ -106,6 +114,10 @@ typedef struct ia_mbcode {
   int16_t dc[16];          /* Intra16x16 luma DC, scan order */
   int16_t cdc[2][4];       /* chroma DC per component */
   int16_t cac[2][4][16];   /* chroma AC per component and 4x4 block, scan order (slot 0 unused) */
+  uint8_t bdir[4];         /* B: per partition 0 direct, 1 list 0, 2 list 1, 3 both */
+  uint8_t buse[4];         /* B: per 8x8 block, bit 0 list 0 used, bit 1 list 1 used */
+  int16_t bmv[2][16][2];   /* B: per list, the vector of each raster 4x4 block */
+  int16_t bmvd[2][4][2];   /* B: per list, each partition's vector difference */
 } ia_mbcode;
 
 /* The costs the decisions were taken on, for checking and for rate studies. */

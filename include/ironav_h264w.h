@@ -68,6 +68,9 @@ typedef struct iw_params {
   int32_t transform8;         /* transform_8x8_mode_flag (High profile) */
   int32_t nrefs;              /* references P pictures use: max_num_ref_frames */
   int32_t slices;             /* slice count: the bands i*mb_h/n .. (i+1)*mb_h/n (enc/mod.rs:603) */
+  int32_t bframes;            /* 0: IPPP with order counts of type 2; > 0: B pictures may be written
+                                 (iw_write_picture_at), order counts of type 0 in the slice headers and
+                                 one picture of reordering (enc/mod.rs:322-327, 370-372) */
 } iw_params;
 
 typedef struct iw_stream iw_stream;
@@ -84,6 +87,14 @@ void iw_stream_destroy(iw_stream *s);
  * copies it. The macroblocks' CABAC bins and contexts are those the analysis counted its rate on. */
 int iw_write_picture(iw_stream *s, int kind, int qp, int nref, const ia_mbcode *codes, size_t *len);
 int iw_read_picture(const iw_stream *s, uint8_t *out, size_t cap);
+
+/* The same for a stream with bframes > 0, in coding order, each picture with its display index (the
+ * reference's `d`, enc/mod.rs:539-547): kind 2 writes a B picture (B slices with spatial direct
+ * prediction against one list 0 and one list 1 picture, not kept as a reference: nal_ref_idc 0, no
+ * frame_num step; enc/mod.rs:566-600, 636-641); its picture order count is 2 x (display - the last IDR
+ * picture's display). */
+int iw_write_picture_at(iw_stream *s, int kind, int qp, int nref, int64_t display, const ia_mbcode *codes,
+                        size_t *len);
 
 /* level_idc for a picture size and frame rate: the first level of Table A-1 whose frame size and
  * macroblock rate hold it, else the first whose frame size does (enc/mod.rs:155-158). */

@@ -6,7 +6,7 @@ import numpy as np
 IA_OK, IA_EINVAL, IA_EDEVICE, IA_ENOMEM, IA_ESTATE, IA_EUNSUPPORTED = 0, -1, -2, -3, -4, -5
 KIND_NONE, KIND_I4X4, KIND_I8X8, KIND_I16X16, KIND_IPCM, KIND_INTER = range(6)
 SEARCH_DIAMOND, SEARCH_FULL, SEARCH_LEARNED = 0, 1, 2
-PIC_IDR, PIC_P = 0, 1
+PIC_IDR, PIC_P, PIC_B = 0, 1, 2
 NPART = 41
 CNN_PARAMS, CNN_CELLS, CNN_FEATURES, CNN_OUTPUTS = 21988, 81, 8, 4
 
@@ -33,7 +33,9 @@ class MbInfo(C.Structure):
     _fields_ = [
         ("slice", C.c_uint16), ("kind", C.c_uint8), ("qp", C.c_int8), ("cbp", C.c_uint8), ("skip", C.c_uint8),
         ("t8x8", C.c_uint8), ("reserved", C.c_uint8), ("nnz", C.c_uint8 * 16), ("mv", (C.c_int16 * 2) * 16),
-        ("ref_idx", C.c_int8 * 4), ("ref_id", C.c_int32 * 4),
+        ("ref_idx", C.c_int8 * 4), ("ref_id", C.c_int32 * 4), ("mv1", (C.c_int16 * 2) * 16),
+        ("ref_idx1", C.c_int8 * 4), ("ref_id1", C.c_int32 * 4), ("bdirect", C.c_uint8), ("direct8", C.c_uint8),
+        ("reserved1", C.c_uint8 * 2),
     ]
 
 
@@ -44,7 +46,8 @@ class MbCode(C.Structure):
         ("modes", C.c_uint8 * 16), ("refs", C.c_uint8 * 4), ("sub", C.c_uint8 * 4), ("nnz", C.c_uint8 * 16),
         ("cnnz", (C.c_uint8 * 4) * 2), ("mvs", (C.c_int16 * 2) * 16), ("mvds", (C.c_int16 * 2) * 16),
         ("luma", (C.c_int16 * 16) * 16), ("luma8", (C.c_int16 * 64) * 4), ("dc", C.c_int16 * 16),
-        ("cdc", (C.c_int16 * 4) * 2), ("cac", ((C.c_int16 * 16) * 4) * 2),
+        ("cdc", (C.c_int16 * 4) * 2), ("cac", ((C.c_int16 * 16) * 4) * 2), ("bdir", C.c_uint8 * 4),
+        ("buse", C.c_uint8 * 4), ("bmv", ((C.c_int16 * 2) * 16) * 2), ("bmvd", ((C.c_int16 * 2) * 4) * 2),
     ]
 
 

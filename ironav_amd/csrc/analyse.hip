@@ -2274,8 +2274,8 @@ struct Coder {
   // rate-distortion decision does not otherwise use: every intra candidate codes the same chroma
   // (enc/frame.rs:1807-1852 depends on the chroma samples and edges only), so the candidates after
   // the first take it from there; their reconstructed chroma is already in the canvas.
-  static_assert(offsetof(ia_mbcode, cac) == offsetof(ia_mbcode, cdc) + 16 && sizeof(ia_mbcode) == offsetof(ia_mbcode, cac) + 256,
-                "the chroma levels end the code, DC first");
+  static_assert(offsetof(ia_mbcode, cac) == offsetof(ia_mbcode, cdc) + 16 && offsetof(ia_mbcode, bdir) == offsetof(ia_mbcode, cac) + 256,
+                "the chroma levels end the levels, DC first");
   __device__ void save_chroma() {
     const uint32_t *s = reinterpret_cast<const uint32_t *>(&L.code);
     uint32_t *d = reinterpret_cast<uint32_t *>(&L.i4);
@@ -2535,6 +2535,8 @@ struct Coder {
     for (int b8 = 0; b8 < 4; b8++) {
       info.ref_idx[b8] = inter ? r8[b8] : -1;
       info.ref_id[b8] = inter ? 1 + r8[b8] : -1;
+      info.ref_idx1[b8] = -1;
+      info.ref_id1[b8] = -1;
     }
   }
 

@@ -22,6 +22,7 @@
 // theirs with cross-lane reads.
 #include <mutex>
 
+#include <cstddef>
 #include "common.h"
 #include "h264_tables.h"
 
@@ -156,8 +157,12 @@ __device__ __forceinline__ Thr thresholds(const DbLds &L, int qpav, int b) {
 
 __device__ __forceinline__ int chroma_qp(const DbLds &L, int qp) { return L.qpc[clip3(0, 51, qp)]; }
 
-constexpr int REC_DWORDS = sizeof(ia_mbinfo) / 4;
-static_assert(sizeof(ia_mbinfo) % 4 == 0 && REC_DWORDS <= 32, "a record is read as dwords by lanes 0-31 / 32-63");
+// The filter runs on P and I pictures (the encoder's references; B pictures are never kept): it reads
+// the records up to the list 0 motion, their first REC_DWORDS dwords, and leaves list 1 as in P pictures.
+constexpr int REC_STRIDE = sizeof(ia_mbinfo) / 4;
+constexpr int REC_DWORDS = offsetof(ia_mbinfo, mv1) / 4;
+static_assert(sizeof(ia_mbinfo) % 4 == 0 && offsetof(ia_mbinfo, mv1) % 4 == 0 && REC_DWORDS <= 32,
+              "a record's list 0 part is read as dwords by lanes 0-31 / 32-63");
 
 // Macroblock row `row` of one picture (one wave). prog[r]: macroblocks of row r finished (the
 // counters start at 0 and are 0 again when the launch ends); err: set when a row waited too long.
@@ -179,7 +184,7 @@ __device__ __forceinline__ void deblock_row(DbLds &L, uint8_t *__restrict__ py, 
   auto fetch_record = [&](int x) -> uint32_t {
     const int k = lane & 31;
     if (k >= REC_DWORDS || (lane >= 32 && row == 0)) return 0u;
-    return grec[(size_t)((lane < 32 ? row : row - 1) * mb_w + x) * REC_DWORDS + k];
+    return grec[(size_t)((lane < 32 ? row : row - 1) * mb_w + x) * REC_STRIDE + k];
   };
   auto store_record = [&](int x, uint32_t v) {
     const int k = lane & 31;

@@ -254,12 +254,13 @@ int iw_cabac_encode_bins(const uint8_t *kinds, const uint16_t *ctx, const uint8_
 
 /* ---- macroblock bins (enc/frame.rs:2028-2279) ---------------------------------------------------- */
 
-/* what the bins of later macroblocks read about a written one (enc/frame.rs:239-272, P/I subset) */
+/* what the bins of later macroblocks read about a written one (enc/frame.rs:239-272) */
 typedef struct {
   uint8_t i4, inter, t8x8, skip, kind, cbp, chroma_mode, cbf_dc;
   uint8_t modes[16], nnz[16], cnnz[2][4];
   int8_t refs[4];
-  uint8_t mvd[16][2];
+  uint8_t bdirect;                /* B_Skip or B_Direct_16x16 */
+  uint8_t mvd[16][2], mvd1[16][2]; /* lists 0 and 1 */
 } mbstate_t;
 
 typedef struct {
@@ -308,13 +309,22 @@ static int count_nz(const int16_t *v, int n) {
   return k;
 }
 
-/* enc/frame.rs:2281-2339, the neighbour state */
-static void finish_state(const ia_mbcode *code, mbstate_t *s) {
+/* enc/frame.rs:2281-2339, the neighbour state; b: a macroblock of a B slice */
+static void finish_state(const ia_mbcode *code, mbstate_t *s, int b) {
   const int inter = code->kind == K_INTER;
+  static const uint8_t Z4[4] = {0, 0, 0, 0};
   memset(s, 0, sizeof *s);
   rect_t rs[16];
-  const int n = rects(code->part & 3, code->sub, rs);
-  if (inter && !code->skip) mvd_grid(rs, n, (const int16_t(*)[2])code->mvds, s->mvd);
+  const int n = rects(code->part & 3, b ? Z4 : code->sub, rs);
+  if (inter && !code->skip) {
+    if (b) {
+      mvd_grid(rs, n, (const int16_t(*)[2])code->bmvd[0], s->mvd);
+      mvd_grid(rs, n, (const int16_t(*)[2])code->bmvd[1], s->mvd1);
+    } else {
+      mvd_grid(rs, n, (const int16_t(*)[2])code->mvds, s->mvd);
+    }
+  }
+  s->bdirect = (uint8_t)(b && inter && (code->part & 3) == 0 && code->bdir[0] == 0);
   const int cbpc = code->cbp >> 4;
   s->cbf_dc = (uint8_t)((code->kind == K_I16 && count_nz(code->dc, 16) > 0) |
                         ((cbpc > 0 && count_nz(code->cdc[0], 4) > 0) << 1) | ((cbpc > 0 && count_nz(code->cdc[1], 4) > 0) << 2));
@@ -413,23 +423,74 @@ static void mvd_bins(cw_t *c, int comp, int d, int sum) {
   }
 }
 
+/* The mb_type of a B macroblock (Table 7-14; enc/frame.rs:2343-2351). */
+static int b_mb_type(const ia_mbcode *code) {
+  static const uint8_t PAIRS[9][2] = {{1, 1}, {2, 2}, {1, 2}, {2, 1}, {1, 3}, {2, 3}, {3, 1}, {3, 2}, {3, 3}};
+  const int part = code->part & 3;
+  if (part == 0) return code->bdir[0];
+  if (part == 3) return 22;
+  int i = 0;
+  while (i < 8 && !(PAIRS[i][0] == code->bdir[0] && PAIRS[i][1] == code->bdir[1])) i++;
+  return 4 + 2 * i + (part == 2);
+}
+
 static void mb_bins(cw_t *c, const ia_mbcode *code, const mbstate_t *left, const mbstate_t *top, int p_slice,
-                    int transform8, int nref) {
+                    int b_slice, int transform8, int nref) {
   const int kind = code->kind;
   if (p_slice) {
-    cw_decision(c, 11 + (left && !left->skip) + (top && !top->skip), code->skip);
+    cw_decision(c, (b_slice ? 24 : 11) + (left && !left->skip) + (top && !top->skip), code->skip);
     if (code->skip) return;
   }
-  const int cbpl = code->cbp & 15, cbpc = code->cbp >> 4, prefix_ok = p_slice, base = p_slice ? 17 : 3;
+  const int cbpl = code->cbp & 15, cbpc = code->cbp >> 4, prefix_ok = p_slice, base = b_slice ? 32 : p_slice ? 17 : 3;
   const int part = code->part & 3;
-  if (kind == K_INTER) {
+  const int b_ctx = 27 + (left && !left->bdirect) + (top && !top->bdirect);
+  if (kind == K_INTER && b_slice) {
+    /* B_Direct_16x16 0, B_L0_16x16 100, B_L1_16x16 101, the others 11 and four or five more bins
+       (Table 9-37) */
+    const int t = b_mb_type(code);
+    cw_decision(c, b_ctx, t != 0);
+    if (t == 1 || t == 2) {
+      cw_decision(c, 30, 0);
+      cw_decision(c, 32, t == 2);
+    } else if (t != 0) {
+      cw_decision(c, 30, 1);
+      const int bits = t >= 3 && t <= 10 ? t - 3 : t == 11 ? 14 : t == 22 ? 15 : (t + 4) >> 1;
+      cw_decision(c, 31, (bits >> 3) & 1);
+      for (int sh = 2; sh >= 0; sh--) cw_decision(c, 32, (bits >> sh) & 1);
+      if (t >= 12 && t <= 21) cw_decision(c, 32, (t + 4) & 1);
+    }
+    if (t == 22)
+      /* sub_mb_type: B_Direct_8x8 0, B_L0_8x8 100, B_L1_8x8 101, B_Bi_8x8 11000 */
+      for (int k = 0; k < 4; k++) {
+        const int d = code->bdir[k];
+        cw_decision(c, 36, d != 0);
+        if (d == 1 || d == 2) {
+          cw_decision(c, 37, 0);
+          cw_decision(c, 39, d == 2);
+        } else if (d == 3) {
+          cw_decision(c, 37, 1);
+          cw_decision(c, 38, 0);
+          cw_decision(c, 39, 0);
+          cw_decision(c, 39, 0);
+        }
+      }
+  } else if (kind == K_INTER) {
     const int b1 = part == 1 || part == 2, b2 = part == 1 || part == 3;
     cw_decision(c, 14, 0);
     cw_decision(c, 15, b1);
     cw_decision(c, b1 ? 17 : 16, b2);
   } else {
     int first;
-    if (prefix_ok) {
+    if (b_slice) {
+      /* 111101: an intra type follows, with the suffix contexts */
+      cw_decision(c, b_ctx, 1);
+      cw_decision(c, 30, 1);
+      cw_decision(c, 31, 1);
+      cw_decision(c, 32, 1);
+      cw_decision(c, 32, 0);
+      cw_decision(c, 32, 1);
+      first = 32;
+    } else if (prefix_ok) {
       cw_decision(c, 14, 1);
       first = 17;
     } else {
@@ -485,7 +546,7 @@ static void mb_bins(cw_t *c, const ia_mbcode *code, const mbstate_t *left, const
       if (m > 1) cw_decision(c, 67, m > 2);
     }
   } else {
-    if (part == 3)
+    if (!b_slice && part == 3)
       for (int k = 0; k < 4; k++) {
         const int s = code->sub[k];
         if (s == 0) cw_decision(c, 21, 1);
@@ -502,7 +563,7 @@ static void mb_bins(cw_t *c, const ia_mbcode *code, const mbstate_t *left, const
           cw_decision(c, 23, 0);
         }
       }
-    if (nref > 1) {
+    if (!b_slice && nref > 1) {
       uint8_t grid[4] = {0, 0, 0, 0};
       rect_t parts[4];
       int np;
@@ -523,16 +584,22 @@ static void mb_bins(cw_t *c, const ia_mbcode *code, const mbstate_t *left, const
           for (int x8 = bx / 2; x8 <= (bx + w - 1) / 2; x8++) grid[y8 * 2 + x8] = code->refs[k];
       }
     }
+    /* list 0's differences for the partitions using it, then list 1's */
+    static const uint8_t Z4[4] = {0, 0, 0, 0};
     rect_t rs[16];
-    const int n = rects(part, code->sub, rs);
-    uint8_t grid[16][2];
-    mvd_grid(rs, n, (const int16_t(*)[2])code->mvds, grid);
-    for (int k = 0; k < n; k++) {
-      const int bx = rs[k].x, by = rs[k].y;
-      for (int comp = 0; comp < 2; comp++) {
-        const int a = bx > 0 ? grid[by * 4 + bx - 1][comp] : (left ? left->mvd[by * 4 + 3][comp] : 0);
-        const int bb = by > 0 ? grid[(by - 1) * 4 + bx][comp] : (top ? top->mvd[12 + bx][comp] : 0);
-        mvd_bins(c, comp, code->mvds[k][comp], a + bb);
+    const int n = rects(part, b_slice ? Z4 : code->sub, rs);
+    for (int l = 0; l < (b_slice ? 2 : 1); l++) {
+      const int16_t(*mvds)[2] = b_slice ? (const int16_t(*)[2])code->bmvd[l] : (const int16_t(*)[2])code->mvds;
+      uint8_t grid[16][2];
+      mvd_grid(rs, n, mvds, grid);
+      for (int k = 0; k < n; k++) {
+        if (b_slice && !((code->bdir[k] >> l) & 1)) continue;
+        const int bx = rs[k].x, by = rs[k].y;
+        for (int comp = 0; comp < 2; comp++) {
+          const int a = bx > 0 ? grid[by * 4 + bx - 1][comp] : (left ? (l ? left->mvd1 : left->mvd)[by * 4 + 3][comp] : 0);
+          const int bb = by > 0 ? grid[(by - 1) * 4 + bx][comp] : (top ? (l ? top->mvd1 : top->mvd)[12 + bx][comp] : 0);
+          mvd_bins(c, comp, mvds[k][comp], a + bb);
+        }
       }
     }
   }
@@ -605,6 +672,7 @@ struct iw_stream {
   iw_params p;
   int mb_w, mb_h, level, nslices;
   uint32_t frame_num, idr_id;
+  int64_t idr_at;  /* display index of the last IDR picture */
   mbstate_t *state;
   bw_t out;
 };
@@ -629,7 +697,7 @@ int iw_level(int width, int height, int fps_num, int fps_den) {
 
 int iw_stream_create(iw_stream **out, const iw_params *p) {
   if (!out || !p || p->width < 16 || p->height < 16 || p->width > 16384 || p->height > 16384 || (p->width & 1) ||
-      (p->height & 1) || p->nrefs < 1 || p->nrefs > 16 || p->slices < 1)
+      (p->height & 1) || p->nrefs < 1 || p->nrefs > 16 || p->slices < 1 || p->bframes < 0)
     return IW_EINVAL;
   if (!p->cabac) return IW_EUNSUPPORTED;
   iw_stream *s = (iw_stream *)calloc(1, sizeof *s);
@@ -674,8 +742,16 @@ static void write_sps(iw_stream *s, bw_t *w) {
     bw_flag(w, 0); /* seq_scaling_matrix_present_flag */
   }
   bw_ue(w, 4); /* log2_max_frame_num_minus4: 8 bits */
-  bw_ue(w, 2); /* pic_order_cnt_type 2: output order is decoding order (no B pictures) */
-  bw_ue(w, (uint32_t)s->p.nrefs); /* max_num_ref_frames */
+  const int b = s->p.bframes > 0;
+  /* the references kept: those P pictures use, and the two either side of B pictures */
+  const uint32_t refs = (uint32_t)(b && s->p.nrefs < 2 ? 2 : s->p.nrefs);
+  if (b) {
+    bw_ue(w, 0); /* pic_order_cnt_type 0: counted in the slice headers */
+    bw_ue(w, 4); /* log2_max_pic_order_cnt_lsb_minus4: 8 bits */
+  } else {
+    bw_ue(w, 2); /* pic_order_cnt_type 2: output order is decoding order */
+  }
+  bw_ue(w, refs); /* max_num_ref_frames */
   bw_flag(w, 0);
   bw_ue(w, (uint32_t)s->mb_w - 1);
   bw_ue(w, (uint32_t)s->mb_h - 1);
@@ -711,8 +787,9 @@ static void write_sps(iw_stream *s, bw_t *w) {
   bw_ue(w, 0);
   bw_ue(w, 15);
   bw_ue(w, 15);
-  bw_ue(w, 0);                                          /* max_num_reorder_frames */
-  bw_ue(w, (uint32_t)(s->p.nrefs > 1 ? s->p.nrefs : 1)); /* max_dec_frame_buffering */
+  /* the P picture after B pictures is shown after them; B pictures are not kept */
+  bw_ue(w, (uint32_t)b);                             /* max_num_reorder_frames */
+  bw_ue(w, refs > 1u + (uint32_t)b ? refs : 1u + (uint32_t)b); /* max_dec_frame_buffering */
   bw_finish(w);
 }
 
@@ -756,14 +833,20 @@ static int emit_nal(iw_stream *s, int ref_idc, int type, const bw_t *rbsp) {
   return IW_OK;
 }
 
-int iw_write_picture(iw_stream *s, int kind, int qp, int nref, const ia_mbcode *codes, size_t *len) {
-  if (!s || !codes || !len || (kind != 0 && kind != 1) || qp < 0 || qp > 51) return IW_EINVAL;
-  const int idr = kind == 0;
-  if (!idr && (nref < 1 || nref > s->p.nrefs)) return IW_EINVAL;
+static int write_picture(iw_stream *s, int kind, int qp, int nref, int64_t display, const ia_mbcode *codes, size_t *len) {
+  if (!s || !codes || !len || kind < 0 || kind > 2 || qp < 0 || qp > 51) return IW_EINVAL;
+  const int idr = kind == 0, b = kind == 2, poc_type0 = s->p.bframes > 0;
+  if (b && !poc_type0) return IW_EINVAL;
+  if (kind == 1 && (nref < 1 || nref > s->p.nrefs)) return IW_EINVAL;
   if (idr) {
     s->frame_num = 0;
+    s->idr_at = display;
     nref = 0;
   }
+  /* B slices use one picture per list: the PPS's default of one active index each (enc/mod.rs:582-586) */
+  if (b) nref = 1;
+  if (poc_type0 && display < s->idr_at) return IW_EINVAL;
+  const uint32_t poc = (uint32_t)(2 * (display - s->idr_at));
   s->out.len = 0;
   s->out.overflow = 0;
   int rc = IW_OK;
@@ -787,24 +870,27 @@ int iw_write_picture(iw_stream *s, int kind, int qp, int nref, const ia_mbcode *
     bw_t *w = &c.w;
     /* the slice header (enc/mod.rs:573-603) */
     bw_ue(w, (uint32_t)(row0 * mb_w));
-    bw_ue(w, idr ? 7 : 5);
+    bw_ue(w, idr ? 7 : b ? 6 : 5); /* slice_type: I, B or P, all slices of the picture alike */
     bw_ue(w, 0);
     bw_put(w, 8, s->frame_num);
     if (idr) bw_ue(w, s->idr_id);
+    if (poc_type0) bw_put(w, 8, poc & 255); /* pic_order_cnt_lsb */
+    if (b) bw_flag(w, 1);                   /* direct_spatial_mv_pred_flag */
     if (!idr) {
-      if (nref > 1) {
+      if (!b && nref > 1) {
         bw_flag(w, 1);
         bw_ue(w, (uint32_t)nref - 1);
       } else {
         bw_flag(w, 0);
       }
-      bw_flag(w, 0); /* ref_pic_list_modification_flag_l0 */
+      bw_flag(w, 0);         /* ref_pic_list_modification_flag_l0 */
+      if (b) bw_flag(w, 0);  /* ref_pic_list_modification_flag_l1 */
     }
     if (idr) {
       bw_flag(w, 0);
       bw_flag(w, 0);
-    } else {
-      bw_flag(w, 0); /* adaptive_ref_pic_marking_mode_flag */
+    } else if (!b) {
+      bw_flag(w, 0); /* adaptive_ref_pic_marking_mode_flag (B pictures are not references) */
     }
     if (!idr) bw_ue(w, 0); /* cabac_init_idc */
     bw_se(w, qp - 26);
@@ -818,16 +904,21 @@ int iw_write_picture(iw_stream *s, int kind, int qp, int nref, const ia_mbcode *
         const ia_mbcode *code = &codes[idx];
         if (code->kind != K_I4 && code->kind != K_I8 && code->kind != K_I16 && code->kind != K_INTER) rc = IW_EINVAL;
         if (idr && code->kind == K_INTER) rc = IW_EINVAL;
-        if (!idr && code->kind == K_INTER && !code->skip)
+        if (kind == 1 && code->kind == K_INTER && !code->skip)
           for (int k = 0; k < 4; k++)
             if (code->refs[k] >= nref) rc = IW_EINVAL;
+        if (b && code->kind == K_INTER && !code->skip) {
+          const int part = code->part & 3;
+          for (int k = 0; k < (part == 0 ? 1 : part == 3 ? 4 : 2); k++)
+            if (code->bdir[k] > 3 || (part != 0 && part != 3 && code->bdir[k] == 0)) rc = IW_EINVAL;
+        }
         if (rc) break;
-        mb_bins(&c, code, left, top, !idr, s->p.cabac && s->p.transform8, nref);
+        mb_bins(&c, code, left, top, !idr, b, s->p.cabac && s->p.transform8, nref);
         cw_terminate(&c, mby + 1 == row1 && mbx + 1 == mb_w);
-        finish_state(code, &s->state[idx]);
+        finish_state(code, &s->state[idx], b);
       }
     bw_pad_zeros(w);
-    if (!rc) rc = w->overflow ? IW_ENOMEM : emit_nal(s, idr ? 3 : 2, idr ? 5 : 1, w);
+    if (!rc) rc = w->overflow ? IW_ENOMEM : emit_nal(s, idr ? 3 : b ? 0 : 2, idr ? 5 : 1, w);
     free(w->buf);
   }
   if (rc) {
@@ -835,9 +926,19 @@ int iw_write_picture(iw_stream *s, int kind, int qp, int nref, const ia_mbcode *
     return rc;
   }
   if (idr) s->idr_id = (s->idr_id + 1) % 65536;
-  s->frame_num = (s->frame_num + 1) % 256;
+  if (!b) s->frame_num = (s->frame_num + 1) % 256;
   *len = s->out.len;
   return IW_OK;
+}
+
+int iw_write_picture(iw_stream *s, int kind, int qp, int nref, const ia_mbcode *codes, size_t *len) {
+  if (!s || kind == 2) return IW_EINVAL;
+  return write_picture(s, kind, qp, nref, s->idr_at, codes, len);
+}
+
+int iw_write_picture_at(iw_stream *s, int kind, int qp, int nref, int64_t display, const ia_mbcode *codes,
+                        size_t *len) {
+  return write_picture(s, kind, qp, nref, display, codes, len);
 }
 
 int iw_read_picture(const iw_stream *s, uint8_t *out, size_t cap) {

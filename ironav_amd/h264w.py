@@ -25,7 +25,8 @@ IW_OK, IW_EINVAL, IW_ENOMEM, IW_EUNSUPPORTED, IW_ESPACE = 0, -1, -3, -5, -6
 
 class Params(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("fps_num", C.c_int32), ("fps_den", C.c_int32),
-                ("cabac", C.c_int32), ("transform8", C.c_int32), ("nrefs", C.c_int32), ("slices", C.c_int32)]
+                ("cabac", C.c_int32), ("transform8", C.c_int32), ("nrefs", C.c_int32), ("slices", C.c_int32),
+                ("bframes", C.c_int32)]
 
 
 def library():
@@ -45,6 +46,7 @@ def library():
             "iw_stream_create": (i, [C.POINTER(vp), C.POINTER(Params)]),
             "iw_stream_destroy": (None, [vp]),
             "iw_write_picture": (i, [vp, i, i, i, vp, C.POINTER(C.c_size_t)]),
+            "iw_write_picture_at": (i, [vp, i, i, i, C.c_int64, vp, C.POINTER(C.c_size_t)]),
             "iw_read_picture": (i, [vp, vp, C.c_size_t]),
             "iw_level": (i, [i, i, i, i]),
             "iw_put_nal": (i, [i, i, vp, C.c_size_t, vp, C.c_size_t, C.POINTER(C.c_size_t)]),
@@ -118,6 +120,33 @@ class Gop:
         return abi.PIC_IDR if k.value == 0 else abi.PIC_P
 
 
+def coding_order(idr, bframes=2):
+    """The order `H264Encoder::encode` codes pictures in (enc/mod.rs:700-731, code_pending :675-686):
+    idr[d] says whether display picture d is an IDR picture (Gop.next); with bframes > 0, the pictures
+    between two P pictures wait and are coded as B pictures after the P picture that follows them; the
+    last waiting picture before an IDR picture, or at the end, is coded as P instead. Returns
+    [(display, kind)] in coding order, kind abi.PIC_IDR / PIC_P / PIC_B."""
+    out, pending = [], []
+
+    def code_pending():
+        if pending:
+            out.append((pending.pop(), abi.PIC_P))
+        out.extend((d, abi.PIC_B) for d in pending)
+        pending.clear()
+
+    for d, is_idr in enumerate(idr):
+        if is_idr or len(pending) == bframes:
+            if is_idr:
+                code_pending()
+            out.append((d, abi.PIC_IDR if is_idr else abi.PIC_P))
+            out.extend((p, abi.PIC_B) for p in pending)
+            pending.clear()
+        else:
+            pending.append(d)
+    code_pending()
+    return out
+
+
 def level(width, height, fps_num=0, fps_den=0):
     """level_idc the SPS carries for this size and rate."""
     return int(library().iw_level(width, height, fps_num, fps_den))
@@ -149,10 +178,11 @@ def cabac_encode_bins(kinds, ctx, vals, intra, qp):
 
 
 class Stream:
-    """One IPPP output stream: each analysed picture written as an Annex B access unit."""
+    """One output stream: each analysed picture written as an Annex B access unit. With bframes > 0
+    pictures are written in coding order with their display index (write_at), B pictures included."""
 
-    def __init__(self, width, height, *, nrefs=3, slices=1, transform8=True, cabac=True, fps=(0, 0)):
-        p = Params(width, height, fps[0], fps[1], int(cabac), int(transform8), nrefs, slices)
+    def __init__(self, width, height, *, nrefs=3, slices=1, transform8=True, cabac=True, fps=(0, 0), bframes=0):
+        p = Params(width, height, fps[0], fps[1], int(cabac), int(transform8), nrefs, slices, bframes)
         h = C.c_void_p()
         _check(library().iw_stream_create(C.byref(h), C.byref(p)))
         self._h = h
@@ -180,6 +210,18 @@ class Stream:
         n = C.c_size_t()
         L = library()
         _check(L.iw_write_picture(self._h, 0 if kind == abi.PIC_IDR else 1, qp, nref, abi.ptr(codes), C.byref(n)))
+        out = C.create_string_buffer(max(1, n.value))
+        _check(L.iw_read_picture(self._h, out, n.value))
+        return out.raw[:n.value]
+
+    def write_at(self, kind, qp, codes, display, nref=0):
+        """The access unit of a picture of `kind` (abi.PIC_IDR, PIC_P or PIC_B) shown at `display`."""
+        codes = np.ascontiguousarray(codes, dtype=abi.MBCODE_DTYPE)
+        if codes.shape != (self.mb_w * self.mb_h,):
+            raise ValueError("one code per macroblock")
+        n = C.c_size_t()
+        L = library()
+        _check(L.iw_write_picture_at(self._h, int(kind), qp, nref, int(display), abi.ptr(codes), C.byref(n)))
         out = C.create_string_buffer(max(1, n.value))
         _check(L.iw_read_picture(self._h, out, n.value))
         return out.raw[:n.value]

@@ -163,6 +163,10 @@ typedef struct {
   int nmods, mod_idc[32], mod_val[32];
   int luma_log2wd, chroma_log2wd;
   int lw[32], lo[32], cw[32][2], co[32][2];
+  /* B slices: direct_spatial_mv_pred_flag, list 1's size, modifications and explicit weights */
+  int direct_spatial, num_ref_idx_active1;
+  int nmods1, mod_idc1[32], mod_val1[32];
+  int lw1[32], lo1[32], cw1[32][2], co1[32][2];
   int no_output_prior, long_term_ref, adaptive, nmmco, mmco[32][3];
   int cabac_init_idc, qp, disable_deblock, alpha_offset, beta_offset;
 } sh_t;
@@ -177,6 +181,11 @@ typedef struct {
   uint8_t mvd[16][2];  /* |mvd| per 4x4 block, clipped */
   int8_t ref[4];
   int16_t mv[16][2];
+  /* list 1 (B slices), and the direct prediction of B_Skip / B_Direct_16x16 and B_Direct_8x8 blocks */
+  uint8_t mvd1[16][2];
+  int8_t ref1[4];
+  int16_t mv1[16][2];
+  uint8_t bdirect, direct8;
   int qp;
 } dmb_t;
 
@@ -359,7 +368,7 @@ static int parse_slice_header(od_dec *d, br_t *b, int nal_type, int nal_ref_idc,
   h->idr = nal_type == 5;
   h->first_mb = (int)br_ue(b);
   h->slice_type = (int)br_ue(b) % 5;
-  if (h->slice_type != 0 && h->slice_type != 2) return fail(d, OD_EUNSUPPORTED, "slice type %d", h->slice_type);
+  if (h->slice_type > 2) return fail(d, OD_EUNSUPPORTED, "slice type %d", h->slice_type);
   h->pps_id = (int)br_ue(b);
   if (h->pps_id > 255 || !d->pps[h->pps_id].valid) return fail(d, OD_EBITSTREAM, "slice pps");
   const pps_t *q = &d->pps[h->pps_id];
@@ -379,43 +388,56 @@ static int parse_slice_header(od_dec *d, br_t *b, int nal_type, int nal_ref_idc,
     if (q->bottom_field_pic_order) h->delta_poc[1] = br_se(b);
   }
   if (q->redundant_pic_cnt_present && br_ue(b) > 0) return fail(d, OD_EUNSUPPORTED, "redundant pictures");
+  const int bslice = h->slice_type == 1;
+  if (bslice) h->direct_spatial = (int)br_u(b, 1);
   h->num_ref_idx_active = q->num_ref_idx_default[0];
-  if (h->slice_type == 0) {
-    if (br_u(b, 1)) h->num_ref_idx_active = (int)br_ue(b) + 1;
-    if (h->num_ref_idx_active > 32) return fail(d, OD_EBITSTREAM, "num_ref_idx");
-    /* ref_pic_list_modification (7.3.3.1) */
+  h->num_ref_idx_active1 = q->num_ref_idx_default[1];
+  if (h->slice_type != 2) {
     if (br_u(b, 1)) {
-      for (;;) {
-        const uint32_t idc = br_ue(b);
-        if (idc == 3) break;
-        if (idc > 2 || h->nmods >= 32) return fail(d, OD_EBITSTREAM, "ref list modification");
-        h->mod_idc[h->nmods] = (int)idc;
-        h->mod_val[h->nmods] = (int)br_ue(b);
-        h->nmods++;
+      h->num_ref_idx_active = (int)br_ue(b) + 1;
+      if (bslice) h->num_ref_idx_active1 = (int)br_ue(b) + 1;
+    }
+    if (h->num_ref_idx_active > 32 || h->num_ref_idx_active1 > 32) return fail(d, OD_EBITSTREAM, "num_ref_idx");
+    /* ref_pic_list_modification (7.3.3.1): list 0, and list 1 of a B slice */
+    for (int l = 0; l < (bslice ? 2 : 1); l++) {
+      int *n = l ? &h->nmods1 : &h->nmods, *idcs = l ? h->mod_idc1 : h->mod_idc, *vals = l ? h->mod_val1 : h->mod_val;
+      if (br_u(b, 1)) {
+        for (;;) {
+          const uint32_t idc = br_ue(b);
+          if (idc == 3) break;
+          if (idc > 2 || *n >= 32) return fail(d, OD_EBITSTREAM, "ref list modification");
+          idcs[*n] = (int)idc;
+          vals[*n] = (int)br_ue(b);
+          (*n)++;
+        }
       }
     }
-    /* pred_weight_table (7.3.3.2) */
+    /* pred_weight_table (7.3.3.2): P slices with weighted_pred_flag, B slices with weighted_bipred_idc 1 */
     for (int i = 0; i < 32; i++) {
-      h->lw[i] = 1 << 0;
-      h->lo[i] = 0;
+      h->lw[i] = h->lw1[i] = 1;
+      h->lo[i] = h->lo1[i] = 0;
     }
-    if (q->weighted_pred) {
+    if ((q->weighted_pred && !bslice) || (q->weighted_bipred == 1 && bslice)) {
       h->luma_log2wd = (int)br_ue(b);
       h->chroma_log2wd = (int)br_ue(b);
-      for (int i = 0; i < h->num_ref_idx_active; i++) {
-        h->lw[i] = 1 << h->luma_log2wd;
-        h->lo[i] = 0;
-        if (br_u(b, 1)) {
-          h->lw[i] = br_se(b);
-          h->lo[i] = br_se(b);
-        }
-        h->cw[i][0] = h->cw[i][1] = 1 << h->chroma_log2wd;
-        h->co[i][0] = h->co[i][1] = 0;
-        if (br_u(b, 1))
-          for (int c = 0; c < 2; c++) {
-            h->cw[i][c] = br_se(b);
-            h->co[i][c] = br_se(b);
+      for (int l = 0; l < (bslice ? 2 : 1); l++) {
+        int *lw = l ? h->lw1 : h->lw, *lo = l ? h->lo1 : h->lo;
+        int(*cw)[2] = l ? h->cw1 : h->cw, (*co)[2] = l ? h->co1 : h->co;
+        for (int i = 0; i < (l ? h->num_ref_idx_active1 : h->num_ref_idx_active); i++) {
+          lw[i] = 1 << h->luma_log2wd;
+          lo[i] = 0;
+          if (br_u(b, 1)) {
+            lw[i] = br_se(b);
+            lo[i] = br_se(b);
           }
+          cw[i][0] = cw[i][1] = 1 << h->chroma_log2wd;
+          co[i][0] = co[i][1] = 0;
+          if (br_u(b, 1))
+            for (int c = 0; c < 2; c++) {
+              cw[i][c] = br_se(b);
+              co[i][c] = br_se(b);
+            }
+        }
       }
     }
   }
@@ -507,48 +529,17 @@ static int num_refs(const od_dec *d) {
   return n;
 }
 
-/* The list-0 references of a P slice (8.2.4.2.1 and 8.2.4.3): short-term by descending PicNum, then
- * long-term by ascending LongTermPicNum, then the modifications; indexes into d->frames. */
-static int ref_list(od_dec *d, const sps_t *s, const sh_t *h, int *list /* 33 entries */) {
+/* The modifications of one list (8.2.4.3.1 and 8.2.4.3.2) on its initial entries (indexes into
+ * d->frames, -1 past the initial list), then the check that na entries name pictures. */
+static int apply_mods(od_dec *d, const sps_t *s, const sh_t *h, int *list, int na, int nmods, const int *idcs,
+                      const int *vals) {
   const int max_fn = 1 << s->log2_max_frame_num;
-  int st[OD_MAX_FRAMES], lt[OD_MAX_FRAMES], ns = 0, nl = 0;
-  for (int i = 0; i < d->nframes; i++) {
-    frame_t *f = &d->frames[i];
-    if (!f->is_ref) continue;
-    if (f->long_term) {
-      lt[nl++] = i;
-    } else {
-      f->frame_num_wrap = f->frame_num > h->frame_num ? f->frame_num - max_fn : f->frame_num;
-      st[ns++] = i;
-    }
-  }
-  for (int a = 0; a < ns; a++)
-    for (int b2 = a + 1; b2 < ns; b2++)
-      if (d->frames[st[b2]].frame_num_wrap > d->frames[st[a]].frame_num_wrap) {
-        int t = st[a];
-        st[a] = st[b2];
-        st[b2] = t;
-      }
-  for (int a = 0; a < nl; a++)
-    for (int b2 = a + 1; b2 < nl; b2++)
-      if (d->frames[lt[b2]].long_term_idx < d->frames[lt[a]].long_term_idx) {
-        int t = lt[a];
-        lt[a] = lt[b2];
-        lt[b2] = t;
-      }
-  int init[64], n = 0;
-  for (int i = 0; i < ns && n < 64; i++) init[n++] = st[i];
-  for (int i = 0; i < nl && n < 64; i++) init[n++] = lt[i];
-  const int na = h->num_ref_idx_active;
-  if (n == 0) return fail(d, OD_EBITSTREAM, "P slice without references");
-  for (int i = 0; i < na; i++) list[i] = i < n ? init[i] : -1;
-  /* modifications (8.2.4.3.1 and 8.2.4.3.2) */
   int pred = h->frame_num, idx = 0;
-  for (int k = 0; k < h->nmods; k++) {
+  for (int k = 0; k < nmods; k++) {
     int target = -1;
-    if (h->mod_idc[k] < 2) {
-      const int diff = h->mod_val[k] + 1;
-      int no_wrap = h->mod_idc[k] == 0 ? pred - diff : pred + diff;
+    if (idcs[k] < 2) {
+      const int diff = vals[k] + 1;
+      int no_wrap = idcs[k] == 0 ? pred - diff : pred + diff;
       if (no_wrap < 0) no_wrap += max_fn;
       if (no_wrap >= max_fn) no_wrap -= max_fn;
       pred = no_wrap;
@@ -557,7 +548,7 @@ static int ref_list(od_dec *d, const sps_t *s, const sh_t *h, int *list /* 33 en
         if (d->frames[i].is_ref && !d->frames[i].long_term && d->frames[i].frame_num_wrap == pic_num) target = i;
     } else {
       for (int i = 0; i < d->nframes; i++)
-        if (d->frames[i].is_ref && d->frames[i].long_term && d->frames[i].long_term_idx == h->mod_val[k]) target = i;
+        if (d->frames[i].is_ref && d->frames[i].long_term && d->frames[i].long_term_idx == vals[k]) target = i;
     }
     if (target < 0) return fail(d, OD_EBITSTREAM, "modification names no reference");
     for (int i = na; i > idx; i--) list[i] = list[i - 1];
@@ -569,6 +560,97 @@ static int ref_list(od_dec *d, const sps_t *s, const sh_t *h, int *list /* 33 en
   for (int i = 0; i < na; i++)
     if (list[i] < 0) return fail(d, OD_EBITSTREAM, "reference list shorter than num_ref_idx_active");
   return OD_OK;
+}
+
+/* Short-term references (with FrameNumWrap set) and long-term ones, unordered. */
+static void collect_refs(od_dec *d, const sps_t *s, const sh_t *h, int *st, int *ns, int *lt, int *nl) {
+  const int max_fn = 1 << s->log2_max_frame_num;
+  *ns = *nl = 0;
+  for (int i = 0; i < d->nframes; i++) {
+    frame_t *f = &d->frames[i];
+    if (!f->is_ref) continue;
+    if (f->long_term) {
+      lt[(*nl)++] = i;
+    } else {
+      f->frame_num_wrap = f->frame_num > h->frame_num ? f->frame_num - max_fn : f->frame_num;
+      st[(*ns)++] = i;
+    }
+  }
+}
+
+/* Sorts frame indexes by key(frame) ascending (descending when desc). */
+static void sort_frames(od_dec *d, int *v, int n, int key /* 0 FrameNumWrap, 1 LongTermPicNum, 2 order count */,
+                        int desc) {
+  for (int a = 0; a < n; a++)
+    for (int b2 = a + 1; b2 < n; b2++) {
+      const frame_t *x = &d->frames[v[a]], *y = &d->frames[v[b2]];
+      const int kx = key == 0 ? x->frame_num_wrap : key == 1 ? x->long_term_idx : x->poc;
+      const int ky = key == 0 ? y->frame_num_wrap : key == 1 ? y->long_term_idx : y->poc;
+      if (desc ? ky > kx : ky < kx) {
+        const int t = v[a];
+        v[a] = v[b2];
+        v[b2] = t;
+      }
+    }
+}
+
+/* The list-0 references of a P slice (8.2.4.2.1 and 8.2.4.3): short-term by descending PicNum, then
+ * long-term by ascending LongTermPicNum, then the modifications; indexes into d->frames. */
+static int ref_list(od_dec *d, const sps_t *s, const sh_t *h, int *list /* 33 entries */) {
+  int st[OD_MAX_FRAMES], lt[OD_MAX_FRAMES], ns, nl;
+  collect_refs(d, s, h, st, &ns, lt, &nl);
+  sort_frames(d, st, ns, 0, 1);
+  sort_frames(d, lt, nl, 1, 0);
+  int init[64], n = 0;
+  for (int i = 0; i < ns && n < 64; i++) init[n++] = st[i];
+  for (int i = 0; i < nl && n < 64; i++) init[n++] = lt[i];
+  const int na = h->num_ref_idx_active;
+  if (n == 0) return fail(d, OD_EBITSTREAM, "P slice without references");
+  for (int i = 0; i <= na && i < 33; i++) list[i] = i < n && i < na ? init[i] : -1;
+  return apply_mods(d, s, h, list, na, h->nmods, h->mod_idc, h->mod_val);
+}
+
+/* The two lists of a B slice (8.2.4.2.3, frames): list 0 the short-term references shown before the
+ * picture by descending order count, then those after it by ascending count, then the long-term
+ * ones; list 1 the same with before and after exchanged, its first two entries swapped when it has
+ * more than one and equals list 0; then each list's modifications. */
+static int ref_lists_b(od_dec *d, const sps_t *s, const sh_t *h, int *list0, int *list1) {
+  int st[OD_MAX_FRAMES], lt[OD_MAX_FRAMES], ns, nl;
+  collect_refs(d, s, h, st, &ns, lt, &nl);
+  sort_frames(d, lt, nl, 1, 0);
+  int before[OD_MAX_FRAMES], after[OD_MAX_FRAMES], nb = 0, na_ = 0;
+  for (int i = 0; i < ns; i++) {
+    if (d->frames[st[i]].poc <= d->cur_poc) before[nb++] = st[i];
+    else after[na_++] = st[i];
+  }
+  sort_frames(d, before, nb, 2, 1);
+  sort_frames(d, after, na_, 2, 0);
+  int init[2][64], n[2] = {0, 0};
+  for (int l = 0; l < 2; l++) {
+    const int *first = l ? after : before, *second = l ? before : after;
+    const int nf = l ? na_ : nb, nsd = l ? nb : na_;
+    for (int i = 0; i < nf && n[l] < 64; i++) init[l][n[l]++] = first[i];
+    for (int i = 0; i < nsd && n[l] < 64; i++) init[l][n[l]++] = second[i];
+    for (int i = 0; i < nl && n[l] < 64; i++) init[l][n[l]++] = lt[i];
+  }
+  if (n[0] == 0) return fail(d, OD_EBITSTREAM, "B slice without references");
+  if (n[1] > 1) {
+    int same = n[0] == n[1];
+    for (int i = 0; same && i < n[1]; i++) same = init[0][i] == init[1][i];
+    if (same) {
+      const int t = init[1][0];
+      init[1][0] = init[1][1];
+      init[1][1] = t;
+    }
+  }
+  const int nact[2] = {h->num_ref_idx_active, h->num_ref_idx_active1};
+  for (int l = 0; l < 2; l++) {
+    int *list = l ? list1 : list0;
+    for (int i = 0; i <= nact[l] && i < 33; i++) list[i] = i < n[l] && i < nact[l] ? init[l][i] : -1;
+  }
+  int rc = apply_mods(d, s, h, list0, nact[0], h->nmods, h->mod_idc, h->mod_val);
+  if (!rc) rc = apply_mods(d, s, h, list1, nact[1], h->nmods1, h->mod_idc1, h->mod_val1);
+  return rc;
 }
 
 /* Marking after the picture (8.2.5): IDR, sliding window, or the adaptive operations. */
@@ -719,6 +801,8 @@ typedef struct {
   cabac_t c;
   int slice_id, mb_w, mb_h, mbx, mby, addr, qp, prev_qp_delta_nz, prev_addr;
   int refs[33]; /* frame indexes of list 0 (one spare entry for the modification shifts) */
+  int refs1[33]; /* list 1 of a B slice */
+  int bslice;
   dmb_t *cur;
   /* the macroblock being decoded */
   int32_t coef[16][16];  /* luma 4x4 blocks, raster block order, raster positions (dequantised later) */
@@ -727,7 +811,8 @@ typedef struct {
   int32_t cdc[2][4];
   int32_t cac[2][4][16];
   int i16_mode;
-  uint32_t done; /* 4x4 blocks whose motion is known, for neighbour availability inside the macroblock */
+  uint32_t done;  /* 4x4 blocks whose list 0 motion is known, for neighbour availability inside the macroblock */
+  uint32_t done1; /* the same for list 1 */
 } sdec_t;
 
 static dmb_t *mb_at(sdec_t *t, int mbx, int mby) {
@@ -751,7 +836,8 @@ static dmb_t *nb4(sdec_t *t, int x, int y, int *bx, int *by, int inside_done) {
   *bx = x;
   *by = y;
   if (dx == 0 && dy == 0) {
-    if (inside_done && !((t->done >> (y * 4 + x)) & 1)) return NULL;
+    /* inside_done: 1 blocks of known list 0 motion, 2 of known list 1 motion */
+    if (inside_done && !(((inside_done == 2 ? t->done1 : t->done) >> (y * 4 + x)) & 1)) return NULL;
     return t->cur;
   }
   return mb_at(t, t->mbx + dx, t->mby + dy);
@@ -761,23 +847,23 @@ static dmb_t *nb4(sdec_t *t, int x, int y, int *bx, int *by, int inside_done) {
 
 static int read_mb_skip(sdec_t *t) {
   const dmb_t *a = left_mb(t), *b = top_mb(t);
-  return decision(&t->c, 11 + (a && !a->skip) + (b && !b->skip));
+  return decision(&t->c, (t->bslice ? 24 : 11) + (a && !a->skip) + (b && !b->skip));
 }
 
 /* I-slice mb_type bins after the first (9.3.2.5, Table 9-36), ctxIdx base 3 (I slice) or 17 (prefix in
  * P slices); returns 0..25 as the I mb_type */
-static int read_i_mb_type(sdec_t *t, int islice) {
+static int read_i_mb_type_at(sdec_t *t, int islice, int prefix_base) {
   int first;
   if (islice) {
     const dmb_t *a = left_mb(t), *b = top_mb(t);
     const int ca = a && a->kind != K_I4 && a->kind != K_I8, cb = b && b->kind != K_I4 && b->kind != K_I8;
     first = decision(&t->c, 3 + ca + cb);
   } else {
-    first = decision(&t->c, 17);
+    first = decision(&t->c, prefix_base);
   }
   if (!first) return 0;
   if (terminate(&t->c)) return 25;
-  const int base = islice ? 3 : 17;
+  const int base = islice ? 3 : prefix_base;
   const int b_cbpl = islice ? base + 3 : base + 1;
   int mt = 1;
   const int luma = decision(&t->c, b_cbpl);
@@ -789,6 +875,8 @@ static int read_i_mb_type(sdec_t *t, int islice) {
   mt += (m1 << 1 | m0) + 4 * cc + 12 * luma;
   return mt;
 }
+
+static int read_i_mb_type(sdec_t *t, int islice) { return read_i_mb_type_at(t, islice, 17); }
 
 /* P mb_type (Table 9-37): 0..3 the P types (16x16, 16x8, 8x16, 8x8), 5 + I mb_type for intra */
 static int read_p_mb_type(sdec_t *t) {
@@ -982,17 +1070,17 @@ static int cbf_chroma_ac(sdec_t *t, int comp, int bx, int by, int intra) {
 
 /* -- motion vector prediction (8.4.1.3, h264/mb.rs nb_motion / pred_mv) -- */
 
-static int nb_motion(sdec_t *t, int x, int y, int *ref, int16_t *mv) {
+static int nb_motion(sdec_t *t, int l, int x, int y, int *ref, int16_t *mv) {
   int nx, ny;
-  const dmb_t *n = nb4(t, x, y, &nx, &ny, 1);
+  const dmb_t *n = nb4(t, x, y, &nx, &ny, 1 + l);
   if (!n) return 0;
   if (n->kind != K_INTER) {
     *ref = -1;
     mv[0] = mv[1] = 0;
   } else {
-    *ref = n->ref[(ny >> 1) * 2 + (nx >> 1)];
-    mv[0] = n->mv[ny * 4 + nx][0];
-    mv[1] = n->mv[ny * 4 + nx][1];
+    *ref = (l ? n->ref1 : n->ref)[(ny >> 1) * 2 + (nx >> 1)];
+    mv[0] = (l ? n->mv1 : n->mv)[ny * 4 + nx][0];
+    mv[1] = (l ? n->mv1 : n->mv)[ny * 4 + nx][1];
   }
   return 1;
 }
@@ -1003,13 +1091,13 @@ static int16_t median3(int16_t a, int16_t b, int16_t c) {
   return mx < u ? mx : u;
 }
 
-static void pred_mv(sdec_t *t, int bx, int by, int w, int dir, int rf, int16_t *out) {
+static void pred_mv(sdec_t *t, int l, int bx, int by, int w, int dir, int rf, int16_t *out) {
   int ra = -1, rb = -1, rc = -1;
   int16_t ma[2] = {0, 0}, mb[2] = {0, 0}, mc[2] = {0, 0};
-  const int ha = nb_motion(t, bx - 1, by, &ra, ma);
-  const int hb = nb_motion(t, bx, by - 1, &rb, mb);
-  int hc = nb_motion(t, bx + w, by - 1, &rc, mc);
-  if (!hc) hc = nb_motion(t, bx - 1, by - 1, &rc, mc);
+  const int ha = nb_motion(t, l, bx - 1, by, &ra, ma);
+  const int hb = nb_motion(t, l, bx, by - 1, &rb, mb);
+  int hc = nb_motion(t, l, bx + w, by - 1, &rc, mc);
+  if (!hc) hc = nb_motion(t, l, bx - 1, by - 1, &rc, mc);
   if (!ha) { ra = -1; ma[0] = ma[1] = 0; }
   if (!hb) { rb = -1; mb[0] = mb[1] = 0; }
   if (!hc) { rc = -1; mc[0] = mc[1] = 0; }
@@ -1253,6 +1341,482 @@ static void mc_part(sdec_t *t, int bx, int by, int w, int h, int ri, const int16
   }
 }
 
+/* -- B slices (h264/mb.rs decode_inter_mb, direct_spatial/direct_temporal, mc_block; 7.3.5, 8.4.1.2,
+ *    8.4.2.3) -- */
+
+/* B mb_type (9.3.2.5, Table 9-37, contexts 27-35): 0..22 the B types, 23 + I mb_type for intra */
+static int read_b_mb_type(sdec_t *t) {
+  const dmb_t *a = left_mb(t), *b = top_mb(t);
+  const int ca = a && !a->bdirect, cb = b && !b->bdirect;
+  if (!decision(&t->c, 27 + ca + cb)) return 0;
+  if (!decision(&t->c, 30)) return 1 + decision(&t->c, 32);
+  int bits = decision(&t->c, 31) << 3;
+  bits |= decision(&t->c, 32) << 2;
+  bits |= decision(&t->c, 32) << 1;
+  bits |= decision(&t->c, 32);
+  if (bits <= 7) return bits + 3;
+  if (bits == 13) return 23 + read_i_mb_type_at(t, 0, 32);
+  if (bits == 14) return 11;
+  if (bits == 15) return 22;
+  return ((bits << 1) | decision(&t->c, 32)) - 4;
+}
+
+/* B sub_mb_type (Table 9-38, contexts 36-39): 0..12 */
+static int read_b_sub_type(sdec_t *t) {
+  if (!decision(&t->c, 36)) return 0;
+  if (!decision(&t->c, 37)) return 1 + decision(&t->c, 39);
+  int v = 3;
+  if (decision(&t->c, 38)) {
+    if (decision(&t->c, 39)) return 11 + decision(&t->c, 39);
+    v += 4;
+  }
+  v += 2 * decision(&t->c, 39);
+  v += decision(&t->c, 39);
+  return v;
+}
+
+/* ref_idx_lX (9.3.3.1.1.6): a neighbouring partition counts when it is inter, not direct, and its
+ * index in the list is above 0 */
+static int read_ref_idx_l(sdec_t *t, int l, int bx, int by, int max) {
+  int cond[2];
+  for (int k = 0; k < 2; k++) {
+    int nx, ny;
+    const dmb_t *n = nb4(t, bx - (k == 0), by - (k == 1), &nx, &ny, 0);
+    const int b8 = (ny >> 1) * 2 + (nx >> 1);
+    cond[k] = n && n->kind == K_INTER && !((n->direct8 >> b8) & 1) && (l ? n->ref1 : n->ref)[b8] > 0;
+  }
+  int r = 0, ctx = 54 + cond[0] + 2 * cond[1];
+  while (decision(&t->c, ctx)) {
+    r++;
+    ctx = r == 1 ? 58 : 59;
+    if (r > max) return -1;
+  }
+  return r;
+}
+
+static int read_mvd_l(sdec_t *t, int l, int bx, int by, int comp) {
+  int nx, ny, sum = 0;
+  const dmb_t *a = nb4(t, bx - 1, by, &nx, &ny, 0);
+  if (a) sum += (l ? a->mvd1 : a->mvd)[ny * 4 + nx][comp];
+  const dmb_t *b = nb4(t, bx, by - 1, &nx, &ny, 0);
+  if (b) sum += (l ? b->mvd1 : b->mvd)[ny * 4 + nx][comp];
+  const int base = comp ? 47 : 40;
+  const int inc = sum < 3 ? 0 : (sum > 32 ? 2 : 1);
+  if (!decision(&t->c, base + inc)) return 0;
+  int v = 1, ctx = base + 3;
+  while (v < 9 && decision(&t->c, ctx)) {
+    v++;
+    if (ctx < base + 6) ctx++;
+  }
+  if (v >= 9) v += (int)exp_golomb_bypass(&t->c, 3);
+  return bypass(&t->c) ? -v : v;
+}
+
+static void set_motion_l(sdec_t *t, int l, int bx, int by, int w, int h, const int16_t *mv) {
+  for (int y = by; y < by + h; y++)
+    for (int x = bx; x < bx + w; x++) {
+      int16_t *d = (l ? t->cur->mv1 : t->cur->mv)[y * 4 + x];
+      d[0] = mv[0];
+      d[1] = mv[1];
+      if (l) t->done1 |= 1u << (y * 4 + x);
+      else t->done |= 1u << (y * 4 + x);
+    }
+}
+
+/* The motion of the co-located 4x4 block (bx, by) in list 1's first picture (h264/mb.rs col_block):
+ * its first used list's index, vector and picture; 0 when it is intra. */
+static int col_block(sdec_t *t, int bx, int by, int *ref, int16_t *mv, int *id) {
+  const frame_t *f = &t->d->frames[t->refs1[0]];
+  const ia_mbinfo *m = &f->mbs[t->addr];
+  const int cx = t->s->direct8x8 ? (bx >> 1) * 3 : bx, cy = t->s->direct8x8 ? (by >> 1) * 3 : by;
+  if (m->kind != K_INTER) return 0;
+  const int b8 = (cy >> 1) * 2 + (cx >> 1);
+  if (m->ref_idx[b8] >= 0) {
+    *ref = m->ref_idx[b8];
+    mv[0] = m->mv[cy * 4 + cx][0];
+    mv[1] = m->mv[cy * 4 + cx][1];
+    *id = m->ref_id[b8];
+    return 1;
+  }
+  if (m->ref_idx1[b8] >= 0) {
+    *ref = m->ref_idx1[b8];
+    mv[0] = m->mv1[cy * 4 + cx][0];
+    mv[1] = m->mv1[cy * 4 + cx][1];
+    *id = m->ref_id1[b8];
+    return 1;
+  }
+  return 0;
+}
+
+typedef struct {
+  int ref[2][4];
+  int16_t mv[2][16][2];
+} direct_t;
+
+static int min_positive(int x, int y) { return x >= 0 && y >= 0 ? mini(x, y) : (x > y ? x : y); }
+
+/* Spatial direct prediction (8.4.1.2.2) */
+static int direct_spatial(sdec_t *t, direct_t *d) {
+  int refs[2];
+  for (int l = 0; l < 2; l++) {
+    int ra = -1, rb = -1, rc = -1, r;
+    int16_t m[2];
+    if (nb_motion(t, l, -1, 0, &r, m)) ra = r;
+    if (nb_motion(t, l, 0, -1, &r, m)) rb = r;
+    if (nb_motion(t, l, 4, -1, &r, m)) rc = r;
+    else if (nb_motion(t, l, -1, -1, &r, m)) rc = r;
+    refs[l] = min_positive(ra, min_positive(rb, rc));
+  }
+  memset(d, 0, sizeof *d);
+  for (int l = 0; l < 2; l++)
+    for (int k = 0; k < 4; k++) d->ref[l][k] = -1;
+  if (refs[0] < 0 && refs[1] < 0) {
+    for (int l = 0; l < 2; l++)
+      for (int k = 0; k < 4; k++) d->ref[l][k] = 0;
+    return OD_OK;
+  }
+  for (int l = 0; l < 2; l++)
+    if (refs[l] >= 0) {
+      int16_t mv[2];
+      pred_mv(t, l, 0, 0, 4, 0, refs[l], mv);
+      for (int k = 0; k < 4; k++) d->ref[l][k] = refs[l];
+      for (int b = 0; b < 16; b++) { d->mv[l][b][0] = mv[0]; d->mv[l][b][1] = mv[1]; }
+    }
+  if (!t->d->frames[t->refs1[0]].long_term)
+    for (int b = 0; b < 16; b++) {
+      int r, id;
+      int16_t mv[2];
+      if (col_block(t, b % 4, b / 4, &r, mv, &id) && r == 0 && absi(mv[0]) <= 1 && absi(mv[1]) <= 1)
+        for (int l = 0; l < 2; l++)
+          if (refs[l] == 0) d->mv[l][b][0] = d->mv[l][b][1] = 0;
+    }
+  return OD_OK;
+}
+
+/* Temporal direct prediction (8.4.1.2.3) */
+static int direct_temporal(sdec_t *t, direct_t *d) {
+  const frame_t *e1 = &t->d->frames[t->refs1[0]];
+  memset(d, 0, sizeof *d);
+  for (int i8 = 0; i8 < 4; i8++) {
+    const int bx0 = (i8 & 1) * 2, by0 = (i8 >> 1) * 2;
+    int r, id, ref0 = 0;
+    int16_t mv[2];
+    if (col_block(t, bx0, by0, &r, mv, &id)) {
+      ref0 = -1;
+      for (int i = 0; i < t->h->num_ref_idx_active && ref0 < 0; i++)
+        if (t->d->frames[t->refs[i]].id == id) ref0 = i;
+      if (ref0 < 0) return fail(t->d, OD_EBITSTREAM, "temporal direct names a picture not in list 0");
+    }
+    const frame_t *e0 = &t->d->frames[t->refs[ref0]];
+    d->ref[0][i8] = ref0;
+    d->ref[1][i8] = 0;
+    const int tb = clampi(t->d->cur_poc - e0->poc, -128, 127), td = clampi(e1->poc - e0->poc, -128, 127);
+    for (int j = 0; j < 4; j++) {
+      const int bx = bx0 + (j & 1), by = by0 + (j >> 1);
+      int16_t col[2] = {0, 0};
+      if (!col_block(t, bx, by, &r, col, &id)) col[0] = col[1] = 0;
+      int16_t m0[2], m1[2];
+      if (e0->long_term || td == 0) {
+        m0[0] = col[0];
+        m0[1] = col[1];
+        m1[0] = m1[1] = 0;
+      } else {
+        const int tx = (16384 + absi(td / 2)) / td;
+        const int dsf = clampi((tb * tx + 32) >> 6, -1024, 1023);
+        for (int c = 0; c < 2; c++) {
+          m0[c] = (int16_t)((dsf * col[c] + 128) >> 8);
+          m1[c] = (int16_t)(m0[c] - col[c]);
+        }
+      }
+      const int b = by * 4 + bx;
+      d->mv[0][b][0] = m0[0];
+      d->mv[0][b][1] = m0[1];
+      d->mv[1][b][0] = m1[0];
+      d->mv[1][b][1] = m1[1];
+    }
+  }
+  return OD_OK;
+}
+
+/* The direct motion of the 8x8 blocks in `mask` into the macroblock (h264/mb.rs put_direct) */
+static int put_direct(sdec_t *t, int mask) {
+  direct_t d;
+  const int rc = t->h->direct_spatial ? direct_spatial(t, &d) : direct_temporal(t, &d);
+  if (rc) return rc;
+  if (!t->h->direct_spatial) t->d->stats[OD_STAT_B_TEMPORAL]++;
+  dmb_t *m = t->cur;
+  for (int i8 = 0; i8 < 4; i8++) {
+    if (!((mask >> i8) & 1)) continue;
+    m->ref[i8] = (int8_t)d.ref[0][i8];
+    m->ref1[i8] = (int8_t)d.ref[1][i8];
+    for (int j = 0; j < 4; j++) {
+      const int b = ((i8 >> 1) * 2 + (j >> 1)) * 4 + (i8 & 1) * 2 + (j & 1);
+      for (int c = 0; c < 2; c++) {
+        m->mv[b][c] = d.ref[0][i8] >= 0 ? d.mv[0][b][c] : 0;
+        m->mv1[b][c] = d.ref[1][i8] >= 0 ? d.mv[1][b][c] : 0;
+      }
+    }
+  }
+  return OD_OK;
+}
+
+static int clip255(int v) { return v < 0 ? 0 : v > 255 ? 255 : v; }
+
+/* The prediction of 4x4 block (bx, by) of a B macroblock from the lists its 8x8 block uses: averaged,
+ * or weighted implicitly (weighted_bipred_idc 2) or explicitly (idc 1) (8.4.2.3; h264/mb.rs mc_block) */
+static int mc_block_b(sdec_t *t, int bx, int by) {
+  const dmb_t *m = t->cur;
+  const int b8 = (by >> 1) * 2 + (bx >> 1), b = by * 4 + bx;
+  const int x = t->mbx * 16 + bx * 4, y = t->mby * 16 + by * 4, pw = t->mb_w * 16, ph = t->mb_h * 16;
+  const int cx = t->mbx * 8 + bx * 2, cy = t->mby * 8 + by * 2;
+  uint16_t lp[2][16], cp[2][2][4];
+  int have[2] = {0, 0}, poc[2] = {0, 0}, lt[2] = {0, 0}, ri[2] = {0, 0};
+  for (int l = 0; l < 2; l++) {
+    const int r = (l ? m->ref1 : m->ref)[b8];
+    if (r < 0) continue;
+    const int na = l ? t->h->num_ref_idx_active1 : t->h->num_ref_idx_active;
+    if (r >= na) return fail(t->d, OD_EBITSTREAM, "reference index past its list");
+    const frame_t *f = &t->d->frames[(l ? t->refs1 : t->refs)[r]];
+    const int16_t *mv = (l ? m->mv1 : m->mv)[b];
+    or_luma_mc_ref(f->y16, pw, pw, ph, x, y, mv[0], mv[1], 4, 4, lp[l], 255);
+    for (int c = 0; c < 2; c++) or_chroma_mc(c ? f->cr16 : f->cb16, pw / 2, ph / 2, cx, cy, mv[0], mv[1], 2, 2, cp[l][c]);
+    have[l] = 1;
+    poc[l] = f->poc;
+    lt[l] = f->long_term;
+    ri[l] = r;
+  }
+  if (!have[0] && !have[1]) return fail(t->d, OD_EBITSTREAM, "inter block without a reference");
+  const sh_t *h = t->h;
+  const int explicit_w = t->q->weighted_bipred == 1;
+  uint16_t outl[16], outc[2][4];
+  if (have[0] && have[1]) {
+    /* (log2 denominator, w0, w1, offset) for luma and each chroma plane */
+    int wt[3][4] = {{0}};
+    int weighted = 0;
+    if (explicit_w) {
+      weighted = 1;
+      wt[0][0] = h->luma_log2wd;
+      wt[0][1] = h->lw[ri[0]];
+      wt[0][2] = h->lw1[ri[1]];
+      wt[0][3] = (h->lo[ri[0]] + h->lo1[ri[1]] + 1) >> 1;
+      for (int c = 0; c < 2; c++) {
+        wt[1 + c][0] = h->chroma_log2wd;
+        wt[1 + c][1] = h->cw[ri[0]][c];
+        wt[1 + c][2] = h->cw1[ri[1]][c];
+        wt[1 + c][3] = (h->co[ri[0]][c] + h->co1[ri[1]][c] + 1) >> 1;
+      }
+      t->d->stats[OD_STAT_B_EXPLICIT]++;
+    } else if (t->q->weighted_bipred == 2) {
+      weighted = 1;
+      int w0 = 32, w1 = 32;
+      const int tb = clampi(t->d->cur_poc - poc[0], -128, 127), td = clampi(poc[1] - poc[0], -128, 127);
+      if (td != 0 && !lt[0] && !lt[1]) {
+        const int tx = (16384 + absi(td / 2)) / td;
+        const int dsf = clampi((tb * tx + 32) >> 6, -1024, 1023);
+        if ((dsf >> 2) >= -64 && (dsf >> 2) <= 128) {
+          w1 = dsf >> 2;
+          w0 = 64 - w1;
+        }
+      }
+      for (int k = 0; k < 3; k++) {
+        wt[k][0] = 5;
+        wt[k][1] = w0;
+        wt[k][2] = w1;
+        wt[k][3] = 0;
+      }
+      t->d->stats[OD_STAT_B_IMPLICIT]++;
+    }
+    for (int i = 0; i < 16; i++)
+      outl[i] = weighted ? (uint16_t)clip255(((lp[0][i] * wt[0][1] + lp[1][i] * wt[0][2] + (1 << wt[0][0])) >> (wt[0][0] + 1)) + wt[0][3])
+                         : (uint16_t)((lp[0][i] + lp[1][i] + 1) >> 1);
+    for (int c = 0; c < 2; c++)
+      for (int i = 0; i < 4; i++)
+        outc[c][i] = weighted ? (uint16_t)clip255(((cp[0][c][i] * wt[1 + c][1] + cp[1][c][i] * wt[1 + c][2] + (1 << wt[1 + c][0])) >> (wt[1 + c][0] + 1)) + wt[1 + c][3])
+                              : (uint16_t)((cp[0][c][i] + cp[1][c][i] + 1) >> 1);
+  } else {
+    const int l = have[0] ? 0 : 1, r = ri[l];
+    for (int i = 0; i < 16; i++) {
+      int v = lp[l][i];
+      if (explicit_w) {
+        const int lwd = h->luma_log2wd, w = (l ? h->lw1 : h->lw)[r], o = (l ? h->lo1 : h->lo)[r];
+        v = clip255(lwd >= 1 ? ((v * w + (1 << (lwd - 1))) >> lwd) + o : v * w + o);
+      }
+      outl[i] = (uint16_t)v;
+    }
+    for (int c = 0; c < 2; c++)
+      for (int i = 0; i < 4; i++) {
+        int v = cp[l][c][i];
+        if (explicit_w) {
+          const int cwd = h->chroma_log2wd, w = (l ? h->cw1 : h->cw)[r][c], o = (l ? h->co1 : h->co)[r][c];
+          v = clip255(cwd >= 1 ? ((v * w + (1 << (cwd - 1))) >> cwd) + o : v * w + o);
+        }
+        outc[c][i] = (uint16_t)v;
+      }
+  }
+  for (int r = 0; r < 4; r++)
+    for (int c = 0; c < 4; c++) *ypix(t, x + c, y + r) = outl[r * 4 + c];
+  for (int c = 0; c < 2; c++)
+    for (int r = 0; r < 2; r++)
+      for (int k = 0; k < 2; k++) *cpix(t, c, cx + k, cy + r) = outc[c][r * 2 + k];
+  return OD_OK;
+}
+
+static int mc_mb_b(sdec_t *t) {
+  for (int b = 0; b < 16; b++) {
+    const int rc = mc_block_b(t, b % 4, b / 4);
+    if (rc) return rc;
+  }
+  return OD_OK;
+}
+
+static void reset_motion(dmb_t *m) {
+  for (int i = 0; i < 4; i++) m->ref[i] = m->ref1[i] = -1;
+  memset(m->mv, 0, sizeof m->mv);
+  memset(m->mv1, 0, sizeof m->mv1);
+  memset(m->mvd, 0, sizeof m->mvd);
+  memset(m->mvd1, 0, sizeof m->mvd1);
+}
+
+/* B_Skip: direct prediction of the whole macroblock, no residual */
+static int decode_b_skip(sdec_t *t) {
+  dmb_t *m = t->cur;
+  m->kind = K_INTER;
+  m->skip = 1;
+  m->cbp = 0;
+  reset_motion(m);
+  m->bdirect = 1;
+  m->direct8 = 15;
+  t->d->stats[OD_STAT_B_SKIP]++;
+  int rc = put_direct(t, 15);
+  if (!rc) rc = mc_mb_b(t);
+  t->prev_qp_delta_nz = 0;
+  return rc;
+}
+
+static int read_residual(sdec_t *t, int intra);
+static void add_luma_residual(sdec_t *t, int intra16);
+static void add_chroma_residual(sdec_t *t);
+
+/* An inter macroblock of a B slice, mb_type bt in 0..22 (Table 7-14) */
+static int decode_b_inter(sdec_t *t, int bt) {
+  static const int SUBM[13][2] = {{0, 0}, {1, 0}, {2, 0}, {3, 0}, {1, 1}, {1, 2}, {2, 1}, {2, 2}, {3, 1}, {3, 2},
+                                  {1, 3}, {2, 3}, {3, 3}}; /* (list mask, shape 0 8x8 1 8x4 2 4x8 3 4x4) */
+  static const uint8_t PAIRS[9][2] = {{1, 1}, {2, 2}, {1, 2}, {2, 1}, {1, 3}, {2, 3}, {3, 1}, {3, 2}, {3, 3}};
+  dmb_t *m = t->cur;
+  m->kind = K_INTER;
+  m->chroma_mode = 0;
+  reset_motion(m);
+  int rp[4][5], nrp = 0;            /* (bx, by, w, h, mask) */
+  int mp[16][7], nmp = 0;           /* (bx, by, w, h, mask, dir, direct) */
+  int direct_mask = 0, sub8_ok = 1;
+#define RP(x, y, w, h, k) do { rp[nrp][0] = x; rp[nrp][1] = y; rp[nrp][2] = w; rp[nrp][3] = h; rp[nrp][4] = k; nrp++; } while (0)
+#define MP(x, y, w, h, k, d, dr) do { mp[nmp][0] = x; mp[nmp][1] = y; mp[nmp][2] = w; mp[nmp][3] = h; mp[nmp][4] = k; mp[nmp][5] = d; mp[nmp][6] = dr; nmp++; } while (0)
+  if (bt == 0) {
+    direct_mask = 15;
+    m->bdirect = 1;
+    t->d->stats[OD_STAT_B_DIRECT16]++;
+  } else if (bt <= 3) {
+    RP(0, 0, 4, 4, bt);
+    MP(0, 0, 4, 4, bt, 0, 0);
+    t->d->stats[OD_STAT_B16 + bt - 1]++;
+  } else if (bt < 22) {
+    const int idx = bt - 4, m0 = PAIRS[idx / 2][0], m1 = PAIRS[idx / 2][1];
+    if (idx % 2 == 0) {
+      RP(0, 0, 4, 2, m0); RP(0, 2, 4, 2, m1);
+      MP(0, 0, 4, 2, m0, 1, 0); MP(0, 2, 4, 2, m1, 1, 0);
+    } else {
+      RP(0, 0, 2, 4, m0); RP(2, 0, 2, 4, m1);
+      MP(0, 0, 2, 4, m0, 2, 0); MP(2, 0, 2, 4, m1, 2, 0);
+    }
+    t->d->stats[OD_STAT_B_PART + idx % 2]++;
+  } else {
+    int sub[4];
+    for (int k = 0; k < 4; k++) {
+      sub[k] = read_b_sub_type(t);
+      t->d->stats[OD_STAT_B_SUB + sub[k]]++;
+    }
+    t->d->stats[OD_STAT_B8X8]++;
+    for (int i = 0; i < 4; i++) {
+      const int x0 = (i & 1) * 2, y0 = (i >> 1) * 2;
+      if (sub[i] == 0) {
+        direct_mask |= 1 << i;
+        MP(x0, y0, 2, 2, 0, 0, 1);
+        continue;
+      }
+      const int mask = SUBM[sub[i]][0], shape = SUBM[sub[i]][1];
+      sub8_ok &= shape == 0;
+      RP(x0, y0, 2, 2, mask);
+      static const int sp[4][4][4] = {{{0, 0, 2, 2}}, {{0, 0, 2, 1}, {0, 1, 2, 1}}, {{0, 0, 1, 2}, {1, 0, 1, 2}},
+                                      {{0, 0, 1, 1}, {1, 0, 1, 1}, {0, 1, 1, 1}, {1, 1, 1, 1}}};
+      static const int nsp[4] = {1, 2, 2, 4};
+      for (int q = 0; q < nsp[shape]; q++) MP(x0 + sp[shape][q][0], y0 + sp[shape][q][1], sp[shape][q][2], sp[shape][q][3], mask, 0, 0);
+    }
+  }
+#undef RP
+#undef MP
+  /* direct motion first: later partitions may predict from it */
+  m->direct8 = (uint8_t)direct_mask;
+  if (direct_mask) {
+    const int rc = put_direct(t, direct_mask);
+    if (rc) return rc;
+  }
+  const int maxr[2] = {t->h->num_ref_idx_active - 1, t->h->num_ref_idx_active1 - 1};
+  for (int l = 0; l < 2; l++)
+    for (int k = 0; k < nrp; k++) {
+      if (!((rp[k][4] >> l) & 1)) continue;
+      const int r = maxr[l] > 0 ? read_ref_idx_l(t, l, rp[k][0], rp[k][1], maxr[l]) : 0;
+      if (r < 0) return fail(t->d, OD_EBITSTREAM, "ref_idx past its list");
+      for (int yy = rp[k][1]; yy < rp[k][1] + rp[k][3]; yy += 2)
+        for (int xx = rp[k][0]; xx < rp[k][0] + rp[k][2]; xx += 2) (l ? m->ref1 : m->ref)[(yy >> 1) * 2 + (xx >> 1)] = (int8_t)r;
+    }
+  for (int l = 0; l < 2; l++) {
+    if (l) t->done1 = 0;
+    else t->done = 0;
+    for (int k = 0; k < nmp; k++) {
+      const int bx = mp[k][0], by = mp[k][1], w = mp[k][2], h = mp[k][3];
+      if (mp[k][6]) {
+        for (int yy = by; yy < by + h; yy++)
+          for (int xx = bx; xx < bx + w; xx++) {
+            if (l) t->done1 |= 1u << (yy * 4 + xx);
+            else t->done |= 1u << (yy * 4 + xx);
+          }
+      } else if ((mp[k][4] >> l) & 1) {
+        const int refi = (l ? m->ref1 : m->ref)[(by >> 1) * 2 + (bx >> 1)];
+        int16_t p[2], mv[2];
+        pred_mv(t, l, bx, by, w, mp[k][5], refi, p);
+        const int d0 = read_mvd_l(t, l, bx, by, 0), d1 = read_mvd_l(t, l, bx, by, 1);
+        for (int yy = by; yy < by + h; yy++)
+          for (int xx = bx; xx < bx + w; xx++) {
+            uint8_t *md = (l ? m->mvd1 : m->mvd)[yy * 4 + xx];
+            md[0] = (uint8_t)mini(absi(d0), 70);
+            md[1] = (uint8_t)mini(absi(d1), 70);
+          }
+        mv[0] = (int16_t)(p[0] + d0);
+        mv[1] = (int16_t)(p[1] + d1);
+        set_motion_l(t, l, bx, by, w, h, mv);
+      } else {
+        const int16_t z[2] = {0, 0};
+        set_motion_l(t, l, bx, by, w, h, z);
+      }
+    }
+  }
+  m->cbp = (uint8_t)read_cbp(t);
+  const int direct_ok = direct_mask == 0 || t->s->direct8x8;
+  if ((m->cbp & 15) && t->q->transform8 && sub8_ok && direct_ok) m->t8 = (uint8_t)read_t8_flag(t);
+  if (m->t8) t->d->stats[OD_STAT_INTER_T8]++;
+  int qd = 0;
+  if (m->cbp) qd = read_qp_delta(t);
+  t->prev_qp_delta_nz = qd != 0;
+  t->qp = (t->qp + qd + 52) % 52;
+  m->qp = t->qp;
+  read_residual(t, 0);
+  const int rc = mc_mb_b(t);
+  if (rc) return rc;
+  add_luma_residual(t, 0);
+  add_chroma_residual(t);
+  return OD_OK;
+}
+
 /* -- macroblock layer -- */
 
 static int read_residual(sdec_t *t, int intra) {
@@ -1350,9 +1914,9 @@ static int decode_skip(sdec_t *t) {
   int16_t mv[2] = {0, 0};
   int ra, rb;
   int16_t ma[2], mb[2];
-  const int ha = nb_motion(t, -1, 0, &ra, ma), hb = nb_motion(t, 0, -1, &rb, mb);
+  const int ha = nb_motion(t, 0, -1, 0, &ra, ma), hb = nb_motion(t, 0, 0, -1, &rb, mb);
   if (!(!ha || !hb || (ra == 0 && ma[0] == 0 && ma[1] == 0) || (rb == 0 && mb[0] == 0 && mb[1] == 0)))
-    pred_mv(t, 0, 0, 4, 0, 0, mv);
+    pred_mv(t, 0, 0, 0, 4, 0, 0, mv);
   const int z[2] = {0, 0};
   set_motion(t, 0, 0, 4, 4, mv, z);
   memset(m->mvd, 0, sizeof m->mvd);
@@ -1363,7 +1927,13 @@ static int decode_skip(sdec_t *t) {
 
 static int decode_mb(sdec_t *t, int islice) {
   dmb_t *m = t->cur;
-  int mt = islice ? read_i_mb_type(t, 1) : read_p_mb_type(t);
+  int mt;
+  if (islice) mt = read_i_mb_type(t, 1);
+  else if (t->bslice) {
+    const int bt = read_b_mb_type(t);
+    if (bt < 23) return decode_b_inter(t, bt);
+    mt = 5 + bt - 23;
+  } else mt = read_p_mb_type(t);
   int intra = islice || mt >= 5;
   if (!islice && mt >= 5) mt -= 5;
   else if (!islice) {
@@ -1373,9 +1943,7 @@ static int decode_mb(sdec_t *t, int islice) {
   if (intra && mt == 25) return decode_pcm(t);
   m->t8 = 0;
   if (intra) {
-    for (int i = 0; i < 4; i++) m->ref[i] = -1;
-    memset(m->mv, 0, sizeof m->mv);
-    memset(m->mvd, 0, sizeof m->mvd);
+    reset_motion(m);
     if (mt == 0) {
       if (t->q->transform8) m->t8 = (uint8_t)read_t8_flag(t);
       m->kind = m->t8 ? K_I8 : K_I4;
@@ -1514,7 +2082,7 @@ static int decode_mb(sdec_t *t, int islice) {
       mvd[0] = read_mvd(t, bx, by, 0);
       mvd[1] = read_mvd(t, bx, by, 1);
       int16_t mvp[2], mv[2];
-      pred_mv(t, bx, by, w, part == 3 ? 0 : dir, refs[k], mvp);
+      pred_mv(t, 0, bx, by, w, part == 3 ? 0 : dir, refs[k], mvp);
       mv[0] = (int16_t)(mvp[0] + mvd[0]);
       mv[1] = (int16_t)(mvp[1] + mvd[1]);
       set_motion(t, bx, by, w, h, mv, mvd);
@@ -1560,14 +2128,21 @@ static void finish_info(sdec_t *t) {
   info->skip = m->skip;
   info->t8x8 = m->t8;
   for (int i = 0; i < 16; i++) info->nnz[i] = m->nz[i];
-  for (int i = 0; i < 16; i++) {
-    info->mv[i][0] = m->kind == K_INTER ? m->mv[i][0] : 0;
-    info->mv[i][1] = m->kind == K_INTER ? m->mv[i][1] : 0;
-  }
+  const int inter = m->kind == K_INTER;
+  for (int i = 0; i < 16; i++)
+    for (int c = 0; c < 2; c++) {
+      info->mv[i][c] = inter && m->ref[i / 8 * 2 + (i % 4) / 2] >= 0 ? m->mv[i][c] : 0;
+      info->mv1[i][c] = inter && t->bslice && m->ref1[i / 8 * 2 + (i % 4) / 2] >= 0 ? m->mv1[i][c] : 0;
+    }
   for (int b8 = 0; b8 < 4; b8++) {
-    info->ref_idx[b8] = m->kind == K_INTER ? m->ref[b8] : -1;
-    info->ref_id[b8] = m->kind == K_INTER ? t->d->frames[t->refs[m->ref[b8]]].id : -1;
+    const int r0 = inter ? m->ref[b8] : -1, r1 = inter && t->bslice ? m->ref1[b8] : -1;
+    info->ref_idx[b8] = (int8_t)r0;
+    info->ref_id[b8] = r0 >= 0 ? t->d->frames[t->refs[r0]].id : -1;
+    info->ref_idx1[b8] = (int8_t)r1;
+    info->ref_id1[b8] = r1 >= 0 ? t->d->frames[t->refs1[r1]].id : -1;
   }
+  info->bdirect = m->bdirect;
+  info->direct8 = m->direct8;
 }
 
 static int decode_slice(od_dec *d, const sh_t *h, br_t *b) {
@@ -1584,7 +2159,11 @@ static int decode_slice(od_dec *d, const sh_t *h, br_t *b) {
   d->sfilter[t.slice_id - 1] = (or_slice_filter){h->disable_deblock, h->alpha_offset, h->beta_offset,
                                                    {t.q->chroma_qp_offset[0], t.q->chroma_qp_offset[1]}};
   const int islice = h->slice_type == 2;
-  if (!islice) {
+  t.bslice = h->slice_type == 1;
+  if (t.bslice) {
+    int rc = ref_lists_b(d, t.s, h, t.refs, t.refs1);
+    if (rc) return rc;
+  } else if (!islice) {
     int rc = ref_list(d, t.s, h, t.refs);
     if (rc) return rc;
   }
@@ -1608,7 +2187,7 @@ static int decode_slice(od_dec *d, const sh_t *h, br_t *b) {
     if (!islice) skip = read_mb_skip(&t);
     int rc;
     if (skip) {
-      rc = decode_skip(&t);
+      rc = t.bslice ? decode_b_skip(&t) : decode_skip(&t);
       t.cur->qp = t.qp;
     } else {
       rc = decode_mb(&t, islice);

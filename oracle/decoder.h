@@ -13,9 +13,10 @@
  * cabac.rs (the arithmetic decoder), mb.rs (CABAC macroblock layer, motion vector prediction,
  * residual, intra and inter reconstruction), dpb.rs (order counts, reference lists, marking),
  * deblock.rs (through the oracle's loop filter). The subset: progressive frames, 8-bit 4:2:0,
- * I and P slices, CABAC, the 8x8 transform, scaling matrices, explicit weighted prediction, I_PCM,
- * several slices and reference pictures. Anything else (CAVLC, B slices, MBAFF, field pictures,
- * other chroma formats or bit depths) is rejected with OD_EUNSUPPORTED.
+ * I, P and B slices, CABAC, the 8x8 transform, scaling matrices, explicit and implicit weighted
+ * prediction, spatial and temporal direct prediction, I_PCM, several slices and reference pictures.
+ * Anything else (CAVLC, MBAFF, field pictures, other chroma formats or bit depths) is rejected with
+ * OD_EUNSUPPORTED.
  */
 #ifndef IRONAV_ORACLE_DECODER_H
 #define IRONAV_ORACLE_DECODER_H
@@ -62,7 +63,12 @@ int od_frame_read_full(const od_dec *d, int k, uint8_t *y, uint8_t *cb, uint8_t 
  * OD_NSTATS and fills min(n, OD_NSTATS) counts. */
 enum {
   OD_STAT_I4 = 0, OD_STAT_I8 = 9, OD_STAT_I16 = 18, OD_STAT_CHROMA = 22, OD_STAT_PART = 26, OD_STAT_SUB = 30,
-  OD_STAT_SKIP = 34, OD_STAT_PCM = 35, OD_STAT_INTER_T8 = 36, OD_STAT_REF_GT0 = 37, OD_NSTATS = 38
+  OD_STAT_SKIP = 34, OD_STAT_PCM = 35, OD_STAT_INTER_T8 = 36, OD_STAT_REF_GT0 = 37,
+  /* B slices: B_Skip, B_Direct_16x16, B 16x16 from list 0 / list 1 / both, B 16x8 / 8x16, B_8x8, its
+     13 sub_mb_types, macroblocks with temporal direct prediction, bi-predicted blocks with implicit and
+     with explicit weights */
+  OD_STAT_B_SKIP = 38, OD_STAT_B_DIRECT16 = 39, OD_STAT_B16 = 40, OD_STAT_B_PART = 43, OD_STAT_B8X8 = 45,
+  OD_STAT_B_SUB = 46, OD_STAT_B_TEMPORAL = 59, OD_STAT_B_IMPLICIT = 60, OD_STAT_B_EXPLICIT = 61, OD_NSTATS = 62
 };
 int od_stats(const od_dec *d, int64_t *out, int n);
 

@@ -764,9 +764,10 @@ static int ref_of(int part, int n, int x, int y) { return part == 3 ? (y / 2) * 
 typedef struct {
   uint8_t i4, inter, t8x8, skip, kind, cbp, chroma_mode, cbf_dc;
   uint8_t modes[16], nnz[16], cnnz[2][4];
-  int16_t mvs[16][2];
-  int8_t refs[4];
-  uint8_t mvd[16][2];
+  int16_t mvs[16][2], mvs1[16][2];   /* lists 0 and 1 */
+  int8_t refs[4], refs1[4];
+  uint8_t bdirect;                    /* B_Skip or B_Direct_16x16 */
+  uint8_t mvd[16][2], mvd1[16][2];
 } mbstate_t;
 
 typedef struct {
@@ -781,6 +782,12 @@ typedef struct {
   int32_t cdc[2][4];
   int32_t cac[2][4][16];
   uint8_t nnz[16], cnnz[2][4];
+  /* B slices (enc/frame.rs:292-296): per partition the direction (0 direct, bit 0 list 0, bit 1
+     list 1), per 8x8 block the lists used, per list the vectors per 4x4 block and the differences
+     per partition */
+  uint8_t bdir[4], buse[4][2];
+  int16_t bmv[2][16][2];
+  int32_t bmvd[2][4][2];
 } mbcode_t;
 
 static void mbcode_new(mbcode_t *c, int kind) {
@@ -800,6 +807,9 @@ typedef struct {
   int bd, mb_w, mb_h, qp, qpc, lambda;
   int row0, row1, slice_id;
   int p_slice, nref, sub8, use_cabac, transform8;
+  /* a B slice: plist holds the list 0 and list 1 pictures, dist their picture-order distances
+     (enc/frame.rs:350-354, with_b :442-447) */
+  int b_slice, dist_tb, dist_td;
   const refpic_t *plist;
   uint16_t *pic[3];
   ia_mbinfo *mbs;
@@ -824,7 +834,7 @@ static const mbstate_t *nb_state(const coder_t *c, int mbx, int mby) {
 }
 
 /* enc/frame.rs:521-539. Returns 1 when available, with *r and mv set. */
-static int nb_motion(const coder_t *c, int mbx, int mby, const int16_t (*cur)[2], const int8_t *cref, uint32_t done,
+static int nb_motion(const coder_t *c, int l, int mbx, int mby, const int16_t (*cur)[2], const int8_t *cref, uint32_t done,
                      int x, int y, int *r, int16_t *mv) {
   int dx, dy, bx, by;
   if (x < 0 && y < 0) { dx = -1; dy = -1; bx = 3; by = 3; }
@@ -841,21 +851,22 @@ static int nb_motion(const coder_t *c, int mbx, int mby, const int16_t (*cur)[2]
   }
   const mbstate_t *s = nb_state(c, mbx + dx, mby + dy);
   if (!s) return 0;
-  int rr = s->refs[(by / 2) * 2 + bx / 2];
-  if (s->inter && rr >= 0) { *r = rr; mv[0] = s->mvs[by * 4 + bx][0]; mv[1] = s->mvs[by * 4 + bx][1]; }
+  int rr = (l ? s->refs1 : s->refs)[(by / 2) * 2 + bx / 2];
+  const int16_t (*sm)[2] = l ? s->mvs1 : s->mvs;
+  if (s->inter && rr >= 0) { *r = rr; mv[0] = sm[by * 4 + bx][0]; mv[1] = sm[by * 4 + bx][1]; }
   else { *r = -1; mv[0] = mv[1] = 0; }
   return 1;
 }
 
 /* enc/frame.rs:544-574: the vector predictor of the partition at (bx, by), w blocks wide (8.4.1.3) */
-static void pred_mv(const coder_t *c, int mbx, int mby, const int16_t (*cur)[2], const int8_t *cref, uint32_t done,
+static void pred_mv(const coder_t *c, int l, int mbx, int mby, const int16_t (*cur)[2], const int8_t *cref, uint32_t done,
                     int bx, int by, int w, int dir, int rf, int16_t *out) {
   int ra = -1, rb = -1, rc = -1;
   int16_t ma[2] = {0, 0}, mb[2] = {0, 0}, mc[2] = {0, 0};
-  int ha = nb_motion(c, mbx, mby, cur, cref, done, bx - 1, by, &ra, ma);
-  int hb = nb_motion(c, mbx, mby, cur, cref, done, bx, by - 1, &rb, mb);
-  int hc = nb_motion(c, mbx, mby, cur, cref, done, bx + w, by - 1, &rc, mc);
-  if (!hc) hc = nb_motion(c, mbx, mby, cur, cref, done, bx - 1, by - 1, &rc, mc);
+  int ha = nb_motion(c, l, mbx, mby, cur, cref, done, bx - 1, by, &ra, ma);
+  int hb = nb_motion(c, l, mbx, mby, cur, cref, done, bx, by - 1, &rb, mb);
+  int hc = nb_motion(c, l, mbx, mby, cur, cref, done, bx + w, by - 1, &rc, mc);
+  if (!hc) hc = nb_motion(c, l, mbx, mby, cur, cref, done, bx - 1, by - 1, &rc, mc);
   if (!ha) { ra = -1; ma[0] = ma[1] = 0; }
   if (!hb) { rb = -1; mb[0] = mb[1] = 0; }
   if (!hc) { rc = -1; mc[0] = mc[1] = 0; }
@@ -882,8 +893,8 @@ static void pred_mv(const coder_t *c, int mbx, int mby, const int16_t (*cur)[2],
 static const int16_t ZMV[16][2];
 static const int8_t NOREF[16];
 
-static void mv_pred(const coder_t *c, int mbx, int mby, int rf, int16_t *out) {
-  pred_mv(c, mbx, mby, ZMV, NOREF, 0, 0, 0, 4, 0, rf, out);
+static void mv_pred(const coder_t *c, int l, int mbx, int mby, int rf, int16_t *out) {
+  pred_mv(c, l, mbx, mby, ZMV, NOREF, 0, 0, 0, 4, 0, rf, out);
 }
 
 /* enc/frame.rs:582-595 */
@@ -897,7 +908,7 @@ static void part_mvps(const coder_t *c, int mbx, int mby, int part, const uint8_
   int dir = (int[]){0, 1, 2, 0}[part];
   for (int k = 0; k < n; k++) {
     int rf = refs[ref_of(part, k, rs[k].x, rs[k].y)];
-    pred_mv(c, mbx, mby, (const int16_t(*)[2])cur, cref, done, rs[k].x, rs[k].y, rs[k].w, dir, rf, out[k]);
+    pred_mv(c, 0, mbx, mby, (const int16_t(*)[2])cur, cref, done, rs[k].x, rs[k].y, rs[k].w, dir, rf, out[k]);
     for (int by = rs[k].y; by < rs[k].y + rs[k].h; by++)
       for (int bx = rs[k].x; bx < rs[k].x + rs[k].w; bx++) {
         cur[by * 4 + bx][0] = mvs[k][0];
@@ -912,13 +923,13 @@ static void part_mvps(const coder_t *c, int mbx, int mby, int part, const uint8_
 static void skip_mv(const coder_t *c, int mbx, int mby, int16_t *out) {
   int ra, rb;
   int16_t ma[2], mb[2];
-  int ha = nb_motion(c, mbx, mby, ZMV, NOREF, 0, -1, 0, &ra, ma);
-  int hb = nb_motion(c, mbx, mby, ZMV, NOREF, 0, 0, -1, &rb, mb);
+  int ha = nb_motion(c, 0, mbx, mby, ZMV, NOREF, 0, -1, 0, &ra, ma);
+  int hb = nb_motion(c, 0, mbx, mby, ZMV, NOREF, 0, 0, -1, &rb, mb);
   if (!ha || !hb || (ra == 0 && ma[0] == 0 && ma[1] == 0) || (rb == 0 && mb[0] == 0 && mb[1] == 0)) {
     out[0] = out[1] = 0;
     return;
   }
-  mv_pred(c, mbx, mby, 0, out);
+  mv_pred(c, 0, mbx, mby, 0, out);
 }
 
 /* enc/frame.rs:619-627 */
@@ -958,10 +969,14 @@ static int satd_block(const coder_t *c, int plane, int x, int y, int w, int h, c
   return total;
 }
 
+/* The reference r of list l: a P slice's list 0 is plist, a B slice has one picture per list
+ * (enc/frame.rs:677-702 ref_half / ref_pic). */
+static const refpic_t *ref_pic(const coder_t *c, int l, int r) { return &c->plist[c->b_slice ? l : r]; }
+
 /* enc/frame.rs:662-671 */
-static int sad_int(const coder_t *c, int r, int x, int y, int w, int h, const int16_t *mv) {
+static int sad_int(const coder_t *c, int l, int r, int x, int y, int w, int h, const int16_t *mv) {
   const int pw = c->mb_w * 16, sw = pw + 2 * OR_PAD, ls = ls_(c);
-  const uint16_t *g = c->plist[r].hp;
+  const uint16_t *g = ref_pic(c, l, r)->hp;
   int gx = x + (mv[0] >> 2), gy = y + (mv[1] >> 2), sad = 0;
   for (int rr = 0; rr < h; rr++)
     for (int k = 0; k < w; k++)
@@ -969,8 +984,12 @@ static int sad_int(const coder_t *c, int r, int x, int y, int w, int h, const in
   return sad;
 }
 
+static void luma_pred_l(const coder_t *c, int l, int r, int x, int y, int w, int h, const int16_t *mv, uint16_t *out) {
+  or_halfpel_predict(ref_pic(c, l, r)->hp, c->mb_w * 16, c->mb_h * 16, x, y, mv[0], mv[1], w, h, out);
+}
+
 static void luma_pred(const coder_t *c, int r, int x, int y, int w, int h, const int16_t *mv, uint16_t *out) {
-  or_halfpel_predict(c->plist[r].hp, c->mb_w * 16, c->mb_h * 16, x, y, mv[0], mv[1], w, h, out);
+  luma_pred_l(c, 0, r, x, y, w, h, mv, out);
 }
 
 /* enc/frame.rs:685-691 */
@@ -982,9 +1001,10 @@ static int ref_bits(const coder_t *c, int r) {
   return 2 * lg + 1;
 }
 
-/* enc/frame.rs:944-947: the middle vector of the co-located macroblock of reference 0, if inter */
+/* enc/frame.rs:944-947: the middle vector of the co-located macroblock of reference 0 (of a B
+ * slice: of its list 1 picture, :1557), if inter */
 static int col_mv(const coder_t *c, int mbx, int mby, int16_t *mv) {
-  const ia_mbinfo *m = c->plist[0].mbs;
+  const ia_mbinfo *m = c->plist[c->b_slice ? 1 : 0].mbs;
   if (!m) return 0;
   const ia_mbinfo *col = &m[mby * c->mb_w + mbx];
   if (col->kind != K_INTER) return 0;
@@ -1005,9 +1025,10 @@ static int fs_index(int x, int y, int w, int h) {
   return 25 + 4 * b8 + (y & 1) * 2 + (x & 1);
 }
 
-/* enc/frame.rs:949-1019, with IA_SEARCH_FULL's exhaustive winner as a last candidate */
-static int motion_search(const coder_t *c, int r, int mbx, int mby, int bx, int by, int bw, int bh,
-                         const int16_t *mvp, const int16_t (*extra)[2], int nextra, int16_t *out) {
+/* enc/frame.rs:949-1019 in list l, with IA_SEARCH_FULL's exhaustive winner as a last candidate (P
+ * slices) */
+static int motion_search_l(const coder_t *c, int l, int r, int mbx, int mby, int bx, int by, int bw, int bh,
+                           const int16_t *mvp, const int16_t (*extra)[2], int nextra, int16_t *out) {
   const int lam = c->lambda;
   const int x0 = mbx * 16 + bx * 4, y0 = mby * 16 + by * 4, w = bw * 4, h = bh * 4;
   const int lo_x = maxi(-(x0 + 24) * 4, -2048), hi_x = mini((c->mb_w * 16 - x0 + 8) * 4, 2047);
@@ -1022,18 +1043,18 @@ static int motion_search(const coder_t *c, int r, int mbx, int mby, int bx, int 
   for (int k = 0; k < 3; k++) {
     int rr;
     int16_t mv[2];
-    if (nb_motion(c, mbx, mby, ZMV, NOREF, 0, NB[k][0], NB[k][1], &rr, mv) && rr == r) {
+    if (nb_motion(c, l, mbx, mby, ZMV, NOREF, 0, NB[k][0], NB[k][1], &rr, mv) && rr == r) {
       CLAMPV(cands[nc], mv[0], mv[1]); INTV(cands[nc]); nc++;
     }
   }
   for (int k = 0; k < nextra; k++) { CLAMPV(cands[nc], extra[k][0], extra[k][1]); INTV(cands[nc]); nc++; }
-  if (c->search_mode == IA_SEARCH_FULL && c->fs) {
+  if (c->search_mode == IA_SEARCH_FULL && c->fs && !c->b_slice) {
     const ia_fsresult *f = &c->fs[((size_t)r * c->mb_w * c->mb_h + (size_t)mby * c->mb_w + mbx) * IA_NPART + fs_index(bx, by, bw, bh)];
     CLAMPV(cands[nc], f->mv[0], f->mv[1]); INTV(cands[nc]); nc++;
   }
   int16_t best[2] = {0, 0};
   int best_cost = 0x7fffffff;
-#define COST_INT(mv) (sad_int(c, r, x0, y0, w, h, (mv)) + lam * mvd_bits((mv), mvp))
+#define COST_INT(mv) (sad_int(c, l, r, x0, y0, w, h, (mv)) + lam * mvd_bits((mv), mvp))
   for (int k = 0; k < nc; k++) {
     int cost = COST_INT(cands[k]);
     if (cost < best_cost) { best[0] = cands[k][0]; best[1] = cands[k][1]; best_cost = cost; }
@@ -1064,7 +1085,7 @@ static int motion_search(const coder_t *c, int r, int mbx, int mby, int bx, int 
   /* sub-sample refinement by SATD: 8 half samples around, then 4 quarter samples (:994-1017);
      realtime coding keeps to the half samples (:1006) */
   uint16_t pred[256];
-#define COST_SUB(mv) (luma_pred(c, r, x0, y0, w, h, (mv), pred), satd_block(c, 0, x0, y0, w, h, pred) + lam * mvd_bits((mv), mvp))
+#define COST_SUB(mv) (luma_pred_l(c, l, r, x0, y0, w, h, (mv), pred), satd_block(c, 0, x0, y0, w, h, pred) + lam * mvd_bits((mv), mvp))
   best_cost = COST_SUB(best);
   static const int SQ[8][2] = {{-1, -1}, {0, -1}, {1, -1}, {-1, 0}, {1, 0}, {-1, 1}, {0, 1}, {1, 1}};
   static const int DM[4][2] = {{0, -1}, {-1, 0}, {1, 0}, {0, 1}};
@@ -1088,6 +1109,11 @@ static int motion_search(const coder_t *c, int r, int mbx, int mby, int bx, int 
 #undef COST_SUB
 }
 
+static int motion_search(const coder_t *c, int r, int mbx, int mby, int bx, int by, int bw, int bh,
+                         const int16_t *mvp, const int16_t (*extra)[2], int nextra, int16_t *out) {
+  return motion_search_l(c, 0, r, mbx, mby, bx, by, bw, bh, mvp, extra, nextra, out);
+}
+
 typedef struct {
   int16_t mvs[16][2];
   int8_t refs[16];
@@ -1106,7 +1132,7 @@ static int sub_search(const coder_t *c, int mbx, int mby, int r, int k, const in
     for (int q = 0; q < NSUBS[s]; q++) {
       int x = ox + SUBS[s][q][0], y = oy + SUBS[s][q][1], w = SUBS[s][q][2], h = SUBS[s][q][3];
       int16_t mvp[2], m[2];
-      pred_mv(c, mbx, mby, (const int16_t(*)[2])t.mvs, t.refs, t.done, x, y, w, 0, r, mvp);
+      pred_mv(c, 0, mbx, mby, (const int16_t(*)[2])t.mvs, t.refs, t.done, x, y, w, 0, r, mvp);
       int cc = motion_search(c, r, mbx, mby, x, y, w, h, mvp, (const int16_t(*)[2])extra, 1, m);
       cost += cc + lam * mvd_bits(m, mvp);
       for (int by = y; by < y + h; by++)
@@ -1136,7 +1162,7 @@ static void best_inter_ref(const coder_t *c, int mbx, int mby, int r, inter_t *o
   const int lam = c->lambda, rb = lam * ref_bits(c, r);
   uint8_t refs[4] = {(uint8_t)r, (uint8_t)r, (uint8_t)r, (uint8_t)r};
   int16_t mvp[2], mv[2], col[1][2];
-  mv_pred(c, mbx, mby, r, mvp);
+  mv_pred(c, 0, mbx, mby, r, mvp);
   int ncol = col_mv(c, mbx, mby, col[0]);
   int cost = motion_search(c, r, mbx, mby, 0, 0, 4, 4, mvp, (const int16_t(*)[2])col, ncol, mv);
   inter_t best;
@@ -1700,22 +1726,65 @@ static void mvd_grid(const rect_t *rs, int n, const int32_t (*mvds)[2], uint8_t 
         for (int q = 0; q < 2; q++) out[by * 4 + bx][q] = (uint8_t)mini(absi(mvds[k][q]), 64);
 }
 
-/* enc/frame.rs:2028-2279, P and I slices */
+static int b_mb_type(const mbcode_t *code);
+
+/* enc/frame.rs:2028-2279 */
 static void mb_bins(const coder_t *c, bins_t *b, const mbcode_t *code, int mbx, int mby) {
   const mbstate_t *left = nb_state(c, mbx - 1, mby), *top = nb_state(c, mbx, mby - 1);
   if (c->p_slice) {
-    decision(b, 11 + (left && !left->skip) + (top && !top->skip), code->skip);
+    decision(b, (c->b_slice ? 24 : 11) + (left && !left->skip) + (top && !top->skip), code->skip);
     if (code->skip) return;
   }
-  const int cbpl = code->cbp & 15, cbpc = code->cbp >> 4, prefix_ok = c->p_slice, base = c->p_slice ? 17 : 3;
-  if (code->kind == K_INTER) {
+  const int cbpl = code->cbp & 15, cbpc = code->cbp >> 4, prefix_ok = c->p_slice;
+  const int base = c->b_slice ? 32 : c->p_slice ? 17 : 3;
+  const int b_ctx = 27 + (left && !left->bdirect) + (top && !top->bdirect);
+  if (code->kind == K_INTER && c->b_slice) {
+    /* B_Direct_16x16 0, B_L0_16x16 100, B_L1_16x16 101, the others 11 and four or five more bins
+       (Table 9-37) */
+    const int t = b_mb_type(code);
+    decision(b, b_ctx, t != 0);
+    if (t == 1 || t == 2) {
+      decision(b, 30, 0);
+      decision(b, 32, t == 2);
+    } else if (t != 0) {
+      decision(b, 30, 1);
+      const int bits = t >= 3 && t <= 10 ? t - 3 : t == 11 ? 14 : t == 22 ? 15 : (t + 4) >> 1;
+      decision(b, 31, (bits >> 3) & 1);
+      for (int sh = 2; sh >= 0; sh--) decision(b, 32, (bits >> sh) & 1);
+      if (t >= 12 && t <= 21) decision(b, 32, (t + 4) & 1);
+    }
+    if (t == 22)
+      /* sub_mb_type: B_Direct_8x8 0, B_L0_8x8 100, B_L1_8x8 101, B_Bi_8x8 11000 */
+      for (int k = 0; k < 4; k++) {
+        const int d = code->bdir[k];
+        decision(b, 36, d != 0);
+        if (d == 1 || d == 2) {
+          decision(b, 37, 0);
+          decision(b, 39, d == 2);
+        } else if (d == 3) {
+          decision(b, 37, 1);
+          decision(b, 38, 0);
+          decision(b, 39, 0);
+          decision(b, 39, 0);
+        }
+      }
+  } else if (code->kind == K_INTER) {
     static const int B1[4] = {0, 1, 1, 0}, B2[4] = {0, 1, 0, 1};
     decision(b, 14, 0);
     decision(b, 15, B1[code->part]);
     decision(b, B1[code->part] ? 17 : 16, B2[code->part]);
   } else {
     int first;
-    if (prefix_ok) {
+    if (c->b_slice) {
+      /* 111101: an intra type follows, with the suffix contexts */
+      decision(b, b_ctx, 1);
+      decision(b, 30, 1);
+      decision(b, 31, 1);
+      decision(b, 32, 1);
+      decision(b, 32, 0);
+      decision(b, 32, 1);
+      first = 32;
+    } else if (prefix_ok) {
       decision(b, 14, 1);
       first = 17;
     } else {
@@ -1767,7 +1836,7 @@ static void mb_bins(const coder_t *c, bins_t *b, const mbcode_t *code, int mbx, 
       if (m > 1) decision(b, 67, m > 2);
     }
   } else {
-    if (code->part == 3) {
+    if (!c->b_slice && code->part == 3) {
       for (int k = 0; k < 4; k++) {
         int s = code->sub[k];
         if (s == 0) decision(b, 21, 1);
@@ -1776,7 +1845,7 @@ static void mb_bins(const coder_t *c, bins_t *b, const mbcode_t *code, int mbx, 
         else { decision(b, 21, 0); decision(b, 22, 1); decision(b, 23, 0); }
       }
     }
-    if (c->nref > 1) {
+    if (!c->b_slice && c->nref > 1) {
       uint8_t grid[4] = {0, 0, 0, 0};
       rect_t parts[4];
       int np;
@@ -1797,16 +1866,22 @@ static void mb_bins(const coder_t *c, bins_t *b, const mbcode_t *code, int mbx, 
           for (int x8 = bx / 2; x8 <= (bx + w - 1) / 2; x8++) grid[y8 * 2 + x8] = code->refs[k];
       }
     }
+    /* list 0's differences for the partitions using it, then list 1's */
+    static const uint8_t Z4[4] = {0, 0, 0, 0};
     rect_t rs[16];
-    int n = rects(code->part, code->sub, rs);
-    uint8_t grid[16][2];
-    mvd_grid(rs, n, code->mvds, grid);
-    for (int k = 0; k < n; k++) {
-      int bx = rs[k].x, by = rs[k].y;
-      for (int comp = 0; comp < 2; comp++) {
-        int a = bx > 0 ? grid[by * 4 + bx - 1][comp] : (left ? left->mvd[by * 4 + 3][comp] : 0);
-        int bb = by > 0 ? grid[(by - 1) * 4 + bx][comp] : (top ? top->mvd[12 + bx][comp] : 0);
-        mvd_bins(b, comp, code->mvds[k][comp], (uint32_t)(a + bb));
+    int n = rects(code->part, c->b_slice ? Z4 : code->sub, rs);
+    for (int l = 0; l < (c->b_slice ? 2 : 1); l++) {
+      const int32_t (*mvds)[2] = c->b_slice ? code->bmvd[l] : code->mvds;
+      uint8_t grid[16][2];
+      mvd_grid(rs, n, mvds, grid);
+      for (int k = 0; k < n; k++) {
+        if (c->b_slice && !((code->bdir[k] >> l) & 1)) continue;
+        int bx = rs[k].x, by = rs[k].y;
+        for (int comp = 0; comp < 2; comp++) {
+          int a = bx > 0 ? grid[by * 4 + bx - 1][comp] : (left ? (l ? left->mvd1 : left->mvd)[by * 4 + 3][comp] : 0);
+          int bb = by > 0 ? grid[(by - 1) * 4 + bx][comp] : (top ? (l ? top->mvd1 : top->mvd)[12 + bx][comp] : 0);
+          mvd_bins(b, comp, mvds[k][comp], (uint32_t)(a + bb));
+        }
       }
     }
   }
@@ -1941,6 +2016,343 @@ static void decide_rd(coder_t *c, int mbx, int mby, int i16, const inter_t *inte
   c->st->rd_cost = k.j;
 }
 
+/* ---- B slices (enc/frame.rs:1515-1780) ---------------------------------------------------------- */
+
+/* The 8x8 block (raster) holding raster 4x4 block b. */
+static int b8_of(int b) { return (b / 8) * 2 + (b % 4) / 2; }
+
+/* The smallest non-negative of two reference indices, or the larger (-1) when neither is (8.4.1.2.2
+ * MinPositive, enc/frame.rs:1724). */
+static int min_positive(int x, int y) { return x >= 0 && y >= 0 ? mini(x, y) : maxi(x, y); }
+
+/* Spatial direct motion of a macroblock (enc/frame.rs:1717-1745, 8.4.1.2.2): per list the neighbours'
+ * smallest reference index and the 16x16 vector predictor, zeroed in the 8x8 blocks whose co-located
+ * corner block of the list 1 picture uses its reference 0 and moves by at most one quarter sample;
+ * both lists with zero vectors when no neighbour uses either. */
+static void direct_mvs(const coder_t *c, int mbx, int mby, int16_t (*d)[16][2], uint8_t *use) {
+  int refs[2];
+  for (int l = 0; l < 2; l++) {
+    int ra = -1, rb = -1, rc = -1, r;
+    int16_t m[2];
+    if (nb_motion(c, l, mbx, mby, ZMV, NOREF, 0, -1, 0, &r, m)) ra = r;
+    if (nb_motion(c, l, mbx, mby, ZMV, NOREF, 0, 0, -1, &r, m)) rb = r;
+    if (nb_motion(c, l, mbx, mby, ZMV, NOREF, 0, 4, -1, &r, m)) rc = r;
+    else if (nb_motion(c, l, mbx, mby, ZMV, NOREF, 0, -1, -1, &r, m)) rc = r;
+    refs[l] = min_positive(ra, min_positive(rb, rc));
+  }
+  memset(d, 0, 2 * 16 * 2 * sizeof(int16_t));
+  if (refs[0] < 0 && refs[1] < 0) {
+    use[0] = use[1] = 1;
+    return;
+  }
+  for (int l = 0; l < 2; l++)
+    if (refs[l] >= 0) {
+      int16_t p[2];
+      mv_pred(c, l, mbx, mby, 0, p);
+      for (int b = 0; b < 16; b++) { d[l][b][0] = p[0]; d[l][b][1] = p[1]; }
+    }
+  const ia_mbinfo *m1 = c->plist[1].mbs;
+  const ia_mbinfo *col = m1 ? &m1[mby * c->mb_w + mbx] : NULL;
+  if (col && col->kind == K_INTER)
+    for (int b = 0; b < 16; b++) {
+      /* the corner 4x4 block of each 8x8 block stands for it (direct_8x8_inference_flag) */
+      const int cx = ((b % 4) >> 1) * 3, cy = ((b / 4) >> 1) * 3;
+      const int16_t *mv = col->mv[cy * 4 + cx];
+      if (col->ref_idx[(cy >> 1) * 2 + (cx >> 1)] == 0 && absi(mv[0]) <= 1 && absi(mv[1]) <= 1)
+        for (int l = 0; l < 2; l++)
+          if (refs[l] == 0) d[l][b][0] = d[l][b][1] = 0;
+    }
+  use[0] = refs[0] >= 0;
+  use[1] = refs[1] >= 0;
+}
+
+/* The prediction of a B macroblock 8x8 block by 8x8 block from the lists used there, averaged when
+ * both are (enc/frame.rs:1749-1780). */
+static void b_pred(const coder_t *c, int mbx, int mby, const int16_t (*mv)[16][2], const uint8_t (*used)[2],
+                   uint16_t *pred, uint16_t (*cpred)[64]) {
+  const int x0 = mbx * 16, y0 = mby * 16, cs = cs_(c);
+  for (int i8 = 0; i8 < 4; i8++) {
+    const int bx = (i8 & 1) * 8, by = (i8 >> 1) * 8;
+    uint16_t lp[2][64], cp[2][2][16];
+    memset(lp, 0, sizeof lp);
+    memset(cp, 0, sizeof cp);
+    for (int l = 0; l < 2; l++) {
+      if (!used[i8][l]) continue;
+      const int16_t *v = mv[l][(by / 4) * 4 + bx / 4];
+      luma_pred_l(c, l, 0, x0 + bx, y0 + by, 8, 8, v, lp[l]);
+      for (int ci = 0; ci < 2; ci++)
+        or_chroma_mc(c->plist[l].planes[1 + ci], cs, c->mb_h * 8, (x0 + bx) / 2, (y0 + by) / 2, v[0], v[1], 4, 4, cp[l][ci]);
+    }
+#define MIX(a, b) (used[i8][0] && used[i8][1] ? (uint16_t)(((a) + (b) + 1) >> 1) : used[i8][0] ? (a) : (b))
+    for (int k = 0; k < 64; k++) pred[(by + k / 8) * 16 + bx + k % 8] = MIX(lp[0][k], lp[1][k]);
+    for (int ci = 0; ci < 2; ci++)
+      for (int k = 0; k < 16; k++) cpred[ci][(by / 2 + k / 4) * 8 + bx / 2 + k % 4] = MIX(cp[0][ci][k], cp[1][ci][k]);
+#undef MIX
+  }
+}
+
+/* Each list's vector differences per partition of a B macroblock (enc/frame.rs:1688-1711): partitions
+ * predicted in order from the earlier ones that use the list, a direct 8x8 block counting with its
+ * derived motion. */
+static void b_mvds(const coder_t *c, int mbx, int mby, int part, const uint8_t *bdir, const int16_t (*bmv)[16][2],
+                   const uint8_t (*buse)[2], int32_t (*out)[4][2]) {
+  memset(out, 0, 2 * 4 * 2 * sizeof(int32_t));
+  const int dir = (int[]){0, 1, 2, 0}[part];
+  for (int l = 0; l < 2; l++) {
+    int16_t cur[16][2];
+    memset(cur, 0, sizeof cur);
+    uint32_t done = 0;
+    for (int k = 0; k < NPARTS[part]; k++) {
+      const int x = PARTS[part][k][0], y = PARTS[part][k][1], w = PARTS[part][k][2], h = PARTS[part][k][3];
+      if ((bdir[k] >> l) & 1) {
+        int16_t mvp[2];
+        pred_mv(c, l, mbx, mby, (const int16_t(*)[2])cur, NOREF, done, x, y, w, dir, 0, mvp);
+        out[l][k][0] = (int)bmv[l][y * 4 + x][0] - mvp[0];
+        out[l][k][1] = (int)bmv[l][y * 4 + x][1] - mvp[1];
+      }
+      for (int by = y; by < y + h; by++)
+        for (int bx = x; bx < x + w; bx++) {
+          const int b = by * 4 + bx;
+          cur[b][0] = bmv[l][b][0];
+          cur[b][1] = bmv[l][b][1];
+          done |= 1u << b;
+          if (!buse[b8_of(b)][l]) done |= 1u << (16 + b);
+        }
+    }
+  }
+}
+
+/* One partition's best direction for best_b_parts (enc/frame.rs:1628-1656): list 0, list 1, both
+ * averaged, and with `direct` B_Direct_8x8 from dpred; returns the direction, mv the lists' vectors. */
+static int b_part_search(const coder_t *c, int mbx, int mby, int bx, int by, int w, int h, const int16_t (*extra0)[2],
+                         int n0, const int16_t (*extra1)[2], int n1, int direct, const int16_t (*mvps)[2],
+                         const uint16_t *dpred, int16_t (*mv)[2], int *cost_out) {
+  const int lam = c->lambda, x0 = mbx * 16 + bx * 4, y0 = mby * 16 + by * 4, pw = w * 4, ph = h * 4;
+  int cost[2];
+  uint16_t preds[2][128], avg[128];
+  for (int l = 0; l < 2; l++) {
+    cost[l] = motion_search_l(c, l, 0, mbx, mby, bx, by, w, h, mvps[l], l ? extra1 : extra0, l ? n1 : n0, mv[l]);
+    luma_pred_l(c, l, 0, x0, y0, pw, ph, mv[l], preds[l]);
+  }
+  for (int i = 0; i < pw * ph; i++) avg[i] = (uint16_t)((preds[0][i] + preds[1][i] + 1) >> 1);
+  const int bi = satd_block(c, 0, x0, y0, pw, ph, avg) + lam * (mvd_bits(mv[0], mvps[0]) + mvd_bits(mv[1], mvps[1]) + 2);
+  int bd, bc;
+  if (bi < mini(cost[0], cost[1])) { bd = 3; bc = bi; }
+  else if (cost[0] <= cost[1]) { bd = 1; bc = cost[0]; }
+  else { bd = 2; bc = cost[1]; }
+  if (direct) {
+    /* sub_mb_type takes about one bin for direct and three for the others */
+    bc += lam * 3;
+    uint16_t sub[128];
+    for (int i = 0; i < pw * ph; i++) sub[i] = dpred[(by * 4 + i / pw) * 16 + bx * 4 + i % pw];
+    const int d = satd_block(c, 0, x0, y0, pw, ph, sub) + lam;
+    if (d < bc) { bd = 0; bc = d; }
+  }
+  *cost_out = bc;
+  return bd;
+}
+
+typedef struct {
+  int part;
+  uint8_t dirs[4];
+  int16_t pmv[2][4][2];
+} bparts_t;
+
+/* The best B partitioning below 16x16 when it costs less than cost16 (enc/frame.rs:1621-1685): each
+ * 8x8 block direct or from list 0, list 1 or both, then 16x8 and 8x16 (without direct); returns 0
+ * when the 8x8 blocks' running cost reaches cost16. */
+static int best_b_parts(const coder_t *c, int mbx, int mby, const int16_t (*mvs16)[2], const int16_t (*mvps)[2],
+                        int cost16, const uint16_t *dpred, bparts_t *out) {
+  const int lam = c->lambda;
+  uint8_t dirs[4] = {0, 0, 0, 0};
+  int16_t pmv[2][4][2];
+  memset(pmv, 0, sizeof pmv);
+  int total = lam * 6;
+  for (int k = 0; k < 4; k++) {
+    int16_t mv[2][2];
+    int cc;
+    dirs[k] = (uint8_t)b_part_search(c, mbx, mby, PARTS[3][k][0], PARTS[3][k][1], 2, 2, &mvs16[0], 1, &mvs16[1], 1, 1,
+                                     mvps, dpred, mv, &cc);
+    for (int l = 0; l < 2; l++) { pmv[l][k][0] = mv[l][0]; pmv[l][k][1] = mv[l][1]; }
+    total += cc;
+    if (total >= cost16) return 0;
+  }
+  out->part = 3;
+  memcpy(out->dirs, dirs, 4);
+  memcpy(out->pmv, pmv, sizeof pmv);
+  int best_total = total;
+  int16_t extra[2][5][2];
+  for (int l = 0; l < 2; l++) {
+    for (int k = 0; k < 4; k++) { extra[l][k][0] = pmv[l][k][0]; extra[l][k][1] = pmv[l][k][1]; }
+    extra[l][4][0] = mvs16[l][0];
+    extra[l][4][1] = mvs16[l][1];
+  }
+  for (int part = 1; part < 3; part++) {
+    uint8_t d2[4] = {0, 0, 0, 0};
+    int16_t m2[2][4][2];
+    memset(m2, 0, sizeof m2);
+    int t2 = lam * 7;
+    for (int k = 0; k < 2; k++) {
+      int16_t mv[2][2];
+      int cc;
+      const uint8_t *r = PARTS[part][k];
+      d2[k] = (uint8_t)b_part_search(c, mbx, mby, r[0], r[1], r[2], r[3], (const int16_t(*)[2])extra[0], 5,
+                                     (const int16_t(*)[2])extra[1], 5, 0, mvps, dpred, mv, &cc);
+      for (int l = 0; l < 2; l++) { m2[l][k][0] = mv[l][0]; m2[l][k][1] = mv[l][1]; }
+      t2 += cc;
+      if (t2 >= best_total) break;
+    }
+    if (t2 < best_total) {
+      out->part = part;
+      memcpy(out->dirs, d2, 4);
+      memcpy(out->pmv, m2, sizeof m2);
+      best_total = t2;
+    }
+  }
+  return 1;
+}
+
+/* The mb_type of a B macroblock (Table 7-14; enc/frame.rs:2343-2351). */
+static int b_mb_type(const mbcode_t *code) {
+  static const uint8_t PAIRS[9][2] = {{1, 1}, {2, 2}, {1, 2}, {2, 1}, {1, 3}, {2, 3}, {3, 1}, {3, 2}, {3, 3}};
+  if (code->part == 0) return code->bdir[0];
+  if (code->part == 3) return 22;
+  int i = 0;
+  while (i < 9 && !(PAIRS[i][0] == code->bdir[0] && PAIRS[i][1] == code->bdir[1])) i++;
+  return 4 + 2 * i + (code->part == 2);
+}
+
+static void put_pred(coder_t *c, int mbx, int mby, const uint16_t *pred, const uint16_t (*cpred)[64]) {
+  uint16_t rec[384];
+  memcpy(rec, pred, 256 * sizeof *rec);
+  memcpy(rec + 256, cpred, 128 * sizeof *rec);
+  restore_mb(c, mbx, mby, rec);
+}
+
+/* A B macroblock by rate-distortion cost among B_Skip and B_Direct_16x16, the best of list 0, list 1
+ * and both at 16x16 and the best smaller partitioning, Intra16x16 and Intra4x4 (enc/frame.rs:1518-1617). */
+static void decide_b(coder_t *c, int mbx, int mby, mbcode_t *out) {
+  const int lam = c->lambda, x0 = mbx * 16, y0 = mby * 16;
+  const double lam2 = 1.6 * pow(2.0, ((double)qpp_(c) - 12.0) / 3.0);
+  static _Thread_local keep_t k;
+  k.have = 0;
+  static _Thread_local mbcode_t code;
+  const int nt8 = c->transform8 ? 2 : 1;
+  int16_t direct[2][16][2];
+  uint8_t duse[2], dused[4][2];
+  direct_mvs(c, mbx, mby, direct, duse);
+  for (int i = 0; i < 4; i++) { dused[i][0] = duse[0]; dused[i][1] = duse[1]; }
+  uint16_t dpred[256], dcpred[2][64];
+  b_pred(c, mbx, mby, (const int16_t(*)[16][2])direct, (const uint8_t(*)[2])dused, dpred, dcpred);
+  mbcode_new(&code, K_INTER);
+  code.skip = 1;
+  memcpy(code.buse, dused, sizeof dused);
+  memcpy(code.bmv, direct, sizeof direct);
+  put_pred(c, mbx, mby, dpred, (const uint16_t(*)[64])dcpred);
+  keep(c, &k, rd_cost(c, &code, mbx, mby, lam2), &code, mbx, mby);
+  for (int t8 = 0; t8 < nt8; t8++) {
+    mbcode_new(&code, K_INTER);
+    memcpy(code.buse, dused, sizeof dused);
+    memcpy(code.bmv, direct, sizeof direct);
+    code_inter_residual(c, &code, mbx, mby, dpred, (const uint16_t(*)[64])dcpred, t8);
+    /* without residual, B_Direct_16x16 is B_Skip at a higher price */
+    if (code.cbp != 0 && (!t8 || code.t8x8)) keep(c, &k, rd_cost(c, &code, mbx, mby, lam2), &code, mbx, mby);
+  }
+  /* 16x16 from either list or both */
+  int16_t mvps[2][2], mvs[2][2];
+  int costs[2];
+  uint16_t preds[2][256];
+  mv_pred(c, 0, mbx, mby, 0, mvps[0]);
+  mv_pred(c, 1, mbx, mby, 0, mvps[1]);
+  const int tb = c->dist_tb, td = c->dist_td;
+  int16_t colv[2];
+  const int ncol = col_mv(c, mbx, mby, colv);
+  for (int l = 0; l < 2; l++) {
+    /* the co-located vector spans both lists' references, split by the picture's distances */
+    int16_t extra[2][2];
+    int ne = 0;
+    if (ncol) {
+      const int num = l == 0 ? tb : tb - td;
+      extra[ne][0] = (int16_t)(colv[0] * num / td);
+      extra[ne][1] = (int16_t)(colv[1] * num / td);
+      ne++;
+    }
+    if (l == 1) {
+      extra[ne][0] = (int16_t)(mvs[0][0] * (tb - td) / tb);
+      extra[ne][1] = (int16_t)(mvs[0][1] * (tb - td) / tb);
+      ne++;
+    }
+    const int cc = motion_search_l(c, l, 0, mbx, mby, 0, 0, 4, 4, mvps[l], (const int16_t(*)[2])extra, ne, mvs[l]);
+    costs[l] = cc + lam * (mvd_bits(mvs[l], mvps[l]) + 3);
+    luma_pred_l(c, l, 0, x0, y0, 16, 16, mvs[l], preds[l]);
+  }
+  uint16_t avg[256];
+  for (int i = 0; i < 256; i++) avg[i] = (uint16_t)((preds[0][i] + preds[1][i] + 1) >> 1);
+  const int bi = satd_block(c, 0, x0, y0, 16, 16, avg) + lam * (mvd_bits(mvs[0], mvps[0]) + mvd_bits(mvs[1], mvps[1]) + 5);
+  const int dir = bi < mini(costs[0], costs[1]) ? 3 : costs[0] <= costs[1] ? 1 : 2;
+  const int best16 = mini(bi, mini(costs[0], costs[1]));
+  /* candidates: the 16x16 one, and the best smaller partitioning if it beats it */
+  bparts_t cands[2];
+  int ncand = 1;
+  memset(cands, 0, sizeof cands);
+  cands[0].part = 0;
+  cands[0].dirs[0] = (uint8_t)dir;
+  for (int l = 0; l < 2; l++)
+    for (int q = 0; q < 4; q++) { cands[0].pmv[l][q][0] = mvs[l][0]; cands[0].pmv[l][q][1] = mvs[l][1]; }
+  if (best_b_parts(c, mbx, mby, (const int16_t(*)[2])mvs, (const int16_t(*)[2])mvps, best16, dpred, &cands[1])) ncand = 2;
+  c->st->inter_cost = best16;
+  c->st->inter_part = cands[ncand - 1].part;
+  for (int ci = 0; ci < ncand; ci++) {
+    const bparts_t *cd = &cands[ci];
+    uint8_t buse[4][2];
+    int16_t bmv[2][16][2];
+    int32_t bmvd[2][4][2];
+    memset(buse, 0, sizeof buse);
+    memset(bmv, 0, sizeof bmv);
+    for (int q = 0; q < NPARTS[cd->part]; q++) {
+      const uint8_t *r = PARTS[cd->part][q];
+      for (int by = r[1]; by < r[1] + r[3]; by++)
+        for (int bx = r[0]; bx < r[0] + r[2]; bx++) {
+          const int b = by * 4 + bx, b8 = b8_of(b);
+          for (int l = 0; l < 2; l++) {
+            buse[b8][l] = cd->dirs[q] == 0 ? duse[l] : (cd->dirs[q] >> l) & 1;
+            if (buse[b8][l]) {
+              const int16_t *v = cd->dirs[q] == 0 ? direct[l][b] : cd->pmv[l][q];
+              bmv[l][b][0] = v[0];
+              bmv[l][b][1] = v[1];
+            }
+          }
+        }
+    }
+    b_mvds(c, mbx, mby, cd->part, cd->dirs, (const int16_t(*)[16][2])bmv, (const uint8_t(*)[2])buse, bmvd);
+    uint16_t pred[256], cpred[2][64];
+    b_pred(c, mbx, mby, (const int16_t(*)[16][2])bmv, (const uint8_t(*)[2])buse, pred, cpred);
+    for (int t8 = 0; t8 < nt8; t8++) {
+      mbcode_new(&code, K_INTER);
+      code.part = cd->part;
+      memcpy(code.bdir, cd->dirs, 4);
+      memcpy(code.buse, buse, sizeof buse);
+      memcpy(code.bmv, bmv, sizeof bmv);
+      memcpy(code.bmvd, bmvd, sizeof bmvd);
+      code_inter_residual(c, &code, mbx, mby, pred, (const uint16_t(*)[64])cpred, t8);
+      if (!t8 || code.t8x8) keep(c, &k, rd_cost(c, &code, mbx, mby, lam2), &code, mbx, mby);
+    }
+  }
+  int i16c;
+  const int i16 = best_i16(c, mbx, mby, &i16c);
+  c->st->i16_cost = i16c;
+  code_i16(c, mbx, mby, i16, &code);
+  code_intra_chroma(c, &code, mbx, mby);
+  keep(c, &k, rd_cost(c, &code, mbx, mby, lam2), &code, mbx, mby);
+  /* Intra8x8 is not tried in B pictures (enc/frame.rs:1611) */
+  code_i4(c, mbx, mby, &code);
+  code_intra_chroma(c, &code, mbx, mby);
+  keep(c, &k, rd_cost(c, &code, mbx, mby, lam2), &code, mbx, mby);
+  restore_mb(c, mbx, mby, k.rec);
+  *out = k.code;
+  c->st->rd_cost = k.j;
+}
+
 /* enc/frame.rs:715-790 */
 static void code_mb(coder_t *c, int mbx, int mby, mbcode_t *out) {
   const int x0 = mbx * 16, y0 = mby * 16, lam = c->lambda;
@@ -1950,6 +2362,10 @@ static void code_mb(coder_t *c, int mbx, int mby, mbcode_t *out) {
   st->inter_cost = -1;
   st->inter_part = -1;
   st->rd_cost = 0.0;
+  if (c->b_slice) {
+    decide_b(c, mbx, mby, out);
+    return;
+  }
   if (c->p_slice) {
     int16_t smv[2];
     skip_mv(c, mbx, mby, smv);
@@ -2036,19 +2452,41 @@ static void code_mb(coder_t *c, int mbx, int mby, mbcode_t *out) {
 /* enc/frame.rs:2281-2339 */
 static void finish_mb(coder_t *c, const mbcode_t *code, int mbx, int mby) {
   const int idx = (mby - c->row0) * c->mb_w + mbx;
-  const int inter = code->kind == K_INTER;
+  const int inter = code->kind == K_INTER, bm = inter && c->b_slice;
+  static const uint8_t Z4[4] = {0, 0, 0, 0};
   mbstate_t *s = &c->state[idx];
   memset(s, 0, sizeof *s);
+  /* the lists each 8x8 block uses */
+  uint8_t used[4][2];
+  for (int b8 = 0; b8 < 4; b8++) {
+    used[b8][0] = inter && (!bm || code->buse[b8][0]);
+    used[b8][1] = bm && code->buse[b8][1];
+  }
   rect_t rs[16];
-  int n = rects(code->part, code->sub, rs);
-  if (inter && !code->skip) mvd_grid(rs, n, code->mvds, s->mvd);
-  if (inter)
+  int n = rects(code->part, bm ? Z4 : code->sub, rs);
+  if (inter && !code->skip) {
+    if (bm) {
+      mvd_grid(rs, n, code->bmvd[0], s->mvd);
+      mvd_grid(rs, n, code->bmvd[1], s->mvd1);
+    } else {
+      mvd_grid(rs, n, code->mvds, s->mvd);
+    }
+  }
+  if (bm) {
+    for (int b = 0; b < 16; b++)
+      for (int q = 0; q < 2; q++) {
+        s->mvs[b][q] = used[b8_of(b)][0] ? code->bmv[0][b][q] : 0;
+        s->mvs1[b][q] = used[b8_of(b)][1] ? code->bmv[1][b][q] : 0;
+      }
+  } else if (inter) {
     for (int k = 0; k < n; k++)
       for (int by = rs[k].y; by < rs[k].y + rs[k].h; by++)
         for (int bx = rs[k].x; bx < rs[k].x + rs[k].w; bx++) {
           s->mvs[by * 4 + bx][0] = code->mvs[k][0];
           s->mvs[by * 4 + bx][1] = code->mvs[k][1];
         }
+  }
+  s->bdirect = (uint8_t)(bm && code->part == 0 && code->bdir[0] == 0);
   const int cbpc = code->cbp >> 4;
   s->cbf_dc = (uint8_t)((code->kind == K_I16 && count_nz(code->dc, 16) > 0) | ((cbpc > 0 && count_nz(code->cdc[0], 4) > 0) << 1) |
                         ((cbpc > 0 && count_nz(code->cdc[1], 4) > 0) << 2));
@@ -2066,7 +2504,10 @@ static void finish_mb(coder_t *c, const mbcode_t *code, int mbx, int mby) {
   memcpy(s->modes, code->modes, 16);
   memcpy(s->nnz, code->nnz, 16);
   memcpy(s->cnnz, code->cnnz, 8);
-  for (int b8 = 0; b8 < 4; b8++) s->refs[b8] = inter ? r8[b8] : -1;
+  for (int b8 = 0; b8 < 4; b8++) {
+    s->refs[b8] = inter && !bm ? r8[b8] : used[b8][0] ? 0 : -1;
+    s->refs1[b8] = used[b8][1] ? 0 : -1;
+  }
   s->skip = (uint8_t)code->skip;
   s->kind = (uint8_t)code->kind;
   s->cbp = (uint8_t)code->cbp;
@@ -2081,10 +2522,21 @@ static void finish_mb(coder_t *c, const mbcode_t *code, int mbx, int mby) {
   info->skip = (uint8_t)code->skip;
   info->t8x8 = (uint8_t)code->t8x8;
   memcpy(info->mv, s->mvs, sizeof info->mv);
+  memcpy(info->mv1, s->mvs1, sizeof info->mv1);
   for (int b8 = 0; b8 < 4; b8++) {
-    info->ref_idx[b8] = inter ? r8[b8] : -1;
-    info->ref_id[b8] = inter ? 1 + r8[b8] : -1;
+    /* the reference index as the decoder's list gives it; the identity tells the loop filter which
+       picture it is (B pictures use one picture per list) */
+    const int r = bm ? 0 : r8[b8];
+    info->ref_idx[b8] = used[b8][0] ? (int8_t)r : -1;
+    info->ref_id[b8] = used[b8][0] ? (bm ? 1 : 1 + r) : -1;
+    info->ref_idx1[b8] = used[b8][1] ? 0 : -1;
+    info->ref_id1[b8] = used[b8][1] ? 2 : -1;
   }
+  info->bdirect = s->bdirect;
+  if (s->bdirect) info->direct8 = 15;
+  else if (bm && code->part == 3)
+    for (int k = 0; k < 4; k++)
+      if (code->bdir[k] == 0) info->direct8 |= (uint8_t)(1 << k);
 }
 
 static void export_code(const mbcode_t *c, ia_mbcode *o) {
@@ -2116,6 +2568,12 @@ static void export_code(const mbcode_t *c, ia_mbcode *o) {
       o->cdc[ci][k] = (int16_t)c->cdc[ci][k];
       for (int q = 0; q < 16; q++) o->cac[ci][k][q] = (int16_t)c->cac[ci][k][q];
     }
+  memcpy(o->bdir, c->bdir, 4);
+  for (int b8 = 0; b8 < 4; b8++) o->buse[b8] = (uint8_t)(c->buse[b8][0] | c->buse[b8][1] << 1);
+  for (int l = 0; l < 2; l++) {
+    for (int b = 0; b < 16; b++) { o->bmv[l][b][0] = c->bmv[l][b][0]; o->bmv[l][b][1] = c->bmv[l][b][1]; }
+    for (int k = 0; k < 4; k++) { o->bmvd[l][k][0] = (int16_t)c->bmvd[l][k][0]; o->bmvd[l][k][1] = (int16_t)c->bmvd[l][k][1]; }
+  }
 }
 
 /* ---------------------------------------------------------------------------------------------
@@ -2203,6 +2661,7 @@ typedef struct {
   uint16_t **src, **pic;
   refpic_t *refs;
   int np, qp, mb_w, mb_h, nslices, threads, first;
+  int b_slice, tb, td;
   mbstate_t *state;
   const ia_fsresult *fs;
   ia_mbcode *codes;
@@ -2224,7 +2683,10 @@ static void code_slice(const slice_job_t *j, int s, mbcode_t *code) {
   c.row1 = (s + 1) * j->mb_h / j->nslices;
   c.slice_id = s + 1;
   c.p_slice = j->np > 0;
-  c.nref = j->np;
+  c.nref = j->b_slice ? 1 : j->np;
+  c.b_slice = j->b_slice;
+  c.dist_tb = j->tb;
+  c.dist_td = j->td;
   c.plist = j->refs;
   c.sub8 = j->cfg->sub8x8 && j->qp <= 20;
   c.use_cabac = j->cfg->cabac;
@@ -2263,21 +2725,24 @@ static void *slice_worker(void *arg) {
   return NULL;
 }
 
-int or_analyse_picture_mt(const ia_config *cfg, const uint8_t *y, const uint8_t *cb, const uint8_t *cr, int qp,
-                          int kind, const uint8_t *const *ref_y, const uint8_t *const *ref_cb,
-                          const uint8_t *const *ref_cr, const ia_mbinfo *const *ref_mbs, int nref,
-                          const ia_fsresult *fs, ia_mbcode *codes, ia_mbstats *stats, uint8_t *recon_y,
-                          uint8_t *recon_cb, uint8_t *recon_cr, ia_mbinfo *mbs, int threads) {
+static int analyse_impl(const ia_config *cfg, const uint8_t *y, const uint8_t *cb, const uint8_t *cr, int qp, int kind,
+                        const uint8_t *const *ref_y, const uint8_t *const *ref_cb, const uint8_t *const *ref_cr,
+                        const ia_mbinfo *const *ref_mbs, int nref, int tb, int td, const ia_fsresult *fs,
+                        ia_mbcode *codes, ia_mbstats *stats, uint8_t *recon_y, uint8_t *recon_cb, uint8_t *recon_cr,
+                        ia_mbinfo *mbs, int threads) {
   const int mb_w = (cfg->width + 15) / 16, mb_h = (cfg->height + 15) / 16, pw = mb_w * 16, ph = mb_h * 16;
   const size_t ylen = (size_t)pw * ph, clen = ylen / 4;
   if (cfg->bit_depth != 8 || qp < 0 || qp > 51 || (kind == IA_PIC_P && (nref < 1 || nref > 16))) return -1;
+  /* B pictures: one picture per list, CABAC only (enc/mod.rs:534-536), 0 < tb < td */
+  if (kind == IA_PIC_B && (nref != 2 || !cfg->cabac || tb <= 0 || td <= tb)) return -1;
+  if (kind != IA_PIC_IDR && kind != IA_PIC_P && kind != IA_PIC_B) return -1;
   entropy_init();
   const int nslices = clampi(cfg->slices, 1, mb_h);
   threads = clampi(threads, 1, nslices);
   uint16_t *src[3] = {widen(y, ylen), widen(cb, clen), widen(cr, clen)};
   uint16_t *pic[3] = {calloc(ylen, 2), calloc(clen, 2), calloc(clen, 2)};
   refpic_t refs[16];
-  int np = kind == IA_PIC_P ? nref : 0;
+  int np = kind == IA_PIC_IDR ? 0 : nref;
   for (int r = 0; r < np; r++) {
     uint16_t *ly = widen(ref_y[r], ylen);
     refs[r].hp = malloc((size_t)4 * (pw + 2 * OR_PAD) * (ph + 2 * OR_PAD) * 2);
@@ -2294,7 +2759,8 @@ int or_analyse_picture_mt(const ia_config *cfg, const uint8_t *y, const uint8_t 
   if (threads > 64) threads = 64;
   int started = 0, rc = 0;
   for (int t = 0; t < threads; t++) {
-    jobs[t] = (slice_job_t){cfg, src, pic, refs, np, qp, mb_w, mb_h, nslices, threads, t, state, fs, codes, stats, mbs};
+    jobs[t] = (slice_job_t){cfg, src, pic, refs, np, qp, mb_w, mb_h, nslices, threads, t, kind == IA_PIC_B, tb, td,
+                            state, fs, codes, stats, mbs};
     if (threads == 1) {
       slice_worker(&jobs[t]);
     } else if (pthread_create(&tids[t], NULL, slice_worker, &jobs[t]) == 0) {
@@ -2315,6 +2781,24 @@ int or_analyse_picture_mt(const ia_config *cfg, const uint8_t *y, const uint8_t 
   for (int k = 0; k < 3; k++) { free(src[k]); free(pic[k]); }
   free(state);
   return rc;
+}
+
+int or_analyse_picture_mt(const ia_config *cfg, const uint8_t *y, const uint8_t *cb, const uint8_t *cr, int qp,
+                          int kind, const uint8_t *const *ref_y, const uint8_t *const *ref_cb,
+                          const uint8_t *const *ref_cr, const ia_mbinfo *const *ref_mbs, int nref,
+                          const ia_fsresult *fs, ia_mbcode *codes, ia_mbstats *stats, uint8_t *recon_y,
+                          uint8_t *recon_cb, uint8_t *recon_cr, ia_mbinfo *mbs, int threads) {
+  if (kind == IA_PIC_B) return -1;
+  return analyse_impl(cfg, y, cb, cr, qp, kind, ref_y, ref_cb, ref_cr, ref_mbs, nref, 0, 0, fs, codes, stats, recon_y,
+                      recon_cb, recon_cr, mbs, threads);
+}
+
+int or_analyse_picture_b(const ia_config *cfg, const uint8_t *y, const uint8_t *cb, const uint8_t *cr, int qp,
+                         const uint8_t *const *ref_y, const uint8_t *const *ref_cb, const uint8_t *const *ref_cr,
+                         const ia_mbinfo *const *ref_mbs, int tb, int td, ia_mbcode *codes, ia_mbstats *stats,
+                         uint8_t *recon_y, uint8_t *recon_cb, uint8_t *recon_cr, ia_mbinfo *mbs, int threads) {
+  return analyse_impl(cfg, y, cb, cr, qp, IA_PIC_B, ref_y, ref_cb, ref_cr, ref_mbs, 2, tb, td, NULL, codes, stats,
+                      recon_y, recon_cb, recon_cr, mbs, threads);
 }
 
 int or_analyse_picture(const ia_config *cfg, const uint8_t *y, const uint8_t *cb, const uint8_t *cr, int qp,
@@ -2352,10 +2836,10 @@ static int db_far(const int16_t *a, const int16_t *b) {
 /* Different references, a different number of vectors, or vectors a sample apart
  * (h264/deblock.rs:352-383); list 1 is unused in P pictures (reference -1, vector 0). */
 static int db_motion_differs(const ia_mbinfo *p, int pbx, int pby, const ia_mbinfo *q, int qbx, int qby) {
-  static const int16_t zero[2] = {0, 0};
   const int p8 = (pby >> 1) * 2 + (pbx >> 1), q8 = (qby >> 1) * 2 + (qbx >> 1);
-  const int32_t pr[2] = {p->ref_id[p8], -1}, qr[2] = {q->ref_id[q8], -1};
-  const int16_t *pm[2] = {p->mv[pby * 4 + pbx], zero}, *qm[2] = {q->mv[qby * 4 + qbx], zero};
+  const int32_t pr[2] = {p->ref_id[p8], p->ref_id1[p8]}, qr[2] = {q->ref_id[q8], q->ref_id1[q8]};
+  const int16_t *pm[2] = {p->mv[pby * 4 + pbx], p->mv1[pby * 4 + pbx]};
+  const int16_t *qm[2] = {q->mv[qby * 4 + qbx], q->mv1[qby * 4 + qbx]};
   const int np = (pr[0] >= 0) + (pr[1] >= 0), nq = (qr[0] >= 0) + (qr[1] >= 0);
   if (np != nq) return 1;
   if (np == 1) {
@@ -2370,13 +2854,17 @@ static int db_motion_differs(const ia_mbinfo *p, int pbx, int pby, const ia_mbin
   return (db_far(pm[0], qm[0]) || db_far(pm[1], qm[1])) && (db_far(pm[0], qm[1]) || db_far(pm[1], qm[0]));
 }
 
-/* Every 4x4 block with the macroblock's first vector and every 8x8 block with its first reference
- * (h264/deblock.rs:385-389, list 0; list 1 is uniformly unused). */
+/* Every 4x4 block with the macroblock's first vector and every 8x8 block with its first reference,
+ * in both lists (h264/deblock.rs:385-389). */
 static int db_uniform_motion(const ia_mbinfo *m) {
   for (int i = 1; i < 4; i++)
-    if (m->ref_idx[i] != m->ref_idx[0] || m->ref_id[i] != m->ref_id[0]) return 0;
+    if (m->ref_idx[i] != m->ref_idx[0] || m->ref_id[i] != m->ref_id[0] || m->ref_idx1[i] != m->ref_idx1[0] ||
+        m->ref_id1[i] != m->ref_id1[0])
+      return 0;
   for (int i = 1; i < 16; i++)
-    if (m->mv[i][0] != m->mv[0][0] || m->mv[i][1] != m->mv[0][1]) return 0;
+    if (m->mv[i][0] != m->mv[0][0] || m->mv[i][1] != m->mv[0][1] || m->mv1[i][0] != m->mv1[0][0] ||
+        m->mv1[i][1] != m->mv1[0][1])
+      return 0;
   return 1;
 }
 

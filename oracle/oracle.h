@@ -122,6 +122,16 @@ int or_analyse_picture_mt(const ia_config *cfg, const uint8_t *y, const uint8_t 
                           const ia_fsresult *fs, ia_mbcode *codes, ia_mbstats *stats, uint8_t *recon_y,
                           uint8_t *recon_cb, uint8_t *recon_cr, ia_mbinfo *mbs, int threads);
 
+/* One B picture's analysis (enc/frame.rs:1518-1780 per macroblock, the slices of code_picture with
+ * `with_b`, enc/mod.rs:551-566,619-621): ref_*[0] is the list 0 picture (the nearest reference shown
+ * before it), ref_*[1] the list 1 picture (the nearest shown after it; its records, ref_mbs[1], give
+ * the co-located motion); tb and td its picture-order distances from the list 0 picture to this one
+ * and to the list 1 picture. CABAC only. Returns 0, or -1 on an invalid argument. */
+int or_analyse_picture_b(const ia_config *cfg, const uint8_t *y, const uint8_t *cb, const uint8_t *cr, int qp,
+                         const uint8_t *const *ref_y, const uint8_t *const *ref_cb, const uint8_t *const *ref_cr,
+                         const ia_mbinfo *const *ref_mbs, int tb, int td, ia_mbcode *codes, ia_mbstats *stats,
+                         uint8_t *recon_y, uint8_t *recon_cb, uint8_t *recon_cr, ia_mbinfo *mbs, int threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -83,7 +83,9 @@ class Decoder:
         out = out[:n]
         return {"i4_modes": out[0:9], "i8_modes": out[9:18], "i16_modes": out[18:22], "chroma_modes": out[22:26],
                 "partitions": out[26:30], "sub_partitions": out[30:34], "skip": int(out[34]), "pcm": int(out[35]),
-                "inter_t8": int(out[36]), "ref_gt0": int(out[37])}
+                "inter_t8": int(out[36]), "ref_gt0": int(out[37]), "b_skip": int(out[38]), "b_direct16": int(out[39]),
+                "b16": out[40:43], "b_partitions": out[43:45], "b8x8": int(out[45]), "b_sub": out[46:59],
+                "b_temporal": int(out[59]), "b_implicit": int(out[60]), "b_explicit": int(out[61])}
 
     def frames(self):
         self._ok(self.L.od_flush(self.h))

@@ -58,6 +58,8 @@ def lib():
                                        vp, vp]),
             "or_analyse_picture_mt": (i, [C.POINTER(abi.Config), vp, vp, vp, i, i, vp, vp, vp, vp, i, vp, vp, vp, vp,
                                           vp, vp, vp, i]),
+            "or_analyse_picture_b": (i, [C.POINTER(abi.Config), vp, vp, vp, i, vp, vp, vp, vp, i, i, vp, vp, vp, vp,
+                                         vp, vp, i]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -136,6 +138,31 @@ def analyse_picture(cfg, pic, qp, kind, refs=(), fs=None, threads=1):
                                      arr([r.cr for r, _ in refs]), arr(ref_mbs) if refs else None, len(refs),
                                      P(fs) if fs is not None else None, P(codes), P(stats), P(ry), P(rcb), P(rcr),
                                      P(mbs), threads)
+    if rc != 0:
+        raise ValueError("oracle rejected the arguments")
+    return codes, stats, (ry, rcb, rcr), mbs
+
+
+def analyse_picture_b(cfg, pic, qp, l0, l1, tb, td, threads=1):
+    """A B picture's analysis: l0 and l1 are (Picture, mbinfo array or None) of the list 0 and list 1
+    pictures, tb/td the picture-order distances from the list 0 picture to this one and to the list 1
+    picture. Returns (codes, stats, (ry, rcb, rcr), mbs)."""
+    mb_w, mb_h = (cfg.width + 15) // 16, (cfg.height + 15) // 16
+    n = mb_w * mb_h
+    codes = np.zeros(n, dtype=abi.MBCODE_DTYPE)
+    stats = np.zeros(n, dtype=abi.MBSTATS_DTYPE)
+    mbs = np.zeros(n, dtype=abi.MBINFO_DTYPE)
+    ry, rcb, rcr = np.empty_like(pic.y), np.empty_like(pic.cb), np.empty_like(pic.cr)
+    refs = (l0, l1)
+    ref_mbs = [None if m is None else np.ascontiguousarray(m, dtype=abi.MBINFO_DTYPE) for _, m in refs]
+
+    def arr(items):
+        return (C.c_void_p * 2)(*[x.ctypes.data if x is not None else None for x in items])
+
+    ay, acb, acr, am = arr([r.y for r, _ in refs]), arr([r.cb for r, _ in refs]), arr([r.cr for r, _ in refs]), arr(ref_mbs)
+    rc = lib().or_analyse_picture_b(C.byref(cfg), P(pic.y), P(pic.cb), P(pic.cr), qp, C.cast(ay, C.c_void_p),
+                                    C.cast(acb, C.c_void_p), C.cast(acr, C.c_void_p), C.cast(am, C.c_void_p), tb, td,
+                                    P(codes), P(stats), P(ry), P(rcb), P(rcr), P(mbs), threads)
     if rc != 0:
         raise ValueError("oracle rejected the arguments")
     return codes, stats, (ry, rcb, rcr), mbs

@@ -12,7 +12,7 @@ import shutil
 import sys
 
 NAMES = ["i_cabac", "i_high8", "i_slices", "i_crop", "i_cqm", "pcm_cabac", "c_8x8dct", "c_odd", "c_parts", "c_ref4",
-         "c_slices", "c_weight", "ci_pc"]
+         "c_slices", "c_weight", "ci_pc", "b_sp_cabac", "b_tm_cabac", "b_pyr", "b_wimp", "ci_b"]
 
 
 def main():

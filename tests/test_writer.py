@@ -128,3 +128,83 @@ def test_extreme_qps(oracle):
 def test_cavlc_is_refused():
     with pytest.raises(NotImplementedError):
         h264w.Stream(64, 48, cabac=False)
+
+
+def b_chain_roundtrip(oracle, w, h, nframes, qp, *, nrefs=3, slices=1, transform8=True, idr=None, seed=1, **clip):
+    """An IBBP chain as the reference's encoder codes it with two B pictures (enc/mod.rs:700-731):
+    pictures in coding order (h264w.coding_order), P pictures against the references kept (newest
+    first), B pictures against the nearest reference shown before and after them, QP offsets -3 / 0 /
+    +5 (enc/mod.rs:36); each analysed by the oracle, written by the back end, loop-filtered; the stream
+    decoded by the oracle's decoder must give every picture (B pictures too) and its records."""
+    from oracle import pydecoder
+
+    pics = frames(w, h, nframes, seed, **clip)
+    cfg = abi.Config(w, h, 8, 1, int(transform8), 0, 0, nrefs, slices, abi.SEARCH_DIAMOND, 16, 1.1)
+    mb_w, mb_h = (w + 15) // 16, (h + 15) // 16
+    idr = idr if idr is not None else [d == 0 for d in range(nframes)]
+    order = h264w.coding_order(idr, 2)
+    ref_count = max(nrefs, 2)
+    refs, want, stream, model_bits = [], {}, bytearray(), []
+    with h264w.Stream(w, h, nrefs=nrefs, slices=slices, transform8=transform8, bframes=2) as wr:
+        for d, kind in order:
+            pic = pics[d]
+            if kind == abi.PIC_IDR:
+                refs.clear()
+                codes, stats, rec, mbs = oracle.analyse_picture(cfg, pic, qp - 3, kind)
+                au = wr.write_at(kind, qp - 3, codes, d)
+            elif kind == abi.PIC_P:
+                plist = refs[::-1][:nrefs]
+                codes, stats, rec, mbs = oracle.analyse_picture(cfg, pic, qp, kind, [(r[1], r[2]) for r in plist])
+                au = wr.write_at(kind, qp, codes, d, len(plist))
+            else:
+                l0 = max((r for r in refs if r[0] < d), key=lambda r: r[0])
+                l1 = min((r for r in refs if r[0] > d), key=lambda r: r[0])
+                tb, td = 2 * (d - l0[0]), 2 * (l1[0] - l0[0])
+                codes, stats, rec, mbs = oracle.analyse_picture_b(cfg, pic, min(qp + 5, 51), (l0[1], l0[2]),
+                                                                  (l1[1], l1[2]), tb, td)
+                au = wr.write_at(kind, min(qp + 5, 51), codes, d)
+            stream += au
+            model_bits.append((d, kind, len(au) * 8, int(stats["bits"].sum()) / 256.0))
+            filt = oracle.deblock_picture(rec[0], rec[1], rec[2], mbs, mb_w, mb_h)
+            want[d] = (filt, codes, mbs, kind)
+            if kind != abi.PIC_B:
+                if len(refs) == ref_count:
+                    refs.pop(0)
+                refs.append((d, Picture(*filt), mbs))
+    got = pydecoder.decode_annexb(bytes(stream))
+    assert len(got) == nframes
+    for d, f in enumerate(got):
+        filt, codes, mbs, kind = want[d]
+        for k in range(3):
+            assert np.array_equal(f.full[k], filt[k]), f"picture {d} (kind {kind}): decoded plane {k} differs"
+        flds = ("kind", "cbp", "skip", "t8x8", "mv", "ref_idx") + (("mv1", "ref_idx1", "bdirect", "direct8") if kind == abi.PIC_B else ())
+        for fld in flds:
+            assert np.array_equal(f.mbs[fld], mbs[fld]), f"picture {d} (kind {kind}): decoded {fld} differs"
+    for d, kind, actual, model in model_bits:
+        assert abs(actual - model) <= 0.02 * model + 128 * slices + (400 if kind == abi.PIC_IDR else 0), (d, kind, actual, model)
+    return got, want
+
+
+def test_coding_order_with_b_pictures():
+    """enc/mod.rs:1110-1159 with two B pictures: IDR pictures at 0, 3 (forced) and 8 (keyint 5); every
+    picture is coded once, each B picture after the P or IDR picture shown next."""
+    idr = [d in (0, 3, 8) for d in range(12)]
+    order = h264w.coding_order(idr, 2)
+    assert sorted(d for d, _ in order) == list(range(12))
+    assert [d for d, k in order if k == abi.PIC_IDR] == [0, 3, 8]
+    assert order == [(0, 0), (2, 1), (1, 2), (3, 0), (6, 1), (4, 2), (5, 2), (7, 1), (8, 0), (11, 1), (9, 2), (10, 2)]
+    assert h264w.coding_order([True] + [False] * 6, 0) == [(0, 0)] + [(d, 1) for d in range(1, 7)]
+
+
+@pytest.mark.parametrize("qp,slices,nrefs", [(24, 1, 3), (30, 2, 1), (36, 1, 2)])
+def test_oracle_b_chain_decodes_to_the_reconstruction(oracle, qp, slices, nrefs):
+    got, want = b_chain_roundtrip(oracle, 96, 64, 8, qp, nrefs=nrefs, slices=slices, seed=qp + 100)
+    # the B pictures used each coding form
+    codes = np.concatenate([want[d][1] for d in want if want[d][3] == abi.PIC_B])
+    inter = codes[codes["kind"] == abi.KIND_INTER]
+    assert (inter["skip"] == 1).any()
+
+
+def test_b_chain_main_profile_and_cropped(oracle):
+    b_chain_roundtrip(oracle, 40, 26, 6, 26, nrefs=1, slices=2, transform8=False, seed=7)
+    b_chain_roundtrip(oracle, 64, 48, 7, 20, nrefs=2, slices=1, seed=9, max_motion=6)
