@@ -211,6 +211,20 @@ int ia_read_motion(ia_ctx *ctx, ia_fsresult *out);
  * Asynchronous on the context's stream. */
 int ia_analyse_uploaded(ia_ctx *ctx, int qp, int kind, const int *ref_slots, int nref);
 
+/* Analyses the uploaded picture as a B picture: `decide_b` for every macroblock of every slice
+ * (enc/frame.rs:1518-1780: B_Skip, B_Direct_16x16, list 0 / list 1 / bi-predicted 16x16, 16x8, 8x16
+ * and 8x8 partitionings with B_Direct_8x8 blocks, Intra16x16, Intra4x4, by rate-distortion cost with
+ * the CABAC bit model, spatial direct prediction). Replaces the slice coders of `code_picture` for a
+ * picture of kind 2 built `.with_b(l1, dist)` (enc/mod.rs:551-566, 619-621, enc/frame.rs:442-447):
+ * l0_slot holds the nearest reference shown before the picture, l1_slot the nearest shown after it
+ * (its records give the co-located motion), dist_l0 and dist_l1 their picture-order distances from
+ * the list 0 picture to this one and to the list 1 picture (0 < dist_l0 < dist_l1). CABAC only (the
+ * reference codes B pictures only with CABAC, enc/mod.rs:534-536). The search mode's exhaustive or
+ * learned candidates apply to P pictures; B pictures use the reference's candidates. A B picture is
+ * not a reference: ia_promote_reference refuses it. The context needs nrefs >= 2 (the reference keeps
+ * two references when it codes B pictures, enc/mod.rs:287-289). Asynchronous on the context's stream. */
+int ia_analyse_b(ia_ctx *ctx, int qp, int l0_slot, int l1_slot, int dist_l0, int dist_l1);
+
 /* Copies the results of the last analysis. Any output pointer may be NULL. Synchronous. */
 int ia_read_analysis(ia_ctx *ctx, ia_mbcode *codes, ia_mbstats *stats, uint8_t *recon_y, uint8_t *recon_cb,
                      uint8_t *recon_cr, ia_mbinfo *mbs);

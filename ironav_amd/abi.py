@@ -94,6 +94,7 @@ PROTOTYPES = {
     "ia_motion_search": (C.c_int, [vp, ip, C.c_int]),
     "ia_read_motion": (C.c_int, [vp, vp]),
     "ia_analyse_uploaded": (C.c_int, [vp, C.c_int, C.c_int, ip, C.c_int]),
+    "ia_analyse_b": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "ia_read_analysis": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
     "ia_analyse_picture": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, ip, C.c_int, vp, vp, vp, vp, vp, vp]),
     "ia_sync": (C.c_int, [vp]),

@@ -6,20 +6,26 @@
 
 namespace ia {
 
-// What later macroblocks of a slice need to know about a coded one (enc/frame.rs:239-272), P/I subset.
+// What later macroblocks of a slice need to know about a coded one (enc/frame.rs:239-272).
 struct MbState {
   uint8_t i4, inter, t8x8, skip, kind, cbp, chroma_mode, cbf_dc;
   uint8_t modes[16], nnz[16], cnnz[2][4];
   int16_t mvs[16][2];
   int8_t refs[4];
   uint8_t mvd[16][2];
+  // list 1 of a B macroblock (vectors, reference index per 8x8 block, -1 unused; |mvd|), and whether
+  // it is B_Skip or B_Direct_16x16
+  int16_t mvs1[16][2];
+  int8_t refs1[4];
+  uint8_t mvd1[16][2];
+  uint8_t bdirect, pad[3];
 };
 
 struct AnParams {
   const uint8_t *src[3];        // padded source planes
   const uint8_t *hp[16];        // half-sample planes of each list-0 reference
   const uint8_t *rcb[16], *rcr[16];
-  const ia_mbinfo *rmbs0;       // macroblock records of reference 0 (co-located vectors), may be null
+  const ia_mbinfo *rmbs0;       // records of the co-located picture (P: reference 0; B: the list 1 picture), may be null
   const ia_fsresult *fs;        // exhaustive-search winners [nref][mb][IA_NPART], or null
   uint8_t *rec[3];              // reconstruction before deblocking
   ia_mbcode *codes;
@@ -33,6 +39,8 @@ struct AnParams {
   int qpc;                      // QP'C
   int lambda;
   int p_slice, sub8, use_cabac, transform8;
+  int b_slice;                  // B picture: hp/rcb/rcr [0] the list 0 picture, [1] the list 1 picture
+  int dist_tb, dist_td;         // B: picture-order distances list 0 -> this picture, list 0 -> list 1
   double lam2;                  // 1.6 * 2^((QP'Y - 12) / 3)
   double skip_thresh;           // 256 * 0.625 * 2^(QP'Y / 6)
   int fast;                     // realtime decisions (enc/frame.rs:348)

@@ -20,6 +20,8 @@
 // J = SSD + lam2 * bits / 256) is computed without contraction (the file is built with
 // -ffp-contract=off), with the powers of two computed on the host by the C library as the
 // reference computes them.
+#include <cstddef>
+
 #include "analyse.h"
 
 namespace ia {
@@ -133,6 +135,25 @@ struct SearchLds {
   int8_t st_refs[4][16];
   SearchRes res[4];
 };
+// A B macroblock's working state: the list 1 prediction (averaged with list 0's in L.pred), each
+// group's search result, the SATD of each 4x4 block of a prediction.
+struct BLds {
+  uint8_t p1[256];
+  uint8_t c1[2][64];
+  int16_t mv[4][2];
+  int32_t cost[4];
+  int32_t sat[16];
+};
+
+// A candidate's chroma coding (mode, nonzero counts, levels, coded pattern), which the rate-distortion
+// decision codes once for the intra candidates; the nonzero counts and levels as consecutive words.
+struct ChromaSave {
+  uint8_t cnnz[2][4];
+  int16_t cdc[2][4];
+  int16_t cac[2][4][16];
+  uint8_t mode, cbp;
+};
+static_assert(offsetof(ChromaSave, cdc) == 8 && offsetof(ChromaSave, cac) == 24, "the chroma save's words");
 
 struct Lds {
   uint8_t src_y[256];
@@ -145,13 +166,17 @@ struct Lds {
   union {
     uint8_t tmp[1024];  // a copy counted on for a rate estimate
     SearchLds ms;       // the motion search's per-group state (not live at the same time)
+    BLds bs;            // a B macroblock's predictions and search results (not live at the same time)
   };
   uint32_t cab[256];    // per (state, LPS?): bit cost in 1/256 bits << 8 | next state (enc/cabac.rs:32-40,162-173)
   MbState nb[4];        // left, top, top-right, top-left neighbours (valid when nb_ok)
   int nb_ok[4];
   ia_mbcode code;       // the candidate being coded
-  ia_mbcode keep;       // the candidate kept by the rate-distortion decision
-  ia_mbcode i4;         // Intra4x4 coded ahead of the SATD decision (CAVLC)
+  union {
+    ia_mbcode keep;     // the candidate kept by the rate-distortion decision (CABAC)
+    ia_mbcode i4;       // Intra4x4 coded ahead of the SATD decision (CAVLC)
+  };
+  ChromaSave csave;     // the chroma coding the intra candidates share (CABAC)
   uint8_t keep_rec[384];
   uint8_t pred[256];
   uint8_t cpred[2][64];
@@ -198,8 +223,10 @@ struct Coder {
     return L.nb_ok[k] ? &L.nb[k] : nullptr;
   }
 
+  // list l's reference index and vector of the 4x4 block (x, y) relative to the macroblock
+  // (enc/frame.rs:521-539); 0 when it is not available
   __device__ int nb_motion(const int16_t (*cur)[2], const int8_t *cref, uint32_t done, int x, int y, int *r,
-                           int16_t *mv) const {
+                           int16_t *mv, int l = 0) const {
     int dx, dy, bx, by;
     if (x < 0 && y < 0) { dx = -1; dy = -1; bx = 3; by = 3; }
     else if (x < 0) { dx = -1; dy = 0; bx = 3; by = y; }
@@ -215,20 +242,25 @@ struct Coder {
     }
     const MbState *s = nb(dx, dy);
     if (!s) return 0;
-    int rr = s->refs[(by / 2) * 2 + bx / 2];
-    if (s->inter && rr >= 0) { *r = rr; mv[0] = s->mvs[by * 4 + bx][0]; mv[1] = s->mvs[by * 4 + bx][1]; }
+    // the list's arrays through pointers (g++ 11 with -fsanitize=undefined, the host build that checks
+    // this kernel, subscripts `(l ? a : b)[i]` of two arrays wrongly)
+    const int8_t *refs = l ? s->refs1 : s->refs;
+    const int16_t(*mvs)[2] = l ? s->mvs1 : s->mvs;
+    int rr = refs[(by / 2) * 2 + bx / 2];
+    const int16_t *m = mvs[by * 4 + bx];
+    if (s->inter && rr >= 0) { *r = rr; mv[0] = m[0]; mv[1] = m[1]; }
     else { *r = -1; mv[0] = mv[1] = 0; }
     return 1;
   }
 
   __device__ void pred_mv(const int16_t (*cur)[2], const int8_t *cref, uint32_t done, int bx, int by, int w, int dir,
-                          int rf, int16_t *out) const {
+                          int rf, int16_t *out, int l = 0) const {
     int ra = -1, rb = -1, rc = -1;
     int16_t ma[2] = {0, 0}, mb[2] = {0, 0}, mc[2] = {0, 0};
-    int ha = nb_motion(cur, cref, done, bx - 1, by, &ra, ma);
-    int hb = nb_motion(cur, cref, done, bx, by - 1, &rb, mb);
-    int hc = nb_motion(cur, cref, done, bx + w, by - 1, &rc, mc);
-    if (!hc) hc = nb_motion(cur, cref, done, bx - 1, by - 1, &rc, mc);
+    int ha = nb_motion(cur, cref, done, bx - 1, by, &ra, ma, l);
+    int hb = nb_motion(cur, cref, done, bx, by - 1, &rb, mb, l);
+    int hc = nb_motion(cur, cref, done, bx + w, by - 1, &rc, mc, l);
+    if (!hc) hc = nb_motion(cur, cref, done, bx - 1, by - 1, &rc, mc, l);
     if (!ha) { ra = -1; ma[0] = ma[1] = 0; }
     if (!hb) { rb = -1; mb[0] = mb[1] = 0; }
     if (!hc) { rc = -1; mc[0] = mc[1] = 0; }
@@ -251,10 +283,10 @@ struct Coder {
     out[1] = median3(ma[1], mb[1], mc[1]);
   }
 
-  __device__ void mv_pred(int rf, int16_t *out) const {
+  __device__ void mv_pred(int rf, int16_t *out, int l = 0) const {
     const int16_t z[16][2] = {};
     const int8_t nr[16] = {};
-    pred_mv(z, nr, 0, 0, 0, 4, 0, rf, out);
+    pred_mv(z, nr, 0, 0, 0, 4, 0, rf, out, l);
   }
 
   __device__ static int rects(int part, const uint8_t *sub, Rect *out) {
@@ -488,9 +520,10 @@ struct Coder {
   // any lane's predicate holds (the same in every lane)
   __device__ static bool any_lane(bool p) { return __ballot(p) != 0; }
 
+  // list l (B slices); r the reference index in it; pi the half-sample planes searched
   __device__ int motion_search(int r, int bx, int by, int bw, int bh, const int16_t *mvp, const int16_t (*extra)[2],
-                               int nextra, int16_t *out) {
-    const int lam = P.lambda;
+                               int nextra, int16_t *out, int l = 0) {
+    const int lam = P.lambda, pi = P.b_slice ? l : r;
     const int x0 = mbx * 16 + bx * 4, y0 = mby * 16 + by * 4, w = bw * 4, h = bh * 4;
     const int lo_x = max(-(x0 + 24) * 4, -2048), hi_x = min((P.mb_w * 16 - x0 + 8) * 4, 2047);
     const int lo_y = max(-(y0 + 24) * 4, -512), hi_y = min((P.mb_h * 16 - y0 + 8) * 4, 511);
@@ -512,11 +545,11 @@ struct Coder {
       for (int k = 0; k < 3; k++) {
         int rr;
         int16_t mv[2];
-        if (nb_motion(z, nr, 0, NB[k][0], NB[k][1], &rr, mv) && rr == r) push(mv[0], mv[1]);
+        if (nb_motion(z, nr, 0, NB[k][0], NB[k][1], &rr, mv, l) && rr == r) push(mv[0], mv[1]);
       }
     }
     for (int k = 0; k < nextra; k++) push(extra[k][0], extra[k][1]);
-    if (P.fs) {
+    if (P.fs && !P.b_slice) {
       const ia_fsresult *f = &P.fs[((size_t)r * P.mb_w * P.mb_h + (size_t)mby * P.mb_w + mbx) * IA_NPART + fs_index(bx, by, bw, bh)];
       push(f->mv[0], f->mv[1]);
     }
@@ -525,7 +558,7 @@ struct Coder {
     for (int k = 0; any_lane(k < nc); k++) {
       const bool on = k < nc;
       const int cx = on ? cands[k][0] : 0, cy = on ? cands[k][1] : 0;
-      const int cost = sad_g(r, bxp, byp, w, h, cx, cy) + lam * mvd_bits(cx, cy, mvp[0], mvp[1]);
+      const int cost = sad_g(pi, bxp, byp, w, h, cx, cy) + lam * mvd_bits(cx, cy, mvp[0], mvp[1]);
       if (on && cost < best_cost) { bestx = cx; besty = cy; best_cost = cost; }
     }
     // the realtime decisions skip the 16-sample step (enc/frame.rs:973)
@@ -537,7 +570,7 @@ struct Coder {
         for (int d = 0; d < 4; d++) {
           int dx = d == 0 ? -step : d == 1 ? step : 0, dy = d == 2 ? -step : d == 3 ? step : 0;
           int cx = clampi(bestx + dx, lo_x, hi_x), cy = clampi(besty + dy, lo_y, hi_y);
-          int cost = sad_g(r, bxp, byp, w, h, cx, cy) + lam * mvd_bits(cx, cy, mvp[0], mvp[1]);
+          int cost = sad_g(pi, bxp, byp, w, h, cx, cy) + lam * mvd_bits(cx, cy, mvp[0], mvp[1]);
           if (going && cost < best_cost) { bestx = cx; besty = cy; best_cost = cost; moved = 1; }
         }
         if (!moved) going = false;
@@ -567,7 +600,7 @@ struct Coder {
         cand[n][1] = clampi(cy0 + ddy * step, lo_y, hi_y);
         n++;
       }
-      cost_sub_g(r, x0, y0, bxp, byp, w, h, cand, n, mvp, cost);
+      cost_sub_g(pi, x0, y0, bxp, byp, w, h, cand, n, mvp, cost);
       if (!stop) {
         int k0 = 0;
         if (pass == 0) {
@@ -1841,6 +1874,16 @@ struct Coder {
     }
   }
 
+  // The mb_type of a B macroblock (Table 7-14; enc/frame.rs:2343-2351)
+  __device__ static int b_mb_type(const ia_mbcode &code) {
+    const uint8_t PAIRS[9][2] = {{1, 1}, {2, 2}, {1, 2}, {2, 1}, {1, 3}, {2, 3}, {3, 1}, {3, 2}, {3, 3}};
+    if (code.part == 0) return code.bdir[0];
+    if (code.part == 3) return 22;
+    int i = 0;
+    while (i < 8 && !(PAIRS[i][0] == code.bdir[0] && PAIRS[i][1] == code.bdir[1])) i++;
+    return 4 + 2 * i + (code.part == 2);
+  }
+
   // mb_skip_flag, mb_type, transform_size_8x8_flag, intra_chroma_pred_mode, sub_mb_type, ref_idx,
   // coded_block_pattern and mb_qp_delta (enc/frame.rs:2028-2110, :2150-2200)
   __device__ uint32_t header_bins(uint8_t *st) const {
@@ -1849,18 +1892,58 @@ struct Coder {
     const int kind = code.kind;
     uint32_t bits = 0;
     if (P.p_slice) {
-      dec_at(st, 11 + (left && !left->skip) + (top && !top->skip), code.skip, bits);
+      dec_at(st, (P.b_slice ? 24 : 11) + (left && !left->skip) + (top && !top->skip), code.skip, bits);
       if (code.skip) return bits;
     }
-    const int cbpl = code.cbp & 15, cbpc = code.cbp >> 4, prefix_ok = P.p_slice, base = P.p_slice ? 17 : 3;
-    if (kind == K_INTER) {
+    const int cbpl = code.cbp & 15, cbpc = code.cbp >> 4, prefix_ok = P.p_slice;
+    const int base = P.b_slice ? 32 : P.p_slice ? 17 : 3;
+    const int b_ctx = 27 + (left && !left->bdirect) + (top && !top->bdirect);
+    if (kind == K_INTER && P.b_slice) {
+      // B_Direct_16x16 0, B_L0_16x16 100, B_L1_16x16 101, the others 11 and four or five more bins
+      // (Table 9-37); sub_mb_type B_Direct_8x8 0, B_L0_8x8 100, B_L1_8x8 101, B_Bi_8x8 11000
+      const int t = b_mb_type(code);
+      dec_at(st, b_ctx, t != 0, bits);
+      if (t == 1 || t == 2) {
+        dec_at(st, 30, 0, bits);
+        dec_at(st, 32, t == 2, bits);
+      } else if (t != 0) {
+        dec_at(st, 30, 1, bits);
+        const int v = t >= 3 && t <= 10 ? t - 3 : t == 11 ? 14 : t == 22 ? 15 : (t + 4) >> 1;
+        dec_at(st, 31, (v >> 3) & 1, bits);
+        for (int sh = 2; sh >= 0; sh--) dec_at(st, 32, (v >> sh) & 1, bits);
+        if (t >= 12 && t <= 21) dec_at(st, 32, (t + 4) & 1, bits);
+      }
+      if (t == 22)
+        for (int k = 0; k < 4; k++) {
+          const int d = code.bdir[k];
+          dec_at(st, 36, d != 0, bits);
+          if (d == 1 || d == 2) {
+            dec_at(st, 37, 0, bits);
+            dec_at(st, 39, d == 2, bits);
+          } else if (d == 3) {
+            dec_at(st, 37, 1, bits);
+            dec_at(st, 38, 0, bits);
+            dec_at(st, 39, 0, bits);
+            dec_at(st, 39, 0, bits);
+          }
+        }
+    } else if (kind == K_INTER) {
       const int b1 = code.part == 1 || code.part == 2, b2 = code.part == 1 || code.part == 3;
       dec_at(st, 14, 0, bits);
       dec_at(st, 15, b1, bits);
       dec_at(st, b1 ? 17 : 16, b2, bits);
     } else {
       int first;
-      if (prefix_ok) {
+      if (P.b_slice) {
+        // 111101: an intra type follows, with the suffix contexts
+        dec_at(st, b_ctx, 1, bits);
+        dec_at(st, 30, 1, bits);
+        dec_at(st, 31, 1, bits);
+        dec_at(st, 32, 1, bits);
+        dec_at(st, 32, 0, bits);
+        dec_at(st, 32, 1, bits);
+        first = 32;
+      } else if (prefix_ok) {
         dec_at(st, 14, 1, bits);
         first = 17;
       } else {
@@ -1889,7 +1972,7 @@ struct Coder {
         if (m > 1) dec_at(st, 67, m > 2, bits);
       }
     } else {
-      if (code.part == 3) {
+      if (!P.b_slice && code.part == 3) {
         for (int k = 0; k < 4; k++) {
           const int s = code.sub[k];
           dec_at(st, 21, s == 0, bits);
@@ -1899,7 +1982,7 @@ struct Coder {
           }
         }
       }
-      if (P.nref > 1) {
+      if (!P.b_slice && P.nref > 1) {
         uint32_t grid = 0;  // reference index per 8x8 block so far, a byte each
         const int np = code.part == 3 ? 4 : NPARTS[code.part];
         for (int k = 0; k < np; k++) {
@@ -1980,21 +2063,24 @@ struct Coder {
     return bits;
   }
 
-  // one component of the motion vector differences, partitions in coding order (enc/frame.rs:2110-2150);
-  // min(|mvd|, 64) of the 4x4 blocks coded so far a byte each in two registers
+  // one component of the motion vector differences, partitions in coding order, list 0's then list 1's
+  // in a B slice (enc/frame.rs:2110-2150; the two lists share the component's contexts); min(|mvd|, 64)
+  // of the 4x4 blocks coded so far a byte each in two registers
   __device__ uint32_t mvd_comp_bins(uint8_t *st, int comp) const {
     const ia_mbcode &code = L.code;
     if (code.kind != K_INTER || code.skip) return 0;
     const MbState *left = nb(-1, 0), *top = nb(0, -1);
-    const int base = comp == 0 ? 40 : 47, part = code.part;
-    uint64_t g[2] = {0, 0};
+    const int base = comp == 0 ? 40 : 47, part = code.part, bsl = P.b_slice;
     uint32_t bits = 0;
+    for (int l = 0; l < (bsl ? 2 : 1); l++) {
+    uint64_t g[2] = {0, 0};
     int k = 0;
-    for (int q = 0; q < (part == 3 ? 4 : NPARTS[part]); q++) {
-      const int sub = part == 3 ? code.sub[q] : 0;
-      for (int s = 0; s < (part == 3 ? NSUBS[sub] : 1); s++, k++) {
+    for (int q = 0; q < (part == 3 && !bsl ? 4 : NPARTS[part]); q++) {
+      const int sub = part == 3 && !bsl ? code.sub[q] : 0;
+      for (int s = 0; s < (part == 3 && !bsl ? NSUBS[sub] : 1); s++, k++) {
+        if (bsl && !((code.bdir[q] >> l) & 1)) continue;
         int bx, by, w, h;
-        if (part == 3) {
+        if (part == 3 && !bsl) {
           bx = (q & 1) * 2 + SUBS[sub][s][0];
           by = (q >> 1) * 2 + SUBS[sub][s][1];
           w = SUBS[sub][s][2];
@@ -2006,10 +2092,12 @@ struct Coder {
           h = PARTS[part][q][3];
         }
         auto at = [&](int i) { return (int)((g[i >> 3] >> (8 * (i & 7))) & 0xff); };
-        const int a = bx > 0 ? at(by * 4 + bx - 1) : (left ? left->mvd[by * 4 + 3][comp] : 0);
-        const int bb = by > 0 ? at((by - 1) * 4 + bx) : (top ? top->mvd[12 + bx][comp] : 0);
+        const uint8_t(*lm)[2] = left ? (l ? left->mvd1 : left->mvd) : nullptr;
+        const uint8_t(*tm)[2] = top ? (l ? top->mvd1 : top->mvd) : nullptr;
+        const int a = bx > 0 ? at(by * 4 + bx - 1) : (lm ? lm[by * 4 + 3][comp] : 0);
+        const int bb = by > 0 ? at((by - 1) * 4 + bx) : (tm ? tm[12 + bx][comp] : 0);
         const uint32_t sum = (uint32_t)(a + bb);
-        const int d = code.mvds[k][comp];
+        const int d = bsl ? code.bmvd[l][k][comp] : code.mvds[k][comp];
         const uint32_t ad = (uint32_t)iabs(d);
         dec_at(st, base + (sum < 3 ? 0 : (sum > 32 ? 2 : 1)), ad > 0, bits);
         if (ad > 0) {
@@ -2030,6 +2118,7 @@ struct Coder {
             else g[1] = (g[1] & ~(0xffull << sh)) | (v << sh);
           }
       }
+    }
     }
     return bits;
   }
@@ -2270,34 +2359,34 @@ struct Coder {
     sync();
   }
 
-  // The chroma of a candidate (mode, levels, nonzero counts, coded pattern) kept in L.i4, which the
-  // rate-distortion decision does not otherwise use: every intra candidate codes the same chroma
-  // (enc/frame.rs:1807-1852 depends on the chroma samples and edges only), so the candidates after
-  // the first take it from there; their reconstructed chroma is already in the canvas.
-  static_assert(offsetof(ia_mbcode, cac) == offsetof(ia_mbcode, cdc) + 16 && offsetof(ia_mbcode, bdir) == offsetof(ia_mbcode, cac) + 256,
-                "the chroma levels end the levels, DC first");
+  // The chroma of a candidate (mode, levels, nonzero counts, coded pattern) kept in L.csave: every
+  // intra candidate codes the same chroma (enc/frame.rs:1807-1852 depends on the chroma samples and
+  // edges only), so the candidates after the first take it from there; their reconstructed chroma is
+  // already in the canvas.
+  static_assert(offsetof(ia_mbcode, cac) == offsetof(ia_mbcode, cdc) + 16 && offsetof(ia_mbcode, cnnz) % 4 == 0 &&
+                offsetof(ia_mbcode, cdc) % 4 == 0, "the chroma levels as words, DC first");
   __device__ void save_chroma() {
     const uint32_t *s = reinterpret_cast<const uint32_t *>(&L.code);
-    uint32_t *d = reinterpret_cast<uint32_t *>(&L.i4);
+    uint32_t *d = reinterpret_cast<uint32_t *>(&L.csave);
     constexpr int cnnz = offsetof(ia_mbcode, cnnz) / 4, cdc = offsetof(ia_mbcode, cdc) / 4;
     // cnnz: 2 words; cdc and cac: 4 + 64 words
-    if (lane() < 2) d[cnnz + lane()] = s[cnnz + lane()];
-    for (int i = lane(); i < 68; i += 64) d[cdc + i] = s[cdc + i];
+    if (lane() < 2) d[lane()] = s[cnnz + lane()];
+    for (int i = lane(); i < 68; i += 64) d[2 + i] = s[cdc + i];
     if (lane() == 0) {
-      L.i4.chroma_mode = L.code.chroma_mode;
-      L.i4.cbp = L.code.cbp & 0xf0;
+      L.csave.mode = L.code.chroma_mode;
+      L.csave.cbp = L.code.cbp & 0xf0;
     }
     sync();
   }
   __device__ void restore_chroma() {
-    const uint32_t *s = reinterpret_cast<const uint32_t *>(&L.i4);
+    const uint32_t *s = reinterpret_cast<const uint32_t *>(&L.csave);
     uint32_t *d = reinterpret_cast<uint32_t *>(&L.code);
     constexpr int cnnz = offsetof(ia_mbcode, cnnz) / 4, cdc = offsetof(ia_mbcode, cdc) / 4;
-    if (lane() < 2) d[cnnz + lane()] = s[cnnz + lane()];
-    for (int i = lane(); i < 68; i += 64) d[cdc + i] = s[cdc + i];
+    if (lane() < 2) d[cnnz + lane()] = s[lane()];
+    for (int i = lane(); i < 68; i += 64) d[cdc + i] = s[2 + i];
     if (lane() == 0) {
-      L.code.chroma_mode = L.i4.chroma_mode;
-      L.code.cbp = (uint8_t)((L.code.cbp & 15) | L.i4.cbp);
+      L.code.chroma_mode = L.csave.mode;
+      L.code.cbp = (uint8_t)((L.code.cbp & 15) | L.csave.cbp);
     }
     sync();
   }
@@ -2378,6 +2467,432 @@ struct Coder {
     rd_kept = 1;
   }
 
+  // ---- B macroblocks (enc/frame.rs:1518-1780) ----
+  //
+  // A B slice predicts from one picture per list: P.hp/rcb/rcr [0] the list 0 picture, [1] the list 1
+  // picture, P.rmbs0 the list 1 picture's records. Every partition a B macroblock considers moves as a
+  // whole per 8x8 block, so a candidate's motion is held per 8x8 block (BMot) and spread over the 4x4
+  // blocks only in the code.
+
+  struct BMot {
+    uint8_t use[4];         // per 8x8 block: bit 0 list 0, bit 1 list 1
+    int16_t mv[2][4][2];    // per list and 8x8 block
+  };
+  struct BDir {
+    int16_t mv[2][2];       // each list's direct vector
+    uint8_t zero;           // bit 4 l + i8: list l's vector is zero in 8x8 block i8
+    uint8_t use;            // lists used
+  };
+
+  __device__ static int min_positive(int x, int y) { return x >= 0 && y >= 0 ? min(x, y) : max(x, y); }
+
+  // Spatial direct motion (enc/frame.rs:1717-1745, 8.4.1.2.2): per list the neighbours' smallest
+  // reference index and the 16x16 predictor, zero in the 8x8 blocks whose co-located corner block of
+  // the list 1 picture uses its reference 0 and moves by at most a quarter sample; both lists with zero
+  // vectors when no neighbour uses either.
+  __device__ void direct_mvs(BDir &d) const {
+    const int16_t z[16][2] = {};
+    const int8_t nr[16] = {};
+    int refs[2];
+    for (int l = 0; l < 2; l++) {
+      int ra = -1, rb = -1, rc = -1, r;
+      int16_t m[2];
+      if (nb_motion(z, nr, 0, -1, 0, &r, m, l)) ra = r;
+      if (nb_motion(z, nr, 0, 0, -1, &r, m, l)) rb = r;
+      if (nb_motion(z, nr, 0, 4, -1, &r, m, l)) rc = r;
+      else if (nb_motion(z, nr, 0, -1, -1, &r, m, l)) rc = r;
+      refs[l] = min_positive(ra, min_positive(rb, rc));
+    }
+    d.mv[0][0] = d.mv[0][1] = d.mv[1][0] = d.mv[1][1] = 0;
+    d.zero = 0;
+    if (refs[0] < 0 && refs[1] < 0) {
+      d.use = 3;
+      return;
+    }
+    for (int l = 0; l < 2; l++)
+      if (refs[l] >= 0) mv_pred(0, d.mv[l], l);
+    const ia_mbinfo *col = P.rmbs0 ? &P.rmbs0[mby * P.mb_w + mbx] : nullptr;
+    if (col && col->kind == K_INTER)
+      for (int i8 = 0; i8 < 4; i8++) {
+        const int cx = (i8 & 1) * 3, cy = (i8 >> 1) * 3;
+        const int16_t *m = col->mv[cy * 4 + cx];
+        if (col->ref_idx[i8] == 0 && iabs(m[0]) <= 1 && iabs(m[1]) <= 1)
+          for (int l = 0; l < 2; l++)
+            if (refs[l] == 0) d.zero |= (uint8_t)(1 << (4 * l + i8));
+      }
+    d.use = (uint8_t)((refs[0] >= 0) | ((refs[1] >= 0) << 1));
+  }
+
+  __device__ static void direct_mot(const BDir &d, BMot &m) {
+    for (int i8 = 0; i8 < 4; i8++) {
+      m.use[i8] = d.use;
+      for (int l = 0; l < 2; l++) {
+        const bool on = ((d.use >> l) & 1) && !((d.zero >> (4 * l + i8)) & 1);
+        m.mv[l][i8][0] = on ? d.mv[l][0] : 0;
+        m.mv[l][i8][1] = on ? d.mv[l][1] : 0;
+      }
+    }
+  }
+
+  // Eighth-sample chroma of the w x h block at chroma (cx, cy) from picture r into out (both
+  // components, row stride 8, at (ox, oy)), the window clamped to the plane (h264/inter.rs:377-422).
+  __device__ void chroma_pred(int r, int cx, int cy, int w, int h, int mx, int my, uint8_t (*out)[64], int ox, int oy) const {
+    const int cw = P.mb_w * 8, ch = P.mb_h * 8;
+    const int fx = mx & 7, fy = my & 7;
+    const int x0 = cx + (mx >> 3), y0 = cy + (my >> 3);
+    const int wa = (8 - fx) * (8 - fy), wb = fx * (8 - fy), wc = (8 - fx) * fy, wd = fx * fy;
+    for (int i = lane(); i < 2 * w * h; i += 64) {
+      const int ci = i / (w * h), j = i % (w * h), rr = j / w, c = j % w;
+      const uint8_t *pl = ci == 0 ? P.rcb[r] : P.rcr[r];
+      const int X = x0 + c, Y = y0 + rr;
+#define S(xx, yy) ((int)pl[(size_t)clampi((yy), 0, ch - 1) * cw + clampi((xx), 0, cw - 1)])
+      out[ci][(oy + rr) * 8 + ox + c] = (uint8_t)((wa * S(X, Y) + wb * S(X + 1, Y) + wc * S(X, Y + 1) + wd * S(X + 1, Y + 1) + 32) >> 6);
+#undef S
+    }
+  }
+
+  // The prediction of a B macroblock into L.pred / L.cpred, 8x8 block by 8x8 block from the lists used
+  // there, averaged when both are (enc/frame.rs:1749-1780).
+  __device__ void b_pred(const BMot &m) {
+    for (int i8 = 0; i8 < 4; i8++) {
+      const int bx = (i8 & 1) * 8, by = (i8 >> 1) * 8;
+      for (int l = 0; l < 2; l++) {
+        if (!((m.use[i8] >> l) & 1)) continue;
+        uint8_t *dst = l ? L.bs.p1 : L.pred;
+        luma_pred(l, mbx * 16 + bx, mby * 16 + by, 8, 8, m.mv[l][i8][0], m.mv[l][i8][1], dst + by * 16 + bx, 16);
+        chroma_pred(l, mbx * 8 + bx / 2, mby * 8 + by / 2, 4, 4, m.mv[l][i8][0], m.mv[l][i8][1], l ? L.bs.c1 : L.cpred, bx / 2, by / 2);
+      }
+    }
+    sync();
+    for (int i = lane(); i < 384; i += 64) {
+      const int c = i >= 256, j = c ? (i - 256) & 63 : i;
+      const int i8 = c ? ((j >> 3) >> 2) * 2 + ((j & 7) >> 2) : ((j >> 4) >> 3) * 2 + ((j & 15) >> 3);
+      const int u = m.use[i8];
+      uint8_t *a = c ? &L.cpred[(i - 256) >> 6][j] : &L.pred[j];
+      const int b = c ? L.bs.c1[(i - 256) >> 6][j] : L.bs.p1[j];
+      if (u == 3) *a = (uint8_t)((*a + b + 1) >> 1);
+      else if (u == 2) *a = (uint8_t)b;
+    }
+    sync();
+  }
+
+  // SATD of each 4x4 block (raster) of the macroblock's luma against pred (stride 16) into L.bs.sat
+  __device__ void satd_blocks(const uint8_t *pred) {
+    if (lane() < 16) {
+      const int ox = (lane() & 3) * 4, oy = (lane() >> 2) * 4;
+      int d[16];
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) d[rr * 4 + c] = (int)L.src_y[(oy + rr) * 16 + ox + c] - (int)pred[(oy + rr) * 16 + ox + c];
+      L.bs.sat[lane()] = satd4(d);
+    }
+    sync();
+  }
+  // the sum of L.bs.sat over the 4x4 blocks of the rectangle (bx, by, w, h) (4x4 units)
+  __device__ static int sat_sum(int bx, int by, int w, int h) {
+    int t = 0;
+    for (int y = by; y < by + h; y++)
+      for (int x = bx; x < bx + w; x++) t += L.bs.sat[y * 4 + x];
+    return t;
+  }
+
+  // The two lists' predictions of partitions (bx[k], by[k], w, h), k < n, averaged into L.pred, and the
+  // SATD of each 4x4 block against it in L.bs.sat (the bi-predicted candidates of
+  // enc/frame.rs:1578-1581, :1637-1639)
+  __device__ void bi_satd(int n, const int *bx, const int *by, int w, int h, const int16_t (*mv0)[2], const int16_t (*mv1)[2]) {
+    for (int k = 0; k < n; k++) {
+      luma_pred(0, mbx * 16 + bx[k] * 4, mby * 16 + by[k] * 4, w * 4, h * 4, mv0[k][0], mv0[k][1], L.pred + by[k] * 64 + bx[k] * 4, 16);
+      luma_pred(1, mbx * 16 + bx[k] * 4, mby * 16 + by[k] * 4, w * 4, h * 4, mv1[k][0], mv1[k][1], L.bs.p1 + by[k] * 64 + bx[k] * 4, 16);
+    }
+    sync();
+    for (int i = lane(); i < 256; i += 64) L.pred[i] = (uint8_t)((L.pred[i] + L.bs.p1[i] + 1) >> 1);
+    sync();
+    satd_blocks(L.pred);
+  }
+
+  // Each group's search of one partition in one list; group g searches (bx[g], by[g]) in list lst[g]
+  // with the predictor of its list and the extra candidates of its list, the results read by every
+  // lane (the groups' rectangles have one size).
+  __device__ void b_search4(const int *bx, const int *by, int w, int h, const int *lst, const int16_t (*mvps)[2],
+                            const int16_t (*extra)[5][2], int nextra, int16_t (*mv)[2], int *cost) {
+    const int g = group();
+    int16_t out[2];
+    const int c = motion_search(0, bx[g], by[g], w, h, mvps[lst[g]], extra[lst[g]], nextra, out, lst[g]);
+    if (glane() == 0) {
+      L.bs.mv[g][0] = out[0];
+      L.bs.mv[g][1] = out[1];
+      L.bs.cost[g] = c;
+    }
+    sync();
+    for (int k = 0; k < 4; k++) {
+      mv[k][0] = L.bs.mv[k][0];
+      mv[k][1] = L.bs.mv[k][1];
+      cost[k] = L.bs.cost[k];
+    }
+    sync();
+  }
+
+  struct BParts {
+    int part;
+    uint8_t dirs[4];
+    int16_t pmv[2][4][2];
+  };
+
+  // The best partitioning below 16x16 when it costs less than cost16 (enc/frame.rs:1621-1685): each
+  // 8x8 block direct (d8: the direct prediction's SATD per 8x8 block) or from list 0, list 1 or both,
+  // then 16x8 and 8x16 without direct. Costs are never negative, so a running total that reaches the
+  // bound does so at the end too: every search runs (four at a time, one per group) and the totals are
+  // compared once. Returns 0 when the 8x8 blocks cost cost16 or more.
+  __device__ int best_b_parts(const int16_t (*mvs16)[2], const int16_t (*mvps)[2], int cost16, const int *d8, BParts &out) {
+    const int lam = P.lambda;
+    const int BX[4] = {0, 2, 0, 2}, BY[4] = {0, 0, 2, 2};
+    int16_t extra[2][5][2];
+    for (int l = 0; l < 2; l++) { extra[l][0][0] = mvs16[l][0]; extra[l][0][1] = mvs16[l][1]; }
+    int16_t pm[2][4][2];
+    int pc[2][4];
+    for (int l = 0; l < 2; l++) {
+      const int lst[4] = {l, l, l, l};
+      b_search4(BX, BY, 2, 2, lst, mvps, extra, 1, pm[l], pc[l]);
+    }
+    bi_satd(4, BX, BY, 2, 2, pm[0], pm[1]);
+    int total = lam * 6;
+    for (int k = 0; k < 4; k++) {
+      const int bi = sat_sum(BX[k], BY[k], 2, 2) + lam * (mvd_bits(pm[0][k][0], pm[0][k][1], mvps[0][0], mvps[0][1]) +
+                                                           mvd_bits(pm[1][k][0], pm[1][k][1], mvps[1][0], mvps[1][1]) + 2);
+      int bd, bc;
+      if (bi < min(pc[0][k], pc[1][k])) { bd = 3; bc = bi; }
+      else if (pc[0][k] <= pc[1][k]) { bd = 1; bc = pc[0][k]; }
+      else { bd = 2; bc = pc[1][k]; }
+      // sub_mb_type takes about one bin for direct and three for the others
+      bc += lam * 3;
+      if (d8[k] + lam < bc) { bd = 0; bc = d8[k] + lam; }
+      out.dirs[k] = (uint8_t)bd;
+      total += bc;
+    }
+    sync();
+    if (total >= cost16) return 0;
+    out.part = 3;
+    for (int l = 0; l < 2; l++)
+      for (int k = 0; k < 4; k++) { out.pmv[l][k][0] = pm[l][k][0]; out.pmv[l][k][1] = pm[l][k][1]; }
+    int best_total = total;
+    for (int l = 0; l < 2; l++) {
+      for (int k = 0; k < 4; k++) { extra[l][k][0] = pm[l][k][0]; extra[l][k][1] = pm[l][k][1]; }
+      extra[l][4][0] = mvs16[l][0];
+      extra[l][4][1] = mvs16[l][1];
+    }
+    for (int part = 1; part < 3; part++) {
+      // groups 0 and 1 the first partition in lists 0 and 1, groups 2 and 3 the second
+      const int w = part == 1 ? 4 : 2, h = part == 1 ? 2 : 4;
+      const int gx[4] = {0, 0, part == 2 ? 2 : 0, part == 2 ? 2 : 0}, gy[4] = {0, 0, part == 1 ? 2 : 0, part == 1 ? 2 : 0};
+      const int lst[4] = {0, 1, 0, 1};
+      int16_t m[4][2];
+      int c[4];
+      b_search4(gx, gy, w, h, lst, mvps, extra, 5, m, c);
+      const int px[2] = {gx[0], gx[2]}, py[2] = {gy[0], gy[2]};
+      const int16_t m0[2][2] = {{m[0][0], m[0][1]}, {m[2][0], m[2][1]}}, m1[2][2] = {{m[1][0], m[1][1]}, {m[3][0], m[3][1]}};
+      bi_satd(2, px, py, w, h, m0, m1);
+      int t2 = lam * 7;
+      uint8_t d2[4] = {0, 0, 0, 0};
+      for (int k = 0; k < 2; k++) {
+        const int c0 = c[2 * k], c1 = c[2 * k + 1];
+        const int bi = sat_sum(px[k], py[k], w, h) + lam * (mvd_bits(m0[k][0], m0[k][1], mvps[0][0], mvps[0][1]) +
+                                                            mvd_bits(m1[k][0], m1[k][1], mvps[1][0], mvps[1][1]) + 2);
+        int bd, bc;
+        if (bi < min(c0, c1)) { bd = 3; bc = bi; }
+        else if (c0 <= c1) { bd = 1; bc = c0; }
+        else { bd = 2; bc = c1; }
+        d2[k] = (uint8_t)bd;
+        t2 += bc;
+      }
+      sync();
+      if (t2 < best_total) {
+        best_total = t2;
+        out.part = part;
+        for (int k = 0; k < 4; k++) out.dirs[k] = d2[k];
+        for (int k = 0; k < 4; k++)
+          for (int q = 0; q < 2; q++) {
+            out.pmv[0][k][q] = k < 2 ? m0[k][q] : 0;
+            out.pmv[1][k][q] = k < 2 ? m1[k][q] : 0;
+          }
+      }
+    }
+    return 1;
+  }
+
+  // The B fields of L.code for partitioning `part` with directions `dirs` and motion m (the vector
+  // differences by enc/frame.rs:1688-1711: partitions predicted in order from the earlier ones using
+  // the list, a direct 8x8 block counting with its derived motion), then its prediction and residual.
+  __device__ void code_b(int part, const uint8_t *dirs, const BMot &m, int skip) {
+    code_new(K_INTER);
+    int16_t mvd[2][4][2] = {};
+    if (!skip) {
+      const int dir = part == 1 ? 1 : part == 2 ? 2 : 0;
+      const int8_t nr[16] = {};
+      for (int l = 0; l < 2; l++) {
+        int16_t cur[16][2] = {};
+        uint32_t done = 0;
+        for (int k = 0; k < NPARTS[part]; k++) {
+          const int x = PARTS[part][k][0], y = PARTS[part][k][1], w = PARTS[part][k][2], h = PARTS[part][k][3];
+          const int i8 = (y / 2) * 2 + x / 2;
+          if ((dirs[k] >> l) & 1) {
+            int16_t p[2];
+            pred_mv(cur, nr, done, x, y, w, dir, 0, p, l);
+            mvd[l][k][0] = (int16_t)(m.mv[l][i8][0] - p[0]);
+            mvd[l][k][1] = (int16_t)(m.mv[l][i8][1] - p[1]);
+          }
+          for (int yy = y; yy < y + h; yy++)
+            for (int xx = x; xx < x + w; xx++) {
+              const int b = yy * 4 + xx, b8 = (yy / 2) * 2 + xx / 2;
+              cur[b][0] = m.mv[l][b8][0];
+              cur[b][1] = m.mv[l][b8][1];
+              done |= 1u << b;
+              if (!((m.use[b8] >> l) & 1)) done |= 1u << (16 + b);
+            }
+        }
+      }
+    }
+    if (lane() == 0) {
+      L.code.skip = (uint8_t)skip;
+      L.code.part = (uint8_t)part;
+      for (int k = 0; k < 4; k++) {
+        L.code.bdir[k] = dirs[k];
+        L.code.buse[k] = m.use[k];
+      }
+      for (int l = 0; l < 2; l++) {
+        for (int b = 0; b < 16; b++) {
+          const int b8 = (b / 8) * 2 + (b % 4) / 2, on = (m.use[b8] >> l) & 1;
+          L.code.bmv[l][b][0] = on ? m.mv[l][b8][0] : 0;
+          L.code.bmv[l][b][1] = on ? m.mv[l][b8][1] : 0;
+        }
+        for (int k = 0; k < 4; k++) {
+          L.code.bmvd[l][k][0] = mvd[l][k][0];
+          L.code.bmvd[l][k][1] = mvd[l][k][1];
+        }
+      }
+    }
+    sync();
+  }
+
+  // A B macroblock by rate-distortion cost among B_Skip and B_Direct_16x16, the best of list 0, list 1
+  // and both at 16x16 and the best smaller partitioning, Intra16x16 and Intra4x4 (enc/frame.rs:1518-1617)
+  __device__ void decide_b(ia_mbstats *st) {
+    const int lam = P.lambda;
+    const double lam2 = P.lam2;
+    int have = 0;
+    double best_j = 0.0;
+    BDir dd;
+    direct_mvs(dd);
+    BMot dm;
+    direct_mot(dd, dm);
+    b_pred(dm);
+    // the direct prediction's SATD per 8x8 block (for B_Direct_8x8 in best_b_parts)
+    satd_blocks(L.pred);
+    int d8[4];
+    for (int k = 0; k < 4; k++) d8[k] = sat_sum((k & 1) * 2, (k >> 1) * 2, 2, 2);
+    sync();
+    const uint8_t zdir[4] = {0, 0, 0, 0};
+    // B_Skip: the direct prediction, nothing coded
+    code_b(0, zdir, dm, 1);
+    for (int i = lane(); i < 256; i += 64) cyv(i & 15, i >> 4) = L.pred[i];
+    for (int i = lane(); i < 128; i += 64) ccv(i >> 6, (i & 63) & 7, (i & 63) >> 3) = L.cpred[i >> 6][i & 63];
+    sync();
+    int none = -1;
+    rd_keep(lam2, &none, &have, &best_j);
+    // B_Direct_16x16 with a residual (without one it is B_Skip at a higher price)
+    int dchroma = -1;
+    code_b(0, zdir, dm, 0);
+    code_inter_residual(0);
+    if (L.code.cbp != 0) rd_keep(lam2, &dchroma, &have, &best_j);
+    if (P.transform8) {
+      recode_inter_t8();
+      if (L.code.cbp != 0 && L.code.t8x8) rd_keep(lam2, &dchroma, &have, &best_j);
+    }
+    // 16x16 from either list or both; the co-located vector split by the picture's distances
+    int16_t mvps[2][2], mvs[2][2];
+    mv_pred(0, mvps[0], 0);
+    mv_pred(0, mvps[1], 1);
+    const int tb = P.dist_tb, td = P.dist_td;
+    int16_t colv[2];
+    const int ncol = col_mv(colv);
+    int costs[2];
+    for (int l = 0; l < 2; l++) {
+      int16_t extra[2][2];
+      int ne = 0;
+      if (ncol) {
+        const int num = l == 0 ? tb : tb - td;
+        extra[ne][0] = (int16_t)(colv[0] * num / td);
+        extra[ne][1] = (int16_t)(colv[1] * num / td);
+        ne++;
+      }
+      if (l == 1) {
+        extra[ne][0] = (int16_t)(mvs[0][0] * (tb - td) / tb);
+        extra[ne][1] = (int16_t)(mvs[0][1] * (tb - td) / tb);
+        ne++;
+      }
+      const int cc = motion_search(0, 0, 0, 4, 4, mvps[l], (const int16_t(*)[2])extra, ne, mvs[l], l);
+      costs[l] = cc + lam * (mvd_bits(mvs[l][0], mvs[l][1], mvps[l][0], mvps[l][1]) + 3);
+    }
+    {
+      const int z = 0;
+      bi_satd(1, &z, &z, 4, 4, mvs, mvs + 1);
+    }
+    const int bi = sat_sum(0, 0, 4, 4) + lam * (mvd_bits(mvs[0][0], mvs[0][1], mvps[0][0], mvps[0][1]) +
+                                                mvd_bits(mvs[1][0], mvs[1][1], mvps[1][0], mvps[1][1]) + 5);
+    sync();
+    const int dir = bi < min(costs[0], costs[1]) ? 3 : costs[0] <= costs[1] ? 1 : 2;
+    const int best16 = min(bi, min(costs[0], costs[1]));
+    BParts cand[2];
+    cand[0].part = 0;
+    for (int k = 0; k < 4; k++) cand[0].dirs[k] = k == 0 ? (uint8_t)dir : 0;
+    for (int l = 0; l < 2; l++)
+      for (int k = 0; k < 4; k++) { cand[0].pmv[l][k][0] = mvs[l][0]; cand[0].pmv[l][k][1] = mvs[l][1]; }
+    const int ncand = best_b_parts(mvs, mvps, best16, d8, cand[1]) ? 2 : 1;
+    st->inter_cost = best16;
+    st->inter_part = cand[ncand - 1].part;
+    for (int ci = 0; ci < ncand; ci++) {
+      const BParts &cd = cand[ci];
+      // each 8x8 block's lists and vectors: those of the partition covering it, or the direct ones
+      BMot m;
+      for (int i8 = 0; i8 < 4; i8++) {
+        const int k = cd.part == 0 ? 0 : cd.part == 1 ? i8 >> 1 : cd.part == 2 ? i8 & 1 : i8;
+        const int d = cd.dirs[k];
+        m.use[i8] = d == 0 ? dm.use[i8] : (uint8_t)d;
+        for (int l = 0; l < 2; l++) {
+          const bool on = (m.use[i8] >> l) & 1;
+          m.mv[l][i8][0] = !on ? 0 : d == 0 ? dm.mv[l][i8][0] : cd.pmv[l][k][0];
+          m.mv[l][i8][1] = !on ? 0 : d == 0 ? dm.mv[l][i8][1] : cd.pmv[l][k][1];
+        }
+      }
+      b_pred(m);
+      int ch = -1;
+      code_b(cd.part, cd.dirs, m, 0);
+      code_inter_residual(0);
+      rd_keep(lam2, &ch, &have, &best_j);
+      if (P.transform8) {
+        recode_inter_t8();
+        if (L.code.t8x8) rd_keep(lam2, &ch, &have, &best_j);
+      }
+    }
+    int i16c;
+    const int i16 = best_i16(&i16c);
+    st->i16_cost = i16c;
+    int intra_chroma = -1;
+    code_i16(i16);
+    code_intra_chroma();
+    save_chroma();
+    rd_keep(lam2, &intra_chroma, &have, &best_j);
+    // Intra8x8 is not tried in B pictures (enc/frame.rs:1611)
+    code_i4();
+    restore_chroma();
+    rd_keep(lam2, &intra_chroma, &have, &best_j);
+    restore_rec(L.keep_rec);
+    copy_code(&L.code, &L.keep);
+    sync();
+    st->rd_cost = best_j;
+    rd_kept = 1;
+  }
+
   // enc/frame.rs:715-790; the decision lands in L.code and the canvas
   __device__ void code_mb(ia_mbstats *st) {
     const int lam = P.lambda;
@@ -2387,6 +2902,10 @@ struct Coder {
     st->inter_cost = -1;
     st->inter_part = -1;
     st->rd_cost = 0.0;
+    if (P.b_slice) {
+      decide_b(st);
+      return;
+    }
     if (P.p_slice) {
       int16_t smv[2];
       skip_mv(smv);
@@ -2483,18 +3002,42 @@ struct Coder {
 
   // enc/frame.rs:2281-2339: the macroblock's state for later macroblocks and its MbInfo
   __device__ void finish_mb(const ia_mbcode &code, MbState &s, ia_mbinfo &info) const {
-    const int inter = code.kind == K_INTER;
+    const int inter = code.kind == K_INTER, bm = inter && P.b_slice;
+    const uint8_t z4[4] = {0, 0, 0, 0};
+    // the lists each 8x8 block uses
+    int used0[4], used1[4];
+    for (int b8 = 0; b8 < 4; b8++) {
+      used0[b8] = inter && (!bm || (code.buse[b8] & 1));
+      used1[b8] = bm && ((code.buse[b8] >> 1) & 1);
+    }
     Rect rs[16];
-    const int n = rects(code.part, code.sub, rs);
+    const int n = rects(code.part, bm ? z4 : code.sub, rs);
     s = MbState{};
-    if (inter && !code.skip) mvd_grid(rs, n, code.mvds, s.mvd);
-    if (inter)
+    if (inter && !code.skip) {
+      if (bm) {
+        mvd_grid(rs, n, code.bmvd[0], s.mvd);
+        mvd_grid(rs, n, code.bmvd[1], s.mvd1);
+      } else {
+        mvd_grid(rs, n, code.mvds, s.mvd);
+      }
+    }
+    if (bm) {
+      for (int b = 0; b < 16; b++) {
+        const int b8 = (b / 8) * 2 + (b % 4) / 2;
+        for (int q = 0; q < 2; q++) {
+          s.mvs[b][q] = used0[b8] ? code.bmv[0][b][q] : 0;
+          s.mvs1[b][q] = used1[b8] ? code.bmv[1][b][q] : 0;
+        }
+      }
+    } else if (inter) {
       for (int k = 0; k < n; k++)
         for (int by = rs[k].y; by < rs[k].y + rs[k].h; by++)
           for (int bx = rs[k].x; bx < rs[k].x + rs[k].w; bx++) {
             s.mvs[by * 4 + bx][0] = code.mvs[k][0];
             s.mvs[by * 4 + bx][1] = code.mvs[k][1];
           }
+    }
+    s.bdirect = (uint8_t)(bm && code.part == 0 && code.bdir[0] == 0);
     const int cbpc = code.cbp >> 4;
     s.cbf_dc = (uint8_t)((code.kind == K_I16 && count_nz(code.dc, 16) > 0) | ((cbpc > 0 && count_nz(code.cdc[0], 4) > 0) << 1) |
                          ((cbpc > 0 && count_nz(code.cdc[1], 4) > 0) << 2));
@@ -2515,7 +3058,10 @@ struct Coder {
     }
     for (int c = 0; c < 2; c++)
       for (int k = 0; k < 4; k++) s.cnnz[c][k] = code.cnnz[c][k];
-    for (int b8 = 0; b8 < 4; b8++) s.refs[b8] = inter ? r8[b8] : -1;
+    for (int b8 = 0; b8 < 4; b8++) {
+      s.refs[b8] = (int8_t)(inter && !bm ? r8[b8] : used0[b8] ? 0 : -1);
+      s.refs1[b8] = (int8_t)(used1[b8] ? 0 : -1);
+    }
     s.skip = code.skip;
     s.kind = code.kind;
     s.cbp = code.cbp;
@@ -2528,16 +3074,24 @@ struct Coder {
     info.cbp = code.cbp;
     info.skip = code.skip;
     info.t8x8 = code.t8x8;
-    for (int i = 0; i < 16; i++) {
-      info.mv[i][0] = s.mvs[i][0];
-      info.mv[i][1] = s.mvs[i][1];
-    }
+    for (int i = 0; i < 16; i++)
+      for (int q = 0; q < 2; q++) {
+        info.mv[i][q] = s.mvs[i][q];
+        info.mv1[i][q] = s.mvs1[i][q];
+      }
     for (int b8 = 0; b8 < 4; b8++) {
-      info.ref_idx[b8] = inter ? r8[b8] : -1;
-      info.ref_id[b8] = inter ? 1 + r8[b8] : -1;
-      info.ref_idx1[b8] = -1;
-      info.ref_id1[b8] = -1;
+      // the identity tells the loop filter which picture a block predicts from (B: one per list)
+      const int r = bm ? 0 : r8[b8];
+      info.ref_idx[b8] = (int8_t)(used0[b8] ? r : -1);
+      info.ref_id[b8] = used0[b8] ? (bm ? 1 : 1 + r) : -1;
+      info.ref_idx1[b8] = (int8_t)(used1[b8] ? 0 : -1);
+      info.ref_id1[b8] = used1[b8] ? 2 : -1;
     }
+    info.bdirect = s.bdirect;
+    if (s.bdirect) info.direct8 = 15;
+    else if (bm && code.part == 3)
+      for (int k = 0; k < 4; k++)
+        if (code.bdir[k] == 0) info.direct8 |= (uint8_t)(1 << k);
   }
 
   // ---- the slice loop (enc/frame.rs:452-487) ----

@@ -263,6 +263,8 @@ int ia_promote_reference(ia_ctx *c, int slot, int deblock) {
   if (!c) return IA_EINVAL;
   if (slot < 0 || slot >= c->cfg.nrefs) return fail(c, IA_EINVAL, "bad reference slot");
   if (!c->an_valid) return fail(c, IA_ESTATE, "no analysed picture to promote");
+  // B pictures are never kept as references (enc/mod.rs:656-667)
+  if (c->an_b) return fail(c, IA_ESTATE, "a B picture is not a reference");
   const Geom &g = c->g;
   const size_t ylen = (size_t)g.pw * g.ph, clen = (size_t)g.cw * g.ch, nmb = (size_t)g.mb_w * g.mb_h;
   IaSlot &s = c->slots[slot];
@@ -387,6 +389,29 @@ int ia_analyse_uploaded(ia_ctx *c, int qp, int kind, const int *ref_slots, int n
   if (rc) return rc;
   IA_HIP(c, ia::launch_analyse(p, c->stream));
   c->an_valid = true;
+  c->an_b = false;
+  return IA_OK;
+}
+
+int ia_analyse_b(ia_ctx *c, int qp, int l0_slot, int l1_slot, int dist_l0, int dist_l1) {
+  if (!c) return IA_EINVAL;
+  if (!c->cfg.cabac) return fail(c, IA_EUNSUPPORTED, "B pictures are coded with CABAC only");
+  if (dist_l0 <= 0 || dist_l1 <= dist_l0)
+    return fail(c, IA_EINVAL, "the B picture must lie between its references: 0 < dist_l0 < dist_l1");
+  const int slots[2] = {l0_slot, l1_slot};
+  ia::AnParams p;
+  int rc = iah::analysis_params(c, qp, IA_PIC_P, slots, 2, nullptr, false, &p);
+  if (rc) return rc;
+  // one picture per list: planes [0] list 0, [1] list 1; the co-located records are list 1's
+  p.fs = nullptr;
+  p.nref = 1;
+  p.b_slice = 1;
+  p.dist_tb = dist_l0;
+  p.dist_td = dist_l1;
+  p.rmbs0 = c->slots[l1_slot].mbs;
+  IA_HIP(c, ia::launch_analyse(p, c->stream));
+  c->an_valid = true;
+  c->an_b = true;
   return IA_OK;
 }
 

@@ -36,6 +36,7 @@ struct ia_ctx {
   ia::MbState *state = nullptr;
   uint32_t *entropy = nullptr;
   bool an_valid = false;
+  bool an_b = false;          // the last analysis was of a B picture (not a reference)
   // learned proposal
   ia::CnnWeights *wts = nullptr;
   bool wts_loaded = false;

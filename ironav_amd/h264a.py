@@ -215,6 +215,14 @@ class Analyser:
                                               abi.ptr(rcb), abi.ptr(rcr), abi.ptr(mbs)))
         return Analysis(codes, stats, Picture(ry, rcb, rcr), mbs, self.mb_w, self.mb_h, self.rows)
 
+    def analyse_b(self, pic, qp, l0_slot, l1_slot, dist_l0, dist_l1):
+        """Analyses `pic` as a B picture between the references in l0_slot (shown before it, dist_l0
+        picture-order units earlier) and l1_slot (shown after it, dist_l1 units after the list 0 one)
+        and reads the results back."""
+        self.upload(pic)
+        self._ok(library().ia_analyse_b(self._h, qp, l0_slot, l1_slot, dist_l0, dist_l1))
+        return self.read_analysis()
+
     def read_analysis(self):
         """The results of the last analysis of this context (e.g. after a batch step)."""
         n = self.mb_w * self.mb_h

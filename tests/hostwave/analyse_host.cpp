@@ -30,10 +30,10 @@ int main(int argc, char **argv) {
   }
   FILE *f = fopen(argv[1], "rb");
   if (!f) return 2;
-  int32_t h[13];
+  int32_t h[15];
   if (fread(h, sizeof h, 1, f) != 1) return 2;
   const int width = h[0], height = h[1], cabac = h[2], t8 = h[3], sub8 = h[4], fast = h[5], slices = h[6];
-  const int qp = h[8], kind = h[9], nref = h[10], has_mbs0 = h[11], has_fs = h[12];
+  const int qp = h[8], kind = h[9], nref = h[10], has_mbs0 = h[11], has_fs = h[12], tb = h[13], td = h[14];
   const ia::Geom g = ia::make_geom(width, height);
   const size_t ylen = (size_t)g.pw * g.ph, clen = (size_t)g.cw * g.ch, hplen = 4 * (size_t)g.sw * g.sh;
   const size_t nmb = (size_t)g.mb_w * g.mb_h;
@@ -63,13 +63,17 @@ int main(int argc, char **argv) {
   p.mb_w = g.mb_w;
   p.mb_h = g.mb_h;
   p.nslices = slices < g.mb_h ? slices : g.mb_h;
-  p.nref = kind == IA_PIC_P ? nref : 0;
+  // a B picture (kind 2): the list 0 and list 1 pictures, the list 1 picture's records
+  p.nref = kind == IA_PIC_P ? nref : kind == IA_PIC_B ? 1 : 0;
+  p.b_slice = kind == IA_PIC_B;
+  p.dist_tb = tb;
+  p.dist_td = td;
   p.qp_slice = qp;
   p.qp = qp;
   p.qpc = chroma_qp(qp);
   p.lambda = (int)round(pow(2.0, ((double)qp - 12.0) / 6.0));
   if (p.lambda < 1) p.lambda = 1;
-  p.p_slice = kind == IA_PIC_P;
+  p.p_slice = kind == IA_PIC_P || kind == IA_PIC_B;
   p.sub8 = sub8 && qp <= 20;
   p.use_cabac = cabac != 0;
   p.transform8 = cabac && t8;

@@ -54,13 +54,17 @@ def frames(w, h, n, seed, **kw):
     return [Picture(*clipgen.pad_planes(*f)) for f in clipgen.make_clip(spec)]
 
 
-def run_host(exe, tmp_path, cfg, pic, qp, kind, refs=(), fs=None, halfpel=None):
+def run_host(exe, tmp_path, cfg, pic, qp, kind, refs=(), fs=None, halfpel=None, dist=(0, 0)):
+    """refs: (Picture, records) in list 0 order; for a B picture (kind abi.PIC_B) the list 0 and list
+    1 pictures, dist their distances (tb, td)."""
     mb_w, mb_h = (cfg.width + 15) // 16, (cfg.height + 15) // 16
     nmb = mb_w * mb_h
     nref = len(refs)
-    mbs0 = refs[0][1] if refs and refs[0][1] is not None else None
+    col = refs[1] if kind == abi.PIC_B else refs[0] if refs else None
+    mbs0 = col[1] if col is not None and col[1] is not None else None
     head = np.array([cfg.width, cfg.height, cfg.cabac, cfg.transform8, cfg.sub8x8, cfg.fast, cfg.slices,
-                     cfg.search_mode, qp, kind, nref, mbs0 is not None, fs is not None], dtype=np.int32)
+                     cfg.search_mode, qp, kind, nref, mbs0 is not None, fs is not None, dist[0], dist[1]],
+                    dtype=np.int32)
     parts = [head.tobytes(), pic.y.tobytes(), pic.cb.tobytes(), pic.cr.tobytes()]
     for r, _ in refs:
         parts += [halfpel(r.y).tobytes(), r.cb.tobytes(), r.cr.tobytes()]
@@ -152,3 +156,46 @@ def test_full_search_and_sub8x8_host(host_exe, tmp_path, oracle):
 def test_cavlc_and_realtime_host(host_exe, tmp_path, oracle):
     chain(host_exe, tmp_path, oracle, 48, 32, 3, 26, nrefs=2, slices=2, cabac=False, transform8=False, seed=6)
     chain(host_exe, tmp_path, oracle, 48, 32, 3, 26, nrefs=1, slices=2, fast=True, seed=77, max_motion=6)
+
+
+def b_chain(exe, tmp_path, oracle, w, h, nframes, qp, *, nrefs=2, slices=1, transform8=True, seed=1, **clip):
+    """An IBBP chain (two B pictures, coding order of the reference's encoder); the P and B pictures
+    analysed by the host build against the oracle, each reference the host build's reconstruction."""
+    from ironav_amd import h264w
+
+    pics = frames(w, h, nframes, seed, **clip)
+    cfg = abi.Config(width=w, height=h, bit_depth=8, cabac=1, transform8=int(transform8), sub8x8=0, fast=0,
+                     nrefs=nrefs, slices=slices, search_mode=abi.SEARCH_DIAMOND, search_range=16, fallback_ratio=1.1)
+
+    def halfpel(y):
+        return oracle.halfpel_build(y.astype(np.uint16), 8).astype(np.uint8)
+
+    refs = []  # (display, Picture, records) in coding order
+    nb = 0
+    for d, kind in h264w.coding_order([t == 0 for t in range(nframes)], 2):
+        pic = pics[d]
+        if kind == abi.PIC_IDR:
+            q, rl, dist = qp - 3, [], (0, 0)
+        elif kind == abi.PIC_P:
+            q, rl, dist = qp, [(r[1], r[2]) for r in refs[::-1][:nrefs]], (0, 0)
+        else:
+            l0 = max((r for r in refs if r[0] < d), key=lambda r: r[0])
+            l1 = min((r for r in refs if r[0] > d), key=lambda r: r[0])
+            q, rl, dist = min(qp + 5, 51), [(l0[1], l0[2]), (l1[1], l1[2])], (2 * (d - l0[0]), 2 * (l1[0] - l0[0]))
+        got = run_host(exe, tmp_path, cfg, pic, q, kind, rl, None, halfpel, dist)
+        if kind == abi.PIC_B:
+            want = oracle.analyse_picture_b(cfg, pic, q, rl[0], rl[1], *dist)
+            nb += 1
+        else:
+            want = oracle.analyse_picture(cfg, pic, q, kind, rl)
+        compare(f"picture {d} kind {kind} qp {q}", got, want)
+        if kind != abi.PIC_B:
+            if len(refs) == max(nrefs, 2):
+                refs.pop(0)
+            refs.append((d, Picture(*got[2]), got[3]))
+    assert nb > 0
+
+
+def test_b_pictures_host(host_exe, tmp_path, oracle):
+    b_chain(host_exe, tmp_path, oracle, 64, 48, 5, 26, nrefs=2, slices=2, seed=31, max_motion=6)
+    b_chain(host_exe, tmp_path, oracle, 48, 32, 4, 34, nrefs=1, slices=1, transform8=False, seed=32)
