@@ -99,7 +99,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (the counters' scale for the traffic figures)")
     ap.add_argument("--config", choices=["learned", "plain", "batched", "learned-batched", "deblock", "search-loop",
-                                         "halfpel-loop", "issue-rates"], default="learned",
+                                         "halfpel-loop", "issue-rates", "b-pictures"], default="learned",
                     help="learned: the learned proposal vs the exhaustive search at 1080p (the metric); "
                          "plain: the exhaustive search + exact analysis at 1080p against the CPU oracle; "
                          "batched: many streams of mixed sizes through the batched analysis; "
@@ -109,7 +109,9 @@ def main():
                          "CPU oracle's filter; search-loop: the 1080p picture's motion search (--mode) "
                          "launched --launches times, for counter and trace runs; halfpel-loop: the half-sample "
                          "planes of a 1080p reference built --launches times beside two torch kernels that "
-                         "move the same bytes; issue-rates: the VALU issue rates behind the SAD peak")
+                         "move the same bytes; issue-rates: the VALU issue rates behind the SAD peak; "
+                         "b-pictures: the analysis of a 1080p B picture (spatial direct, list 0/1/bi partitions) "
+                         "against the CPU oracle")
     ap.add_argument("--mode", choices=["full", "learned", "learned-fallback"], default="full",
                     help="search-loop: exhaustive search, the proposal alone (fallback off), or with the fallback")
     ap.add_argument("--launches", type=int, default=5, help="search-loop, halfpel-loop: launches")
@@ -143,6 +145,8 @@ def main():
         return bench_halfpel_loop(args)
     if args.config == "issue-rates":
         return bench_issue_rates(args)
+    if args.config == "b-pictures":
+        return bench_bpictures(args)
 
     import numpy as np
     import torch
@@ -660,6 +664,101 @@ def bench_plain(args):
         "config": {"workload": f"P picture at {width}x{height}, QP {qp}, {nref} reference(s), {search}, sub-8x8 partitions "
                                f"on, 8x8 transform, CABAC, {slices} slices",
                    "macroblocks_per_step": mbs},
+        "bitexact_vs_oracle": exact,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def bench_bpictures(args):
+    """The analysis of one 1080p B picture per step (ia_analyse_b), inputs resident, against the CPU
+    oracle on host threads, bit-checked on the whole picture. The workload clip is coded as the
+    encoder codes it with two B pictures: picture 0 IDR (QP 25), picture 3 P against it (QP 28), both
+    loop-filtered into reference slots on the device; the step analyses picture 1 as a B picture
+    (QP 33) between them (picture-order distances 2 and 6). --slices as for plain."""
+    import numpy as np
+    import torch
+
+    from ironav_amd import abi, clipgen
+    from ironav_amd.h264a import Analyser, Picture
+
+    width, height, qp = 1920, 1080, 28
+    mb_h = (height + 15) // 16
+    if args.slices > 0:
+        slices = args.slices
+    elif args.slices < 0:
+        slices = max(1, min(args.threads, mb_h // 8))  # the default encoder's rule (enc/mod.rs:192)
+    else:
+        slices = mb_h
+    spec = clipgen.workload_1080p()
+    spec.frames = 4
+    pics = [Picture(*clipgen.pad_planes(*f)) for f in clipgen.make_clip(spec)]
+    torch.cuda.init()
+    stream = torch.cuda.Stream()
+    an = Analyser(width, height, nrefs=2, slices=slices)
+    an.set_stream(stream.cuda_stream)
+    an.analyse(pics[0], qp - 3, abi.PIC_IDR)
+    an.promote_reference(0, deblock=True)
+    an.analyse(pics[3], qp, abi.PIC_P, [0])
+    an.promote_reference(1, deblock=True)
+    qb = qp + 5
+    an.upload(pics[1])
+
+    def step():
+        an.analyse_b_uploaded(qb, 0, 1, 2, 6)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    got = an.read_analysis()
+    mbs = an.mb_w * an.mb_h
+    kinds = {"B_Skip": int(((got.codes["kind"] == abi.KIND_INTER) & (got.codes["skip"] == 1)).sum()),
+             "inter": int(((got.codes["kind"] == abi.KIND_INTER) & (got.codes["skip"] == 0)).sum()),
+             "intra": int((got.codes["kind"] != abi.KIND_INTER).sum())}
+    print(f"[b-pictures] GPU {ms:.2f} ms per picture; CPU oracle next", file=sys.stderr, flush=True)
+    cpu, exact = None, None
+    if not args.no_cpu:
+        from oracle import pyoracle
+
+        pyoracle.build()
+        r0, r1 = an.read_reference(0), an.read_reference(1)
+        used = min(args.threads, slices)
+        best, want = None, None
+        for _ in range(3):
+            t0 = time.perf_counter()
+            want = pyoracle.analyse_picture_b(an.cfg, pics[1], qb, r0, r1, 2, 6, threads=args.threads)
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        codes, stats, rec, mbinfo = want
+        exact = bool(np.array_equal(codes, got.codes) and np.array_equal(stats, got.stats)
+                     and np.array_equal(mbinfo, got.mbs)
+                     and all(np.array_equal(a, b) for a, b in zip(rec, (got.recon.y, got.recon.cb, got.recon.cr))))
+        cpu = {"value": round(mbs / best, 1), "unit": "MB/s", "cores": used, "kind": "port",
+               "sample": f"the oracle's analysis of the same B picture ({mbs} macroblocks), {slices} slices on {used} "
+                         f"threads, best of 3, {best * 1000:.1f} ms; host: {os.cpu_count()} logical CPUs, {cpu_model()}"}
+    out = {
+        "metric": "MB/s B-picture analysis @1080p: spatial direct + list 0/1/bi partitions + exact code_mb, GPU vs CPU oracle",
+        "value": round(mbs / (ms / 1000.0), 1),
+        "unit": "MB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded generator clip 1920x1080 4:2:0; picture 1 between the loop-filtered IDR picture 0 "
+                "and P picture 3; inputs resident in HBM)",
+        "config": {"workload": f"B picture at 1920x1080, QP {qb}, the reference's diamond search per list, CABAC + 8x8 "
+                               f"transform, {slices} slices", "macroblocks_per_step": mbs, "decisions": kinds},
         "bitexact_vs_oracle": exact,
         "cpu_baseline": cpu,
     }

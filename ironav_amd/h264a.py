@@ -220,8 +220,12 @@ class Analyser:
         picture-order units earlier) and l1_slot (shown after it, dist_l1 units after the list 0 one)
         and reads the results back."""
         self.upload(pic)
-        self._ok(library().ia_analyse_b(self._h, qp, l0_slot, l1_slot, dist_l0, dist_l1))
+        self.analyse_b_uploaded(qp, l0_slot, l1_slot, dist_l0, dist_l1)
         return self.read_analysis()
+
+    def analyse_b_uploaded(self, qp, l0_slot, l1_slot, dist_l0, dist_l1):
+        """Enqueues the B-picture analysis of the uploaded picture (results stay on the device)."""
+        self._ok(library().ia_analyse_b(self._h, qp, l0_slot, l1_slot, dist_l0, dist_l1))
 
     def read_analysis(self):
         """The results of the last analysis of this context (e.g. after a batch step)."""
