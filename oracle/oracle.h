@@ -31,6 +31,10 @@
  *    product's back end and decoded by decoder.c, which recomputes every vector from its mvd and its
  *    own predictor; the decoded pictures equal the loop-filtered reconstruction and the decoded records
  *    the analysis', and the written size is within 2% of the rate mb_bins counts (tests/test_writer.py).
+ *  - B pictures (or_analyse_picture_b: direct_mvs, b_pred, the B bins, the list 1 records): the same
+ *    round trip on IBBP chains written with B slices; decoder.c's B slices are pinned by the reference's
+ *    five CABAC B fixtures (spatial and temporal direct, implicit weights, a B pyramid), and it
+ *    recomputes every B vector from its mvd, its own predictor and its own direct derivation.
  *  - motion_search's candidate order and diamond, best_inter_ref/sub_search, code_mb's thresholds and
  *    decide_rd's order and costs: parity unpinned (the reference commits no encoder outputs and cannot be
  *    built here; any decision the syntax allows decodes correctly). Restated line by line.
