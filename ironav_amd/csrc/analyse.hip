@@ -1874,6 +1874,21 @@ struct Coder {
     }
   }
 
+  // The slice parameters the rate count reads, loaded once per count (read through P inside its
+  // branches, they were loaded from memory again in every branch)
+  struct CabEnv {
+    int p_slice, b_slice, transform8, nref;
+  };
+  // A category's first significance, last and level contexts (frame coding) as selections among
+  // constants, so that a lane's category reads no table from memory
+#define IA_CAT_SELECT(T)                                                                                \
+  constexpr int v0 = T[0], v1 = T[1], v2 = T[2], v3 = T[3], v4 = T[4], v5 = T[5];                      \
+  return cat == 0 ? v0 : cat == 1 ? v1 : cat == 2 ? v2 : cat == 3 ? v3 : cat == 4 ? v4 : v5;
+  __device__ static int sig_off(int cat) { IA_CAT_SELECT(IA_SIG_OFF[0]) }
+  __device__ static int last_off(int cat) { IA_CAT_SELECT(IA_LAST_OFF[0]) }
+  __device__ static int abs_off(int cat) { IA_CAT_SELECT(IA_ABS_OFF) }
+#undef IA_CAT_SELECT
+
   // The mb_type of a B macroblock (Table 7-14; enc/frame.rs:2343-2351)
   __device__ static int b_mb_type(const ia_mbcode &code) {
     const uint8_t PAIRS[9][2] = {{1, 1}, {2, 2}, {1, 2}, {2, 1}, {1, 3}, {2, 3}, {3, 1}, {3, 2}, {3, 3}};
@@ -1886,19 +1901,19 @@ struct Coder {
 
   // mb_skip_flag, mb_type, transform_size_8x8_flag, intra_chroma_pred_mode, sub_mb_type, ref_idx,
   // coded_block_pattern and mb_qp_delta (enc/frame.rs:2028-2110, :2150-2200)
-  __device__ uint32_t header_bins(uint8_t *st) const {
+  __device__ uint32_t header_bins(uint8_t *st, const CabEnv &E) const {
     const ia_mbcode &code = L.code;
     const MbState *left = nb(-1, 0), *top = nb(0, -1);
     const int kind = code.kind;
     uint32_t bits = 0;
-    if (P.p_slice) {
-      dec_at(st, (P.b_slice ? 24 : 11) + (left && !left->skip) + (top && !top->skip), code.skip, bits);
+    if (E.p_slice) {
+      dec_at(st, (E.b_slice ? 24 : 11) + (left && !left->skip) + (top && !top->skip), code.skip, bits);
       if (code.skip) return bits;
     }
-    const int cbpl = code.cbp & 15, cbpc = code.cbp >> 4, prefix_ok = P.p_slice;
-    const int base = P.b_slice ? 32 : P.p_slice ? 17 : 3;
+    const int cbpl = code.cbp & 15, cbpc = code.cbp >> 4, prefix_ok = E.p_slice;
+    const int base = E.b_slice ? 32 : E.p_slice ? 17 : 3;
     const int b_ctx = 27 + (left && !left->bdirect) + (top && !top->bdirect);
-    if (kind == K_INTER && P.b_slice) {
+    if (kind == K_INTER && E.b_slice) {
       // B_Direct_16x16 0, B_L0_16x16 100, B_L1_16x16 101, the others 11 and four or five more bins
       // (Table 9-37); sub_mb_type B_Direct_8x8 0, B_L0_8x8 100, B_L1_8x8 101, B_Bi_8x8 11000
       const int t = b_mb_type(code);
@@ -1934,7 +1949,7 @@ struct Coder {
       dec_at(st, b1 ? 17 : 16, b2, bits);
     } else {
       int first;
-      if (P.b_slice) {
+      if (E.b_slice) {
         // 111101: an intra type follows, with the suffix contexts
         dec_at(st, b_ctx, 1, bits);
         dec_at(st, 30, 1, bits);
@@ -1951,7 +1966,7 @@ struct Coder {
       }
       if (kind == K_I4 || kind == K_I8) {
         dec_at(st, first, 0, bits);
-        if (P.transform8) dec_at(st, 399 + (left && left->t8x8) + (top && top->t8x8), kind == K_I8, bits);
+        if (E.transform8) dec_at(st, 399 + (left && left->t8x8) + (top && top->t8x8), kind == K_I8, bits);
       } else {
         dec_at(st, first, 1, bits);
         bits += 1;  // terminate bin 0
@@ -1972,7 +1987,7 @@ struct Coder {
         if (m > 1) dec_at(st, 67, m > 2, bits);
       }
     } else {
-      if (!P.b_slice && code.part == 3) {
+      if (!E.b_slice && code.part == 3) {
         for (int k = 0; k < 4; k++) {
           const int s = code.sub[k];
           dec_at(st, 21, s == 0, bits);
@@ -1982,7 +1997,7 @@ struct Coder {
           }
         }
       }
-      if (!P.b_slice && P.nref > 1) {
+      if (!E.b_slice && E.nref > 1) {
         uint32_t grid = 0;  // reference index per 8x8 block so far, a byte each
         const int np = code.part == 3 ? 4 : NPARTS[code.part];
         for (int k = 0; k < np; k++) {
@@ -2018,7 +2033,7 @@ struct Coder {
         dec_at(st, 81 + ca + 2 * cb, cbpc == 2, bits);
       }
     }
-    if (kind == K_INTER && P.transform8 && cbpl != 0 && (code.part != 3 || (code.sub[0] | code.sub[1] | code.sub[2] | code.sub[3]) == 0))
+    if (kind == K_INTER && E.transform8 && cbpl != 0 && (code.part != 3 || (code.sub[0] | code.sub[1] | code.sub[2] | code.sub[3]) == 0))
       dec_at(st, 399 + (left && left->t8x8) + (top && top->t8x8), code.t8x8, bits);
     if (code.cbp != 0 || kind == K_I16) dec_at(st, 60, 0, bits);
     return bits;
@@ -2066,11 +2081,11 @@ struct Coder {
   // one component of the motion vector differences, partitions in coding order, list 0's then list 1's
   // in a B slice (enc/frame.rs:2110-2150; the two lists share the component's contexts); min(|mvd|, 64)
   // of the 4x4 blocks coded so far a byte each in two registers
-  __device__ uint32_t mvd_comp_bins(uint8_t *st, int comp) const {
+  __device__ uint32_t mvd_comp_bins(uint8_t *st, int comp, const CabEnv &E) const {
     const ia_mbcode &code = L.code;
     if (code.kind != K_INTER || code.skip) return 0;
     const MbState *left = nb(-1, 0), *top = nb(0, -1);
-    const int base = comp == 0 ? 40 : 47, part = code.part, bsl = P.b_slice;
+    const int base = comp == 0 ? 40 : 47, part = code.part, bsl = E.b_slice;
     uint32_t bits = 0;
     for (int l = 0; l < (bsl ? 2 : 1); l++) {
     uint64_t g[2] = {0, 0};
@@ -2175,8 +2190,8 @@ struct Coder {
       lctx = IA_LAST_OFF[0][5] + j - 15;
     } else {
       smask = tmask = 1ull << j;
-      sctx = IA_SIG_OFF[0][cat] + j;
-      lctx = IA_LAST_OFF[0][cat] + j;
+      sctx = sig_off(cat) + j;
+      lctx = last_off(cat) + j;
     }
     const Words nzw = cat_words(0, cat);
     uint32_t ss = smask ? st[sctx] : 0, ts = tmask ? st[lctx] : 0, bits = 0;
@@ -2206,7 +2221,7 @@ struct Coder {
   // the levels above 1 are read. Lane k = 0 also counts the bypass bins of the signs, the lanes of the
   // further bins the Exp-Golomb suffixes of their levels.
   __device__ uint32_t level_bins(uint8_t *st, int cat, int k) const {
-    const int ctx = IA_ABS_OFF[cat] + k, gmax = 4 - (cat == 3);
+    const int ctx = abs_off(cat) + k, gmax = 4 - (cat == 3);
     const Words nzw = cat_words(0, cat), g1w = cat_words(1, cat);
     uint32_t s = st[ctx], bits = 0;
     for (int b = 0; b < cat_blocks(cat); b++) {
@@ -2272,11 +2287,13 @@ struct Coder {
     if (cbpl && t8) add(5);
     if (count_chroma && cbpc > 0) add(3);
     if (count_chroma && cbpc == 2) add(4);
+    const CabEnv E{__builtin_amdgcn_readfirstlane(P.p_slice), __builtin_amdgcn_readfirstlane(P.b_slice),
+                   __builtin_amdgcn_readfirstlane(P.transform8), __builtin_amdgcn_readfirstlane(P.nref)};
     uint32_t bits = 0, cbits = 0;
     for (int j = lane(); j < total; j += 64) {
-      if (j == 0) bits += header_bins(st);
+      if (j == 0) bits += header_bins(st, E);
       else if (j == 1) bits += mode_bins(st);
-      else if (j < 4) bits += mvd_comp_bins(st, j - 2);
+      else if (j < 4) bits += mvd_comp_bins(st, j - 2, E);
       else {
         int t = j - 4, ci = 0;
         while (t >= cat_jobs((cats >> (4 * ci)) & 15)) t -= cat_jobs((cats >> (4 * ci++)) & 15);

@@ -90,7 +90,7 @@ def emit(name, base, value, prefix):
         return "{\n  " + ",\n  ".join(init(y, depth + 1) for y in x) + "}"
 
     ds = "".join(f"[{n}]" for n in dims(value))
-    return f"static const {CTYPE[base]} {prefix}{name}{ds} = " + init(value) + ";\n"
+    return f"{prefix}TABLE {CTYPE[base]} {prefix}{name}{ds} = " + init(value) + ";\n"
 
 
 def main():
@@ -102,6 +102,12 @@ def main():
         f"#ifndef {prefix}TABLES_H",
         f"#define {prefix}TABLES_H",
         "#include <stdint.h>",
+        "/* constant expressions in C++, so that code may select among entries without reading memory */",
+        "#ifdef __cplusplus",
+        f"#define {prefix}TABLE static constexpr",
+        "#else",
+        f"#define {prefix}TABLE static const",
+        "#endif",
         "",
     ]
     for rel, names in (WANT_DECODER if decoder else WANT).items():
