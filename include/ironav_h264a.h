@@ -281,15 +281,19 @@ typedef struct ia_batch ia_batch;
 
 typedef struct ia_frame {
   int qp;                 /* slice QP, 0..51 */
-  int kind;               /* IA_PIC_IDR or IA_PIC_P */
-  int nref;               /* P pictures: references used, 1..nrefs */
-  int ref_slots[16];      /* P pictures: slots, newest first (as plist, enc/mod.rs:559) */
+  int kind;               /* IA_PIC_IDR, IA_PIC_P or IA_PIC_B (CABAC streams) */
+  int nref;               /* P pictures: references used, 1..nrefs; B pictures: 2 */
+  int ref_slots[16];      /* P pictures: slots, newest first (as plist, enc/mod.rs:559); B pictures:
+                             [0] the list 0 picture, [1] the list 1 picture (as ia_analyse_b) */
   int promote_slot;       /* -1, or the slot that receives this picture's reconstruction and records
-                             as a reference when the step is done */
+                             as a reference when the step is done (-1 for B pictures, which are not
+                             references) */
   const uint8_t *y, *cb, *cr;  /* the picture's padded planes in device memory, or null: the context's
                                   uploaded source (ia_upload_source) */
   int promote_deblock;    /* non-zero: the promoted reference is loop-filtered first, as the encoder
                              keeps its references (enc/mod.rs:653-654); 0: kept as reconstructed */
+  int dist_l0, dist_l1;   /* B pictures: picture-order distances list 0 -> this picture and list 0 ->
+                             list 1, 0 < dist_l0 < dist_l1 (as ia_analyse_b) */
 } ia_frame;
 
 int ia_batch_create(ia_batch **out, ia_ctx *const *ctxs, int n, int use_graph);

@@ -67,7 +67,7 @@ class Frame(C.Structure):
     _fields_ = [
         ("qp", C.c_int), ("kind", C.c_int), ("nref", C.c_int), ("ref_slots", C.c_int * 16),
         ("promote_slot", C.c_int), ("y", C.c_void_p), ("cb", C.c_void_p), ("cr", C.c_void_p),
-        ("promote_deblock", C.c_int),
+        ("promote_deblock", C.c_int), ("dist_l0", C.c_int), ("dist_l1", C.c_int),
     ]
 
 

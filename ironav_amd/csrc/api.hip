@@ -102,6 +102,24 @@ int analysis_params(ia_ctx *c, int qp, int kind, const int *ref_slots, int nref,
   return IA_OK;
 }
 
+int b_params(ia_ctx *c, int qp, int l0, int l1, int dist_l0, int dist_l1, const uint8_t *const *src, ia::AnParams *pp) {
+  if (!c->cfg.cabac) return fail(c, IA_EUNSUPPORTED, "B pictures are coded with CABAC only");
+  if (dist_l0 <= 0 || dist_l1 <= dist_l0)
+    return fail(c, IA_EINVAL, "the B picture must lie between its references: 0 < dist_l0 < dist_l1");
+  const int slots[2] = {l0, l1};
+  int rc = analysis_params(c, qp, IA_PIC_P, slots, 2, src, false, pp);
+  if (rc) return rc;
+  // one picture per list: planes [0] list 0, [1] list 1; the co-located records are list 1's
+  ia::AnParams &p = *pp;
+  p.fs = nullptr;
+  p.nref = 1;
+  p.b_slice = 1;
+  p.dist_tb = dist_l0;
+  p.dist_td = dist_l1;
+  p.rmbs0 = c->slots[l1].mbs;
+  return IA_OK;
+}
+
 }  // namespace iah
 
 using iah::fail;
@@ -395,20 +413,9 @@ int ia_analyse_uploaded(ia_ctx *c, int qp, int kind, const int *ref_slots, int n
 
 int ia_analyse_b(ia_ctx *c, int qp, int l0_slot, int l1_slot, int dist_l0, int dist_l1) {
   if (!c) return IA_EINVAL;
-  if (!c->cfg.cabac) return fail(c, IA_EUNSUPPORTED, "B pictures are coded with CABAC only");
-  if (dist_l0 <= 0 || dist_l1 <= dist_l0)
-    return fail(c, IA_EINVAL, "the B picture must lie between its references: 0 < dist_l0 < dist_l1");
-  const int slots[2] = {l0_slot, l1_slot};
   ia::AnParams p;
-  int rc = iah::analysis_params(c, qp, IA_PIC_P, slots, 2, nullptr, false, &p);
+  int rc = iah::b_params(c, qp, l0_slot, l1_slot, dist_l0, dist_l1, nullptr, &p);
   if (rc) return rc;
-  // one picture per list: planes [0] list 0, [1] list 1; the co-located records are list 1's
-  p.fs = nullptr;
-  p.nref = 1;
-  p.b_slice = 1;
-  p.dist_tb = dist_l0;
-  p.dist_td = dist_l1;
-  p.rmbs0 = c->slots[l1_slot].mbs;
   IA_HIP(c, ia::launch_analyse(p, c->stream));
   c->an_valid = true;
   c->an_b = true;

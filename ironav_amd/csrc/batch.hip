@@ -309,8 +309,16 @@ int ia_batch_step(ia_batch *b, const ia_frame *frames) {
     if (f.promote_slot < -1 || f.promote_slot >= c->cfg.nrefs)
       return bfail(b, IA_EINVAL, "stream " + std::to_string(s) + ": promote_slot out of range");
     const uint8_t *src[3] = {f.y, f.cb, f.cr};
+    // the searches run for P pictures only (a B picture searches inside the analysis)
     const int nref = f.kind == IA_PIC_P ? f.nref : 0;
-    int rc = iah::analysis_params(c, f.qp, f.kind, f.ref_slots, nref, src, false, &b->an_h()[s]);
+    int rc;
+    if (f.kind == IA_PIC_B) {
+      if (f.nref != 2) return bfail(b, IA_EINVAL, "stream " + std::to_string(s) + ": a B picture takes one picture per list (nref 2)");
+      if (f.promote_slot >= 0) return bfail(b, IA_EINVAL, "stream " + std::to_string(s) + ": a B picture is not a reference");
+      rc = iah::b_params(c, f.qp, f.ref_slots[0], f.ref_slots[1], f.dist_l0, f.dist_l1, src, &b->an_h()[s]);
+    } else {
+      rc = iah::analysis_params(c, f.qp, f.kind, f.ref_slots, nref, src, false, &b->an_h()[s]);
+    }
     if (rc) return bfail(b, rc, "stream " + std::to_string(s) + ": " + c->err);
     ia::FsArgs &fa = b->fs_h()[s];
     ia::PropArgs &pa = b->pp_h()[s];
@@ -412,6 +420,7 @@ int ia_batch_step(ia_batch *b, const ia_frame *frames) {
       c->fs_valid = false;
     }
     c->an_valid = true;
+    c->an_b = f.kind == IA_PIC_B;
     if (f.promote_slot >= 0) c->slots[f.promote_slot].loaded = true;
   }
   return IA_OK;

@@ -74,6 +74,11 @@ int check_refs(ia_ctx *c, const int *ref_slots, int nref);
 int analysis_params(ia_ctx *c, int qp, int kind, const int *ref_slots, int nref, const uint8_t *const *src,
                     bool need_fs, ia::AnParams *p);
 
+// The analysis parameters of a B picture of c between the pictures in slots l0 (list 0) and l1
+// (list 1) at picture-order distances dist_l0 (list 0 -> this picture) and dist_l1 (list 0 -> list 1);
+// src as for analysis_params. Fails on invalid arguments (CABAC only, 0 < dist_l0 < dist_l1).
+int b_params(ia_ctx *c, int qp, int l0, int l1, int dist_l0, int dist_l1, const uint8_t *const *src, ia::AnParams *p);
+
 // The exhaustive search's arguments for c against ref_slots (src_y: the picture's luma, null: the
 // context's uploaded source).
 void search_args(ia_ctx *c, const int *ref_slots, int nref, const uint8_t *src_y, ia::FsArgs *a);

@@ -308,7 +308,8 @@ class Batch:
     def step(self, frames):
         """frames[i] for analyser i: dict(qp, kind, refs=[slots newest first], promote=slot or -1,
         deblock=True (the promoted reference loop-filtered first, as the encoder keeps it),
-        planes=(y, cb, cr) device addresses or None for the analyser's uploaded source)."""
+        planes=(y, cb, cr) device addresses or None for the analyser's uploaded source). A B picture
+        (kind abi.PIC_B) takes refs=[list 0 slot, list 1 slot] and dist=(dist_l0, dist_l1)."""
         if len(frames) != len(self.analysers):
             raise ValueError("one frame per stream")
         arr = (abi.Frame * len(frames))()
@@ -319,6 +320,7 @@ class Batch:
                 f.ref_slots[i] = r
             f.promote_slot = d.get("promote", -1)
             f.promote_deblock = 1 if d.get("deblock", True) else 0
+            f.dist_l0, f.dist_l1 = d.get("dist", (0, 0))
             planes = d.get("planes")
             if planes is not None:
                 f.y, f.cb, f.cr = planes

@@ -202,3 +202,97 @@ def test_batch_with_learned_streams_matches_single_streams(use_graph):
                 assert same(an.read_analysis(), want[i][t]), (i, t)
     for an in ans:
         an.close()
+
+
+# B pictures in the batch: CABAC streams (width, height, seed, search range, nrefs, slices) coded with
+# two B pictures between references, beside a CAVLC stream coding IDR and P pictures in the same steps
+B_STREAMS = [(64, 48, 11, 16, 2, 1), (96, 64, 12, 16, 3, 2), (128, 96, 13, 32, 2, 3)]
+B_ORDER = [0, 3, 1, 2, 6, 4, 5]  # display index of each step's picture (the encoder's coding order)
+
+
+def b_schedule(nrefs, t):
+    """Step t's frame: references (display index a multiple of 3) go to slot (index / 3) % nrefs and
+    are predicted from the references before them, newest first; a B picture lies between the
+    references before and after it."""
+    d = B_ORDER[t]
+    k = d // 3
+    if d % 3 == 0:
+        if d == 0:
+            return dict(qp=25, kind=abi.PIC_IDR, refs=[], promote=0)
+        return dict(qp=28, kind=abi.PIC_P, refs=[(k - 1 - j) % nrefs for j in range(min(k, nrefs))], promote=k % nrefs)
+    return dict(qp=31, kind=abi.PIC_B, refs=[k % nrefs, (k + 1) % nrefs], promote=-1, dist=(d - 3 * k, 3))
+
+
+def b_clip(w, h, seed):
+    return [Picture(*clipgen.pad_planes(*f)) for f in clipgen.make_clip(clipgen.ClipSpec(w, h, len(B_ORDER), seed))]
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_b_pictures_match_single_streams(use_graph):
+    """B pictures through the batched step (eager and as a replayed graph) equal the single-context
+    path's ia_analyse_b, with the P and IDR pictures of the same streams and a CAVLC stream's P
+    pictures in the same steps."""
+    import torch
+
+    p_stream = STREAMS[3]  # CAVLC
+    want = []
+    for (w, h, seed, rng, nrefs, slices) in B_STREAMS:
+        pics = b_clip(w, h, seed)
+        res = []
+        with analyser(w, h, rng, nrefs, slices, 1, 1) as an:
+            for t in range(len(B_ORDER)):
+                f = b_schedule(nrefs, t)
+                pic = pics[B_ORDER[t]]
+                if f["kind"] == abi.PIC_B:
+                    res.append(an.analyse_b(pic, f["qp"], f["refs"][0], f["refs"][1], *f["dist"]))
+                else:
+                    res.append(an.analyse(pic, f["qp"], f["kind"], f["refs"]))
+                    an.promote_reference(f["promote"], deblock=True)
+        want.append(res)
+    (pw, ph, pseed, prng, pnrefs, pslices, pcabac, pt8) = p_stream
+    ppics = b_clip(pw, ph, pseed)
+    pwant = []
+    with analyser(pw, ph, prng, pnrefs, pslices, pcabac, pt8) as an:
+        for t in range(len(B_ORDER)):
+            qp, kind, refs, promote = schedule(pnrefs, t)
+            pwant.append(an.analyse(ppics[t], qp, kind, refs))
+            an.promote_reference(promote, deblock=True)
+
+    clips = [b_clip(w, h, seed) for (w, h, seed, *_rest) in B_STREAMS]
+    ans = [analyser(w, h, rng, nrefs, slices, 1, 1) for (w, h, seed, rng, nrefs, slices) in B_STREAMS]
+    ans.append(analyser(pw, ph, prng, pnrefs, pslices, pcabac, pt8))
+    try:
+        with Batch(ans, use_graph=use_graph) as b:
+            for t in range(len(B_ORDER)):
+                frames = []
+                for i, (w, h, seed, rng, nrefs, slices) in enumerate(B_STREAMS):
+                    ans[i].upload(clips[i][B_ORDER[t]])
+                    frames.append(dict(b_schedule(nrefs, t), deblock=True))
+                qp, kind, refs, promote = schedule(pnrefs, t)
+                ans[-1].upload(ppics[t])
+                frames.append(dict(qp=qp, kind=kind, refs=refs, promote=promote, deblock=True))
+                torch.cuda.synchronize()
+                b.step(frames)
+                b.sync()
+                for i in range(len(B_STREAMS)):
+                    assert same(ans[i].read_analysis(), want[i][t]), (B_STREAMS[i], t, B_ORDER[t])
+                assert same(ans[-1].read_analysis(), pwant[t]), (p_stream, t)
+    finally:
+        for an in ans:
+            an.close()
+
+
+def test_batch_refuses_b_picture_as_reference():
+    with analyser(64, 48, 16, 2, 1, 1, 1) as an, Batch([an], use_graph=False) as b:
+        pic = b_clip(64, 48, 3)[0]
+        an.upload(pic)
+        b.step([dict(qp=25, kind=abi.PIC_IDR, refs=[], promote=0)])
+        b.sync()
+        an.upload(pic)
+        b.step([dict(qp=28, kind=abi.PIC_P, refs=[0], promote=1)])
+        b.sync()
+        an.upload(pic)
+        with pytest.raises(ValueError, match="not a reference"):
+            b.step([dict(qp=31, kind=abi.PIC_B, refs=[0, 1], promote=0, dist=(1, 2))])
+        with pytest.raises(ValueError, match="0 < dist_l0 < dist_l1"):
+            b.step([dict(qp=31, kind=abi.PIC_B, refs=[0, 1], promote=-1, dist=(2, 2))])
