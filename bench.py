@@ -126,6 +126,8 @@ def main():
                          "the diamond search)")
     ap.add_argument("--streams", type=int, default=64, help="batched: streams")
     ap.add_argument("--frames", type=int, default=30, help="batched: frames per stream")
+    ap.add_argument("--bframes", type=int, default=0,
+                    help="batched: B pictures between references (the encoder's coding order; 0: IPPP)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (whole streams, or slice bands with a "
@@ -867,7 +869,9 @@ def bench_batched(args):
     are generated on the device and stay resident; the timed region is the P frames 2..F-1 (frame 0
     is the IDR, frame 1 the first P and the graph's capture). Each stream's results at frames 0 and 1
     and at the last timed frame are compared with the single-stream path's coding the same chain
-    (digests)."""
+    (digests). With --bframes N the streams code the encoder's order with N B pictures between
+    references (h264w.coding_order; B QP 33, picture-order distances two per picture, each B picture
+    between the references coded before and after it, not itself a reference)."""
     import numpy as np
     import torch
 
@@ -892,6 +896,8 @@ def bench_batched(args):
     slices = [min(max(1, round(n / target)), (h + 15) // 16) for n, (w, h) in zip(nmbs, order)]
     # the live mode (enc/mod.rs:249-254): realtime decisions, one reference, the reference's diamond search
     nrefs = 1 if args.realtime else 3
+    if args.bframes and args.realtime:
+        raise SystemExit("--bframes: the live mode codes P pictures only (one reference)")
     mode = abi.SEARCH_DIAMOND if args.realtime else abi.SEARCH_FULL
     table = autotune.load()
     ans = []
@@ -901,11 +907,27 @@ def bench_batched(args):
         autotune.apply(an, table)
         ans.append(an)
 
+    # the frame of each step: references (IDR, P) in slot (their count so far) % nrefs, P pictures
+    # against the newest nrefs references; a B picture between the references coded before and after it
+    from ironav_amd import h264w
+
+    gop = h264w.coding_order([True] + [False] * (args.frames - 1), args.bframes)
+    sched, coded = [], []  # coded: (display, slot) of the references so far
+    for d, kind in gop:
+        if kind == abi.PIC_B:
+            (d0, s0), (d1, s1) = coded[-2], coded[-1]
+            assert d0 < d < d1
+            sched.append(dict(display=d, qp=33, kind=kind, refs=[s0, s1], promote=-1, dist=(2 * (d - d0), 2 * (d1 - d0))))
+        else:
+            k = len(coded)
+            refs = [coded[-1 - j][1] for j in range(min(k, nrefs))] if kind == abi.PIC_P else []
+            sched.append(dict(display=d, qp=25 if kind == abi.PIC_IDR else 28, kind=kind, refs=refs, promote=k % nrefs))
+            coded.append((d, k % nrefs))
+
     def frame_args(i, t):
-        y, cb, cr = frames[i][t]
-        refs = [(t - 1 - k) % nrefs for k in range(min(t, nrefs))]
-        return dict(qp=25 if t == 0 else 28, kind=abi.PIC_IDR if t == 0 else abi.PIC_P, refs=refs,
-                    promote=t % nrefs, planes=(y.data_ptr(), cb.data_ptr(), cr.data_ptr()))
+        f = dict(sched[t])
+        y, cb, cr = frames[i][f.pop("display")]
+        return dict(f, planes=(y.data_ptr(), cb.data_ptr(), cr.data_ptr()))
 
     want = [[None, None] for _ in ans]
     with Batch(ans, use_graph=True) as b:
@@ -938,10 +960,13 @@ def bench_batched(args):
             autotune.apply(an, table)
             ok = True
             for t in range(args.frames):
-                pic = Picture(*(p.cpu().numpy() for p in frames[i][t]))
+                pic = Picture(*(p.cpu().numpy() for p in frames[i][sched[t]["display"]]))
                 a = frame_args(i, t)
-                r = an.analyse(pic, a["qp"], a["kind"], a["refs"])
-                an.promote_reference(a["promote"])  # loop filter on, as the batch promotes
+                if a["kind"] == abi.PIC_B:
+                    r = an.analyse_b(pic, a["qp"], a["refs"][0], a["refs"][1], *a["dist"])
+                else:
+                    r = an.analyse(pic, a["qp"], a["kind"], a["refs"])
+                    an.promote_reference(a["promote"])  # loop filter on, as the batch promotes
                 if t in checked:
                     ok = ok and digest(r) == want[i][checked.index(t)]
             equal += ok
@@ -966,9 +991,12 @@ def bench_batched(args):
         "vs_baseline": None,
         "dtype": "u8",
         "data": f"synthetic (device-generated clips, {gen_s:.1f} s; resident in HBM)",
-        "config": {"workload": f"{args.streams} streams x {args.frames} frames, sizes {sorted(set(order))}, IPPP "
-                               f"(IDR QP 25, P QP 28), {nrefs} reference(s), {search}, CABAC + 8x8 transform, "
-                               "one hipGraph per frame step",
+        "config": {"workload": f"{args.streams} streams x {args.frames} frames, sizes {sorted(set(order))}, "
+                               + (f"{args.bframes} B pictures between references (IDR QP 25, P QP 28, B QP 33), "
+                                  if args.bframes else "IPPP (IDR QP 25, P QP 28), ")
+                               + f"{nrefs} reference(s), {search}, CABAC + 8x8 transform, one hipGraph per frame step",
+                   "picture_types_timed": {k: sum(1 for s in sched[2:] if s["kind"] == v)
+                                           for k, v in (("P", abi.PIC_P), ("B", abi.PIC_B))},
                    "macroblocks_per_step": mb_per_step,
                    "streams_per_size": {f"{w}x{h}": order.count((w, h)) for w, h in sizes},
                    "slices": {f"{w}x{h}": s for (w, h), s in zip(order, slices)},
