@@ -38,8 +38,11 @@ typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) u32x2 LdsU32x2;
 
 constexpr int WAVES = 8;
-constexpr int PG = CNN_GRID + 2;   // padded grid side (one cell of zeros around)
-constexpr int PCELLS = PG * PG;
+// The grid in LDS: rows of CNN_GRID cells with no padding between them, a row of zeros above and
+// below (a 3x3 tap off the grid's left or right edge reads cell 0, in the zero row above): so 16
+// consecutive cells of an MFMA tile lie at 16 consecutive positions, whose 80-byte activation
+// cells fall on distinct LDS banks.
+constexpr int PCELLS = CNN_GRID * (CNN_GRID + 2);
 constexpr int AST = 40;            // bf16 per cell in an activation buffer (32 channels + 8 of padding)
 constexpr int WIN = 52;            // staged reference window side: +-18 displacements around the centre + 16
 constexpr int WST = 19;            // dwords per staged window row in LDS (padded against bank conflicts)
@@ -147,8 +150,14 @@ struct PropLds {
 // 256 bytes.
 __device__ __forceinline__ int wswz(int r, int g) { return (g + (r >> 2) + 2 * ((r >> 1) & 1)) & 3; }
 
-__device__ __forceinline__ int padpos(int p) { return (p / CNN_GRID + 1) * PG + (p % CNN_GRID) + 1; }
-__device__ __forceinline__ int tapoff(int t) { return (t / 3 - 1) * PG + (t % 3 - 1); }
+__device__ __forceinline__ int padpos(int p) { return p + CNN_GRID; }
+__device__ __forceinline__ int tapoff(int t) { return (t / 3 - 1) * CNN_GRID + (t % 3 - 1); }
+// the position tap t of the cell at position pp (grid column cx) reads: 0 (a zero cell) off the
+// grid's left or right edge; rows above and below the grid are zero rows
+__device__ __forceinline__ int tappos(int pp, int cx, int t) {
+  const int dx = t % 3 - 1;
+  return (dx < 0 && cx == 0) || (dx > 0 && cx == CNN_GRID - 1) ? 0 : pp + tapoff(t);
+}
 
 // One 3x3 convolution layer on a wave's 9x9 grid. IN0: the first layer (8-channel input); NT: 16-wide
 // output tiles; RELU: activation. Reads `in` (stride `ist`), writes bf16 to `outb` or f32 to `outf`.
@@ -181,10 +190,12 @@ __device__ void conv_layer(int lane, const LdsU16 *in, int ist, const LdsU16 *w,
   // K steps outermost: the 6 x NT accumulators are independent, so consecutive MFMAs never wait on
   // each other; each accumulator still sums its K steps in order
   f32x4 acc[6][NT];
-  int pp[6];
+  int pp[6], cx[6];
 #pragma unroll
   for (int mt = 0; mt < 6; mt++) {
-    pp[mt] = padpos(min(mt * 16 + m, CNN_CELLS - 1));  // this lane's B column (grid cell) of tile mt
+    const int q = min(mt * 16 + m, CNN_CELLS - 1);  // this lane's B column (grid cell) of tile mt
+    pp[mt] = padpos(q);
+    cx[mt] = q % CNN_GRID;
 #pragma unroll
     for (int nt = 0; nt < NT; nt++) acc[mt][nt] = bv[nt];
   }
@@ -195,9 +206,9 @@ __device__ void conv_layer(int lane, const LdsU16 *in, int ist, const LdsU16 *w,
     for (int mt = 0; mt < 6; mt++) {
       if (IN0) {
         const int t = ks * 4 + g;
-        x[mt] = t < 9 ? *(const LdsBf16x8 *)(in + (pp[mt] + tapoff(t)) * ist) : bf16x8{};
+        x[mt] = t < 9 ? *(const LdsBf16x8 *)(in + tappos(pp[mt], cx[mt], t) * ist) : bf16x8{};
       } else {
-        x[mt] = *(const LdsBf16x8 *)(in + (pp[mt] + tapoff(ks)) * ist + g * 8);
+        x[mt] = *(const LdsBf16x8 *)(in + tappos(pp[mt], cx[mt], ks) * ist + g * 8);
       }
     }
     // one activation fragment feeds every output tile
