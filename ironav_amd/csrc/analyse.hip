@@ -189,6 +189,7 @@ struct Lds {
   uint64_t nzm[2][16];  // masks of the nonzero levels and of the levels above 1 (count_bins)
   uint64_t pos8[24];    // 8x8 scan positions of each significance (0-14) and last (15-23) context
   int etop[16], eleft[8], etl;  // the edges of the Intra4x4 block being coded
+  int e4[16];                   // the same as one array: the left column bottom to top, the corner, the row above
   uint32_t kept_bits;           // the kept candidate's rate
   uint8_t kept_st[440];         // the context states (0-439, those the slice codes on) after its bins
   uint8_t chroma_st[64];        // the chroma residual's context states after the last full count
@@ -1151,6 +1152,66 @@ struct Coder {
 #undef LF
   }
 
+  // Intra4x4 sample (x, y) of mode m (not DC, h264/intra.rs:19-127) as taps on the edge array E (E[0..3]
+  // the left column from the bottom up, E[4] the corner, E[5..12] the row above and above-right): the
+  // sample is (w0 E[i0] + w1 E[i1] + w2 E[i2] + 2) >> 2, packed i0 | i1 << 4 | i2 << 8 | w0 << 12 |
+  // w1 << 15 | w2 << 18. Each mode's cases of nxn_sample in that form, so that lanes of different
+  // modes run one branch-free evaluation.
+  __device__ static uint32_t tap4(int m, int x, int y) {
+    int i0 = 0, i1 = 0, i2 = 0, w0 = 0, w1 = 0, w2 = 0;
+    auto t3 = [&](int a) {  // E[a] + 2 E[a + 1] + E[a + 2]
+      i0 = a; i1 = a + 1; i2 = a + 2; w0 = 1; w1 = 2; w2 = 1;
+    };
+    auto t2 = [&](int a, int b) {  // (E[a] + E[b] + 1) >> 1
+      i0 = a; i1 = b; w0 = 2; w1 = 2;
+    };
+    switch (m) {
+      case 0: i0 = 5 + x; w0 = 4; break;
+      case 1: i0 = 3 - y; w0 = 4; break;
+      case 3:
+        if (x == 3 && y == 3) { i0 = 11; i1 = 12; w0 = 1; w1 = 3; }
+        else t3(5 + x + y);
+        break;
+      case 4: t3(3 + x - y); break;
+      case 5: {
+        const int z = 2 * x - y, k = x - (y >> 1);
+        if (z >= 0 && !(z & 1)) t2(4 + k, 5 + k);
+        else if (z >= 0) t3(3 + k);
+        else if (z == -1) t3(3);
+        else t3(4 + z);
+        break;
+      }
+      case 6: {
+        const int z = 2 * y - x, k = y - (x >> 1);
+        if (z >= 0 && !(z & 1)) t2(4 - k, 3 - k);
+        else if (z >= 0) t3(3 - k);
+        else if (z == -1) t3(3);
+        else t3(2 - z);
+        break;
+      }
+      case 7: {
+        const int k = x + (y >> 1);
+        if (!(y & 1)) t2(5 + k, 6 + k);
+        else t3(5 + k);
+        break;
+      }
+      case 8: {
+        const int z = x + 2 * y, k = y + (x >> 1);
+        if (z > 5) { i0 = 0; w0 = 4; }
+        else if (z == 5) { i0 = 1; i1 = 0; w0 = 1; w1 = 3; }
+        else if (!(z & 1)) t2(3 - k, 2 - k);
+        else t3(1 - k);
+        break;
+      }
+      default: break;
+    }
+    return (uint32_t)(i0 | i1 << 4 | i2 << 8 | w0 << 12 | w1 << 15 | w2 << 18);
+  }
+  __device__ static int tap_eval(uint32_t t, const int *e) {
+    return ((int)((t >> 12) & 7) * e[t & 15] + (int)((t >> 15) & 7) * e[(t >> 4) & 15] +
+            (int)((t >> 18) & 7) * e[(t >> 8) & 15] + 2) >> 2;
+  }
+
   __device__ static int mode_ok(int mode, int have_top, int have_left, int have_tl) {
     if (mode == 0 || mode == 3 || mode == 7) return have_top;
     if (mode == 1 || mode == 8) return have_left;
@@ -1165,6 +1226,10 @@ struct Coder {
     const int qp = P.qp;
     const int l = lane();
     int cost = 0;
+    // the lane's taps for its mode of each pass below (mode 4 pass + l / 16, sample l % 16)
+    uint32_t taps[3];
+#pragma unroll
+    for (int pass = 0; pass < 3; pass++) taps[pass] = tap4(4 * pass + (l >> 4), l & 3, (l >> 2) & 3);
     for (int n = 0; n < 16; n++) {
       int bx, by;
       block_pos(n, &bx, &by);
@@ -1183,9 +1248,28 @@ struct Coder {
       if (l < 8) L.etop[l] = have_top ? cyv(x + (l < 4 || have_tr ? l : 3), y - 1) : 0;
       else if (l < 12) L.eleft[l - 8] = have_left ? cyv(x - 1, y + l - 8) : 0;
       else if (l == 12) L.etl = have_tl ? cyv(x - 1, y - 1) : 0;
+      else if (l >= 16 && l < 29) {
+        const int j = l - 16;  // E[j]
+        int v;
+        if (j < 4) v = have_left ? cyv(x - 1, y + 3 - j) : 0;
+        else if (j == 4) v = have_tl ? cyv(x - 1, y - 1) : 0;
+        else v = have_top ? cyv(x + (j - 5 < 4 || have_tr ? j - 5 : 3), y - 1) : 0;
+        L.e4[j] = v;
+      }
       sync();
       const int tl = L.etl;
       const int predicted = pred_i4_mode(L.code.modes, bx, by);
+      // DC (mode 2) for every lane at once (nxn_sample's case 2)
+      int dc;
+      {
+        int st = 0, sl = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          st += L.e4[5 + i];
+          sl += L.e4[3 - i];
+        }
+        dc = have_top && have_left ? (st + sl + 4) >> 3 : have_left ? (sl + 2) >> 2 : have_top ? (st + 2) >> 2 : 128;
+      }
       // the nine modes' SATDs, four modes a pass: lane 16 g + i takes sample i (row i >> 2, column
       // i & 3) of mode 4 pass + g; the horizontal Hadamard within the lane's quad (a row), the
       // vertical one through LDS, the absolute values summed over the 16 lanes (enc/frame.rs:641-659)
@@ -1195,7 +1279,7 @@ struct Coder {
         const int m = 4 * pass + (l >> 4), i = l & 15, xx = i & 3, yy = i >> 2;
         const bool ok = m < 9 && mode_ok(m, have_top, have_left, have_tl);
         int v = 0;
-        if (ok) v = (int)L.src_y[(y + yy) * 16 + x + xx] - nxn_sample(4, m, xx, yy, L.etop, L.eleft, tl, have_top, have_left);
+        if (ok) v = (int)L.src_y[(y + yy) * 16 + x + xx] - (m == 2 ? dc : tap_eval(taps[pass], L.e4));
         int p = __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, false);  // quad_perm [1, 0, 3, 2]
         v = (xx & 1) ? p - v : v + p;
         p = __builtin_amdgcn_update_dpp(0, v, 0x4e, 0xf, 0xf, false);  // quad_perm [2, 3, 0, 1]
