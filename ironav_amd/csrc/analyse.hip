@@ -2354,9 +2354,13 @@ struct Coder {
   // there; otherwise *chroma is taken as their cost (the same chroma coded before on the same
   // states) and their contexts are left alone.
   __device__ uint32_t count_bins(uint8_t *st, int *chroma) {
-    const ia_mbcode &code = L.code;
     nz_masks();
     sync();
+    return count_masked(st, chroma);
+  }
+  // the same, the masks of nz_masks already in L.nzm
+  __device__ uint32_t count_masked(uint8_t *st, int *chroma) {
+    const ia_mbcode &code = L.code;
     const int cbpl = code.cbp & 15, cbpc = code.cbp >> 4, i16 = code.kind == K_I16, t8 = code.t8x8;
     const int count_chroma = *chroma < 0;
     // the categories coded, in a fixed order, four bits each
@@ -2421,7 +2425,8 @@ struct Coder {
   // J of the candidate in L.code; its rate in *bits and the context states after its bins in L.tmp.
   // When its chroma is that of the last full count (*chroma >= 0), the chroma contexts start at the
   // states that count left them in, so L.tmp still ends as the writer's would.
-  __device__ double rd_cost(double lam2, int *chroma, uint32_t *bits) {
+  // (The masks of nz_masks are in L.nzm and ssd is the candidate's distortion.)
+  __device__ double rd_cost(double lam2, int *chroma, uint32_t *bits, long long ssd) {
     const int full = *chroma < 0;
     {
       const uint32_t *s = reinterpret_cast<const uint32_t *>(L.cabac);
@@ -2431,11 +2436,23 @@ struct Coder {
     sync();
     if (!full && lane() < 61) L.tmp[chroma_ctx(lane())] = L.chroma_st[lane()];
     sync();
-    *bits = count_bins(L.tmp, chroma);
+    *bits = count_masked(L.tmp, chroma);
     if (full && lane() < 61) L.chroma_st[lane()] = L.tmp[chroma_ctx(lane())];
-    const long long ssd = ssd_mb();
     sync();
     return (double)ssd + lam2 * ((double)*bits / 256.0);
+  }
+
+  // A floor of the rate (1/256 bits) of the candidate whose masks nz_masks left in L.nzm: one bypass
+  // bin per nonzero level (its sign), the chroma levels' replaced by the chroma's rate when it is
+  // known (chroma >= 0). The same in every lane.
+  __device__ uint32_t rate_floor(int chroma) const {
+    uint32_t n = 0;
+    for (int w = 0; w < 8; w++) n += (uint32_t)__popcll(L.nzm[0][w]);  // luma 4x4 and 8x8 blocks
+    const uint64_t w10 = L.nzm[0][10];
+    n += (uint32_t)__popcll(w10 & 0xffffull);  // Intra16x16 DC
+    if (chroma >= 0) return 256u * n + (uint32_t)chroma;
+    n += (uint32_t)(__popcll(L.nzm[0][8]) + __popcll(L.nzm[0][9]) + __popcll(w10 >> 16));
+    return 256u * n;
   }
 
   __device__ void copy_code(ia_mbcode *dst, const ia_mbcode *src) {
@@ -2511,9 +2528,21 @@ struct Coder {
     code_luma8();
   }
 
+  // Costs the candidate in L.code and keeps it when its J is the least so far. Its J is at least
+  // its distortion plus lam2 times a floor of its rate (rate_floor; the same monotone double
+  // arithmetic as J's); a candidate whose floor already reaches the kept J could not be kept, so its
+  // bins are not counted (it leaves *chroma and the chroma states alone: the next candidate with
+  // the same chroma counts them).
   __device__ void rd_keep(double lam2, int *chroma, int *have, double *best_j) {
+    nz_masks();
+    sync();
+    const long long ssd = ssd_mb();
+    if (*have && !((double)ssd + lam2 * ((double)rate_floor(*chroma) / 256.0) < *best_j)) {
+      sync();
+      return;
+    }
     uint32_t b;
-    const double j = rd_cost(lam2, chroma, &b);
+    const double j = rd_cost(lam2, chroma, &b, ssd);
     keep(j, b, have, best_j);
   }
 
