@@ -421,6 +421,23 @@ struct Coder {
     return wave_sum(s);
   }
 
+  // The 4x4 Hadamard transform of the 16 lanes of each 16-lane group (lane 16 q + i holds sample i:
+  // row i >> 2, column i & 3), by butterflies between lanes i and i ^ 1, i ^ 2 (quad_perm), i ^ 4
+  // (row_half_mirror then quad_perm [3, 2, 1, 0]) and i ^ 8 (row_mirror then row_half_mirror). The
+  // lanes end with the transform's outputs in another order than its matrix's, which a sum of
+  // magnitudes (the SATD) does not see.
+  __device__ static int hadamard16(int v, int i) {
+    int p = __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, false);
+    v = (i & 1) ? p - v : v + p;
+    p = __builtin_amdgcn_update_dpp(0, v, 0x4e, 0xf, 0xf, false);
+    v = (i & 2) ? p - v : v + p;
+    p = __builtin_amdgcn_update_dpp(0, __builtin_amdgcn_update_dpp(0, v, 0x141, 0xf, 0xf, false), 0x1b, 0xf, 0xf, false);
+    v = (i & 4) ? p - v : v + p;
+    p = __builtin_amdgcn_update_dpp(0, __builtin_amdgcn_update_dpp(0, v, 0x140, 0xf, 0xf, false), 0x141, 0xf, 0xf, false);
+    v = (i & 8) ? p - v : v + p;
+    return v;
+  }
+
   __device__ static int satd4(const int *d) {
     int t[16];
 #pragma unroll
@@ -1271,8 +1288,8 @@ struct Coder {
         dc = have_top && have_left ? (st + sl + 4) >> 3 : have_left ? (sl + 2) >> 2 : have_top ? (st + 2) >> 2 : 128;
       }
       // the nine modes' SATDs, four modes a pass: lane 16 g + i takes sample i (row i >> 2, column
-      // i & 3) of mode 4 pass + g; the horizontal Hadamard within the lane's quad (a row), the
-      // vertical one through LDS, the absolute values summed over the 16 lanes (enc/frame.rs:641-659)
+      // i & 3) of mode 4 pass + g; the Hadamard across the group's lanes (hadamard16), the absolute
+      // values summed over the 16 lanes (enc/frame.rs:641-659)
       int cpass[3];
 #pragma unroll
       for (int pass = 0; pass < 3; pass++) {
@@ -1280,19 +1297,8 @@ struct Coder {
         const bool ok = m < 9 && mode_ok(m, have_top, have_left, have_tl);
         int v = 0;
         if (ok) v = (int)L.src_y[(y + yy) * 16 + x + xx] - (m == 2 ? dc : tap_eval(taps[pass], L.e4));
-        int p = __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, false);  // quad_perm [1, 0, 3, 2]
-        v = (xx & 1) ? p - v : v + p;
-        p = __builtin_amdgcn_update_dpp(0, v, 0x4e, 0xf, 0xf, false);  // quad_perm [2, 3, 0, 1]
-        v = (xx & 2) ? p - v : v + p;
-        L.ival[l] = v;
-        sync();
-        const int *col = &L.ival[(l & ~15) | xx];
-        const int a0 = col[0], a1 = col[4], a2 = col[8], a3 = col[12];
-        const int s0 = a0 + a1, s1 = a2 + a3, d0 = a0 - a1, d1 = a2 - a3;
-        const int o = yy == 0 ? s0 + s1 : yy == 1 ? s0 - s1 : yy == 2 ? d0 - d1 : d0 + d1;
-        const int sum = group_sum(iabs(o), 16) >> 1;
+        const int sum = group_sum(iabs(hadamard16(v, i)), 16) >> 1;
         cpass[pass] = ok ? sum + P.lambda * (m == predicted ? 1 : 4) : 0x7fffffff;
-        sync();
       }
       // the first mode of least cost (strict <, in mode order)
       int best_m = 2, best_c = 0x7fffffff;
@@ -1680,31 +1686,20 @@ struct Coder {
       const int *top = L.etop, *left = L.eleft, tl = L.etl;
       const int predicted = pred_i4_mode(L.code.modes, bx * 2, by * 2);
       // the nine modes' SATDs, a mode a pass (no divergence): lane 16 q + i takes sample i of the
-      // 4x4 quadrant q; the horizontal Hadamard within the lane's quad, the vertical one through LDS,
-      // each quadrant's SATD summed over its 16 lanes, the mode's over the quadrants
+      // 4x4 quadrant q; the Hadamard across the quadrant's 16 lanes (hadamard16), each quadrant's
+      // SATD summed over its 16 lanes, the mode's over the quadrants
       int best_m = 2, best_c = 0x7fffffff;
       {
         const int l = lane(), qd = l >> 4, i = l & 15, xx = (qd & 1) * 4 + (i & 3), yy = (qd >> 1) * 4 + (i >> 2);
         const int sv = L.src_y[(y + yy) * 16 + x + xx];
         for (int m = 0; m < 9; m++) {
           if (!mode_ok(m, have_top, have_left, have_tl)) continue;
-          int v = sv - nxn_sample(8, m, xx, yy, top, left, tl, have_top, have_left);
-          int p = __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, false);  // quad_perm [1, 0, 3, 2]
-          v = (i & 1) ? p - v : v + p;
-          p = __builtin_amdgcn_update_dpp(0, v, 0x4e, 0xf, 0xf, false);  // quad_perm [2, 3, 0, 1]
-          v = (i & 2) ? p - v : v + p;
-          L.ival[l] = v;
-          sync();
-          const int *col = &L.ival[(l & ~15) | (i & 3)];
-          const int a0 = col[0], a1 = col[4], a2 = col[8], a3 = col[12];
-          const int s0 = a0 + a1, s1 = a2 + a3, d0 = a0 - a1, d1 = a2 - a3;
-          const int r = i >> 2, o = r == 0 ? s0 + s1 : r == 1 ? s0 - s1 : r == 2 ? d0 - d1 : d0 + d1;
-          const int q = group_sum(iabs(o), 16) >> 1;
+          const int v = sv - nxn_sample(8, m, xx, yy, top, left, tl, have_top, have_left);
+          const int q = group_sum(iabs(hadamard16(v, i)), 16) >> 1;
           const int sat = __builtin_amdgcn_readlane(q, 0) + __builtin_amdgcn_readlane(q, 16) + __builtin_amdgcn_readlane(q, 32) +
                           __builtin_amdgcn_readlane(q, 48);
           const int c = sat + P.lambda * (m == predicted ? 1 : 4);
           if (c < best_c) { best_c = c; best_m = m; }
-          sync();
         }
       }
       // the prediction into the canvas and the residual, a lane per sample
@@ -1743,21 +1738,10 @@ struct Coder {
         if (!ok) continue;
         int cost = P.lambda * (mm == 0 ? 1 : 3);
         for (int ci = 0; ci < 2; ci++) {
-          int v = (int)L.src_c[ci][yy * 8 + xx] - chroma_sample(ci, mm, xx, yy, ta, la);
-          int p = __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, false);  // quad_perm [1, 0, 3, 2]
-          v = (i & 1) ? p - v : v + p;
-          p = __builtin_amdgcn_update_dpp(0, v, 0x4e, 0xf, 0xf, false);  // quad_perm [2, 3, 0, 1]
-          v = (i & 2) ? p - v : v + p;
-          L.ival[l] = v;
-          sync();
-          const int *col = &L.ival[(l & ~15) | (i & 3)];
-          const int a0 = col[0], a1 = col[4], a2 = col[8], a3 = col[12];
-          const int s0 = a0 + a1, s1 = a2 + a3, d0 = a0 - a1, d1 = a2 - a3;
-          const int r = i >> 2, o = r == 0 ? s0 + s1 : r == 1 ? s0 - s1 : r == 2 ? d0 - d1 : d0 + d1;
-          const int q = group_sum(iabs(o), 16) >> 1;
+          const int v = (int)L.src_c[ci][yy * 8 + xx] - chroma_sample(ci, mm, xx, yy, ta, la);
+          const int q = group_sum(iabs(hadamard16(v, i)), 16) >> 1;
           cost += __builtin_amdgcn_readlane(q, 0) + __builtin_amdgcn_readlane(q, 16) + __builtin_amdgcn_readlane(q, 32) +
                   __builtin_amdgcn_readlane(q, 48);
-          sync();
         }
         if (cost < best_c) { best_c = cost; best_m = mm; }
       }
