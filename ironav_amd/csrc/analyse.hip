@@ -49,7 +49,19 @@ __constant__ uint8_t SUBS[4][4][4] = {
     {{0, 0, 1, 1}, {1, 0, 1, 1}, {0, 1, 1, 1}, {1, 1, 1, 1}}};
 __constant__ int NSUBS[4] = {1, 2, 2, 4};
 
-__constant__ uint8_t ZIG4[16] = {0, 1, 4, 8, 5, 2, 3, 6, 9, 12, 13, 10, 7, 11, 14, 15};
+#define IA_ZIG4 0, 1, 4, 8, 5, 2, 3, 6, 9, 12, 13, 10, 7, 11, 14, 15
+__constant__ uint8_t ZIG4[16] = {IA_ZIG4};
+// the scan position of each raster position, four bits each
+constexpr uint64_t izig4() {
+  constexpr uint8_t z[16] = {IA_ZIG4};
+  uint64_t k = 0;
+  for (int s = 0; s < 16; s++) k |= (uint64_t)s << (4 * z[s]);
+  return k;
+}
+constexpr uint64_t IZIG4 = izig4();
+#undef IA_ZIG4
+// the frequency each position k of a lane-parallel transform dimension holds (fwd4, inv4): 0, 2, 3, 1
+constexpr int perm4(int k) { return (0x1320 >> (4 * k)) & 15; }
 __constant__ uint8_t ZIG8[64] = {0,  1,  8,  16, 9,  2,  3,  10, 17, 24, 32, 25, 18, 11, 4,  5,  12, 19, 26, 33, 40, 48,
                                  41, 34, 27, 20, 13, 6,  7,  14, 21, 28, 35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15, 23,
                                  30, 37, 44, 51, 58, 59, 52, 45, 38, 31, 39, 46, 53, 60, 61, 54, 47, 55, 62, 63};
@@ -436,6 +448,39 @@ struct Coder {
     p = __builtin_amdgcn_update_dpp(0, __builtin_amdgcn_update_dpp(0, v, 0x140, 0xf, 0xf, false), 0x141, 0xf, 0xf, false);
     v = (i & 8) ? p - v : v + p;
     return v;
+  }
+
+  // v of lane i ^ K within each 16-lane group (K 1, 2, 3 by quad_perm; 4, 8, 12 by row mirrors and
+  // quad_perm)
+  template <int K>
+  __device__ static int lane_xor(int v) {
+    if constexpr (K == 1) return __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, false);
+    if constexpr (K == 2) return __builtin_amdgcn_update_dpp(0, v, 0x4e, 0xf, 0xf, false);
+    if constexpr (K == 3) return __builtin_amdgcn_update_dpp(0, v, 0x1b, 0xf, 0xf, false);
+    if constexpr (K == 4)
+      return __builtin_amdgcn_update_dpp(0, __builtin_amdgcn_update_dpp(0, v, 0x141, 0xf, 0xf, false), 0x1b, 0xf, 0xf, false);
+    if constexpr (K == 8)
+      return __builtin_amdgcn_update_dpp(0, __builtin_amdgcn_update_dpp(0, v, 0x140, 0xf, 0xf, false), 0x141, 0xf, 0xf, false);
+    return __builtin_amdgcn_update_dpp(0, __builtin_amdgcn_update_dpp(0, v, 0x140, 0xf, 0xf, false), 0x1b, 0xf, 0xf, false);
+  }
+  // One dimension of the 4x4 forward core transform (enc/frame.rs:2443-2459) over the four lanes
+  // k = 0..3 that lane_xor<X3> and lane_xor<X1> pair as k ^ 3 and k ^ 1: sums and differences of
+  // samples 0, 3 and 1, 2, then of those; position k ends with frequency perm4(k).
+  template <int X3, int X1>
+  __device__ static int fwd4(int v, int k) {
+    int p = lane_xor<X3>(v);
+    v = k < 2 ? v + p : p - v;  // a0 + a3, a1 + a2, a1 - a2, a0 - a3
+    p = lane_xor<X1>(v);
+    return k == 0 ? v + p : k == 1 ? p - v : k == 2 ? p - 2 * v : 2 * v + p;
+  }
+  // One dimension of the inverse core transform (h264/idct.rs:8-32, wrapping) over four lanes holding
+  // frequencies perm4(k) (paired k ^ 1, then k ^ 3): position k ends with sample k.
+  template <int X1, int X3>
+  __device__ static int inv4(int v, int k) {
+    int p = lane_xor<X1>(v);
+    v = k == 0 ? wadd(v, p) : k == 1 ? wsub(p, v) : k == 2 ? wsub(p >> 1, v) : wadd(v, p >> 1);  // e0, e1, e2, e3
+    p = lane_xor<X3>(v);
+    return k < 2 ? wadd(v, p) : wsub(p, v);
   }
 
   __device__ static int satd4(const int *d) {
@@ -1247,6 +1292,12 @@ struct Coder {
     uint32_t taps[3];
 #pragma unroll
     for (int pass = 0; pass < 3; pass++) taps[pass] = tap4(4 * pass + (l >> 4), l & 3, (l >> 2) & 3);
+    // the coefficient the lane holds after the forward transform, its scan position and its
+    // quantiser and dequantiser (enc/frame.rs:126-138)
+    const int coef = perm4((l >> 2) & 3) * 4 + perm4(l & 3);
+    const int scan = (int)((IZIG4 >> (4 * coef)) & 15);
+    const int qbits = 15 + qp / 6, qf = (1 << qbits) / 3;
+    const int mf = IA_MF[qp % 6][class4(coef)], vq = IA_V[qp % 6][class4(coef)];
     for (int n = 0; n < 16; n++) {
       int bx, by;
       block_pos(n, &bx, &by);
@@ -1308,71 +1359,32 @@ struct Coder {
         if (ck < best_c) { best_c = ck; best_m = k; }
       }
       cost += best_c;
-      // the block coded with it (enc/frame.rs:1025-1046, h264/idct.rs:8-32): lane i < 16 predicts
-      // sample i into the canvas and quantises coefficient i of the scan; the transforms take a lane
-      // per row, then per column, through L.co of the block
-      int32_t *co = L.co[by * 4 + bx];
-      if (l < 16) {
-        const int pv = nxn_sample(4, best_m, l & 3, l >> 2, L.etop, L.eleft, tl, have_top, have_left);
-        co[l] = (int)L.src_y[(y + (l >> 2)) * 16 + x + (l & 3)] - pv;
-        cyv(x + (l & 3), y + (l >> 2)) = (uint8_t)pv;
-      }
-      sync();
-      if (l < 4) {  // rows
-        int32_t *a = co + 4 * l;
-        const int s03 = a[0] + a[3], d03 = a[0] - a[3], s12 = a[1] + a[2], d12 = a[1] - a[2];
-        a[0] = s03 + s12;
-        a[1] = 2 * d03 + d12;
-        a[2] = s03 - s12;
-        a[3] = d03 - 2 * d12;
-      }
-      sync();
-      if (l < 4) {  // columns
-        int32_t *a = co + l;
-        const int s03 = a[0] + a[12], d03 = a[0] - a[12], s12 = a[4] + a[8], d12 = a[4] - a[8];
-        a[0] = s03 + s12;
-        a[4] = 2 * d03 + d12;
-        a[8] = s03 - s12;
-        a[12] = d03 - 2 * d12;
-      }
-      sync();
-      int lvl = 0, pz = 0;
-      if (l < 16) {
-        const int qbits = 15 + qp / 6, f = (1 << qbits) / 3, m = qp % 6;
-        pz = ZIG4[l];
-        lvl = quant(co[pz], IA_MF[m][class4(pz)], f, qbits);
-        L.code.luma[by * 4 + bx][l] = (int16_t)lvl;
-      }
-      const int nz = count_lanes(lvl != 0);
-      sync();
-      if (l < 16) co[pz] = shl(lvl * IA_V[qp % 6][class4(pz)], qp / 6);
-      if (l == 0) {
-        L.code.nnz[by * 4 + bx] = (uint8_t)nz;
-        L.code.modes[by * 4 + bx] = (uint8_t)best_m;
-      }
-      sync();
-      if (nz) {
-        if (l < 4) {  // rows
-          int32_t *d = co + 4 * l;
-          const int e0 = wadd(d[0], d[2]), e1 = wsub(d[0], d[2]), e2 = wsub(d[1] >> 1, d[3]), e3 = wadd(d[1], d[3] >> 1);
-          d[0] = wadd(e0, e3);
-          d[1] = wadd(e1, e2);
-          d[2] = wsub(e1, e2);
-          d[3] = wsub(e0, e3);
+      // the block coded with it (enc/frame.rs:1025-1046, h264/idct.rs:8-32) in the registers of the
+      // 16-lane groups: lane i holds sample (r, c) = (i >> 2, i & 3) and after the forward transform
+      // coefficient (perm4(r), perm4(c)); lanes i < 16 quantise it and write the block
+      {
+        const int i = l & 15, r = i >> 2, c = i & 3;
+        const int pv = nxn_sample(4, best_m, c, r, L.etop, L.eleft, tl, have_top, have_left);
+        int v = (int)L.src_y[(y + r) * 16 + x + c] - pv;
+        v = fwd4<3, 1>(v, c);   // along the row
+        v = fwd4<12, 4>(v, r);  // along the column
+        const int lvl = quant(v, mf, qf, qbits);
+        if (l < 16) L.code.luma[by * 4 + bx][scan] = (int16_t)lvl;
+        const int nz = count_lanes(l < 16 && lvl != 0);
+        int rec = pv;
+        if (nz) {
+          int d = shl(lvl * vq, qp / 6);
+          d = inv4<1, 3>(d, c);   // the rows first, as h264/idct.rs
+          d = inv4<4, 12>(d, r);
+          rec = clampi(pv + (wadd(d, 32) >> 6), 0, 255);
         }
-        sync();
-        if (l < 4) {  // columns, added to the prediction
-          const int32_t *t = co + l;
-          const int g0 = wadd(t[0], t[8]), g1 = wsub(t[0], t[8]), g2 = wsub(t[4] >> 1, t[12]), g3 = wadd(t[4], t[12] >> 1);
-          const int hh[4] = {wadd(g0, g3), wadd(g1, g2), wsub(g1, g2), wsub(g0, g3)};
-#pragma unroll
-          for (int i = 0; i < 4; i++) {
-            uint8_t &q = cyv(x + l, y + i);
-            q = (uint8_t)clampi(q + (wadd(hh[i], 32) >> 6), 0, 255);
-          }
+        if (l < 16) cyv(x + c, y + r) = (uint8_t)rec;
+        if (l == 0) {
+          L.code.nnz[by * 4 + bx] = (uint8_t)nz;
+          L.code.modes[by * 4 + bx] = (uint8_t)best_m;
         }
-        sync();
       }
+      sync();
     }
     if (lane() == 0) {
       int cbp = 0;
