@@ -377,22 +377,46 @@ struct Coder {
   // the value v of lane src (ds_bpermute; each lane its own source)
   __device__ static int bperm(int v, int src) { return __builtin_amdgcn_ds_bpermute(src << 2, v); }
 
-  // SAD of the w x h source block at (bx, by) of the macroblock against reference r's integer plane
-  // at the integer part of (mx, my) (enc/frame.rs:662-671), over the group: its lanes take the block's
-  // dwords (four samples of a row) in turn, v_sad_u8 of the source dword and the reference read as
-  // two aligned dwords byte-aligned (the plane's border covers the dword past a block's row).
-  __device__ int sad_g(int r, int bx, int by, int w, int h, int mx, int my) const {
+  // A reference plane as a motion search reads it: its base at the macroblock's top-left sample
+  // and its row stride, taken from the slice parameters once per search (read inside the SAD loop
+  // they were loaded from memory again before every SAD).
+  struct PlaneAt {
+    const uint8_t *mb;  // the macroblock's sample (0, 0) in the padded plane
+    int sw;
+  };
+  __device__ PlaneAt plane_at(int pi) const {
     const int sw = P.mb_w * 16 + 2 * PAD;
+    return PlaneAt{P.hp[pi] + (size_t)(mby * 16 + PAD) * sw + (mbx * 16 + PAD), sw};
+  }
+
+  // SAD of the w x h source block at (bx, by) of the macroblock against the integer plane pa at the
+  // integer part of (mx, my) (enc/frame.rs:662-671), over the group: its lanes take the block's dwords
+  // (four samples of a row) in turn, at most four each, all loads issued before the first is used;
+  // v_sad_u8 of the source dword and the reference read as two aligned dwords byte-aligned (the
+  // plane's border covers the dword past a block's row).
+  __device__ int sad_g(const PlaneAt &pa, int bx, int by, int w, int h, int mx, int my) const {
+    typedef __attribute__((address_space(1))) const uint32_t GU32;
+    const int sw = pa.sw;
     const int lwd = __builtin_ctz(w) - 2, nd = (w * h) >> 2;
-    const uint8_t *g = P.hp[r] + (size_t)(mby * 16 + by + (my >> 2) + PAD) * sw + (mbx * 16 + bx + (mx >> 2) + PAD);
-    uint32_t v = 0;
-    for (int i = glane(); i < nd; i += 16) {
-      const int rr = i >> lwd, q = i & ((1 << lwd) - 1);
-      const uint32_t src = *reinterpret_cast<const uint32_t *>(&L.src_y[(by + rr) * 16 + bx + 4 * q]);
-      const uint8_t *a = g + (size_t)rr * sw + 4 * q;
-      const uint32_t *d = reinterpret_cast<const uint32_t *>((uintptr_t)a & ~(uintptr_t)3);
-      v = __builtin_amdgcn_sad_u8(src, __builtin_amdgcn_alignbyte(d[1], d[0], (uint32_t)((uintptr_t)a & 3)), v);
+    const uint8_t *g = pa.mb + (ptrdiff_t)(by + (my >> 2)) * sw + (bx + (mx >> 2));
+    uint32_t lo[4], hi[4], sh[4], src[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int i = glane() + 16 * k;
+      lo[k] = hi[k] = sh[k] = src[k] = 0;
+      if (i < nd) {
+        const int rr = i >> lwd, q = i & ((1 << lwd) - 1);
+        const uint8_t *a = g + (ptrdiff_t)rr * sw + 4 * q;
+        GU32 *d = reinterpret_cast<GU32 *>((uintptr_t)a & ~(uintptr_t)3);
+        lo[k] = d[0];
+        hi[k] = d[1];
+        sh[k] = (uint32_t)((uintptr_t)a & 3);
+        src[k] = *reinterpret_cast<const uint32_t *>(&L.src_y[(by + rr) * 16 + bx + 4 * q]);
+      }
     }
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) v = __builtin_amdgcn_sad_u8(src[k], __builtin_amdgcn_alignbyte(hi[k], lo[k], sh[k]), v);
     return group_sum((int)v, 16);
   }
 
@@ -617,11 +641,12 @@ struct Coder {
       push(f->mv[0], f->mv[1]);
     }
     const int bxp = bx * 4, byp = by * 4;
+    const PlaneAt pa = plane_at(pi);
     int bestx = 0, besty = 0, best_cost = 0x7fffffff;
     for (int k = 0; any_lane(k < nc); k++) {
       const bool on = k < nc;
       const int cx = on ? cands[k][0] : 0, cy = on ? cands[k][1] : 0;
-      const int cost = sad_g(pi, bxp, byp, w, h, cx, cy) + lam * mvd_bits(cx, cy, mvp[0], mvp[1]);
+      const int cost = sad_g(pa, bxp, byp, w, h, cx, cy) + lam * mvd_bits(cx, cy, mvp[0], mvp[1]);
       if (on && cost < best_cost) { bestx = cx; besty = cy; best_cost = cost; }
     }
     // the realtime decisions skip the 16-sample step (enc/frame.rs:973)
@@ -633,7 +658,7 @@ struct Coder {
         for (int d = 0; d < 4; d++) {
           int dx = d == 0 ? -step : d == 1 ? step : 0, dy = d == 2 ? -step : d == 3 ? step : 0;
           int cx = clampi(bestx + dx, lo_x, hi_x), cy = clampi(besty + dy, lo_y, hi_y);
-          int cost = sad_g(pi, bxp, byp, w, h, cx, cy) + lam * mvd_bits(cx, cy, mvp[0], mvp[1]);
+          int cost = sad_g(pa, bxp, byp, w, h, cx, cy) + lam * mvd_bits(cx, cy, mvp[0], mvp[1]);
           if (going && cost < best_cost) { bestx = cx; besty = cy; best_cost = cost; moved = 1; }
         }
         if (!moved) going = false;
