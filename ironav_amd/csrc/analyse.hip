@@ -24,6 +24,17 @@
 
 #include "analyse.h"
 
+// Pointers qualified with it are global memory: their loads are global_load instructions, which wait
+// on the vector memory counter alone (generic pointers' FLAT loads wait on the LDS counter too).
+#ifndef IA_GLOBAL_AS
+#define IA_GLOBAL_AS __attribute__((address_space(1)))
+#endif
+// a generic pointer to global memory as such (a pointer argument, so that no integer converts)
+template <class T>
+__device__ __forceinline__ IA_GLOBAL_AS const T *as_global(const T *p) {
+  return (IA_GLOBAL_AS const T *)p;
+}
+
 namespace ia {
 namespace {
 
@@ -33,12 +44,28 @@ enum { K_NONE = 0, K_I4 = 1, K_I8 = 2, K_I16 = 3, K_INTER = 5 };
 
 // quarter-sample fraction -> (plane a, dx, dy, plane b, dx, dy), indexed [fy][fx]; planes 0 G, 1 B,
 // 2 H, 3 J: the two integer/half samples H.264 8.4.2.2.1 averages (enc/subpel.rs:26-38)
-__constant__ uint8_t QMIX[4][4][6] = {
+constexpr uint8_t QMIX[4][4][6] = {
     {{0, 0, 0, 0, 0, 0}, {0, 0, 0, 1, 0, 0}, {1, 0, 0, 1, 0, 0}, {0, 1, 0, 1, 0, 0}},
     {{0, 0, 0, 2, 0, 0}, {1, 0, 0, 2, 0, 0}, {1, 0, 0, 3, 0, 0}, {1, 0, 0, 2, 1, 0}},
     {{2, 0, 0, 2, 0, 0}, {2, 0, 0, 3, 0, 0}, {3, 0, 0, 3, 0, 0}, {3, 0, 0, 2, 1, 0}},
     {{0, 0, 1, 2, 0, 0}, {2, 0, 0, 1, 0, 1}, {3, 0, 0, 1, 0, 1}, {2, 1, 0, 1, 0, 1}},
 };
+// QMIX in registers: the entries of phase e = 4 fy + fx a byte each (plane a in bits 0-1, its column
+// and row offsets in bits 2 and 3, plane b in bits 4-5, its offsets in bits 6 and 7), eight to a word,
+// so a lane's phase reads no table from memory
+constexpr uint64_t qmix_pack(int half) {
+  uint64_t k = 0;
+  for (int e = 0; e < 8; e++) {
+    const uint8_t *m = QMIX[(e + 8 * half) >> 2][(e + 8 * half) & 3];
+    k |= (uint64_t)(m[0] | m[1] << 2 | m[2] << 3 | m[3] << 4 | m[4] << 6 | m[5] << 7) << (8 * e);
+  }
+  return k;
+}
+constexpr uint64_t QMIX_LO = qmix_pack(0), QMIX_HI = qmix_pack(1);
+__device__ __forceinline__ uint32_t qmix(int fy, int fx) {
+  const int e = fy * 4 + fx;
+  return (uint32_t)(((e < 8 ? QMIX_LO : QMIX_HI) >> (8 * (e & 7))) & 0xffu);
+}
 
 __constant__ uint8_t PARTS[4][4][4] = {
     {{0, 0, 4, 4}}, {{0, 0, 4, 2}, {0, 2, 4, 2}}, {{0, 0, 2, 4}, {2, 0, 2, 4}},
@@ -395,7 +422,7 @@ struct Coder {
   // v_sad_u8 of the source dword and the reference read as two aligned dwords byte-aligned (the
   // plane's border covers the dword past a block's row).
   __device__ int sad_g(const PlaneAt &pa, int bx, int by, int w, int h, int mx, int my) const {
-    typedef __attribute__((address_space(1))) const uint32_t GU32;
+    typedef IA_GLOBAL_AS const uint32_t GU32;
     const int sw = pa.sw;
     const int lwd = __builtin_ctz(w) - 2, nd = (w * h) >> 2;
     const uint8_t *g = pa.mb + (ptrdiff_t)(by + (my >> 2)) * sw + (bx + (mx >> 2));
@@ -422,19 +449,32 @@ struct Coder {
 
   // quarter-sample prediction of the w x h block at picture (x, y) from reference r into out
   // (row stride ostride); enc/subpel.rs:122-145
+  // (the parameters read once, a lane's at most four sample pairs loaded before any is used)
   __device__ void luma_pred(int r, int x, int y, int w, int h, int mx, int my, uint8_t *out, int ostride) const {
+    typedef IA_GLOBAL_AS const uint8_t GU8;
     const int pw = P.mb_w * 16, ph = P.mb_h * 16, sw = pw + 2 * PAD, sh = ph + 2 * PAD;
+    const uint8_t *base = P.hp[r];
     const size_t n = (size_t)sw * sh;
     const int fx = mx & 3, fy = my & 3;
     const int ix = clampi(x + (mx >> 2), -w - 3, pw + 1), iy = clampi(y + (my >> 2), -h - 3, ph + 1);
-    const uint8_t *m = QMIX[fy][fx];
-    const uint8_t *a = P.hp[r] + m[0] * n + (size_t)(iy + m[2] + PAD) * sw + (ix + m[1] + PAD);
-    const uint8_t *b = P.hp[r] + m[3] * n + (size_t)(iy + m[5] + PAD) * sw + (ix + m[4] + PAD);
-    const int lw = __builtin_ctz(w);
-    for (int i = lane(); i < w * h; i += 64) {
-      int rr = i >> lw, c = i & (w - 1);
-      int va = a[(size_t)rr * sw + c], vb = b[(size_t)rr * sw + c];
-      out[rr * ostride + c] = (uint8_t)((va + vb + 1) >> 1);
+    const uint32_t m = qmix(fy, fx);
+    GU8 *a = as_global(base + (m & 3) * n + (size_t)(iy + ((m >> 3) & 1) + PAD) * sw + (ix + ((m >> 2) & 1) + PAD));
+    GU8 *b = as_global(base + ((m >> 4) & 3) * n + (size_t)(iy + ((m >> 7) & 1) + PAD) * sw + (ix + ((m >> 6) & 1) + PAD));
+    const int lw = __builtin_ctz(w), nn = w * h;
+    int va[4], vb[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int i = lane() + 64 * k, rr = i >> lw, c = i & (w - 1);
+      va[k] = vb[k] = 0;
+      if (i < nn) {
+        va[k] = a[(size_t)rr * sw + c];
+        vb[k] = b[(size_t)rr * sw + c];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int i = lane() + 64 * k, rr = i >> lw, c = i & (w - 1);
+      if (i < nn) out[rr * ostride + c] = (uint8_t)((va[k] + vb[k] + 1) >> 1);
     }
   }
 
@@ -563,8 +603,10 @@ struct Coder {
   // each candidate's blocks are summed within its lanes.
   __device__ void cost_sub_g(int r, int x0, int y0, int bxp, int byp, int w, int h, const int (*cand)[2], int n,
                              const int16_t *mvp, int *costs) const {
+    typedef IA_GLOBAL_AS const uint8_t GU8;
     const int pw = P.mb_w * 16, ph = P.mb_h * 16, sw = pw + 2 * PAD, sh = ph + 2 * PAD;
     const size_t plane = (size_t)sw * sh;
+    const uint8_t *hp = P.hp[r];
     const int nbx = w >> 2, nb = nbx * (h >> 2), lnb = __builtin_ctz(nb), per = 16 >> lnb;
     const int lam = P.lambda, gb = lane() & ~15;
     for (int base = 0; base < n; base += per) {
@@ -577,10 +619,10 @@ struct Coder {
           if (k == gi) { vx = cand[k][0]; vy = cand[k][1]; }
         const int fx = vx & 3, fy = vy & 3;
         const int ix = clampi(x0 + (vx >> 2), -w - 3, pw + 1), iy = clampi(y0 + (vy >> 2), -h - 3, ph + 1);
-        const uint8_t *m = QMIX[fy][fx];
+        const uint32_t m = qmix(fy, fx);
         const int ox = (blk % nbx) * 4, oy = (blk / nbx) * 4;
-        const uint8_t *a = P.hp[r] + m[0] * plane + (size_t)(iy + oy + m[2] + PAD) * sw + (ix + ox + m[1] + PAD);
-        const uint8_t *b = P.hp[r] + m[3] * plane + (size_t)(iy + oy + m[5] + PAD) * sw + (ix + ox + m[4] + PAD);
+        GU8 *a = as_global(hp + (m & 3) * plane + (size_t)(iy + oy + ((m >> 3) & 1) + PAD) * sw + (ix + ox + ((m >> 2) & 1) + PAD));
+        GU8 *b = as_global(hp + ((m >> 4) & 3) * plane + (size_t)(iy + oy + ((m >> 7) & 1) + PAD) * sw + (ix + ox + ((m >> 6) & 1) + PAD));
         int d[16];
 #pragma unroll
         for (int rr = 0; rr < 4; rr++)

@@ -38,6 +38,8 @@
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __shared__ static
+// the kernels' global-memory pointer qualifier (address_space(1) for the GPU): plain host memory here
+#define IA_GLOBAL_AS
 #define __constant__
 // the occupancy attribute of the GPU build means nothing on the host
 #define amdgpu_waves_per_eu(...) unused
