@@ -228,7 +228,6 @@ struct Lds {
   uint64_t nzm[2][16];  // masks of the nonzero levels and of the levels above 1 (count_bins)
   uint64_t pos8[24];    // 8x8 scan positions of each significance (0-14) and last (15-23) context
   int etop[16], eleft[8], etl;  // the edges of the Intra4x4 block being coded
-  int e4[16];                   // the same as one array: the left column bottom to top, the corner, the row above
   uint32_t kept_bits;           // the kept candidate's rate
   uint8_t kept_st[440];         // the context states (0-439, those the slice codes on) after its bins
   uint8_t chroma_st[64];        // the chroma residual's context states after the last full count
@@ -1281,7 +1280,7 @@ struct Coder {
 #undef LF
   }
 
-  // Intra4x4 sample (x, y) of mode m (not DC, h264/intra.rs:19-127) as taps on the edge array E (E[0..3]
+  // Intra4x4 sample (x, y) of mode m (not DC, h264/intra.rs:19-127) as taps on the edge samples E (E[0..3]
   // the left column from the bottom up, E[4] the corner, E[5..12] the row above and above-right): the
   // sample is (w0 E[i0] + w1 E[i1] + w2 E[i2] + 2) >> 2, packed i0 | i1 << 4 | i2 << 8 | w0 << 12 |
   // w1 << 15 | w2 << 18. Each mode's cases of nxn_sample in that form, so that lanes of different
@@ -1336,10 +1335,6 @@ struct Coder {
     }
     return (uint32_t)(i0 | i1 << 4 | i2 << 8 | w0 << 12 | w1 << 15 | w2 << 18);
   }
-  __device__ static int tap_eval(uint32_t t, const int *e) {
-    return ((int)((t >> 12) & 7) * e[t & 15] + (int)((t >> 15) & 7) * e[(t >> 4) & 15] +
-            (int)((t >> 18) & 7) * e[(t >> 8) & 15] + 2) >> 2;
-  }
 
   __device__ static int mode_ok(int mode, int have_top, int have_left, int have_tl) {
     if (mode == 0 || mode == 3 || mode == 7) return have_top;
@@ -1378,21 +1373,18 @@ struct Coder {
       if (by == 0) have_tr = bx < 3 ? ta : tra;
       else if (bx == 3) have_tr = 0;
       else have_tr = zorder(bx + 1, by - 1) < zorder(bx, by);
-      // the block's edges, shared in LDS: lanes 0-7 the row above (4-7 above-right, or sample 3
-      // repeated), 8-11 the column to the left, 12 the corner; zero where missing
-      if (l < 8) L.etop[l] = have_top ? cyv(x + (l < 4 || have_tr ? l : 3), y - 1) : 0;
-      else if (l < 12) L.eleft[l - 8] = have_left ? cyv(x - 1, y + l - 8) : 0;
-      else if (l == 12) L.etl = have_tl ? cyv(x - 1, y - 1) : 0;
-      else if (l >= 16 && l < 29) {
-        const int j = l - 16;  // E[j]
-        int v;
-        if (j < 4) v = have_left ? cyv(x - 1, y + 3 - j) : 0;
-        else if (j == 4) v = have_tl ? cyv(x - 1, y - 1) : 0;
-        else v = have_top ? cyv(x + (j - 5 < 4 || have_tr ? j - 5 : 3), y - 1) : 0;
-        L.e4[j] = v;
-      }
-      sync();
-      const int tl = L.etl;
+      // the edge samples E[j] (tap4) read from the canvas as the taps need them: E[0..3] the column
+      // to the left from the bottom up, E[4] the corner, E[5..12] the row above and above-right
+      // (sample 3 repeated when the above-right is missing). A mode whose edges are missing is not
+      // allowed, so the taps of an allowed mode with a nonzero weight read only present edges.
+      auto edge = [&](int j) -> int {
+        const int ex = j < 5 ? x - 1 : x + (j - 5 < 4 || have_tr ? j - 5 : 3), ey = j < 4 ? y + 3 - j : y - 1;
+        return cyv(ex, ey);
+      };
+      auto tap_at = [&](uint32_t t) {
+        return ((int)((t >> 12) & 7) * edge(t & 15) + (int)((t >> 15) & 7) * edge((t >> 4) & 15) +
+                (int)((t >> 18) & 7) * edge((t >> 8) & 15) + 2) >> 2;
+      };
       const int predicted = pred_i4_mode(L.code.modes, bx, by);
       // DC (mode 2) for every lane at once (nxn_sample's case 2)
       int dc;
@@ -1400,8 +1392,8 @@ struct Coder {
         int st = 0, sl = 0;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-          st += L.e4[5 + i];
-          sl += L.e4[3 - i];
+          st += cyv(x + i, y - 1);
+          sl += cyv(x - 1, y + i);
         }
         dc = have_top && have_left ? (st + sl + 4) >> 3 : have_left ? (sl + 2) >> 2 : have_top ? (st + 2) >> 2 : 128;
       }
@@ -1414,7 +1406,7 @@ struct Coder {
         const int m = 4 * pass + (l >> 4), i = l & 15, xx = i & 3, yy = i >> 2;
         const bool ok = m < 9 && mode_ok(m, have_top, have_left, have_tl);
         int v = 0;
-        if (ok) v = (int)L.src_y[(y + yy) * 16 + x + xx] - (m == 2 ? dc : tap_eval(taps[pass], L.e4));
+        if (ok) v = (int)L.src_y[(y + yy) * 16 + x + xx] - (m == 2 ? dc : tap_at(taps[pass]));
         const int sum = group_sum(iabs(hadamard16(v, i)), 16) >> 1;
         cpass[pass] = ok ? sum + P.lambda * (m == predicted ? 1 : 4) : 0x7fffffff;
       }
@@ -1431,7 +1423,7 @@ struct Coder {
       // coefficient (perm4(r), perm4(c)); lanes i < 16 quantise it and write the block
       {
         const int i = l & 15, r = i >> 2, c = i & 3;
-        const int pv = nxn_sample(4, best_m, c, r, L.etop, L.eleft, tl, have_top, have_left);
+        const int pv = best_m == 2 ? dc : tap_at(tap4(best_m, c, r));
         int v = (int)L.src_y[(y + r) * 16 + x + c] - pv;
         v = fwd4<3, 1>(v, c);   // along the row
         v = fwd4<12, 4>(v, r);  // along the column
