@@ -151,7 +151,9 @@ __device__ __forceinline__ void sync() { __syncthreads(); }
 // ---- exp-Golomb lengths and the MV cost (enc/bitw.rs:81-89, enc/frame.rs:164-171) -----------------
 
 __device__ __forceinline__ int ue_bits(uint32_t v) { return 2 * (32 - __clz((int)(v + 1))) - 1; }
-__device__ __forceinline__ int se_bits(int v) { return ue_bits(v > 0 ? (uint32_t)(2 * v - 1) : (uint32_t)(-2 * v)); }
+// the signed mapping 2|v| - (v > 0) written without a select (as a select of two expressions the
+// compiler branched on it)
+__device__ __forceinline__ int se_bits(int v) { return ue_bits(2u * (uint32_t)(v < 0 ? -v : v) - (uint32_t)(v > 0)); }
 __device__ __forceinline__ int mvd_bits(int mx, int my, int px, int py) { return se_bits(mx - px) + se_bits(my - py); }
 
 __device__ __forceinline__ int16_t median3(int16_t a, int16_t b, int16_t c) {
@@ -415,34 +417,52 @@ struct Coder {
     return PlaneAt{P.hp[pi] + (size_t)(mby * 16 + PAD) * sw + (mbx * 16 + PAD), sw};
   }
 
-  // SAD of the w x h source block at (bx, by) of the macroblock against the integer plane pa at the
-  // integer part of (mx, my) (enc/frame.rs:662-671), over the group: its lanes take the block's dwords
-  // (four samples of a row) in turn, at most four each, all loads issued before the first is used;
-  // v_sad_u8 of the source dword and the reference read as two aligned dwords byte-aligned (the
-  // plane's border covers the dword past a block's row).
-  __device__ int sad_g(const PlaneAt &pa, int bx, int by, int w, int h, int mx, int my) const {
-    typedef IA_GLOBAL_AS const uint32_t GU32;
-    const int sw = pa.sw;
-    const int lwd = __builtin_ctz(w) - 2, nd = (w * h) >> 2;
-    const uint8_t *g = pa.mb + (ptrdiff_t)(by + (my >> 2)) * sw + (bx + (mx >> 2));
-    uint32_t lo[4], hi[4], sh[4], src[4];
+  // The w x h source block at (bx, by) of the macroblock as the SADs of one search read it, over the
+  // group: its lanes take the block's dwords (four samples of a row) in turn, at most four each. Set
+  // once per search: each lane's source dwords and the addresses of its dwords in the integer plane
+  // at vector (0, 0), so that a candidate adds one offset to them.
+  struct SadBlock {
+    const uint8_t *row[4];
+    uint32_t src[4];
+    int sw, nd;
+    uint32_t a0;  // the byte alignment of row[] (the same for all: the row stride is a multiple of 4)
+  };
+  __device__ SadBlock sad_block(const PlaneAt &pa, int bx, int by, int w, int h) const {
+    SadBlock sb;
+    const int lwd = __builtin_ctz(w) - 2;
+    sb.sw = pa.sw;
+    sb.nd = (w * h) >> 2;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      const int i = glane() + 16 * k;
-      lo[k] = hi[k] = sh[k] = src[k] = 0;
-      if (i < nd) {
-        const int rr = i >> lwd, q = i & ((1 << lwd) - 1);
-        const uint8_t *a = g + (ptrdiff_t)rr * sw + 4 * q;
-        GU32 *d = reinterpret_cast<GU32 *>((uintptr_t)a & ~(uintptr_t)3);
+      const int i = glane() + 16 * k, rr = i >> lwd, q = i & ((1 << lwd) - 1);
+      sb.row[k] = pa.mb + (ptrdiff_t)(by + rr) * pa.sw + (bx + 4 * q);
+      sb.src[k] = i < sb.nd ? *reinterpret_cast<const uint32_t *>(&L.src_y[(by + rr) * 16 + bx + 4 * q]) : 0u;
+    }
+    sb.a0 = (uint32_t)((uintptr_t)pa.mb & 3);
+    return sb;
+  }
+
+  // SAD of the block against the integer plane at the integer part of (mx, my) (enc/frame.rs:662-671):
+  // all loads issued before the first is used; v_sad_u8 of the source dword and the reference read as
+  // two aligned dwords byte-aligned (the plane's border covers the dword past a block's row).
+  __device__ int sad_g(const SadBlock &sb, int mx, int my) const {
+    typedef IA_GLOBAL_AS const uint32_t GU32;
+    const int off = (my >> 2) * sb.sw + (mx >> 2);
+    const uint32_t sh = (sb.a0 + (uint32_t)off) & 3;
+    const ptrdiff_t offa = off - (int)sh;
+    uint32_t lo[4], hi[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      lo[k] = hi[k] = 0;
+      if (glane() + 16 * k < sb.nd) {
+        GU32 *d = reinterpret_cast<GU32 *>(as_global(sb.row[k] + offa));
         lo[k] = d[0];
         hi[k] = d[1];
-        sh[k] = (uint32_t)((uintptr_t)a & 3);
-        src[k] = *reinterpret_cast<const uint32_t *>(&L.src_y[(by + rr) * 16 + bx + 4 * q]);
       }
     }
     uint32_t v = 0;
 #pragma unroll
-    for (int k = 0; k < 4; k++) v = __builtin_amdgcn_sad_u8(src[k], __builtin_amdgcn_alignbyte(hi[k], lo[k], sh[k]), v);
+    for (int k = 0; k < 4; k++) v = __builtin_amdgcn_sad_u8(sb.src[k], __builtin_amdgcn_alignbyte(hi[k], lo[k], sh), v);
     return group_sum((int)v, 16);
   }
 
@@ -682,12 +702,12 @@ struct Coder {
       push(f->mv[0], f->mv[1]);
     }
     const int bxp = bx * 4, byp = by * 4;
-    const PlaneAt pa = plane_at(pi);
+    const SadBlock sb = sad_block(plane_at(pi), bxp, byp, w, h);
     int bestx = 0, besty = 0, best_cost = 0x7fffffff;
     for (int k = 0; any_lane(k < nc); k++) {
       const bool on = k < nc;
       const int cx = on ? cands[k][0] : 0, cy = on ? cands[k][1] : 0;
-      const int cost = sad_g(pa, bxp, byp, w, h, cx, cy) + lam * mvd_bits(cx, cy, mvp[0], mvp[1]);
+      const int cost = sad_g(sb, cx, cy) + lam * mvd_bits(cx, cy, mvp[0], mvp[1]);
       if (on && cost < best_cost) { bestx = cx; besty = cy; best_cost = cost; }
     }
     // the realtime decisions skip the 16-sample step (enc/frame.rs:973)
@@ -699,7 +719,7 @@ struct Coder {
         for (int d = 0; d < 4; d++) {
           int dx = d == 0 ? -step : d == 1 ? step : 0, dy = d == 2 ? -step : d == 3 ? step : 0;
           int cx = clampi(bestx + dx, lo_x, hi_x), cy = clampi(besty + dy, lo_y, hi_y);
-          int cost = sad_g(pa, bxp, byp, w, h, cx, cy) + lam * mvd_bits(cx, cy, mvp[0], mvp[1]);
+          int cost = sad_g(sb, cx, cy) + lam * mvd_bits(cx, cy, mvp[0], mvp[1]);
           if (going && cost < best_cost) { bestx = cx; besty = cy; best_cost = cost; moved = 1; }
         }
         if (!moved) going = false;
