@@ -566,23 +566,35 @@ struct Coder {
     return k < 2 ? wadd(v, p) : wsub(p, v);
   }
 
+  // SATD of a 4x4 block of sample differences (each within +-255): the Hadamard transform with rows 0
+  // and 2, and rows 1 and 3, packed in the halves of a dword, so each add or subtract handles two rows
+  // (every intermediate is within +-16 * 255, which 16 bits hold). In the column pass both halves
+  // of the four sums end as the same column total, and one half is taken.
+  typedef int16_t S16x2 __attribute__((vector_size(4)));
+  __device__ static S16x2 abs16x2(S16x2 v) {
+    const S16x2 m = v >> 15;
+    return (v ^ m) - m;
+  }
   __device__ static int satd4(const int *d) {
-    int t[16];
+    S16x2 t[2][4];
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      int a = d[r * 4], b = d[r * 4 + 1], c = d[r * 4 + 2], e = d[r * 4 + 3];
-      int s0 = a + b, s1 = c + e, d0 = a - b, d1 = c - e;
-      t[r * 4] = s0 + s1;
-      t[r * 4 + 1] = s0 - s1;
-      t[r * 4 + 2] = d0 - d1;
-      t[r * 4 + 3] = d0 + d1;
+    for (int h = 0; h < 2; h++) {
+      S16x2 p[4];
+#pragma unroll
+      for (int c = 0; c < 4; c++) p[c] = S16x2{(int16_t)d[h * 4 + c], (int16_t)d[(h + 2) * 4 + c]};
+      const S16x2 s0 = p[0] + p[1], s1 = p[2] + p[3], d0 = p[0] - p[1], d1 = p[2] - p[3];
+      t[h][0] = s0 + s1;
+      t[h][1] = s0 - s1;
+      t[h][2] = d0 - d1;
+      t[h][3] = d0 + d1;
     }
     int sum = 0;
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-      int a = t[c], b = t[4 + c], cc = t[8 + c], e = t[12 + c];
-      int s0 = a + b, s1 = cc + e, d0 = a - b, d1 = cc - e;
-      sum += iabs(s0 + s1) + iabs(s0 - s1) + iabs(d0 - d1) + iabs(d0 + d1);
+      const S16x2 sv = t[0][c] + t[1][c], dv = t[0][c] - t[1][c];  // (s0, s1), (d0, d1)
+      const S16x2 sx = S16x2{sv[1], sv[0]}, dx = S16x2{dv[1], dv[0]};
+      const S16x2 tot = abs16x2(sv + sx) + abs16x2(sv - sx) + abs16x2(dv - dx) + abs16x2(dv + dx);
+      sum += tot[0];
     }
     return sum >> 1;
   }
@@ -623,6 +635,7 @@ struct Coder {
   __device__ void cost_sub_g(int r, int x0, int y0, int bxp, int byp, int w, int h, const int (*cand)[2], int n,
                              const int16_t *mvp, int *costs) const {
     typedef IA_GLOBAL_AS const uint8_t GU8;
+    typedef IA_GLOBAL_AS const uint32_t GU32u __attribute__((aligned(1)));  // a dword at any byte address
     const int pw = P.mb_w * 16, ph = P.mb_h * 16, sw = pw + 2 * PAD, sh = ph + 2 * PAD;
     const size_t plane = (size_t)sw * sh;
     const uint8_t *hp = P.hp[r];
@@ -642,14 +655,18 @@ struct Coder {
         const int ox = (blk % nbx) * 4, oy = (blk / nbx) * 4;
         GU8 *a = as_global(hp + (m & 3) * plane + (size_t)(iy + oy + ((m >> 3) & 1) + PAD) * sw + (ix + ox + ((m >> 2) & 1) + PAD));
         GU8 *b = as_global(hp + ((m >> 4) & 3) * plane + (size_t)(iy + oy + ((m >> 7) & 1) + PAD) * sw + (ix + ox + ((m >> 6) & 1) + PAD));
+        // the rounded mean of the two planes' samples four at a time: (a + b + 1) >> 1 of each byte is
+        // (a | b) - ((a ^ b) >> 1), with no carry between bytes
         int d[16];
 #pragma unroll
-        for (int rr = 0; rr < 4; rr++)
+        for (int rr = 0; rr < 4; rr++) {
+          const uint32_t pa = *reinterpret_cast<GU32u *>(a + (size_t)rr * sw);
+          const uint32_t pb = *reinterpret_cast<GU32u *>(b + (size_t)rr * sw);
+          const uint32_t pv = (pa | pb) - (((pa ^ pb) >> 1) & 0x7f7f7f7fu);
+          const uint32_t sr = *reinterpret_cast<const uint32_t *>(&L.src_y[(byp + oy + rr) * 16 + bxp + ox]);
 #pragma unroll
-          for (int c = 0; c < 4; c++) {
-            const int pv = ((int)a[(size_t)rr * sw + c] + (int)b[(size_t)rr * sw + c] + 1) >> 1;
-            d[rr * 4 + c] = (int)L.src_y[(byp + oy + rr) * 16 + bxp + ox + c] - pv;
-          }
+          for (int c = 0; c < 4; c++) d[rr * 4 + c] = (int)((sr >> (8 * c)) & 255) - (int)((pv >> (8 * c)) & 255);
+        }
         sv = satd4(d);
       }
       sv = group_sum(sv, nb);
