@@ -627,34 +627,57 @@ struct Coder {
     return 25 + 4 * b8 + (y & 1) * 2 + (x & 1);
   }
 
+  // The candidates of one sub-sample pass: around its centre the eight half-sample neighbours (pass 0,
+  // the centre itself first) or the four quarter-sample ones (pass 1), each clamped to the search
+  // window. Candidate k's offset from the centre plus one, per coordinate, in two bits of a word.
+  struct SubPass {
+    int cx0, cy0, pass, lo_x, hi_x, lo_y, hi_y;
+    static constexpr int OFF[2][9][2] = {{{0, 0}, {-1, -1}, {0, -1}, {1, -1}, {-1, 0}, {1, 0}, {-1, 1}, {0, 1}, {1, 1}},
+                                         {{0, -1}, {-1, 0}, {1, 0}, {0, 1}}};
+    static constexpr uint32_t pack(int ps, int c) {
+      uint32_t v = 0;
+      for (int k = 0; k < 9; k++) v |= (uint32_t)(OFF[ps][k][c] + 1) << (2 * k);
+      return v;
+    }
+    __device__ int n() const { return pass == 0 ? 9 : 4; }
+    __device__ void at(int k, int &x, int &y) const {
+      constexpr uint32_t X0 = pack(0, 0), Y0 = pack(0, 1), X1 = pack(1, 0), Y1 = pack(1, 1);
+      const uint32_t fx = pass == 0 ? X0 : X1, fy = pass == 0 ? Y0 : Y1;
+      const int step = pass == 0 ? 2 : 1;
+      x = clampi(cx0 + ((int)((fx >> (2 * k)) & 3) - 1) * step, lo_x, hi_x);
+      y = clampi(cy0 + ((int)((fy >> (2 * k)) & 3) - 1) * step, lo_y, hi_y);
+    }
+  };
+
+
   // The costs SATD + lambda * mvd_bits that the sub-sample refinement gives n <= 9 candidate vectors
   // of the w x h partition at (bxp, byp) of the macroblock (picture position (x0, y0)) against
   // reference r, over the group: a lane takes one 4x4 block of one candidate's quarter-sample
   // prediction (enc/subpel.rs:122-145, as luma_pred forms it), 16 / blocks candidates at a time, and
   // each candidate's blocks are summed within its lanes.
-  __device__ void cost_sub_g(int r, int x0, int y0, int bxp, int byp, int w, int h, const int (*cand)[2], int n,
+  __device__ void cost_sub_g(int r, int x0, int y0, int bxp, int byp, int w, int h, const SubPass &sp,
                              const int16_t *mvp, int *costs) const {
     typedef IA_GLOBAL_AS const uint8_t GU8;
     typedef IA_GLOBAL_AS const uint32_t GU32u __attribute__((aligned(1)));  // a dword at any byte address
     const int pw = P.mb_w * 16, ph = P.mb_h * 16, sw = pw + 2 * PAD, sh = ph + 2 * PAD;
-    const size_t plane = (size_t)sw * sh;
-    const uint8_t *hp = P.hp[r];
+    const int plane = sw * sh;  // the four planes of a reference are within 2^31 bytes
     const int nbx = w >> 2, nb = nbx * (h >> 2), lnb = __builtin_ctz(nb), per = 16 >> lnb;
-    const int lam = P.lambda, gb = lane() & ~15;
+    const int lam = P.lambda, gb = lane() & ~15, n = sp.n();
+    // the lane's block, the same for every candidate: its sample (0, 0) in the planes at vector (0, 0)
+    const int blk = glane() & (nb - 1), ox = (blk % nbx) * 4, oy = (blk / nbx) * 4;
+    const uint8_t *lb = P.hp[r] + (ptrdiff_t)(oy + PAD) * sw + (ox + PAD);
+    const uint8_t *ls = &L.src_y[(byp + oy) * 16 + bxp + ox];
     for (int base = 0; base < n; base += per) {
-      const int gi = base + (glane() >> lnb), blk = glane() & (nb - 1);
+      const int gi = base + (glane() >> lnb);
       int sv = 0;
       if (gi < n) {
-        int vx = 0, vy = 0;
-#pragma unroll
-        for (int k = 0; k < 9; k++)
-          if (k == gi) { vx = cand[k][0]; vy = cand[k][1]; }
+        int vx, vy;
+        sp.at(gi, vx, vy);
         const int fx = vx & 3, fy = vy & 3;
         const int ix = clampi(x0 + (vx >> 2), -w - 3, pw + 1), iy = clampi(y0 + (vy >> 2), -h - 3, ph + 1);
         const uint32_t m = qmix(fy, fx);
-        const int ox = (blk % nbx) * 4, oy = (blk / nbx) * 4;
-        GU8 *a = as_global(hp + (m & 3) * plane + (size_t)(iy + oy + ((m >> 3) & 1) + PAD) * sw + (ix + ox + ((m >> 2) & 1) + PAD));
-        GU8 *b = as_global(hp + ((m >> 4) & 3) * plane + (size_t)(iy + oy + ((m >> 7) & 1) + PAD) * sw + (ix + ox + ((m >> 6) & 1) + PAD));
+        GU8 *a = as_global(lb + ((int)(m & 3) * plane + (iy + (int)((m >> 3) & 1)) * sw + ix + (int)((m >> 2) & 1)));
+        GU8 *b = as_global(lb + ((int)((m >> 4) & 3) * plane + (iy + (int)((m >> 7) & 1)) * sw + ix + (int)((m >> 6) & 1)));
         // the rounded mean of the two planes' samples four at a time: (a + b + 1) >> 1 of each byte is
         // (a | b) - ((a ^ b) >> 1), with no carry between bytes
         int d[16];
@@ -663,15 +686,18 @@ struct Coder {
           const uint32_t pa = *reinterpret_cast<GU32u *>(a + (size_t)rr * sw);
           const uint32_t pb = *reinterpret_cast<GU32u *>(b + (size_t)rr * sw);
           const uint32_t pv = (pa | pb) - (((pa ^ pb) >> 1) & 0x7f7f7f7fu);
-          const uint32_t sr = *reinterpret_cast<const uint32_t *>(&L.src_y[(byp + oy + rr) * 16 + bxp + ox]);
+          const uint32_t sr = *reinterpret_cast<const uint32_t *>(ls + rr * 16);
 #pragma unroll
           for (int c = 0; c < 4; c++) d[rr * 4 + c] = (int)((sr >> (8 * c)) & 255) - (int)((pv >> (8 * c)) & 255);
         }
         sv = satd4(d);
       }
       sv = group_sum(sv, nb);
-      for (int k = 0; k < per && base + k < n; k++)
-        costs[base + k] = bperm(sv, gb + (k << lnb)) + lam * mvd_bits(cand[base + k][0], cand[base + k][1], mvp[0], mvp[1]);
+      for (int k = 0; k < per && base + k < n; k++) {
+        int cx, cy;
+        sp.at(base + k, cx, cy);
+        costs[base + k] = bperm(sv, gb + (k << lnb)) + lam * mvd_bits(cx, cy, mvp[0], mvp[1]);
+      }
     }
   }
 
@@ -748,33 +774,22 @@ struct Coder {
     // keeps to the half samples, enc/frame.rs:1006). Each pass probes points around a centre fixed for
     // the pass, so its candidates' costs are computed together and then taken in the reference's order
     // with its strict minimum; the first pass carries the centre's own cost.
-    const int SQ[8][2] = {{-1, -1}, {0, -1}, {1, -1}, {-1, 0}, {1, 0}, {-1, 1}, {0, 1}, {1, 1}};
-    const int DM[4][2] = {{0, -1}, {-1, 0}, {1, 0}, {0, 1}};
     for (int pass = 0; pass < (P.fast ? 1 : 2) && any_lane(!stop); pass++) {
-      const int step = pass == 0 ? 2 : 1, np = pass == 0 ? 8 : 4;
-      const int cx0 = bestx, cy0 = besty;
-      int cand[9][2], cost[9];
-      int n = 0;
-      if (pass == 0) {
-        cand[0][0] = cx0;
-        cand[0][1] = cy0;
-        n = 1;
-      }
-      for (int k = 0; k < np; k++) {
-        const int ddx = pass == 0 ? SQ[k][0] : DM[k][0], ddy = pass == 0 ? SQ[k][1] : DM[k][1];
-        cand[n][0] = clampi(cx0 + ddx * step, lo_x, hi_x);
-        cand[n][1] = clampi(cy0 + ddy * step, lo_y, hi_y);
-        n++;
-      }
-      cost_sub_g(pi, x0, y0, bxp, byp, w, h, cand, n, mvp, cost);
+      const SubPass sp{bestx, besty, pass, lo_x, hi_x, lo_y, hi_y};
+      const int n = sp.n();
+      int cost[9];
+      cost_sub_g(pi, x0, y0, bxp, byp, w, h, sp, mvp, cost);
       if (!stop) {
         int k0 = 0;
         if (pass == 0) {
           best_cost = cost[0];
           k0 = 1;
         }
-        for (int k = k0; k < n; k++)
-          if (cost[k] < best_cost) { bestx = cand[k][0]; besty = cand[k][1]; best_cost = cost[k]; }
+        for (int k = k0; k < n; k++) {
+          int cx, cy;
+          sp.at(k, cx, cy);
+          if (cost[k] < best_cost) { bestx = cx; besty = cy; best_cost = cost[k]; }
+        }
       }
     }
     out[0] = (int16_t)bestx;
