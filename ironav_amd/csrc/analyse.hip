@@ -174,6 +174,8 @@ struct SearchRes {
 struct SearchLds {
   int16_t st_mvs[4][16][2];
   int8_t st_refs[4][16];
+  int16_t tr_mvs[4][16][2];  // the state under trial by the sub-partition search
+  int8_t tr_refs[4][16];
   SearchRes res[4];
 };
 // A B macroblock's working state: the list 1 prediction (averaged with list 0's in L.pred), each
@@ -797,53 +799,51 @@ struct Coder {
     return best_cost - lam * mvd_bits(bestx, besty, mvp[0], mvp[1]);
   }
 
-  // enc/frame.rs:911-940; the search state lives in the group's L.ms.st_*
+  // enc/frame.rs:911-940; the search state lives in the group's L.ms.st_*, and each sub-partitioning
+  // is tried on a copy of it in L.ms.tr_*, lane i of the group keeping 4x4 block i (raster): its entry
+  // of the state is copied, painted when a sub-partition covering it is found, and taken as the kept
+  // state's when the sub-partitioning is the best so far.
   __device__ int sub_search(int r, int k, const int16_t *extra, uint32_t *done, int *cost_out) {
-    const int lam = P.lambda, ox = (k & 1) * 2, oy = (k >> 1) * 2, g = group();
+    const int lam = P.lambda, ox = (k & 1) * 2, oy = (k >> 1) * 2, g = group(), i = glane(), ix = i & 3, iy = i >> 2;
     int best_s = -1, best_cost = 0;
-    int16_t bmv[4][2] = {};
+    int16_t bm0 = L.ms.st_mvs[g][i][0], bm1 = L.ms.st_mvs[g][i][1];
+    int8_t br = L.ms.st_refs[g][i];
     uint32_t bdone = 0;
     for (int s = 0; s < (P.sub8 ? 4 : 1); s++) {
-      int16_t tm[16][2];
-      int8_t tr[16];
-      for (int i = 0; i < 16; i++) { tm[i][0] = L.ms.st_mvs[g][i][0]; tm[i][1] = L.ms.st_mvs[g][i][1]; tr[i] = L.ms.st_refs[g][i]; }
+      L.ms.tr_mvs[g][i][0] = L.ms.st_mvs[g][i][0];
+      L.ms.tr_mvs[g][i][1] = L.ms.st_mvs[g][i][1];
+      L.ms.tr_refs[g][i] = L.ms.st_refs[g][i];
+      sync();
       uint32_t td = *done;
       int cost = lam * (s == 0 ? 0 : 2);
-      int16_t sm[4][2];
       for (int q = 0; q < NSUBS[s]; q++) {
         const int x = ox + SUBS[s][q][0], y = oy + SUBS[s][q][1], w = SUBS[s][q][2], h = SUBS[s][q][3];
         int16_t mvp[2], m[2];
-        pred_mv(tm, tr, td, x, y, w, 0, r, mvp);
+        pred_mv(L.ms.tr_mvs[g], L.ms.tr_refs[g], td, x, y, w, 0, r, mvp);
         int cc = motion_search(r, x, y, w, h, mvp, (const int16_t(*)[2])extra, 1, m);
         cost += cc + lam * mvd_bits(m[0], m[1], mvp[0], mvp[1]);
-        sm[q][0] = m[0];
-        sm[q][1] = m[1];
-        for (int yy = y; yy < y + h; yy++)
-          for (int xx = x; xx < x + w; xx++) {
-            tm[yy * 4 + xx][0] = m[0];
-            tm[yy * 4 + xx][1] = m[1];
-            tr[yy * 4 + xx] = (int8_t)r;
-            td |= 1u << (yy * 4 + xx);
-          }
+        if (ix >= x && ix < x + w && iy >= y && iy < y + h) {
+          L.ms.tr_mvs[g][i][0] = m[0];
+          L.ms.tr_mvs[g][i][1] = m[1];
+          L.ms.tr_refs[g][i] = (int8_t)r;
+        }
+        for (int yy = y; yy < y + h; yy++) td |= ((1u << w) - 1) << (yy * 4 + x);
+        sync();
       }
       if (best_s < 0 || cost < best_cost) {
         best_s = s;
         best_cost = cost;
         bdone = td;
-        for (int q = 0; q < NSUBS[s]; q++) { bmv[q][0] = sm[q][0]; bmv[q][1] = sm[q][1]; }
+        bm0 = L.ms.tr_mvs[g][i][0];
+        bm1 = L.ms.tr_mvs[g][i][1];
+        br = L.ms.tr_refs[g][i];
       }
+      sync();
     }
-    // paint the kept sub-partitioning's vectors into the group's state
-    if (glane() == 0)
-      for (int q = 0; q < NSUBS[best_s]; q++) {
-        const int x = ox + SUBS[best_s][q][0], y = oy + SUBS[best_s][q][1], w = SUBS[best_s][q][2], h = SUBS[best_s][q][3];
-        for (int yy = y; yy < y + h; yy++)
-          for (int xx = x; xx < x + w; xx++) {
-            L.ms.st_mvs[g][yy * 4 + xx][0] = bmv[q][0];
-            L.ms.st_mvs[g][yy * 4 + xx][1] = bmv[q][1];
-            L.ms.st_refs[g][yy * 4 + xx] = (int8_t)r;
-          }
-      }
+    // the kept sub-partitioning's state becomes the group's
+    L.ms.st_mvs[g][i][0] = bm0;
+    L.ms.st_mvs[g][i][1] = bm1;
+    L.ms.st_refs[g][i] = br;
     sync();
     *done = bdone;
     *cost_out = best_cost;
