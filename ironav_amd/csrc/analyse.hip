@@ -720,25 +720,24 @@ struct Coder {
     const int x0 = mbx * 16 + bx * 4, y0 = mby * 16 + by * 4, w = bw * 4, h = bh * 4;
     const int lo_x = max(-(x0 + 24) * 4, -2048), hi_x = min((P.mb_w * 16 - x0 + 8) * 4, 2047);
     const int lo_y = max(-(y0 + 24) * 4, -512), hi_y = min((P.mb_h * 16 - y0 + 8) * 4, 511);
-    int16_t cands[16][2];
-    int nc = 0;
+    // the candidate list across the group's lanes: lane j holds candidate j (at most 16), its two
+    // components as the halves of a dword
+    int lst = 0, nc = 0;
     auto push = [&](int vx, int vy) {
-      cands[nc][0] = (int16_t)((clampi(vx, lo_x, hi_x) + 2) & ~3);
-      cands[nc][1] = (int16_t)((clampi(vy, lo_y, hi_y) + 2) & ~3);
+      const int cx = (clampi(vx, lo_x, hi_x) + 2) & ~3, cy = (clampi(vy, lo_y, hi_y) + 2) & ~3;
+      if (glane() == nc) lst = (int)(((uint32_t)cx & 0xffffu) | ((uint32_t)cy << 16));
       nc++;
     };
     push(mvp[0], mvp[1]);
-    cands[nc][0] = 0;
-    cands[nc][1] = 0;
-    nc++;
+    push(0, 0);  // (the window holds the zero vector, which the rounding keeps)
     {
-      const int16_t z[16][2] = {};
-      const int8_t nr[16] = {};
+      // the neighbours' vectors only: positions outside the macroblock never read the partition state
       const int NB[3][2] = {{-1, 0}, {0, -1}, {4, -1}};
+#pragma unroll
       for (int k = 0; k < 3; k++) {
         int rr;
         int16_t mv[2];
-        if (nb_motion(z, nr, 0, NB[k][0], NB[k][1], &rr, mv, l) && rr == r) push(mv[0], mv[1]);
+        if (nb_motion(nullptr, nullptr, 0, NB[k][0], NB[k][1], &rr, mv, l) && rr == r) push(mv[0], mv[1]);
       }
     }
     for (int k = 0; k < nextra; k++) push(extra[k][0], extra[k][1]);
@@ -746,12 +745,13 @@ struct Coder {
       const ia_fsresult *f = &P.fs[((size_t)r * P.mb_w * P.mb_h + (size_t)mby * P.mb_w + mbx) * IA_NPART + fs_index(bx, by, bw, bh)];
       push(f->mv[0], f->mv[1]);
     }
-    const int bxp = bx * 4, byp = by * 4;
+    const int bxp = bx * 4, byp = by * 4, gb = lane() & ~15;
     const SadBlock sb = sad_block(plane_at(pi), bxp, byp, w, h);
     int bestx = 0, besty = 0, best_cost = 0x7fffffff;
     for (int k = 0; any_lane(k < nc); k++) {
       const bool on = k < nc;
-      const int cx = on ? cands[k][0] : 0, cy = on ? cands[k][1] : 0;
+      const int pk = bperm(lst, gb + (k & 15));
+      const int cx = on ? (int)(int16_t)(pk & 0xffff) : 0, cy = on ? (pk >> 16) : 0;
       const int cost = sad_g(sb, cx, cy) + lam * mvd_bits(cx, cy, mvp[0], mvp[1]);
       if (on && cost < best_cost) { bestx = cx; besty = cy; best_cost = cost; }
     }
