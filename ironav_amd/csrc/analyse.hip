@@ -210,6 +210,11 @@ struct Lds {
     uint8_t tmp[1024];  // a copy counted on for a rate estimate
     SearchLds ms;       // the motion search's per-group state (not live at the same time)
     BLds bs;            // a B macroblock's predictions and search results (not live at the same time)
+    struct {            // the finished macroblock's outputs, formed by one lane and stored by all
+      MbState st;
+      ia_mbinfo info;
+      ia_mbstats stats;
+    } fin;
   };
   uint32_t cab[256];    // per (state, LPS?): bit cost in 1/256 bits << 8 | next state (enc/cabac.rs:32-40,162-173)
   MbState nb[4];        // left, top, top-right, top-left neighbours (valid when nb_ok)
@@ -3398,16 +3403,23 @@ struct Coder {
         bits = count_bins(L.cabac, &chroma);
       }
     }
+    // the state, info and statistics formed in LDS by one lane, then stored a dword per lane (the
+    // three structures are whole dwords)
     if (lane() == 0) {
-      MbState ms;
-      ia_mbinfo info;
-      finish_mb(L.code, ms, info);
-      P.state[idx] = ms;
-      P.mbs[idx] = info;
-      ia_mbstats out = st;
-      out.bits = (int32_t)bits;
-      P.stats[idx] = out;
+      finish_mb(L.code, L.fin.st, L.fin.info);
+      L.fin.stats = st;
+      L.fin.stats.bits = (int32_t)bits;
     }
+    sync();
+    auto store_words = [&](void *dst, const void *src, int bytes) {
+      const uint32_t *s4 = reinterpret_cast<const uint32_t *>(src);
+      uint32_t *d4 = reinterpret_cast<uint32_t *>(dst);
+      for (int i = lane(); i < bytes / 4; i += 64) d4[i] = s4[i];
+    };
+    static_assert(sizeof(MbState) % 4 == 0 && sizeof(ia_mbinfo) % 4 == 0 && sizeof(ia_mbstats) % 4 == 0, "whole dwords");
+    store_words(&P.state[idx], &L.fin.st, (int)sizeof(MbState));
+    store_words(&P.mbs[idx], &L.fin.info, (int)sizeof(ia_mbinfo));
+    store_words(&P.stats[idx], &L.fin.stats, (int)sizeof(ia_mbstats));
     __threadfence_block();
     sync();
   }
